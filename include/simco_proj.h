@@ -1,0 +1,192 @@
+/*
+ * simco_proj.h -- C ABI of the MI355X pressure-projection engine.
+ *
+ * One call replaces what SIMCOFlow's Fortran host does for the pressure
+ * projection today:
+ *   PoissonEquationSolver(PGrid,UGrid,VGrid,WGrid,PCell,UCell,VCell,WCell,
+ *                         vofO,TVar,TPred,PU,PV,PW,BCp,PoCoef,Proj,dt)
+ *        (src/ComputePUVW.f90:127-129, signature src/ProjectionP.f90:33-50)
+ *   followed by the velocity-correction block (src/ComputePUVW.f90:179-232).
+ * The GFM face coefficients (ProjectionP.f90:123-265), the Poisson matrix and
+ * right-hand side (ProjectionP.f90:301-576), the HYPRE PCG/BoomerAMG solve
+ * (ProjectionP.f90:95-116, 267-299), the pressure gather and ghosts
+ * (ProjectionP.f90:578-643) and the correction all run on the GPU.
+ *
+ * Array conventions (exactly the host's Fortran arrays, passed by address):
+ *   halo arrays      (0:Imax+1, 0:Jmax+1, 0:Kmax+1) float64, column-major,
+ *                    element (i,j,k) at i + (Imax+2)*(j + (Jmax+2)*k)
+ *   cell arrays      (1:Imax, 1:Jmax, 1:Kmax) int32 (Cell_Type, MoExCell)
+ *   PoCoef           (1:Imax, 1:Jmax, 1:Kmax, 6) float64, faces W,S,B,E,N,T
+ *   grid arrays      (1:Imax, 1:Jmax, 1:Kmax) float64 (Grid%dx, %dy, %dz)
+ *   BC face values   VarW/VarE (Jmax,Kmax), VarS/VarN (Imax,Kmax),
+ *                    VarB/VarT (Imax,Jmax)
+ * Kinds: real(c_double) = real(dp); integer(c_int32_t) = integer(it4b).
+ *
+ * Threading: calls on one context are synchronous and not thread-safe; use one
+ * context per host thread.  Every call returns an sp_status; none aborts.
+ */
+#ifndef SIMCO_PROJ_H
+#define SIMCO_PROJ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sp_status {
+  SP_OK = 0,
+  SP_ERR_INVALID_ARG = -1,  /* null pointer, bad size or option */
+  SP_ERR_HIP = -2,          /* a HIP runtime call failed */
+  SP_ERR_NOT_CONVERGED = -3,/* outputs written, stats.converged == 0 */
+  SP_ERR_NONFINITE = -4,    /* NaN/Inf in the inputs or during the solve */
+  SP_ERR_OOM = -5,          /* device allocation failed */
+  SP_ERR_GRID = -6,         /* grid missing or not a tensor-product grid */
+  SP_ERR_STATE = -7,        /* adjoint without a preceding forward call */
+  SP_ERR_NO_DEVICE = -8     /* no usable gfx950 device */
+} sp_status;
+
+typedef enum sp_precision {
+  SP_PREC_FP64 = 0,  /* V-cycle and outer CG in FP64 */
+  SP_PREC_MIXED = 1  /* FP32 V-cycle inside an FP64 outer CG */
+} sp_precision;
+
+typedef enum sp_precond {
+  SP_PRECOND_MG = 0,     /* geometric multigrid V-cycle */
+  SP_PRECOND_JACOBI = 1  /* diagonal (for comparison) */
+} sp_precond;
+
+/* Solver and physics parameters.  sp_params_default() fills the values the
+ * reference hard-codes: densities (StateVariables.f90:10), epsi
+ * (CutCell.f90:11), Roref = row (ProjectionP.f90:53). */
+typedef struct sp_params {
+  int32_t imax, jmax, kmax;
+  int32_t precision;       /* sp_precision */
+  int32_t precond;         /* sp_precond */
+  int32_t mg_levels;       /* 0: coarsen while every dimension >= 4 */
+  int32_t mg_pre, mg_post; /* red-black Gauss-Seidel sweeps */
+  int32_t mg_coarse_sweeps;
+  double mg_coarse_scale;  /* coarse operator = scale * Galerkin face sums */
+  int32_t max_iter;        /* outer CG iterations */
+  double div_tol;          /* stop when max |div u| <= div_tol (1/time units) */
+  double rho_l, rho_g, rho_ref; /* row, roa, Roref */
+  double epsi;
+  int32_t pin_i, pin_j, pin_k;  /* 1-based gauge cell of a singular system;
+                                   0,0,0 -> zero mean over fluid cells */
+  int32_t flags;                /* reserved, 0 */
+} sp_params;
+
+typedef struct sp_ctx sp_ctx;
+
+/* Inputs of one projection.  Pointers are host memory for sp_project and
+ * device memory for sp_project_dev. */
+typedef struct sp_inputs {
+  const double *u_star, *v_star, *w_star;  /* TPred%u,v,w (Predictor)     halo */
+  const double *dp_u, *dp_v, *dp_w;        /* PU%Dp, PV%Dp, PW%Dp          halo */
+  const double *ee_area, *ne_area, *te_area; /* PCell%EEArea,NEArea,TEArea halo */
+  const double *vof, *vof_old;             /* PCell%vof, PCellO%vof        halo */
+  const double *vof_l, *phi_l;             /* PCell%vofL, PCell%phiL       halo */
+  const int32_t *cell_type_p;              /* PCell%Cell_Type              cell */
+  const int32_t *cell_type_u, *cell_type_v, *cell_type_w; /* U/V/WCell%Cell_Type */
+  const int32_t *moex_u, *moex_v, *moex_w; /* U/V/WCell%MoExCell           cell */
+  int32_t bc_flag[6];                      /* BCp%flag: W,E,S,N,B,T; 0 Dirichlet, 1 Neumann */
+  const double *bc_val[6];                 /* BCp%VarW..VarT, NULL for Neumann faces */
+  double dt;
+  /* Optional: face coefficients beta = 1/rho' on the E,N,T faces
+   * (cell arrays).  When set the GFM step is skipped and
+   * PoCoef(:,4..6) = Dp/delta * beta; W,S,B faces take the neighbour's value.
+   * This is the parametrisation the adjoint differentiates. */
+  const double *beta_e, *beta_n, *beta_t;
+} sp_inputs;
+
+typedef struct sp_outputs {
+  double *p;          /* Proj%Pp with ghosts (halo array, every entry written) */
+  double *u, *v, *w;  /* TVar%u,v,w: entries (1:I,1:J,1:K) written, others kept */
+  double *po_coef;    /* optional PoCoef (cell array x6), NULL to skip */
+} sp_outputs;
+
+typedef struct sp_stats {
+  int32_t iterations;
+  int32_t converged;
+  int32_t levels;
+  int32_t singular;       /* all-Neumann operator (gauge applied) */
+  int64_t active_cells;
+  int64_t stale_faces;    /* GFM faces that reused the previous beta */
+  double initial_div;     /* max |b|/V before the solve */
+  double final_div;       /* max |div u| of the corrected field */
+  double residual_div;    /* max |A p - b|/V of the recursive residual */
+  double t_assemble_ms, t_solve_ms, t_correct_ms, t_total_ms;
+} sp_stats;
+
+/* Adjoint inputs: dL/du, dL/dv, dL/dw on the corrected faces (halo arrays,
+ * entries (1:I,1:J,1:K) read). */
+typedef struct sp_adj_inputs {
+  const double *gu, *gv, *gw;
+} sp_adj_inputs;
+
+/* Adjoint outputs: dL/du*, dL/dv*, dL/dw* (halo arrays, every entry written)
+ * and dL/dK for K = PoCoef(:,4..6) (cell arrays); optionally dL/dbeta for the
+ * beta parametrisation (cell arrays, NULL to skip). */
+typedef struct sp_adj_outputs {
+  double *gu_star, *gv_star, *gw_star;
+  double *gk_e, *gk_n, *gk_t;
+  double *gbeta_e, *gbeta_n, *gbeta_t;
+} sp_adj_outputs;
+
+/* Fill the defaults: FP64, multigrid, automatic levels, 2+2 sweeps, coarse
+ * scale 0.5, 200 iterations, div_tol 1e-10, rho 998.3/1.205, epsi 1e-3. */
+void sp_params_default(sp_params* p);
+
+/* Create a context on HIP device `device`. */
+sp_status sp_create(sp_ctx** ctx, const sp_params* params, int device);
+sp_status sp_destroy(sp_ctx* ctx);
+
+/* Replace the solver parameters (same grid size). */
+sp_status sp_set_params(sp_ctx* ctx, const sp_params* params);
+
+/* Grid (Mesh.f90:14-18): PGrid%dx,dy,dz and the staggered centre distances
+ * UGrid%dx, VGrid%dy, WGrid%dz (Mesh.f90:160-247), as the host's 3-D arrays.
+ * The grid must be a tensor product (dx depends on i only, ...). */
+sp_status sp_set_grid(sp_ctx* ctx, const double* dxp, const double* dyp, const double* dzp,
+                      const double* dxu, const double* dyv, const double* dzw);
+/* Same from 1-D spacings (lengths Imax, Jmax, Kmax). */
+sp_status sp_set_grid_1d(sp_ctx* ctx, const double* dxp, const double* dyp, const double* dzp,
+                         const double* dxu, const double* dyv, const double* dzw);
+
+/* Projection with host arrays: the drop-in for PoissonEquationSolver + the
+ * correction block.  stats may be NULL. */
+sp_status sp_project(sp_ctx* ctx, const sp_inputs* in, sp_outputs* out, sp_stats* stats);
+/* Same with device arrays (no host<->device copies). */
+sp_status sp_project_dev(sp_ctx* ctx, const sp_inputs* in, sp_outputs* out, sp_stats* stats);
+
+/* Reverse-mode derivative of the most recent projection on this context.
+ * Reuses the multigrid-preconditioned CG on the backward source (the
+ * operator is symmetric); only the forward pressure is kept from the
+ * forward call.  Host and device variants. */
+sp_status sp_adjoint(sp_ctx* ctx, const sp_adj_inputs* in, sp_adj_outputs* out, sp_stats* stats);
+sp_status sp_adjoint_dev(sp_ctx* ctx, const sp_adj_inputs* in, sp_adj_outputs* out,
+                         sp_stats* stats);
+
+/* HIP stream the context runs on (hipStream_t as void*). */
+void* sp_stream(sp_ctx* ctx);
+
+/* Per-kernel device timings of the last call, measured with HIP events:
+ * fills up to `cap` records, returns how many exist. */
+typedef struct sp_kernel_time {
+  char name[48];
+  int32_t launches;
+  double total_ms;
+  double bytes;  /* algorithmic bytes moved by all launches */
+} sp_kernel_time;
+int sp_kernel_times(sp_ctx* ctx, sp_kernel_time* out, int cap);
+/* Enable (1) / disable (0) the per-kernel event timing (off by default). */
+void sp_set_profiling(sp_ctx* ctx, int on);
+
+const char* sp_strerror(sp_status s);
+const char* sp_last_error(sp_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIMCO_PROJ_H */

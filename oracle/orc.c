@@ -1,0 +1,735 @@
+/*
+ * CPU FP64 restatement of SIMCOFlow's pressure projection (see orc.h).
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP engine and the timed CPU
+ * baseline.  Built with -ffp-contract=off so the GFM coefficients and the
+ * assembled rows follow the reference's operation order exactly.
+ */
+#include "orc.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+#define HX(g, i, j, k) ((int64_t)(i) + (int64_t)((g)->I + 2) * ((int64_t)(j) + (int64_t)((g)->J + 2) * (int64_t)(k)))
+#define NX(g, i, j, k) ((int64_t)((i) - 1) + (int64_t)(g)->I * ((int64_t)((j) - 1) + (int64_t)(g)->J * (int64_t)((k) - 1)))
+
+static inline int imax_(int a, int b) { return a > b ? a : b; }
+
+/* ------------------------------------------------------------------ */
+/* GFM face coefficients: ProjectionP.f90:123-265, called for the six  */
+/* faces W,S,B,E,N,T at ProjectionP.f90:53-64.                          */
+/* ------------------------------------------------------------------ */
+void orc_gfm_coef(const orc_grid* g, const double* phiL, const double* vofL, const double* vof,
+                  const double* dpu, const double* dpv, const double* dpw, double rho_l,
+                  double rho_g, double rho_ref, double epsi, double* pocoef) {
+  const int I = g->I, J = g->J, K = g->K;
+  const int64_t N = (int64_t)I * J * K;
+  const double tol = 1.0e-30;                       /* ProjectionP.f90:18 */
+  /* (ium,jvm,kwm,iup,jvp,kwp) per face, ProjectionP.f90:53-64 */
+  static const int sel[6][6] = {{1, 0, 0, 0, 0, 0}, {0, 1, 0, 0, 0, 0}, {0, 0, 1, 0, 0, 0},
+                                {0, 0, 0, 1, 0, 0}, {0, 0, 0, 0, 1, 0}, {0, 0, 0, 0, 0, 1}};
+  for (int f = 0; f < 6; ++f) {
+    const int ium = sel[f][0], jvm = sel[f][1], kwm = sel[f][2];
+    const int iup = sel[f][3], jvp = sel[f][4], kwp = sel[f][5];
+    const double* dp = (f % 3 == 0) ? dpu : ((f % 3 == 1) ? dpv : dpw);
+    double* T = pocoef + (int64_t)f * N;
+    for (int64_t q = 0; q < N; ++q) T[q] = 0.0;    /* :138 */
+    const double BetaP = 1.0 / (rho_l / rho_ref);   /* :140 */
+    const double BetaM = 1.0 / (rho_g / rho_ref);   /* :141 */
+    /* BetaW/BetaD are locals that persist across loop iterations; when a
+     * neighbour is neither wet nor dry (solid) the previous value is reused
+     * (:170-248).  Their value before the first assignment is undefined in
+     * the reference; the restatement starts from the P cell's own phase. */
+    double BetaW = BetaM, BetaD = BetaP;
+    for (int i = 1; i <= I; ++i)
+      for (int j = 1; j <= J; ++j)
+        for (int k = 1; k <= K; ++k) {
+          const int ii = i - ium + iup, jj = j - jvm + jvp, kk = k - kwm + kwp;
+          const int im = i - ium, jm = j - jvm, km = k - kwm;
+          /* :151-153 (only one term survives: 1*d + 0*.. + 0*..) */
+          double DGrid;
+          if (ium + iup) DGrid = g->dxu[imax_(im, 1) - 1];
+          else if (jvm + jvp) DGrid = g->dyv[imax_(jm, 1) - 1];
+          else DGrid = g->dzw[imax_(km, 1) - 1];
+          const int64_t P = HX(g, i, j, k), Q = HX(g, ii, jj, kk);
+          const double Lamda = fabs(phiL[P]) / (fabs(phiL[P]) + fabs(phiL[Q]) + tol); /* :155 */
+          const double one_m = 1.0 - epsi;
+          const int wetP = (vofL[P] >= 0.5 && vof[P] > one_m) ||
+                           (phiL[P] < 0.0 && vof[P] <= one_m && vof[P] > epsi);
+          const int dryP = (vofL[P] < 0.5 && vof[P] > one_m) ||
+                           (phiL[P] >= 0.0 && vof[P] <= one_m && vof[P] > epsi);
+          const int wetQ = (vofL[Q] >= 0.5 && vof[Q] > one_m) ||
+                           (phiL[Q] < 0.0 && vof[Q] <= one_m && vof[Q] > epsi);
+          const int dryQ = (vofL[Q] < 0.5 && vof[Q] > one_m) ||
+                           (phiL[Q] >= 0.0 && vof[Q] <= one_m && vof[Q] > epsi);
+          const double Dp = dp[HX(g, im, jm, km)];
+          const int64_t n = NX(g, i, j, k);
+          if (wetP) {                                          /* :170 */
+            if (dryQ) BetaW = Lamda * BetaM + (1.0 - Lamda) * BetaP;  /* :176 */
+            else if (wetQ) BetaW = BetaM;                             /* :181 */
+            T[n] = Dp / DGrid * BetaP * BetaM / BetaW;               /* :194 */
+          } else if (dryP) {                                   /* :212 */
+            if (wetQ) BetaD = Lamda * BetaP + (1.0 - Lamda) * BetaM;  /* :218 */
+            else if (dryQ) BetaD = BetaP;                             /* :223 */
+            T[n] = Dp / DGrid * BetaP * BetaM / BetaD;               /* :235 */
+          }
+        }
+    /* boundary faces copy the first interior value, :256-264 */
+    if (ium == 1 || iup == 1) {
+      const int dst = ium * 1 + iup * I, src = ium * 2 + iup * (I - 1);
+      for (int k = 1; k <= K; ++k)
+        for (int j = 1; j <= J; ++j) T[NX(g, dst, j, k)] = T[NX(g, src, j, k)];
+    }
+    if (jvm == 1 || jvp == 1) {
+      const int dst = jvm * 1 + jvp * J, src = jvm * 2 + jvp * (J - 1);
+      for (int k = 1; k <= K; ++k)
+        for (int i = 1; i <= I; ++i) T[NX(g, i, dst, k)] = T[NX(g, i, src, k)];
+    }
+    if (kwm == 1 || kwp == 1) {
+      const int dst = kwm * 1 + kwp * K, src = kwm * 2 + kwp * (K - 1);
+      for (int j = 1; j <= J; ++j)
+        for (int i = 1; i <= I; ++i) T[NX(g, i, j, dst)] = T[NX(g, i, j, src)];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Assembled rows: ProjectionP.f90:330-478                              */
+/* ------------------------------------------------------------------ */
+void orc_matrix_rows(const orc_grid* g, const double* pc, const double* ee, const double* ne,
+                     const double* te, const int32_t* ct, const int32_t* fl, double* vals,
+                     int64_t* cols) {
+  const int I = g->I, J = g->J, K = g->K;
+  const int64_t N = (int64_t)I * J * K;
+  const double *P1 = pc, *P2 = pc + N, *P3 = pc + 2 * N, *P4 = pc + 3 * N, *P5 = pc + 4 * N,
+               *P6 = pc + 5 * N;
+  const int WB = fl[0], EB = fl[1], SB = fl[2], NB = fl[3], BB = fl[4], TB = fl[5];
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k <= K; ++k)
+    for (int j = 1; j <= J; ++j)
+      for (int i = 1; i <= I; ++i) {
+        const int64_t n = NX(g, i, j, k);
+        double* v = vals + 7 * n;
+        int64_t* c = cols + 7 * n;
+        for (int q = 0; q < 7; ++q) { v[q] = 0.0; c[q] = -1; }
+        if (ct[n] == 2) continue;                                   /* :333 */
+        const double dx = g->dxp[i - 1], dy = g->dyp[j - 1], dz = g->dzp[k - 1];
+        if (i > 1 && ct[NX(g, i - 1, j, k)] != 2) {                /* :348-356 */
+          c[0] = NX(g, i - 1, j, k);
+          v[0] = -P1[n] * ee[HX(g, i - 1, j, k)] * dy * dz;
+        }
+        if (j > 1 && ct[NX(g, i, j - 1, k)] != 2) {                /* :358-366 */
+          c[1] = NX(g, i, j - 1, k);
+          v[1] = -P2[n] * ne[HX(g, i, j - 1, k)] * dx * dz;
+        }
+        if (k > 1 && ct[NX(g, i, j, k - 1)] != 2) {                /* :368-376 */
+          c[2] = NX(g, i, j, k - 1);
+          v[2] = -P3[n] * te[HX(g, i, j, k - 1)] * dx * dy;
+        }
+        double d = P1[n] * ee[HX(g, i - 1, j, k)] * dy * dz +        /* :379-390 */
+                   P2[n] * ne[HX(g, i, j - 1, k)] * dx * dz +
+                   P3[n] * te[HX(g, i, j, k - 1)] * dx * dy +
+                   P4[n] * ee[HX(g, i, j, k)] * dy * dz +
+                   P5[n] * ne[HX(g, i, j, k)] * dx * dz +
+                   P6[n] * te[HX(g, i, j, k)] * dx * dy;
+        if (WB == 1 && i == 1) d = d - P1[n] * ee[HX(g, i - 1, j, k)] * dy * dz; /* :407-430 */
+        if (SB == 1 && j == 1) d = d - P2[n] * ne[HX(g, i, j - 1, k)] * dx * dz;
+        if (BB == 1 && k == 1) d = d - P3[n] * te[HX(g, i, j, k - 1)] * dx * dy;
+        if (EB == 1 && i == I) d = d - P4[n] * ee[HX(g, i, j, k)] * dy * dz;
+        if (NB == 1 && j == J) d = d - P5[n] * ne[HX(g, i, j, k)] * dx * dz;
+        if (TB == 1 && k == K) d = d - P6[n] * te[HX(g, i, j, k)] * dx * dy;
+        c[3] = n;
+        v[3] = d;
+        if (i < I && ct[NX(g, i + 1, j, k)] != 2) {                /* :433-441 */
+          c[4] = NX(g, i + 1, j, k);
+          v[4] = -P4[n] * ee[HX(g, i, j, k)] * dy * dz;
+        }
+        if (j < J && ct[NX(g, i, j + 1, k)] != 2) {                /* :443-451 */
+          c[5] = NX(g, i, j + 1, k);
+          v[5] = -P5[n] * ne[HX(g, i, j, k)] * dx * dz;
+        }
+        if (k < K && ct[NX(g, i, j, k + 1)] != 2) {                /* :453-461 */
+          c[6] = NX(g, i, j, k + 1);
+          v[6] = -P6[n] * te[HX(g, i, j, k)] * dx * dy;
+        }
+      }
+}
+
+/* RHS: ProjectionP.f90:513-564 */
+void orc_rhs(const orc_grid* g, const double* pc, const double* ee, const double* ne,
+             const double* te, const int32_t* ct, const int32_t* fl, const double* const* bv,
+             const double* u, const double* v, const double* w, const double* vof,
+             const double* vofO, double dt, double* b) {
+  const int I = g->I, J = g->J, K = g->K;
+  const int64_t N = (int64_t)I * J * K;
+  const double *P1 = pc, *P2 = pc + N, *P3 = pc + 2 * N, *P4 = pc + 3 * N, *P5 = pc + 4 * N,
+               *P6 = pc + 5 * N;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k <= K; ++k)
+    for (int j = 1; j <= J; ++j)
+      for (int i = 1; i <= I; ++i) {
+        const int64_t n = NX(g, i, j, k);
+        if (ct[n] == 2) { b[n] = 0.0; continue; }
+        const double dx = g->dxp[i - 1], dy = g->dyp[j - 1], dz = g->dzp[k - 1];
+        const int64_t h = HX(g, i, j, k);
+        double r = -dy * dz * (ee[h] * u[h] - ee[HX(g, i - 1, j, k)] * u[HX(g, i - 1, j, k)]) -
+                   dx * dz * (ne[h] * v[h] - ne[HX(g, i, j - 1, k)] * v[HX(g, i, j - 1, k)]) -
+                   dx * dy * (te[h] * w[h] - te[HX(g, i, j, k - 1)] * w[HX(g, i, j, k - 1)]);
+        r = r + (vof[h] - vofO[h]) * dx * dy * dz / dt;                  /* :527 */
+        if (fl[0] == 0 && i == 1) r = r + P1[n] * bv[0][(j - 1) + (int64_t)J * (k - 1)]; /* :529-546 */
+        if (fl[1] == 0 && i == I) r = r + P4[n] * bv[1][(j - 1) + (int64_t)J * (k - 1)];
+        if (fl[2] == 0 && j == 1) r = r + P2[n] * bv[2][(i - 1) + (int64_t)I * (k - 1)];
+        if (fl[3] == 0 && j == J) r = r + P5[n] * bv[3][(i - 1) + (int64_t)I * (k - 1)];
+        if (fl[4] == 0 && k == 1) r = r + P3[n] * bv[4][(i - 1) + (int64_t)I * (j - 1)];
+        if (fl[5] == 0 && k == K) r = r + P6[n] * bv[5][(i - 1) + (int64_t)I * (j - 1)];
+        b[n] = r;
+      }
+}
+
+void orc_rows_apply(int64_t n, const double* vals, const int64_t* cols, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < n; ++r) {
+    double s = 0.0;
+    for (int q = 0; q < 7; ++q)
+      if (cols[7 * r + q] >= 0) s += vals[7 * r + q] * x[cols[7 * r + q]];
+    y[r] = s;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Symmetric face operator of the engine                               */
+/* ------------------------------------------------------------------ */
+void orc_sym_operator(const orc_grid* g, const double* pc, const double* ee, const double* ne,
+                      const double* te, const int32_t* ct, const int32_t* fl, double* cx,
+                      double* cy, double* cz, double* e) {
+  const int I = g->I, J = g->J, K = g->K;
+  const int64_t N = (int64_t)I * J * K;
+  const double *P1 = pc, *P2 = pc + N, *P3 = pc + 2 * N, *P4 = pc + 3 * N, *P5 = pc + 4 * N,
+               *P6 = pc + 5 * N;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k <= K; ++k)
+    for (int j = 1; j <= J; ++j)
+      for (int i = 1; i <= I; ++i) {
+        const int64_t n = NX(g, i, j, k);
+        cx[n] = cy[n] = cz[n] = e[n] = 0.0;
+        if (ct[n] == 2) continue;
+        const double dx = g->dxp[i - 1], dy = g->dyp[j - 1], dz = g->dzp[k - 1];
+        const int64_t h = HX(g, i, j, k);
+        if (i < I && ct[NX(g, i + 1, j, k)] != 2) cx[n] = P4[n] * ee[h] * dy * dz;
+        if (j < J && ct[NX(g, i, j + 1, k)] != 2) cy[n] = P5[n] * ne[h] * dx * dz;
+        if (k < K && ct[NX(g, i, j, k + 1)] != 2) cz[n] = P6[n] * te[h] * dx * dy;
+        double ex = 0.0;
+        if (i == 1 ? fl[0] == 0 : ct[NX(g, i - 1, j, k)] == 2)
+          ex = ex + P1[n] * ee[HX(g, i - 1, j, k)] * dy * dz;
+        if (j == 1 ? fl[2] == 0 : ct[NX(g, i, j - 1, k)] == 2)
+          ex = ex + P2[n] * ne[HX(g, i, j - 1, k)] * dx * dz;
+        if (k == 1 ? fl[4] == 0 : ct[NX(g, i, j, k - 1)] == 2)
+          ex = ex + P3[n] * te[HX(g, i, j, k - 1)] * dx * dy;
+        if (i == I ? fl[1] == 0 : ct[NX(g, i + 1, j, k)] == 2) ex = ex + P4[n] * ee[h] * dy * dz;
+        if (j == J ? fl[3] == 0 : ct[NX(g, i, j + 1, k)] == 2) ex = ex + P5[n] * ne[h] * dx * dz;
+        if (k == K ? fl[5] == 0 : ct[NX(g, i, j, k + 1)] == 2) ex = ex + P6[n] * te[h] * dx * dy;
+        e[n] = ex;
+      }
+}
+
+/* d = ((((cW + cS) + cB) + cE) + cN) + cT + e; inactive cells get d = 0 */
+static int64_t sym_diag_level(int nx, int ny, int nz, const double* cx, const double* cy,
+                              const double* cz, const double* e, double* d) {
+  int64_t active = 0;
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+#pragma omp parallel for collapse(2) schedule(static) reduction(+ : active)
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int64_t n = i + sy * j + sz * k;
+        double s = (i > 0 ? cx[n - 1] : 0.0);
+        s = s + (j > 0 ? cy[n - sy] : 0.0);
+        s = s + (k > 0 ? cz[n - sz] : 0.0);
+        s = s + cx[n];
+        s = s + cy[n];
+        s = s + cz[n];
+        s = s + e[n];
+        d[n] = s > 0.0 ? s : 0.0;
+        active += s > 0.0;
+      }
+  return active;
+}
+
+int64_t orc_sym_diag(int I, int J, int K, const double* cx, const double* cy, const double* cz,
+                     const double* e, double* d) {
+  return sym_diag_level(I, J, K, cx, cy, cz, e, d);
+}
+
+static void sym_apply_level(int nx, int ny, int nz, const double* cx, const double* cy,
+                            const double* cz, const double* d, const double* x, double* y) {
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int64_t n = i + sy * j + sz * k;
+        if (d[n] == 0.0) { y[n] = 0.0; continue; }
+        double s = d[n] * x[n];
+        if (i > 0) s -= cx[n - 1] * x[n - 1];
+        if (j > 0) s -= cy[n - sy] * x[n - sy];
+        if (k > 0) s -= cz[n - sz] * x[n - sz];
+        if (i < nx - 1) s -= cx[n] * x[n + 1];
+        if (j < ny - 1) s -= cy[n] * x[n + sy];
+        if (k < nz - 1) s -= cz[n] * x[n + sz];
+        y[n] = s;
+      }
+}
+
+void orc_sym_apply(int I, int J, int K, const double* cx, const double* cy, const double* cz,
+                   const double* d, const double* x, double* y) {
+  sym_apply_level(I, J, K, cx, cy, cz, d, x, y);
+}
+
+/* ------------------------------------------------------------------ */
+/* Cell-centred multigrid (piecewise-constant transfer, summed faces)  */
+/* ------------------------------------------------------------------ */
+typedef struct mg_level {
+  int nx, ny, nz;
+  int64_t n, active;
+  double *cx, *cy, *cz, *e, *d, *x, *b, *r;
+} mg_level;
+
+typedef struct mg_hier {
+  int L, singular;
+  mg_level lv[24];
+} mg_hier;
+
+static double* dalloc(int64_t n) {
+  double* p = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+  return p;
+}
+
+int orc_auto_levels(int I, int J, int K) {
+  int L = 1, nx = I, ny = J, nz = K;
+  while (nx >= 4 && ny >= 4 && nz >= 4 && L < 24) {
+    nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
+    ++L;
+  }
+  return L;
+}
+
+static void mg_build(mg_hier* H, int I, int J, int K, const double* cx, const double* cy,
+                     const double* cz, const double* e, int levels, double scale) {
+  memset(H, 0, sizeof(*H));
+  int L = levels > 0 ? levels : orc_auto_levels(I, J, K);
+  int maxL = orc_auto_levels(I, J, K);
+  if (L > maxL) L = maxL;
+  H->L = L;
+  for (int l = 0; l < L; ++l) {
+    mg_level* A = &H->lv[l];
+    if (l == 0) { A->nx = I; A->ny = J; A->nz = K; }
+    else {
+      A->nx = (H->lv[l - 1].nx + 1) / 2;
+      A->ny = (H->lv[l - 1].ny + 1) / 2;
+      A->nz = (H->lv[l - 1].nz + 1) / 2;
+    }
+    A->n = (int64_t)A->nx * A->ny * A->nz;
+    A->cx = dalloc(A->n); A->cy = dalloc(A->n); A->cz = dalloc(A->n); A->e = dalloc(A->n);
+    A->d = dalloc(A->n); A->x = dalloc(A->n); A->b = dalloc(A->n); A->r = dalloc(A->n);
+    if (l == 0) {
+      memcpy(A->cx, cx, sizeof(double) * A->n);
+      memcpy(A->cy, cy, sizeof(double) * A->n);
+      memcpy(A->cz, cz, sizeof(double) * A->n);
+      memcpy(A->e, e, sizeof(double) * A->n);
+    } else {
+      const mg_level* F = &H->lv[l - 1];
+      const int64_t fsy = F->nx, fsz = (int64_t)F->nx * F->ny;
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int k = 0; k < A->nz; ++k)
+        for (int j = 0; j < A->ny; ++j)
+          for (int i = 0; i < A->nx; ++i) {
+            const int64_t n = i + (int64_t)A->nx * (j + (int64_t)A->ny * k);
+            double sx = 0, sy = 0, sz = 0, se = 0;
+            for (int c = 0; c < 2; ++c)
+              for (int b = 0; b < 2; ++b)
+                for (int a = 0; a < 2; ++a) {
+                  const int fi = 2 * i + a, fj = 2 * j + b, fk = 2 * k + c;
+                  if (fi >= F->nx || fj >= F->ny || fk >= F->nz) continue;
+                  const int64_t m = fi + fsy * fj + fsz * fk;
+                  se += F->e[m];
+                  if (a == 1) sx += F->cx[m];
+                  if (b == 1) sy += F->cy[m];
+                  if (c == 1) sz += F->cz[m];
+                }
+            A->cx[n] = scale * sx; A->cy[n] = scale * sy; A->cz[n] = scale * sz;
+            A->e[n] = scale * se;
+          }
+    }
+    A->active = sym_diag_level(A->nx, A->ny, A->nz, A->cx, A->cy, A->cz, A->e, A->d);
+  }
+  double esum = 0;
+  for (int64_t q = 0; q < H->lv[0].n; ++q) esum += H->lv[0].d[q] > 0 ? fabs(H->lv[0].e[q]) : 0.0;
+  H->singular = esum == 0.0;
+}
+
+static void mg_free(mg_hier* H) {
+  for (int l = 0; l < H->L; ++l) {
+    mg_level* A = &H->lv[l];
+    free(A->cx); free(A->cy); free(A->cz); free(A->e); free(A->d); free(A->x); free(A->b);
+    free(A->r);
+  }
+}
+
+/* one colour of Gauss-Seidel: x_i = (b_i + sum c x_nb)/d_i */
+static void gs_color(const mg_level* A, int color) {
+  const int nx = A->nx, ny = A->ny, nz = A->nz;
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  const double *cx = A->cx, *cy = A->cy, *cz = A->cz, *d = A->d, *b = A->b;
+  double* x = A->x;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = ((j + k + color) & 1); i < nx; i += 2) {
+        const int64_t n = i + sy * j + sz * k;
+        if (d[n] == 0.0) continue;
+        double s = b[n];
+        if (i > 0) s += cx[n - 1] * x[n - 1];
+        if (j > 0) s += cy[n - sy] * x[n - sy];
+        if (k > 0) s += cz[n - sz] * x[n - sz];
+        if (i < nx - 1) s += cx[n] * x[n + 1];
+        if (j < ny - 1) s += cy[n] * x[n + sy];
+        if (k < nz - 1) s += cz[n] * x[n + sz];
+        x[n] = s / d[n];
+      }
+}
+
+static void project_mean(const mg_level* A, double* v) {
+  double s = 0;
+#pragma omp parallel for reduction(+ : s)
+  for (int64_t q = 0; q < A->n; ++q) s += A->d[q] > 0 ? v[q] : 0.0;
+  if (A->active == 0) return;
+  const double m = s / (double)A->active;
+#pragma omp parallel for
+  for (int64_t q = 0; q < A->n; ++q) if (A->d[q] > 0) v[q] -= m;
+}
+
+static void vcycle(mg_hier* H, int l, const orc_solver_opts* o) {
+  mg_level* A = &H->lv[l];
+  memset(A->x, 0, sizeof(double) * A->n);
+  if (l == H->L - 1) {
+    if (H->singular) project_mean(A, A->b);
+    for (int s = 0; s < o->coarse_sweeps; ++s) { gs_color(A, 0); gs_color(A, 1); }
+    for (int s = 0; s < o->coarse_sweeps; ++s) { gs_color(A, 1); gs_color(A, 0); }
+    if (H->singular) project_mean(A, A->x);
+    return;
+  }
+  for (int s = 0; s < o->pre; ++s) { gs_color(A, 0); gs_color(A, 1); }
+  sym_apply_level(A->nx, A->ny, A->nz, A->cx, A->cy, A->cz, A->d, A->x, A->r);
+  mg_level* C = &H->lv[l + 1];
+  const int64_t fsy = A->nx, fsz = (int64_t)A->nx * A->ny;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < C->nz; ++k)
+    for (int j = 0; j < C->ny; ++j)
+      for (int i = 0; i < C->nx; ++i) {
+        double s = 0;
+        for (int c = 0; c < 2; ++c)
+          for (int b = 0; b < 2; ++b)
+            for (int a = 0; a < 2; ++a) {
+              const int fi = 2 * i + a, fj = 2 * j + b, fk = 2 * k + c;
+              if (fi >= A->nx || fj >= A->ny || fk >= A->nz) continue;
+              const int64_t m = fi + fsy * fj + fsz * fk;
+              if (A->d[m] > 0) s += A->b[m] - A->r[m];
+            }
+        C->b[i + (int64_t)C->nx * (j + (int64_t)C->ny * k)] = s;
+      }
+  vcycle(H, l + 1, o);
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < A->nz; ++k)
+    for (int j = 0; j < A->ny; ++j)
+      for (int i = 0; i < A->nx; ++i) {
+        const int64_t m = i + fsy * j + fsz * k;
+        if (A->d[m] > 0) A->x[m] += C->x[(i / 2) + (int64_t)C->nx * ((j / 2) + (int64_t)C->ny * (k / 2))];
+      }
+  for (int s = 0; s < o->post; ++s) { gs_color(A, 1); gs_color(A, 0); }
+}
+
+static double dot_active(int64_t n, const double* d, const double* a, const double* b) {
+  double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (int64_t q = 0; q < n; ++q) s += d[q] > 0 ? a[q] * b[q] : 0.0;
+  return s;
+}
+
+static double max_div(const orc_grid* g, const double* d, const double* r) {
+  const int I = g->I, J = g->J, K = g->K;
+  double m = 0;
+#pragma omp parallel for collapse(2) reduction(max : m) schedule(static)
+  for (int k = 0; k < K; ++k)
+    for (int j = 0; j < J; ++j)
+      for (int i = 0; i < I; ++i) {
+        const int64_t n = i + (int64_t)I * (j + (int64_t)J * k);
+        if (d[n] > 0) {
+          const double v = fabs(r[n]) / (g->dxp[i] * g->dyp[j] * g->dzp[k]);
+          if (v > m) m = v;
+        }
+      }
+  return m;
+}
+
+int orc_pcg(const orc_grid* g, const double* cx, const double* cy, const double* cz,
+            const double* e, const double* vals, const int64_t* cols, const double* b0,
+            const orc_solver_opts* o, double* x, orc_solver_stats* st) {
+  const int I = g->I, J = g->J, K = g->K;
+  const int64_t n = (int64_t)I * J * K;
+  double t0 = omp_get_wtime();
+  mg_hier H;
+  mg_build(&H, I, J, K, cx, cy, cz, e, o->levels, o->coarse_scale);
+  const mg_level* F = &H.lv[0];
+  const double* d = F->d;
+  double *r = dalloc(n), *p = dalloc(n), *q = dalloc(n), *z = dalloc(n);
+  for (int64_t i = 0; i < n; ++i) { x[i] = 0.0; r[i] = d[i] > 0 ? b0[i] : 0.0; }
+  if (H.singular) project_mean(F, r);
+  int it = 0, conv = 0;
+  double rmax = max_div(g, d, r);
+  st->initial_div = rmax;
+  if (rmax <= o->div_tol) conv = 1;
+  double rz = 0;
+  mg_level* M = &H.lv[0];
+  /* preconditioner */
+#define APPLY_PREC(src, dst)                                                   \
+  do {                                                                         \
+    if (o->precond == 2) {                                                     \
+      memcpy(M->b, src, sizeof(double) * n);                                   \
+      vcycle(&H, 0, o);                                                        \
+      memcpy(dst, M->x, sizeof(double) * n);                                   \
+    } else if (o->precond == 1) {                                              \
+      for (int64_t i_ = 0; i_ < n; ++i_) dst[i_] = d[i_] > 0 ? src[i_] / d[i_] : 0.0; \
+    } else {                                                                   \
+      memcpy(dst, src, sizeof(double) * n);                                    \
+    }                                                                          \
+  } while (0)
+  /* singular systems: shift z by its diagonal-weighted mean so the iterate
+   * keeps the stiff phase's pressure near zero (representation floor) */
+  double dsum = 0;
+  for (int64_t i = 0; i < n; ++i) dsum += d[i] > 0 ? d[i] : 0.0;
+#define SHIFT_Z()                                                              \
+  do {                                                                         \
+    if (H.singular && dsum > 0) {                                              \
+      const double m_ = dot_active(n, d, d, z) / dsum;                         \
+      for (int64_t i_ = 0; i_ < n; ++i_) if (d[i_] > 0) z[i_] -= m_;           \
+    }                                                                          \
+  } while (0)
+  if (!conv) {
+    APPLY_PREC(r, z);
+    SHIFT_Z();
+    memcpy(p, z, sizeof(double) * n);
+    rz = dot_active(n, d, r, z);
+  }
+  while (!conv && it < o->max_iter) {
+    if (o->use_rows) orc_rows_apply(n, vals, cols, p, q);
+    else sym_apply_level(I, J, K, F->cx, F->cy, F->cz, d, p, q);
+    const double pq = dot_active(n, d, p, q);
+    const double alpha = rz / pq;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i)
+      if (d[i] > 0) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
+    ++it;
+    rmax = max_div(g, d, r);
+    if (rmax <= o->div_tol) { conv = 1; break; }
+    APPLY_PREC(r, z);
+    SHIFT_Z();
+    const double rz_new = dot_active(n, d, r, z);
+    const double zq = dot_active(n, d, z, q);
+    const double beta = -alpha * zq / rz;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) p[i] = d[i] > 0 ? z[i] + beta * p[i] : 0.0;
+    rz = rz_new;
+  }
+#undef APPLY_PREC
+#undef SHIFT_Z
+  st->iterations = it;
+  st->converged = conv;
+  st->final_div = rmax;
+  st->levels = H.L;
+  st->seconds = omp_get_wtime() - t0;
+  free(r); free(p); free(q); free(z);
+  const int singular = H.singular;
+  mg_free(&H);
+  return singular;
+}
+
+void orc_gauge(int I, int J, int K, const double* d, int singular, int pi, int pj, int pk,
+               double* x) {
+  if (!singular) return;
+  const int64_t n = (int64_t)I * J * K;
+  double shift;
+  if (pi >= 1 && pj >= 1 && pk >= 1 && pi <= I && pj <= J && pk <= K &&
+      d[(pi - 1) + (int64_t)I * ((pj - 1) + (int64_t)J * (pk - 1))] > 0) {
+    shift = x[(pi - 1) + (int64_t)I * ((pj - 1) + (int64_t)J * (pk - 1))];
+  } else {
+    double s = 0; int64_t c = 0;
+    for (int64_t q = 0; q < n; ++q) if (d[q] > 0) { s += x[q]; ++c; }
+    shift = c ? s / (double)c : 0.0;
+  }
+  for (int64_t q = 0; q < n; ++q) if (d[q] > 0) x[q] -= shift;
+}
+
+/* ProjectionP.f90:591-616 gather (solid cells 0) and :620-643 ghosts */
+void orc_pressure_field(const orc_grid* g, const int32_t* ct, const int32_t* fl, const double* x,
+                        double* Pp) {
+  const int I = g->I, J = g->J, K = g->K;
+  const int64_t Nh = (int64_t)(I + 2) * (J + 2) * (K + 2);
+  for (int64_t q = 0; q < Nh; ++q) Pp[q] = 0.0;
+  for (int k = 1; k <= K; ++k)
+    for (int j = 1; j <= J; ++j)
+      for (int i = 1; i <= I; ++i) {
+        const int64_t n = NX(g, i, j, k);
+        Pp[HX(g, i, j, k)] = ct[n] != 2 ? x[n] : 0.0;
+      }
+  for (int j = 1; j <= J; ++j)
+    for (int k = 1; k <= K; ++k) {
+      Pp[HX(g, 0, j, k)] = (double)fl[0] * Pp[HX(g, 1, j, k)];
+      Pp[HX(g, I + 1, j, k)] = (double)fl[1] * Pp[HX(g, I, j, k)];
+    }
+  for (int i = 1; i <= I; ++i)
+    for (int k = 1; k <= K; ++k) {
+      Pp[HX(g, i, 0, k)] = (double)fl[2] * Pp[HX(g, i, 1, k)];
+      Pp[HX(g, i, J + 1, k)] = (double)fl[3] * Pp[HX(g, i, J, k)];
+    }
+  for (int i = 1; i <= I; ++i)
+    for (int j = 1; j <= J; ++j) {
+      Pp[HX(g, i, j, 0)] = (double)fl[4] * Pp[HX(g, i, j, 1)];
+      Pp[HX(g, i, j, K + 1)] = (double)fl[5] * Pp[HX(g, i, j, K)];
+    }
+}
+
+/* ComputePUVW.f90:181-232 */
+void orc_correct(const orc_grid* g, const double* pc, const double* Pp, const int32_t* ctu,
+                 const int32_t* ctv, const int32_t* ctw, const int32_t* mu, const int32_t* mv,
+                 const int32_t* mw, const double* us, const double* vs, const double* ws,
+                 double* u, double* v, double* w) {
+  const int I = g->I, J = g->J, K = g->K;
+  const int64_t N = (int64_t)I * J * K;
+  const double *P4 = pc + 3 * N, *P5 = pc + 4 * N, *P6 = pc + 5 * N;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k <= K; ++k)
+    for (int j = 1; j <= J; ++j)
+      for (int i = 1; i <= I; ++i) {
+        const int64_t n = NX(g, i, j, k), h = HX(g, i, j, k);
+        u[h] = (mu[n] != 1 && ctu[n] != 2) ? us[h] - (Pp[HX(g, i + 1, j, k)] - Pp[h]) * P4[n] : 0.0;
+        v[h] = (mv[n] != 1 && ctv[n] != 2) ? vs[h] - (Pp[HX(g, i, j + 1, k)] - Pp[h]) * P5[n] : 0.0;
+        w[h] = (mw[n] != 1 && ctw[n] != 2) ? ws[h] - (Pp[HX(g, i, j, k + 1)] - Pp[h]) * P6[n] : 0.0;
+      }
+}
+
+/* Solver.f90:236-250 */
+double orc_mres(const orc_grid* g, const double* ee, const double* ne, const double* te,
+                const double* u, const double* v, const double* w, double* mres) {
+  const int I = g->I, J = g->J, K = g->K;
+  double m = 0;
+#pragma omp parallel for collapse(2) reduction(max : m) schedule(static)
+  for (int k = 1; k <= K; ++k)
+    for (int j = 1; j <= J; ++j)
+      for (int i = 1; i <= I; ++i) {
+        const double dx = g->dxp[i - 1], dy = g->dyp[j - 1], dz = g->dzp[k - 1];
+        const int64_t h = HX(g, i, j, k);
+        const double r = dy * dz * (ee[h] * u[h] - ee[HX(g, i - 1, j, k)] * u[HX(g, i - 1, j, k)]) +
+                         dx * dz * (ne[h] * v[h] - ne[HX(g, i, j - 1, k)] * v[HX(g, i, j - 1, k)]) +
+                         dx * dy * (te[h] * w[h] - te[HX(g, i, j, k - 1)] * w[HX(g, i, j, k - 1)]);
+        if (mres) mres[NX(g, i, j, k)] = r;
+        const double a = fabs(r) / (dx * dy * dz);
+        if (a > m) m = a;
+      }
+  return m;
+}
+
+/* ------------------------------------------------------------------ */
+/* Adjoint (derivation in the header of tests/test_oracle_adjoint.py)  */
+/* ------------------------------------------------------------------ */
+int orc_adjoint(const orc_grid* g, const double* pc, const double* ee, const double* ne,
+                const double* te, const int32_t* ct, const int32_t* fl, const int32_t* ctu,
+                const int32_t* ctv, const int32_t* ctw, const int32_t* mu, const int32_t* mv,
+                const int32_t* mw, const double* Pp, const double* gu, const double* gv,
+                const double* gw, const orc_solver_opts* o, double* gus, double* gvs,
+                double* gws, double* gkx, double* gky, double* gkz, double* lambda_out) {
+  const int I = g->I, J = g->J, K = g->K;
+  const int64_t N = (int64_t)I * J * K, Nh = (int64_t)(I + 2) * (J + 2) * (K + 2);
+  const double *P4 = pc + 3 * N, *P5 = pc + 4 * N, *P6 = pc + 5 * N;
+  double *cx = dalloc(N), *cy = dalloc(N), *cz = dalloc(N), *e = dalloc(N), *d = dalloc(N);
+  orc_sym_operator(g, pc, ee, ne, te, ct, fl, cx, cy, cz, e);
+  sym_diag_level(I, J, K, cx, cy, cz, e, d);
+  double* pbar = dalloc(N);
+  for (int64_t q = 0; q < Nh; ++q) { gus[q] = 0; gvs[q] = 0; gws[q] = 0; }
+  /* correction part: u = m (u* - (P(i+1) - P(i)) K) */
+  for (int k = 1; k <= K; ++k)
+    for (int j = 1; j <= J; ++j)
+      for (int i = 1; i <= I; ++i) {
+        const int64_t n = NX(g, i, j, k), h = HX(g, i, j, k);
+        const int m_u = mu[n] != 1 && ctu[n] != 2, m_v = mv[n] != 1 && ctv[n] != 2,
+                  m_w = mw[n] != 1 && ctw[n] != 2;
+        const double tu = m_u ? gu[h] : 0.0, tv = m_v ? gv[h] : 0.0, tw = m_w ? gw[h] : 0.0;
+        gus[h] = tu; gvs[h] = tv; gws[h] = tw;
+        gkx[n] = -tu * (Pp[HX(g, i + 1, j, k)] - Pp[h]);
+        gky[n] = -tv * (Pp[HX(g, i, j + 1, k)] - Pp[h]);
+        gkz[n] = -tw * (Pp[HX(g, i, j, k + 1)] - Pp[h]);
+        /* dP(i+1) gets -t*K, dP(i) gets +t*K; the ghost P(I+1) = flag*P(I) */
+        if (d[n] > 0) pbar[n] += tu * P4[n] + tv * P5[n] + tw * P6[n];
+        if (i < I) { if (d[NX(g, i + 1, j, k)] > 0) pbar[NX(g, i + 1, j, k)] -= tu * P4[n]; }
+        else if (d[n] > 0) pbar[n] -= (double)fl[1] * tu * P4[n];
+        if (j < J) { if (d[NX(g, i, j + 1, k)] > 0) pbar[NX(g, i, j + 1, k)] -= tv * P5[n]; }
+        else if (d[n] > 0) pbar[n] -= (double)fl[3] * tv * P5[n];
+        if (k < K) { if (d[NX(g, i, j, k + 1)] > 0) pbar[NX(g, i, j, k + 1)] -= tw * P6[n]; }
+        else if (d[n] > 0) pbar[n] -= (double)fl[5] * tw * P6[n];
+      }
+  /* lambda = A^{-1} pbar */
+  double* lam = dalloc(N);
+  orc_solver_stats st;
+  orc_solver_opts oo = *o;
+  oo.use_rows = 0;
+  const int singular = orc_pcg(g, cx, cy, cz, e, NULL, NULL, pbar, &oo, lam, &st);
+  orc_gauge(I, J, K, d, singular, 0, 0, 0, lam);
+  /* RHS part: b(i) = -dy dz (EE(i) u*(i) - EE(i-1) u*(i-1)) - ... */
+  for (int k = 1; k <= K; ++k)
+    for (int j = 1; j <= J; ++j)
+      for (int i = 0; i <= I; ++i) {
+        const double lp = (i >= 1 && d[NX(g, i, j, k)] > 0) ? lam[NX(g, i, j, k)] : 0.0;
+        const double ln = (i + 1 <= I && d[NX(g, i + 1, j, k)] > 0) ? lam[NX(g, i + 1, j, k)] : 0.0;
+        const double area = g->dyp[j - 1] * g->dzp[k - 1];
+        gus[HX(g, i, j, k)] += area * ee[HX(g, i, j, k)] * (ln - lp);
+      }
+  for (int k = 1; k <= K; ++k)
+    for (int j = 0; j <= J; ++j)
+      for (int i = 1; i <= I; ++i) {
+        const double lp = (j >= 1 && d[NX(g, i, j, k)] > 0) ? lam[NX(g, i, j, k)] : 0.0;
+        const double ln = (j + 1 <= J && d[NX(g, i, j + 1, k)] > 0) ? lam[NX(g, i, j + 1, k)] : 0.0;
+        const double area = g->dxp[i - 1] * g->dzp[k - 1];
+        gvs[HX(g, i, j, k)] += area * ne[HX(g, i, j, k)] * (ln - lp);
+      }
+  for (int k = 0; k <= K; ++k)
+    for (int j = 1; j <= J; ++j)
+      for (int i = 1; i <= I; ++i) {
+        const double lp = (k >= 1 && d[NX(g, i, j, k)] > 0) ? lam[NX(g, i, j, k)] : 0.0;
+        const double ln = (k + 1 <= K && d[NX(g, i, j, k + 1)] > 0) ? lam[NX(g, i, j, k + 1)] : 0.0;
+        const double area = g->dxp[i - 1] * g->dyp[j - 1];
+        gws[HX(g, i, j, k)] += area * te[HX(g, i, j, k)] * (ln - lp);
+      }
+  /* operator part: dL/dc_f = -(lam_N - lam_P)(P_N - P_P) on interior faces between active cells */
+  for (int k = 1; k <= K; ++k)
+    for (int j = 1; j <= J; ++j)
+      for (int i = 1; i <= I; ++i) {
+        const int64_t n = NX(g, i, j, k), h = HX(g, i, j, k);
+        if (d[n] <= 0) continue;
+        const double dx = g->dxp[i - 1], dy = g->dyp[j - 1], dz = g->dzp[k - 1];
+        if (i < I && ct[NX(g, i + 1, j, k)] != 2 && d[NX(g, i + 1, j, k)] > 0) {
+          const int64_t m = NX(g, i + 1, j, k);
+          gkx[n] -= ee[h] * dy * dz * (lam[m] - lam[n]) * (Pp[HX(g, i + 1, j, k)] - Pp[h]);
+        }
+        if (j < J && ct[NX(g, i, j + 1, k)] != 2 && d[NX(g, i, j + 1, k)] > 0) {
+          const int64_t m = NX(g, i, j + 1, k);
+          gky[n] -= ne[h] * dx * dz * (lam[m] - lam[n]) * (Pp[HX(g, i, j + 1, k)] - Pp[h]);
+        }
+        if (k < K && ct[NX(g, i, j, k + 1)] != 2 && d[NX(g, i, j, k + 1)] > 0) {
+          const int64_t m = NX(g, i, j, k + 1);
+          gkz[n] -= te[h] * dx * dy * (lam[m] - lam[n]) * (Pp[HX(g, i, j, k + 1)] - Pp[h]);
+        }
+      }
+  if (lambda_out) memcpy(lambda_out, lam, sizeof(double) * N);
+  free(cx); free(cy); free(cz); free(e); free(d); free(pbar); free(lam);
+  return st.converged ? 0 : -3;
+}

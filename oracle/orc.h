@@ -1,0 +1,127 @@
+/*
+ * CPU FP64 restatement of SIMCOFlow's pressure projection -- TEST
+ * INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load this library, and only as the checker or as the
+ * timed CPU baseline; the engine never calls it.
+ *
+ * Layouts (all Fortran column-major):
+ *   halo arrays      (0:I+1, 0:J+1, 0:K+1)  index i + (I+2)*(j + (J+2)*k)
+ *   interior arrays  (1:I, 1:J, 1:K)        index (i-1) + I*((j-1) + J*(k-1))
+ *   PoCoef           (1:I, 1:J, 1:K, 6)     faces W,S,B,E,N,T
+ *   1-D spacings     dxp/dyp/dzp = P-cell widths, dxu/dyv/dzw = centre
+ *                    distances of the staggered U/V/W grids (Mesh.f90:160-247)
+ *
+ * Pinning status of each function is listed in oracle/README.md and in the
+ * test files that check it.
+ */
+#ifndef SIMCO_ORACLE_H
+#define SIMCO_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_grid {
+  int32_t I, J, K;
+  const double *dxp, *dyp, *dzp;
+  const double *dxu, *dyv, *dzw;
+} orc_grid;
+
+/* Compute1DGFMCoefficient x6 (ProjectionP.f90:53-64, 123-265). */
+void orc_gfm_coef(const orc_grid* g, const double* phiL, const double* vofL, const double* vof,
+                  const double* dpu, const double* dpv, const double* dpw,
+                  double rho_l, double rho_g, double rho_ref, double epsi, double* pocoef);
+
+/* SetPoissonMatrix rows (ProjectionP.f90:301-482): for every cell, 7 values
+ * in the order W,S,B,diag,E,N,T and the neighbour's interior index (-1 when the
+ * reference omits the entry).  Solid rows (cell_type==2) get all -1 / 0. */
+void orc_matrix_rows(const orc_grid* g, const double* pocoef, const double* ee, const double* ne,
+                     const double* te, const int32_t* ct, const int32_t* bc_flag,
+                     double* vals, int64_t* cols);
+
+/* SetPoissonVectors RHS (ProjectionP.f90:484-576), interior layout, 0 in solid cells.
+ * bc_val[f] may be NULL for Neumann faces; sizes (J,K) for W/E, (I,K) for S/N, (I,J) for B/T. */
+void orc_rhs(const orc_grid* g, const double* pocoef, const double* ee, const double* ne,
+             const double* te, const int32_t* ct, const int32_t* bc_flag, const double* const* bc_val,
+             const double* us, const double* vs, const double* ws, const double* vof,
+             const double* vof_old, double dt, double* b);
+
+/* y = A x with the reference rows. */
+void orc_rows_apply(int64_t n, const double* vals, const int64_t* cols, const double* x, double* y);
+
+/* Symmetric face operator used by the engine: c_x/c_y/c_z hold the coupling
+ * of cell (i,j,k) to its +x/+y/+z neighbour taken from the correction's own
+ * coefficient PoCoef(:,4..6) (ComputePUVW.f90:187,199,211); e holds the
+ * diagonal-only terms (Dirichlet walls, faces towards solid cells) with the
+ * row's own coefficient as in ProjectionP.f90:379-430. */
+void orc_sym_operator(const orc_grid* g, const double* pocoef, const double* ee, const double* ne,
+                      const double* te, const int32_t* ct, const int32_t* bc_flag,
+                      double* cx, double* cy, double* cz, double* e);
+
+/* Diagonal of the symmetric operator; returns the number of active cells. */
+int64_t orc_sym_diag(int I, int J, int K, const double* cx, const double* cy, const double* cz,
+                     const double* e, double* d);
+void orc_sym_apply(int I, int J, int K, const double* cx, const double* cy, const double* cz,
+                   const double* d, const double* x, double* y);
+
+typedef struct orc_solver_opts {
+  int32_t levels;       /* 0 = automatic */
+  int32_t pre, post;    /* RB-GS sweeps */
+  int32_t coarse_sweeps;
+  int32_t max_iter;
+  double  coarse_scale; /* coarse operator = scale * Galerkin sum */
+  double  div_tol;      /* stop when max |r|/V <= div_tol */
+  int32_t use_rows;     /* 1: outer operator = reference rows (vals/cols) */
+  int32_t precond;      /* 0 none, 1 Jacobi, 2 multigrid */
+} orc_solver_opts;
+
+typedef struct orc_solver_stats {
+  int32_t iterations, converged, levels;
+  double  initial_div, final_div;
+  double  seconds;
+} orc_solver_stats;
+
+/* Flexible PCG with a cell-centred multigrid V-cycle preconditioner.
+ * Singular (all-Neumann) systems are solved for the mean-projected b.
+ * x is overwritten (interior layout); 0 in inactive cells. */
+int orc_pcg(const orc_grid* g, const double* cx, const double* cy, const double* cz, const double* e,
+            const double* vals, const int64_t* cols, const double* b, const orc_solver_opts* o,
+            double* x, orc_solver_stats* st);
+
+/* Gauge: pin cell (1-based) set to 0, or mean over active cells = 0 when pin
+ * is (0,0,0) or inactive.  Only applied when the operator is singular. */
+void orc_gauge(int I, int J, int K, const double* d, int singular, int pi, int pj, int pk, double* x);
+
+/* DeltaPressureGetValues + DeltaPressureBoundaryCondition
+ * (ProjectionP.f90:578-643): Pp halo array from the interior solution. */
+void orc_pressure_field(const orc_grid* g, const int32_t* ct, const int32_t* bc_flag,
+                        const double* x, double* Pp);
+
+/* Velocity correction (ComputePUVW.f90:181-232).  u/v/w are written on
+ * (1:I,1:J,1:K); other entries are left untouched. */
+void orc_correct(const orc_grid* g, const double* pocoef, const double* Pp,
+                 const int32_t* ctu, const int32_t* ctv, const int32_t* ctw,
+                 const int32_t* mu, const int32_t* mv, const int32_t* mw,
+                 const double* us, const double* vs, const double* ws,
+                 double* u, double* v, double* w);
+
+/* mres (Solver.f90:236-250), interior layout; returns max |mres|/V. */
+double orc_mres(const orc_grid* g, const double* ee, const double* ne, const double* te,
+                const double* u, const double* v, const double* w, double* mres);
+
+/* Adjoint of the projection for L with dL/du = gu, dL/dv = gv, dL/dw = gw
+ * (halo arrays, read on the corrected faces (1:I,1:J,1:K)).  Gradients are
+ * returned for u*,v*,w* (halo arrays, written on the faces the forward reads)
+ * and for the E/N/T face coefficients K = PoCoef(:,4..6) (interior arrays). */
+int orc_adjoint(const orc_grid* g, const double* pocoef, const double* ee, const double* ne,
+                const double* te, const int32_t* ct, const int32_t* bc_flag,
+                const int32_t* ctu, const int32_t* ctv, const int32_t* ctw,
+                const int32_t* mu, const int32_t* mv, const int32_t* mw,
+                const double* Pp, const double* gu, const double* gv, const double* gw,
+                const orc_solver_opts* o, double* gus, double* gvs, double* gws,
+                double* gkx, double* gky, double* gkz, double* lambda_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,307 @@
+"""ctypes binding of ``oracle/_build/liborc.so`` (test infrastructure only).
+
+Every function restates a piece of the reference (file:line in orc.h).  The
+high-level ``project`` runs the whole path the way ``PoissonEquationSolver``
+plus the correction block do: GFM coefficients -> reference rows/RHS -> a
+tightly converged solve -> pressure field with ghosts -> corrected velocity.
+"""
+
+import ctypes
+import dataclasses
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
+
+F64P = ctypes.POINTER(ctypes.c_double)
+I32P = ctypes.POINTER(ctypes.c_int32)
+I64P = ctypes.POINTER(ctypes.c_int64)
+
+
+class OrcGrid(ctypes.Structure):
+    _fields_ = [("I", ctypes.c_int32), ("J", ctypes.c_int32), ("K", ctypes.c_int32)] + [
+        (n, F64P) for n in ("dxp", "dyp", "dzp", "dxu", "dyv", "dzw")]
+
+
+class OrcOpts(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int32), ("pre", ctypes.c_int32), ("post", ctypes.c_int32),
+                ("coarse_sweeps", ctypes.c_int32), ("max_iter", ctypes.c_int32),
+                ("coarse_scale", ctypes.c_double), ("div_tol", ctypes.c_double),
+                ("use_rows", ctypes.c_int32), ("precond", ctypes.c_int32)]
+
+
+class OrcStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("levels", ctypes.c_int32), ("initial_div", ctypes.c_double),
+                ("final_div", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        G = ctypes.POINTER(OrcGrid)
+        L.orc_gfm_coef.argtypes = [G] + [F64P] * 6 + [ctypes.c_double] * 4 + [F64P]
+        L.orc_gfm_coef.restype = None
+        L.orc_matrix_rows.argtypes = [G, F64P, F64P, F64P, F64P, I32P, I32P, F64P, I64P]
+        L.orc_matrix_rows.restype = None
+        L.orc_rhs.argtypes = [G, F64P, F64P, F64P, F64P, I32P, I32P, ctypes.POINTER(F64P),
+                              F64P, F64P, F64P, F64P, F64P, ctypes.c_double, F64P]
+        L.orc_rhs.restype = None
+        L.orc_rows_apply.argtypes = [ctypes.c_int64, F64P, I64P, F64P, F64P]
+        L.orc_rows_apply.restype = None
+        L.orc_sym_operator.argtypes = [G, F64P, F64P, F64P, F64P, I32P, I32P, F64P, F64P, F64P, F64P]
+        L.orc_sym_operator.restype = None
+        L.orc_sym_diag.argtypes = [ctypes.c_int] * 3 + [F64P] * 5
+        L.orc_sym_diag.restype = ctypes.c_int64
+        L.orc_sym_apply.argtypes = [ctypes.c_int] * 3 + [F64P] * 6
+        L.orc_sym_apply.restype = None
+        L.orc_pcg.argtypes = [G, F64P, F64P, F64P, F64P, F64P, I64P, F64P,
+                              ctypes.POINTER(OrcOpts), F64P, ctypes.POINTER(OrcStats)]
+        L.orc_pcg.restype = ctypes.c_int
+        L.orc_gauge.argtypes = [ctypes.c_int] * 3 + [F64P, ctypes.c_int, ctypes.c_int,
+                                                     ctypes.c_int, ctypes.c_int, F64P]
+        L.orc_gauge.restype = None
+        L.orc_pressure_field.argtypes = [G, I32P, I32P, F64P, F64P]
+        L.orc_pressure_field.restype = None
+        L.orc_correct.argtypes = [G, F64P, F64P] + [I32P] * 6 + [F64P] * 6
+        L.orc_correct.restype = None
+        L.orc_mres.argtypes = [G, F64P, F64P, F64P, F64P, F64P, F64P, F64P]
+        L.orc_mres.restype = ctypes.c_double
+        L.orc_adjoint.argtypes = ([G, F64P, F64P, F64P, F64P, I32P, I32P] + [I32P] * 6 +
+                                  [F64P] * 4 + [ctypes.POINTER(OrcOpts)] + [F64P] * 7)
+        L.orc_adjoint.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    if a.dtype == np.float64:
+        assert a.flags.f_contiguous or a.ndim == 1
+        return a.ctypes.data_as(F64P)
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(I32P)
+    if a.dtype == np.int64:
+        return a.ctypes.data_as(I64P)
+    raise TypeError(a.dtype)
+
+
+def grid_of(case):
+    g = OrcGrid(case.I, case.J, case.K, _p(case.dxp), _p(case.dyp), _p(case.dzp),
+                _p(case.dxu), _p(case.dyv), _p(case.dzw))
+    g._keep = case
+    return g
+
+
+def opts(levels=0, pre=2, post=2, coarse_sweeps=4, max_iter=200, coarse_scale=1.0,
+         div_tol=1e-10, use_rows=0, precond=2):
+    return OrcOpts(levels, pre, post, coarse_sweeps, max_iter, coarse_scale, div_tol, use_rows,
+                   precond)
+
+
+def gfm_coef(case, rho_l=None, rho_g=None, rho_ref=None, epsi=1.0e-3, phi_l=None, vof_l=None,
+             vof=None):
+    rho_l = case.rho_l if rho_l is None else rho_l
+    rho_g = case.rho_g if rho_g is None else rho_g
+    rho_ref = rho_l if rho_ref is None else rho_ref
+    pc = np.zeros((case.I, case.J, case.K, 6), order="F")
+    g = grid_of(case)
+    lib().orc_gfm_coef(ctypes.byref(g), _p(case.phi_l if phi_l is None else phi_l),
+                       _p(case.vof_l if vof_l is None else vof_l),
+                       _p(case.vof if vof is None else vof), _p(case.dp_u), _p(case.dp_v),
+                       _p(case.dp_w), rho_l, rho_g, rho_ref, epsi, _p(pc))
+    return pc
+
+
+def matrix_rows(case, pc):
+    n = case.I * case.J * case.K
+    vals = np.zeros(7 * n)
+    cols = np.zeros(7 * n, dtype=np.int64)
+    g = grid_of(case)
+    lib().orc_matrix_rows(ctypes.byref(g), _p(pc), _p(case.ee_area), _p(case.ne_area),
+                          _p(case.te_area), _p(case.cell_type_p), _p(case.bc_flag), _p(vals),
+                          _p(cols))
+    return vals.reshape(n, 7), cols.reshape(n, 7)
+
+
+def rows_to_scipy(vals, cols):
+    import scipy.sparse as sp
+    n = vals.shape[0]
+    r = np.repeat(np.arange(n), 7)
+    c = cols.ravel()
+    m = c >= 0
+    return sp.csr_matrix((vals.ravel()[m], (r[m], c[m])), shape=(n, n))
+
+
+def rhs(case, pc, bc_val=None):
+    n = case.I * case.J * case.K
+    b = np.zeros(n)
+    g = grid_of(case)
+    bv = (F64P * 6)()
+    keep = []
+    for f in range(6):
+        if bc_val is not None and bc_val[f] is not None:
+            a = np.asfortranarray(bc_val[f], dtype=np.float64)
+            keep.append(a)
+            bv[f] = _p(a)
+    lib().orc_rhs(ctypes.byref(g), _p(pc), _p(case.ee_area), _p(case.ne_area), _p(case.te_area),
+                  _p(case.cell_type_p), _p(case.bc_flag), bv, _p(case.u_star), _p(case.v_star),
+                  _p(case.w_star), _p(case.vof), _p(case.vof_old), case.dt, _p(b))
+    return b
+
+
+def sym_operator(case, pc):
+    n = case.I * case.J * case.K
+    cx, cy, cz, e = (np.zeros(n) for _ in range(4))
+    g = grid_of(case)
+    lib().orc_sym_operator(ctypes.byref(g), _p(pc), _p(case.ee_area), _p(case.ne_area),
+                           _p(case.te_area), _p(case.cell_type_p), _p(case.bc_flag), _p(cx),
+                           _p(cy), _p(cz), _p(e))
+    return cx, cy, cz, e
+
+
+def sym_diag(case, cx, cy, cz, e):
+    d = np.zeros_like(cx)
+    act = lib().orc_sym_diag(case.I, case.J, case.K, _p(cx), _p(cy), _p(cz), _p(e), _p(d))
+    return d, act
+
+
+def sym_apply(case, cx, cy, cz, d, x):
+    y = np.zeros_like(x)
+    lib().orc_sym_apply(case.I, case.J, case.K, _p(cx), _p(cy), _p(cz), _p(d), _p(x), _p(y))
+    return y
+
+
+def pcg(case, op, b, o=None, rows=None):
+    cx, cy, cz, e = op
+    o = o or opts()
+    x = np.zeros_like(b)
+    st = OrcStats()
+    g = grid_of(case)
+    vals = cols = None
+    if rows is not None:
+        vals = np.ascontiguousarray(rows[0].ravel())
+        cols = np.ascontiguousarray(rows[1].ravel())
+    singular = lib().orc_pcg(ctypes.byref(g), _p(cx), _p(cy), _p(cz), _p(e), _p(vals), _p(cols),
+                             _p(np.ascontiguousarray(b)), ctypes.byref(o), _p(x), ctypes.byref(st))
+    return x, st, bool(singular)
+
+
+def gauge(case, d, singular, x, pin=(1, 1, 1)):
+    x = x.copy()
+    lib().orc_gauge(case.I, case.J, case.K, _p(d), int(singular), pin[0], pin[1], pin[2], _p(x))
+    return x
+
+
+def pressure_field(case, x):
+    Pp = np.zeros(case.halo_shape, order="F")
+    g = grid_of(case)
+    lib().orc_pressure_field(ctypes.byref(g), _p(case.cell_type_p), _p(case.bc_flag),
+                             _p(np.ascontiguousarray(x)), _p(Pp))
+    return Pp
+
+
+def correct(case, pc, Pp, u=None, v=None, w=None):
+    u = np.zeros(case.halo_shape, order="F") if u is None else u.copy(order="F")
+    v = np.zeros(case.halo_shape, order="F") if v is None else v.copy(order="F")
+    w = np.zeros(case.halo_shape, order="F") if w is None else w.copy(order="F")
+    g = grid_of(case)
+    lib().orc_correct(ctypes.byref(g), _p(pc), _p(Pp), _p(case.cell_type_u), _p(case.cell_type_v),
+                      _p(case.cell_type_w), _p(case.moex_u), _p(case.moex_v), _p(case.moex_w),
+                      _p(case.u_star), _p(case.v_star), _p(case.w_star), _p(u), _p(v), _p(w))
+    return u, v, w
+
+
+def mres(case, u, v, w):
+    out = np.zeros((case.I, case.J, case.K), order="F")
+    g = grid_of(case)
+    m = lib().orc_mres(ctypes.byref(g), _p(case.ee_area), _p(case.ne_area), _p(case.te_area),
+                       _p(u), _p(v), _p(w), _p(out))
+    return out, m
+
+
+@dataclasses.dataclass
+class OracleResult:
+    pocoef: np.ndarray
+    b: np.ndarray
+    x: np.ndarray
+    Pp: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    div_inf: float
+    stats: OrcStats
+    singular: bool
+    op: tuple
+    d: np.ndarray
+
+
+def project(case, pin=(1, 1, 1), tol=None, rows=True, o=None, u_init=None):
+    """Reference path on the CPU.  ``rows=True`` solves the reference's own
+    assembled rows (ProjectionP.f90:330-478); ``rows=False`` the engine's
+    symmetric face operator.  ``tol`` defaults to a machine-precision stop."""
+    pc = gfm_coef(case)
+    b = rhs(case, pc)
+    op = sym_operator(case, pc)
+    d, _ = sym_diag(case, *op)
+    o = o or opts(max_iter=1000, div_tol=tol if tol is not None else 0.0)
+    R = matrix_rows(case, pc) if rows else None
+    if tol is None:
+        # converge until the residual stops improving: run at a tiny target
+        o.div_tol = 1e-300
+        o.max_iter = 1000
+        x, st, sing = _pcg_to_floor(case, op, b, o, R)
+    else:
+        x, st, sing = pcg(case, op, b, o, R)
+    x = gauge(case, d, sing, x, pin)
+    Pp = pressure_field(case, x)
+    u, v, w = correct(case, pc, Pp, u_init, u_init, u_init)
+    _, dinf = mres(case, u, v, w)
+    return OracleResult(pc, b, x, Pp, u, v, w, dinf, st, sing, op, d)
+
+
+def _pcg_to_floor(case, op, b, o, R):
+    """Run PCG until the divergence stops decreasing (machine-precision floor)."""
+    best = None
+    o.max_iter = 4
+    o.div_tol = 0.0
+    x, st, sing = pcg(case, op, b, o, R)
+    # iterate by increasing max_iter until no further improvement
+    prev = st.final_div
+    it = 4
+    while it < 400:
+        it = int(it * 1.5) + 1
+        o.max_iter = it
+        x2, st2, sing = pcg(case, op, b, o, R)
+        if st2.final_div >= 0.5 * prev:
+            if st2.final_div < prev:
+                x, st = x2, st2
+            break
+        x, st, prev = x2, st2, st2.final_div
+    return x, st, sing
+
+
+def adjoint(case, pc, Pp, gu, gv, gw, o=None):
+    o = o or opts(max_iter=1000, div_tol=1e-300)
+    hs = case.halo_shape
+    gus, gvs, gws = (np.zeros(hs, order="F") for _ in range(3))
+    gkx, gky, gkz = (np.zeros((case.I, case.J, case.K), order="F") for _ in range(3))
+    lam = np.zeros(case.I * case.J * case.K)
+    g = grid_of(case)
+    rc = lib().orc_adjoint(ctypes.byref(g), _p(pc), _p(case.ee_area), _p(case.ne_area),
+                           _p(case.te_area), _p(case.cell_type_p), _p(case.bc_flag),
+                           _p(case.cell_type_u), _p(case.cell_type_v), _p(case.cell_type_w),
+                           _p(case.moex_u), _p(case.moex_v), _p(case.moex_w), _p(Pp),
+                           _p(np.asfortranarray(gu)), _p(np.asfortranarray(gv)),
+                           _p(np.asfortranarray(gw)), ctypes.byref(o), _p(gus), _p(gvs),
+                           _p(gws), _p(gkx), _p(gky), _p(gkz), _p(lam))
+    return dict(gus=gus, gvs=gvs, gws=gws, gkx=gkx, gky=gky, gkz=gkz, lam=lam, rc=rc)

@@ -1,0 +1,228 @@
+// Reverse-mode derivative of the projection.
+//
+// Forward (per face f between cells P and N, aperture-area a_f, coefficient
+// K_f = PoCoef(:,4..6), mask m_f = not MoEx and not solid):
+//   b   = G^T diag(a) u*            (SetPoissonVectors, ProjectionP.f90:521-526)
+//   A p = b,  A = G^T diag(a K) G    (symmetric, SetPoissonMatrix)
+//   u   = m (u* - K G p)             (ComputePUVW.f90:185-219)
+// Backward for dL/du = g:
+//   pbar = -G^T (m K g)              (with the pressure ghost rule at walls)
+//   A lam = pbar                     (same multigrid PCG: A is self-adjoint)
+//   dL/du* = m g + a G lam
+//   dL/dK  = -m g (G p) - a (G lam)(G p)   (second term on faces between active cells)
+// Only the forward pressure is kept from the forward call; everything else is
+// recomputed.
+
+#include <hip/hip_runtime.h>
+
+#include "sp_device.h"
+#include "sp_geom.h"
+#include "sp_internal.h"
+
+namespace sp {
+namespace {
+
+struct AdjArgs {
+  int I, J, K, SX, SY, kper;
+  int64_t N;
+  const double* xp;       // forward pressure (interior, shift applied below)
+  double shift;
+  const uint8_t* active;
+  const double* pocoef;
+  const int32_t *ctu, *ctv, *ctw, *mu, *mv, *mw;
+  int32_t fl[6];
+  const double *gu, *gv, *gw;
+  const double *ee, *ne, *te;
+  const double *dxp, *dyp, *dzp;
+  const double *dp[3];
+  const double *dgrid[3];
+  const double* lam;
+  double* pbar;
+  double *gus, *gvs, *gws, *gk[3], *gb[3];
+  double* partials;
+  Scalars* S;
+};
+
+__device__ inline int64_t nidx(const AdjArgs& A, int a, int b, int c) {
+  return a + (int64_t)A.I * (b + (int64_t)A.J * c);
+}
+__device__ inline int64_t hidx(const AdjArgs& A, int a, int b, int c) {
+  return (a + 1) + (int64_t)A.SX * ((b + 1) + (int64_t)A.SY * (c + 1));
+}
+
+// m_f * g_f of the face on the + side of cell n in direction d
+__device__ inline double tface(const AdjArgs& A, int d, int64_t n, int64_t h) {
+  if (d == 0) return (A.mu[n] != 1 && A.ctu[n] != 2) ? A.gu[h] : 0.0;
+  if (d == 1) return (A.mv[n] != 1 && A.ctv[n] != 2) ? A.gv[h] : 0.0;
+  return (A.mw[n] != 1 && A.ctw[n] != 2) ? A.gw[h] : 0.0;
+}
+
+// pbar (gathered per cell) and the correction part of the gradients
+__global__ void __launch_bounds__(256) k_adj_source(AdjArgs A) {
+  uint32_t bx, by, bz;
+  xcd_block3(bx, by, bz);
+  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
+  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
+  const int64_t N = A.N;
+  double psum = 0.0;
+  if (a < A.I && b < A.J) {
+    for (int c = c0; c < c1; ++c) {
+      const int64_t n = nidx(A, a, b, c), h = hidx(A, a, b, c);
+      const int co[3] = {a, b, c};
+      const int lim[3] = {A.I, A.J, A.K};
+      const int64_t ns[3] = {1, A.I, (int64_t)A.I * A.J};
+      const int64_t hs[3] = {1, A.SX, (int64_t)A.SX * A.SY};
+      double* gs[3] = {A.gus, A.gvs, A.gws};
+      const double pP = A.active[n] ? A.xp[n] - A.shift : 0.0;
+      double pb = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double t = tface(A, d, n, h);
+        const double K = A.pocoef[(3 + d) * N + n];
+        gs[d][h] = t;  // direct part m g
+        const bool last = co[d] == lim[d] - 1;
+        const double pN = !last ? (A.active[n + ns[d]] ? A.xp[n + ns[d]] - A.shift : 0.0)
+                                : (double)A.fl[2 * d + 1] * pP;
+        A.gk[d][n] = -t * (pN - pP);
+        // u_f depends on p(P) with +K and on p(N) (or the ghost) with -K
+        pb += t * K;
+        if (last) pb -= (double)A.fl[2 * d + 1] * t * K;
+        if (co[d] > 0) {
+          const int64_t m = n - ns[d];
+          pb -= tface(A, d, m, h - hs[d]) * A.pocoef[(3 + d) * N + m];
+        }
+      }
+      pb = A.active[n] ? pb : 0.0;
+      A.pbar[n] = pb;
+      psum += pb;
+    }
+  }
+  const double t = block_sum<kTileThreads>(psum);
+  if (flat_tid() == 0) store_partial(A.partials, flat_block(), t);
+  if (last_block(&A.S->counter[kCtrAdj])) {
+    const double tot = reduce_partials_sum<kTileThreads>(A.partials, num_blocks());
+    if (flat_tid() == 0) {
+      Scalars* S = A.S;
+      S->bsum = tot;
+      S->bmean = (S->singular && S->active > 0) ? tot / (double)S->active : 0.0;
+    }
+    reset_counter(&A.S->counter[kCtrAdj]);
+  }
+}
+
+// mean of lam over active cells (the gauge of a singular system)
+__global__ void __launch_bounds__(256) k_adj_lam_mean(AdjArgs A) {
+  uint32_t bx, by, bz;
+  xcd_block3(bx, by, bz);
+  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
+  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
+  double s = 0.0;
+  if (a < A.I && b < A.J)
+    for (int c = c0; c < c1; ++c) {
+      const int64_t n = nidx(A, a, b, c);
+      if (A.active[n]) s += A.lam[n];
+    }
+  const double t = block_sum<kTileThreads>(s);
+  if (flat_tid() == 0) store_partial(A.partials, flat_block(), t);
+  if (last_block(&A.S->counter[kCtrAdj2])) {
+    const double tot = reduce_partials_sum<kTileThreads>(A.partials, num_blocks());
+    if (flat_tid() == 0) {
+      Scalars* S = A.S;
+      S->npart[0] = (S->singular && S->active > 0) ? tot / (double)S->active : 0.0;
+    }
+    reset_counter(&A.S->counter[kCtrAdj2]);
+  }
+}
+
+// dL/du* += a G lam ; dL/dK -= a (G lam)(G p) ; dL/dbeta = dL/dK * Dp/delta
+__global__ void __launch_bounds__(256) k_adj_grad(AdjArgs A) {
+  uint32_t bx, by, bz;
+  xcd_block3(bx, by, bz);
+  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
+  if (a >= A.I || b >= A.J) return;
+  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
+  const double lmean = A.S->npart[0];
+  for (int c = c0; c < c1; ++c) {
+    const int64_t n = nidx(A, a, b, c), h = hidx(A, a, b, c);
+    const int co[3] = {a, b, c};
+    const int lim[3] = {A.I, A.J, A.K};
+    const int64_t ns[3] = {1, A.I, (int64_t)A.I * A.J};
+    const int64_t hs[3] = {1, A.SX, (int64_t)A.SX * A.SY};
+    const double* ap[3] = {A.ee, A.ne, A.te};
+    double* gs[3] = {A.gus, A.gvs, A.gws};
+    const double dxyz[3] = {A.dxp[a], A.dyp[b], A.dzp[c]};
+    const bool actP = A.active[n] != 0;
+    const double lP = actP ? A.lam[n] - lmean : 0.0;
+    const double pP = actP ? A.xp[n] - A.shift : 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double area = dxyz[(d + 1) % 3] * dxyz[(d + 2) % 3];
+      const bool last = co[d] == lim[d] - 1;
+      const bool actN = !last && A.active[n + ns[d]] != 0;
+      const double lN = actN ? A.lam[n + ns[d]] - lmean : 0.0;
+      // + face of this cell
+      gs[d][h] += area * ap[d][h] * (lN - lP);
+      double gk = A.gk[d][n];
+      if (actP && actN) {
+        const double pN = A.xp[n + ns[d]] - A.shift;
+        gk -= ap[d][h] * area * (lN - lP) * (pN - pP);
+      }
+      A.gk[d][n] = gk;
+      if (A.gb[d]) A.gb[d][n] = gk * (A.dp[d][h] / A.dgrid[d][co[d]]);
+      // the wall face on the minus side (index 0) of the first cell
+      if (co[d] == 0) gs[d][h - hs[d]] = area * ap[d][h - hs[d]] * lP;
+    }
+  }
+}
+
+}  // namespace
+
+sp_status adjoint(sp_ctx* c, const sp_adj_inputs& in, sp_adj_outputs& out, sp_stats* st) {
+  const Dims& D = c->dm;
+  const sp_inputs& fin = c->last_in;
+  Geom g = make_geom(D.I, D.J, D.K, 2048);
+  AdjArgs A{};
+  A.I = D.I; A.J = D.J; A.K = D.K; A.SX = D.SX; A.SY = D.SY; A.kper = g.kper; A.N = D.N;
+  A.xp = c->x_ckpt;
+  A.shift = c->S_host->shift;
+  A.active = c->active;
+  A.pocoef = c->pocoef;
+  A.ctu = fin.cell_type_u; A.ctv = fin.cell_type_v; A.ctw = fin.cell_type_w;
+  A.mu = fin.moex_u; A.mv = fin.moex_v; A.mw = fin.moex_w;
+  for (int f = 0; f < 6; ++f) A.fl[f] = fin.bc_flag[f];
+  A.gu = in.gu; A.gv = in.gv; A.gw = in.gw;
+  A.ee = fin.ee_area; A.ne = fin.ne_area; A.te = fin.te_area;
+  A.dxp = c->dxp; A.dyp = c->dyp; A.dzp = c->dzp;
+  A.dp[0] = fin.dp_u; A.dp[1] = fin.dp_v; A.dp[2] = fin.dp_w;
+  A.dgrid[0] = c->dxu; A.dgrid[1] = c->dyv; A.dgrid[2] = c->dzw;
+  A.lam = c->x;
+  A.pbar = c->pbar;
+  A.gus = out.gu_star; A.gvs = out.gv_star; A.gws = out.gw_star;
+  A.gk[0] = out.gk_e; A.gk[1] = out.gk_n; A.gk[2] = out.gk_t;
+  A.gb[0] = out.gbeta_e; A.gb[1] = out.gbeta_n; A.gb[2] = out.gbeta_t;
+  A.partials = c->partials;
+  A.S = c->S;
+  const size_t hb = sizeof(double) * (size_t)D.Nh;
+  for (double* p : {out.gu_star, out.gv_star, out.gw_star}) {
+    hipError_t e = hipMemsetAsync(p, 0, hb, c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "hipMemsetAsync");
+  }
+  const double NB = (double)D.N;
+  int slot;
+  timer_begin(c, "adj_source", NB * (3 * 8 + 6 * 4 + 3 * 8 + 8 + 1 + 3 * 8 + 3 * 8 + 8), &slot);
+  hipLaunchKernelGGL(k_adj_source, g.grid, g.block, 0, c->stream, A);
+  timer_end(c, slot);
+  // solve A lam = pbar (mean-projected when singular) with the forward's
+  // operator and multigrid hierarchy
+  sp_status s = solve(c, c->pbar, c->x, false, st);
+  if (s != SP_OK) return s;
+  timer_begin(c, "adj_lam_mean", NB * 9, &slot);
+  hipLaunchKernelGGL(k_adj_lam_mean, g.grid, g.block, 0, c->stream, A);
+  timer_end(c, slot);
+  timer_begin(c, "adj_grad", NB * (8 * 3 + 1 + 3 * 8 + 3 * 8 * 2 + 3 * 8 * 2), &slot);
+  hipLaunchKernelGGL(k_adj_grad, g.grid, g.block, 0, c->stream, A);
+  timer_end(c, slot);
+  return SP_OK;
+}
+
+}  // namespace sp

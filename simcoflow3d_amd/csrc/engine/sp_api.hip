@@ -1,0 +1,614 @@
+// C ABI of the projection engine (include/simco_proj.h): context, device
+// buffers, the host- and device-pointer projection entry points, the adjoint
+// and per-kernel timing.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "sp_internal.h"
+
+namespace sp {
+
+sp_status fail(sp_ctx* c, sp_status s, const std::string& msg) {
+  if (c) c->last_error = msg;
+  return s;
+}
+
+sp_status hip_check(sp_ctx* c, hipError_t e, const char* what) {
+  if (e == hipSuccess) return SP_OK;
+  if (c) c->last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? SP_ERR_OOM : SP_ERR_HIP;
+}
+
+void timer_begin(sp_ctx* c, const char* name, double bytes, int* slot) {
+  *slot = -1;
+  if (!c->profiling) return;
+  int ti = -1;
+  for (size_t q = 0; q < c->timers.size(); ++q)
+    if (c->timers[q].name == name) { ti = (int)q; break; }
+  if (ti < 0) {
+    c->timers.push_back(Timer{name, 0, 0.0, 0.0});
+    ti = (int)c->timers.size() - 1;
+  }
+  c->timers[ti].launches += 1;
+  c->timers[ti].bytes += bytes;
+  const size_t ei = c->ev_used.size();
+  if (ei >= c->ev_pool.size()) {
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    c->ev_pool.push_back({a, b});
+  }
+  hipEventRecord(c->ev_pool[ei].first, c->stream);
+  c->ev_used.push_back({ti, (int)ei});
+  *slot = (int)ei;
+}
+
+void timer_end(sp_ctx* c, int slot) {
+  if (slot < 0) return;
+  hipEventRecord(c->ev_pool[slot].second, c->stream);
+}
+
+static void timers_collect(sp_ctx* c) {
+  if (!c->profiling) return;
+  hipStreamSynchronize(c->stream);
+  for (auto& u : c->ev_used) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev_pool[u.second].first, c->ev_pool[u.second].second);
+    c->timers[u.first].total_ms += ms;
+  }
+  c->ev_used.clear();
+}
+
+static int auto_levels(int I, int J, int K) {
+  int L = 1, nx = I, ny = J, nz = K;
+  while (nx >= 4 && ny >= 4 && nz >= 4 && L < kMaxLevels) {
+    nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
+    ++L;
+  }
+  return L;
+}
+
+template <typename T>
+static sp_status dmalloc(sp_ctx* c, T** p, size_t count) {
+  void* v = nullptr;
+  hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) { *p = nullptr; return hip_check(c, e, "hipMalloc"); }
+  c->allocs.push_back(v);
+  *p = static_cast<T*>(v);
+  return SP_OK;
+}
+
+static void free_buffers(sp_ctx* c) {
+  for (void* p : c->allocs) hipFree(p);
+  c->allocs.clear();
+  c->allocated_for = 0;
+  c->allocated_prec = -1;
+  c->allocated_precond = -1;
+  c->allocated_levels = -1;
+  c->have_forward = false;
+}
+
+template <typename T>
+static sp_status alloc_levels(sp_ctx* c, Level<T>* lv, bool level0_owned) {
+  const Dims& D = c->dm;
+  for (int l = 0; l < c->levels; ++l) {
+    Level<T>& L = lv[l];
+    if (l == 0) { L.nx = D.I; L.ny = D.J; L.nz = D.K; }
+    else {
+      L.nx = (lv[l - 1].nx + 1) / 2; L.ny = (lv[l - 1].ny + 1) / 2; L.nz = (lv[l - 1].nz + 1) / 2;
+    }
+    L.n = (int64_t)L.nx * L.ny * L.nz;
+    if (l == 0 && !level0_owned) continue;
+    sp_status s;
+    if ((s = dmalloc(c, &L.cx, L.n)) || (s = dmalloc(c, &L.cy, L.n)) || (s = dmalloc(c, &L.cz, L.n)) ||
+        (s = dmalloc(c, &L.e_store, L.n)) || (s = dmalloc(c, &L.b, L.n)) || (s = dmalloc(c, &L.x, L.n)))
+      return s;
+    L.e = L.e_store;
+  }
+  return SP_OK;
+}
+
+static sp_status ensure_buffers(sp_ctx* c) {
+  const Dims& D = c->dm;
+  const int prec = c->prm.precision, pc = c->prm.precond;
+  int L = auto_levels(D.I, D.J, D.K);
+  if (c->prm.mg_levels > 0) L = std::min(L, c->prm.mg_levels);
+  if (pc != SP_PRECOND_MG) L = 1;
+  if (c->allocated_for == (size_t)D.N && c->allocated_prec == prec && c->allocated_precond == pc &&
+      c->allocated_levels == L)
+    return SP_OK;
+  free_buffers(c);
+  c->levels = L;
+  const int64_t N = D.N;
+  sp_status s;
+  if ((s = dmalloc(c, &c->pocoef, 6 * N)) || (s = dmalloc(c, &c->cx, N)) || (s = dmalloc(c, &c->cy, N)) ||
+      (s = dmalloc(c, &c->cz, N)) || (s = dmalloc(c, &c->ee, N)) || (s = dmalloc(c, &c->b, N)) ||
+      (s = dmalloc(c, &c->active, N)) || (s = dmalloc(c, &c->dw, N)) || (s = dmalloc(c, &c->x, N)) || (s = dmalloc(c, &c->r, N)) ||
+      (s = dmalloc(c, &c->q, N)) || (s = dmalloc(c, &c->p0, N)) || (s = dmalloc(c, &c->p1, N)) ||
+      (s = dmalloc(c, &c->pbar, N)) || (s = dmalloc(c, &c->x_ckpt, N)))
+    return s;
+  const bool mixed_mg = prec == SP_PREC_MIXED && pc == SP_PRECOND_MG;
+  if (mixed_mg) {
+    if ((s = dmalloc(c, &c->r32, N)) || (s = dmalloc(c, &c->z32, N))) return s;
+    if ((s = alloc_levels<float>(c, c->lf, true))) return s;
+    // level 0 works on r32 -> z32
+    c->lf[0].b = c->r32;
+    c->lf[0].x = c->z32;
+  } else {
+    if ((s = dmalloc(c, &c->z64, N))) return s;
+    c->r32 = nullptr;
+    c->z32 = nullptr;
+    if ((s = alloc_levels<double>(c, c->ld, false))) return s;
+    Level<double>& L0 = c->ld[0];
+    L0.cx = c->cx; L0.cy = c->cy; L0.cz = c->cz; L0.e_store = c->ee; L0.e = c->ee;
+    L0.b = c->r; L0.x = c->z64;
+  }
+  c->tail_level = c->levels - 1;
+  for (int l = 0; l < c->levels; ++l) {
+    const int64_t n = mixed_mg ? c->lf[l].n : c->ld[l].n;
+    if (n <= kTailCells) { c->tail_level = l; break; }
+  }
+  if ((s = dmalloc(c, reinterpret_cast<char**>(&c->mg_desc), 128 * kMaxLevels))) return s;
+  c->stale_cap = std::min<int64_t>(6 * N, std::max<int64_t>(1 << 20, N / 2));
+  if ((s = dmalloc(c, &c->stale_list, 3 * c->stale_cap))) return s;
+  if ((s = dmalloc(c, &c->col_last, 12 * (int64_t)D.I * D.J)) ||
+      (s = dmalloc(c, &c->col_prefix, 12 * (int64_t)D.I * D.J)))
+    return s;
+  if ((s = dmalloc(c, &c->partials, (size_t)kNumPartSlots * kMaxPartials))) return s;
+  for (int f = 0; f < 6; ++f) {
+    const int64_t m = std::max<int64_t>({(int64_t)D.J * D.K, (int64_t)D.I * D.K, (int64_t)D.I * D.J});
+    if ((s = dmalloc(c, &c->bcv[f], m))) return s;
+  }
+  c->allocated_for = (size_t)N;
+  c->allocated_prec = prec;
+  c->allocated_precond = pc;
+  c->allocated_levels = L;
+  return SP_OK;
+}
+
+static bool check_params(const sp_params* p, std::string& why) {
+  if (p->imax < 2 || p->jmax < 2 || p->kmax < 2) { why = "imax, jmax, kmax must be >= 2"; return false; }
+  if ((int64_t)p->imax * p->jmax * p->kmax >= ((int64_t)1 << 31)) { why = "grid too large"; return false; }
+  if (p->precision != SP_PREC_FP64 && p->precision != SP_PREC_MIXED) { why = "precision"; return false; }
+  if (p->precond != SP_PRECOND_MG && p->precond != SP_PRECOND_JACOBI) { why = "precond"; return false; }
+  if (p->mg_pre < 1 || p->mg_post < 0 || p->mg_coarse_sweeps < 1) { why = "sweeps"; return false; }
+  if (p->max_iter < 1) { why = "max_iter"; return false; }
+  if (!(p->div_tol >= 0.0)) { why = "div_tol"; return false; }
+  if (!(p->rho_l > 0) || !(p->rho_g > 0) || !(p->rho_ref > 0)) { why = "densities"; return false; }
+  if (!(p->mg_coarse_scale > 0)) { why = "mg_coarse_scale"; return false; }
+  return true;
+}
+
+static sp_status check_inputs(sp_ctx* c, const sp_inputs* in, const sp_outputs* out) {
+  if (!in || !out) return fail(c, SP_ERR_INVALID_ARG, "null inputs/outputs");
+  const void* req[] = {in->u_star, in->v_star, in->w_star, in->dp_u, in->dp_v, in->dp_w,
+                       in->ee_area, in->ne_area, in->te_area, in->vof, in->vof_old,
+                       in->cell_type_p, in->cell_type_u, in->cell_type_v, in->cell_type_w,
+                       in->moex_u, in->moex_v, in->moex_w, out->p, out->u, out->v, out->w};
+  for (const void* p : req)
+    if (!p) return fail(c, SP_ERR_INVALID_ARG, "required input/output pointer is null");
+  const bool beta = in->beta_e || in->beta_n || in->beta_t;
+  if (beta && !(in->beta_e && in->beta_n && in->beta_t))
+    return fail(c, SP_ERR_INVALID_ARG, "beta_e, beta_n, beta_t must be given together");
+  if (!beta && (!in->vof_l || !in->phi_l))
+    return fail(c, SP_ERR_INVALID_ARG, "vof_l and phi_l are required for the GFM coefficients");
+  for (int f = 0; f < 6; ++f) {
+    if (in->bc_flag[f] != 0 && in->bc_flag[f] != 1)
+      return fail(c, SP_ERR_INVALID_ARG, "bc_flag must be 0 (Dirichlet) or 1 (Neumann)");
+    if (in->bc_flag[f] == 0 && !in->bc_val[f])
+      return fail(c, SP_ERR_INVALID_ARG, "bc_val required for a Dirichlet face");
+  }
+  if (!(in->dt > 0.0) || !std::isfinite(in->dt)) return fail(c, SP_ERR_INVALID_ARG, "dt must be > 0");
+  if (!c->grid_set) return fail(c, SP_ERR_GRID, "sp_set_grid has not been called");
+  return SP_OK;
+}
+
+static size_t bc_len(const Dims& D, int f) {
+  return f < 2 ? (size_t)D.J * D.K : (f < 4 ? (size_t)D.I * D.K : (size_t)D.I * D.J);
+}
+
+// Forward on device arrays.  `host_bc`: bc_val pointers are host memory.
+static sp_status project_device(sp_ctx* c, const sp_inputs* in, sp_outputs* out, sp_stats* st,
+                                bool host_bc) {
+  auto t0 = std::chrono::steady_clock::now();
+  sp_status s = ensure_buffers(c);
+  if (s) return s;
+  const Dims& D = c->dm;
+  hipError_t e = hipMemsetAsync(c->S, 0, sizeof(Scalars), c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "hipMemsetAsync");
+  for (int f = 0; f < 6; ++f) {
+    if (in->bc_flag[f] != 0) continue;
+    e = hipMemcpyAsync(c->bcv[f], in->bc_val[f], sizeof(double) * bc_len(D, f),
+                       host_bc ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(bc)");
+  }
+  c->have_forward = false;
+  int64_t stale = 0;
+  if ((s = assemble(c, *in, &stale))) return s;
+  e = hipMemcpyAsync(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "assembly");
+  auto t1 = std::chrono::steady_clock::now();
+  if (c->S_host->nonfinite) return fail(c, SP_ERR_NONFINITE, "non-finite value in the assembled system");
+  c->has_e = c->S_host->esum > 0.0;
+  if ((s = mg_setup(c))) return s;
+  sp_stats local{};
+  sp_stats* S = st ? st : &local;
+  std::memset(S, 0, sizeof(*S));
+  S->initial_div = c->S_host->bmax;
+  S->singular = c->S_host->singular;
+  S->active_cells = c->S_host->active;
+  S->stale_faces = stale;
+  s = solve(c, c->b, c->x, true, S);
+  if (s != SP_OK && s != SP_ERR_NOT_CONVERGED) return s;
+  auto t2 = std::chrono::steady_clock::now();
+  sp_status sc = correct(c, *in, *out, S);
+  if (sc) return sc;
+  if (out->po_coef) {
+    e = hipMemcpyAsync(out->po_coef, c->pocoef, sizeof(double) * 6 * D.N, hipMemcpyDeviceToDevice,
+                       c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(po_coef)");
+  }
+  // checkpoint of the forward pressure for the adjoint
+  e = hipMemcpyAsync(c->x_ckpt, c->x, sizeof(double) * D.N, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "correction");
+  auto t3 = std::chrono::steady_clock::now();
+  S->final_div = c->S_host->divmax;
+  S->converged = S->converged && S->final_div <= c->prm.div_tol;
+  S->t_assemble_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  S->t_solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  S->t_correct_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
+  S->t_total_ms = std::chrono::duration<double, std::milli>(t3 - t0).count();
+  c->last_in = *in;
+  c->have_forward = true;
+  timers_collect(c);
+  return S->converged ? SP_OK : SP_ERR_NOT_CONVERGED;
+}
+
+// Device staging of host arrays
+static sp_status stage(sp_ctx* c, size_t idx, size_t bytes, void** out) {
+  if (c->stage.size() <= idx) c->stage.resize(idx + 1);
+  DeviceBuf& b = c->stage[idx];
+  if (b.bytes < bytes) {
+    if (b.ptr) hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess) return hip_check(c, e, "hipMalloc(stage)");
+    b.bytes = bytes;
+  }
+  *out = b.ptr;
+  return SP_OK;
+}
+
+template <typename T>
+static sp_status upload(sp_ctx* c, size_t idx, const T* src, size_t count, const T** dst) {
+  if (!src) { *dst = nullptr; return SP_OK; }
+  void* d;
+  sp_status s = stage(c, idx, count * sizeof(T), &d);
+  if (s) return s;
+  hipError_t e = hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(H2D)");
+  *dst = static_cast<const T*>(d);
+  return SP_OK;
+}
+
+}  // namespace sp
+
+using namespace sp;
+
+extern "C" {
+
+void sp_params_default(sp_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->precision = SP_PREC_FP64;
+  p->precond = SP_PRECOND_MG;
+  p->mg_levels = 0;
+  p->mg_pre = 2;
+  p->mg_post = 2;
+  p->mg_coarse_sweeps = 4;
+  p->mg_coarse_scale = 0.5;
+  p->max_iter = 200;
+  p->div_tol = 1.0e-10;
+  p->rho_l = 998.3;   // row, StateVariables.f90:10
+  p->rho_g = 1.205;   // roa
+  p->rho_ref = 998.3; // Roref = row (ProjectionP.f90:53)
+  p->epsi = 1.0e-3;   // CutCell.f90:11
+  p->pin_i = p->pin_j = p->pin_k = 0;
+}
+
+sp_status sp_create(sp_ctx** out, const sp_params* params, int device) {
+  if (!out || !params) return SP_ERR_INVALID_ARG;
+  *out = nullptr;
+  std::string why;
+  if (!check_params(params, why)) return SP_ERR_INVALID_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev)
+    return SP_ERR_NO_DEVICE;
+  sp_ctx* c = new (std::nothrow) sp_ctx();
+  if (!c) return SP_ERR_OOM;
+  c->prm = *params;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return SP_ERR_HIP;
+  }
+  Dims& D = c->dm;
+  D.I = params->imax; D.J = params->jmax; D.K = params->kmax;
+  D.N = (int64_t)D.I * D.J * D.K;
+  D.SX = D.I + 2; D.SY = D.J + 2;
+  D.Nh = (int64_t)D.SX * D.SY * (D.K + 2);
+  if (hipMalloc(&c->S, sizeof(Scalars)) != hipSuccess ||
+      hipHostMalloc(&c->S_host, sizeof(Scalars), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&c->hflags, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+      hipMalloc(&c->dxp, sizeof(double) * D.I) != hipSuccess ||
+      hipMalloc(&c->dyp, sizeof(double) * D.J) != hipSuccess ||
+      hipMalloc(&c->dzp, sizeof(double) * D.K) != hipSuccess ||
+      hipMalloc(&c->dxu, sizeof(double) * D.I) != hipSuccess ||
+      hipMalloc(&c->dyv, sizeof(double) * D.J) != hipSuccess ||
+      hipMalloc(&c->dzw, sizeof(double) * D.K) != hipSuccess) {
+    sp_destroy(c);
+    return SP_ERR_OOM;
+  }
+  std::memset(c->S_host, 0, sizeof(Scalars));
+  *out = c;
+  return SP_OK;
+}
+
+sp_status sp_destroy(sp_ctx* c) {
+  if (!c) return SP_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_buffers(c);
+  for (auto& s : c->stage) if (s.ptr) hipFree(s.ptr);
+  c->stage.clear();
+  for (auto& p : c->ev_pool) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
+  if (c->S) hipFree(c->S);
+  if (c->S_host) hipHostFree(c->S_host);
+  if (c->hflags) hipHostFree(c->hflags);
+  for (double* p : {c->dxp, c->dyp, c->dzp, c->dxu, c->dyv, c->dzw}) if (p) hipFree(p);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return SP_OK;
+}
+
+sp_status sp_set_params(sp_ctx* c, const sp_params* p) {
+  if (!c || !p) return SP_ERR_INVALID_ARG;
+  std::string why;
+  if (!check_params(p, why)) return fail(c, SP_ERR_INVALID_ARG, why);
+  if (p->imax != c->dm.I || p->jmax != c->dm.J || p->kmax != c->dm.K)
+    return fail(c, SP_ERR_INVALID_ARG, "sp_set_params cannot change the grid size");
+  c->prm = *p;
+  c->have_forward = false;
+  return SP_OK;
+}
+
+sp_status sp_set_grid_1d(sp_ctx* c, const double* dxp, const double* dyp, const double* dzp,
+                         const double* dxu, const double* dyv, const double* dzw) {
+  if (!c || !dxp || !dyp || !dzp || !dxu || !dyv || !dzw) return SP_ERR_INVALID_ARG;
+  const Dims& D = c->dm;
+  const double* in[6] = {dxp, dyp, dzp, dxu, dyv, dzw};
+  const int len[6] = {D.I, D.J, D.K, D.I, D.J, D.K};
+  for (int q = 0; q < 6; ++q)
+    for (int i = 0; i < len[q]; ++i)
+      if (!(in[q][i] > 0.0) || !std::isfinite(in[q][i]))
+        return fail(c, SP_ERR_GRID, "grid spacings must be finite and > 0");
+  hipSetDevice(c->device);
+  double* dst[6] = {c->dxp, c->dyp, c->dzp, c->dxu, c->dyv, c->dzw};
+  for (int q = 0; q < 6; ++q) {
+    hipError_t e = hipMemcpy(dst[q], in[q], sizeof(double) * len[q], hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_check(c, e, "hipMemcpy(grid)");
+  }
+  c->grid_set = true;
+  c->have_forward = false;
+  return SP_OK;
+}
+
+sp_status sp_set_grid(sp_ctx* c, const double* dxp, const double* dyp, const double* dzp,
+                      const double* dxu, const double* dyv, const double* dzw) {
+  if (!c || !dxp || !dyp || !dzp || !dxu || !dyv || !dzw) return SP_ERR_INVALID_ARG;
+  const Dims& D = c->dm;
+  std::vector<double> x(D.I), y(D.J), z(D.K), xu(D.I), yv(D.J), zw(D.K);
+  auto at = [&](const double* a, int i, int j, int k) {
+    return a[i + (int64_t)D.I * (j + (int64_t)D.J * k)];
+  };
+  for (int i = 0; i < D.I; ++i) { x[i] = at(dxp, i, 0, 0); xu[i] = at(dxu, i, 0, 0); }
+  for (int j = 0; j < D.J; ++j) { y[j] = at(dyp, 0, j, 0); yv[j] = at(dyv, 0, j, 0); }
+  for (int k = 0; k < D.K; ++k) { z[k] = at(dzp, 0, 0, k); zw[k] = at(dzw, 0, 0, k); }
+  for (int k = 0; k < D.K; ++k)
+    for (int j = 0; j < D.J; ++j)
+      for (int i = 0; i < D.I; ++i)
+        if (at(dxp, i, j, k) != x[i] || at(dxu, i, j, k) != xu[i] || at(dyp, i, j, k) != y[j] ||
+            at(dyv, i, j, k) != yv[j] || at(dzp, i, j, k) != z[k] || at(dzw, i, j, k) != zw[k])
+          return fail(c, SP_ERR_GRID, "grid is not a tensor product");
+  return sp_set_grid_1d(c, x.data(), y.data(), z.data(), xu.data(), yv.data(), zw.data());
+}
+
+sp_status sp_project_dev(sp_ctx* c, const sp_inputs* in, sp_outputs* out, sp_stats* st) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  sp_status s = check_inputs(c, in, out);
+  if (s) return s;
+  c->last_in_host = false;
+  return project_device(c, in, out, st, false);
+}
+
+sp_status sp_project(sp_ctx* c, const sp_inputs* in, sp_outputs* out, sp_stats* st) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  sp_status s = check_inputs(c, in, out);
+  if (s) return s;
+  if ((s = ensure_buffers(c))) return s;
+  const Dims& D = c->dm;
+  const size_t Nh = (size_t)D.Nh, N = (size_t)D.N;
+  sp_inputs d = *in;
+  size_t k = 0;
+  const double** hp[] = {&d.u_star, &d.v_star, &d.w_star, &d.dp_u, &d.dp_v, &d.dp_w, &d.ee_area,
+                         &d.ne_area, &d.te_area, &d.vof, &d.vof_old, &d.vof_l, &d.phi_l};
+  for (const double** p : hp)
+    if ((s = upload(c, k++, *p, Nh, p))) return s;
+  const int32_t** ip[] = {&d.cell_type_p, &d.cell_type_u, &d.cell_type_v, &d.cell_type_w,
+                          &d.moex_u, &d.moex_v, &d.moex_w};
+  for (const int32_t** p : ip)
+    if ((s = upload(c, k++, *p, N, p))) return s;
+  const double** bp[] = {&d.beta_e, &d.beta_n, &d.beta_t};
+  for (const double** p : bp)
+    if ((s = upload(c, k++, *p, N, p))) return s;
+  // outputs: upload the caller's u,v,w so entries outside (1:I,1:J,1:K) are kept
+  sp_outputs o{};
+  double* outs_h[4] = {out->p, out->u, out->v, out->w};
+  double* outs_d[4];
+  for (int q = 0; q < 4; ++q) {
+    void* p;
+    if ((s = stage(c, k++, Nh * sizeof(double), &p))) return s;
+    outs_d[q] = static_cast<double*>(p);
+    if (q > 0) {
+      hipError_t e = hipMemcpyAsync(p, outs_h[q], Nh * sizeof(double), hipMemcpyHostToDevice, c->stream);
+      if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(H2D)");
+    }
+  }
+  o.p = outs_d[0]; o.u = outs_d[1]; o.v = outs_d[2]; o.w = outs_d[3];
+  o.po_coef = nullptr;
+  c->last_in_host = true;
+  sp_status r = project_device(c, &d, &o, st, true);
+  if (r != SP_OK && r != SP_ERR_NOT_CONVERGED) return r;
+  for (int q = 0; q < 4; ++q) {
+    hipError_t e = hipMemcpyAsync(outs_h[q], outs_d[q], Nh * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(D2H)");
+  }
+  if (out->po_coef) {
+    hipError_t e = hipMemcpyAsync(out->po_coef, c->pocoef, 6 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(po_coef)");
+  }
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "sp_project");
+  return r;
+}
+
+static sp_status adjoint_common(sp_ctx* c, const sp_adj_inputs* in, sp_adj_outputs* out,
+                                sp_stats* st) {
+  sp_stats local{};
+  sp_stats* S = st ? st : &local;
+  std::memset(S, 0, sizeof(*S));
+  auto t0 = std::chrono::steady_clock::now();
+  hipError_t e0 = hipMemsetAsync(&c->S->iter, 0, sizeof(int32_t), c->stream);
+  if (e0 == hipSuccess) e0 = hipMemsetAsync(&c->S->nonfinite, 0, sizeof(int32_t), c->stream);
+  if (e0 != hipSuccess) return hip_check(c, e0, "hipMemsetAsync");
+  sp_status s = adjoint(c, *in, *out, S);
+  if (s != SP_OK && s != SP_ERR_NOT_CONVERGED) return s;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "adjoint");
+  S->t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  S->singular = c->S_host->singular;
+  S->active_cells = c->S_host->active;
+  timers_collect(c);
+  // the forward's x has been reused as the adjoint variable; the checkpoint
+  // keeps the pressure, so further adjoint calls stay valid
+  return S->converged ? SP_OK : SP_ERR_NOT_CONVERGED;
+}
+
+sp_status sp_adjoint_dev(sp_ctx* c, const sp_adj_inputs* in, sp_adj_outputs* out, sp_stats* st) {
+  if (!c || !in || !out) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  if (!c->have_forward) return fail(c, SP_ERR_STATE, "no forward projection to differentiate");
+  if (!in->gu || !in->gv || !in->gw || !out->gu_star || !out->gv_star || !out->gw_star ||
+      !out->gk_e || !out->gk_n || !out->gk_t)
+    return fail(c, SP_ERR_INVALID_ARG, "null adjoint pointer");
+  return adjoint_common(c, in, out, st);
+}
+
+sp_status sp_adjoint(sp_ctx* c, const sp_adj_inputs* in, sp_adj_outputs* out, sp_stats* st) {
+  if (!c || !in || !out) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  if (!c->have_forward) return fail(c, SP_ERR_STATE, "no forward projection to differentiate");
+  if (!c->last_in_host)
+    return fail(c, SP_ERR_STATE, "forward ran on device arrays; use sp_adjoint_dev");
+  if (!in->gu || !in->gv || !in->gw || !out->gu_star || !out->gv_star || !out->gw_star ||
+      !out->gk_e || !out->gk_n || !out->gk_t)
+    return fail(c, SP_ERR_INVALID_ARG, "null adjoint pointer");
+  const Dims& D = c->dm;
+  const size_t Nh = (size_t)D.Nh, N = (size_t)D.N;
+  sp_status s;
+  size_t k = 40;  // staging slots after the forward's
+  sp_adj_inputs di{};
+  if ((s = upload(c, k++, in->gu, Nh, &di.gu)) || (s = upload(c, k++, in->gv, Nh, &di.gv)) ||
+      (s = upload(c, k++, in->gw, Nh, &di.gw)))
+    return s;
+  sp_adj_outputs dout{};
+  double* hs[9] = {out->gu_star, out->gv_star, out->gw_star, out->gk_e, out->gk_n, out->gk_t,
+                   out->gbeta_e, out->gbeta_n, out->gbeta_t};
+  double* ds[9];
+  for (int q = 0; q < 9; ++q) {
+    ds[q] = nullptr;
+    if (!hs[q]) continue;
+    void* p;
+    if ((s = stage(c, k++, (q < 3 ? Nh : N) * sizeof(double), &p))) return s;
+    ds[q] = static_cast<double*>(p);
+  }
+  dout.gu_star = ds[0]; dout.gv_star = ds[1]; dout.gw_star = ds[2];
+  dout.gk_e = ds[3]; dout.gk_n = ds[4]; dout.gk_t = ds[5];
+  dout.gbeta_e = ds[6]; dout.gbeta_n = ds[7]; dout.gbeta_t = ds[8];
+  sp_status r = adjoint_common(c, &di, &dout, st);
+  if (r != SP_OK && r != SP_ERR_NOT_CONVERGED) return r;
+  for (int q = 0; q < 9; ++q) {
+    if (!hs[q]) continue;
+    hipError_t e = hipMemcpyAsync(hs[q], ds[q], (q < 3 ? Nh : N) * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(D2H)");
+  }
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "sp_adjoint");
+  return r;
+}
+
+void* sp_stream(sp_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+void sp_set_profiling(sp_ctx* c, int on) {
+  if (!c) return;
+  c->profiling = on != 0;
+  c->timers.clear();
+  c->ev_used.clear();
+}
+
+int sp_kernel_times(sp_ctx* c, sp_kernel_time* out, int cap) {
+  if (!c) return 0;
+  const int n = (int)c->timers.size();
+  for (int q = 0; q < n && q < cap && out; ++q) {
+    std::memset(&out[q], 0, sizeof(out[q]));
+    std::snprintf(out[q].name, sizeof(out[q].name), "%s", c->timers[q].name.c_str());
+    out[q].launches = c->timers[q].launches;
+    out[q].total_ms = c->timers[q].total_ms;
+    out[q].bytes = c->timers[q].bytes;
+  }
+  return n;
+}
+
+const char* sp_strerror(sp_status s) {
+  switch (s) {
+    case SP_OK: return "ok";
+    case SP_ERR_INVALID_ARG: return "invalid argument";
+    case SP_ERR_HIP: return "HIP runtime error";
+    case SP_ERR_NOT_CONVERGED: return "solver did not reach the divergence tolerance";
+    case SP_ERR_NONFINITE: return "non-finite value";
+    case SP_ERR_OOM: return "device out of memory";
+    case SP_ERR_GRID: return "grid not set or not a tensor-product grid";
+    case SP_ERR_STATE: return "no forward projection to differentiate";
+    case SP_ERR_NO_DEVICE: return "no usable HIP device";
+  }
+  return "unknown status";
+}
+
+const char* sp_last_error(sp_ctx* c) { return c ? c->last_error.c_str() : ""; }
+
+}  // extern "C"

@@ -1,0 +1,486 @@
+// Assembly on the GPU: GFM face coefficients (ProjectionP.f90:123-265),
+// the symmetric 7-point operator and the right-hand side
+// (ProjectionP.f90:301-576).
+//
+// Every kernel here evaluates the reference's expressions in the reference's
+// operation order with floating-point contraction disabled, so PoCoef and b
+// are bit-identical to a serial evaluation of the Fortran code.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "sp_device.h"
+#include "sp_geom.h"
+#include "sp_internal.h"
+
+namespace sp {
+namespace {
+
+// Cell classes of the GFM (ProjectionP.f90:170-174, 212-216).
+constexpr int kNone = 0, kWet = 1, kDry = 2;
+
+__device__ inline int gfm_class(double vof, double vofL, double phiL, double epsi) {
+  const double one_m = 1.0 - epsi;
+  if ((vofL >= 0.5 && vof > one_m) || (phiL < 0.0 && vof <= one_m && vof > epsi)) return kWet;
+  if ((vofL < 0.5 && vof > one_m) || (phiL >= 0.0 && vof <= one_m && vof > epsi)) return kDry;
+  return kNone;
+}
+
+struct GfmArgs {
+  int I, J, K, SX, SY;
+  int64_t N;
+  const double *phiL, *vofL, *vof;
+  const double* dp[3];     // Dp on U, V, W faces (halo)
+  const double* dgrid[3];  // dxu, dyv, dzw
+  double betaP, betaM, epsi;
+  double* pocoef;
+  int32_t* stale;          // (n, face, kind) triples
+  uint32_t stale_cap;
+  uint32_t* stale_count;
+  int64_t* col_last;       // [12][I*J] loop position of the last assignment
+  int64_t* col_prefix;     // [12][I*J]
+};
+
+__device__ inline int64_t hstride(const GfmArgs& A, int dir) {
+  return dir == 0 ? 1 : (dir == 1 ? (int64_t)A.SX : (int64_t)A.SX * A.SY);
+}
+
+// What the reference's loop body does for face f at 0-based cell (a,b,c):
+//   returns kWet / kDry when the branch assigns BetaW / BetaD (value in beta),
+//   -kWet / -kDry when the neighbour is unclassified (stale value reused),
+//   kNone when the P cell is unclassified (coefficient stays 0).
+__device__ inline int gfm_face(const GfmArgs& A, int f, int a, int b, int c, double& beta,
+                               double& pref) {
+#pragma clang fp contract(off)
+  const int dir = f % 3;
+  const int side = f < 3 ? -1 : 1;
+  const int64_t h = (int64_t)(a + 1) + (int64_t)A.SX * ((int64_t)(b + 1) + (int64_t)A.SY * (c + 1));
+  const int64_t s = hstride(A, dir);
+  const int64_t q = h + side * s;
+  const int coord = dir == 0 ? a : (dir == 1 ? b : c);
+  // DGrid: TGrid%d(max(im,1)) with im = i-1 on the minus side (:151-153)
+  const double dgrid = A.dgrid[dir][side < 0 ? (coord > 0 ? coord - 1 : 0) : coord];
+  const double Dp = A.dp[dir][side < 0 ? h - s : h];
+  const double fP = A.phiL[h], fQ = A.phiL[q];
+  const double lam = fabs(fP) / (fabs(fP) + fabs(fQ) + 1.0e-30);  // :155
+  const int cP = gfm_class(A.vof[h], A.vofL[h], fP, A.epsi);
+  // prefactor Dp/DGrid*BetaP*BetaM of :194 / :235
+  pref = Dp / dgrid * A.betaP * A.betaM;
+  if (cP == kNone) return kNone;
+  const int cQ = gfm_class(A.vof[q], A.vofL[q], fQ, A.epsi);
+  if (cP == kWet) {
+    if (cQ == kDry) { beta = lam * A.betaM + (1.0 - lam) * A.betaP; return kWet; }  // :176
+    if (cQ == kWet) { beta = A.betaM; return kWet; }                                 // :181
+    return -kWet;
+  }
+  if (cQ == kWet) { beta = lam * A.betaP + (1.0 - lam) * A.betaM; return kDry; }     // :218
+  if (cQ == kDry) { beta = A.betaP; return kDry; }                                   // :223
+  return -kDry;
+}
+
+__global__ void __launch_bounds__(256) k_gfm(GfmArgs A) {
+#pragma clang fp contract(off)
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y * blockDim.y + threadIdx.y;
+  const int c = blockIdx.z;
+  if (a >= A.I || b >= A.J) return;
+  const int64_t n = (int64_t)a + (int64_t)A.I * ((int64_t)b + (int64_t)A.J * c);
+#pragma unroll 1
+  for (int f = 0; f < 6; ++f) {
+    double beta = 0.0, pref = 0.0;
+    const int r = gfm_face(A, f, a, b, c, beta, pref);
+    double coef = 0.0;
+    if (r > 0) {
+      coef = pref / beta;
+    } else if (r < 0) {
+      const uint32_t slot = atomicAdd(A.stale_count, 1u);
+      if (slot < A.stale_cap) {
+        A.stale[3 * (int64_t)slot + 0] = (int32_t)n;
+        A.stale[3 * (int64_t)slot + 1] = f;
+        A.stale[3 * (int64_t)slot + 2] = -r;
+      }
+    }
+    A.pocoef[(int64_t)f * A.N + n] = coef;
+  }
+}
+
+// Per (column (i,j), face): loop position of the last cell of the column that
+// assigns BetaW (kind 1) and BetaD (kind 2), -1 if none.  Columns are the
+// innermost k loop of the reference (ProjectionP.f90:142-144).
+__global__ void __launch_bounds__(256) k_gfm_col_last(GfmArgs A) {
+  const int64_t cols = (int64_t)A.I * A.J;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cols * 6) return;
+  const int f = (int)(t / cols);
+  const int64_t col = t % cols;          // col = a*J + b (i outer, j inner)
+  const int a = (int)(col / A.J), b = (int)(col % A.J);
+  int64_t lastW = -1, lastD = -1;
+  for (int c = A.K - 1; c >= 0 && (lastW < 0 || lastD < 0); --c) {
+    double beta, pref;
+    const int r = gfm_face(A, f, a, b, c, beta, pref);
+    const int64_t pos = col * A.K + c;
+    if (r == kWet && lastW < 0) lastW = pos;
+    if (r == kDry && lastD < 0) lastD = pos;
+  }
+  A.col_last[(int64_t)(2 * f + 0) * cols + col] = lastW;
+  A.col_last[(int64_t)(2 * f + 1) * cols + col] = lastD;
+}
+
+// Exclusive prefix-max over columns (positions grow along the loop order, so
+// "last assignment before column c" is the maximum over earlier columns).
+// One workgroup of 1024 threads per (face, kind).
+__global__ void __launch_bounds__(1024) k_gfm_col_prefix(GfmArgs A) {
+  const int64_t cols = (int64_t)A.I * A.J;
+  const int64_t* src = A.col_last + (int64_t)blockIdx.x * cols;
+  int64_t* dst = A.col_prefix + (int64_t)blockIdx.x * cols;
+  const int tid = threadIdx.x;
+  const int64_t chunk = (cols + 1023) / 1024;
+  const int64_t lo = tid * chunk, hi = lo + chunk < cols ? lo + chunk : cols;
+  int64_t m = -1;
+  for (int64_t q = lo; q < hi; ++q) m = src[q] > m ? src[q] : m;
+  __shared__ int64_t sm[1024];
+  sm[tid] = m;
+  __syncthreads();
+  // inclusive Hillis-Steele max-scan
+  for (int off = 1; off < 1024; off <<= 1) {
+    int64_t v = tid >= off ? sm[tid - off] : -1;
+    __syncthreads();
+    sm[tid] = v > sm[tid] ? v : sm[tid];
+    __syncthreads();
+  }
+  int64_t run = tid > 0 ? sm[tid - 1] : -1;
+  for (int64_t q = lo; q < hi; ++q) {
+    dst[q] = run;
+    run = src[q] > run ? src[q] : run;
+  }
+}
+
+// Resolve the faces whose neighbour is unclassified: the reference reuses the
+// BetaW/BetaD left by the most recent assigning cell in its i-j-k loop.
+__global__ void __launch_bounds__(256) k_gfm_fixup(GfmArgs A, uint32_t count) {
+#pragma clang fp contract(off)
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int64_t n = A.stale[3 * (int64_t)t + 0];
+  const int f = A.stale[3 * (int64_t)t + 1];
+  const int kind = A.stale[3 * (int64_t)t + 2];
+  const int a = (int)(n % A.I), b = (int)((n / A.I) % A.J), c = (int)(n / ((int64_t)A.I * A.J));
+  const int64_t cols = (int64_t)A.I * A.J;
+  const int64_t col = (int64_t)a * A.J + b;
+  int64_t pos = -1;
+  for (int cc = c - 1; cc >= 0; --cc) {
+    double beta, pref;
+    if (gfm_face(A, f, a, b, cc, beta, pref) == kind) { pos = col * A.K + cc; break; }
+  }
+  if (pos < 0) pos = A.col_prefix[(int64_t)(2 * f + kind - 1) * cols + col];
+  double beta;
+  if (pos >= 0) {
+    const int64_t pc = pos / A.K;
+    const int cc = (int)(pos % A.K);
+    double pref2;
+    gfm_face(A, f, (int)(pc / A.J), (int)(pc % A.J), cc, beta, pref2);
+  } else {
+    // no earlier assignment: the reference's local is undefined here; use the
+    // P cell's own phase
+    beta = kind == kWet ? A.betaM : A.betaP;
+  }
+  double b0, pref;
+  gfm_face(A, f, a, b, c, b0, pref);
+  A.pocoef[(int64_t)f * A.N + n] = pref / beta;
+}
+
+// Boundary faces copy the first interior value (ProjectionP.f90:256-264).
+__global__ void __launch_bounds__(256) k_gfm_boundary(GfmArgs A) {
+  const int I = A.I, J = A.J, K = A.K;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int face = blockIdx.y;  // 0..5
+  double* T = A.pocoef + (int64_t)face * A.N;
+  auto idx = [&](int a, int b, int c) { return (int64_t)a + (int64_t)I * ((int64_t)b + (int64_t)J * c); };
+  if (face == 0 || face == 3) {  // W / E: plane over (j,k)
+    if (t >= (int64_t)J * K) return;
+    const int b = (int)(t % J), c = (int)(t / J);
+    if (face == 0) T[idx(0, b, c)] = T[idx(1, b, c)];
+    else T[idx(I - 1, b, c)] = T[idx(I - 2, b, c)];
+  } else if (face == 1 || face == 4) {
+    if (t >= (int64_t)I * K) return;
+    const int a = (int)(t % I), c = (int)(t / I);
+    if (face == 1) T[idx(a, 0, c)] = T[idx(a, 1, c)];
+    else T[idx(a, J - 1, c)] = T[idx(a, J - 2, c)];
+  } else {
+    if (t >= (int64_t)I * J) return;
+    const int a = (int)(t % I), b = (int)(t / I);
+    if (face == 2) T[idx(a, b, 0)] = T[idx(a, b, 1)];
+    else T[idx(a, b, K - 1)] = T[idx(a, b, K - 2)];
+  }
+}
+
+// Beta parametrisation: PoCoef(:,4..6) = Dp/delta * beta, W,S,B from the
+// neighbour's E,N,T value (first plane uses its own).
+struct BetaArgs {
+  int I, J, K, SX, SY;
+  int64_t N;
+  const double* beta[3];
+  const double* dp[3];
+  const double* dgrid[3];
+  double* pocoef;
+};
+
+__global__ void __launch_bounds__(256) k_beta_coef(BetaArgs A) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y * blockDim.y + threadIdx.y;
+  const int c = blockIdx.z;
+  if (a >= A.I || b >= A.J) return;
+  const int64_t n = (int64_t)a + (int64_t)A.I * ((int64_t)b + (int64_t)A.J * c);
+  const int64_t h = (int64_t)(a + 1) + (int64_t)A.SX * ((int64_t)(b + 1) + (int64_t)A.SY * (c + 1));
+  const int coord[3] = {a, b, c};
+  const int64_t ns[3] = {1, A.I, (int64_t)A.I * A.J};
+  const int64_t hs[3] = {1, A.SX, (int64_t)A.SX * A.SY};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double kp = A.dp[d][h] / A.dgrid[d][coord[d]] * A.beta[d][n];
+    A.pocoef[(int64_t)(3 + d) * A.N + n] = kp;
+    const int64_t nm = coord[d] > 0 ? n - ns[d] : n;
+    const int64_t hm = coord[d] > 0 ? h - hs[d] : h;
+    const int cm = coord[d] > 0 ? coord[d] - 1 : coord[d];
+    A.pocoef[(int64_t)d * A.N + n] = A.dp[d][hm] / A.dgrid[d][cm] * A.beta[d][nm];
+  }
+}
+
+struct OpArgs {
+  int I, J, K, SX, SY, kper;
+  int64_t N;
+  const double* pocoef;
+  const double *ee, *ne, *te;
+  const int32_t* ct;
+  int32_t fl[6];
+  const double* bcv[6];
+  const double *u, *v, *w, *vof, *vofO;
+  const double *dxp, *dyp, *dzp;
+  double dt;
+  double *cx, *cy, *cz, *e, *b;
+  float *cx32, *cy32, *cz32, *e32;
+  uint8_t* active;
+  float* dw;
+  double* partials;
+  Scalars* S;
+};
+
+// Symmetric operator, diagonal-only terms and RHS for one cell.
+__global__ void __launch_bounds__(256) k_operator(OpArgs A) {
+#pragma clang fp contract(off)
+  uint32_t bx, by, bz;
+  xcd_block3(bx, by, bz);
+  const int a = bx * kTX + threadIdx.x;
+  const int b = by * kTY + threadIdx.y;
+  const int c0 = bz * A.kper;
+  const int c1 = c0 + A.kper < A.K ? c0 + A.kper : A.K;
+  const int64_t IJ = (int64_t)A.I * A.J, SXY = (int64_t)A.SX * A.SY, N = A.N;
+  double bsum = 0.0, esum = 0.0, bmax = 0.0, cnt = 0.0, bad = 0.0, dsum = 0.0;
+  if (a < A.I && b < A.J) {
+    const double dx = A.dxp[a], dy = A.dyp[b];
+    for (int c = c0; c < c1; ++c) {
+      const double dz = A.dzp[c];
+      const int64_t n = (int64_t)a + (int64_t)A.I * ((int64_t)b + (int64_t)A.J * c);
+      const int64_t h = (int64_t)(a + 1) + (int64_t)A.SX * ((int64_t)(b + 1) + (int64_t)A.SY * (c + 1));
+      const int ctP = A.ct[n];
+      double cxv = 0, cyv = 0, czv = 0, cW = 0, cS = 0, cB = 0, ex = 0, rhs = 0;
+      bool act = false;
+      double dval = 0.0;
+      if (ctP != 2) {
+        const double W = A.pocoef[n], S = A.pocoef[N + n], B = A.pocoef[2 * N + n];
+        const double E = A.pocoef[3 * N + n], Nn = A.pocoef[4 * N + n], T = A.pocoef[5 * N + n];
+        const bool wS = a > 0 ? A.ct[n - 1] == 2 : A.fl[0] == 0;
+        const bool sS = b > 0 ? A.ct[n - A.I] == 2 : A.fl[2] == 0;
+        const bool bS = c > 0 ? A.ct[n - IJ] == 2 : A.fl[4] == 0;
+        const bool eS = a < A.I - 1 ? A.ct[n + 1] == 2 : A.fl[1] == 0;
+        const bool nS = b < A.J - 1 ? A.ct[n + A.I] == 2 : A.fl[3] == 0;
+        const bool tS = c < A.K - 1 ? A.ct[n + IJ] == 2 : A.fl[5] == 0;
+        // couplings to active neighbours: the lower cell's E/N/T coefficient
+        if (a < A.I - 1 && A.ct[n + 1] != 2) cxv = E * A.ee[h] * dy * dz;
+        if (b < A.J - 1 && A.ct[n + A.I] != 2) cyv = Nn * A.ne[h] * dx * dz;
+        if (c < A.K - 1 && A.ct[n + IJ] != 2) czv = T * A.te[h] * dx * dy;
+        if (a > 0 && A.ct[n - 1] != 2) cW = A.pocoef[3 * N + n - 1] * A.ee[h - 1] * dy * dz;
+        if (b > 0 && A.ct[n - A.I] != 2) cS = A.pocoef[4 * N + n - A.I] * A.ne[h - A.SX] * dx * dz;
+        if (c > 0 && A.ct[n - IJ] != 2) cB = A.pocoef[5 * N + n - IJ] * A.te[h - SXY] * dx * dy;
+        // diagonal-only terms with the row's own coefficient (:379-430)
+        if (wS) ex = ex + W * A.ee[h - 1] * dy * dz;
+        if (sS) ex = ex + S * A.ne[h - A.SX] * dx * dz;
+        if (bS) ex = ex + B * A.te[h - SXY] * dx * dy;
+        if (eS) ex = ex + E * A.ee[h] * dy * dz;
+        if (nS) ex = ex + Nn * A.ne[h] * dx * dz;
+        if (tS) ex = ex + T * A.te[h] * dx * dy;
+        double d = cW;
+        d = d + cS;
+        d = d + cB;
+        d = d + cxv;
+        d = d + cyv;
+        d = d + czv;
+        d = d + ex;
+        act = d > 0.0;
+        dval = act ? d : 0.0;
+        if (!is_finite(d)) bad = 1.0;
+        // RHS (:521-546)
+        rhs = -dy * dz * (A.ee[h] * A.u[h] - A.ee[h - 1] * A.u[h - 1]) -
+              dx * dz * (A.ne[h] * A.v[h] - A.ne[h - A.SX] * A.v[h - A.SX]) -
+              dx * dy * (A.te[h] * A.w[h] - A.te[h - SXY] * A.w[h - SXY]);
+        rhs = rhs + (A.vof[h] - A.vofO[h]) * dx * dy * dz / A.dt;
+        if (A.fl[0] == 0 && a == 0) rhs = rhs + W * A.bcv[0][b + (int64_t)A.J * c];
+        if (A.fl[1] == 0 && a == A.I - 1) rhs = rhs + E * A.bcv[1][b + (int64_t)A.J * c];
+        if (A.fl[2] == 0 && b == 0) rhs = rhs + S * A.bcv[2][a + (int64_t)A.I * c];
+        if (A.fl[3] == 0 && b == A.J - 1) rhs = rhs + Nn * A.bcv[3][a + (int64_t)A.I * c];
+        if (A.fl[4] == 0 && c == 0) rhs = rhs + B * A.bcv[4][a + (int64_t)A.I * b];
+        if (A.fl[5] == 0 && c == A.K - 1) rhs = rhs + T * A.bcv[5][a + (int64_t)A.I * b];
+        if (act) {
+          if (!is_finite(rhs)) bad = 1.0;
+          bsum += rhs;
+          esum += fabs(ex);
+          cnt += 1.0;
+          dsum += (double)(float)dval;
+          const double vb = fabs(rhs) / (dx * dy * dz);
+          bmax = vb > bmax ? vb : bmax;
+        }
+      }
+      A.cx[n] = cxv; A.cy[n] = cyv; A.cz[n] = czv; A.e[n] = ex; A.b[n] = rhs;
+      A.active[n] = act ? 1 : 0;
+      A.dw[n] = (float)dval;
+      if (A.cx32) {
+        A.cx32[n] = (float)cxv; A.cy32[n] = (float)cyv; A.cz32[n] = (float)czv;
+        A.e32[n] = (float)ex;
+      }
+    }
+  }
+  const uint32_t blk = flat_block(), nb = num_blocks();
+  double v0 = block_sum<kTileThreads>(bsum);
+  double v1 = block_sum<kTileThreads>(esum);
+  double v2 = block_max<kTileThreads>(bmax);
+  double v3 = block_sum<kTileThreads>(cnt);
+  double v4 = block_max<kTileThreads>(bad);
+  double v5 = block_sum<kTileThreads>(dsum);
+  if (flat_tid() == 0) {
+    store_partial(A.partials + 0 * (int64_t)kMaxPartials, blk, v0);
+    store_partial(A.partials + 1 * (int64_t)kMaxPartials, blk, v1);
+    store_partial(A.partials + 2 * (int64_t)kMaxPartials, blk, v2);
+    store_partial(A.partials + 3 * (int64_t)kMaxPartials, blk, v3);
+    store_partial(A.partials + 4 * (int64_t)kMaxPartials, blk, v4);
+    store_partial(A.partials + 5 * (int64_t)kMaxPartials, blk, v5);
+  }
+  if (last_block(&A.S->counter[kCtrOperator])) {
+    const double s0 = reduce_partials_sum<kTileThreads>(A.partials, nb);
+    const double s1 = reduce_partials_sum<kTileThreads>(A.partials + kMaxPartials, nb);
+    const double s2 = reduce_partials_max<kTileThreads>(A.partials + 2 * (int64_t)kMaxPartials, nb);
+    const double act = reduce_partials_sum<kTileThreads>(A.partials + 3 * (int64_t)kMaxPartials, nb);
+    const double s4 = reduce_partials_max<kTileThreads>(A.partials + 4 * (int64_t)kMaxPartials, nb);
+    const double s5 = reduce_partials_sum<kTileThreads>(A.partials + 5 * (int64_t)kMaxPartials, nb);
+    if (flat_tid() == 0) {
+      const bool nonfin = s4 > 0.0;
+      Scalars* S = A.S;
+      S->bsum = s0;
+      S->esum = s1;
+      S->bmax = s2;
+      S->dsum = s5;
+      S->active = (int64_t)act;
+      S->singular = s1 == 0.0 ? 1 : 0;
+      S->bmean = (s1 == 0.0 && act > 0) ? s0 / act : 0.0;
+      S->nonfinite = (nonfin || !is_finite(s0) || !is_finite(s2)) ? 1 : 0;
+    }
+    reset_counter(&A.S->counter[kCtrOperator]);
+  }
+}
+
+}  // namespace
+
+sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_out) {
+  const Dims& D = c->dm;
+  const bool mixed = c->prm.precision == SP_PREC_MIXED;
+  dim3 blk(64, 4, 1), grd((D.I + 63) / 64, (D.J + 3) / 4, D.K);
+  int slot;
+  uint32_t stale = 0;
+  if (in.beta_e) {
+    BetaArgs B{};
+    B.I = D.I; B.J = D.J; B.K = D.K; B.SX = D.SX; B.SY = D.SY; B.N = D.N;
+    B.beta[0] = in.beta_e; B.beta[1] = in.beta_n; B.beta[2] = in.beta_t;
+    B.dp[0] = in.dp_u; B.dp[1] = in.dp_v; B.dp[2] = in.dp_w;
+    B.dgrid[0] = c->dxu; B.dgrid[1] = c->dyv; B.dgrid[2] = c->dzw;
+    B.pocoef = c->pocoef;
+    timer_begin(c, "beta_coef", (double)D.N * (3 * 8 + 6 * 8 + 3 * 8), &slot);
+    hipLaunchKernelGGL(k_beta_coef, grd, blk, 0, c->stream, B);
+    timer_end(c, slot);
+  } else {
+    GfmArgs G{};
+    G.I = D.I; G.J = D.J; G.K = D.K; G.SX = D.SX; G.SY = D.SY; G.N = D.N;
+    G.phiL = in.phi_l; G.vofL = in.vof_l; G.vof = in.vof;
+    G.dp[0] = in.dp_u; G.dp[1] = in.dp_v; G.dp[2] = in.dp_w;
+    G.dgrid[0] = c->dxu; G.dgrid[1] = c->dyv; G.dgrid[2] = c->dzw;
+    G.betaP = 1.0 / (c->prm.rho_l / c->prm.rho_ref);  // ProjectionP.f90:140
+    G.betaM = 1.0 / (c->prm.rho_g / c->prm.rho_ref);  // ProjectionP.f90:141
+    G.epsi = c->prm.epsi;
+    G.pocoef = c->pocoef;
+    G.stale = c->stale_list;
+    G.stale_cap = (uint32_t)c->stale_cap;
+    G.stale_count = &c->S->stale_count;
+    G.col_last = c->col_last;
+    G.col_prefix = c->col_prefix;
+    hipError_t e = hipMemsetAsync(&c->S->stale_count, 0, sizeof(uint32_t), c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "hipMemsetAsync");
+    timer_begin(c, "gfm", (double)D.N * (3 * 8 + 3 * 8 + 6 * 8), &slot);
+    hipLaunchKernelGGL(k_gfm, grd, blk, 0, c->stream, G);
+    timer_end(c, slot);
+    e = hipMemcpyAsync(&c->S_host->stale_count, &c->S->stale_count, sizeof(uint32_t),
+                       hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync");
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "k_gfm");
+    stale = c->S_host->stale_count;
+    if (stale > c->stale_cap)
+      return fail(c, SP_ERR_INVALID_ARG, "too many GFM faces next to unclassified cells");
+    if (stale > 0) {
+      const int64_t cols = (int64_t)D.I * D.J;
+      timer_begin(c, "gfm_col_last", 0, &slot);
+      hipLaunchKernelGGL(k_gfm_col_last, dim3((unsigned)((cols * 6 + 255) / 256)), dim3(256), 0,
+                         c->stream, G);
+      timer_end(c, slot);
+      hipLaunchKernelGGL(k_gfm_col_prefix, dim3(12), dim3(1024), 0, c->stream, G);
+      timer_begin(c, "gfm_fixup", 0, &slot);
+      hipLaunchKernelGGL(k_gfm_fixup, dim3((stale + 255) / 256), dim3(256), 0, c->stream, G,
+                         stale);
+      timer_end(c, slot);
+    }
+    int64_t plane = std::max<int64_t>(std::max<int64_t>((int64_t)D.J * D.K, (int64_t)D.I * D.K),
+                                      (int64_t)D.I * D.J);
+    hipLaunchKernelGGL(k_gfm_boundary, dim3((unsigned)((plane + 255) / 256), 6), dim3(256), 0,
+                       c->stream, G);
+  }
+  if (stale_out) *stale_out = stale;
+
+  OpArgs O{};
+  O.I = D.I; O.J = D.J; O.K = D.K; O.SX = D.SX; O.SY = D.SY; O.N = D.N;
+  Geom g = make_geom(D.I, D.J, D.K, 2048);
+  O.kper = g.kper;
+  O.pocoef = c->pocoef;
+  O.ee = in.ee_area; O.ne = in.ne_area; O.te = in.te_area;
+  O.ct = in.cell_type_p;
+  for (int f = 0; f < 6; ++f) {
+    O.fl[f] = in.bc_flag[f];
+    O.bcv[f] = c->bcv[f];
+  }
+  O.u = in.u_star; O.v = in.v_star; O.w = in.w_star; O.vof = in.vof; O.vofO = in.vof_old;
+  O.dxp = c->dxp; O.dyp = c->dyp; O.dzp = c->dzp;
+  O.dt = in.dt;
+  O.cx = c->cx; O.cy = c->cy; O.cz = c->cz; O.e = c->ee; O.b = c->b;
+  if (mixed) {
+    O.cx32 = c->lf[0].cx; O.cy32 = c->lf[0].cy; O.cz32 = c->lf[0].cz; O.e32 = c->lf[0].e_store;
+  }
+  O.active = c->active;
+  O.dw = c->dw;
+  O.partials = c->partials;
+  O.S = c->S;
+  if (g.blocks() > (uint32_t)kMaxPartials) return fail(c, SP_ERR_INVALID_ARG, "grid too large");
+  timer_begin(c, "operator_rhs",
+              (double)D.N * (6 * 8 + 3 * 8 + 4 + 5 * 8 + 4 * 8 + 8 + 1 + (mixed ? 16 : 0)), &slot);
+  hipLaunchKernelGGL(k_operator, g.grid, g.block, 0, c->stream, O);
+  timer_end(c, slot);
+  return SP_OK;
+}
+
+}  // namespace sp

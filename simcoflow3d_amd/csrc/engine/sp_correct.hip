@@ -1,0 +1,212 @@
+// Pressure field and velocity correction on the GPU.
+//
+//   gauge shift of a singular (all-Neumann) system,
+//   DeltaPressureGetValues + DeltaPressureBoundaryCondition (ProjectionP.f90:578-643),
+//   correction u = u* - (Pp(i+1)-Pp(i)) PoCoef(:,4) ... (ComputePUVW.f90:181-232),
+//   the mass residual mres of Solver.f90:236-250 as the convergence gate.
+
+#include <hip/hip_runtime.h>
+
+#include "sp_device.h"
+#include "sp_geom.h"
+#include "sp_internal.h"
+
+namespace sp {
+namespace {
+
+struct CorrArgs {
+  int I, J, K, SX, SY, kper;
+  int64_t N;
+  const double* x;
+  const uint8_t* active;
+  const double* pocoef;
+  const int32_t *ctu, *ctv, *ctw, *mu, *mv, *mw;
+  int32_t fl[6];
+  const double *us, *vs, *ws;
+  const double *ee, *ne, *te;
+  const double *dxp, *dyp, *dzp;
+  double *u, *v, *w, *Pp;
+  int64_t pin;  // interior index of the gauge cell or -1
+  double* partials;
+  Scalars* S;
+};
+
+// shift = x(pin) or mean of x over active cells (singular systems only)
+__global__ void __launch_bounds__(256) k_shift(CorrArgs A) {
+  Scalars* S = A.S;
+  uint32_t bx, by, bz;
+  xcd_block3(bx, by, bz);
+  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
+  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
+  double s = 0.0;
+  if (a < A.I && b < A.J)
+    for (int c = c0; c < c1; ++c) {
+      const int64_t n = a + (int64_t)A.I * (b + (int64_t)A.J * c);
+      if (A.active[n]) s += A.x[n];
+    }
+  const double t = block_sum<kTileThreads>(s);
+  if (flat_tid() == 0) store_partial(A.partials, flat_block(), t);
+  if (last_block(&S->counter[kCtrShift])) {
+    const double tot = reduce_partials_sum<kTileThreads>(A.partials, num_blocks());
+    if (flat_tid() == 0) {
+      double sh = 0.0;
+      if (S->singular) {
+        if (A.pin >= 0 && A.active[A.pin]) sh = A.x[A.pin];
+        else if (S->active > 0) sh = tot / (double)S->active;
+      }
+      S->shift = sh;
+    }
+    reset_counter(&S->counter[kCtrShift]);
+  }
+}
+
+__device__ inline double pval(const CorrArgs& A, int64_t n, double shift) {
+  return A.active[n] ? A.x[n] - shift : 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_correct(CorrArgs A) {
+#pragma clang fp contract(off)
+  uint32_t bx, by, bz;
+  xcd_block3(bx, by, bz);
+  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
+  if (a >= A.I || b >= A.J) return;
+  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
+  const double shift = A.S->shift;
+  const int64_t IJ = (int64_t)A.I * A.J, N = A.N;
+  for (int c = c0; c < c1; ++c) {
+    const int64_t n = a + (int64_t)A.I * (b + (int64_t)A.J * c);
+    const int64_t h = (a + 1) + (int64_t)A.SX * ((b + 1) + (int64_t)A.SY * (c + 1));
+    // Pp(i+1)-Pp(i): between two active cells the gauge cancels, so the
+    // difference is taken on the solver's iterate, whose stiff-phase values
+    // sit near zero; faces touching inactive cells or walls use the output
+    // field with its ghosts (ProjectionP.f90:591-643).
+    const bool aP = A.active[n] != 0;
+    const double pP = pval(A, n, shift);
+    auto dpres = [&](bool inside, int64_t m, int fl) {
+      if (inside && aP && A.active[m]) return A.x[m] - A.x[n];
+      const double pN = inside ? pval(A, m, shift) : (double)fl * pP;
+      return pN - pP;
+    };
+    const double dE = dpres(a < A.I - 1, n + 1, A.fl[1]);
+    const double dN = dpres(b < A.J - 1, n + A.I, A.fl[3]);
+    const double dT = dpres(c < A.K - 1, n + IJ, A.fl[5]);
+    double uu = 0.0, vv = 0.0, ww = 0.0;
+    if (A.mu[n] != 1 && A.ctu[n] != 2) {
+      const double g = dE * A.pocoef[3 * N + n];
+      uu = A.us[h] - g;
+    }
+    if (A.mv[n] != 1 && A.ctv[n] != 2) {
+      const double g = dN * A.pocoef[4 * N + n];
+      vv = A.vs[h] - g;
+    }
+    if (A.mw[n] != 1 && A.ctw[n] != 2) {
+      const double g = dT * A.pocoef[5 * N + n];
+      ww = A.ws[h] - g;
+    }
+    A.u[h] = uu;
+    A.v[h] = vv;
+    A.w[h] = ww;
+  }
+}
+
+// Proj%Pp over the whole halo array: interior values, ghost faces
+// flag*Pp(inner) (ProjectionP.f90:620-643), edges and corners 0.
+__global__ void __launch_bounds__(256) k_pressure_field(CorrArgs A) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..I+1
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..J+1
+  const int k = blockIdx.z;                             // 0..K+1
+  if (i > A.I + 1 || j > A.J + 1) return;
+  const double shift = A.S->shift;
+  const int64_t h = i + (int64_t)A.SX * (j + (int64_t)A.SY * k);
+  const bool ix = i >= 1 && i <= A.I, jx = j >= 1 && j <= A.J, kx = k >= 1 && k <= A.K;
+  auto P = [&](int ii, int jj, int kk) {
+    return pval(A, (ii - 1) + (int64_t)A.I * ((jj - 1) + (int64_t)A.J * (kk - 1)), shift);
+  };
+  double v = 0.0;
+  if (ix && jx && kx) v = P(i, j, k);
+  else if (!ix && jx && kx) v = (double)A.fl[i == 0 ? 0 : 1] * P(i == 0 ? 1 : A.I, j, k);
+  else if (ix && !jx && kx) v = (double)A.fl[j == 0 ? 2 : 3] * P(i, j == 0 ? 1 : A.J, k);
+  else if (ix && jx && !kx) v = (double)A.fl[k == 0 ? 4 : 5] * P(i, j, k == 0 ? 1 : A.K);
+  A.Pp[h] = v;
+}
+
+// mres of the corrected field (Solver.f90:239-247); the west/south/bottom
+// wall faces (index 0) take the predictor's value, which the correction does
+// not touch.
+__global__ void __launch_bounds__(256) k_divergence(CorrArgs A) {
+#pragma clang fp contract(off)
+  Scalars* S = A.S;
+  uint32_t bx, by, bz;
+  xcd_block3(bx, by, bz);
+  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
+  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
+  const int64_t SXY = (int64_t)A.SX * A.SY;
+  double m = 0.0;
+  if (a < A.I && b < A.J) {
+    const double dx = A.dxp[a], dy = A.dyp[b];
+    for (int c = c0; c < c1; ++c) {
+      const double dz = A.dzp[c];
+      const int64_t h = (a + 1) + (int64_t)A.SX * ((b + 1) + (int64_t)A.SY * (c + 1));
+      const double uw = a > 0 ? A.u[h - 1] : A.us[h - 1];
+      const double vs = b > 0 ? A.v[h - A.SX] : A.vs[h - A.SX];
+      const double wb = c > 0 ? A.w[h - SXY] : A.ws[h - SXY];
+      const double r = dy * dz * (A.ee[h] * A.u[h] - A.ee[h - 1] * uw) +
+                       dx * dz * (A.ne[h] * A.v[h] - A.ne[h - A.SX] * vs) +
+                       dx * dy * (A.te[h] * A.w[h] - A.te[h - SXY] * wb);
+      double q = fabs(r) / (dx * dy * dz);
+      if (!(q <= 1.0e300)) q = 1.0e300;  // NaN / Inf
+      m = q > m ? q : m;
+    }
+  }
+  const double t = block_max<kTileThreads>(m);
+  if (flat_tid() == 0) store_partial(A.partials, flat_block(), t);
+  if (last_block(&S->counter[kCtrDiv])) {
+    const double mm = reduce_partials_max<kTileThreads>(A.partials, num_blocks());
+    if (flat_tid() == 0) S->divmax = mm;
+    reset_counter(&S->counter[kCtrDiv]);
+  }
+}
+
+}  // namespace
+
+sp_status correct(sp_ctx* c, const sp_inputs& in, sp_outputs& out, sp_stats* st) {
+  const Dims& D = c->dm;
+  Geom g = make_geom(D.I, D.J, D.K, 2048);
+  CorrArgs A{};
+  A.I = D.I; A.J = D.J; A.K = D.K; A.SX = D.SX; A.SY = D.SY; A.kper = g.kper; A.N = D.N;
+  A.x = c->x; A.active = c->active; A.pocoef = c->pocoef;
+  A.ctu = in.cell_type_u; A.ctv = in.cell_type_v; A.ctw = in.cell_type_w;
+  A.mu = in.moex_u; A.mv = in.moex_v; A.mw = in.moex_w;
+  for (int f = 0; f < 6; ++f) A.fl[f] = in.bc_flag[f];
+  A.us = in.u_star; A.vs = in.v_star; A.ws = in.w_star;
+  A.ee = in.ee_area; A.ne = in.ne_area; A.te = in.te_area;
+  A.dxp = c->dxp; A.dyp = c->dyp; A.dzp = c->dzp;
+  A.u = out.u; A.v = out.v; A.w = out.w; A.Pp = out.p;
+  const sp_params& P = c->prm;
+  if (P.pin_i >= 1 && P.pin_j >= 1 && P.pin_k >= 1 && P.pin_i <= D.I && P.pin_j <= D.J &&
+      P.pin_k <= D.K)
+    A.pin = (P.pin_i - 1) + (int64_t)D.I * ((P.pin_j - 1) + (int64_t)D.J * (P.pin_k - 1));
+  else
+    A.pin = -1;
+  A.partials = c->partials;
+  A.S = c->S;
+  const double NB = (double)D.N;
+  int slot;
+  timer_begin(c, "shift", NB * 9, &slot);
+  hipLaunchKernelGGL(k_shift, g.grid, g.block, 0, c->stream, A);
+  timer_end(c, slot);
+  timer_begin(c, "correct", NB * (8 + 1 + 3 * 8 + 6 * 4 + 3 * 8 + 3 * 8), &slot);
+  hipLaunchKernelGGL(k_correct, g.grid, g.block, 0, c->stream, A);
+  timer_end(c, slot);
+  dim3 blk(64, 4, 1), grd((D.I + 2 + 63) / 64, (D.J + 2 + 3) / 4, D.K + 2);
+  timer_begin(c, "pressure_field", (double)D.Nh * 8 + NB * 9, &slot);
+  hipLaunchKernelGGL(k_pressure_field, grd, blk, 0, c->stream, A);
+  timer_end(c, slot);
+  timer_begin(c, "divergence", NB * 6 * 8, &slot);
+  hipLaunchKernelGGL(k_divergence, g.grid, g.block, 0, c->stream, A);
+  timer_end(c, slot);
+  (void)st;
+  return SP_OK;
+}
+
+}  // namespace sp

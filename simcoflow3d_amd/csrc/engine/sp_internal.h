@@ -1,0 +1,188 @@
+// Internal declarations of the projection engine (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "simco_proj.h"
+
+namespace sp {
+
+constexpr int kMaxLevels = 24;
+constexpr int64_t kTailCells = 4096;  // levels this small run in one workgroup
+constexpr int kRedBlockThreads = 256;
+constexpr int kMaxPartials = 1 << 16;  // reduction partials per slot (workgroups)
+
+// Scalars of the outer CG, resident in device memory so no host round trip
+// is needed between the kernels of one iteration.
+struct Scalars {
+  double rz;        // (r, z) of the current iteration
+  double alpha, beta;
+  double pq, zq;
+  double rmax;      // max |r_i| / V_i of the recursive residual
+  double bmean;     // mean of b over active cells (singular systems)
+  double bsum, esum, bmax, shift, divmax;
+  double dsum;      // sum of the diagonal over active cells
+  double zshift;    // diagonal-weighted mean of z removed in singular systems
+  double npart[8];  // scratch results of small reductions
+  int64_t active;
+  int32_t done;     // 1 once converged (kernels then return immediately)
+  int32_t iter;
+  int32_t first;
+  int32_t nonfinite;
+  int32_t singular;
+  int32_t pad;
+  uint32_t counter[16];  // last-block counters, one per reduction site
+  uint32_t stale_count;  // GFM faces that need the previous-beta fixup
+  uint32_t pad2[3];
+};
+
+// Counter slots
+enum CounterSlot : int {
+  kCtrOperator = 0,
+  kCtrInit = 1,
+  kCtrDots = 2,
+  kCtrSpmv = 3,
+  kCtrUpdate = 4,
+  kCtrShift = 5,
+  kCtrDiv = 6,
+  kCtrAdj = 7,
+  kCtrAdj2 = 8,
+  kCtrCheck = 9,
+};
+
+// Partial-sum slots (each kMaxPartials doubles)
+constexpr int kNumPartSlots = 8;
+
+// One multigrid level (views; the storage is owned by sp_ctx::allocs).
+template <typename T>
+struct Level {
+  int nx, ny, nz;
+  int64_t n;
+  T *cx, *cy, *cz;
+  T* e;        // diagonal-only terms, nullptr when they are all zero
+  T* e_store;  // storage for e
+  T *x, *b;
+};
+
+struct Dims {
+  int I, J, K;
+  int64_t N, Nh;
+  int SX, SY;  // halo strides: SX = I+2, SY = J+2
+};
+
+struct Timer {
+  std::string name;
+  int launches = 0;
+  double total_ms = 0;
+  double bytes = 0;
+};
+
+struct DeviceBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace sp
+
+struct sp_ctx {
+  sp_params prm;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  sp::Dims dm{};
+  bool grid_set = false;
+  std::string last_error;
+
+  // grid (device)
+  double *dxp = nullptr, *dyp = nullptr, *dzp = nullptr;
+  double *dxu = nullptr, *dyv = nullptr, *dzw = nullptr;
+
+  // staged inputs for the host-pointer entry points (device copies)
+  std::vector<sp::DeviceBuf> stage;
+
+  // assembled operator (FP64, interior layout)
+  double* pocoef = nullptr;  // 6*N, faces W,S,B,E,N,T
+  double *cx = nullptr, *cy = nullptr, *cz = nullptr, *ee = nullptr;
+  uint8_t* active = nullptr;
+  float* dw = nullptr;       // diagonal as float: weight of the null-space projection
+  double* b = nullptr;
+  bool has_e = false;
+
+  // CG vectors (FP64)
+  double *x = nullptr, *r = nullptr, *q = nullptr, *p0 = nullptr, *p1 = nullptr, *z64 = nullptr;
+  float *r32 = nullptr, *z32 = nullptr;
+
+  // multigrid hierarchies (only one is populated, by precision)
+  int levels = 0;
+  int tail_level = 0;
+  sp::Level<float> lf[sp::kMaxLevels];
+  sp::Level<double> ld[sp::kMaxLevels];
+  void* mg_desc = nullptr;  // device copy of the level table for the tail kernel
+
+  // GFM stale-beta machinery
+  int32_t* stale_list = nullptr;  // (cell, face, kind) triples
+  int64_t stale_cap = 0;
+  int64_t* col_last = nullptr;    // 12 * I*J loop positions
+  int64_t* col_prefix = nullptr;  // 12 * I*J
+
+  // scalars + partials
+  sp::Scalars* S = nullptr;
+  sp::Scalars* S_host = nullptr;  // pinned
+  double* partials = nullptr;
+
+  // adjoint checkpoint
+  bool have_forward = false;
+  double* x_ckpt = nullptr;       // forward pressure (interior, gauge applied)
+  double* dp_save[3] = {nullptr, nullptr, nullptr};  // Dp on E,N,T faces (cell arrays)
+  const void* fwd_inputs_tag = nullptr;
+  // inputs of the last forward kept on device for the adjoint (staged copies
+  // or the caller's device pointers)
+  sp_inputs last_in{};
+  bool last_in_host = false;
+
+  // adjoint work
+  double *pbar = nullptr, *lam = nullptr;
+
+  // bc values staged
+  double* bcv[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+
+  // profiling
+  bool profiling = false;
+  std::vector<sp::Timer> timers;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+  std::vector<std::pair<int, int>> ev_used;  // (timer index, event index)
+  double bytes_scale = 1.0;
+
+  size_t allocated_for = 0;  // N the buffers were sized for
+  int allocated_prec = -1;
+  int allocated_precond = -1;
+  int allocated_levels = -1;
+  std::vector<void*> allocs;    // every device allocation of the buffers above
+  int32_t* hflags = nullptr;    // pinned convergence flags (2 slots)
+};
+
+namespace sp {
+
+// ---- host helpers (sp_api.hip) ----
+sp_status fail(sp_ctx* c, sp_status s, const std::string& msg);
+sp_status hip_check(sp_ctx* c, hipError_t e, const char* what);
+void timer_begin(sp_ctx* c, const char* name, double bytes, int* slot);
+void timer_end(sp_ctx* c, int slot);
+
+// ---- assembly (sp_assemble.hip) ----
+sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_count);
+
+// ---- solver (sp_solver.hip) ----
+sp_status mg_setup(sp_ctx* c);
+sp_status solve(sp_ctx* c, const double* rhs, double* xout, bool forward, sp_stats* st);
+
+// ---- correction (sp_correct.hip) ----
+sp_status correct(sp_ctx* c, const sp_inputs& in, sp_outputs& out, sp_stats* st);
+
+// ---- adjoint (sp_adjoint.hip) ----
+sp_status adjoint(sp_ctx* c, const sp_adj_inputs& in, sp_adj_outputs& out, sp_stats* st);
+
+}  // namespace sp

@@ -1,0 +1,290 @@
+"""ctypes binding of ``libsimcoproj.so`` (the C ABI in include/simco_proj.h).
+
+The binding loads the in-tree library built by ``__graft_entry__.build()``
+and raises if it is missing: there is no CPU fallback.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+from ._paths import ENGINE_LIB
+
+F64P = ctypes.POINTER(ctypes.c_double)
+I32P = ctypes.POINTER(ctypes.c_int32)
+
+SP_OK = 0
+SP_ERR_INVALID_ARG = -1
+SP_ERR_HIP = -2
+SP_ERR_NOT_CONVERGED = -3
+SP_ERR_NONFINITE = -4
+SP_ERR_OOM = -5
+SP_ERR_GRID = -6
+SP_ERR_STATE = -7
+SP_ERR_NO_DEVICE = -8
+
+SP_PREC_FP64 = 0
+SP_PREC_MIXED = 1
+SP_PRECOND_MG = 0
+SP_PRECOND_JACOBI = 1
+
+EXPORTED_SYMBOLS = (
+    "sp_params_default", "sp_create", "sp_destroy", "sp_set_params", "sp_set_grid",
+    "sp_set_grid_1d", "sp_project", "sp_project_dev", "sp_adjoint", "sp_adjoint_dev",
+    "sp_stream", "sp_kernel_times", "sp_set_profiling", "sp_strerror", "sp_last_error",
+)
+
+
+class SpParams(ctypes.Structure):
+    _fields_ = [
+        ("imax", ctypes.c_int32), ("jmax", ctypes.c_int32), ("kmax", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("precond", ctypes.c_int32),
+        ("mg_levels", ctypes.c_int32), ("mg_pre", ctypes.c_int32), ("mg_post", ctypes.c_int32),
+        ("mg_coarse_sweeps", ctypes.c_int32), ("mg_coarse_scale", ctypes.c_double),
+        ("max_iter", ctypes.c_int32), ("div_tol", ctypes.c_double),
+        ("rho_l", ctypes.c_double), ("rho_g", ctypes.c_double), ("rho_ref", ctypes.c_double),
+        ("epsi", ctypes.c_double),
+        ("pin_i", ctypes.c_int32), ("pin_j", ctypes.c_int32), ("pin_k", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+    ]
+
+
+class SpInputs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "u_star", "v_star", "w_star", "dp_u", "dp_v", "dp_w", "ee_area", "ne_area", "te_area",
+        "vof", "vof_old", "vof_l", "phi_l", "cell_type_p", "cell_type_u", "cell_type_v",
+        "cell_type_w", "moex_u", "moex_v", "moex_w")] + [
+        ("bc_flag", ctypes.c_int32 * 6), ("bc_val", ctypes.c_void_p * 6), ("dt", ctypes.c_double),
+        ("beta_e", ctypes.c_void_p), ("beta_n", ctypes.c_void_p), ("beta_t", ctypes.c_void_p)]
+
+
+class SpOutputs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("p", "u", "v", "w", "po_coef")]
+
+
+class SpStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int32), ("converged", ctypes.c_int32), ("levels", ctypes.c_int32),
+        ("singular", ctypes.c_int32), ("active_cells", ctypes.c_int64),
+        ("stale_faces", ctypes.c_int64), ("initial_div", ctypes.c_double),
+        ("final_div", ctypes.c_double), ("residual_div", ctypes.c_double),
+        ("t_assemble_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double),
+        ("t_correct_ms", ctypes.c_double), ("t_total_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class SpAdjInputs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("gu", "gv", "gw")]
+
+
+class SpAdjOutputs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "gu_star", "gv_star", "gw_star", "gk_e", "gk_n", "gk_t", "gbeta_e", "gbeta_n", "gbeta_t")]
+
+
+class SpKernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_int32),
+                ("total_ms", ctypes.c_double), ("bytes", ctypes.c_double)]
+
+
+class SpError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{msg} (status {status})")
+        self.status = status
+
+
+_lib = None
+
+
+def load():
+    """Load the engine library; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(ENGINE_LIB):
+        raise RuntimeError(f"HIP engine library missing: {ENGINE_LIB} (run __graft_entry__.build())")
+    L = ctypes.CDLL(ENGINE_LIB)
+    P = ctypes.POINTER
+    V = ctypes.c_void_p
+    L.sp_params_default.argtypes = [P(SpParams)]
+    L.sp_params_default.restype = None
+    L.sp_create.argtypes = [P(V), P(SpParams), ctypes.c_int]
+    L.sp_create.restype = ctypes.c_int
+    L.sp_destroy.argtypes = [V]
+    L.sp_destroy.restype = ctypes.c_int
+    L.sp_set_params.argtypes = [V, P(SpParams)]
+    L.sp_set_params.restype = ctypes.c_int
+    L.sp_set_grid.argtypes = [V] + [F64P] * 6
+    L.sp_set_grid.restype = ctypes.c_int
+    L.sp_set_grid_1d.argtypes = [V] + [F64P] * 6
+    L.sp_set_grid_1d.restype = ctypes.c_int
+    for name in ("sp_project", "sp_project_dev"):
+        f = getattr(L, name)
+        f.argtypes = [V, P(SpInputs), P(SpOutputs), P(SpStats)]
+        f.restype = ctypes.c_int
+    for name in ("sp_adjoint", "sp_adjoint_dev"):
+        f = getattr(L, name)
+        f.argtypes = [V, P(SpAdjInputs), P(SpAdjOutputs), P(SpStats)]
+        f.restype = ctypes.c_int
+    L.sp_stream.argtypes = [V]
+    L.sp_stream.restype = V
+    L.sp_kernel_times.argtypes = [V, P(SpKernelTime), ctypes.c_int]
+    L.sp_kernel_times.restype = ctypes.c_int
+    L.sp_set_profiling.argtypes = [V, ctypes.c_int]
+    L.sp_set_profiling.restype = None
+    L.sp_strerror.argtypes = [ctypes.c_int]
+    L.sp_strerror.restype = ctypes.c_char_p
+    L.sp_last_error.argtypes = [V]
+    L.sp_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def default_params(shape, **kw):
+    L = load()
+    p = SpParams()
+    L.sp_params_default(ctypes.byref(p))
+    p.imax, p.jmax, p.kmax = shape
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def _addr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()  # torch tensor
+
+
+class Projector:
+    """One engine context (one grid, one set of solver parameters)."""
+
+    def __init__(self, shape, device=0, **params):
+        self.lib = load()
+        self.shape = tuple(shape)
+        self.params = default_params(shape, **params)
+        h = ctypes.c_void_p()
+        st = self.lib.sp_create(ctypes.byref(h), ctypes.byref(self.params), device)
+        if st != SP_OK:
+            raise SpError(st, "sp_create failed: " + self.lib.sp_strerror(st).decode())
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what, allow=()):
+        if st != SP_OK and st not in allow:
+            raise SpError(st, f"{what}: {self.lib.sp_strerror(st).decode()}: "
+                              f"{self.lib.sp_last_error(self.h).decode()}")
+        return st
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        self._check(self.lib.sp_set_params(self.h, ctypes.byref(self.params)), "sp_set_params")
+
+    def set_grid_1d(self, dxp, dyp, dzp, dxu, dyv, dzw):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (dxp, dyp, dzp, dxu, dyv, dzw)]
+        self._check(self.lib.sp_set_grid_1d(self.h, *[a.ctypes.data_as(F64P) for a in arrs]),
+                    "sp_set_grid_1d")
+
+    def set_grid(self, dxp, dyp, dzp, dxu, dyv, dzw):
+        arrs = [np.asfortranarray(a, dtype=np.float64) for a in (dxp, dyp, dzp, dxu, dyv, dzw)]
+        self._check(self.lib.sp_set_grid(self.h, *[a.ctypes.data_as(F64P) for a in arrs]),
+                    "sp_set_grid")
+
+    @staticmethod
+    def make_inputs(f, bc_flag, dt, bc_val=None, beta=None):
+        """f: mapping with the sp_inputs field names (numpy arrays or torch tensors)."""
+        inp = SpInputs()
+        for n in ("u_star", "v_star", "w_star", "dp_u", "dp_v", "dp_w", "ee_area", "ne_area",
+                  "te_area", "vof", "vof_old", "vof_l", "phi_l", "cell_type_p", "cell_type_u",
+                  "cell_type_v", "cell_type_w", "moex_u", "moex_v", "moex_w"):
+            setattr(inp, n, _addr(f.get(n)))
+        for q in range(6):
+            inp.bc_flag[q] = int(bc_flag[q])
+            inp.bc_val[q] = _addr(bc_val[q]) if bc_val is not None and bc_val[q] is not None else None
+        inp.dt = dt
+        if beta is not None:
+            inp.beta_e, inp.beta_n, inp.beta_t = (_addr(b) for b in beta)
+        return inp
+
+    def project(self, fields, bc_flag, dt, out=None, bc_val=None, beta=None, po_coef=False):
+        """Host-array projection (sp_project).  Returns (outputs dict, stats)."""
+        I, J, K = self.shape
+        hs = (I + 2, J + 2, K + 2)
+        inp = self.make_inputs(fields, bc_flag, dt, bc_val, beta)
+        if out is None:
+            out = {n: np.zeros(hs, order="F") for n in ("p", "u", "v", "w")}
+        if po_coef:
+            out["po_coef"] = np.zeros((I, J, K, 6), order="F")
+        o = SpOutputs(*(_addr(out.get(n)) for n in ("p", "u", "v", "w", "po_coef")))
+        st = SpStats()
+        rc = self.lib.sp_project(self.h, ctypes.byref(inp), ctypes.byref(o), ctypes.byref(st))
+        self._check(rc, "sp_project", allow=(SP_ERR_NOT_CONVERGED,))
+        return out, st
+
+    def project_dev(self, fields, bc_flag, dt, out, bc_val=None, beta=None):
+        """Device-array projection (sp_project_dev) on torch tensors."""
+        inp = self.make_inputs(fields, bc_flag, dt, bc_val, beta)
+        o = SpOutputs(*(_addr(out.get(n)) for n in ("p", "u", "v", "w", "po_coef")))
+        st = SpStats()
+        rc = self.lib.sp_project_dev(self.h, ctypes.byref(inp), ctypes.byref(o), ctypes.byref(st))
+        self._check(rc, "sp_project_dev", allow=(SP_ERR_NOT_CONVERGED,))
+        return st
+
+    def adjoint(self, gu, gv, gw, beta_grad=False):
+        I, J, K = self.shape
+        hs = (I + 2, J + 2, K + 2)
+        ai = SpAdjInputs(*(_addr(np.asfortranarray(g)) for g in (gu, gv, gw)))
+        keep = [np.asfortranarray(g) for g in (gu, gv, gw)]
+        ai = SpAdjInputs(*(_addr(g) for g in keep))
+        res = {n: np.zeros(hs, order="F") for n in ("gu_star", "gv_star", "gw_star")}
+        for n in ("gk_e", "gk_n", "gk_t") + (("gbeta_e", "gbeta_n", "gbeta_t") if beta_grad else ()):
+            res[n] = np.zeros((I, J, K), order="F")
+        ao = SpAdjOutputs(*(_addr(res.get(n)) for n in (
+            "gu_star", "gv_star", "gw_star", "gk_e", "gk_n", "gk_t", "gbeta_e", "gbeta_n", "gbeta_t")))
+        st = SpStats()
+        rc = self.lib.sp_adjoint(self.h, ctypes.byref(ai), ctypes.byref(ao), ctypes.byref(st))
+        self._check(rc, "sp_adjoint", allow=(SP_ERR_NOT_CONVERGED,))
+        return res, st
+
+    def adjoint_dev(self, g, res):
+        """g: dict gu,gv,gw (device tensors); res: dict of device output tensors."""
+        ai = SpAdjInputs(*(_addr(g[n]) for n in ("gu", "gv", "gw")))
+        ao = SpAdjOutputs(*(_addr(res.get(n)) for n in (
+            "gu_star", "gv_star", "gw_star", "gk_e", "gk_n", "gk_t", "gbeta_e", "gbeta_n", "gbeta_t")))
+        st = SpStats()
+        rc = self.lib.sp_adjoint_dev(self.h, ctypes.byref(ai), ctypes.byref(ao), ctypes.byref(st))
+        self._check(rc, "sp_adjoint_dev", allow=(SP_ERR_NOT_CONVERGED,))
+        return st
+
+    def set_profiling(self, on):
+        self.lib.sp_set_profiling(self.h, 1 if on else 0)
+
+    def kernel_times(self):
+        n = self.lib.sp_kernel_times(self.h, None, 0)
+        arr = (SpKernelTime * max(n, 1))()
+        self.lib.sp_kernel_times(self.h, arr, n)
+        return [dict(name=a.name.decode(), launches=a.launches, total_ms=a.total_ms, bytes=a.bytes)
+                for a in arr[:n]]
+
+    def stream(self):
+        return self.lib.sp_stream(self.h)
