@@ -1,0 +1,212 @@
+"""Projection throughput on one MI355X.
+
+A step is one full pressure projection (GFM assembly, multigrid-preconditioned
+CG to max|div u| <= 1e-10, velocity correction) of the 256^3 two-phase case
+(rho_l/rho_g = 1000, breaking-wave interface, no solid) with every input
+already resident in HBM.  The headline value is mixed-precision (FP32 V-cycle
+inside FP64 CG) projection solves/s; the all-FP64 solver is timed the same
+way in the same run.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--no-cpu]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    0: "64^3 two-phase, sphere solid (config 0)",
+    1: "128^3 z-stretched two-phase, cylinder solid (config 1)",
+    2: "256^3 two-phase rho 1000:1 breaking-wave interface, no solid (config 2)",
+    3: "128^3 two-phase, sphere solid (config 3)",
+    4: "512x256x256 wave tank, three submerged boxes (config 4)",
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--size", type=int, nargs=3, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--levels", type=int, default=0)
+    return ap.parse_args()
+
+
+def to_device(case):
+    import torch
+    dev = {}
+    for n, a in case.fields.items():
+        # Fortran-ordered (I+2,J+2,K+2) == C-ordered (K+2,J+2,I+2): same bytes
+        dev[n] = torch.from_numpy(a.T.copy()).cuda()
+    return dev
+
+
+def make_outputs(case, dev_like):
+    import torch
+    I, J, K = case.shape
+    hs = (K + 2, J + 2, I + 2)
+    return {n: torch.zeros(hs, dtype=torch.float64, device="cuda") for n in ("p", "u", "v", "w")}
+
+
+def run_solves(P, dev, out, case, count):
+    import torch
+    stats = []
+    for _ in range(count):
+        st = P.project_dev(dev, case.bc_flag, case.dt, out)
+        stats.append(st)
+    torch.cuda.synchronize()
+    return stats
+
+
+def timed(P, dev, out, case, warmup, steps):
+    import torch
+    run_solves(P, dev, out, case, warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = run_solves(P, dev, out, case, steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return (t1 - t0), stats
+
+
+def kernel_table(P, dev, out, case):
+    """Per-kernel device time from HIP events on the engine's stream (one
+    profiled solve after the timed region)."""
+    P.set_profiling(True)
+    run_solves(P, dev, out, case, 1)
+    rows = P.kernel_times()
+    P.set_profiling(False)
+    table = []
+    for r in rows:
+        if r["launches"] == 0:
+            continue
+        avg_ms = r["total_ms"] / r["launches"]
+        per_launch = r["bytes"] / r["launches"]
+        gbs = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 and per_launch > 0 else 0.0
+        table.append(dict(name=r["name"], launches=r["launches"], total_ms=r["total_ms"],
+                          avg_us=avg_ms * 1e3, bytes_per_launch=per_launch, gbs=gbs))
+    table.sort(key=lambda t: -t["total_ms"])
+    return table
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only in this build)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from simcoflow3d_amd import configs
+    from simcoflow3d_amd.engine import Projector, SP_PREC_FP64, SP_PREC_MIXED
+
+    cfg = args.config
+    size = tuple(args.size) if args.size else None
+    tg = time.perf_counter()
+    case = configs.generate(cfg, size=size)
+    gen_s = time.perf_counter() - tg
+    I, J, K = case.shape
+    N = I * J * K
+    dev = to_device(case)
+    out = make_outputs(case, dev)
+    torch.cuda.synchronize()
+
+    common = dict(rho_l=case.rho_l, rho_g=case.rho_g, rho_ref=case.rho_l, pin_i=1, pin_j=1,
+                  pin_k=1, div_tol=1e-10, mg_levels=args.levels)
+    Pm = Projector(case.shape, precision=SP_PREC_MIXED, **common)
+    Pm.set_grid_1d(case.dxp, case.dyp, case.dzp, case.dxu, case.dyv, case.dzw)
+    el, st_m = timed(Pm, dev, out, case, args.warmup, args.steps)
+    sm = st_m[-1]
+    p_mixed = out["p"].cpu().numpy()
+    value = args.steps / el
+    ms_per_step = el / args.steps * 1e3
+    ktab = [] if args.no_profile else kernel_table(Pm, dev, out, case)
+    Pm.close()
+
+    fp64 = {}
+    if not args.no_fp64:
+        P64 = Projector(case.shape, precision=SP_PREC_FP64, **common)
+        P64.set_grid_1d(case.dxp, case.dyp, case.dzp, case.dxu, case.dyv, case.dzw)
+        el64, st_64 = timed(P64, dev, out, case, max(1, args.warmup), args.steps)
+        p64 = out["p"].cpu().numpy()
+        ktab64 = [] if args.no_profile else kernel_table(P64, dev, out, case)
+        P64.close()
+        s64 = st_64[-1]
+        fp64 = dict(solves_per_s=args.steps / el64, ms_per_step=el64 / args.steps * 1e3,
+                    iterations=s64.iterations, final_div=s64.final_div,
+                    dominant=(ktab64[0]["name"] if ktab64 else None))
+        err = float(np.max(np.abs(p_mixed - p64)) / max(np.max(np.abs(p64)), 1e-300))
+        fp64["mixed_p_rel_err_vs_fp64"] = err
+        fp64["mixed_speedup"] = value / fp64["solves_per_s"]
+
+    roof = None
+    if ktab:
+        dom = ktab[0]
+        roof = dict(bound="hbm", kernel=dom["name"], achieved=round(dom["gbs"], 1),
+                    peak=HBM_PEAK_GBS, unit="GB/s", frac=round(dom["gbs"] / HBM_PEAK_GBS, 4),
+                    traffic=None, avg_us=round(dom["avg_us"], 2),
+                    bytes_per_launch=dom["bytes_per_launch"])
+        tfile = os.path.join(REPO, "profiles", "traffic.json")
+        if os.path.exists(tfile):
+            with open(tfile) as fh:
+                tr = json.load(fh)
+            key = f"{dom['name']}@{I}x{J}x{K}"
+            if key in tr:
+                roof["traffic"] = tr[key]
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import orc
+        ncores = os.cpu_count()
+        if cfg == 2 and size is None:
+            sample_case = case
+            sample = "one full FP64 projection of the same 256^3 case"
+        else:
+            sample_case = case
+            sample = f"one full FP64 projection of the same {I}x{J}x{K} case"
+        secs, info = orc.baseline_project(sample_case)
+        cpu = dict(value=round(1.0 / secs, 4), unit="projection solves/s", cores=ncores,
+                   kind="port", sample=sample, seconds=round(secs, 3),
+                   iterations=info["iterations"], div_inf=info["div_inf"])
+
+    line = {
+        "metric": "projection solves/s at 256^3 to ||div u||<=1e-10, mixed vs FP64 GPU; HBM GB/s",
+        "value": round(value, 3),
+        "unit": "projection solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 outer CG + f32 V-cycle",
+        "data": "synthetic (seeded generator, inputs resident in HBM)",
+        "config": {"workload": WORKLOADS.get(cfg, f"config {cfg}"), "grid": [I, J, K],
+                   "cells": N, "precision": "mixed", "mg_levels": sm.levels,
+                   "iterations": sm.iterations, "final_div": sm.final_div,
+                   "converged": bool(sm.converged)},
+        "fp64": fp64,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": [dict(name=k["name"], launches=k["launches"], total_ms=round(k["total_ms"], 3),
+                         avg_us=round(k["avg_us"], 2), gbs=round(k["gbs"], 1)) for k in ktab[:12]],
+        "gen_s": round(gen_s, 2),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -73,8 +73,14 @@ typedef struct sp_params {
   double epsi;
   int32_t pin_i, pin_j, pin_k;  /* 1-based gauge cell of a singular system;
                                    0,0,0 -> zero mean over fluid cells */
-  int32_t flags;                /* reserved, 0 */
+  int32_t flags;                /* SP_FLAG_* bits, 0 by default */
 } sp_params;
+
+/* sp_params.flags bits */
+enum {
+  SP_FLAG_UNFUSED_SMOOTHER = 1  /* one kernel per half-sweep instead of the
+                                   temporally blocked smoother (same result) */
+};
 
 typedef struct sp_ctx sp_ctx;
 

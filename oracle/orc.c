@@ -94,6 +94,128 @@ void orc_gfm_coef(const orc_grid* g, const double* phiL, const double* vofL, con
   }
 }
 
+/* Same result as orc_gfm_coef, computed in parallel: every face whose
+ * neighbour is classified is independent; a face next to an unclassified cell
+ * takes the value assigned by the last earlier cell of the reference's i-j-k
+ * loop, found by a look-back within its k column and a prefix over columns. */
+static int gfm_face_kind(const orc_grid* g, const double* phiL, const double* vofL,
+                         const double* vof, double epsi, double BetaP, double BetaM, int f, int i,
+                         int j, int k, double* beta) {
+  static const int sel[6][3] = {{-1, 0, 0}, {0, -1, 0}, {0, 0, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  const int ii = i + sel[f][0], jj = j + sel[f][1], kk = k + sel[f][2];
+  const int64_t P = HX(g, i, j, k), Q = HX(g, ii, jj, kk);
+  const double one_m = 1.0 - epsi;
+  const int wetP = (vofL[P] >= 0.5 && vof[P] > one_m) || (phiL[P] < 0.0 && vof[P] <= one_m && vof[P] > epsi);
+  const int dryP = !wetP && ((vofL[P] < 0.5 && vof[P] > one_m) || (phiL[P] >= 0.0 && vof[P] <= one_m && vof[P] > epsi));
+  if (!wetP && !dryP) return 0;
+  const int wetQ = (vofL[Q] >= 0.5 && vof[Q] > one_m) || (phiL[Q] < 0.0 && vof[Q] <= one_m && vof[Q] > epsi);
+  const int dryQ = !wetQ && ((vofL[Q] < 0.5 && vof[Q] > one_m) || (phiL[Q] >= 0.0 && vof[Q] <= one_m && vof[Q] > epsi));
+  const double Lamda = fabs(phiL[P]) / (fabs(phiL[P]) + fabs(phiL[Q]) + 1.0e-30);
+  if (wetP) {
+    if (dryQ) { *beta = Lamda * BetaM + (1.0 - Lamda) * BetaP; return 1; }
+    if (wetQ) { *beta = BetaM; return 1; }
+    return -1;
+  }
+  if (wetQ) { *beta = Lamda * BetaP + (1.0 - Lamda) * BetaM; return 2; }
+  if (dryQ) { *beta = BetaP; return 2; }
+  return -2;
+}
+
+void orc_gfm_coef_fast(const orc_grid* g, const double* phiL, const double* vofL, const double* vof,
+                       const double* dpu, const double* dpv, const double* dpw, double rho_l,
+                       double rho_g, double rho_ref, double epsi, double* pocoef) {
+  const int I = g->I, J = g->J, K = g->K;
+  const int64_t N = (int64_t)I * J * K, cols = (int64_t)I * J;
+  const double BetaP = 1.0 / (rho_l / rho_ref), BetaM = 1.0 / (rho_g / rho_ref);
+  int64_t* last = (int64_t*)malloc(sizeof(int64_t) * 2 * cols);
+  int64_t* pre = (int64_t*)malloc(sizeof(int64_t) * 2 * cols);
+  for (int f = 0; f < 6; ++f) {
+    const int dir = f % 3, minus = f < 3;
+    const double* dp = dir == 0 ? dpu : (dir == 1 ? dpv : dpw);
+    const double* dg = dir == 0 ? g->dxu : (dir == 1 ? g->dyv : g->dzw);
+    double* T = pocoef + (int64_t)f * N;
+    int64_t nstale = 0;
+#pragma omp parallel for collapse(2) schedule(static) reduction(+ : nstale)
+    for (int k = 1; k <= K; ++k)
+      for (int j = 1; j <= J; ++j)
+        for (int i = 1; i <= I; ++i) {
+          double beta = 0;
+          const int r = gfm_face_kind(g, phiL, vofL, vof, epsi, BetaP, BetaM, f, i, j, k, &beta);
+          const int c = dir == 0 ? i : (dir == 1 ? j : k);
+          const double DGrid = dg[(minus ? (c - 1 > 1 ? c - 1 : 1) : c) - 1];
+          const int64_t hd = minus ? HX(g, i - (dir == 0), j - (dir == 1), k - (dir == 2)) : HX(g, i, j, k);
+          const double pref = dp[hd] / DGrid * BetaP * BetaM;
+          T[NX(g, i, j, k)] = r > 0 ? pref / beta : (r < 0 ? NAN : 0.0);
+          nstale += r < 0;
+        }
+    if (nstale) {
+#pragma omp parallel for schedule(static)
+      for (int64_t col = 0; col < cols; ++col) {
+        const int i = (int)(col / J) + 1, j = (int)(col % J) + 1;
+        int64_t lw = -1, ld = -1;
+        for (int k = K; k >= 1 && (lw < 0 || ld < 0); --k) {
+          double beta;
+          const int r = gfm_face_kind(g, phiL, vofL, vof, epsi, BetaP, BetaM, f, i, j, k, &beta);
+          if (r == 1 && lw < 0) lw = col * K + (k - 1);
+          if (r == 2 && ld < 0) ld = col * K + (k - 1);
+        }
+        last[col] = lw;
+        last[cols + col] = ld;
+      }
+      int64_t rw = -1, rd = -1;
+      for (int64_t col = 0; col < cols; ++col) {
+        pre[col] = rw; pre[cols + col] = rd;
+        if (last[col] > rw) rw = last[col];
+        if (last[cols + col] > rd) rd = last[cols + col];
+      }
+#pragma omp parallel for collapse(2) schedule(dynamic, 64)
+      for (int k = 1; k <= K; ++k)
+        for (int j = 1; j <= J; ++j)
+          for (int i = 1; i <= I; ++i) {
+            const int64_t n = NX(g, i, j, k);
+            if (!(T[n] != T[n])) continue;
+            double beta0;
+            const int kind = -gfm_face_kind(g, phiL, vofL, vof, epsi, BetaP, BetaM, f, i, j, k, &beta0);
+            const int64_t col = (int64_t)(i - 1) * J + (j - 1);
+            int64_t pos = -1;
+            for (int kk = k - 1; kk >= 1; --kk) {
+              double b2;
+              if (gfm_face_kind(g, phiL, vofL, vof, epsi, BetaP, BetaM, f, i, j, kk, &b2) == kind) {
+                pos = col * K + (kk - 1);
+                break;
+              }
+            }
+            if (pos < 0) pos = pre[(kind - 1) * cols + col];
+            double beta = kind == 1 ? BetaM : BetaP;
+            if (pos >= 0) {
+              const int64_t pc = pos / K;
+              gfm_face_kind(g, phiL, vofL, vof, epsi, BetaP, BetaM, f, (int)(pc / J) + 1,
+                            (int)(pc % J) + 1, (int)(pos % K) + 1, &beta);
+            }
+            const int c = dir == 0 ? i : (dir == 1 ? j : k);
+            const double DGrid = dg[(minus ? (c - 1 > 1 ? c - 1 : 1) : c) - 1];
+            const int64_t hd = minus ? HX(g, i - (dir == 0), j - (dir == 1), k - (dir == 2)) : HX(g, i, j, k);
+            T[n] = dp[hd] / DGrid * BetaP * BetaM / beta;
+          }
+    }
+    if (dir == 0) {
+      const int dst = minus ? 1 : I, src = minus ? 2 : I - 1;
+      for (int k = 1; k <= K; ++k)
+        for (int j = 1; j <= J; ++j) T[NX(g, dst, j, k)] = T[NX(g, src, j, k)];
+    } else if (dir == 1) {
+      const int dst = minus ? 1 : J, src = minus ? 2 : J - 1;
+      for (int k = 1; k <= K; ++k)
+        for (int i = 1; i <= I; ++i) T[NX(g, i, dst, k)] = T[NX(g, i, src, k)];
+    } else {
+      const int dst = minus ? 1 : K, src = minus ? 2 : K - 1;
+      for (int j = 1; j <= J; ++j)
+        for (int i = 1; i <= I; ++i) T[NX(g, i, j, dst)] = T[NX(g, i, j, src)];
+    }
+  }
+  free(last);
+  free(pre);
+}
+
 /* ------------------------------------------------------------------ */
 /* Assembled rows: ProjectionP.f90:330-478                              */
 /* ------------------------------------------------------------------ */

@@ -32,6 +32,11 @@ void orc_gfm_coef(const orc_grid* g, const double* phiL, const double* vofL, con
                   const double* dpu, const double* dpv, const double* dpw,
                   double rho_l, double rho_g, double rho_ref, double epsi, double* pocoef);
 
+/* Parallel evaluation with the same result as orc_gfm_coef (CPU baseline). */
+void orc_gfm_coef_fast(const orc_grid* g, const double* phiL, const double* vofL, const double* vof,
+                       const double* dpu, const double* dpv, const double* dpw,
+                       double rho_l, double rho_g, double rho_ref, double epsi, double* pocoef);
+
 /* SetPoissonMatrix rows (ProjectionP.f90:301-482): for every cell, 7 values
  * in the order W,S,B,diag,E,N,T and the neighbour's interior index (-1 when the
  * reference omits the entry).  Solid rows (cell_type==2) get all -1 / 0. */

@@ -50,6 +50,8 @@ def lib():
         G = ctypes.POINTER(OrcGrid)
         L.orc_gfm_coef.argtypes = [G] + [F64P] * 6 + [ctypes.c_double] * 4 + [F64P]
         L.orc_gfm_coef.restype = None
+        L.orc_gfm_coef_fast.argtypes = L.orc_gfm_coef.argtypes
+        L.orc_gfm_coef_fast.restype = None
         L.orc_matrix_rows.argtypes = [G, F64P, F64P, F64P, F64P, I32P, I32P, F64P, I64P]
         L.orc_matrix_rows.restype = None
         L.orc_rhs.argtypes = [G, F64P, F64P, F64P, F64P, I32P, I32P, ctypes.POINTER(F64P),
@@ -120,6 +122,38 @@ def gfm_coef(case, rho_l=None, rho_g=None, rho_ref=None, epsi=1.0e-3, phi_l=None
                        _p(case.vof if vof is None else vof), _p(case.dp_u), _p(case.dp_v),
                        _p(case.dp_w), rho_l, rho_g, rho_ref, epsi, _p(pc))
     return pc
+
+
+def gfm_coef_fast(case, epsi=1.0e-3):
+    pc = np.zeros((case.I, case.J, case.K, 6), order="F")
+    g = grid_of(case)
+    lib().orc_gfm_coef_fast(ctypes.byref(g), _p(case.phi_l), _p(case.vof_l), _p(case.vof),
+                            _p(case.dp_u), _p(case.dp_v), _p(case.dp_w), case.rho_l, case.rho_g,
+                            case.rho_l, epsi, _p(pc))
+    return pc
+
+
+def baseline_project(case, levels=0, pre=2, post=2, coarse_sweeps=4, coarse_scale=0.5,
+                     div_tol=1e-10, max_iter=200):
+    """The CPU baseline: the same path and multigrid-preconditioned CG as the
+    engine, FP64, OpenMP over the host cores.  Returns (seconds, info)."""
+    import time
+    t0 = time.perf_counter()
+    pc = gfm_coef_fast(case)
+    b = rhs(case, pc)
+    op = sym_operator(case, pc)
+    d, _ = sym_diag(case, *op)
+    t1 = time.perf_counter()
+    o = opts(levels=levels, pre=pre, post=post, coarse_sweeps=coarse_sweeps,
+             coarse_scale=coarse_scale, div_tol=div_tol, max_iter=max_iter)
+    x, st, sing = pcg(case, op, b, o)
+    t2 = time.perf_counter()
+    Pp = pressure_field(case, x)
+    u, v, w = correct(case, pc, Pp)
+    _, dinf = mres(case, u, v, w)
+    t3 = time.perf_counter()
+    return t3 - t0, dict(assemble_s=t1 - t0, solve_s=t2 - t1, correct_s=t3 - t2,
+                         iterations=st.iterations, converged=st.converged, div_inf=dinf)
 
 
 def matrix_rows(case, pc):

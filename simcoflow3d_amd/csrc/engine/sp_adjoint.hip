@@ -23,7 +23,7 @@ namespace sp {
 namespace {
 
 struct AdjArgs {
-  int I, J, K, SX, SY, kper;
+  int I, J, K, SX, SY;
   int64_t N;
   const double* xp;       // forward pressure (interior, shift applied below)
   double shift;
@@ -39,7 +39,7 @@ struct AdjArgs {
   const double* lam;
   double* pbar;
   double *gus, *gvs, *gws, *gk[3], *gb[3];
-  double* partials;
+  RedSite site;
   Scalars* S;
 };
 
@@ -59,14 +59,12 @@ __device__ inline double tface(const AdjArgs& A, int d, int64_t n, int64_t h) {
 
 // pbar (gathered per cell) and the correction part of the gradients
 __global__ void __launch_bounds__(256) k_adj_source(AdjArgs A) {
-  uint32_t bx, by, bz;
-  xcd_block3(bx, by, bz);
-  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
-  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
+  const Cell3 cq = cell_of(A.I, A.J, A.K);
+  const int a = cq.a, b = cq.b, c = cq.c;
   const int64_t N = A.N;
   double psum = 0.0;
-  if (a < A.I && b < A.J) {
-    for (int c = c0; c < c1; ++c) {
+  if (cq.in) {
+    {
       const int64_t n = nidx(A, a, b, c), h = hidx(A, a, b, c);
       const int co[3] = {a, b, c};
       const int lim[3] = {A.I, A.J, A.K};
@@ -97,52 +95,47 @@ __global__ void __launch_bounds__(256) k_adj_source(AdjArgs A) {
       psum += pb;
     }
   }
-  const double t = block_sum<kTileThreads>(psum);
-  if (flat_tid() == 0) store_partial(A.partials, flat_block(), t);
-  if (last_block(&A.S->counter[kCtrAdj])) {
-    const double tot = reduce_partials_sum<kTileThreads>(A.partials, num_blocks());
-    if (flat_tid() == 0) {
-      Scalars* S = A.S;
-      S->bsum = tot;
-      S->bmean = (S->singular && S->active > 0) ? tot / (double)S->active : 0.0;
-    }
-    reset_counter(&A.S->counter[kCtrAdj]);
+  double v[1] = {psum};
+  block_partials<kTileThreads, 1, 0u>(A.site, v);
+}
+
+__global__ void __launch_bounds__(kFinThreads) k_adj_source_fin(AdjArgs A, uint32_t nb) {
+  double t[1];
+  finalize_partials<kFinThreads, 1, 0u>(A.site, nb, t);
+  if (flat_tid() == 0) {
+    Scalars* S = A.S;
+    S->bsum = t[0];
+    S->bmean = (S->singular && S->active > 0) ? t[0] / (double)S->active : 0.0;
   }
 }
 
 // mean of lam over active cells (the gauge of a singular system)
 __global__ void __launch_bounds__(256) k_adj_lam_mean(AdjArgs A) {
-  uint32_t bx, by, bz;
-  xcd_block3(bx, by, bz);
-  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
-  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
-  double s = 0.0;
-  if (a < A.I && b < A.J)
-    for (int c = c0; c < c1; ++c) {
-      const int64_t n = nidx(A, a, b, c);
-      if (A.active[n]) s += A.lam[n];
-    }
-  const double t = block_sum<kTileThreads>(s);
-  if (flat_tid() == 0) store_partial(A.partials, flat_block(), t);
-  if (last_block(&A.S->counter[kCtrAdj2])) {
-    const double tot = reduce_partials_sum<kTileThreads>(A.partials, num_blocks());
-    if (flat_tid() == 0) {
-      Scalars* S = A.S;
-      S->npart[0] = (S->singular && S->active > 0) ? tot / (double)S->active : 0.0;
-    }
-    reset_counter(&A.S->counter[kCtrAdj2]);
+  const Cell3 q = cell_of(A.I, A.J, A.K);
+  double v[1] = {0.0};
+  if (q.in) {
+    const int64_t n = nidx(A, q.a, q.b, q.c);
+    if (A.active[n]) v[0] = A.lam[n];
+  }
+  block_partials<kTileThreads, 1, 0u>(A.site, v);
+}
+
+__global__ void __launch_bounds__(kFinThreads) k_adj_lam_mean_fin(AdjArgs A, uint32_t nb) {
+  double t[1];
+  finalize_partials<kFinThreads, 1, 0u>(A.site, nb, t);
+  if (flat_tid() == 0) {
+    Scalars* S = A.S;
+    S->npart[0] = (S->singular && S->active > 0) ? t[0] / (double)S->active : 0.0;
   }
 }
 
 // dL/du* += a G lam ; dL/dK -= a (G lam)(G p) ; dL/dbeta = dL/dK * Dp/delta
 __global__ void __launch_bounds__(256) k_adj_grad(AdjArgs A) {
-  uint32_t bx, by, bz;
-  xcd_block3(bx, by, bz);
-  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
-  if (a >= A.I || b >= A.J) return;
-  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
+  const Cell3 cq = cell_of(A.I, A.J, A.K);
+  if (!cq.in) return;
+  const int a = cq.a, b = cq.b, c = cq.c;
   const double lmean = A.S->npart[0];
-  for (int c = c0; c < c1; ++c) {
+  {
     const int64_t n = nidx(A, a, b, c), h = hidx(A, a, b, c);
     const int co[3] = {a, b, c};
     const int lim[3] = {A.I, A.J, A.K};
@@ -180,9 +173,9 @@ __global__ void __launch_bounds__(256) k_adj_grad(AdjArgs A) {
 sp_status adjoint(sp_ctx* c, const sp_adj_inputs& in, sp_adj_outputs& out, sp_stats* st) {
   const Dims& D = c->dm;
   const sp_inputs& fin = c->last_in;
-  Geom g = make_geom(D.I, D.J, D.K, 2048);
+  const dim3 fg = flat_grid(D.I, D.J, D.K), fb(64, 4, 1);
   AdjArgs A{};
-  A.I = D.I; A.J = D.J; A.K = D.K; A.SX = D.SX; A.SY = D.SY; A.kper = g.kper; A.N = D.N;
+  A.I = D.I; A.J = D.J; A.K = D.K; A.SX = D.SX; A.SY = D.SY; A.N = D.N;
   A.xp = c->x_ckpt;
   A.shift = c->S_host->shift;
   A.active = c->active;
@@ -200,7 +193,6 @@ sp_status adjoint(sp_ctx* c, const sp_adj_inputs& in, sp_adj_outputs& out, sp_st
   A.gus = out.gu_star; A.gvs = out.gv_star; A.gws = out.gw_star;
   A.gk[0] = out.gk_e; A.gk[1] = out.gk_n; A.gk[2] = out.gk_t;
   A.gb[0] = out.gbeta_e; A.gb[1] = out.gbeta_n; A.gb[2] = out.gbeta_t;
-  A.partials = c->partials;
   A.S = c->S;
   const size_t hb = sizeof(double) * (size_t)D.Nh;
   for (double* p : {out.gu_star, out.gv_star, out.gw_star}) {
@@ -209,18 +201,24 @@ sp_status adjoint(sp_ctx* c, const sp_adj_inputs& in, sp_adj_outputs& out, sp_st
   }
   const double NB = (double)D.N;
   int slot;
+  A.site = red_site(c, kSiteAdj);
   timer_begin(c, "adj_source", NB * (3 * 8 + 6 * 4 + 3 * 8 + 8 + 1 + 3 * 8 + 3 * 8 + 8), &slot);
-  hipLaunchKernelGGL(k_adj_source, g.grid, g.block, 0, c->stream, A);
+  hipLaunchKernelGGL(k_adj_source, fg, fb, 0, c->stream, A);
+  hipLaunchKernelGGL(k_adj_source_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A,
+                     flat_blocks(D.I, D.J, D.K));
   timer_end(c, slot);
   // solve A lam = pbar (mean-projected when singular) with the forward's
   // operator and multigrid hierarchy
   sp_status s = solve(c, c->pbar, c->x, false, st);
   if (s != SP_OK) return s;
+  A.site = red_site(c, kSiteAdj2);
   timer_begin(c, "adj_lam_mean", NB * 9, &slot);
-  hipLaunchKernelGGL(k_adj_lam_mean, g.grid, g.block, 0, c->stream, A);
+  hipLaunchKernelGGL(k_adj_lam_mean, fg, fb, 0, c->stream, A);
+  hipLaunchKernelGGL(k_adj_lam_mean_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A,
+                     flat_blocks(D.I, D.J, D.K));
   timer_end(c, slot);
   timer_begin(c, "adj_grad", NB * (8 * 3 + 1 + 3 * 8 + 3 * 8 * 2 + 3 * 8 * 2), &slot);
-  hipLaunchKernelGGL(k_adj_grad, g.grid, g.block, 0, c->stream, A);
+  hipLaunchKernelGGL(k_adj_grad, fg, fb, 0, c->stream, A);
   timer_end(c, slot);
   return SP_OK;
 }

@@ -16,6 +16,7 @@
 
 namespace sp {
 
+
 sp_status fail(sp_ctx* c, sp_status s, const std::string& msg) {
   if (c) c->last_error = msg;
   return s;
@@ -106,8 +107,9 @@ static sp_status alloc_levels(sp_ctx* c, Level<T>* lv, bool level0_owned) {
       L.nx = (lv[l - 1].nx + 1) / 2; L.ny = (lv[l - 1].ny + 1) / 2; L.nz = (lv[l - 1].nz + 1) / 2;
     }
     L.n = (int64_t)L.nx * L.ny * L.nz;
-    if (l == 0 && !level0_owned) continue;
     sp_status s;
+    if ((s = dmalloc(c, &L.xs, L.n))) return s;
+    if (l == 0 && !level0_owned) continue;
     if ((s = dmalloc(c, &L.cx, L.n)) || (s = dmalloc(c, &L.cy, L.n)) || (s = dmalloc(c, &L.cz, L.n)) ||
         (s = dmalloc(c, &L.e_store, L.n)) || (s = dmalloc(c, &L.b, L.n)) || (s = dmalloc(c, &L.x, L.n)))
       return s;
@@ -162,7 +164,9 @@ static sp_status ensure_buffers(sp_ctx* c) {
   if ((s = dmalloc(c, &c->col_last, 12 * (int64_t)D.I * D.J)) ||
       (s = dmalloc(c, &c->col_prefix, 12 * (int64_t)D.I * D.J)))
     return s;
-  if ((s = dmalloc(c, &c->partials, (size_t)kNumPartSlots * kMaxPartials))) return s;
+  c->red_max_blocks = (uint32_t)std::max<int64_t>(
+      4096, (int64_t)((D.I + 2 + 63) / 64) * ((D.J + 2 + 3) / 4) * (D.K + 2));
+  if ((s = dmalloc(c, &c->red_part, (size_t)kNumSites * kSiteValues * c->red_max_blocks))) return s;
   for (int f = 0; f < 6; ++f) {
     const int64_t m = std::max<int64_t>({(int64_t)D.J * D.K, (int64_t)D.I * D.K, (int64_t)D.I * D.J});
     if ((s = dmalloc(c, &c->bcv[f], m))) return s;
@@ -352,7 +356,8 @@ sp_status sp_create(sp_ctx** out, const sp_params* params, int device) {
   D.Nh = (int64_t)D.SX * D.SY * (D.K + 2);
   if (hipMalloc(&c->S, sizeof(Scalars)) != hipSuccess ||
       hipHostMalloc(&c->S_host, sizeof(Scalars), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(&c->hflags, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&c->hflags, 4 * sizeof(int32_t), hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&c->hflags_dev, c->hflags, 0) != hipSuccess ||
       hipMalloc(&c->dxp, sizeof(double) * D.I) != hipSuccess ||
       hipMalloc(&c->dyp, sizeof(double) * D.J) != hipSuccess ||
       hipMalloc(&c->dzp, sizeof(double) * D.K) != hipSuccess ||

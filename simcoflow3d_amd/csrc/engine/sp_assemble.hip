@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(256) k_beta_coef(BetaArgs A) {
 }
 
 struct OpArgs {
-  int I, J, K, SX, SY, kper;
+  int I, J, K, SX, SY;
   int64_t N;
   const double* pocoef;
   const double *ee, *ne, *te;
@@ -262,129 +262,103 @@ struct OpArgs {
   float *cx32, *cy32, *cz32, *e32;
   uint8_t* active;
   float* dw;
-  double* partials;
+  RedSite site;
   Scalars* S;
 };
 
 // Symmetric operator, diagonal-only terms and RHS for one cell.
 __global__ void __launch_bounds__(256) k_operator(OpArgs A) {
 #pragma clang fp contract(off)
-  uint32_t bx, by, bz;
-  xcd_block3(bx, by, bz);
-  const int a = bx * kTX + threadIdx.x;
-  const int b = by * kTY + threadIdx.y;
-  const int c0 = bz * A.kper;
-  const int c1 = c0 + A.kper < A.K ? c0 + A.kper : A.K;
+  const Cell3 q = cell_of(A.I, A.J, A.K);
+  const int a = q.a, b = q.b, c = q.c;
   const int64_t IJ = (int64_t)A.I * A.J, SXY = (int64_t)A.SX * A.SY, N = A.N;
-  double bsum = 0.0, esum = 0.0, bmax = 0.0, cnt = 0.0, bad = 0.0, dsum = 0.0;
-  if (a < A.I && b < A.J) {
-    const double dx = A.dxp[a], dy = A.dyp[b];
-    for (int c = c0; c < c1; ++c) {
-      const double dz = A.dzp[c];
-      const int64_t n = (int64_t)a + (int64_t)A.I * ((int64_t)b + (int64_t)A.J * c);
-      const int64_t h = (int64_t)(a + 1) + (int64_t)A.SX * ((int64_t)(b + 1) + (int64_t)A.SY * (c + 1));
-      const int ctP = A.ct[n];
-      double cxv = 0, cyv = 0, czv = 0, cW = 0, cS = 0, cB = 0, ex = 0, rhs = 0;
-      bool act = false;
-      double dval = 0.0;
-      if (ctP != 2) {
-        const double W = A.pocoef[n], S = A.pocoef[N + n], B = A.pocoef[2 * N + n];
-        const double E = A.pocoef[3 * N + n], Nn = A.pocoef[4 * N + n], T = A.pocoef[5 * N + n];
-        const bool wS = a > 0 ? A.ct[n - 1] == 2 : A.fl[0] == 0;
-        const bool sS = b > 0 ? A.ct[n - A.I] == 2 : A.fl[2] == 0;
-        const bool bS = c > 0 ? A.ct[n - IJ] == 2 : A.fl[4] == 0;
-        const bool eS = a < A.I - 1 ? A.ct[n + 1] == 2 : A.fl[1] == 0;
-        const bool nS = b < A.J - 1 ? A.ct[n + A.I] == 2 : A.fl[3] == 0;
-        const bool tS = c < A.K - 1 ? A.ct[n + IJ] == 2 : A.fl[5] == 0;
-        // couplings to active neighbours: the lower cell's E/N/T coefficient
-        if (a < A.I - 1 && A.ct[n + 1] != 2) cxv = E * A.ee[h] * dy * dz;
-        if (b < A.J - 1 && A.ct[n + A.I] != 2) cyv = Nn * A.ne[h] * dx * dz;
-        if (c < A.K - 1 && A.ct[n + IJ] != 2) czv = T * A.te[h] * dx * dy;
-        if (a > 0 && A.ct[n - 1] != 2) cW = A.pocoef[3 * N + n - 1] * A.ee[h - 1] * dy * dz;
-        if (b > 0 && A.ct[n - A.I] != 2) cS = A.pocoef[4 * N + n - A.I] * A.ne[h - A.SX] * dx * dz;
-        if (c > 0 && A.ct[n - IJ] != 2) cB = A.pocoef[5 * N + n - IJ] * A.te[h - SXY] * dx * dy;
-        // diagonal-only terms with the row's own coefficient (:379-430)
-        if (wS) ex = ex + W * A.ee[h - 1] * dy * dz;
-        if (sS) ex = ex + S * A.ne[h - A.SX] * dx * dz;
-        if (bS) ex = ex + B * A.te[h - SXY] * dx * dy;
-        if (eS) ex = ex + E * A.ee[h] * dy * dz;
-        if (nS) ex = ex + Nn * A.ne[h] * dx * dz;
-        if (tS) ex = ex + T * A.te[h] * dx * dy;
-        double d = cW;
-        d = d + cS;
-        d = d + cB;
-        d = d + cxv;
-        d = d + cyv;
-        d = d + czv;
-        d = d + ex;
-        act = d > 0.0;
-        dval = act ? d : 0.0;
-        if (!is_finite(d)) bad = 1.0;
-        // RHS (:521-546)
-        rhs = -dy * dz * (A.ee[h] * A.u[h] - A.ee[h - 1] * A.u[h - 1]) -
-              dx * dz * (A.ne[h] * A.v[h] - A.ne[h - A.SX] * A.v[h - A.SX]) -
-              dx * dy * (A.te[h] * A.w[h] - A.te[h - SXY] * A.w[h - SXY]);
-        rhs = rhs + (A.vof[h] - A.vofO[h]) * dx * dy * dz / A.dt;
-        if (A.fl[0] == 0 && a == 0) rhs = rhs + W * A.bcv[0][b + (int64_t)A.J * c];
-        if (A.fl[1] == 0 && a == A.I - 1) rhs = rhs + E * A.bcv[1][b + (int64_t)A.J * c];
-        if (A.fl[2] == 0 && b == 0) rhs = rhs + S * A.bcv[2][a + (int64_t)A.I * c];
-        if (A.fl[3] == 0 && b == A.J - 1) rhs = rhs + Nn * A.bcv[3][a + (int64_t)A.I * c];
-        if (A.fl[4] == 0 && c == 0) rhs = rhs + B * A.bcv[4][a + (int64_t)A.I * b];
-        if (A.fl[5] == 0 && c == A.K - 1) rhs = rhs + T * A.bcv[5][a + (int64_t)A.I * b];
-        if (act) {
-          if (!is_finite(rhs)) bad = 1.0;
-          bsum += rhs;
-          esum += fabs(ex);
-          cnt += 1.0;
-          dsum += (double)(float)dval;
-          const double vb = fabs(rhs) / (dx * dy * dz);
-          bmax = vb > bmax ? vb : bmax;
-        }
-      }
-      A.cx[n] = cxv; A.cy[n] = cyv; A.cz[n] = czv; A.e[n] = ex; A.b[n] = rhs;
-      A.active[n] = act ? 1 : 0;
-      A.dw[n] = (float)dval;
-      if (A.cx32) {
-        A.cx32[n] = (float)cxv; A.cy32[n] = (float)cyv; A.cz32[n] = (float)czv;
-        A.e32[n] = (float)ex;
+  // sum b, sum |e|, max |b|/V, active count, non-finite, sum d (as stored)
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (q.in) {
+    const double dx = A.dxp[a], dy = A.dyp[b], dz = A.dzp[c];
+    const int64_t n = (int64_t)a + (int64_t)A.I * ((int64_t)b + (int64_t)A.J * c);
+    const int64_t h = (int64_t)(a + 1) + (int64_t)A.SX * ((int64_t)(b + 1) + (int64_t)A.SY * (c + 1));
+    const int ctP = A.ct[n];
+    double cxv = 0, cyv = 0, czv = 0, cW = 0, cS = 0, cB = 0, ex = 0, rhs = 0, dval = 0;
+    bool act = false;
+    if (ctP != 2) {
+      const double W = A.pocoef[n], S = A.pocoef[N + n], B = A.pocoef[2 * N + n];
+      const double E = A.pocoef[3 * N + n], Nn = A.pocoef[4 * N + n], T = A.pocoef[5 * N + n];
+      const bool wS = a > 0 ? A.ct[n - 1] == 2 : A.fl[0] == 0;
+      const bool sS = b > 0 ? A.ct[n - A.I] == 2 : A.fl[2] == 0;
+      const bool bS = c > 0 ? A.ct[n - IJ] == 2 : A.fl[4] == 0;
+      const bool eS = a < A.I - 1 ? A.ct[n + 1] == 2 : A.fl[1] == 0;
+      const bool nS = b < A.J - 1 ? A.ct[n + A.I] == 2 : A.fl[3] == 0;
+      const bool tS = c < A.K - 1 ? A.ct[n + IJ] == 2 : A.fl[5] == 0;
+      // couplings to active neighbours: the lower cell's E/N/T coefficient
+      if (a < A.I - 1 && A.ct[n + 1] != 2) cxv = E * A.ee[h] * dy * dz;
+      if (b < A.J - 1 && A.ct[n + A.I] != 2) cyv = Nn * A.ne[h] * dx * dz;
+      if (c < A.K - 1 && A.ct[n + IJ] != 2) czv = T * A.te[h] * dx * dy;
+      if (a > 0 && A.ct[n - 1] != 2) cW = A.pocoef[3 * N + n - 1] * A.ee[h - 1] * dy * dz;
+      if (b > 0 && A.ct[n - A.I] != 2) cS = A.pocoef[4 * N + n - A.I] * A.ne[h - A.SX] * dx * dz;
+      if (c > 0 && A.ct[n - IJ] != 2) cB = A.pocoef[5 * N + n - IJ] * A.te[h - SXY] * dx * dy;
+      // diagonal-only terms with the row's own coefficient (:379-430)
+      if (wS) ex = ex + W * A.ee[h - 1] * dy * dz;
+      if (sS) ex = ex + S * A.ne[h - A.SX] * dx * dz;
+      if (bS) ex = ex + B * A.te[h - SXY] * dx * dy;
+      if (eS) ex = ex + E * A.ee[h] * dy * dz;
+      if (nS) ex = ex + Nn * A.ne[h] * dx * dz;
+      if (tS) ex = ex + T * A.te[h] * dx * dy;
+      double d = cW;
+      d = d + cS;
+      d = d + cB;
+      d = d + cxv;
+      d = d + cyv;
+      d = d + czv;
+      d = d + ex;
+      act = d > 0.0;
+      dval = act ? d : 0.0;
+      if (!is_finite(d)) v[4] = 1.0;
+      // RHS (:521-546)
+      rhs = -dy * dz * (A.ee[h] * A.u[h] - A.ee[h - 1] * A.u[h - 1]) -
+            dx * dz * (A.ne[h] * A.v[h] - A.ne[h - A.SX] * A.v[h - A.SX]) -
+            dx * dy * (A.te[h] * A.w[h] - A.te[h - SXY] * A.w[h - SXY]);
+      rhs = rhs + (A.vof[h] - A.vofO[h]) * dx * dy * dz / A.dt;
+      if (A.fl[0] == 0 && a == 0) rhs = rhs + W * A.bcv[0][b + (int64_t)A.J * c];
+      if (A.fl[1] == 0 && a == A.I - 1) rhs = rhs + E * A.bcv[1][b + (int64_t)A.J * c];
+      if (A.fl[2] == 0 && b == 0) rhs = rhs + S * A.bcv[2][a + (int64_t)A.I * c];
+      if (A.fl[3] == 0 && b == A.J - 1) rhs = rhs + Nn * A.bcv[3][a + (int64_t)A.I * c];
+      if (A.fl[4] == 0 && c == 0) rhs = rhs + B * A.bcv[4][a + (int64_t)A.I * b];
+      if (A.fl[5] == 0 && c == A.K - 1) rhs = rhs + T * A.bcv[5][a + (int64_t)A.I * b];
+      if (act) {
+        if (!is_finite(rhs)) v[4] = 1.0;
+        v[0] = rhs;
+        v[1] = fabs(ex);
+        v[2] = fabs(rhs) / (dx * dy * dz);
+        v[3] = 1.0;
+        v[5] = (double)(float)dval;
       }
     }
+    A.cx[n] = cxv; A.cy[n] = cyv; A.cz[n] = czv; A.e[n] = ex; A.b[n] = rhs;
+    A.active[n] = act ? 1 : 0;
+    A.dw[n] = (float)dval;
+    if (A.cx32) {
+      A.cx32[n] = (float)cxv; A.cy32[n] = (float)cyv; A.cz32[n] = (float)czv;
+      A.e32[n] = (float)ex;
+    }
   }
-  const uint32_t blk = flat_block(), nb = num_blocks();
-  double v0 = block_sum<kTileThreads>(bsum);
-  double v1 = block_sum<kTileThreads>(esum);
-  double v2 = block_max<kTileThreads>(bmax);
-  double v3 = block_sum<kTileThreads>(cnt);
-  double v4 = block_max<kTileThreads>(bad);
-  double v5 = block_sum<kTileThreads>(dsum);
+  // value 2 (max |b|/V) and 4 (non-finite flag) are maxima
+  block_partials<kTileThreads, 6, 0x14u>(A.site, v);
+}
+
+__global__ void __launch_bounds__(kFinThreads) k_operator_fin(OpArgs A, uint32_t nb) {
+  double t[6];
+  finalize_partials<kFinThreads, 6, 0x14u>(A.site, nb, t);
   if (flat_tid() == 0) {
-    store_partial(A.partials + 0 * (int64_t)kMaxPartials, blk, v0);
-    store_partial(A.partials + 1 * (int64_t)kMaxPartials, blk, v1);
-    store_partial(A.partials + 2 * (int64_t)kMaxPartials, blk, v2);
-    store_partial(A.partials + 3 * (int64_t)kMaxPartials, blk, v3);
-    store_partial(A.partials + 4 * (int64_t)kMaxPartials, blk, v4);
-    store_partial(A.partials + 5 * (int64_t)kMaxPartials, blk, v5);
-  }
-  if (last_block(&A.S->counter[kCtrOperator])) {
-    const double s0 = reduce_partials_sum<kTileThreads>(A.partials, nb);
-    const double s1 = reduce_partials_sum<kTileThreads>(A.partials + kMaxPartials, nb);
-    const double s2 = reduce_partials_max<kTileThreads>(A.partials + 2 * (int64_t)kMaxPartials, nb);
-    const double act = reduce_partials_sum<kTileThreads>(A.partials + 3 * (int64_t)kMaxPartials, nb);
-    const double s4 = reduce_partials_max<kTileThreads>(A.partials + 4 * (int64_t)kMaxPartials, nb);
-    const double s5 = reduce_partials_sum<kTileThreads>(A.partials + 5 * (int64_t)kMaxPartials, nb);
-    if (flat_tid() == 0) {
-      const bool nonfin = s4 > 0.0;
-      Scalars* S = A.S;
-      S->bsum = s0;
-      S->esum = s1;
-      S->bmax = s2;
-      S->dsum = s5;
-      S->active = (int64_t)act;
-      S->singular = s1 == 0.0 ? 1 : 0;
-      S->bmean = (s1 == 0.0 && act > 0) ? s0 / act : 0.0;
-      S->nonfinite = (nonfin || !is_finite(s0) || !is_finite(s2)) ? 1 : 0;
-    }
-    reset_counter(&A.S->counter[kCtrOperator]);
+    Scalars* S = A.S;
+    S->bsum = t[0];
+    S->esum = t[1];
+    S->bmax = t[2];
+    S->active = (int64_t)t[3];
+    S->dsum = t[5];
+    S->singular = t[1] == 0.0 ? 1 : 0;
+    S->bmean = (t[1] == 0.0 && t[3] > 0) ? t[0] / t[3] : 0.0;
+    S->nonfinite = (t[4] > 0.0 || !is_finite(t[0]) || !is_finite(t[2])) ? 1 : 0;
   }
 }
 
@@ -455,8 +429,6 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_out) {
 
   OpArgs O{};
   O.I = D.I; O.J = D.J; O.K = D.K; O.SX = D.SX; O.SY = D.SY; O.N = D.N;
-  Geom g = make_geom(D.I, D.J, D.K, 2048);
-  O.kper = g.kper;
   O.pocoef = c->pocoef;
   O.ee = in.ee_area; O.ne = in.ne_area; O.te = in.te_area;
   O.ct = in.cell_type_p;
@@ -468,18 +440,21 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_out) {
   O.dxp = c->dxp; O.dyp = c->dyp; O.dzp = c->dzp;
   O.dt = in.dt;
   O.cx = c->cx; O.cy = c->cy; O.cz = c->cz; O.e = c->ee; O.b = c->b;
-  if (mixed) {
+  if (mixed && c->lf[0].cx) {
     O.cx32 = c->lf[0].cx; O.cy32 = c->lf[0].cy; O.cz32 = c->lf[0].cz; O.e32 = c->lf[0].e_store;
   }
   O.active = c->active;
   O.dw = c->dw;
-  O.partials = c->partials;
+  O.site = red_site(c, kSiteOperator);
   O.S = c->S;
-  if (g.blocks() > (uint32_t)kMaxPartials) return fail(c, SP_ERR_INVALID_ARG, "grid too large");
+  if (flat_blocks(D.I, D.J, D.K) > c->red_max_blocks)
+    return fail(c, SP_ERR_INVALID_ARG, "grid too large for the reduction buffers");
   timer_begin(c, "operator_rhs",
-              (double)D.N * (6 * 8 + 3 * 8 + 4 + 5 * 8 + 4 * 8 + 8 + 1 + (mixed ? 16 : 0)), &slot);
-  hipLaunchKernelGGL(k_operator, g.grid, g.block, 0, c->stream, O);
+              (double)D.N * (6 * 8 + 3 * 8 + 4 + 5 * 8 + 4 * 8 + 8 + 1 + 4 + (mixed ? 16 : 0)), &slot);
+  hipLaunchKernelGGL(k_operator, flat_grid(D.I, D.J, D.K), dim3(64, 4, 1), 0, c->stream, O);
   timer_end(c, slot);
+  hipLaunchKernelGGL(k_operator_fin, dim3(1), dim3(kFinThreads), 0, c->stream, O,
+                     flat_blocks(D.I, D.J, D.K));
   return SP_OK;
 }
 

@@ -15,7 +15,7 @@ namespace sp {
 namespace {
 
 struct CorrArgs {
-  int I, J, K, SX, SY, kper;
+  int I, J, K, SX, SY;
   int64_t N;
   const double* x;
   const uint8_t* active;
@@ -27,36 +27,33 @@ struct CorrArgs {
   const double *dxp, *dyp, *dzp;
   double *u, *v, *w, *Pp;
   int64_t pin;  // interior index of the gauge cell or -1
-  double* partials;
+  RedSite site;
   Scalars* S;
 };
 
 // shift = x(pin) or mean of x over active cells (singular systems only)
 __global__ void __launch_bounds__(256) k_shift(CorrArgs A) {
   Scalars* S = A.S;
-  uint32_t bx, by, bz;
-  xcd_block3(bx, by, bz);
-  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
-  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
-  double s = 0.0;
-  if (a < A.I && b < A.J)
-    for (int c = c0; c < c1; ++c) {
-      const int64_t n = a + (int64_t)A.I * (b + (int64_t)A.J * c);
-      if (A.active[n]) s += A.x[n];
+  const Cell3 q = cell_of(A.I, A.J, A.K);
+  double v[1] = {0.0};
+  if (q.in) {
+    const int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c);
+    if (A.active[n]) v[0] = A.x[n];
+  }
+  block_partials<kTileThreads, 1, 0u>(A.site, v);
+}
+
+__global__ void __launch_bounds__(kFinThreads) k_shift_fin(CorrArgs A, uint32_t nb) {
+  double t[1];
+  finalize_partials<kFinThreads, 1, 0u>(A.site, nb, t);
+  if (flat_tid() == 0) {
+    Scalars* S = A.S;
+    double sh = 0.0;
+    if (S->singular) {
+      if (A.pin >= 0 && A.active[A.pin]) sh = A.x[A.pin];
+      else if (S->active > 0) sh = t[0] / (double)S->active;
     }
-  const double t = block_sum<kTileThreads>(s);
-  if (flat_tid() == 0) store_partial(A.partials, flat_block(), t);
-  if (last_block(&S->counter[kCtrShift])) {
-    const double tot = reduce_partials_sum<kTileThreads>(A.partials, num_blocks());
-    if (flat_tid() == 0) {
-      double sh = 0.0;
-      if (S->singular) {
-        if (A.pin >= 0 && A.active[A.pin]) sh = A.x[A.pin];
-        else if (S->active > 0) sh = tot / (double)S->active;
-      }
-      S->shift = sh;
-    }
-    reset_counter(&S->counter[kCtrShift]);
+    S->shift = sh;
   }
 }
 
@@ -66,14 +63,12 @@ __device__ inline double pval(const CorrArgs& A, int64_t n, double shift) {
 
 __global__ void __launch_bounds__(256) k_correct(CorrArgs A) {
 #pragma clang fp contract(off)
-  uint32_t bx, by, bz;
-  xcd_block3(bx, by, bz);
-  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
-  if (a >= A.I || b >= A.J) return;
-  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
+  const Cell3 q = cell_of(A.I, A.J, A.K);
+  if (!q.in) return;
+  const int a = q.a, b = q.b, c = q.c;
   const double shift = A.S->shift;
   const int64_t IJ = (int64_t)A.I * A.J, N = A.N;
-  for (int c = c0; c < c1; ++c) {
+  {
     const int64_t n = a + (int64_t)A.I * (b + (int64_t)A.J * c);
     const int64_t h = (a + 1) + (int64_t)A.SX * ((b + 1) + (int64_t)A.SY * (c + 1));
     // Pp(i+1)-Pp(i): between two active cells the gauge cancels, so the
@@ -135,45 +130,39 @@ __global__ void __launch_bounds__(256) k_pressure_field(CorrArgs A) {
 // not touch.
 __global__ void __launch_bounds__(256) k_divergence(CorrArgs A) {
 #pragma clang fp contract(off)
-  Scalars* S = A.S;
-  uint32_t bx, by, bz;
-  xcd_block3(bx, by, bz);
-  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
-  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
+  const Cell3 q = cell_of(A.I, A.J, A.K);
   const int64_t SXY = (int64_t)A.SX * A.SY;
-  double m = 0.0;
-  if (a < A.I && b < A.J) {
-    const double dx = A.dxp[a], dy = A.dyp[b];
-    for (int c = c0; c < c1; ++c) {
-      const double dz = A.dzp[c];
-      const int64_t h = (a + 1) + (int64_t)A.SX * ((b + 1) + (int64_t)A.SY * (c + 1));
-      const double uw = a > 0 ? A.u[h - 1] : A.us[h - 1];
-      const double vs = b > 0 ? A.v[h - A.SX] : A.vs[h - A.SX];
-      const double wb = c > 0 ? A.w[h - SXY] : A.ws[h - SXY];
-      const double r = dy * dz * (A.ee[h] * A.u[h] - A.ee[h - 1] * uw) +
-                       dx * dz * (A.ne[h] * A.v[h] - A.ne[h - A.SX] * vs) +
-                       dx * dy * (A.te[h] * A.w[h] - A.te[h - SXY] * wb);
-      double q = fabs(r) / (dx * dy * dz);
-      if (!(q <= 1.0e300)) q = 1.0e300;  // NaN / Inf
-      m = q > m ? q : m;
-    }
+  double v[1] = {0.0};
+  if (q.in) {
+    const int a = q.a, b = q.b, c = q.c;
+    const double dx = A.dxp[a], dy = A.dyp[b], dz = A.dzp[c];
+    const int64_t h = (a + 1) + (int64_t)A.SX * ((b + 1) + (int64_t)A.SY * (c + 1));
+    const double uw = a > 0 ? A.u[h - 1] : A.us[h - 1];
+    const double vs = b > 0 ? A.v[h - A.SX] : A.vs[h - A.SX];
+    const double wb = c > 0 ? A.w[h - SXY] : A.ws[h - SXY];
+    const double r = dy * dz * (A.ee[h] * A.u[h] - A.ee[h - 1] * uw) +
+                     dx * dz * (A.ne[h] * A.v[h] - A.ne[h - A.SX] * vs) +
+                     dx * dy * (A.te[h] * A.w[h] - A.te[h - SXY] * wb);
+    double m = fabs(r) / (dx * dy * dz);
+    if (!(m <= 1.0e300)) m = 1.0e300;  // NaN / Inf
+    v[0] = m;
   }
-  const double t = block_max<kTileThreads>(m);
-  if (flat_tid() == 0) store_partial(A.partials, flat_block(), t);
-  if (last_block(&S->counter[kCtrDiv])) {
-    const double mm = reduce_partials_max<kTileThreads>(A.partials, num_blocks());
-    if (flat_tid() == 0) S->divmax = mm;
-    reset_counter(&S->counter[kCtrDiv]);
-  }
+  block_partials<kTileThreads, 1, 1u>(A.site, v);
+}
+
+__global__ void __launch_bounds__(kFinThreads) k_divergence_fin(CorrArgs A, uint32_t nb) {
+  double t[1];
+  finalize_partials<kFinThreads, 1, 1u>(A.site, nb, t);
+  if (flat_tid() == 0) A.S->divmax = t[0];
 }
 
 }  // namespace
 
 sp_status correct(sp_ctx* c, const sp_inputs& in, sp_outputs& out, sp_stats* st) {
   const Dims& D = c->dm;
-  Geom g = make_geom(D.I, D.J, D.K, 2048);
   CorrArgs A{};
-  A.I = D.I; A.J = D.J; A.K = D.K; A.SX = D.SX; A.SY = D.SY; A.kper = g.kper; A.N = D.N;
+  A.I = D.I; A.J = D.J; A.K = D.K; A.SX = D.SX; A.SY = D.SY; A.N = D.N;
+  const dim3 fg = flat_grid(D.I, D.J, D.K), fb(64, 4, 1);
   A.x = c->x; A.active = c->active; A.pocoef = c->pocoef;
   A.ctu = in.cell_type_u; A.ctv = in.cell_type_v; A.ctw = in.cell_type_w;
   A.mu = in.moex_u; A.mv = in.moex_v; A.mw = in.moex_w;
@@ -188,22 +177,27 @@ sp_status correct(sp_ctx* c, const sp_inputs& in, sp_outputs& out, sp_stats* st)
     A.pin = (P.pin_i - 1) + (int64_t)D.I * ((P.pin_j - 1) + (int64_t)D.J * (P.pin_k - 1));
   else
     A.pin = -1;
-  A.partials = c->partials;
   A.S = c->S;
   const double NB = (double)D.N;
   int slot;
+  A.site = red_site(c, kSiteShift);
   timer_begin(c, "shift", NB * 9, &slot);
-  hipLaunchKernelGGL(k_shift, g.grid, g.block, 0, c->stream, A);
+  hipLaunchKernelGGL(k_shift, fg, fb, 0, c->stream, A);
   timer_end(c, slot);
+  hipLaunchKernelGGL(k_shift_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A,
+                     flat_blocks(D.I, D.J, D.K));
   timer_begin(c, "correct", NB * (8 + 1 + 3 * 8 + 6 * 4 + 3 * 8 + 3 * 8), &slot);
-  hipLaunchKernelGGL(k_correct, g.grid, g.block, 0, c->stream, A);
+  hipLaunchKernelGGL(k_correct, fg, fb, 0, c->stream, A);
   timer_end(c, slot);
   dim3 blk(64, 4, 1), grd((D.I + 2 + 63) / 64, (D.J + 2 + 3) / 4, D.K + 2);
   timer_begin(c, "pressure_field", (double)D.Nh * 8 + NB * 9, &slot);
   hipLaunchKernelGGL(k_pressure_field, grd, blk, 0, c->stream, A);
   timer_end(c, slot);
+  A.site = red_site(c, kSiteDiv);
   timer_begin(c, "divergence", NB * 6 * 8, &slot);
-  hipLaunchKernelGGL(k_divergence, g.grid, g.block, 0, c->stream, A);
+  hipLaunchKernelGGL(k_divergence, fg, fb, 0, c->stream, A);
+  hipLaunchKernelGGL(k_divergence_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A,
+                     flat_blocks(D.I, D.J, D.K));
   timer_end(c, slot);
   (void)st;
   return SP_OK;

@@ -132,4 +132,124 @@ __device__ inline double reduce_partials_max(const double* slot, uint32_t nb) {
 
 __device__ inline bool is_finite(double v) { return v - v == 0.0; }
 
+// ---- deterministic grid reductions ----
+//
+// Every workgroup folds its values (fixed tree) and stores one partial per
+// value with a plain store; a one-workgroup finalize kernel launched next on
+// the same stream folds the partials in workgroup order.  The kernel boundary
+// makes the partials visible, so no device-scope fences are needed, and the
+// summation order depends only on the launch shape (bitwise reproducible).
+// Layout of one reduction site: [kSiteValues][max_blocks] partials.
+struct RedSite {
+  double* part;
+  uint32_t max_blocks;
+};
+
+// Reduce NV values over the workgroup with one barrier; results valid in
+// thread 0 (fixed order: lanes, then waves).
+template <int NT, int NV, unsigned MAXMASK>
+__device__ inline void block_reduce_multi(double (&v)[NV]) {
+  __shared__ double sm[NV][NT / 64];
+  const int tid = flat_tid();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = ((MAXMASK >> q) & 1u) ? wave_max(v[q]) : wave_sum(v[q]);
+  __syncthreads();
+  if ((tid & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sm[q][tid >> 6] = v[q];
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const bool mx = (MAXMASK >> q) & 1u;
+      double s = sm[q][0];
+      for (int w = 1; w < NT / 64; ++w) s = mx ? (sm[q][w] > s ? sm[q][w] : s) : s + sm[q][w];
+      v[q] = s;
+    }
+  }
+}
+
+template <int NT, int NV, unsigned MAXMASK>
+__device__ inline void block_partials(const RedSite& R, double (&v)[NV]) {
+  const uint32_t b = flat_block();
+  block_reduce_multi<NT, NV, MAXMASK>(v);
+  if (flat_tid() == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) R.part[(size_t)q * R.max_blocks + b] = v[q];
+  }
+}
+
+// Executed by one workgroup of NT threads: folds nb partials per value.
+// Each thread keeps 8 independent accumulators so loads stay in flight.
+template <int NT, int NV, unsigned MAXMASK>
+__device__ inline void finalize_partials(const RedSite& R, uint32_t nb, double (&out)[NV]) {
+  __shared__ double tot[NV];
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const bool mx = (MAXMASK >> q) & 1u;
+    const double* p = R.part + (size_t)q * R.max_blocks;
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t k = flat_tid();
+    for (; k + 7 * NT < nb; k += 8 * NT) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const double x = p[k + u * NT];
+        s[u] = mx ? (x > s[u] ? x : s[u]) : s[u] + x;
+      }
+    }
+    for (int u = 0; k < nb; k += NT, ++u) {
+      const double x = p[k];
+      s[u] = mx ? (x > s[u] ? x : s[u]) : s[u] + x;
+    }
+    double t = s[0];
+#pragma unroll
+    for (int u = 1; u < 8; ++u) t = mx ? (s[u] > t ? s[u] : t) : t + s[u];
+    acc[q] = t;
+  }
+  block_reduce_multi<NT, NV, MAXMASK>(acc);
+  if (flat_tid() == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) tot[q] = acc[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) out[q] = tot[q];
+}
+
+inline RedSite red_site(const sp_ctx* c, int id) {
+  RedSite r;
+  r.part = c->red_part + (size_t)kSiteValues * c->red_max_blocks * id;
+  r.max_blocks = c->red_max_blocks;
+  return r;
+}
+
+constexpr int kFinThreads = 1024;
+
+// Flat launch: one cell per thread, 64x4 workgroups.
+inline dim3 flat_grid(int nx, int ny, int nz) {
+  return dim3((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)nz);
+}
+inline uint32_t flat_blocks(int nx, int ny, int nz) {
+  return (uint32_t)((nx + 63) / 64) * (uint32_t)((ny + 3) / 4) * (uint32_t)nz;
+}
+
+// One cell per thread over a 3-D grid of 64x4 workgroups, workgroup order
+// remapped so that every XCD owns a contiguous slab.
+struct Cell3 {
+  int a, b, c;
+  bool in;
+};
+__device__ inline Cell3 cell_of(int nx, int ny, int nz) {
+  uint32_t bx, by, bz;
+  xcd_block3(bx, by, bz);
+  Cell3 r;
+  r.a = (int)(bx * blockDim.x + threadIdx.x);
+  r.b = (int)(by * blockDim.y + threadIdx.y);
+  r.c = (int)bz;
+  r.in = r.a < nx && r.b < ny && r.c < nz;
+  return r;
+}
+
 }  // namespace sp

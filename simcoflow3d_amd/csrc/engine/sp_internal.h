@@ -40,7 +40,22 @@ struct Scalars {
   uint32_t pad2[3];
 };
 
-// Counter slots
+// Reduction sites (each owns partials and counters, see grid_reduce)
+enum RedSiteId : int {
+  kSiteOperator = 0,
+  kSiteResidual,
+  kSiteDots,
+  kSiteSpmv,
+  kSiteUpdate,
+  kSiteShift,
+  kSiteDiv,
+  kSiteAdj,
+  kSiteAdj2,
+  kNumSites
+};
+constexpr int kSiteValues = 8;  // values per site at most
+
+// Counter slots (legacy single-level reductions)
 enum CounterSlot : int {
   kCtrOperator = 0,
   kCtrInit = 1,
@@ -66,6 +81,7 @@ struct Level {
   T* e;        // diagonal-only terms, nullptr when they are all zero
   T* e_store;  // storage for e
   T *x, *b;
+  T* xs;       // pre-smoothed x (input of the fused post-smoother)
 };
 
 struct Dims {
@@ -128,6 +144,11 @@ struct sp_ctx {
   int64_t* col_last = nullptr;    // 12 * I*J loop positions
   int64_t* col_prefix = nullptr;  // 12 * I*J
 
+  // two-level reductions
+  double* red_part = nullptr;
+  uint32_t red_max_blocks = 0;
+  int32_t* hflags_dev = nullptr;  // device view of hflags
+
   // scalars + partials
   sp::Scalars* S = nullptr;
   sp::Scalars* S_host = nullptr;  // pinned
@@ -166,6 +187,7 @@ struct sp_ctx {
 
 namespace sp {
 
+struct RedSite;
 // ---- host helpers (sp_api.hip) ----
 sp_status fail(sp_ctx* c, sp_status s, const std::string& msg);
 sp_status hip_check(sp_ctx* c, hipError_t e, const char* what);

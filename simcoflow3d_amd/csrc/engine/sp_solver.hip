@@ -3,7 +3,8 @@
 // Replaces HYPRE PCG + BoomerAMG (ProjectionP.f90:95-116, 267-299).  The
 // outer CG, its dot products and the residual are FP64; the V-cycle runs in
 // T = float (mixed) or double (FP64).  CG scalars stay in device memory: the
-// host only polls the convergence flag, one iteration behind.
+// host only polls a convergence flag the finalize kernel writes into pinned
+// host memory, one iteration behind.
 //
 // Multigrid: cell-centred coarsening by 2, piecewise-constant transfers
 // (restriction = sum over children, prolongation = injection), coarse face
@@ -11,6 +12,11 @@
 // Galerkin product for these transfers, scaled), red-black Gauss-Seidel
 // (pre: red,black; post: black,red -- a symmetric V-cycle).  Levels holding
 // at most kTailCells cells run in one single-workgroup kernel.
+//
+// Grid kernels run one cell per thread on 64x4 workgroups whose order is
+// remapped so that each XCD owns a contiguous slab (stencil halos stay in one
+// L2); reductions store one partial per workgroup and a one-workgroup
+// finalize kernel folds them in a fixed order.
 
 #include <hip/hip_runtime.h>
 
@@ -20,9 +26,9 @@
 #include "sp_device.h"
 #include "sp_geom.h"
 #include "sp_internal.h"
+#include "sp_smooth_fused.h"
 
 namespace sp {
-
 namespace {
 
 template <typename T>
@@ -31,6 +37,7 @@ struct LevelView {
   const T *cx, *cy, *cz, *e;
   T* x;
   T* b;
+  const uint8_t* act;  // active mask (level 0 only), else nullptr
 };
 
 template <typename T>
@@ -60,24 +67,26 @@ __device__ inline T offdiag_sum(const LevelView<T>& L, int i, int j, int k, int6
   return s;
 }
 
-// One colour of Gauss-Seidel.  zero_init: x starts at 0 (the other colour
-// is written as 0).
-template <typename T>
+// One colour of Gauss-Seidel; one thread per cell pair along i (the pair
+// holds one cell of each colour).  zero_init: x starts at 0 (the other colour
+// is written as 0).  FINE only distinguishes the finest level in profiles.
+template <typename T, bool FINE>
 __global__ void __launch_bounds__(256) k_gs(LevelView<T> L, int color, int zero_init,
                                             const Scalars* S) {
   if (S->done) return;
-  const int ia = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  if (j >= L.ny) return;
+  const int nxh = (L.nx + 1) >> 1;
+  const Cell3 q = cell_of(nxh, L.ny, L.nz);
+  if (!q.in) return;
+  const int j = q.b, k = q.c;
   const int par = (j + k + color) & 1;
-  const int i = 2 * ia + par;
+  const int i = 2 * q.a + par;
+  const int64_t row = (int64_t)L.nx * (j + (int64_t)L.ny * k);
   if (zero_init) {
-    const int io = 2 * ia + (par ^ 1);
-    if (io < L.nx) L.x[io + (int64_t)L.nx * (j + (int64_t)L.ny * k)] = T(0);
+    const int io = 2 * q.a + (par ^ 1);
+    if (io < L.nx) L.x[row + io] = T(0);
   }
   if (i >= L.nx) return;
-  const int64_t n = i + (int64_t)L.nx * (j + (int64_t)L.ny * k);
+  const int64_t n = row + i;
   const T d = diag_of(L, i, j, k, n);
   if (!(d > T(0))) {
     if (zero_init) L.x[n] = T(0);
@@ -89,14 +98,12 @@ __global__ void __launch_bounds__(256) k_gs(LevelView<T> L, int color, int zero_
 }
 
 // Coarse b = sum over children of the fine residual.
-template <typename T>
+template <typename T, bool FINE>
 __global__ void __launch_bounds__(256) k_resid_restrict(LevelView<T> F, LevelView<T> C,
                                                         const Scalars* S) {
   if (S->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  if (i >= C.nx || j >= C.ny) return;
+  const Cell3 q = cell_of(C.nx, C.ny, C.nz);
+  if (!q.in) return;
   T s = T(0);
 #pragma unroll
   for (int cc = 0; cc < 2; ++cc)
@@ -104,33 +111,31 @@ __global__ void __launch_bounds__(256) k_resid_restrict(LevelView<T> F, LevelVie
     for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
       for (int aa = 0; aa < 2; ++aa) {
-        const int fi = 2 * i + aa, fj = 2 * j + bb, fk = 2 * k + cc;
+        const int fi = 2 * q.a + aa, fj = 2 * q.b + bb, fk = 2 * q.c + cc;
         if (fi >= F.nx || fj >= F.ny || fk >= F.nz) continue;
         const int64_t m = fi + (int64_t)F.nx * (fj + (int64_t)F.ny * fk);
         const T d = diag_of(F, fi, fj, fk, m);
         if (d > T(0)) s += F.b[m] - (d * F.x[m] - offdiag_sum(F, fi, fj, fk, m));
       }
-  C.b[i + (int64_t)C.nx * (j + (int64_t)C.ny * k)] = s;
+  C.b[q.a + (int64_t)C.nx * (q.b + (int64_t)C.ny * q.c)] = s;
 }
 
-template <typename T>
+template <typename T, bool FINE>
 __global__ void __launch_bounds__(256) k_prolong(LevelView<T> F, LevelView<T> C, const Scalars* S) {
   if (S->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  if (i >= F.nx || j >= F.ny) return;
-  const int64_t m = i + (int64_t)F.nx * (j + (int64_t)F.ny * k);
-  if (diag_of(F, i, j, k, m) > T(0))
-    F.x[m] += C.x[(i >> 1) + (int64_t)C.nx * ((j >> 1) + (int64_t)C.ny * (k >> 1))];
+  const Cell3 q = cell_of(F.nx, F.ny, F.nz);
+  if (!q.in) return;
+  const int64_t m = q.a + (int64_t)F.nx * (q.b + (int64_t)F.ny * q.c);
+  const bool act = F.act ? F.act[m] != 0 : diag_of(F, q.a, q.b, q.c, m) > T(0);
+  if (act) F.x[m] += C.x[(q.a >> 1) + (int64_t)C.nx * ((q.b >> 1) + (int64_t)C.ny * (q.c >> 1))];
 }
 
 // ---- single-workgroup V-cycle over the small levels ----
 
 template <typename T>
 __device__ void tail_gs(const LevelView<T>& L, int color, int zero_init) {
-  const int64_t half = ((int64_t)(L.nx + 1) / 2) * L.ny * L.nz;
   const int nxh = (L.nx + 1) / 2;
+  const int64_t half = (int64_t)nxh * L.ny * L.nz;
   for (int64_t t = threadIdx.x; t < half; t += blockDim.x) {
     const int ia = (int)(t % nxh);
     const int j = (int)((t / nxh) % L.ny);
@@ -159,7 +164,8 @@ template <typename T>
 __device__ void tail_project_mean(const LevelView<T>& L, T* v) {
   __shared__ double red[2][16];
   double s = 0.0, c = 0.0;
-  for (int64_t n = threadIdx.x; n < (int64_t)L.nx * L.ny * L.nz; n += blockDim.x) {
+  const int64_t nn = (int64_t)L.nx * L.ny * L.nz;
+  for (int64_t n = threadIdx.x; n < nn; n += blockDim.x) {
     const int i = (int)(n % L.nx), j = (int)((n / L.nx) % L.ny), k = (int)(n / ((int64_t)L.nx * L.ny));
     if (diag_of(L, i, j, k, n) > T(0)) { s += (double)v[n]; c += 1.0; }
   }
@@ -171,7 +177,7 @@ __device__ void tail_project_mean(const LevelView<T>& L, T* v) {
   for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { ts += red[0][w]; tc += red[1][w]; }
   const T mean = tc > 0 ? (T)(ts / tc) : T(0);
   __syncthreads();
-  for (int64_t n = threadIdx.x; n < (int64_t)L.nx * L.ny * L.nz; n += blockDim.x) {
+  for (int64_t n = threadIdx.x; n < nn; n += blockDim.x) {
     const int i = (int)(n % L.nx), j = (int)((n / L.nx) % L.ny), k = (int)(n / ((int64_t)L.nx * L.ny));
     if (diag_of(L, i, j, k, n) > T(0)) v[n] -= mean;
   }
@@ -246,15 +252,13 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A, const Scalars* S) 
 template <typename T>
 __global__ void __launch_bounds__(256) k_coarsen(LevelView<T> F, LevelView<T> C, T* ccx, T* ccy,
                                                  T* ccz, T* ce, T scale) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  if (i >= C.nx || j >= C.ny) return;
+  const Cell3 q = cell_of(C.nx, C.ny, C.nz);
+  if (!q.in) return;
   T sx = 0, sy = 0, sz = 0, se = 0;
   for (int cc = 0; cc < 2; ++cc)
     for (int bb = 0; bb < 2; ++bb)
       for (int aa = 0; aa < 2; ++aa) {
-        const int fi = 2 * i + aa, fj = 2 * j + bb, fk = 2 * k + cc;
+        const int fi = 2 * q.a + aa, fj = 2 * q.b + bb, fk = 2 * q.c + cc;
         if (fi >= F.nx || fj >= F.ny || fk >= F.nz) continue;
         const int64_t m = fi + (int64_t)F.nx * (fj + (int64_t)F.ny * fk);
         if (F.e) se += F.e[m];
@@ -262,7 +266,7 @@ __global__ void __launch_bounds__(256) k_coarsen(LevelView<T> F, LevelView<T> C,
         if (bb == 1) sy += F.cy[m];
         if (cc == 1) sz += F.cz[m];
       }
-  const int64_t n = i + (int64_t)C.nx * (j + (int64_t)C.ny * k);
+  const int64_t n = q.a + (int64_t)C.nx * (q.b + (int64_t)C.ny * q.c);
   ccx[n] = scale * sx;
   ccy[n] = scale * sy;
   ccz[n] = scale * sz;
@@ -272,88 +276,83 @@ __global__ void __launch_bounds__(256) k_coarsen(LevelView<T> F, LevelView<T> C,
 // ---- outer CG (FP64) ----
 
 struct CgArgs {
-  int I, J, K, kper;
+  int I, J, K;
   const double *cx, *cy, *cz, *e;
   const uint8_t* active;
   const float* dw;
   const double *dxp, *dyp, *dzp;
   const double* b;
-  double *x, *r, *q, *pold, *pnew;
+  double *x, *r, *q, *p;
   const double* z64;
   const float* z32;
   float* r32;
-  double* partials;
   Scalars* S;
+  int32_t* hflag;  // pinned host flag (device view)
   double tol;
+  RedSite site;
 };
 
 __device__ inline double zval(const CgArgs& A, int64_t n) {
   return A.z32 ? (double)A.z32[n] : A.z64[n];
 }
 
-// r = b - mean(b) - A x  (x = 0 on the first call)
-__global__ void __launch_bounds__(256) k_cg_residual(CgArgs A, int from_zero) {
-  uint32_t bx, by, bz;
-  xcd_block3(bx, by, bz);
-  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
-  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
+__device__ inline double apply_row(const CgArgs& A, const double* v, int a, int b, int c,
+                                   int64_t n) {
   const int64_t IJ = (int64_t)A.I * A.J;
+  double d = a > 0 ? A.cx[n - 1] : 0.0;
+  d = d + (b > 0 ? A.cy[n - A.I] : 0.0);
+  d = d + (c > 0 ? A.cz[n - IJ] : 0.0);
+  d = d + A.cx[n];
+  d = d + A.cy[n];
+  d = d + A.cz[n];
+  if (A.e) d = d + A.e[n];
+  double s = d * v[n];
+  if (a > 0) s -= A.cx[n - 1] * v[n - 1];
+  if (b > 0) s -= A.cy[n - A.I] * v[n - A.I];
+  if (c > 0) s -= A.cz[n - IJ] * v[n - IJ];
+  if (a < A.I - 1) s -= A.cx[n] * v[n + 1];
+  if (b < A.J - 1) s -= A.cy[n] * v[n + A.I];
+  if (c < A.K - 1) s -= A.cz[n] * v[n + IJ];
+  return s;
+}
+
+// r = b - mean(b) - A x  (x = 0 when from_zero); resets p and q
+__global__ void __launch_bounds__(256) k_cg_residual(CgArgs A, int from_zero) {
+  const Cell3 q = cell_of(A.I, A.J, A.K);
   Scalars* S = A.S;
   const double bmean = S->bmean;
-  double rmax = 0.0, bad = 0.0;
-  if (a < A.I && b < A.J) {
-    for (int c = c0; c < c1; ++c) {
-      const int64_t n = a + (int64_t)A.I * (b + (int64_t)A.J * c);
-      double rv = 0.0;
-      if (A.active[n]) {
-        rv = A.b[n] - bmean;
-        if (!from_zero) {
-          double d = a > 0 ? A.cx[n - 1] : 0.0;
-          d = d + (b > 0 ? A.cy[n - A.I] : 0.0);
-          d = d + (c > 0 ? A.cz[n - IJ] : 0.0);
-          d = d + A.cx[n];
-          d = d + A.cy[n];
-          d = d + A.cz[n];
-          if (A.e) d = d + A.e[n];
-          double ax = d * A.x[n];
-          if (a > 0) ax -= A.cx[n - 1] * A.x[n - 1];
-          if (b > 0) ax -= A.cy[n - A.I] * A.x[n - A.I];
-          if (c > 0) ax -= A.cz[n - IJ] * A.x[n - IJ];
-          if (a < A.I - 1) ax -= A.cx[n] * A.x[n + 1];
-          if (b < A.J - 1) ax -= A.cy[n] * A.x[n + A.I];
-          if (c < A.K - 1) ax -= A.cz[n] * A.x[n + IJ];
-          rv -= ax;
-        }
-        const double v = fabs(rv) / (A.dxp[a] * A.dyp[b] * A.dzp[c]);
-        rmax = v > rmax ? v : rmax;
-        if (!is_finite(rv)) bad = 1.0;
-      }
-      A.r[n] = rv;
-      if (A.r32) A.r32[n] = (float)rv;
-      if (from_zero) A.x[n] = 0.0;
-      A.q[n] = 0.0;
-      A.pold[n] = 0.0;
+  double v[2] = {0.0, 0.0};  // max |r|/V, non-finite
+  if (q.in) {
+    const int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c);
+    double rv = 0.0;
+    if (A.active[n]) {
+      rv = A.b[n] - bmean;
+      if (!from_zero) rv -= apply_row(A, A.x, q.a, q.b, q.c, n);
+      v[0] = fabs(rv) / (A.dxp[q.a] * A.dyp[q.b] * A.dzp[q.c]);
+      if (!is_finite(rv)) v[1] = 1.0;
     }
+    A.r[n] = rv;
+    if (A.r32) A.r32[n] = (float)rv;
+    if (from_zero) A.x[n] = 0.0;
+    A.q[n] = 0.0;
+    A.p[n] = 0.0;
   }
-  const double m = block_max<kTileThreads>(rmax);
-  const double f = block_max<kTileThreads>(bad);
+  block_partials<kTileThreads, 2, 3u>(A.site, v);
+}
+
+__global__ void __launch_bounds__(kFinThreads) k_cg_residual_fin(CgArgs A, uint32_t nb) {
+  double out[2];
+  finalize_partials<kFinThreads, 2, 3u>(A.site, nb, out);
   if (flat_tid() == 0) {
-    store_partial(A.partials, flat_block(), m);
-    store_partial(A.partials + kMaxPartials, flat_block(), f);
-  }
-  if (last_block(&S->counter[kCtrInit])) {
-    const double mm = reduce_partials_max<kTileThreads>(A.partials, num_blocks());
-    const double ff = reduce_partials_max<kTileThreads>(A.partials + kMaxPartials, num_blocks());
-    if (flat_tid() == 0) {
-      S->rmax = mm;
-      S->nonfinite = S->nonfinite || ff > 0.0 || !is_finite(mm);
-      S->done = (mm <= A.tol || S->nonfinite) ? 1 : 0;
-      S->first = 1;
-      S->alpha = 0.0;
-      S->beta = 0.0;
-      S->rz = 0.0;
-    }
-    reset_counter(&S->counter[kCtrInit]);
+    Scalars* S = A.S;
+    S->rmax = out[0];
+    S->nonfinite = S->nonfinite || out[1] > 0.0 || !is_finite(out[0]);
+    S->done = (out[0] <= A.tol || S->nonfinite) ? 1 : 0;
+    S->first = 1;
+    S->alpha = 0.0;
+    S->beta = 0.0;
+    S->rz = 0.0;
+    *A.hflag = S->done;
   }
 }
 
@@ -364,106 +363,77 @@ __global__ void __launch_bounds__(256) k_cg_residual(CgArgs A, int from_zero) {
 __global__ void __launch_bounds__(256) k_cg_dots(CgArgs A) {
   Scalars* S = A.S;
   if (S->done) return;
-  uint32_t bx, by, bz;
-  xcd_block3(bx, by, bz);
-  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
-  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
-  double rz = 0.0, zq = 0.0, dz = 0.0, rs = 0.0, qs = 0.0;
-  if (a < A.I && b < A.J) {
-    for (int c = c0; c < c1; ++c) {
-      const int64_t n = a + (int64_t)A.I * (b + (int64_t)A.J * c);
-      if (A.active[n]) {
-        const double z = zval(A, n), r = A.r[n], q = A.q[n];
-        rz += r * z;
-        zq += z * q;
-        dz += (double)A.dw[n] * z;
-        rs += r;
-        qs += q;
-      }
+  const Cell3 q = cell_of(A.I, A.J, A.K);
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (q.in) {
+    const int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c);
+    if (A.active[n]) {
+      const double z = zval(A, n), r = A.r[n], qq = A.q[n];
+      v[0] = r * z;
+      v[1] = z * qq;
+      v[2] = (double)A.dw[n] * z;
+      v[3] = r;
+      v[4] = qq;
     }
   }
-  const double s0 = block_sum<kTileThreads>(rz);
-  const double s1 = block_sum<kTileThreads>(zq);
-  const double s2 = block_sum<kTileThreads>(dz);
-  const double s3 = block_sum<kTileThreads>(rs);
-  const double s4 = block_sum<kTileThreads>(qs);
+  block_partials<kTileThreads, 5, 0u>(A.site, v);
+}
+
+__global__ void __launch_bounds__(kFinThreads) k_cg_dots_fin(CgArgs A, uint32_t nb) {
+  Scalars* S = A.S;
+  if (S->done) return;
+  double t[5];
+  finalize_partials<kFinThreads, 5, 0u>(A.site, nb, t);
   if (flat_tid() == 0) {
-    const uint32_t fb = flat_block();
-    store_partial(A.partials, fb, s0);
-    store_partial(A.partials + kMaxPartials, fb, s1);
-    store_partial(A.partials + 2 * (int64_t)kMaxPartials, fb, s2);
-    store_partial(A.partials + 3 * (int64_t)kMaxPartials, fb, s3);
-    store_partial(A.partials + 4 * (int64_t)kMaxPartials, fb, s4);
-  }
-  if (last_block(&S->counter[kCtrDots])) {
-    const uint32_t nb = num_blocks();
-    const double t0 = reduce_partials_sum<kTileThreads>(A.partials, nb);
-    const double t1 = reduce_partials_sum<kTileThreads>(A.partials + kMaxPartials, nb);
-    const double t2 = reduce_partials_sum<kTileThreads>(A.partials + 2 * (int64_t)kMaxPartials, nb);
-    const double t3 = reduce_partials_sum<kTileThreads>(A.partials + 3 * (int64_t)kMaxPartials, nb);
-    const double t4 = reduce_partials_sum<kTileThreads>(A.partials + 4 * (int64_t)kMaxPartials, nb);
-    if (flat_tid() == 0) {
-      const double m = (S->singular && S->dsum > 0.0) ? t2 / S->dsum : 0.0;
-      const double rzm = t0 - m * t3, zqm = t1 - m * t4;
-      S->zshift = m;
-      S->beta = S->first ? 0.0 : -S->alpha * zqm / S->rz;
-      S->rz = rzm;
-      S->zq = zqm;
-      S->first = 0;
-      if (!is_finite(rzm) || !is_finite(S->beta)) { S->nonfinite = 1; S->done = 1; }
-    }
-    reset_counter(&S->counter[kCtrDots]);
+    const double m = (S->singular && S->dsum > 0.0) ? t[2] / S->dsum : 0.0;
+    const double rzm = t[0] - m * t[3], zqm = t[1] - m * t[4];
+    S->zshift = m;
+    S->beta = S->first ? 0.0 : -S->alpha * zqm / S->rz;
+    S->rz = rzm;
+    S->zq = zqm;
+    S->first = 0;
+    if (!is_finite(rzm) || !is_finite(S->beta)) { S->nonfinite = 1; S->done = 1; }
   }
 }
 
-// p_new = z + beta p_old ; q = A p_new ; pq = (p_new, q) ; alpha = rz / pq
+// p = (z - m) + beta p on active cells
+__global__ void __launch_bounds__(256) k_cg_pupdate(CgArgs A) {
+  const Scalars* S = A.S;
+  if (S->done) return;
+  const Cell3 q = cell_of(A.I, A.J, A.K);
+  if (!q.in) return;
+  const int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c);
+  const double beta = S->beta, m = S->zshift;
+  A.p[n] = A.active[n] ? (zval(A, n) - m) + beta * A.p[n] : 0.0;
+}
+
+// q = A p ; pq = (p, q) ; alpha = rz / pq
 __global__ void __launch_bounds__(256) k_cg_spmv(CgArgs A) {
   Scalars* S = A.S;
   if (S->done) return;
-  uint32_t bx, by, bz;
-  xcd_block3(bx, by, bz);
-  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
-  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
-  const int64_t IJ = (int64_t)A.I * A.J;
-  const double beta = S->beta;
-  const double m = S->zshift;
-  double pq = 0.0;
-  if (a < A.I && b < A.J) {
-    for (int c = c0; c < c1; ++c) {
-      const int64_t n = a + (int64_t)A.I * (b + (int64_t)A.J * c);
-      const double pc = (zval(A, n) - m) + beta * A.pold[n];
-      double qv = 0.0;
-      if (A.active[n]) {
-        double d = a > 0 ? A.cx[n - 1] : 0.0;
-        d = d + (b > 0 ? A.cy[n - A.I] : 0.0);
-        d = d + (c > 0 ? A.cz[n - IJ] : 0.0);
-        d = d + A.cx[n];
-        d = d + A.cy[n];
-        d = d + A.cz[n];
-        if (A.e) d = d + A.e[n];
-        qv = d * pc;
-        if (a > 0) qv -= A.cx[n - 1] * ((zval(A, n - 1) - m) + beta * A.pold[n - 1]);
-        if (b > 0) qv -= A.cy[n - A.I] * ((zval(A, n - A.I) - m) + beta * A.pold[n - A.I]);
-        if (c > 0) qv -= A.cz[n - IJ] * ((zval(A, n - IJ) - m) + beta * A.pold[n - IJ]);
-        if (a < A.I - 1) qv -= A.cx[n] * ((zval(A, n + 1) - m) + beta * A.pold[n + 1]);
-        if (b < A.J - 1) qv -= A.cy[n] * ((zval(A, n + A.I) - m) + beta * A.pold[n + A.I]);
-        if (c < A.K - 1) qv -= A.cz[n] * ((zval(A, n + IJ) - m) + beta * A.pold[n + IJ]);
-        pq += pc * qv;
-      }
-      A.pnew[n] = A.active[n] ? pc : 0.0;
-      A.q[n] = qv;
+  const Cell3 q = cell_of(A.I, A.J, A.K);
+  double v[1] = {0.0};
+  if (q.in) {
+    const int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c);
+    double qv = 0.0;
+    if (A.active[n]) {
+      qv = apply_row(A, A.p, q.a, q.b, q.c, n);
+      v[0] = A.p[n] * qv;
     }
+    A.q[n] = qv;
   }
-  const double s0 = block_sum<kTileThreads>(pq);
-  if (flat_tid() == 0) store_partial(A.partials, flat_block(), s0);
-  if (last_block(&S->counter[kCtrSpmv])) {
-    const double t0 = reduce_partials_sum<kTileThreads>(A.partials, num_blocks());
-    if (flat_tid() == 0) {
-      S->pq = t0;
-      S->alpha = S->rz / t0;
-      if (!is_finite(S->alpha)) { S->nonfinite = 1; S->done = 1; }
-    }
-    reset_counter(&S->counter[kCtrSpmv]);
+  block_partials<kTileThreads, 1, 0u>(A.site, v);
+}
+
+__global__ void __launch_bounds__(kFinThreads) k_cg_spmv_fin(CgArgs A, uint32_t nb) {
+  Scalars* S = A.S;
+  if (S->done) return;
+  double t[1];
+  finalize_partials<kFinThreads, 1, 0u>(A.site, nb, t);
+  if (flat_tid() == 0) {
+    S->pq = t[0];
+    S->alpha = S->rz / t[0];
+    if (!is_finite(S->alpha)) { S->nonfinite = 1; S->done = 1; }
   }
 }
 
@@ -471,41 +441,37 @@ __global__ void __launch_bounds__(256) k_cg_spmv(CgArgs A) {
 __global__ void __launch_bounds__(256) k_cg_update(CgArgs A) {
   Scalars* S = A.S;
   if (S->done) return;
-  uint32_t bx, by, bz;
-  xcd_block3(bx, by, bz);
-  const int a = bx * kTX + threadIdx.x, b = by * kTY + threadIdx.y;
-  const int c0 = bz * A.kper, c1 = min(c0 + A.kper, A.K);
+  const Cell3 q = cell_of(A.I, A.J, A.K);
   const double alpha = S->alpha;
-  double rmax = 0.0;
-  if (a < A.I && b < A.J) {
-    const double dxy = A.dxp[a] * A.dyp[b];
-    for (int c = c0; c < c1; ++c) {
-      const int64_t n = a + (int64_t)A.I * (b + (int64_t)A.J * c);
-      A.x[n] += alpha * A.pnew[n];
-      const double rv = A.r[n] - alpha * A.q[n];
-      A.r[n] = rv;
-      if (A.r32) A.r32[n] = (float)rv;
-      const double v = fabs(rv) / (dxy * A.dzp[c]);
-      rmax = v > rmax ? v : rmax;
-    }
+  double v[1] = {0.0};
+  if (q.in) {
+    const int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c);
+    A.x[n] += alpha * A.p[n];
+    const double rv = A.r[n] - alpha * A.q[n];
+    A.r[n] = rv;
+    if (A.r32) A.r32[n] = (float)rv;
+    v[0] = fabs(rv) / (A.dxp[q.a] * A.dyp[q.b] * A.dzp[q.c]);
   }
-  const double m = block_max<kTileThreads>(rmax);
-  if (flat_tid() == 0) store_partial(A.partials, flat_block(), m);
-  if (last_block(&S->counter[kCtrUpdate])) {
-    const double mm = reduce_partials_max<kTileThreads>(A.partials, num_blocks());
-    if (flat_tid() == 0) {
-      S->rmax = mm;
-      S->iter += 1;
-      if (!is_finite(mm)) S->nonfinite = 1;
-      S->done = (mm <= A.tol || S->nonfinite) ? 1 : 0;
-    }
-    reset_counter(&S->counter[kCtrUpdate]);
+  block_partials<kTileThreads, 1, 1u>(A.site, v);
+}
+
+__global__ void __launch_bounds__(kFinThreads) k_cg_update_fin(CgArgs A, uint32_t nb) {
+  Scalars* S = A.S;
+  if (S->done) return;
+  double t[1];
+  finalize_partials<kFinThreads, 1, 1u>(A.site, nb, t);
+  if (flat_tid() == 0) {
+    S->rmax = t[0];
+    S->iter += 1;
+    if (!is_finite(t[0])) S->nonfinite = 1;
+    S->done = (t[0] <= A.tol || S->nonfinite) ? 1 : 0;
+    *A.hflag = S->done;
   }
 }
 
 template <typename T>
-LevelView<T> view_of(const Level<T>& L) {
-  return LevelView<T>{L.nx, L.ny, L.nz, L.cx, L.cy, L.cz, L.e, L.x, L.b};
+LevelView<T> view_of(const Level<T>& L, const uint8_t* act = nullptr) {
+  return LevelView<T>{L.nx, L.ny, L.nz, L.cx, L.cy, L.cz, L.e, L.x, L.b, act};
 }
 
 template <typename T>
@@ -514,16 +480,67 @@ double level_bytes(const Level<T>& L, double per_cell_words) {
 }
 
 template <typename T>
-void launch_gs(sp_ctx* c, const Level<T>& L, int color, int zero_init) {
+void launch_gs(sp_ctx* c, const Level<T>& L, int color, int zero_init, bool fine) {
   const int nxh = (L.nx + 1) / 2;
-  dim3 blk(64, 4, 1), grd((nxh + 63) / 64, (L.ny + 3) / 4, L.nz);
+  dim3 blk(64, 4, 1), grd = flat_grid(nxh, L.ny, L.nz);
   int slot;
-  // half the cells updated; every array streamed once
-  const double words = (zero_init ? 1.0 : 2.0) + 3.0 + (L.e ? 1.0 : 0.0) + 0.5;
-  timer_begin(c, L.n == c->dm.N ? (zero_init ? "mg_gs0_fine" : "mg_gs_fine") : "mg_gs_coarse",
+  // algorithmic words per cell: b for one colour, the three face arrays,
+  // x read (all) and written (one colour); a zero-init sweep writes all of x
+  const double words = (zero_init ? 0.5 + 3.0 + 1.0 : 1.0 + 0.5 + 3.0 + 0.5) + (L.e ? 1.0 : 0.0);
+  timer_begin(c, fine ? (zero_init ? "mg_gs0_fine" : "mg_gs_fine") : "mg_gs_coarse",
               level_bytes(L, words), &slot);
-  hipLaunchKernelGGL(k_gs<T>, grd, blk, 0, c->stream, view_of(L), color, zero_init,
-                     (const Scalars*)c->S);
+  if (fine)
+    hipLaunchKernelGGL((k_gs<T, true>), grd, blk, 0, c->stream, view_of(L), color, zero_init,
+                       (const Scalars*)c->S);
+  else
+    hipLaunchKernelGGL((k_gs<T, false>), grd, blk, 0, c->stream, view_of(L), color, zero_init,
+                       (const Scalars*)c->S);
+  timer_end(c, slot);
+}
+
+template <typename T, int NU>
+void launch_fused(sp_ctx* c, Level<T>* lv, int l, int mode) {
+  using Sh = FusedShape<NU>;
+  const Level<T>& L = lv[l];
+  const Level<T>& C = lv[l + 1];
+  FusedArgs<T> A{};
+  A.nx = L.nx; A.ny = L.ny; A.nz = L.nz;
+  A.cx = L.cx; A.cy = L.cy; A.cz = L.cz; A.e = L.e;
+  A.b = L.b;
+  A.cnx = C.nx; A.cny = C.ny;
+  if (mode == 0) { A.xout = L.xs; A.bc = C.b; }
+  else { A.xin = L.xs; A.xc = C.x; A.xout = L.x; }
+  const int gx = (L.nx + Sh::TX - 1) / Sh::TX, gy = (L.ny + Sh::TY - 1) / Sh::TY;
+  // k chunk: even (coarse cells never straddle chunks), about 512 workgroups
+  int64_t kc = ((int64_t)L.nz * gx * gy + 511) / 512;
+  kc = std::max<int64_t>(kc, 16);
+  kc = (kc + 1) & ~int64_t(1);
+  A.kchunk = (int)std::min<int64_t>(kc, (L.nz + 1) & ~1);
+  const dim3 grd(gx, gy, (L.nz + A.kchunk - 1) / A.kchunk), blk(kFEX, kFEY, 1);
+  const bool fine = l == 0;
+  int slot;
+  // algorithmic words per cell: pre b + 3 faces -> x (+ coarse b);
+  // post x_pre + 3 faces + b -> x (+ coarse x)
+  const double words = mode == 0 ? 5.0 + 0.125 : 6.0 + 0.125;
+  timer_begin(c, fine ? (mode == 0 ? "mg_pre_fine" : "mg_post_fine")
+                      : (mode == 0 ? "mg_pre_coarse" : "mg_post_coarse"),
+              level_bytes(L, words + (L.e ? 1.0 : 0.0)), &slot);
+  const Scalars* S = c->S;
+#define SP_FUSED_LAUNCH(MODE_, FINE_)                                                         \
+  do {                                                                                        \
+    if (A.e)                                                                                  \
+      hipLaunchKernelGGL((k_smooth_fused<T, NU, MODE_, FINE_, true>), grd, blk, 0, c->stream, A, S); \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_smooth_fused<T, NU, MODE_, FINE_, false>), grd, blk, 0, c->stream, A, S); \
+  } while (0)
+  if (mode == 0) {
+    if (fine) SP_FUSED_LAUNCH(0, true);
+    else SP_FUSED_LAUNCH(0, false);
+  } else {
+    if (fine) SP_FUSED_LAUNCH(1, true);
+    else SP_FUSED_LAUNCH(1, false);
+  }
+#undef SP_FUSED_LAUNCH
   timer_end(c, slot);
 }
 
@@ -531,17 +548,28 @@ template <typename T>
 void vcycle(sp_ctx* c, Level<T>* lv) {
   const int L = c->levels, Lt = c->tail_level;
   const int pre = std::max(1, c->prm.mg_pre), post = c->prm.mg_post;
+  const bool fused = pre == post && (pre == 1 || pre == 2) && !(c->prm.flags & 1);
   dim3 blk(64, 4, 1);
   int slot;
   for (int l = 0; l < Lt; ++l) {
-    launch_gs(c, lv[l], 0, 1);
-    launch_gs(c, lv[l], 1, 0);
-    for (int s = 1; s < pre; ++s) { launch_gs(c, lv[l], 0, 0); launch_gs(c, lv[l], 1, 0); }
-    dim3 grd((lv[l + 1].nx + 63) / 64, (lv[l + 1].ny + 3) / 4, lv[l + 1].nz);
-    timer_begin(c, l == 0 ? "mg_restrict_fine" : "mg_restrict_coarse",
+    const bool fine = l == 0;
+    if (fused) {
+      if (pre == 2) launch_fused<T, 2>(c, lv, l, 0);
+      else launch_fused<T, 1>(c, lv, l, 0);
+      continue;
+    }
+    launch_gs(c, lv[l], 0, 1, fine);
+    launch_gs(c, lv[l], 1, 0, fine);
+    for (int s = 1; s < pre; ++s) { launch_gs(c, lv[l], 0, 0, fine); launch_gs(c, lv[l], 1, 0, fine); }
+    dim3 grd = flat_grid(lv[l + 1].nx, lv[l + 1].ny, lv[l + 1].nz);
+    timer_begin(c, fine ? "mg_restrict_fine" : "mg_restrict_coarse",
                 level_bytes(lv[l], 5.0 + (lv[l].e ? 1.0 : 0.0)) + level_bytes(lv[l + 1], 1.0), &slot);
-    hipLaunchKernelGGL(k_resid_restrict<T>, grd, blk, 0, c->stream, view_of(lv[l]),
-                       view_of(lv[l + 1]), (const Scalars*)c->S);
+    if (fine)
+      hipLaunchKernelGGL((k_resid_restrict<T, true>), grd, blk, 0, c->stream, view_of(lv[l]),
+                         view_of(lv[l + 1]), (const Scalars*)c->S);
+    else
+      hipLaunchKernelGGL((k_resid_restrict<T, false>), grd, blk, 0, c->stream, view_of(lv[l]),
+                         view_of(lv[l + 1]), (const Scalars*)c->S);
     timer_end(c, slot);
   }
   {
@@ -558,13 +586,26 @@ void vcycle(sp_ctx* c, Level<T>* lv) {
     timer_end(c, slot);
   }
   for (int l = Lt - 1; l >= 0; --l) {
-    dim3 grd((lv[l].nx + 63) / 64, (lv[l].ny + 3) / 4, lv[l].nz);
-    timer_begin(c, l == 0 ? "mg_prolong_fine" : "mg_prolong_coarse",
-                level_bytes(lv[l], 5.0 + (lv[l].e ? 1.0 : 0.0)) + level_bytes(lv[l + 1], 1.0), &slot);
-    hipLaunchKernelGGL(k_prolong<T>, grd, blk, 0, c->stream, view_of(lv[l]), view_of(lv[l + 1]),
-                       (const Scalars*)c->S);
+    const bool fine = l == 0;
+    if (fused) {
+      if (pre == 2) launch_fused<T, 2>(c, lv, l, 1);
+      else launch_fused<T, 1>(c, lv, l, 1);
+      continue;
+    }
+    dim3 grd = flat_grid(lv[l].nx, lv[l].ny, lv[l].nz);
+    const uint8_t* act = fine ? c->active : nullptr;
+    timer_begin(c, fine ? "mg_prolong_fine" : "mg_prolong_coarse",
+                fine ? level_bytes(lv[l], 2.0) + (double)lv[l].n + level_bytes(lv[l + 1], 1.0)
+                     : level_bytes(lv[l], 5.0 + (lv[l].e ? 1.0 : 0.0)) + level_bytes(lv[l + 1], 1.0),
+                &slot);
+    if (fine)
+      hipLaunchKernelGGL((k_prolong<T, true>), grd, blk, 0, c->stream, view_of(lv[l], act),
+                         view_of(lv[l + 1]), (const Scalars*)c->S);
+    else
+      hipLaunchKernelGGL((k_prolong<T, false>), grd, blk, 0, c->stream, view_of(lv[l], act),
+                         view_of(lv[l + 1]), (const Scalars*)c->S);
     timer_end(c, slot);
-    for (int s = 0; s < post; ++s) { launch_gs(c, lv[l], 1, 0); launch_gs(c, lv[l], 0, 0); }
+    for (int s = 0; s < post; ++s) { launch_gs(c, lv[l], 1, 0, fine); launch_gs(c, lv[l], 0, 0, fine); }
   }
 }
 
@@ -575,9 +616,9 @@ sp_status setup_levels(sp_ctx* c, Level<T>* lv) {
   for (int l = 0; l < L; ++l) lv[l].e = c->has_e ? lv[l].e_store : nullptr;
   for (int l = 1; l < L; ++l) {
     Level<T>& C = lv[l];
-    dim3 grd((C.nx + 63) / 64, (C.ny + 3) / 4, C.nz);
-    hipLaunchKernelGGL(k_coarsen<T>, grd, blk, 0, c->stream, view_of(lv[l - 1]), view_of(C), C.cx,
-                       C.cy, C.cz, C.e, (T)c->prm.mg_coarse_scale);
+    hipLaunchKernelGGL(k_coarsen<T>, flat_grid(C.nx, C.ny, C.nz), blk, 0, c->stream,
+                       view_of(lv[l - 1]), view_of(C), C.cx, C.cy, C.cz, C.e,
+                       (T)c->prm.mg_coarse_scale);
   }
   // level table for the tail kernel
   std::vector<LevelView<T>> desc(kMaxLevels);
@@ -585,6 +626,9 @@ sp_status setup_levels(sp_ctx* c, Level<T>* lv) {
   hipError_t e = hipMemcpyAsync(c->mg_desc, desc.data(), sizeof(LevelView<T>) * kMaxLevels,
                                 hipMemcpyHostToDevice, c->stream);
   if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(mg_desc)");
+  // the host table must stay valid until the copy has been consumed
+  e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "mg setup");
   return SP_OK;
 }
 
@@ -603,18 +647,19 @@ sp_status solve(sp_ctx* c, const double* rhs, double* /*xout*/, bool /*forward*/
   const Dims& D = c->dm;
   const bool mixed = c->prm.precision == SP_PREC_MIXED;
   const bool mg = c->prm.precond == SP_PRECOND_MG;
-  Geom g = make_geom(D.I, D.J, D.K, 2048);
-  if (g.blocks() > (uint32_t)kMaxPartials) return fail(c, SP_ERR_INVALID_ARG, "grid too large");
+  const dim3 grd = flat_grid(D.I, D.J, D.K), blk(64, 4, 1);
+  if (flat_blocks(D.I, D.J, D.K) > c->red_max_blocks)
+    return fail(c, SP_ERR_INVALID_ARG, "grid too large for the reduction buffers");
   CgArgs A{};
-  A.I = D.I; A.J = D.J; A.K = D.K; A.kper = g.kper;
+  A.I = D.I; A.J = D.J; A.K = D.K;
   A.cx = c->cx; A.cy = c->cy; A.cz = c->cz; A.e = c->has_e ? c->ee : nullptr;
   A.active = c->active;
   A.dw = c->dw;
   A.dxp = c->dxp; A.dyp = c->dyp; A.dzp = c->dzp;
   A.b = rhs;
-  A.x = c->x; A.r = c->r; A.q = c->q;
-  A.partials = c->partials;
+  A.x = c->x; A.r = c->r; A.q = c->q; A.p = c->p0;
   A.S = c->S;
+  A.hflag = c->hflags_dev;
   A.tol = c->prm.div_tol;
   if (mixed && mg) {
     A.z32 = c->z32;
@@ -622,100 +667,95 @@ sp_status solve(sp_ctx* c, const double* rhs, double* /*xout*/, bool /*forward*/
   } else {
     A.z64 = c->z64;
   }
-  double* pbuf[2] = {c->p0, c->p1};
   int slot;
   const double NB = (double)D.N;
   const double eb = c->has_e ? 8.0 : 0.0;
   const double zb = (mixed && mg) ? 4.0 : 8.0;
 
-  auto enqueue_residual = [&](int from_zero) {
-    A.pold = pbuf[0];
-    A.pnew = pbuf[1];
-    timer_begin(c, "cg_residual", NB * (from_zero ? 8 + 1 + 8 * 4 : 8 * 5 + eb + 1 + 8 * 4), &slot);
-    hipLaunchKernelGGL(k_cg_residual, g.grid, g.block, 0, c->stream, A, from_zero);
+  const uint32_t nblk = flat_blocks(D.I, D.J, D.K);
+  auto launch = [&](auto kern, int site, const char* name, double bytes, auto... extra) {
+    A.site = red_site(c, site);
+    timer_begin(c, name, bytes, &slot);
+    hipLaunchKernelGGL(kern, grd, blk, 0, c->stream, A, extra...);
     timer_end(c, slot);
+  };
+  auto finalize = [&](auto kern, int site) {
+    A.site = red_site(c, site);
+    timer_begin(c, "cg_finalize", 0, &slot);
+    hipLaunchKernelGGL(kern, dim3(1), dim3(kFinThreads), 0, c->stream, A, nblk);
+    timer_end(c, slot);
+  };
+  auto enqueue_residual = [&](int from_zero) {
+    launch(k_cg_residual, kSiteResidual, "cg_residual",
+           NB * (from_zero ? 8 + 1 + 8 * 4 : 8 * 5 + eb + 1 + 8 * 4), from_zero);
+    finalize(k_cg_residual_fin, kSiteResidual);
   };
   auto precondition = [&]() {
     if (mg) {
       if (mixed) vcycle<float>(c, c->lf);
       else vcycle<double>(c, c->ld);
     } else {
-      // Jacobi: z = r / d through a zero-init smoothing step of level 0
+      // Jacobi: z = r / d through two zero-init colour sweeps of level 0
       Level<double> L0{};
       L0.nx = D.I; L0.ny = D.J; L0.nz = D.K; L0.n = D.N;
       L0.cx = c->cx; L0.cy = c->cy; L0.cz = c->cz; L0.e = c->has_e ? c->ee : nullptr;
       L0.x = c->z64; L0.b = c->r;
-      LevelView<double> V = view_of(L0);
-      const int nxh = (D.I + 1) / 2;
-      dim3 blk(64, 4, 1), grd((nxh + 63) / 64, (D.J + 3) / 4, D.K);
-      timer_begin(c, "jacobi", NB * 8 * 6, &slot);
-      hipLaunchKernelGGL(k_gs<double>, grd, blk, 0, c->stream, V, 0, 1, (const Scalars*)c->S);
-      hipLaunchKernelGGL(k_gs<double>, grd, blk, 0, c->stream, V, 1, 1, (const Scalars*)c->S);
-      timer_end(c, slot);
+      launch_gs(c, L0, 0, 1, true);
+      launch_gs(c, L0, 1, 1, true);
     }
   };
 
   hipError_t e;
-  int32_t* hflag = c->hflags;
-  int pp = 0;  // parity of the p double buffer since the last (re)start
-  enqueue_residual(1);
-  int it = 0, cur = 0;
-  e = hipMemcpyAsync(&hflag[cur], &c->S->done, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
-  if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync");
+  volatile int32_t* hflag = c->hflags;
   hipEvent_t evs[2];
   hipEventCreateWithFlags(&evs[0], hipEventDisableTiming);
   hipEventCreateWithFlags(&evs[1], hipEventDisableTiming);
-  hipEventRecord(evs[cur], c->stream);
+  enqueue_residual(1);
+  int it = 0;
   int restarts = 0;
   const int max_iter = std::max(1, c->prm.max_iter);
   bool finished = false;
   while (!finished) {
+    // one iteration in flight: enqueue iteration `it`, then wait for the
+    // flag written at the end of iteration `it - 1`
+    bool prev_pending = false;
+    int cur = 0;
     while (it < max_iter) {
       precondition();
-      timer_begin(c, "cg_dots", NB * (8 + zb + 8 + 4 + 1), &slot);
-      hipLaunchKernelGGL(k_cg_dots, g.grid, g.block, 0, c->stream, A);
-      timer_end(c, slot);
-      A.pold = pbuf[pp & 1];
-      A.pnew = pbuf[(pp + 1) & 1];
-      ++pp;
-      timer_begin(c, "cg_spmv", NB * (zb + 8 + 24 + eb + 1 + 8 + 8), &slot);
-      hipLaunchKernelGGL(k_cg_spmv, g.grid, g.block, 0, c->stream, A);
-      timer_end(c, slot);
-      timer_begin(c, "cg_update", NB * (8 * 4 + 8 * 2 + (A.r32 ? 4 : 0)), &slot);
-      hipLaunchKernelGGL(k_cg_update, g.grid, g.block, 0, c->stream, A);
-      timer_end(c, slot);
-      const int nxt = cur ^ 1;
-      hipMemcpyAsync(&hflag[nxt], &c->S->done, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
-      hipEventRecord(evs[nxt], c->stream);
-      // wait for the flag of the previous iteration (one iteration in flight)
-      e = hipEventSynchronize(evs[cur]);
-      if (e != hipSuccess) { hipEventDestroy(evs[0]); hipEventDestroy(evs[1]); return hip_check(c, e, "solve"); }
-      cur = nxt;
+      launch(k_cg_dots, kSiteDots, "cg_dots", NB * (8 + zb + 8 + 4 + 1));
+      finalize(k_cg_dots_fin, kSiteDots);
+      launch(k_cg_pupdate, kSiteDots, "cg_pupdate", NB * (zb + 8 + 8 + 1));
+      launch(k_cg_spmv, kSiteSpmv, "cg_spmv", NB * (8 + 24 + eb + 1 + 8));
+      finalize(k_cg_spmv_fin, kSiteSpmv);
+      launch(k_cg_update, kSiteUpdate, "cg_update", NB * (8 * 4 + 8 * 2 + (A.r32 ? 4 : 0)));
+      finalize(k_cg_update_fin, kSiteUpdate);
+      hipEventRecord(evs[cur], c->stream);
       ++it;
-      if (hflag[cur ^ 1]) break;
+      if (prev_pending) {
+        e = hipEventSynchronize(evs[cur ^ 1]);
+        if (e != hipSuccess) { hipEventDestroy(evs[0]); hipEventDestroy(evs[1]); return hip_check(c, e, "solve"); }
+        if (*hflag) break;
+      }
+      prev_pending = true;
+      cur ^= 1;
     }
     e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) { hipEventDestroy(evs[0]); hipEventDestroy(evs[1]); return hip_check(c, e, "solve"); }
-    // true residual of the final iterate
-    e = hipMemcpyAsync(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream);
-    hipStreamSynchronize(c->stream);
+    e = hipMemcpy(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { hipEventDestroy(evs[0]); hipEventDestroy(evs[1]); return hip_check(c, e, "solve"); }
     const int iters = c->S_host->iter;
     if (c->S_host->nonfinite) break;
+    // true residual of the final iterate
     hipMemsetAsync(&c->S->done, 0, sizeof(int32_t), c->stream);
-    pp = 0;
     enqueue_residual(0);
-    e = hipMemcpyAsync(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream);
-    hipStreamSynchronize(c->stream);
-    c->S_host->iter = iters;
+    e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { hipEventDestroy(evs[0]); hipEventDestroy(evs[1]); return hip_check(c, e, "solve"); }
     if (c->S_host->done || iters >= max_iter || restarts >= 3) {
       finished = true;
     } else {
       ++restarts;  // recursive residual drifted: restart from the true residual
       it = iters;
-      hflag[2] = iters;
-      hipMemcpyAsync(&c->S->iter, &hflag[2], sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
-      hipMemcpyAsync(&hflag[cur], &c->S->done, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
-      hipEventRecord(evs[cur], c->stream);
     }
   }
   hipEventDestroy(evs[0]);

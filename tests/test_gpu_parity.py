@@ -55,3 +55,21 @@ def test_projection_matches_oracle(gpu_available, oracle_results, cfg, prec):
         mask = np.ones(hs, bool)
         mask[1:-1, 1:-1, 1:-1] = False
         assert np.all(a[mask] == sentinel), n
+
+
+@pytest.mark.parametrize("cfg,prec", [(0, 1), (1, 0), (2, 1), (4, 0)])
+def test_fused_smoother_matches_plain_sweeps(gpu_available, cfg, prec):
+    """The temporally blocked smoother computes the same V-cycle as one
+    kernel per half-sweep (SP_FLAG_UNFUSED_SMOOTHER)."""
+    c = H.case(cfg)
+    print(f"[fused] cfg={cfg} prec={prec} shape={c.shape}", flush=True)
+    res = []
+    for flags in (0, 1):
+        P = H.projector(c, precision=prec, flags=flags)
+        out, st = P.project(H.fields(c), c.bc_flag, c.dt)
+        P.close()
+        assert st.converged == 1
+        res.append((out, st))
+    (a, sa), (b, sb) = res
+    assert abs(sa.iterations - sb.iterations) <= 1
+    assert H.rel_inf(a["p"], b["p"]) <= 1e-9
