@@ -1,0 +1,267 @@
+"""Python mirror of SIMCOFlow's projection interface on the HIP engine.
+
+* :func:`poisson_equation_solver` follows ``PoissonEquationSolver``
+  (ProjectionP.f90:33-121) followed by the correction block of
+  ``UpdatePUVW`` (ComputePUVW.f90:179-232): same arguments (grids, cells,
+  old solid vof, variables, predictor, d-coefficients, pressure BC,
+  PoCoef, projection, dt), same outputs written in place (PoCoef, Proj%Pp,
+  TVar%u/v/w/p).
+* :class:`DeviceProjection` runs the projection on torch tensors resident on
+  the GPU and keeps the forward pressure for the adjoint.
+* :class:`ProjectionFunction` is the ``torch.autograd.Function`` whose
+  backward is the engine's hand-written adjoint: gradients with respect to
+  u*, v*, w* and to the face coefficients beta = 1/rho' (E, N, T faces).
+
+Array layouts are the host's Fortran arrays.  On the torch side a Fortran
+(I+2, J+2, K+2) halo array is a C-contiguous tensor of shape (K+2, J+2, I+2)
+(same bytes); cell arrays are (K, J, I).
+"""
+
+import dataclasses
+from typing import Optional
+
+import numpy as np
+
+from . import engine as _eng
+
+# ---------------------------------------------------------------------------
+# Fortran-shaped containers (only the members the projection reads or writes)
+# ---------------------------------------------------------------------------
+
+
+@dataclasses.dataclass
+class Grid:  # Mesh.f90:14-18 (tensor product: 1-D spacings)
+    dx: np.ndarray
+    dy: np.ndarray
+    dz: np.ndarray
+
+
+@dataclasses.dataclass
+class Cell:  # CutCell.f90:13-39
+    Cell_Type: np.ndarray
+    vof: Optional[np.ndarray] = None
+    vofL: Optional[np.ndarray] = None
+    phiL: Optional[np.ndarray] = None
+    EEArea: Optional[np.ndarray] = None
+    NEArea: Optional[np.ndarray] = None
+    TEArea: Optional[np.ndarray] = None
+    MoExCell: Optional[np.ndarray] = None
+
+
+@dataclasses.dataclass
+class Variables:  # StateVariables.f90:21-26
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    p: np.ndarray
+
+
+@dataclasses.dataclass
+class Predictor:  # PredictUVW.f90:36-38
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+
+
+@dataclasses.dataclass
+class PoissonCoefficient:  # PredictUVW.f90:39-41
+    Dp: np.ndarray
+
+
+@dataclasses.dataclass
+class BCBase:  # BoundaryInterface.f90:7-21
+    flag: np.ndarray  # W,E,S,N,B,T: 0 Dirichlet, 1 Neumann
+    VarW: Optional[np.ndarray] = None
+    VarE: Optional[np.ndarray] = None
+    VarS: Optional[np.ndarray] = None
+    VarN: Optional[np.ndarray] = None
+    VarB: Optional[np.ndarray] = None
+    VarT: Optional[np.ndarray] = None
+
+
+@dataclasses.dataclass
+class Projection:  # ProjectionP.f90:21-23
+    Pp: np.ndarray
+
+
+class ProjectionError(RuntimeError):
+    pass
+
+
+_projectors = {}
+
+
+def _projector(shape, params):
+    key = (shape, tuple(sorted(params.items())))
+    P = _projectors.get(key)
+    if P is None:
+        P = _eng.Projector(shape, **params)
+        _projectors[key] = P
+    return P
+
+
+def poisson_equation_solver(PGrid, UGrid, VGrid, WGrid, PCell, UCell, VCell, WCell, vofO, TVar,
+                            TPred, PU, PV, PW, BCp, PoCoef, Proj, dt, *, rho_l=998.3, rho_g=1.205,
+                            precision=_eng.SP_PREC_FP64, div_tol=1e-10, pin=(0, 0, 0), **solver):
+    """Projection with the reference's argument list; outputs in place.
+
+    Writes PoCoef (Imax,Jmax,Kmax,6), Proj.Pp (with ghosts) and the corrected
+    TVar.u/v/w on (1:I,1:J,1:K) plus TVar.p = Pp there, like
+    ComputePUVW.f90:181-221.  Returns the engine statistics."""
+    I, J, K = PCell.Cell_Type.shape
+    params = dict(precision=precision, div_tol=div_tol, rho_l=rho_l, rho_g=rho_g, rho_ref=rho_l,
+                  pin_i=pin[0], pin_j=pin[1], pin_k=pin[2], **solver)
+    P = _projector((I, J, K), params)
+    P.set_grid(PGrid.dx, PGrid.dy, PGrid.dz, UGrid.dx, VGrid.dy, WGrid.dz)
+    f = dict(u_star=TPred.u, v_star=TPred.v, w_star=TPred.w, dp_u=PU.Dp, dp_v=PV.Dp,
+             dp_w=PW.Dp, ee_area=PCell.EEArea, ne_area=PCell.NEArea, te_area=PCell.TEArea,
+             vof=PCell.vof, vof_old=vofO, vof_l=PCell.vofL, phi_l=PCell.phiL,
+             cell_type_p=PCell.Cell_Type, cell_type_u=UCell.Cell_Type,
+             cell_type_v=VCell.Cell_Type, cell_type_w=WCell.Cell_Type, moex_u=UCell.MoExCell,
+             moex_v=VCell.MoExCell, moex_w=WCell.MoExCell)
+    f = {k: np.asfortranarray(v) for k, v in f.items()}
+    bcv = [BCp.VarW, BCp.VarE, BCp.VarS, BCp.VarN, BCp.VarB, BCp.VarT]
+    bcv = [None if v is None else np.asfortranarray(v, dtype=np.float64) for v in bcv]
+    out = {"p": Proj.Pp, "u": TVar.u, "v": TVar.v, "w": TVar.w}
+    for k, a in out.items():
+        if not (isinstance(a, np.ndarray) and a.flags.f_contiguous and a.dtype == np.float64):
+            raise ProjectionError(f"{k} must be a Fortran-ordered float64 array")
+    res, st = P.project(f, np.asarray(BCp.flag, dtype=np.int32), dt, out=out, bc_val=bcv,
+                        po_coef=True)
+    PoCoef[...] = res["po_coef"]
+    TVar.p[1:-1, 1:-1, 1:-1] = Proj.Pp[1:-1, 1:-1, 1:-1]
+    return st
+
+
+# ---------------------------------------------------------------------------
+# Device path and autograd
+# ---------------------------------------------------------------------------
+
+
+def to_torch_halo(a, device="cuda"):
+    import torch
+    return torch.from_numpy(np.asfortranarray(a).T.copy()).to(device)
+
+
+def to_torch_cell(a, device="cuda"):
+    import torch
+    return torch.from_numpy(np.asfortranarray(a).T.copy()).to(device)
+
+
+class DeviceProjection:
+    """Projection of fixed geometry on device tensors.
+
+    ``geometry`` holds the sp_inputs fields other than u*, v*, w* as device
+    tensors (d-coefficients, apertures, vof, level set, cell flags);
+    ``bc_flag`` the six pressure BC flags (Neumann only here: the beta
+    parametrisation leaves Dirichlet wall terms fixed)."""
+
+    def __init__(self, shape, grid, geometry, bc_flag, dt, device_index=0, **params):
+        import torch
+        self.shape = tuple(shape)
+        I, J, K = self.shape
+        self.torch = torch
+        self.geometry = geometry
+        self.bc_flag = [int(v) for v in bc_flag]
+        self.dt = float(dt)
+        self.P = _eng.Projector(self.shape, device=device_index, **params)
+        self.P.set_grid_1d(*grid)
+        self.grid = [np.asarray(g, dtype=np.float64) for g in grid]
+        self.forward_id = 0
+        self.hs = (K + 2, J + 2, I + 2)
+        self.cs = (K, J, I)
+
+    def _fields(self, us, vs, ws):
+        f = dict(self.geometry)
+        f.update(u_star=us, v_star=vs, w_star=ws)
+        return f
+
+    def forward(self, us, vs, ws, beta=None, po_coef=False):
+        torch = self.torch
+        out = {n: torch.zeros(self.hs, dtype=torch.float64, device=us.device)
+               for n in ("p", "u", "v", "w")}
+        if po_coef:
+            out["po_coef"] = torch.zeros((6,) + self.cs, dtype=torch.float64, device=us.device)
+        tens = [us, vs, ws] + (list(beta) if beta is not None else [])
+        for t in tens:
+            if not t.is_contiguous() or t.dtype != torch.float64:
+                raise ProjectionError("inputs must be contiguous float64 device tensors")
+        torch.cuda.synchronize()
+        st = self.P.project_dev(self._fields(us, vs, ws), self.bc_flag, self.dt, out,
+                                beta=beta)
+        self.forward_id += 1
+        self.last_stats = st
+        return out, st
+
+    def gfm_beta(self, us, vs, ws):
+        """beta = PoCoef(:,4..6) * delta / Dp from the GFM coefficients."""
+        out, _ = self.forward(us, vs, ws, po_coef=True)
+        torch = self.torch
+        I, J, K = self.shape
+        dxu, dyv, dzw = (torch.from_numpy(self.grid[q]).to(us.device) for q in (3, 4, 5))
+        pc = out["po_coef"]
+        dp = [self.geometry[n][1:-1, 1:-1, 1:-1] for n in ("dp_u", "dp_v", "dp_w")]
+        be = pc[3] * dxu.view(1, 1, I) / dp[0]
+        bn = pc[4] * dyv.view(1, J, 1) / dp[1]
+        bt = pc[5] * dzw.view(K, 1, 1) / dp[2]
+        return [be.contiguous(), bn.contiguous(), bt.contiguous()]
+
+    def adjoint(self, gu, gv, gw):
+        torch = self.torch
+        res = {n: torch.zeros(self.hs, dtype=torch.float64, device=gu.device)
+               for n in ("gu_star", "gv_star", "gw_star")}
+        for n in ("gk_e", "gk_n", "gk_t", "gbeta_e", "gbeta_n", "gbeta_t"):
+            res[n] = torch.zeros(self.cs, dtype=torch.float64, device=gu.device)
+        g = {"gu": gu.contiguous(), "gv": gv.contiguous(), "gw": gw.contiguous()}
+        torch.cuda.synchronize()
+        st = self.P.adjoint_dev(g, res)
+        return res, st
+
+
+def _ProjectionFunction():
+    import torch
+
+    class ProjectionFunction(torch.autograd.Function):
+        """(u*, v*, w*, beta_e, beta_n, beta_t) -> (u, v, w, p).
+
+        The backward reuses the engine's multigrid PCG on the adjoint source;
+        only the forward pressure is kept (if another forward ran on the same
+        engine in between, the forward is recomputed first)."""
+
+        @staticmethod
+        def forward(ctx, dp_engine, us, vs, ws, be, bn, bt):
+            out, _ = dp_engine.forward(us, vs, ws, beta=[be, bn, bt])
+            ctx.dp_engine = dp_engine
+            ctx.fid = dp_engine.forward_id
+            ctx.save_for_backward(us, vs, ws, be, bn, bt)
+            ctx.mark_non_differentiable(out["p"])
+            return out["u"], out["v"], out["w"], out["p"]
+
+        @staticmethod
+        def backward(ctx, gu, gv, gw, gp):
+            E = ctx.dp_engine
+            us, vs, ws, be, bn, bt = ctx.saved_tensors
+            if E.forward_id != ctx.fid:
+                E.forward(us, vs, ws, beta=[be, bn, bt])
+            z = torch.zeros_like(us)
+            res, _ = E.adjoint(gu if gu is not None else z, gv if gv is not None else z,
+                               gw if gw is not None else z)
+            return (None, res["gu_star"], res["gv_star"], res["gw_star"], res["gbeta_e"],
+                    res["gbeta_n"], res["gbeta_t"])
+
+    return ProjectionFunction
+
+
+_pf = None
+
+
+def projection_function():
+    global _pf
+    if _pf is None:
+        _pf = _ProjectionFunction()
+    return _pf
+
+
+def project(dp_engine, us, vs, ws, beta):
+    """Differentiable projection: returns (u, v, w, p) tensors."""
+    return projection_function().apply(dp_engine, us, vs, ws, *beta)

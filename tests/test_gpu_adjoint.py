@@ -1,0 +1,131 @@
+"""The engine's adjoint of the projection on the GPU.
+
+* against the oracle's adjoint (same PoCoef and forward pressure), to 1e-7
+  relative in max norm;
+* through torch.autograd (simcoflow3d_amd.projection.project) against
+  central finite differences of L = sum W (u^2 + v^2 + w^2) in random
+  directions of u*, v*, w* and of the face coefficients beta, relative
+  error <= 1e-6.  The map u* -> u is linear, so the u* differences are exact
+  for any step; the beta differences use a relative step of 1e-4
+  (truncation ~1e-8)."""
+
+import numpy as np
+import pytest
+
+from oracle import orc
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def interior(a):
+    z = np.zeros_like(a)
+    z[1:-1, 1:-1, 1:-1] = a[1:-1, 1:-1, 1:-1]
+    return z
+
+
+@pytest.mark.parametrize("cfg,prec", [(3, 0), (3, 1), (0, 0), (4, 1)])
+def test_adjoint_matches_oracle(gpu_available, cfg, prec):
+    c = H.case(cfg, size=(16, 16, 20) if cfg != 4 else (32, 16, 16))
+    print(f"[adjoint] cfg={cfg} prec={prec} shape={c.shape}", flush=True)
+    P = H.projector(c, precision=prec)
+    out, st = P.project(H.fields(c), c.bc_flag, c.dt, po_coef=True)
+    assert st.converged == 1
+    rng = np.random.default_rng(5)
+    W = [interior(rng.uniform(0.5, 1.5, c.halo_shape)) for _ in range(3)]
+    g = [2 * Wq * interior(out[n]) for Wq, n in zip(W, ("u", "v", "w"))]
+    res, ast = P.adjoint(*g, beta_grad=True)
+    P.close()
+    assert ast.converged == 1
+    ref = orc.adjoint(c, out["po_coef"], out["p"], *g)
+    for a, b in (("gu_star", "gus"), ("gv_star", "gvs"), ("gw_star", "gws"), ("gk_e", "gkx"),
+                 ("gk_n", "gky"), ("gk_t", "gkz")):
+        assert H.rel_inf(res[a], ref[b]) <= 1e-7, (a, H.rel_inf(res[a], ref[b]))
+    # dL/dbeta = dL/dK * Dp/delta on the E,N,T faces
+    I, J, K = c.shape
+    dpe = c.dp_u[1:-1, 1:-1, 1:-1] / c.dxu[:, None, None]
+    assert np.allclose(res["gbeta_e"], res["gk_e"] * dpe, rtol=1e-14, atol=0)
+
+
+def _device_setup(cfg, size, prec, div_tol):
+    import torch
+    from simcoflow3d_amd import projection as PR
+    c = H.case(cfg, size=size)
+    f = H.fields(c)
+    geo = {}
+    for n, a in f.items():
+        if n in ("u_star", "v_star", "w_star"):
+            continue
+        geo[n] = PR.to_torch_halo(a) if a.ndim == 3 and a.shape == c.halo_shape \
+            else PR.to_torch_cell(a)
+    grid = (c.dxp, c.dyp, c.dzp, c.dxu, c.dyv, c.dzw)
+    E = PR.DeviceProjection(c.shape, grid, geo, c.bc_flag, c.dt, precision=prec,
+                            rho_l=c.rho_l, rho_g=c.rho_g, rho_ref=c.rho_l, div_tol=div_tol,
+                            max_iter=400)
+    us = [PR.to_torch_halo(f[n]) for n in ("u_star", "v_star", "w_star")]
+    beta = E.gfm_beta(*us)
+    return torch, PR, c, E, us, beta
+
+
+@pytest.mark.parametrize("cfg,size,prec", [(3, (128, 128, 128), 0), (3, (64, 64, 64), 1),
+                                           (1, (48, 48, 64), 0)])
+def test_autograd_matches_finite_differences(gpu_available, cfg, size, prec):
+    torch, PR, c, E, us, beta = _device_setup(cfg, size, prec, 1e-11)
+    print(f"[autograd-fd] cfg={cfg} size={size} prec={prec}", flush=True)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    hs = us[0].shape
+    mask = torch.zeros(hs, dtype=torch.float64, device="cuda")
+    mask[1:-1, 1:-1, 1:-1] = 1.0
+    W = [mask * (0.5 + torch.rand(hs, generator=gen, device="cuda", dtype=torch.float64))
+         for _ in range(3)]
+
+    def loss(u3, b3):
+        u, v, w, _ = PR.project(E, *u3, b3)
+        return sum((Wq * a * a).sum() for Wq, a in zip(W, (u, v, w)))
+
+    leaves = [t.clone().requires_grad_(True) for t in us + beta]
+    L = loss(leaves[:3], leaves[3:])
+    L.backward()
+    grads = [t.grad.detach() for t in leaves]
+    assert E.last_stats.converged == 1
+    worst = 0.0
+    with torch.no_grad():
+        for trial in range(32):
+            # u*, v*, w* direction (interior faces)
+            d = [mask * torch.randn(hs, generator=gen, device="cuda", dtype=torch.float64)
+                 for _ in range(3)]
+            h = 1e-2
+            lp = loss([a + h * b for a, b in zip(us, d)], beta)
+            lm = loss([a - h * b for a, b in zip(us, d)], beta)
+            fd = ((lp - lm) / (2 * h)).item()
+            an = sum((a * b).sum() for a, b in zip(d, grads[:3])).item()
+            worst = max(worst, abs(fd - an) / abs(an))
+            assert abs(fd - an) <= 1e-6 * abs(an), ("ustar", trial, fd, an)
+            # beta direction (relative perturbation of every face coefficient)
+            db = [torch.randn(b.shape, generator=gen, device="cuda", dtype=torch.float64) * b
+                  for b in beta]
+            h = 1e-4
+            lp = loss(us, [a + h * b for a, b in zip(beta, db)])
+            lm = loss(us, [a - h * b for a, b in zip(beta, db)])
+            fd = ((lp - lm) / (2 * h)).item()
+            an = sum((a * b).sum() for a, b in zip(db, grads[3:])).item()
+            worst = max(worst, abs(fd - an) / abs(an))
+            assert abs(fd - an) <= 1e-6 * abs(an), ("beta", trial, fd, an)
+    print(f"[autograd-fd] worst relative error {worst:.3e}", flush=True)
+
+
+def test_backward_after_interleaved_forward(gpu_available):
+    """The engine keeps one forward pressure: a backward whose forward was
+    followed by another forward recomputes its own first."""
+    torch, PR, c, E, us, beta = _device_setup(0, (24, 24, 24), 0, 1e-11)
+    print("[autograd-interleaved]", flush=True)
+    a = [t.clone().requires_grad_(True) for t in us]
+    u, v, w, _ = PR.project(E, *a, beta)
+    (u * u).sum().backward()
+    ref = a[0].grad.clone()
+    b = [t.clone().requires_grad_(True) for t in us]
+    u, v, w, _ = PR.project(E, *b, beta)
+    other = [2.0 * t for t in us]
+    PR.project(E, *other, beta)  # a second forward in between
+    (u * u).sum().backward()
+    assert torch.allclose(b[0].grad, ref, rtol=1e-9, atol=1e-12 * ref.abs().max().item())
