@@ -187,6 +187,9 @@ typedef struct sp_kernel_time {
 int sp_kernel_times(sp_ctx* ctx, sp_kernel_time* out, int cap);
 /* Enable (1) / disable (0) the per-kernel event timing (off by default). */
 void sp_set_profiling(sp_ctx* ctx, int on);
+/* Restrict the timing to launches of the kernel `name` (as reported by
+ * sp_kernel_times); NULL or "" times every kernel.  Clears the records. */
+void sp_set_profiling_filter(sp_ctx* ctx, const char* name);
 
 const char* sp_strerror(sp_status s);
 const char* sp_last_error(sp_ctx* ctx);

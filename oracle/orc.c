@@ -855,3 +855,5 @@ int orc_adjoint(const orc_grid* g, const double* pc, const double* ee, const dou
   free(cx); free(cy); free(cz); free(e); free(d); free(pbar); free(lam);
   return st.converged ? 0 : -3;
 }
+
+int orc_num_threads(void) { return omp_get_max_threads(); }

@@ -11,8 +11,26 @@
  *   1-D spacings     dxp/dyp/dzp = P-cell widths, dxu/dyv/dzw = centre
  *                    distances of the staggered U/V/W grids (Mesh.f90:160-247)
  *
- * Pinning status of each function is listed in oracle/README.md and in the
- * test files that check it.
+ * Pinning (the checks that would fail on a dropped term, wrong sign, index or
+ * transposed operand; tests/test_oracle_*.py and tests/test_golden.py):
+ *   orc_gfm_coef        closed forms of the four wet/dry cases, layered
+ *                       series average, face symmetry, boundary copy, the
+ *                       stale-beta rule on a hand-built column
+ *   orc_gfm_coef_fast   bit-identical to orc_gfm_coef on every config
+ *   orc_matrix_rows,    textbook 7-point Laplacian (single phase), symmetry
+ *   orc_rhs             and zero row sums, b = -V for u* = (x,0,0), b = 0 for a
+ *                       discrete curl, solid rows, Dirichlet wall terms
+ *   orc_sym_*           equal to the reference rows on the active cells
+ *   orc_pcg             sparse direct (SuperLU) and dense solves, O(h^2) on a
+ *                       manufactured cosine solution
+ *   orc_correct,        divergence-free output, idempotence, div-free input
+ *   orc_pressure_field, returned unchanged, masks and pressure ghosts
+ *   orc_mres
+ *   orc_adjoint         dot-product test of the linear map, central
+ *                       differences in u* and in PoCoef(:,4..6)
+ * No reference golden vector exists for this path and HYPRE is absent, so the
+ * HYPRE iterate itself is not pinned (any converged solve of the pinned,
+ * uniquely solvable system is).
  */
 #ifndef SIMCO_ORACLE_H
 #define SIMCO_ORACLE_H
@@ -125,6 +143,9 @@ int orc_adjoint(const orc_grid* g, const double* pocoef, const double* ee, const
                 const double* Pp, const double* gu, const double* gv, const double* gw,
                 const orc_solver_opts* o, double* gus, double* gvs, double* gws,
                 double* gkx, double* gky, double* gkz, double* lambda_out);
+
+/* OpenMP threads the parallel functions use. */
+int orc_num_threads(void);
 
 #ifdef __cplusplus
 }

@@ -50,6 +50,8 @@ def lib():
         G = ctypes.POINTER(OrcGrid)
         L.orc_gfm_coef.argtypes = [G] + [F64P] * 6 + [ctypes.c_double] * 4 + [F64P]
         L.orc_gfm_coef.restype = None
+        L.orc_num_threads.argtypes = []
+        L.orc_num_threads.restype = ctypes.c_int
         L.orc_gfm_coef_fast.argtypes = L.orc_gfm_coef.argtypes
         L.orc_gfm_coef_fast.restype = None
         L.orc_matrix_rows.argtypes = [G, F64P, F64P, F64P, F64P, I32P, I32P, F64P, I64P]
@@ -131,6 +133,10 @@ def gfm_coef_fast(case, epsi=1.0e-3):
                             _p(case.dp_u), _p(case.dp_v), _p(case.dp_w), case.rho_l, case.rho_g,
                             case.rho_l, epsi, _p(pc))
     return pc
+
+
+def num_threads():
+    return int(lib().orc_num_threads())
 
 
 def baseline_project(case, levels=0, pre=2, post=2, coarse_sweeps=4, coarse_scale=0.5,

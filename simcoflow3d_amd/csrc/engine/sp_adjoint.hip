@@ -15,6 +15,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "sp_device.h"
 #include "sp_geom.h"
 #include "sp_internal.h"
@@ -62,7 +64,7 @@ __global__ void __launch_bounds__(256) k_adj_source(AdjArgs A) {
   const Cell3 cq = cell_of(A.I, A.J, A.K);
   const int a = cq.a, b = cq.b, c = cq.c;
   const int64_t N = A.N;
-  double psum = 0.0;
+  double psum = 0.0, pmax = 0.0;
   if (cq.in) {
     {
       const int64_t n = nidx(A, a, b, c), h = hidx(A, a, b, c);
@@ -93,18 +95,20 @@ __global__ void __launch_bounds__(256) k_adj_source(AdjArgs A) {
       pb = A.active[n] ? pb : 0.0;
       A.pbar[n] = pb;
       psum += pb;
+      pmax = fabs(pb) / (A.dxp[a] * A.dyp[b] * A.dzp[c]);
     }
   }
-  double v[1] = {psum};
-  block_partials<kTileThreads, 1, 0u>(A.site, v);
+  double v[2] = {psum, pmax};
+  block_partials<kTileThreads, 2, 2u>(A.site, v);
 }
 
 __global__ void __launch_bounds__(kFinThreads) k_adj_source_fin(AdjArgs A, uint32_t nb) {
-  double t[1];
-  finalize_partials<kFinThreads, 1, 0u>(A.site, nb, t);
+  double t[2];
+  finalize_partials<kFinThreads, 2, 2u>(A.site, nb, t);
   if (flat_tid() == 0) {
     Scalars* S = A.S;
     S->bsum = t[0];
+    S->src_max = t[1];
     S->bmean = (S->singular && S->active > 0) ? t[0] / (double)S->active : 0.0;
   }
 }
@@ -207,9 +211,16 @@ sp_status adjoint(sp_ctx* c, const sp_adj_inputs& in, sp_adj_outputs& out, sp_st
   hipLaunchKernelGGL(k_adj_source_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A,
                      flat_blocks(D.I, D.J, D.K));
   timer_end(c, slot);
-  // solve A lam = pbar (mean-projected when singular) with the forward's
-  // operator and multigrid hierarchy
-  sp_status s = solve(c, c->pbar, c->x, false, st);
+  // Solve A lam = pbar (mean-projected when singular) with the forward's
+  // operator and multigrid hierarchy, to the relative reduction the forward
+  // was asked for (div_tol / initial divergence), kept within [1e-13, 1e-10]:
+  // the adjoint source has other units than the divergence.
+  hipError_t e = hipMemcpyAsync(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "adjoint source");
+  double rel = c->fwd_initial_div > 0.0 ? c->prm.div_tol / c->fwd_initial_div : 1e-10;
+  rel = std::min(std::max(rel, 1e-13), 1e-10);
+  sp_status s = solve(c, c->pbar, rel * c->S_host->src_max, st);
   if (s != SP_OK) return s;
   A.site = red_site(c, kSiteAdj2);
   timer_begin(c, "adj_lam_mean", NB * 9, &slot);

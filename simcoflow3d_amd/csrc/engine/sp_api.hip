@@ -31,6 +31,7 @@ sp_status hip_check(sp_ctx* c, hipError_t e, const char* what) {
 void timer_begin(sp_ctx* c, const char* name, double bytes, int* slot) {
   *slot = -1;
   if (!c->profiling) return;
+  if (!c->prof_filter.empty() && c->prof_filter != name) return;
   int ti = -1;
   for (size_t q = 0; q < c->timers.size(); ++q)
     if (c->timers[q].name == name) { ti = (int)q; break; }
@@ -251,7 +252,8 @@ static sp_status project_device(sp_ctx* c, const sp_inputs* in, sp_outputs* out,
   S->singular = c->S_host->singular;
   S->active_cells = c->S_host->active;
   S->stale_faces = stale;
-  s = solve(c, c->b, c->x, true, S);
+  c->fwd_initial_div = c->S_host->bmax;
+  s = solve(c, c->b, c->prm.div_tol, S);
   if (s != SP_OK && s != SP_ERR_NOT_CONVERGED) return s;
   auto t2 = std::chrono::steady_clock::now();
   sp_status sc = correct(c, *in, *out, S);
@@ -582,6 +584,13 @@ void* sp_stream(sp_ctx* c) { return c ? (void*)c->stream : nullptr; }
 void sp_set_profiling(sp_ctx* c, int on) {
   if (!c) return;
   c->profiling = on != 0;
+  c->timers.clear();
+  c->ev_used.clear();
+}
+
+void sp_set_profiling_filter(sp_ctx* c, const char* name) {
+  if (!c) return;
+  c->prof_filter = name ? name : "";
   c->timers.clear();
   c->ev_used.clear();
 }

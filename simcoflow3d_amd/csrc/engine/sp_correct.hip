@@ -33,7 +33,6 @@ struct CorrArgs {
 
 // shift = x(pin) or mean of x over active cells (singular systems only)
 __global__ void __launch_bounds__(256) k_shift(CorrArgs A) {
-  Scalars* S = A.S;
   const Cell3 q = cell_of(A.I, A.J, A.K);
   double v[1] = {0.0};
   if (q.in) {

@@ -27,6 +27,7 @@ struct Scalars {
   double bsum, esum, bmax, shift, divmax;
   double dsum;      // sum of the diagonal over active cells
   double zshift;    // diagonal-weighted mean of z removed in singular systems
+  double src_max;   // max |pbar_i| / V_i of the adjoint source
   double npart[8];  // scratch results of small reductions
   int64_t active;
   int32_t done;     // 1 once converged (kernels then return immediately)
@@ -166,12 +167,14 @@ struct sp_ctx {
 
   // adjoint work
   double *pbar = nullptr, *lam = nullptr;
+  double fwd_initial_div = 0.0;  // max |b|/V of the last forward (adjoint tolerance)
 
   // bc values staged
   double* bcv[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 
   // profiling
   bool profiling = false;
+  std::string prof_filter;  // time only launches of this kernel (empty: all)
   std::vector<sp::Timer> timers;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
   std::vector<std::pair<int, int>> ev_used;  // (timer index, event index)
@@ -199,7 +202,7 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_count);
 
 // ---- solver (sp_solver.hip) ----
 sp_status mg_setup(sp_ctx* c);
-sp_status solve(sp_ctx* c, const double* rhs, double* xout, bool forward, sp_stats* st);
+sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st);
 
 // ---- correction (sp_correct.hip) ----
 sp_status correct(sp_ctx* c, const sp_inputs& in, sp_outputs& out, sp_stats* st);

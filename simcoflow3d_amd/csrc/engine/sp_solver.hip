@@ -641,9 +641,9 @@ sp_status mg_setup(sp_ctx* c) {
 }
 
 // Solve A x = rhs - mean (singular) with the multigrid-preconditioned
-// flexible CG.  The operator arrays and c->S (bmean, singular) come from the
-// assembly.  Writes x into c->x.
-sp_status solve(sp_ctx* c, const double* rhs, double* /*xout*/, bool /*forward*/, sp_stats* st) {
+// flexible CG until max |r_i| / V_i <= tol.  The operator arrays and c->S
+// (bmean, singular) come from the assembly.  Writes x into c->x.
+sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st) {
   const Dims& D = c->dm;
   const bool mixed = c->prm.precision == SP_PREC_MIXED;
   const bool mg = c->prm.precond == SP_PRECOND_MG;
@@ -660,7 +660,7 @@ sp_status solve(sp_ctx* c, const double* rhs, double* /*xout*/, bool /*forward*/
   A.x = c->x; A.r = c->r; A.q = c->q; A.p = c->p0;
   A.S = c->S;
   A.hflag = c->hflags_dev;
-  A.tol = c->prm.div_tol;
+  A.tol = tol;
   if (mixed && mg) {
     A.z32 = c->z32;
     A.r32 = c->r32;

@@ -32,7 +32,8 @@ SP_PRECOND_JACOBI = 1
 EXPORTED_SYMBOLS = (
     "sp_params_default", "sp_create", "sp_destroy", "sp_set_params", "sp_set_grid",
     "sp_set_grid_1d", "sp_project", "sp_project_dev", "sp_adjoint", "sp_adjoint_dev",
-    "sp_stream", "sp_kernel_times", "sp_set_profiling", "sp_strerror", "sp_last_error",
+    "sp_stream", "sp_kernel_times", "sp_set_profiling", "sp_set_profiling_filter",
+    "sp_strerror", "sp_last_error",
 )
 
 
@@ -136,6 +137,8 @@ def load():
     L.sp_kernel_times.restype = ctypes.c_int
     L.sp_set_profiling.argtypes = [V, ctypes.c_int]
     L.sp_set_profiling.restype = None
+    L.sp_set_profiling_filter.argtypes = [V, ctypes.c_char_p]
+    L.sp_set_profiling_filter.restype = None
     L.sp_strerror.argtypes = [ctypes.c_int]
     L.sp_strerror.restype = ctypes.c_char_p
     L.sp_last_error.argtypes = [V]
@@ -278,6 +281,10 @@ class Projector:
 
     def set_profiling(self, on):
         self.lib.sp_set_profiling(self.h, 1 if on else 0)
+
+    def set_profiling_filter(self, name=None):
+        """Time only the launches of kernel `name` (None: every kernel)."""
+        self.lib.sp_set_profiling_filter(self.h, name.encode() if name else None)
 
     def kernel_times(self):
         n = self.lib.sp_kernel_times(self.h, None, 0)
