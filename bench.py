@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--levels", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run one 1 GiB device-to-device copy (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
     return ap.parse_args()
 
 
@@ -134,6 +137,15 @@ def main():
 
     from simcoflow3d_amd import configs
     from simcoflow3d_amd.engine import Projector, SP_PREC_FP64, SP_PREC_MIXED
+
+    if args.copy_probe:
+        # plain copy of known size: reads 2^30 B, writes 2^30 B
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     cfg = args.config
     size = tuple(args.size) if args.size else None

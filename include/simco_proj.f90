@@ -1,0 +1,202 @@
+! simco_proj.f90 -- Fortran 2003 (ISO_C_BINDING) interface to the C ABI in
+! simco_proj.h, for SIMCOFlow's Fortran host.
+!
+! The derived types are bind(C) twins of the C structs (same members, same
+! order; the companion C compiler's layout rules apply to both).  Array
+! arguments are passed by address with the host's own bounds:
+!   halo arrays  real(dp)   (0:Imax+1, 0:Jmax+1, 0:Kmax+1)
+!   cell arrays  integer(it4b) (1:Imax, 1:Jmax, 1:Kmax)
+!   PoCoef       real(dp)   (1:Imax, 1:Jmax, 1:Kmax, 6)
+! and stored in the type(c_ptr) members with c_loc (the actual arrays need
+! the TARGET attribute).  real(dp) = real(c_double), integer(it4b) =
+! integer(c_int32_t) (Precision.f90:4,7).
+module simco_proj
+  use iso_c_binding
+  implicit none
+  private
+  public :: sp_params, sp_inputs, sp_outputs, sp_stats, sp_adj_inputs, sp_adj_outputs
+  public :: sp_params_default, sp_create, sp_destroy, sp_set_params, sp_set_grid, &
+            sp_set_grid_1d, sp_project, sp_project_dev, sp_adjoint, sp_adjoint_dev, &
+            sp_strerror_f, sp_last_error_f
+  public :: SP_OK, SP_ERR_INVALID_ARG, SP_ERR_HIP, SP_ERR_NOT_CONVERGED, SP_ERR_NONFINITE, &
+            SP_ERR_OOM, SP_ERR_GRID, SP_ERR_STATE, SP_ERR_NO_DEVICE, SP_PREC_FP64, &
+            SP_PREC_MIXED, SP_PRECOND_MG, SP_PRECOND_JACOBI
+
+  integer(c_int), parameter :: SP_OK = 0, SP_ERR_INVALID_ARG = -1, SP_ERR_HIP = -2, &
+                               SP_ERR_NOT_CONVERGED = -3, SP_ERR_NONFINITE = -4, &
+                               SP_ERR_OOM = -5, SP_ERR_GRID = -6, SP_ERR_STATE = -7, &
+                               SP_ERR_NO_DEVICE = -8
+  integer(c_int32_t), parameter :: SP_PREC_FP64 = 0, SP_PREC_MIXED = 1
+  integer(c_int32_t), parameter :: SP_PRECOND_MG = 0, SP_PRECOND_JACOBI = 1
+
+  type, bind(C) :: sp_params
+    integer(c_int32_t) :: imax, jmax, kmax
+    integer(c_int32_t) :: precision, precond
+    integer(c_int32_t) :: mg_levels, mg_pre, mg_post, mg_coarse_sweeps
+    real(c_double) :: mg_coarse_scale
+    integer(c_int32_t) :: max_iter
+    real(c_double) :: div_tol
+    real(c_double) :: rho_l, rho_g, rho_ref
+    real(c_double) :: epsi
+    integer(c_int32_t) :: pin_i, pin_j, pin_k
+    integer(c_int32_t) :: flags
+  end type sp_params
+
+  type, bind(C) :: sp_inputs
+    type(c_ptr) :: u_star = c_null_ptr, v_star = c_null_ptr, w_star = c_null_ptr
+    type(c_ptr) :: dp_u = c_null_ptr, dp_v = c_null_ptr, dp_w = c_null_ptr
+    type(c_ptr) :: ee_area = c_null_ptr, ne_area = c_null_ptr, te_area = c_null_ptr
+    type(c_ptr) :: vof = c_null_ptr, vof_old = c_null_ptr
+    type(c_ptr) :: vof_l = c_null_ptr, phi_l = c_null_ptr
+    type(c_ptr) :: cell_type_p = c_null_ptr
+    type(c_ptr) :: cell_type_u = c_null_ptr, cell_type_v = c_null_ptr, cell_type_w = c_null_ptr
+    type(c_ptr) :: moex_u = c_null_ptr, moex_v = c_null_ptr, moex_w = c_null_ptr
+    integer(c_int32_t) :: bc_flag(6) = 1
+    type(c_ptr) :: bc_val(6) = c_null_ptr
+    real(c_double) :: dt = 0.0_c_double
+    type(c_ptr) :: beta_e = c_null_ptr, beta_n = c_null_ptr, beta_t = c_null_ptr
+  end type sp_inputs
+
+  type, bind(C) :: sp_outputs
+    type(c_ptr) :: p = c_null_ptr
+    type(c_ptr) :: u = c_null_ptr, v = c_null_ptr, w = c_null_ptr
+    type(c_ptr) :: po_coef = c_null_ptr
+  end type sp_outputs
+
+  type, bind(C) :: sp_stats
+    integer(c_int32_t) :: iterations, converged, levels, singular
+    integer(c_int64_t) :: active_cells, stale_faces
+    real(c_double) :: initial_div, final_div, residual_div
+    real(c_double) :: t_assemble_ms, t_solve_ms, t_correct_ms, t_total_ms
+  end type sp_stats
+
+  type, bind(C) :: sp_adj_inputs
+    type(c_ptr) :: gu = c_null_ptr, gv = c_null_ptr, gw = c_null_ptr
+  end type sp_adj_inputs
+
+  type, bind(C) :: sp_adj_outputs
+    type(c_ptr) :: gu_star = c_null_ptr, gv_star = c_null_ptr, gw_star = c_null_ptr
+    type(c_ptr) :: gk_e = c_null_ptr, gk_n = c_null_ptr, gk_t = c_null_ptr
+    type(c_ptr) :: gbeta_e = c_null_ptr, gbeta_n = c_null_ptr, gbeta_t = c_null_ptr
+  end type sp_adj_outputs
+
+  interface
+    subroutine sp_params_default(p) bind(C, name="sp_params_default")
+      import :: sp_params
+      type(sp_params), intent(out) :: p
+    end subroutine sp_params_default
+
+    integer(c_int) function sp_create(ctx, params, device) bind(C, name="sp_create")
+      import :: c_ptr, c_int, sp_params
+      type(c_ptr), intent(out) :: ctx
+      type(sp_params), intent(in) :: params
+      integer(c_int), value :: device
+    end function sp_create
+
+    integer(c_int) function sp_destroy(ctx) bind(C, name="sp_destroy")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function sp_destroy
+
+    integer(c_int) function sp_set_params(ctx, params) bind(C, name="sp_set_params")
+      import :: c_ptr, c_int, sp_params
+      type(c_ptr), value :: ctx
+      type(sp_params), intent(in) :: params
+    end function sp_set_params
+
+    ! PGrid%dx,dy,dz, UGrid%dx, VGrid%dy, WGrid%dz (Imax x Jmax x Kmax each)
+    integer(c_int) function sp_set_grid(ctx, dxp, dyp, dzp, dxu, dyv, dzw) &
+        bind(C, name="sp_set_grid")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: dxp(*), dyp(*), dzp(*), dxu(*), dyv(*), dzw(*)
+    end function sp_set_grid
+
+    integer(c_int) function sp_set_grid_1d(ctx, dxp, dyp, dzp, dxu, dyv, dzw) &
+        bind(C, name="sp_set_grid_1d")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: dxp(*), dyp(*), dzp(*), dxu(*), dyv(*), dzw(*)
+    end function sp_set_grid_1d
+
+    integer(c_int) function sp_project(ctx, inp, outp, st) bind(C, name="sp_project")
+      import :: c_ptr, c_int, sp_inputs, sp_outputs, sp_stats
+      type(c_ptr), value :: ctx
+      type(sp_inputs), intent(in) :: inp
+      type(sp_outputs), intent(in) :: outp
+      type(sp_stats), intent(out) :: st
+    end function sp_project
+
+    integer(c_int) function sp_project_dev(ctx, inp, outp, st) bind(C, name="sp_project_dev")
+      import :: c_ptr, c_int, sp_inputs, sp_outputs, sp_stats
+      type(c_ptr), value :: ctx
+      type(sp_inputs), intent(in) :: inp
+      type(sp_outputs), intent(in) :: outp
+      type(sp_stats), intent(out) :: st
+    end function sp_project_dev
+
+    integer(c_int) function sp_adjoint(ctx, inp, outp, st) bind(C, name="sp_adjoint")
+      import :: c_ptr, c_int, sp_adj_inputs, sp_adj_outputs, sp_stats
+      type(c_ptr), value :: ctx
+      type(sp_adj_inputs), intent(in) :: inp
+      type(sp_adj_outputs), intent(in) :: outp
+      type(sp_stats), intent(out) :: st
+    end function sp_adjoint
+
+    integer(c_int) function sp_adjoint_dev(ctx, inp, outp, st) bind(C, name="sp_adjoint_dev")
+      import :: c_ptr, c_int, sp_adj_inputs, sp_adj_outputs, sp_stats
+      type(c_ptr), value :: ctx
+      type(sp_adj_inputs), intent(in) :: inp
+      type(sp_adj_outputs), intent(in) :: outp
+      type(sp_stats), intent(out) :: st
+    end function sp_adjoint_dev
+
+    type(c_ptr) function sp_strerror_c(s) bind(C, name="sp_strerror")
+      import :: c_ptr, c_int
+      integer(c_int), value :: s
+    end function sp_strerror_c
+
+    type(c_ptr) function sp_last_error_c(ctx) bind(C, name="sp_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function sp_last_error_c
+
+    integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+      import :: c_ptr, c_size_t
+      type(c_ptr), value :: s
+    end function c_strlen
+  end interface
+
+contains
+
+  ! C string -> Fortran character
+  function c_to_f(p) result(s)
+    type(c_ptr), intent(in) :: p
+    character(len=:), allocatable :: s
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n, i
+    if (.not. c_associated(p)) then
+      s = ''
+      return
+    end if
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [n])
+    allocate(character(len=n) :: s)
+    do i = 1, n
+      s(i:i) = chars(i)
+    end do
+  end function c_to_f
+
+  function sp_strerror_f(status) result(s)
+    integer(c_int), intent(in) :: status
+    character(len=:), allocatable :: s
+    s = c_to_f(sp_strerror_c(status))
+  end function sp_strerror_f
+
+  function sp_last_error_f(ctx) result(s)
+    type(c_ptr), intent(in) :: ctx
+    character(len=:), allocatable :: s
+    s = c_to_f(sp_last_error_c(ctx))
+  end function sp_last_error_f
+
+end module simco_proj

@@ -10,7 +10,7 @@ import os
 import subprocess
 import sys
 
-from ._paths import INCLUDE_DIR, LIB_DIR, PKG_DIR
+from ._paths import ENGINE_LIB, INCLUDE_DIR, LIB_DIR, PKG_DIR
 
 CSRC = os.path.join(PKG_DIR, "csrc")
 
@@ -82,13 +82,39 @@ def build_oracle():
     _run(["make", "-s", "-C", os.path.join(repo, "oracle")])
 
 
+FLANG = "/opt/rocm/lib/llvm/bin/flang"
+
+
+def build_fortran():
+    """The Fortran binding module (include/simco_proj.f90) and the Fortran
+    test hosts under tests/fortran, linked against the engine library."""
+    repo = os.path.dirname(PKG_DIR)
+    if not os.path.exists(FLANG):
+        print("flang not found: Fortran binding not built")
+        return None
+    src = os.path.join(repo, "tests", "fortran")
+    out = os.path.join(src, "_build")
+    os.makedirs(out, exist_ok=True)
+    mod_obj = os.path.join(out, "simco_proj.o")
+    _run([FLANG, "-O2", "-J", out, "-c", os.path.join(repo, "include", "simco_proj.f90"),
+          "-o", mod_obj])
+    lib_dir = os.path.dirname(ENGINE_LIB)
+    rpath = "-Wl,-rpath,$ORIGIN/" + os.path.relpath(lib_dir, out)
+    for prog in ("sp_layout", "sp_drive"):
+        _run([FLANG, "-O2", "-J", out, os.path.join(src, prog + ".f90"), mod_obj,
+              "-L" + lib_dir, "-lsimcoproj", rpath, "-o", os.path.join(out, prog)])
+    return out
+
+
 def build_all():
     build_gen()
     build_oracle()
     build_engine()
+    build_fortran()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"gen": build_gen, "engine": build_engine, "oracle": build_oracle, "all": build_all}[w]()
+        {"gen": build_gen, "engine": build_engine, "oracle": build_oracle,
+         "fortran": build_fortran, "all": build_all}[w]()

@@ -1,0 +1,38 @@
+! Prints the size and selected member offsets of the bind(C) types so the
+! tests can compare them with the C structs of simco_proj.h.
+program sp_layout
+  use iso_c_binding
+  use simco_proj
+  implicit none
+  type(sp_params), target :: prm
+  type(sp_inputs), target :: inp
+  type(sp_outputs), target :: outp
+  type(sp_stats), target :: st
+  type(sp_adj_inputs), target :: ai
+  type(sp_adj_outputs), target :: ao
+  print '(a,1x,i0)', 'sp_params', c_sizeof(prm)
+  print '(a,1x,i0)', 'sp_inputs', c_sizeof(inp)
+  print '(a,1x,i0)', 'sp_outputs', c_sizeof(outp)
+  print '(a,1x,i0)', 'sp_stats', c_sizeof(st)
+  print '(a,1x,i0)', 'sp_adj_inputs', c_sizeof(ai)
+  print '(a,1x,i0)', 'sp_adj_outputs', c_sizeof(ao)
+  print '(a,1x,i0)', 'sp_params.mg_coarse_scale', off(c_loc(prm%mg_coarse_scale), c_loc(prm))
+  print '(a,1x,i0)', 'sp_params.max_iter', off(c_loc(prm%max_iter), c_loc(prm))
+  print '(a,1x,i0)', 'sp_params.div_tol', off(c_loc(prm%div_tol), c_loc(prm))
+  print '(a,1x,i0)', 'sp_params.pin_i', off(c_loc(prm%pin_i), c_loc(prm))
+  print '(a,1x,i0)', 'sp_params.flags', off(c_loc(prm%flags), c_loc(prm))
+  print '(a,1x,i0)', 'sp_inputs.bc_flag', off(c_loc(inp%bc_flag), c_loc(inp))
+  print '(a,1x,i0)', 'sp_inputs.bc_val', off(c_loc(inp%bc_val), c_loc(inp))
+  print '(a,1x,i0)', 'sp_inputs.dt', off(c_loc(inp%dt), c_loc(inp))
+  print '(a,1x,i0)', 'sp_inputs.beta_t', off(c_loc(inp%beta_t), c_loc(inp))
+  print '(a,1x,i0)', 'sp_stats.active_cells', off(c_loc(st%active_cells), c_loc(st))
+  print '(a,1x,i0)', 'sp_stats.t_total_ms', off(c_loc(st%t_total_ms), c_loc(st))
+contains
+  integer function off(a, b)
+    type(c_ptr), intent(in) :: a, b
+    integer(c_intptr_t) :: ia, ib
+    ia = transfer(a, ia)
+    ib = transfer(b, ib)
+    off = int(ia - ib)
+  end function off
+end program sp_layout
