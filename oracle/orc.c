@@ -648,6 +648,9 @@ int orc_pcg(const orc_grid* g, const double* cx, const double* cy, const double*
     else sym_apply_level(I, J, K, F->cx, F->cy, F->cz, d, p, q);
     const double pq = dot_active(n, d, p, q);
     const double alpha = rz / pq;
+    /* breakdown below the FP64 floor (solves run to a tolerance they cannot
+     * reach): keep the last finite iterate */
+    if (!(pq > 0.0) || !isfinite(alpha)) break;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i)
       if (d[i] > 0) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
@@ -659,6 +662,7 @@ int orc_pcg(const orc_grid* g, const double* cx, const double* cy, const double*
     const double rz_new = dot_active(n, d, r, z);
     const double zq = dot_active(n, d, z, q);
     const double beta = -alpha * zq / rz;
+    if (!(rz_new != 0.0) || !isfinite(beta)) break;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) p[i] = d[i] > 0 ? z[i] + beta * p[i] : 0.0;
     rz = rz_new;
