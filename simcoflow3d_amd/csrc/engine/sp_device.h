@@ -252,4 +252,32 @@ __device__ inline Cell3 cell_of(int nx, int ny, int nz) {
   return r;
 }
 
+// Column launch: 64x4 workgroups, every thread walks kColK consecutive cells
+// up k at one (i, j).  Block reductions are paid once per kColK cells, and a
+// stencil keeps its k-1 / k+1 values in registers.
+constexpr int kColK = 16;
+inline dim3 col_grid(int nx, int ny, int nz) {
+  return dim3((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4),
+              (unsigned)((nz + kColK - 1) / kColK));
+}
+inline uint32_t col_blocks(int nx, int ny, int nz) {
+  return (uint32_t)((nx + 63) / 64) * (uint32_t)((ny + 3) / 4) *
+         (uint32_t)((nz + kColK - 1) / kColK);
+}
+struct Col3 {
+  int a, b, c0, c1;  // cells (a, b, c) for c0 <= c < c1
+  bool in;
+};
+__device__ inline Col3 col_of(int nx, int ny, int nz) {
+  uint32_t bx, by, bz;
+  xcd_block3(bx, by, bz);
+  Col3 r;
+  r.a = (int)(bx * blockDim.x + threadIdx.x);
+  r.b = (int)(by * blockDim.y + threadIdx.y);
+  r.c0 = (int)bz * kColK;
+  r.c1 = r.c0 + kColK < nz ? r.c0 + kColK : nz;
+  r.in = r.a < nx && r.b < ny;
+  return r;
+}
+
 }  // namespace sp

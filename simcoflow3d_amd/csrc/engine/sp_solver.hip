@@ -282,7 +282,8 @@ struct CgArgs {
   const float* dw;
   const double *dxp, *dyp, *dzp;
   const double* b;
-  double *x, *r, *q, *p;
+  double *x, *r, *q;
+  double* p;  // search direction
   const double* z64;
   const float* z32;
   float* r32;
@@ -363,17 +364,21 @@ __global__ void __launch_bounds__(kFinThreads) k_cg_residual_fin(CgArgs A, uint3
 __global__ void __launch_bounds__(256) k_cg_dots(CgArgs A) {
   Scalars* S = A.S;
   if (S->done) return;
-  const Cell3 q = cell_of(A.I, A.J, A.K);
+  const Col3 q = col_of(A.I, A.J, A.K);
   double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (q.in) {
-    const int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c);
-    if (A.active[n]) {
-      const double z = zval(A, n), r = A.r[n], qq = A.q[n];
-      v[0] = r * z;
-      v[1] = z * qq;
-      v[2] = (double)A.dw[n] * z;
-      v[3] = r;
-      v[4] = qq;
+    const int64_t IJ = (int64_t)A.I * A.J;
+    int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c0);
+#pragma unroll 4
+    for (int c = q.c0; c < q.c1; ++c, n += IJ) {
+      if (A.active[n]) {
+        const double z = zval(A, n), r = A.r[n], qq = A.q[n];
+        v[0] += r * z;
+        v[1] += z * qq;
+        v[2] += (double)A.dw[n] * z;
+        v[3] += r;
+        v[4] += qq;
+      }
     }
   }
   block_partials<kTileThreads, 5, 0u>(A.site, v);
@@ -400,27 +405,66 @@ __global__ void __launch_bounds__(kFinThreads) k_cg_dots_fin(CgArgs A, uint32_t 
 __global__ void __launch_bounds__(256) k_cg_pupdate(CgArgs A) {
   const Scalars* S = A.S;
   if (S->done) return;
-  const Cell3 q = cell_of(A.I, A.J, A.K);
+  const Col3 q = col_of(A.I, A.J, A.K);
   if (!q.in) return;
-  const int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c);
   const double beta = S->beta, m = S->zshift;
-  A.p[n] = A.active[n] ? (zval(A, n) - m) + beta * A.p[n] : 0.0;
+  const int64_t IJ = (int64_t)A.I * A.J;
+  int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c0);
+#pragma unroll 4
+  for (int c = q.c0; c < q.c1; ++c, n += IJ) {
+    const double z = zval(A, n), po = A.p[n];
+    A.p[n] = A.active[n] ? (z - m) + beta * po : 0.0;
+  }
 }
 
-// q = A p ; pq = (p, q) ; alpha = rz / pq
+// q = A p ; pq = (p, q).  Each thread walks its column keeping p and the
+// z-coupling of planes k-1 / k / k+1 in registers; in-plane neighbours come
+// through the cache.  Neighbour indices are clamped at the domain edge,
+// where the coupling is zero.
 __global__ void __launch_bounds__(256) k_cg_spmv(CgArgs A) {
   Scalars* S = A.S;
   if (S->done) return;
-  const Cell3 q = cell_of(A.I, A.J, A.K);
+  const Col3 q = col_of(A.I, A.J, A.K);
   double v[1] = {0.0};
   if (q.in) {
-    const int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c);
-    double qv = 0.0;
-    if (A.active[n]) {
-      qv = apply_row(A, A.p, q.a, q.b, q.c, n);
-      v[0] = A.p[n] * qv;
+    const int64_t I = A.I, IJ = (int64_t)A.I * A.J;
+    const int a = q.a, b = q.b;
+    const int64_t dW = a > 0 ? 1 : 0, dE = a < A.I - 1 ? 1 : 0;
+    const int64_t dS = b > 0 ? I : 0, dN = b < A.J - 1 ? I : 0;
+    const double* P = A.p;
+    int64_t n = a + I * (b + (int64_t)A.J * q.c0);
+    double pm = 0.0, czm = 0.0;
+    if (q.c0 > 0) {
+      pm = P[n - IJ];
+      czm = A.cz[n - IJ];
     }
-    A.q[n] = qv;
+    double pc = P[n];
+    for (int c = q.c0; c < q.c1; ++c, n += IJ) {
+      const double pp = c < A.K - 1 ? P[n + IJ] : 0.0;
+      const double cxm = a > 0 ? A.cx[n - dW] : 0.0;
+      const double cym = b > 0 ? A.cy[n - dS] : 0.0;
+      const double cx = A.cx[n], cy = A.cy[n], cz = A.cz[n];
+      const double pw = P[n - dW], pe = P[n + dE], ps = P[n - dS], pn = P[n + dN];
+      double d = cxm + cym;
+      d = d + czm;
+      d = d + cx;
+      d = d + cy;
+      d = d + cz;
+      if (A.e) d = d + A.e[n];
+      double sacc = d * pc;
+      sacc -= cxm * pw;
+      sacc -= cym * ps;
+      sacc -= czm * pm;
+      sacc -= cx * pe;
+      sacc -= cy * pn;
+      sacc -= cz * pp;
+      const double qv = A.active[n] ? sacc : 0.0;
+      A.q[n] = qv;
+      v[0] += pc * qv;
+      pm = pc;
+      pc = pp;
+      czm = cz;
+    }
   }
   block_partials<kTileThreads, 1, 0u>(A.site, v);
 }
@@ -441,16 +485,22 @@ __global__ void __launch_bounds__(kFinThreads) k_cg_spmv_fin(CgArgs A, uint32_t 
 __global__ void __launch_bounds__(256) k_cg_update(CgArgs A) {
   Scalars* S = A.S;
   if (S->done) return;
-  const Cell3 q = cell_of(A.I, A.J, A.K);
+  const Col3 q = col_of(A.I, A.J, A.K);
   const double alpha = S->alpha;
   double v[1] = {0.0};
   if (q.in) {
-    const int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c);
-    A.x[n] += alpha * A.p[n];
-    const double rv = A.r[n] - alpha * A.q[n];
-    A.r[n] = rv;
-    if (A.r32) A.r32[n] = (float)rv;
-    v[0] = fabs(rv) / (A.dxp[q.a] * A.dyp[q.b] * A.dzp[q.c]);
+    const int64_t IJ = (int64_t)A.I * A.J;
+    const double dxy = A.dxp[q.a] * A.dyp[q.b];
+    int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c0);
+#pragma unroll 4
+    for (int c = q.c0; c < q.c1; ++c, n += IJ) {
+      A.x[n] += alpha * A.p[n];
+      const double rv = A.r[n] - alpha * A.q[n];
+      A.r[n] = rv;
+      if (A.r32) A.r32[n] = (float)rv;
+      const double rr = fabs(rv) / (dxy * A.dzp[c]);
+      v[0] = (rr > v[0] || rr != rr) ? rr : v[0];  // a NaN sticks
+    }
   }
   block_partials<kTileThreads, 1, 1u>(A.site, v);
 }
@@ -673,22 +723,30 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st) {
   const double zb = (mixed && mg) ? 4.0 : 8.0;
 
   const uint32_t nblk = flat_blocks(D.I, D.J, D.K);
+  const uint32_t ncol = col_blocks(D.I, D.J, D.K);
+  const dim3 cgrd = col_grid(D.I, D.J, D.K);
   auto launch = [&](auto kern, int site, const char* name, double bytes, auto... extra) {
     A.site = red_site(c, site);
     timer_begin(c, name, bytes, &slot);
     hipLaunchKernelGGL(kern, grd, blk, 0, c->stream, A, extra...);
     timer_end(c, slot);
   };
-  auto finalize = [&](auto kern, int site) {
+  auto launch_col = [&](auto kern, int site, const char* name, double bytes) {
+    A.site = red_site(c, site);
+    timer_begin(c, name, bytes, &slot);
+    hipLaunchKernelGGL(kern, cgrd, blk, 0, c->stream, A);
+    timer_end(c, slot);
+  };
+  auto finalize = [&](auto kern, int site, uint32_t nb) {
     A.site = red_site(c, site);
     timer_begin(c, "cg_finalize", 0, &slot);
-    hipLaunchKernelGGL(kern, dim3(1), dim3(kFinThreads), 0, c->stream, A, nblk);
+    hipLaunchKernelGGL(kern, dim3(1), dim3(kFinThreads), 0, c->stream, A, nb);
     timer_end(c, slot);
   };
   auto enqueue_residual = [&](int from_zero) {
     launch(k_cg_residual, kSiteResidual, "cg_residual",
            NB * (from_zero ? 8 + 1 + 8 * 4 : 8 * 5 + eb + 1 + 8 * 4), from_zero);
-    finalize(k_cg_residual_fin, kSiteResidual);
+    finalize(k_cg_residual_fin, kSiteResidual, nblk);
   };
   auto precondition = [&]() {
     if (mg) {
@@ -722,13 +780,13 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st) {
     int cur = 0;
     while (it < max_iter) {
       precondition();
-      launch(k_cg_dots, kSiteDots, "cg_dots", NB * (8 + zb + 8 + 4 + 1));
-      finalize(k_cg_dots_fin, kSiteDots);
-      launch(k_cg_pupdate, kSiteDots, "cg_pupdate", NB * (zb + 8 + 8 + 1));
-      launch(k_cg_spmv, kSiteSpmv, "cg_spmv", NB * (8 + 24 + eb + 1 + 8));
-      finalize(k_cg_spmv_fin, kSiteSpmv);
-      launch(k_cg_update, kSiteUpdate, "cg_update", NB * (8 * 4 + 8 * 2 + (A.r32 ? 4 : 0)));
-      finalize(k_cg_update_fin, kSiteUpdate);
+      launch_col(k_cg_dots, kSiteDots, "cg_dots", NB * (8 + zb + 8 + 4 + 1));
+      finalize(k_cg_dots_fin, kSiteDots, ncol);
+      launch_col(k_cg_pupdate, kSiteDots, "cg_pupdate", NB * (zb + 8 + 8 + 1));
+      launch_col(k_cg_spmv, kSiteSpmv, "cg_spmv", NB * (8 + 24 + eb + 1 + 8));
+      finalize(k_cg_spmv_fin, kSiteSpmv, ncol);
+      launch_col(k_cg_update, kSiteUpdate, "cg_update", NB * (8 * 4 + 8 * 2 + (A.r32 ? 4 : 0)));
+      finalize(k_cg_update_fin, kSiteUpdate, ncol);
       hipEventRecord(evs[cur], c->stream);
       ++it;
       if (prev_pending) {
