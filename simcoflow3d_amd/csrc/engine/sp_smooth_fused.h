@@ -92,42 +92,45 @@ __device__ inline void load_plane(PlaneLoad<T>& L, const FusedArgs<T>& A, bool i
   }
 }
 
+// Register windows along k.  Logical plane o (0 = newest) of step u lives in
+// physical slot (o + W - u % W) % W, so advancing one plane renames slots
+// instead of moving registers (the march is unrolled by W).
 template <typename T, int NU, int MODE>
 struct FusedState {
   static constexpr int H = 2 * NU, W = H + 2;
-  T xw[W], bw[W], dw[W], czw[W], cxw[W], cxmw[W], cyw[W], cymw[W];
+  T xw[W], bw[W], dw[W], rdw[W], czw[W], cxw[W], cxmw[W], cyw[W], cymw[W];
   T racc;
 };
 
 // One pipeline step: plane t enters (from L), half-sweeps on the planes
 // behind it, output/residual of plane t - H, restriction after the barrier.
-template <typename T, int NU, int MODE>
+// U = position of this step in the W-step unrolled march.
+template <typename T, int NU, int MODE, int U>
 __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>& L,
                                   const FusedArgs<T>& A, T (*xs)[kFEY][kFEX], T (*rs)[kFEY][kFEX],
                                   int t, int k0, int k1, int i, int j, int tx, int ty, bool in_ij,
                                   bool tile, int64_t col, int64_t plane) {
   using Sh = FusedShape<NU>;
   constexpr int H = Sh::H, W = Sh::W, R = Sh::R;
-#pragma unroll
-  for (int o = W - 1; o > 0; --o) {
-    F.xw[o] = F.xw[o - 1]; F.bw[o] = F.bw[o - 1]; F.dw[o] = F.dw[o - 1]; F.czw[o] = F.czw[o - 1];
-    F.cxw[o] = F.cxw[o - 1]; F.cxmw[o] = F.cxmw[o - 1]; F.cyw[o] = F.cyw[o - 1];
-    F.cymw[o] = F.cymw[o - 1];
-  }
+  auto S = [](int o) constexpr { return (o + W - U) % W; };
+  // FP32 keeps a window of 1/d (one division per cell and plane); FP64 has no
+  // registers to spare for it and divides in every half-sweep
+  constexpr bool kRcp = sizeof(T) == 4;
   const T lb = L.vt ? L.b : T(0), lcx = L.vt ? L.cx : T(0), lcy = L.vt ? L.cy : T(0);
   const T lcz = L.vt ? L.cz : T(0), le = L.vt ? L.e : T(0);
   const T lcxm = (L.vt && L.vxm) ? L.cxm : T(0), lcym = (L.vt && L.vym) ? L.cym : T(0);
   T d0 = lcxm;
   d0 = d0 + lcym;
-  d0 = d0 + F.czw[1];
+  d0 = d0 + F.czw[S(1)];
   d0 = d0 + lcx;
   d0 = d0 + lcy;
   d0 = d0 + lcz;
   d0 = d0 + le;
   T x0 = T(0);
   if (MODE == 1 && d0 > T(0)) x0 = L.xin + L.xc;
-  F.bw[0] = lb; F.cxw[0] = lcx; F.cxmw[0] = lcxm; F.cyw[0] = lcy; F.cymw[0] = lcym;
-  F.czw[0] = lcz; F.dw[0] = d0; F.xw[0] = x0;
+  F.bw[S(0)] = lb; F.cxw[S(0)] = lcx; F.cxmw[S(0)] = lcxm; F.cyw[S(0)] = lcy;
+  F.cymw[S(0)] = lcym; F.czw[S(0)] = lcz; F.dw[S(0)] = d0; F.xw[S(0)] = x0;
+  if (kRcp) F.rdw[S(0)] = d0 > T(0) ? T(1) / d0 : T(0);
   xs[((t % R) + R) % R][ty][tx] = x0;
 
 #pragma unroll
@@ -135,10 +138,10 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
     const int o = MODE == 0 ? s - 1 : s;
     const int p = t - o;
     const int color = MODE == 0 ? ((s - 1) & 1) : (s & 1);
-    if (p < 0 || p >= A.nz || ((i + j + p) & 1) != color || !(F.dw[o] > T(0))) continue;
+    if (p < 0 || p >= A.nz || ((i + j + p) & 1) != color || !(F.dw[S(o)] > T(0))) continue;
     T v;
     if (MODE == 0 && s == 1) {
-      v = F.bw[0] / F.dw[0];
+      v = kRcp ? F.bw[S(0)] * F.rdw[S(0)] : F.bw[S(0)] / F.dw[S(0)];
     } else {
       const int slot = ((p % R) + R) % R;
       const T xW = tx > 0 ? xs[slot][ty][tx - 1] : T(0);
@@ -146,15 +149,15 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
       const T xE = tx < kFEX - 1 ? xs[slot][ty][tx + 1] : T(0);
       const T xN = ty < kFEY - 1 ? xs[slot][ty + 1][tx] : T(0);
       T acc = T(0);
-      acc += F.cxmw[o] * xW;
-      acc += F.cymw[o] * xS;
-      acc += F.czw[o + 1] * F.xw[o + 1];
-      acc += F.cxw[o] * xE;
-      acc += F.cyw[o] * xN;
-      acc += F.czw[o] * F.xw[o - 1];
-      v = (F.bw[o] + acc) / F.dw[o];
+      acc += F.cxmw[S(o)] * xW;
+      acc += F.cymw[S(o)] * xS;
+      acc += F.czw[S(o + 1)] * F.xw[S(o + 1)];
+      acc += F.cxw[S(o)] * xE;
+      acc += F.cyw[S(o)] * xN;
+      acc += F.czw[S(o)] * F.xw[S(o - 1)];
+      v = kRcp ? (F.bw[S(o)] + acc) * F.rdw[S(o)] : (F.bw[S(o)] + acc) / F.dw[S(o)];
     }
-    F.xw[o] = v;
+    F.xw[S(o)] = v;
     xs[((p % R) + R) % R][ty][tx] = v;
   }
 
@@ -163,24 +166,24 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
   if (MODE == 0) {
     T r = T(0);
     if (outq && tile && in_ij) {
-      if (F.dw[H] > T(0)) {
+      if (F.dw[S(H)] > T(0)) {
         const int slot = ((q % R) + R) % R;
         const T xW = xs[slot][ty][tx - 1], xS = xs[slot][ty - 1][tx];
         const T xE = xs[slot][ty][tx + 1], xN = xs[slot][ty + 1][tx];
         T acc = T(0);
-        acc += F.cxmw[H] * xW;
-        acc += F.cymw[H] * xS;
-        acc += F.czw[H + 1] * F.xw[H + 1];
-        acc += F.cxw[H] * xE;
-        acc += F.cyw[H] * xN;
-        acc += F.czw[H] * F.xw[H - 1];
-        r = F.bw[H] - (F.dw[H] * F.xw[H] - acc);
+        acc += F.cxmw[S(H)] * xW;
+        acc += F.cymw[S(H)] * xS;
+        acc += F.czw[S(H + 1)] * F.xw[S(H + 1)];
+        acc += F.cxw[S(H)] * xE;
+        acc += F.cyw[S(H)] * xN;
+        acc += F.czw[S(H)] * F.xw[S(H - 1)];
+        r = F.bw[S(H)] - (F.dw[S(H)] * F.xw[S(H)] - acc);
       }
-      A.xout[col + plane * q] = F.xw[H];
+      A.xout[col + plane * q] = F.xw[S(H)];
     }
     if (q >= 0) rs[q & 1][ty][tx] = r;
   } else if (outq && tile && in_ij) {
-    A.xout[col + plane * q] = F.xw[H];
+    A.xout[col + plane * q] = F.xw[S(H)];
   }
   lds_barrier();
   if (MODE == 0 && outq && tile && ((tx - H) & 1) == 0 && ((ty - H) & 1) == 0 && in_ij) {
@@ -193,6 +196,27 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
     F.racc = first ? ps : F.racc + ps;
     if (!first || q == A.nz - 1)
       A.bc[(i >> 1) + (int64_t)A.cnx * ((j >> 1) + (int64_t)A.cny * (q >> 1))] = F.racc;
+  }
+}
+
+// Steps t + U, t + U + 1, ..., t + W - 1 of the march (stopping at t1), each
+// followed by the load of the plane two steps ahead into the buffer it used.
+template <typename T, int NU, int MODE, bool HAS_E, int U>
+__device__ inline void march_steps(FusedState<T, NU, MODE>& F, PlaneLoad<T>& LA, PlaneLoad<T>& LB,
+                                   const FusedArgs<T>& A, T (*xs)[kFEY][kFEX],
+                                   T (*rs)[kFEY][kFEX], int t, int t1, int k0, int k1, int i,
+                                   int j, int tx, int ty, bool in_ij, bool tile, int64_t col,
+                                   int64_t plane) {
+  if constexpr (U < FusedShape<NU>::W) {
+    if (t + U >= t1) return;
+    // FP32: two planes of loads in flight; FP64: one (register budget)
+    constexpr int kPipe = sizeof(T) == 4 ? 2 : 1;
+    PlaneLoad<T>& L = (kPipe == 2 && (U & 1)) ? LB : LA;
+    fused_step<T, NU, MODE, U>(F, L, A, xs, rs, t + U, k0, k1, i, j, tx, ty, in_ij, tile, col,
+                               plane);
+    load_plane<T, MODE, HAS_E>(L, A, in_ij, i, j, col, plane, t + U + kPipe);
+    march_steps<T, NU, MODE, HAS_E, U + 1>(F, LA, LB, A, xs, rs, t, t1, k0, k1, i, j, tx, ty,
+                                           in_ij, tile, col, plane);
   }
 }
 
@@ -217,29 +241,30 @@ __global__ void __launch_bounds__(kFEX * kFEY) k_smooth_fused(FusedArgs<T> A, co
   FusedState<T, NU, MODE> F;
 #pragma unroll
   for (int o = 0; o < W; ++o) {
-    F.xw[o] = F.bw[o] = F.dw[o] = F.czw[o] = F.cxw[o] = F.cxmw[o] = F.cyw[o] = F.cymw[o] = T(0);
+    F.xw[o] = F.bw[o] = F.dw[o] = F.rdw[o] = F.czw[o] = F.cxw[o] = F.cxmw[o] = F.cyw[o] =
+        F.cymw[o] = T(0);
   }
   F.racc = T(0);
   for (int q = ty * kFEX + tx; q < R * kFEY * kFEX; q += kFEX * kFEY) (&xs[0][0][0])[q] = T(0);
   for (int q = ty * kFEX + tx; q < 2 * kFEY * kFEX; q += kFEX * kFEY) (&rs[0][0][0])[q] = T(0);
-  // cz of the plane below the first loaded one (bottom face of that plane)
+  // cz of the plane below the first loaded one (bottom face of that plane):
+  // logical plane 1 of the first step, physical slot 1
   {
     const int pb = k0 - H - 1;
-    if (in_ij && pb >= 0 && pb < A.nz) F.czw[0] = A.cz[col + plane * pb];
+    if (in_ij && pb >= 0 && pb < A.nz) F.czw[1] = A.cz[col + plane * pb];
   }
   __syncthreads();
 
-  // two planes of loads in flight: buffer LA holds even steps, LB odd steps
+  // FP32: two planes of loads in flight, buffer LA holds even steps, LB odd
+  // steps (W is even, so the parity of a step's position in the unrolled
+  // march is the parity of the step).  FP64: LA only.
   const int t0 = k0 - H, t1 = k1 + H;  // steps [t0, t1)
   PlaneLoad<T> LA, LB;
   load_plane<T, MODE, HAS_E>(LA, A, in_ij, i, j, col, plane, t0);
-  load_plane<T, MODE, HAS_E>(LB, A, in_ij, i, j, col, plane, t0 + 1);
-  for (int t = t0; t < t1; t += 2) {
-    fused_step<T, NU, MODE>(F, LA, A, xs, rs, t, k0, k1, i, j, tx, ty, in_ij, tile, col, plane);
-    load_plane<T, MODE, HAS_E>(LA, A, in_ij, i, j, col, plane, t + 2);
-    if (t + 1 < t1)
-      fused_step<T, NU, MODE>(F, LB, A, xs, rs, t + 1, k0, k1, i, j, tx, ty, in_ij, tile, col, plane);
-    load_plane<T, MODE, HAS_E>(LB, A, in_ij, i, j, col, plane, t + 3);
+  if (sizeof(T) == 4) load_plane<T, MODE, HAS_E>(LB, A, in_ij, i, j, col, plane, t0 + 1);
+  for (int t = t0; t < t1; t += W) {
+    march_steps<T, NU, MODE, HAS_E, 0>(F, LA, LB, A, xs, rs, t, t1, k0, k1, i, j, tx, ty, in_ij,
+                                       tile, col, plane);
   }
 }
 

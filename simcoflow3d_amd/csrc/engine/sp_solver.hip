@@ -94,7 +94,8 @@ __global__ void __launch_bounds__(256) k_gs(LevelView<T> L, int color, int zero_
   }
   T s = L.b[n];
   if (!zero_init) s += offdiag_sum(L, i, j, k, n);
-  L.x[n] = s / d;
+  // same rounding as the fused smoother (FP32: times 1/d, FP64: divided by d)
+  L.x[n] = sizeof(T) == 4 ? s * (T(1) / d) : s / d;
 }
 
 // Coarse b = sum over children of the fine residual.
