@@ -351,6 +351,10 @@ sp_status sp_create(sp_ctx** out, const sp_params* params, int device) {
     delete c;
     return SP_ERR_HIP;
   }
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+      ncu > 0)
+    c->num_cu = ncu;
   Dims& D = c->dm;
   D.I = params->imax; D.J = params->jmax; D.K = params->kmax;
   D.N = (int64_t)D.I * D.J * D.K;

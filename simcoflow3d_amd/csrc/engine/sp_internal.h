@@ -108,6 +108,7 @@ struct DeviceBuf {
 struct sp_ctx {
   sp_params prm;
   int device = 0;
+  int num_cu = 256;  // compute units of the device (workgroup sizing)
   hipStream_t stream = nullptr;
   sp::Dims dm{};
   bool grid_set = false;

@@ -562,11 +562,21 @@ void launch_fused(sp_ctx* c, Level<T>* lv, int l, int mode) {
   if (mode == 0) { A.xout = L.xs; A.bc = C.b; }
   else { A.xin = L.xs; A.xc = C.x; A.xout = L.x; }
   const int gx = (L.nx + Sh::TX - 1) / Sh::TX, gy = (L.ny + Sh::TY - 1) / Sh::TY;
-  // k chunk: even (coarse cells never straddle chunks), about 512 workgroups
-  int64_t kc = ((int64_t)L.nz * gx * gy + 511) / 512;
-  kc = std::max<int64_t>(kc, 16);
-  kc = (kc + 1) & ~int64_t(1);
-  A.kchunk = (int)std::min<int64_t>(kc, (L.nz + 1) & ~1);
+  // k chunk: even (coarse cells never straddle chunks).  One workgroup fills a
+  // CU, and a workgroup marches kchunk + 2H planes one after the other, so
+  // pick the chunk that minimises rounds-of-workgroups x planes marched.
+  {
+    const int64_t tiles = (int64_t)gx * gy, ncu = std::max(1, c->num_cu);
+    const int kmax = (L.nz + 1) & ~1;
+    int best = kmax;
+    int64_t best_cost = -1;
+    for (int kc = 2; kc <= kmax; kc += 2) {
+      const int64_t blocks = tiles * ((L.nz + kc - 1) / kc);
+      const int64_t cost = ((blocks + ncu - 1) / ncu) * (kc + 2 * Sh::H);
+      if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = kc; }
+    }
+    A.kchunk = best;
+  }
   const dim3 grd(gx, gy, (L.nz + A.kchunk - 1) / A.kchunk), blk(kFEX, kFEY, 1);
   const bool fine = l == 0;
   int slot;
