@@ -154,10 +154,18 @@ static sp_status ensure_buffers(sp_ctx* c) {
     L0.cx = c->cx; L0.cy = c->cy; L0.cz = c->cz; L0.e_store = c->ee; L0.e = c->ee;
     L0.b = c->r; L0.x = c->z64;
   }
-  c->tail_level = c->levels - 1;
-  for (int l = 0; l < c->levels; ++l) {
-    const int64_t n = mixed_mg ? c->lf[l].n : c->ld[l].n;
-    if (n <= kTailCells) { c->tail_level = l; break; }
+  // tail: the finest level of at most kTailCells cells from which every
+  // coarser level fits the tail kernel's LDS (seven arrays per level)
+  {
+    const int64_t esz = mixed_mg ? 4 : 8;
+    c->tail_level = c->levels - 1;
+    int64_t bytes = 0;
+    for (int l = c->levels - 1; l >= 0; --l) {
+      const int64_t n = mixed_mg ? c->lf[l].n : c->ld[l].n;
+      bytes += 7 * esz * n;
+      if (n > kTailCells || bytes > kTailLdsBytes) break;
+      c->tail_level = l;
+    }
   }
   if ((s = dmalloc(c, reinterpret_cast<char**>(&c->mg_desc), 128 * kMaxLevels))) return s;
   c->stale_cap = std::min<int64_t>(6 * N, std::max<int64_t>(1 << 20, N / 2));

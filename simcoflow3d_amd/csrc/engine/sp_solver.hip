@@ -133,42 +133,93 @@ __global__ void __launch_bounds__(256) k_prolong(LevelView<T> F, LevelView<T> C,
 
 // ---- single-workgroup V-cycle over the small levels ----
 
+// ---- coarse levels in one workgroup, resident in LDS ----
+
+// One tail level in LDS: couplings, diagonal and its reciprocal, b and x.
 template <typename T>
-__device__ void tail_gs(const LevelView<T>& L, int color, int zero_init) {
-  const int nxh = (L.nx + 1) / 2;
-  const int64_t half = (int64_t)nxh * L.ny * L.nz;
-  for (int64_t t = threadIdx.x; t < half; t += blockDim.x) {
-    const int ia = (int)(t % nxh);
-    const int j = (int)((t / nxh) % L.ny);
-    const int k = (int)(t / ((int64_t)nxh * L.ny));
-    const int par = (j + k + color) & 1;
-    const int i = 2 * ia + par;
+struct TailLevel {
+  int nx, ny, nz;
+  T *cx, *cy, *cz, *d, *rd, *b, *x;
+};
+constexpr int kTailArrays = 7;
+
+template <typename T>
+struct TailArgs {
+  int first, last;  // levels [first, last)
+  int pre, post, coarse_sweeps, singular, has_e;
+  const LevelView<T>* lv;   // the levels in global memory
+  int off[kMaxLevels];      // element offset of each tail level in LDS
+};
+
+template <typename T>
+__device__ inline TailLevel<T> tail_level(T* lds, const TailArgs<T>& A, int l) {
+  const LevelView<T>& G = A.lv[l];
+  const int n = G.nx * G.ny * G.nz;
+  T* base = lds + A.off[l];
+  return TailLevel<T>{G.nx, G.ny, G.nz, base, base + n, base + 2 * n, base + 3 * n,
+                      base + 4 * n, base + 5 * n, base + 6 * n};
+}
+
+// (i, j, k) of cell n (32-bit index arithmetic)
+struct Ijk {
+  int i, j, k;
+};
+__device__ inline Ijk ijk_of(unsigned n, unsigned nx, unsigned nxy) {
+  const unsigned k = n / nxy, r = n - k * nxy, j = r / nx;
+  return Ijk{(int)(r - j * nx), (int)j, (int)k};
+}
+
+template <typename T>
+__device__ inline T tail_offdiag(const TailLevel<T>& L, int i, int j, int k, int n) {
+  const int sy = L.nx, sz = L.nx * L.ny;
+  T s = T(0);
+  if (i > 0) s += L.cx[n - 1] * L.x[n - 1];
+  if (j > 0) s += L.cy[n - sy] * L.x[n - sy];
+  if (k > 0) s += L.cz[n - sz] * L.x[n - sz];
+  if (i < L.nx - 1) s += L.cx[n] * L.x[n + 1];
+  if (j < L.ny - 1) s += L.cy[n] * L.x[n + sy];
+  if (k < L.nz - 1) s += L.cz[n] * L.x[n + sz];
+  return s;
+}
+
+// One colour of Gauss-Seidel (one thread per cell pair along i); zero_init:
+// x starts at 0 and the other colour is written as 0.  FP32 multiplies by
+// 1/d, FP64 divides by d, as the other smoothers do.
+template <typename T>
+__device__ void tail_gs(const TailLevel<T>& L, int color, int zero_init) {
+  const unsigned nxh = (unsigned)(L.nx + 1) / 2, nxy = nxh * (unsigned)L.ny;
+  const unsigned half = nxy * (unsigned)L.nz;
+  for (unsigned t = threadIdx.x; t < half; t += blockDim.x) {
+    const Ijk q = ijk_of(t, nxh, nxy);
+    const int par = (q.j + q.k + color) & 1;
+    const int i = 2 * q.i + par;
+    const int row = L.nx * (q.j + L.ny * q.k);
     if (zero_init) {
-      const int io = 2 * ia + (par ^ 1);
-      if (io < L.nx) L.x[io + (int64_t)L.nx * (j + (int64_t)L.ny * k)] = T(0);
+      const int io = 2 * q.i + (par ^ 1);
+      if (io < L.nx) L.x[row + io] = T(0);
     }
     if (i >= L.nx) continue;
-    const int64_t n = i + (int64_t)L.nx * (j + (int64_t)L.ny * k);
-    const T d = diag_of(L, i, j, k, n);
+    const int n = row + i;
+    const T d = L.d[n];
     if (!(d > T(0))) {
       if (zero_init) L.x[n] = T(0);
       continue;
     }
     T s = L.b[n];
-    if (!zero_init) s += offdiag_sum(L, i, j, k, n);
-    L.x[n] = s / d;
+    if (!zero_init) s += tail_offdiag(L, i, q.j, q.k, n);
+    L.x[n] = sizeof(T) == 4 ? s * L.rd[n] : s / d;
   }
   __syncthreads();
 }
 
+// v -= mean of v over the cells with d > 0
 template <typename T>
-__device__ void tail_project_mean(const LevelView<T>& L, T* v) {
+__device__ void tail_project_mean(const TailLevel<T>& L, T* v) {
   __shared__ double red[2][16];
   double s = 0.0, c = 0.0;
-  const int64_t nn = (int64_t)L.nx * L.ny * L.nz;
-  for (int64_t n = threadIdx.x; n < nn; n += blockDim.x) {
-    const int i = (int)(n % L.nx), j = (int)((n / L.nx) % L.ny), k = (int)(n / ((int64_t)L.nx * L.ny));
-    if (diag_of(L, i, j, k, n) > T(0)) { s += (double)v[n]; c += 1.0; }
+  const int nn = L.nx * L.ny * L.nz;
+  for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+    if (L.d[n] > T(0)) { s += (double)v[n]; c += 1.0; }
   }
   s = wave_sum(s);
   c = wave_sum(c);
@@ -178,27 +229,27 @@ __device__ void tail_project_mean(const LevelView<T>& L, T* v) {
   for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { ts += red[0][w]; tc += red[1][w]; }
   const T mean = tc > 0 ? (T)(ts / tc) : T(0);
   __syncthreads();
-  for (int64_t n = threadIdx.x; n < nn; n += blockDim.x) {
-    const int i = (int)(n % L.nx), j = (int)((n / L.nx) % L.ny), k = (int)(n / ((int64_t)L.nx * L.ny));
-    if (diag_of(L, i, j, k, n) > T(0)) v[n] -= mean;
+  for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+    if (L.d[n] > T(0)) v[n] -= mean;
   }
   __syncthreads();
 }
 
+// coarse b = sum over the 2x2x2 children of the fine residual
 template <typename T>
-__device__ void tail_restrict(const LevelView<T>& F, const LevelView<T>& C) {
-  const int64_t nc = (int64_t)C.nx * C.ny * C.nz;
-  for (int64_t t = threadIdx.x; t < nc; t += blockDim.x) {
-    const int i = (int)(t % C.nx), j = (int)((t / C.nx) % C.ny), k = (int)(t / ((int64_t)C.nx * C.ny));
+__device__ void tail_restrict(const TailLevel<T>& F, const TailLevel<T>& C) {
+  const unsigned nxy = (unsigned)C.nx * C.ny, nc = nxy * (unsigned)C.nz;
+  for (unsigned t = threadIdx.x; t < nc; t += blockDim.x) {
+    const Ijk q = ijk_of(t, (unsigned)C.nx, nxy);
     T s = T(0);
     for (int cc = 0; cc < 2; ++cc)
       for (int bb = 0; bb < 2; ++bb)
         for (int aa = 0; aa < 2; ++aa) {
-          const int fi = 2 * i + aa, fj = 2 * j + bb, fk = 2 * k + cc;
+          const int fi = 2 * q.i + aa, fj = 2 * q.j + bb, fk = 2 * q.k + cc;
           if (fi >= F.nx || fj >= F.ny || fk >= F.nz) continue;
-          const int64_t m = fi + (int64_t)F.nx * (fj + (int64_t)F.ny * fk);
-          const T d = diag_of(F, fi, fj, fk, m);
-          if (d > T(0)) s += F.b[m] - (d * F.x[m] - offdiag_sum(F, fi, fj, fk, m));
+          const int m = fi + F.nx * (fj + F.ny * fk);
+          const T d = F.d[m];
+          if (d > T(0)) s += F.b[m] - (d * F.x[m] - tail_offdiag(F, fi, fj, fk, m));
         }
     C.b[t] = s;
   }
@@ -206,36 +257,67 @@ __device__ void tail_restrict(const LevelView<T>& F, const LevelView<T>& C) {
 }
 
 template <typename T>
-__device__ void tail_prolong(const LevelView<T>& F, const LevelView<T>& C) {
-  const int64_t nf = (int64_t)F.nx * F.ny * F.nz;
-  for (int64_t m = threadIdx.x; m < nf; m += blockDim.x) {
-    const int i = (int)(m % F.nx), j = (int)((m / F.nx) % F.ny), k = (int)(m / ((int64_t)F.nx * F.ny));
-    if (diag_of(F, i, j, k, m) > T(0))
-      F.x[m] += C.x[(i >> 1) + (int64_t)C.nx * ((j >> 1) + (int64_t)C.ny * (k >> 1))];
+__device__ void tail_prolong(const TailLevel<T>& F, const TailLevel<T>& C) {
+  const unsigned nxy = (unsigned)F.nx * F.ny, nf = nxy * (unsigned)F.nz;
+  for (unsigned m = threadIdx.x; m < nf; m += blockDim.x) {
+    const Ijk q = ijk_of(m, (unsigned)F.nx, nxy);
+    if (F.d[m] > T(0))
+      F.x[m] += C.x[(q.i >> 1) + C.nx * ((q.j >> 1) + C.ny * (q.k >> 1))];
   }
   __syncthreads();
 }
 
-template <typename T>
-struct TailArgs {
-  int first, last;  // levels [first, last)
-  int pre, post, coarse_sweeps, singular;
-  const LevelView<T>* lv;
-};
-
+// The coarse levels of the V-cycle in one workgroup, resident in LDS: the
+// couplings of every tail level and the right-hand side of the first are
+// copied in (the diagonal and 1/d are formed once), the whole sub-cycle
+// runs on LDS, and x of the first tail level is written back for the
+// prolongation above it.
 template <typename T>
 __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A, const Scalars* S) {
   if (S->done) return;
+  __shared__ double lds_raw[kTailLdsBytes / sizeof(double)];
+  T* lds = reinterpret_cast<T*>(lds_raw);
   const int L = A.last;
+  for (int l = A.first; l < L; ++l) {
+    const LevelView<T> G = A.lv[l];
+    const TailLevel<T> V = tail_level(lds, A, l);
+    const int n = G.nx * G.ny * G.nz;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+      V.cx[q] = G.cx[q];
+      V.cy[q] = G.cy[q];
+      V.cz[q] = G.cz[q];
+      V.d[q] = A.has_e ? G.e[q] : T(0);  // diagonal-only term, completed below
+      if (l == A.first) V.b[q] = G.b[q];
+    }
+  }
+  __syncthreads();
+  for (int l = A.first; l < L; ++l) {
+    const TailLevel<T> V = tail_level(lds, A, l);
+    const unsigned nxy = (unsigned)V.nx * V.ny, n = nxy * (unsigned)V.nz;
+    for (unsigned m = threadIdx.x; m < n; m += blockDim.x) {
+      const Ijk q = ijk_of(m, (unsigned)V.nx, nxy);
+      // same order as the other kernels: W, S, B, E, N, T, then e
+      T d = q.i > 0 ? V.cx[m - 1] : T(0);
+      d = d + (q.j > 0 ? V.cy[m - V.nx] : T(0));
+      d = d + (q.k > 0 ? V.cz[m - nxy] : T(0));
+      d = d + V.cx[m];
+      d = d + V.cy[m];
+      d = d + V.cz[m];
+      d = d + V.d[m];
+      V.d[m] = d;
+      V.rd[m] = d > T(0) ? T(1) / d : T(0);
+    }
+  }
+  __syncthreads();
   for (int l = A.first; l < L - 1; ++l) {
-    const LevelView<T> F = A.lv[l];
+    const TailLevel<T> F = tail_level(lds, A, l);
     tail_gs(F, 0, 1);
     tail_gs(F, 1, 0);
     for (int s = 1; s < A.pre; ++s) { tail_gs(F, 0, 0); tail_gs(F, 1, 0); }
-    tail_restrict(F, A.lv[l + 1]);
+    tail_restrict(F, tail_level(lds, A, l + 1));
   }
   {
-    const LevelView<T> C = A.lv[L - 1];
+    const TailLevel<T> C = tail_level(lds, A, L - 1);
     if (A.singular) tail_project_mean(C, C.b);
     tail_gs(C, 0, 1);
     tail_gs(C, 1, 0);
@@ -244,9 +326,15 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A, const Scalars* S) 
     if (A.singular) tail_project_mean(C, C.x);
   }
   for (int l = L - 2; l >= A.first; --l) {
-    const LevelView<T> F = A.lv[l];
-    tail_prolong(F, A.lv[l + 1]);
+    const TailLevel<T> F = tail_level(lds, A, l);
+    tail_prolong(F, tail_level(lds, A, l + 1));
     for (int s = 0; s < A.post; ++s) { tail_gs(F, 1, 0); tail_gs(F, 0, 0); }
+  }
+  {
+    const LevelView<T> G = A.lv[A.first];
+    const TailLevel<T> V = tail_level(lds, A, A.first);
+    const int n = G.nx * G.ny * G.nz;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) G.x[q] = V.x[q];
   }
 }
 
@@ -641,7 +729,13 @@ void vcycle(sp_ctx* c, Level<T>* lv) {
     A.post = post;
     A.coarse_sweeps = std::max(1, c->prm.mg_coarse_sweeps);
     A.singular = c->S_host->singular;
+    A.has_e = lv[Lt].e ? 1 : 0;
     A.lv = reinterpret_cast<const LevelView<T>*>(c->mg_desc);
+    int64_t off = 0;
+    for (int l = 0; l < kMaxLevels; ++l) {
+      A.off[l] = (int)off;
+      if (l >= Lt && l < L) off += lv[l].n * kTailArrays;
+    }
     timer_begin(c, "mg_tail", 0, &slot);
     hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(1024), 0, c->stream, A, (const Scalars*)c->S);
     timer_end(c, slot);
