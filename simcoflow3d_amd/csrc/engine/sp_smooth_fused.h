@@ -15,6 +15,13 @@
 //   post (MODE 1): x = xin + P xc (active cells); black, red, black, red ...;
 //                  x is written to xout.
 // The neighbour sums use the order of the plain kernels (W,S,B,E,N,T).
+//
+// In both modes the cell (i,j) is updated at step t exactly when i + j + t is
+// even (half-sweep s touches plane t - s + 1 with colour s - 1, or plane
+// t - s with colour s), so one predicate per step replaces the per-half-sweep
+// colour tests, and all lanes run the same instructions.  The march is
+// unrolled by the window length W; with the chunk start t0 even and R = W,
+// every register-window slot and LDS ring slot is a compile-time constant.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -36,7 +43,7 @@ struct FusedArgs {
   T* xout;
   T* bc;         // pre: coarse right-hand side
   int cnx, cny;
-  int kchunk;
+  int kchunk;    // even
 };
 
 template <int NU>
@@ -45,7 +52,7 @@ struct FusedShape {
   static constexpr int TX = kFEX - 2 * H;
   static constexpr int TY = kFEY - 2 * H;
   static constexpr int W = H + 2;  // register window (planes)
-  static constexpr int R = H + 2;  // LDS ring (planes)
+  static constexpr int R = W;      // LDS ring (planes)
 };
 
 // Barrier that orders LDS only: global loads issued for later planes stay in
@@ -57,6 +64,17 @@ __device__ inline void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Element at a 32-bit byte offset (every level array is below 4 GiB), so the
+// loads use a scalar base plus one 32-bit vector offset.
+template <typename T>
+__device__ inline T at_off(const T* base, uint32_t byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <typename T>
+__device__ inline void put_off(T* base, uint32_t byte_off, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
 // Raw loads of one plane.  The loads always target a valid address (the
 // plane index is clamped, neighbours fall back to the cell itself) so their
 // destination registers carry straight into the next pipeline step; validity
@@ -64,28 +82,36 @@ __device__ inline void lds_barrier() {
 template <typename T>
 struct PlaneLoad {
   T b, cx, cy, cz, cxm, cym, e, xin, xc;
-  bool vt, vxm, vym;
+  bool vt;
+};
+
+// Per-thread constants of the march.
+struct FusedCol {
+  int i, j, tx, ty;
+  bool in_ij, tile, vxm, vym;
+  bool act_even;         // updated at even steps (i + j even)
+  uint32_t col;          // byte offset of the (i, j) column in plane 0
+  uint32_t plane;        // bytes per plane
+  uint32_t dxm, dym;     // byte distance to the -x / -y neighbour (0 at the edge)
+  uint32_t ccol;         // post: byte offset of the coarse column
+  uint32_t cplane;       // post: bytes per coarse plane
 };
 
 template <typename T, int MODE, bool HAS_E>
-__device__ inline void load_plane(PlaneLoad<T>& L, const FusedArgs<T>& A, bool in_ij, int i, int j,
-                                  int64_t col, int64_t plane, int t) {
+__device__ inline void load_plane(PlaneLoad<T>& L, const FusedArgs<T>& A, const FusedCol& C, int t) {
   const int tc = t < 0 ? 0 : (t >= A.nz ? A.nz - 1 : t);
-  const int64_t n = col + plane * tc;
-  L.vt = in_ij && t >= 0 && t < A.nz;
-  L.vxm = i > 0;
-  L.vym = j > 0;
-  L.b = A.b[n];
-  L.cx = A.cx[n];
-  L.cy = A.cy[n];
-  L.cz = A.cz[n];
-  L.cxm = A.cx[L.vxm && in_ij ? n - 1 : n];
-  L.cym = A.cy[L.vym && in_ij ? n - A.nx : n];
-  L.e = HAS_E ? A.e[n] : T(0);
+  const uint32_t n = C.col + C.plane * (uint32_t)tc;
+  L.vt = C.in_ij && t >= 0 && t < A.nz;
+  L.b = at_off(A.b, n);
+  L.cx = at_off(A.cx, n);
+  L.cy = at_off(A.cy, n);
+  L.cz = at_off(A.cz, n);
+  L.cxm = at_off(A.cx, n - C.dxm);
+  L.cym = at_off(A.cy, n - C.dym);
+  L.e = HAS_E ? at_off(A.e, n) : T(0);
   if (MODE == 1) {
-    L.xin = A.xin[n];
-    const int ic = in_ij ? (i >> 1) : 0, jc = in_ij ? (j >> 1) : 0;
-    L.xc = A.xc[ic + (int64_t)A.cnx * (jc + (int64_t)A.cny * (tc >> 1))];
+    L.xin = at_off(A.xin, n);
+    L.xc = at_off(A.xc, C.ccol + C.cplane * (uint32_t)(tc >> 1));
   } else {
     L.xin = T(0);
     L.xc = T(0);
@@ -104,21 +130,27 @@ struct FusedState {
 
 // One pipeline step: plane t enters (from L), half-sweeps on the planes
 // behind it, output/residual of plane t - H, restriction after the barrier.
-// U = position of this step in the W-step unrolled march.
+// U = position of this step in the W-step unrolled march (t - t0 = U mod W).
 template <typename T, int NU, int MODE, int U>
 __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>& L,
-                                  const FusedArgs<T>& A, T (*xs)[kFEY][kFEX], T (*rs)[kFEY][kFEX],
-                                  int t, int k0, int k1, int i, int j, int tx, int ty, bool in_ij,
-                                  bool tile, int64_t col, int64_t plane) {
+                                  const FusedArgs<T>& A, const FusedCol& C,
+                                  T (*xs)[kFEY + 2][kFEX + 2], T (*rs)[kFEY][kFEX], int t, int k0, int k1) {
   using Sh = FusedShape<NU>;
   constexpr int H = Sh::H, W = Sh::W, R = Sh::R;
-  auto S = [](int o) constexpr { return (o + W - U) % W; };
+  static_assert(R == W && (W % 2) == 0, "ring slots are compile-time only for R == W, W even");
+  auto S = [](int o) constexpr { return (o + W - U) % W; };        // register slot
+  auto Q = [](int o) constexpr { return ((U - o) % R + R) % R; };  // LDS ring slot
   // FP32 keeps a window of 1/d (one division per cell and plane); FP64 has no
   // registers to spare for it and divides in every half-sweep
   constexpr bool kRcp = sizeof(T) == 4;
+  const int tx = C.tx, ty = C.ty;
+  // LDS planes carry a one-cell border of zeros: cell (tx, ty) is at
+  // [ty + 1][tx + 1] and every neighbour read is unconditional
+  const int lx = tx + 1, ly = ty + 1;
+
   const T lb = L.vt ? L.b : T(0), lcx = L.vt ? L.cx : T(0), lcy = L.vt ? L.cy : T(0);
   const T lcz = L.vt ? L.cz : T(0), le = L.vt ? L.e : T(0);
-  const T lcxm = (L.vt && L.vxm) ? L.cxm : T(0), lcym = (L.vt && L.vym) ? L.cym : T(0);
+  const T lcxm = (L.vt && C.vxm) ? L.cxm : T(0), lcym = (L.vt && C.vym) ? L.cym : T(0);
   T d0 = lcxm;
   d0 = d0 + lcym;
   d0 = d0 + F.czw[S(1)];
@@ -127,27 +159,25 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
   d0 = d0 + lcz;
   d0 = d0 + le;
   T x0 = T(0);
-  if (MODE == 1 && d0 > T(0)) x0 = L.xin + L.xc;
+  if (MODE == 1) x0 = d0 > T(0) ? L.xin + L.xc : T(0);
   F.bw[S(0)] = lb; F.cxw[S(0)] = lcx; F.cxmw[S(0)] = lcxm; F.cyw[S(0)] = lcy;
   F.cymw[S(0)] = lcym; F.czw[S(0)] = lcz; F.dw[S(0)] = d0; F.xw[S(0)] = x0;
   if (kRcp) F.rdw[S(0)] = d0 > T(0) ? T(1) / d0 : T(0);
-  xs[((t % R) + R) % R][ty][tx] = x0;
+  xs[Q(0)][ly][lx] = x0;
 
+  // cells of this thread's colour move this step (t0 even, W even)
+  const bool act = (U & 1) ? !C.act_even : C.act_even;
 #pragma unroll
   for (int s = 1; s <= H; ++s) {
     const int o = MODE == 0 ? s - 1 : s;
     const int p = t - o;
-    const int color = MODE == 0 ? ((s - 1) & 1) : (s & 1);
-    if (p < 0 || p >= A.nz || ((i + j + p) & 1) != color || !(F.dw[S(o)] > T(0))) continue;
+    if (p < 0 || p >= A.nz) continue;  // uniform over the workgroup
     T v;
     if (MODE == 0 && s == 1) {
       v = kRcp ? F.bw[S(0)] * F.rdw[S(0)] : F.bw[S(0)] / F.dw[S(0)];
     } else {
-      const int slot = ((p % R) + R) % R;
-      const T xW = tx > 0 ? xs[slot][ty][tx - 1] : T(0);
-      const T xS = ty > 0 ? xs[slot][ty - 1][tx] : T(0);
-      const T xE = tx < kFEX - 1 ? xs[slot][ty][tx + 1] : T(0);
-      const T xN = ty < kFEY - 1 ? xs[slot][ty + 1][tx] : T(0);
+      const T xW = xs[Q(o)][ly][lx - 1], xS = xs[Q(o)][ly - 1][lx];
+      const T xE = xs[Q(o)][ly][lx + 1], xN = xs[Q(o)][ly + 1][lx];
       T acc = T(0);
       acc += F.cxmw[S(o)] * xW;
       acc += F.cymw[S(o)] * xS;
@@ -157,66 +187,66 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
       acc += F.czw[S(o)] * F.xw[S(o - 1)];
       v = kRcp ? (F.bw[S(o)] + acc) * F.rdw[S(o)] : (F.bw[S(o)] + acc) / F.dw[S(o)];
     }
-    F.xw[S(o)] = v;
-    xs[((p % R) + R) % R][ty][tx] = v;
+    const T xn = (act && F.dw[S(o)] > T(0)) ? v : F.xw[S(o)];
+    F.xw[S(o)] = xn;
+    xs[Q(o)][ly][lx] = xn;
   }
 
   const int q = t - H;
   const bool outq = q >= k0 && q < k1;
+  const bool wr = outq && C.tile && C.in_ij;
   if (MODE == 0) {
     T r = T(0);
-    if (outq && tile && in_ij) {
-      if (F.dw[S(H)] > T(0)) {
-        const int slot = ((q % R) + R) % R;
-        const T xW = xs[slot][ty][tx - 1], xS = xs[slot][ty - 1][tx];
-        const T xE = xs[slot][ty][tx + 1], xN = xs[slot][ty + 1][tx];
-        T acc = T(0);
-        acc += F.cxmw[S(H)] * xW;
-        acc += F.cymw[S(H)] * xS;
-        acc += F.czw[S(H + 1)] * F.xw[S(H + 1)];
-        acc += F.cxw[S(H)] * xE;
-        acc += F.cyw[S(H)] * xN;
-        acc += F.czw[S(H)] * F.xw[S(H - 1)];
-        r = F.bw[S(H)] - (F.dw[S(H)] * F.xw[S(H)] - acc);
-      }
-      A.xout[col + plane * q] = F.xw[S(H)];
+    if (wr && F.dw[S(H)] > T(0)) {
+      const T xW = xs[Q(H)][ly][lx - 1], xS = xs[Q(H)][ly - 1][lx];
+      const T xE = xs[Q(H)][ly][lx + 1], xN = xs[Q(H)][ly + 1][lx];
+      T acc = T(0);
+      acc += F.cxmw[S(H)] * xW;
+      acc += F.cymw[S(H)] * xS;
+      acc += F.czw[S(H + 1)] * F.xw[S(H + 1)];
+      acc += F.cxw[S(H)] * xE;
+      acc += F.cyw[S(H)] * xN;
+      acc += F.czw[S(H)] * F.xw[S(H - 1)];
+      r = F.bw[S(H)] - (F.dw[S(H)] * F.xw[S(H)] - acc);
     }
-    if (q >= 0) rs[q & 1][ty][tx] = r;
-  } else if (outq && tile && in_ij) {
-    A.xout[col + plane * q] = F.xw[S(H)];
+    if (wr) put_off(A.xout, C.col + C.plane * (uint32_t)q, F.xw[S(H)]);
+    // q = t0 + U - H (mod W) with t0 and H even: parity of q = parity of U
+    if (q >= 0) rs[U & 1][ty][tx] = r;
+  } else if (wr) {
+    put_off(A.xout, C.col + C.plane * (uint32_t)q, F.xw[S(H)]);
   }
   lds_barrier();
-  if (MODE == 0 && outq && tile && ((tx - H) & 1) == 0 && ((ty - H) & 1) == 0 && in_ij) {
-    const int sl = q & 1;
+  if (MODE == 0 && wr && ((tx - H) & 1) == 0 && ((ty - H) & 1) == 0) {
+    constexpr int sl = U & 1;
     T ps = rs[sl][ty][tx];
     ps += rs[sl][ty][tx + 1];
     ps += rs[sl][ty + 1][tx];
     ps += rs[sl][ty + 1][tx + 1];
-    const bool first = (q & 1) == 0;
+    constexpr bool first = (U & 1) == 0;  // q even
     F.racc = first ? ps : F.racc + ps;
-    if (!first || q == A.nz - 1)
-      A.bc[(i >> 1) + (int64_t)A.cnx * ((j >> 1) + (int64_t)A.cny * (q >> 1))] = F.racc;
+    if (!first || q == A.nz - 1) {
+      const uint32_t cidx = (uint32_t)(C.i >> 1) +
+                            (uint32_t)A.cnx * ((uint32_t)(C.j >> 1) + (uint32_t)A.cny * (uint32_t)(q >> 1));
+      A.bc[cidx] = F.racc;
+    }
   }
 }
 
 // Steps t + U, t + U + 1, ..., t + W - 1 of the march (stopping at t1), each
-// followed by the load of the plane two steps ahead into the buffer it used.
+// followed by the load of a later plane into the buffer it used.
 template <typename T, int NU, int MODE, bool HAS_E, int U>
 __device__ inline void march_steps(FusedState<T, NU, MODE>& F, PlaneLoad<T>& LA, PlaneLoad<T>& LB,
-                                   const FusedArgs<T>& A, T (*xs)[kFEY][kFEX],
-                                   T (*rs)[kFEY][kFEX], int t, int t1, int k0, int k1, int i,
-                                   int j, int tx, int ty, bool in_ij, bool tile, int64_t col,
-                                   int64_t plane) {
+                                   const FusedArgs<T>& A, const FusedCol& C,
+                                   T (*xs)[kFEY + 2][kFEX + 2], T (*rs)[kFEY][kFEX], int t, int t1,
+                                   int k0, int k1) {
   if constexpr (U < FusedShape<NU>::W) {
     if (t + U >= t1) return;
     // FP32: two planes of loads in flight; FP64: one (register budget)
     constexpr int kPipe = sizeof(T) == 4 ? 2 : 1;
     PlaneLoad<T>& L = (kPipe == 2 && (U & 1)) ? LB : LA;
-    fused_step<T, NU, MODE, U>(F, L, A, xs, rs, t + U, k0, k1, i, j, tx, ty, in_ij, tile, col,
-                               plane);
-    load_plane<T, MODE, HAS_E>(L, A, in_ij, i, j, col, plane, t + U + kPipe);
-    march_steps<T, NU, MODE, HAS_E, U + 1>(F, LA, LB, A, xs, rs, t, t1, k0, k1, i, j, tx, ty,
-                                           in_ij, tile, col, plane);
+    fused_step<T, NU, MODE, U>(F, L, A, C, xs, rs, t + U, k0, k1);
+    load_plane<T, MODE, HAS_E>(L, A, C, t + U + kPipe);
+    march_steps<T, NU, MODE, HAS_E, U + 1>(F, LA, LB, A, C, xs, rs, t, t1, k0, k1);
   }
 }
 
@@ -225,18 +255,28 @@ __global__ void __launch_bounds__(kFEX * kFEY) k_smooth_fused(FusedArgs<T> A, co
   if (S->done) return;
   using Sh = FusedShape<NU>;
   constexpr int H = Sh::H, TX = Sh::TX, TY = Sh::TY, W = Sh::W, R = Sh::R;
-  __shared__ T xs[R][kFEY][kFEX];
+  __shared__ T xs[R][kFEY + 2][kFEX + 2];
   __shared__ T rs[2][kFEY][kFEX];
   uint32_t bx, by, bz;
   xcd_block3(bx, by, bz);
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i = (int)bx * TX - H + tx, j = (int)by * TY - H + ty;
-  const bool in_ij = i >= 0 && i < A.nx && j >= 0 && j < A.ny;
-  const bool tile = tx >= H && tx < H + TX && ty >= H && ty < H + TY;
+  FusedCol C;
+  C.tx = threadIdx.x;
+  C.ty = threadIdx.y;
+  C.i = (int)bx * TX - H + C.tx;
+  C.j = (int)by * TY - H + C.ty;
+  C.in_ij = C.i >= 0 && C.i < A.nx && C.j >= 0 && C.j < A.ny;
+  C.tile = C.tx >= H && C.tx < H + TX && C.ty >= H && C.ty < H + TY;
+  C.vxm = C.i > 0;
+  C.vym = C.j > 0;
+  C.act_even = ((C.i + C.j) & 1) == 0;
+  C.col = C.in_ij ? (uint32_t)(C.i + A.nx * C.j) * (uint32_t)sizeof(T) : 0u;
+  C.plane = (uint32_t)A.nx * (uint32_t)A.ny * (uint32_t)sizeof(T);
+  C.dxm = (C.in_ij && C.vxm) ? (uint32_t)sizeof(T) : 0u;
+  C.dym = (C.in_ij && C.vym) ? (uint32_t)(A.nx * sizeof(T)) : 0u;
+  C.ccol = C.in_ij ? (uint32_t)((C.i >> 1) + A.cnx * (C.j >> 1)) * (uint32_t)sizeof(T) : 0u;
+  C.cplane = (uint32_t)A.cnx * (uint32_t)A.cny * (uint32_t)sizeof(T);
   const int k0 = (int)bz * A.kchunk;
   const int k1 = min(k0 + A.kchunk, A.nz);
-  const int64_t plane = (int64_t)A.nx * A.ny;
-  const int64_t col = in_ij ? (int64_t)i + (int64_t)A.nx * j : 0;
 
   FusedState<T, NU, MODE> F;
 #pragma unroll
@@ -245,27 +285,26 @@ __global__ void __launch_bounds__(kFEX * kFEY) k_smooth_fused(FusedArgs<T> A, co
         F.cymw[o] = T(0);
   }
   F.racc = T(0);
-  for (int q = ty * kFEX + tx; q < R * kFEY * kFEX; q += kFEX * kFEY) (&xs[0][0][0])[q] = T(0);
-  for (int q = ty * kFEX + tx; q < 2 * kFEY * kFEX; q += kFEX * kFEY) (&rs[0][0][0])[q] = T(0);
+  for (int q = C.ty * kFEX + C.tx; q < R * (kFEY + 2) * (kFEX + 2); q += kFEX * kFEY)
+    (&xs[0][0][0])[q] = T(0);
+  for (int q = C.ty * kFEX + C.tx; q < 2 * kFEY * kFEX; q += kFEX * kFEY) (&rs[0][0][0])[q] = T(0);
   // cz of the plane below the first loaded one (bottom face of that plane):
   // logical plane 1 of the first step, physical slot 1
   {
     const int pb = k0 - H - 1;
-    if (in_ij && pb >= 0 && pb < A.nz) F.czw[1] = A.cz[col + plane * pb];
+    if (C.in_ij && pb >= 0 && pb < A.nz) F.czw[1] = at_off(A.cz, C.col + C.plane * (uint32_t)pb);
   }
   __syncthreads();
 
   // FP32: two planes of loads in flight, buffer LA holds even steps, LB odd
   // steps (W is even, so the parity of a step's position in the unrolled
   // march is the parity of the step).  FP64: LA only.
-  const int t0 = k0 - H, t1 = k1 + H;  // steps [t0, t1)
+  const int t0 = k0 - H, t1 = k1 + H;  // steps [t0, t1); t0 even
   PlaneLoad<T> LA, LB;
-  load_plane<T, MODE, HAS_E>(LA, A, in_ij, i, j, col, plane, t0);
-  if (sizeof(T) == 4) load_plane<T, MODE, HAS_E>(LB, A, in_ij, i, j, col, plane, t0 + 1);
-  for (int t = t0; t < t1; t += W) {
-    march_steps<T, NU, MODE, HAS_E, 0>(F, LA, LB, A, xs, rs, t, t1, k0, k1, i, j, tx, ty, in_ij,
-                                       tile, col, plane);
-  }
+  load_plane<T, MODE, HAS_E>(LA, A, C, t0);
+  if (sizeof(T) == 4) load_plane<T, MODE, HAS_E>(LB, A, C, t0 + 1);
+  for (int t = t0; t < t1; t += W)
+    march_steps<T, NU, MODE, HAS_E, 0>(F, LA, LB, A, C, xs, rs, t, t1, k0, k1);
 }
 
 }  // namespace sp
