@@ -29,6 +29,7 @@ struct Scalars {
   double dsum;      // sum of the diagonal over active cells
   double zshift;    // diagonal-weighted mean of z removed in singular systems
   double src_max;   // max |pbar_i| / V_i of the adjoint source
+  double rsum, qsum;  // sums of r and q over the cells (for the z shift)
   double npart[8];  // scratch results of small reductions
   int64_t active;
   int32_t done;     // 1 once converged (kernels then return immediately)
@@ -110,6 +111,8 @@ struct sp_ctx {
   sp_params prm;
   int device = 0;
   int num_cu = 256;  // compute units of the device (workgroup sizing)
+  bool fuse_dots = false;  // the finest post-smoother also forms (r,z), (z,q), (dw,z)
+  uint32_t dots_nb = 0;    // workgroups of that post-smoother (its partials)
   hipStream_t stream = nullptr;
   sp::Dims dm{};
   bool grid_set = false;

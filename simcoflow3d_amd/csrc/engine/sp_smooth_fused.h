@@ -44,6 +44,12 @@ struct FusedArgs {
   T* bc;         // pre: coarse right-hand side
   int cnx, cny;
   int kchunk;    // even
+  // finest post-smoother only: the CG dot products (r,z), (z,q), (dw,z) of
+  // the z it writes, as per-workgroup partials
+  int dots;
+  const double *dr, *dq;
+  const float* ddw;
+  RedSite dsite;
 };
 
 template <int NU>
@@ -75,6 +81,11 @@ __device__ inline void put_off(T* base, uint32_t byte_off, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
 }
 
+// 1/d of a smoother: FP32 uses the hardware reciprocal (1 ulp), which is
+// ample for a preconditioner; FP64 divides exactly.
+__device__ inline float smoother_rcp(float d) { return __builtin_amdgcn_rcpf(d); }
+__device__ inline double smoother_rcp(double d) { return 1.0 / d; }
+
 // Raw loads of one plane.  The loads always target a valid address (the
 // plane index is clamped, neighbours fall back to the cell itself) so their
 // destination registers carry straight into the next pipeline step; validity
@@ -83,6 +94,8 @@ template <typename T>
 struct PlaneLoad {
   T b, cx, cy, cz, cxm, cym, e, xin, xc;
   bool vt;
+  double dr, dq;  // dot-product inputs of the step's output plane t - H
+  float ddw;
 };
 
 // Per-thread constants of the march.
@@ -97,7 +110,7 @@ struct FusedCol {
   uint32_t cplane;       // post: bytes per coarse plane
 };
 
-template <typename T, int MODE, bool HAS_E>
+template <typename T, int MODE, bool HAS_E, bool DOTS, int H>
 __device__ inline void load_plane(PlaneLoad<T>& L, const FusedArgs<T>& A, const FusedCol& C, int t) {
   const int tc = t < 0 ? 0 : (t >= A.nz ? A.nz - 1 : t);
   const uint32_t n = C.col + C.plane * (uint32_t)tc;
@@ -116,6 +129,15 @@ __device__ inline void load_plane(PlaneLoad<T>& L, const FusedArgs<T>& A, const 
     L.xin = T(0);
     L.xc = T(0);
   }
+  // with the plane loads, so the in-order load counter never makes a step
+  // wait for planes fetched after them
+  if (DOTS && A.dots && C.tile && C.in_ij) {
+    const int q = t - H, qc = q < 0 ? 0 : (q >= A.nz ? A.nz - 1 : q);
+    const uint32_t e = C.col / (uint32_t)sizeof(T) + (C.plane / (uint32_t)sizeof(T)) * (uint32_t)qc;
+    L.dr = at_off(A.dr, e * 8u);
+    L.dq = at_off(A.dq, e * 8u);
+    L.ddw = at_off(A.ddw, e * 4u);
+  }
 }
 
 // Register windows along k.  Logical plane o (0 = newest) of step u lives in
@@ -126,12 +148,13 @@ struct FusedState {
   static constexpr int H = 2 * NU, W = H + 2;
   T xw[W], bw[W], dw[W], rdw[W], czw[W], cxw[W], cxmw[W], cyw[W], cymw[W];
   T racc;
+  double dacc[3];  // CG dot products (finest post-smoother)
 };
 
 // One pipeline step: plane t enters (from L), half-sweeps on the planes
 // behind it, output/residual of plane t - H, restriction after the barrier.
 // U = position of this step in the W-step unrolled march (t - t0 = U mod W).
-template <typename T, int NU, int MODE, int U>
+template <typename T, int NU, int MODE, bool DOTS, int U>
 __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>& L,
                                   const FusedArgs<T>& A, const FusedCol& C,
                                   T (*xs)[kFEY + 2][kFEX + 2], T (*rs)[kFEY][kFEX], int t, int k0, int k1) {
@@ -140,13 +163,17 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
   static_assert(R == W && (W % 2) == 0, "ring slots are compile-time only for R == W, W even");
   auto S = [](int o) constexpr { return (o + W - U) % W; };        // register slot
   auto Q = [](int o) constexpr { return ((U - o) % R + R) % R; };  // LDS ring slot
-  // FP32 keeps a window of 1/d (one division per cell and plane); FP64 has no
-  // registers to spare for it and divides in every half-sweep
+  // FP32 keeps a window of 1/d (one reciprocal per cell and plane); FP64 has
+  // no registers to spare for it and divides in every half-sweep
   constexpr bool kRcp = sizeof(T) == 4;
   const int tx = C.tx, ty = C.ty;
   // LDS planes carry a one-cell border of zeros: cell (tx, ty) is at
   // [ty + 1][tx + 1] and every neighbour read is unconditional
   const int lx = tx + 1, ly = ty + 1;
+
+  const int q = t - H;
+  const bool outq = q >= k0 && q < k1;
+  const bool wr = outq && C.tile && C.in_ij;
 
   const T lb = L.vt ? L.b : T(0), lcx = L.vt ? L.cx : T(0), lcy = L.vt ? L.cy : T(0);
   const T lcz = L.vt ? L.cz : T(0), le = L.vt ? L.e : T(0);
@@ -162,7 +189,7 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
   if (MODE == 1) x0 = d0 > T(0) ? L.xin + L.xc : T(0);
   F.bw[S(0)] = lb; F.cxw[S(0)] = lcx; F.cxmw[S(0)] = lcxm; F.cyw[S(0)] = lcy;
   F.cymw[S(0)] = lcym; F.czw[S(0)] = lcz; F.dw[S(0)] = d0; F.xw[S(0)] = x0;
-  if (kRcp) F.rdw[S(0)] = d0 > T(0) ? T(1) / d0 : T(0);
+  if (kRcp) F.rdw[S(0)] = d0 > T(0) ? smoother_rcp(d0) : T(0);
   xs[Q(0)][ly][lx] = x0;
 
   // cells of this thread's colour move this step (t0 even, W even)
@@ -192,9 +219,6 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
     xs[Q(o)][ly][lx] = xn;
   }
 
-  const int q = t - H;
-  const bool outq = q >= k0 && q < k1;
-  const bool wr = outq && C.tile && C.in_ij;
   if (MODE == 0) {
     T r = T(0);
     if (wr && F.dw[S(H)] > T(0)) {
@@ -214,6 +238,12 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
     if (q >= 0) rs[U & 1][ty][tx] = r;
   } else if (wr) {
     put_off(A.xout, C.col + C.plane * (uint32_t)q, F.xw[S(H)]);
+    if (DOTS && A.dots) {
+      const double z = (double)F.xw[S(H)];
+      F.dacc[0] += L.dr * z;
+      F.dacc[1] += z * L.dq;
+      F.dacc[2] += (double)L.ddw * z;
+    }
   }
   lds_barrier();
   if (MODE == 0 && wr && ((tx - H) & 1) == 0 && ((ty - H) & 1) == 0) {
@@ -232,21 +262,29 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
   }
 }
 
+// Planes of loads in flight: FP32 three, FP64 one (register budget).  The
+// window length W is a multiple of it, so the buffer of every step of the
+// unrolled march is a compile-time choice.
+template <typename T, int NU>
+constexpr int fused_pipe() {
+  return sizeof(T) == 4 ? (FusedShape<NU>::W % 3 == 0 ? 3 : 2) : 1;
+}
+
 // Steps t + U, t + U + 1, ..., t + W - 1 of the march (stopping at t1), each
 // followed by the load of a later plane into the buffer it used.
-template <typename T, int NU, int MODE, bool HAS_E, int U>
-__device__ inline void march_steps(FusedState<T, NU, MODE>& F, PlaneLoad<T>& LA, PlaneLoad<T>& LB,
+template <typename T, int NU, int MODE, bool HAS_E, bool DOTS, int U>
+__device__ inline void march_steps(FusedState<T, NU, MODE>& F, PlaneLoad<T> (&LB)[fused_pipe<T, NU>()],
                                    const FusedArgs<T>& A, const FusedCol& C,
                                    T (*xs)[kFEY + 2][kFEX + 2], T (*rs)[kFEY][kFEX], int t, int t1,
                                    int k0, int k1) {
   if constexpr (U < FusedShape<NU>::W) {
     if (t + U >= t1) return;
-    // FP32: two planes of loads in flight; FP64: one (register budget)
-    constexpr int kPipe = sizeof(T) == 4 ? 2 : 1;
-    PlaneLoad<T>& L = (kPipe == 2 && (U & 1)) ? LB : LA;
-    fused_step<T, NU, MODE, U>(F, L, A, C, xs, rs, t + U, k0, k1);
-    load_plane<T, MODE, HAS_E>(L, A, C, t + U + kPipe);
-    march_steps<T, NU, MODE, HAS_E, U + 1>(F, LA, LB, A, C, xs, rs, t, t1, k0, k1);
+    constexpr int P = fused_pipe<T, NU>();
+    static_assert(FusedShape<NU>::W % P == 0, "load buffers must tile the window");
+    PlaneLoad<T>& L = LB[U % P];
+    fused_step<T, NU, MODE, DOTS, U>(F, L, A, C, xs, rs, t + U, k0, k1);
+    load_plane<T, MODE, HAS_E, DOTS, FusedShape<NU>::H>(L, A, C, t + U + P);
+    march_steps<T, NU, MODE, HAS_E, DOTS, U + 1>(F, LB, A, C, xs, rs, t, t1, k0, k1);
   }
 }
 
@@ -285,6 +323,7 @@ __global__ void __launch_bounds__(kFEX * kFEY) k_smooth_fused(FusedArgs<T> A, co
         F.cymw[o] = T(0);
   }
   F.racc = T(0);
+  F.dacc[0] = F.dacc[1] = F.dacc[2] = 0.0;
   for (int q = C.ty * kFEX + C.tx; q < R * (kFEY + 2) * (kFEX + 2); q += kFEX * kFEY)
     (&xs[0][0][0])[q] = T(0);
   for (int q = C.ty * kFEX + C.tx; q < 2 * kFEY * kFEX; q += kFEX * kFEY) (&rs[0][0][0])[q] = T(0);
@@ -296,15 +335,18 @@ __global__ void __launch_bounds__(kFEX * kFEY) k_smooth_fused(FusedArgs<T> A, co
   }
   __syncthreads();
 
-  // FP32: two planes of loads in flight, buffer LA holds even steps, LB odd
-  // steps (W is even, so the parity of a step's position in the unrolled
-  // march is the parity of the step).  FP64: LA only.
+  // buffer u % P holds the planes of steps t0 + u (mod P)
   const int t0 = k0 - H, t1 = k1 + H;  // steps [t0, t1); t0 even
-  PlaneLoad<T> LA, LB;
-  load_plane<T, MODE, HAS_E>(LA, A, C, t0);
-  if (sizeof(T) == 4) load_plane<T, MODE, HAS_E>(LB, A, C, t0 + 1);
+  constexpr int P = fused_pipe<T, NU>();
+  constexpr bool DOTS = FINE && MODE == 1 && sizeof(T) == 4;  // FP64: separate dots kernel
+  PlaneLoad<T> LB[P];
+#pragma unroll
+  for (int u = 0; u < P; ++u) load_plane<T, MODE, HAS_E, DOTS, H>(LB[u], A, C, t0 + u);
   for (int t = t0; t < t1; t += W)
-    march_steps<T, NU, MODE, HAS_E, 0>(F, LA, LB, A, C, xs, rs, t, t1, k0, k1);
+    march_steps<T, NU, MODE, HAS_E, DOTS, 0>(F, LB, A, C, xs, rs, t, t1, k0, k1);
+  if constexpr (DOTS) {
+    if (A.dots) block_partials<kFEX * kFEY, 3, 0u>(A.dsite, F.dacc);
+  }
 }
 
 }  // namespace sp

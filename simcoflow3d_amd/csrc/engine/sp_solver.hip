@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(256) k_gs(LevelView<T> L, int color, int zero_
   T s = L.b[n];
   if (!zero_init) s += offdiag_sum(L, i, j, k, n);
   // same rounding as the fused smoother (FP32: times 1/d, FP64: divided by d)
-  L.x[n] = sizeof(T) == 4 ? s * (T(1) / d) : s / d;
+  L.x[n] = sizeof(T) == 4 ? s * smoother_rcp(d) : s / d;
 }
 
 // Coarse b = sum over children of the fine residual.
@@ -411,7 +411,7 @@ __global__ void __launch_bounds__(256) k_cg_residual(CgArgs A, int from_zero) {
   const Cell3 q = cell_of(A.I, A.J, A.K);
   Scalars* S = A.S;
   const double bmean = S->bmean;
-  double v[2] = {0.0, 0.0};  // max |r|/V, non-finite
+  double v[3] = {0.0, 0.0, 0.0};  // max |r|/V, non-finite, sum r
   if (q.in) {
     const int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c);
     double rv = 0.0;
@@ -420,6 +420,7 @@ __global__ void __launch_bounds__(256) k_cg_residual(CgArgs A, int from_zero) {
       if (!from_zero) rv -= apply_row(A, A.x, q.a, q.b, q.c, n);
       v[0] = fabs(rv) / (A.dxp[q.a] * A.dyp[q.b] * A.dzp[q.c]);
       if (!is_finite(rv)) v[1] = 1.0;
+      v[2] = rv;
     }
     A.r[n] = rv;
     if (A.r32) A.r32[n] = (float)rv;
@@ -427,15 +428,17 @@ __global__ void __launch_bounds__(256) k_cg_residual(CgArgs A, int from_zero) {
     A.q[n] = 0.0;
     A.p[n] = 0.0;
   }
-  block_partials<kTileThreads, 2, 3u>(A.site, v);
+  block_partials<kTileThreads, 3, 3u>(A.site, v);
 }
 
 __global__ void __launch_bounds__(kFinThreads) k_cg_residual_fin(CgArgs A, uint32_t nb) {
-  double out[2];
-  finalize_partials<kFinThreads, 2, 3u>(A.site, nb, out);
+  double out[3];
+  finalize_partials<kFinThreads, 3, 3u>(A.site, nb, out);
   if (flat_tid() == 0) {
     Scalars* S = A.S;
     S->rmax = out[0];
+    S->rsum = out[2];
+    S->qsum = 0.0;  // q was reset
     S->nonfinite = S->nonfinite || out[1] > 0.0 || !is_finite(out[0]);
     S->done = (out[0] <= A.tol || S->nonfinite) ? 1 : 0;
     S->first = 1;
@@ -490,6 +493,26 @@ __global__ void __launch_bounds__(kFinThreads) k_cg_dots_fin(CgArgs A, uint32_t 
   }
 }
 
+// Same as k_cg_dots_fin with (r,z), (z,q), (dw,z) from the finest
+// post-smoother's partials and the sums of r and q from the update and the
+// SpMV of the previous iteration.
+__global__ void __launch_bounds__(kFinThreads) k_cg_dots3_fin(CgArgs A, uint32_t nb) {
+  Scalars* S = A.S;
+  if (S->done) return;
+  double t[3];
+  finalize_partials<kFinThreads, 3, 0u>(A.site, nb, t);
+  if (flat_tid() == 0) {
+    const double m = (S->singular && S->dsum > 0.0) ? t[2] / S->dsum : 0.0;
+    const double rzm = t[0] - m * S->rsum, zqm = t[1] - m * S->qsum;
+    S->zshift = m;
+    S->beta = S->first ? 0.0 : -S->alpha * zqm / S->rz;
+    S->rz = rzm;
+    S->zq = zqm;
+    S->first = 0;
+    if (!is_finite(rzm) || !is_finite(S->beta)) { S->nonfinite = 1; S->done = 1; }
+  }
+}
+
 // p = (z - m) + beta p on active cells
 __global__ void __launch_bounds__(256) k_cg_pupdate(CgArgs A) {
   const Scalars* S = A.S;
@@ -514,7 +537,7 @@ __global__ void __launch_bounds__(256) k_cg_spmv(CgArgs A) {
   Scalars* S = A.S;
   if (S->done) return;
   const Col3 q = col_of(A.I, A.J, A.K);
-  double v[1] = {0.0};
+  double v[2] = {0.0, 0.0};  // (p, q), sum q
   if (q.in) {
     const int64_t I = A.I, IJ = (int64_t)A.I * A.J;
     const int a = q.a, b = q.b;
@@ -550,21 +573,23 @@ __global__ void __launch_bounds__(256) k_cg_spmv(CgArgs A) {
       const double qv = A.active[n] ? sacc : 0.0;
       A.q[n] = qv;
       v[0] += pc * qv;
+      v[1] += qv;
       pm = pc;
       pc = pp;
       czm = cz;
     }
   }
-  block_partials<kTileThreads, 1, 0u>(A.site, v);
+  block_partials<kTileThreads, 2, 0u>(A.site, v);
 }
 
 __global__ void __launch_bounds__(kFinThreads) k_cg_spmv_fin(CgArgs A, uint32_t nb) {
   Scalars* S = A.S;
   if (S->done) return;
-  double t[1];
-  finalize_partials<kFinThreads, 1, 0u>(A.site, nb, t);
+  double t[2];
+  finalize_partials<kFinThreads, 2, 0u>(A.site, nb, t);
   if (flat_tid() == 0) {
     S->pq = t[0];
+    S->qsum = t[1];
     S->alpha = S->rz / t[0];
     if (!is_finite(S->alpha)) { S->nonfinite = 1; S->done = 1; }
   }
@@ -576,7 +601,7 @@ __global__ void __launch_bounds__(256) k_cg_update(CgArgs A) {
   if (S->done) return;
   const Col3 q = col_of(A.I, A.J, A.K);
   const double alpha = S->alpha;
-  double v[1] = {0.0};
+  double v[2] = {0.0, 0.0};  // max |r|/V, sum r
   if (q.in) {
     const int64_t IJ = (int64_t)A.I * A.J;
     const double dxy = A.dxp[q.a] * A.dyp[q.b];
@@ -589,18 +614,20 @@ __global__ void __launch_bounds__(256) k_cg_update(CgArgs A) {
       if (A.r32) A.r32[n] = (float)rv;
       const double rr = fabs(rv) / (dxy * A.dzp[c]);
       v[0] = (rr > v[0] || rr != rr) ? rr : v[0];  // a NaN sticks
+      v[1] += rv;
     }
   }
-  block_partials<kTileThreads, 1, 1u>(A.site, v);
+  block_partials<kTileThreads, 2, 1u>(A.site, v);
 }
 
 __global__ void __launch_bounds__(kFinThreads) k_cg_update_fin(CgArgs A, uint32_t nb) {
   Scalars* S = A.S;
   if (S->done) return;
-  double t[1];
-  finalize_partials<kFinThreads, 1, 1u>(A.site, nb, t);
+  double t[2];
+  finalize_partials<kFinThreads, 2, 1u>(A.site, nb, t);
   if (flat_tid() == 0) {
     S->rmax = t[0];
+    S->rsum = t[1];
     S->iter += 1;
     if (!is_finite(t[0])) S->nonfinite = 1;
     S->done = (t[0] <= A.tol || S->nonfinite) ? 1 : 0;
@@ -649,6 +676,13 @@ void launch_fused(sp_ctx* c, Level<T>* lv, int l, int mode) {
   A.cnx = C.nx; A.cny = C.ny;
   if (mode == 0) { A.xout = L.xs; A.bc = C.b; }
   else { A.xin = L.xs; A.xc = C.x; A.xout = L.x; }
+  if (mode == 1 && l == 0 && c->fuse_dots) {
+    A.dots = 1;
+    A.dr = c->r;
+    A.dq = c->q;
+    A.ddw = c->dw;
+    A.dsite = red_site(c, kSiteDots);
+  }
   const int gx = (L.nx + Sh::TX - 1) / Sh::TX, gy = (L.ny + Sh::TY - 1) / Sh::TY;
   // k chunk: even (coarse cells never straddle chunks).  One workgroup fills a
   // CU, and a workgroup marches kchunk + 2H planes one after the other, so
@@ -668,12 +702,15 @@ void launch_fused(sp_ctx* c, Level<T>* lv, int l, int mode) {
   const dim3 grd(gx, gy, (L.nz + A.kchunk - 1) / A.kchunk), blk(kFEX, kFEY, 1);
   const bool fine = l == 0;
   int slot;
+  if (A.dots) c->dots_nb = grd.x * grd.y * grd.z;
   // algorithmic words per cell: pre b + 3 faces -> x (+ coarse b);
-  // post x_pre + 3 faces + b -> x (+ coarse x)
+  // post x_pre + 3 faces + b -> x (+ coarse x); the dot products add r, q
+  // (FP64) and the diagonal weight (FP32)
   const double words = mode == 0 ? 5.0 + 0.125 : 6.0 + 0.125;
   timer_begin(c, fine ? (mode == 0 ? "mg_pre_fine" : "mg_post_fine")
                       : (mode == 0 ? "mg_pre_coarse" : "mg_post_coarse"),
-              level_bytes(L, words + (L.e ? 1.0 : 0.0)), &slot);
+              level_bytes(L, words + (L.e ? 1.0 : 0.0)) + (A.dots ? 20.0 * (double)L.n : 0.0),
+              &slot);
   const Scalars* S = c->S;
 #define SP_FUSED_LAUNCH(MODE_, FINE_)                                                         \
   do {                                                                                        \
@@ -693,11 +730,17 @@ void launch_fused(sp_ctx* c, Level<T>* lv, int l, int mode) {
   timer_end(c, slot);
 }
 
+// The V-cycle uses the temporally blocked smoother legs
+bool fused_vcycle(const sp_ctx* c) {
+  const int pre = std::max(1, c->prm.mg_pre), post = c->prm.mg_post;
+  return pre == post && (pre == 1 || pre == 2) && !(c->prm.flags & SP_FLAG_UNFUSED_SMOOTHER);
+}
+
 template <typename T>
 void vcycle(sp_ctx* c, Level<T>* lv) {
   const int L = c->levels, Lt = c->tail_level;
   const int pre = std::max(1, c->prm.mg_pre), post = c->prm.mg_post;
-  const bool fused = pre == post && (pre == 1 || pre == 2) && !(c->prm.flags & 1);
+  const bool fused = fused_vcycle(c);
   dim3 blk(64, 4, 1);
   int slot;
   for (int l = 0; l < Lt; ++l) {
@@ -870,6 +913,11 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st) {
 
   hipError_t e;
   volatile int32_t* hflag = c->hflags;
+  // the finest post-smoother forms the CG dot products when the V-cycle has
+  // a fused level above the tail (FP32 V-cycle: the FP64 smoother has no
+  // registers to spare for the extra inputs)
+  c->fuse_dots = mg && mixed && fused_vcycle(c) && c->tail_level > 0;
+  c->dots_nb = 0;
   hipEvent_t evs[2];
   hipEventCreateWithFlags(&evs[0], hipEventDisableTiming);
   hipEventCreateWithFlags(&evs[1], hipEventDisableTiming);
@@ -885,8 +933,12 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st) {
     int cur = 0;
     while (it < max_iter) {
       precondition();
-      launch_col(k_cg_dots, kSiteDots, "cg_dots", NB * (8 + zb + 8 + 4 + 1));
-      finalize(k_cg_dots_fin, kSiteDots, ncol);
+      if (c->fuse_dots) {
+        finalize(k_cg_dots3_fin, kSiteDots, c->dots_nb);
+      } else {
+        launch_col(k_cg_dots, kSiteDots, "cg_dots", NB * (8 + zb + 8 + 4 + 1));
+        finalize(k_cg_dots_fin, kSiteDots, ncol);
+      }
       launch_col(k_cg_pupdate, kSiteDots, "cg_pupdate", NB * (zb + 8 + 8 + 1));
       launch_col(k_cg_spmv, kSiteSpmv, "cg_spmv", NB * (8 + 24 + eb + 1 + 8));
       finalize(k_cg_spmv_fin, kSiteSpmv, ncol);
