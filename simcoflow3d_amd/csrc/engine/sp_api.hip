@@ -130,6 +130,7 @@ static sp_status ensure_buffers(sp_ctx* c) {
     return SP_OK;
   free_buffers(c);
   c->levels = L;
+  c->desc_state = -1;  // level table must be uploaded again
   const int64_t N = D.N;
   sp_status s;
   if ((s = dmalloc(c, &c->pocoef, 6 * N)) || (s = dmalloc(c, &c->cx, N)) || (s = dmalloc(c, &c->cy, N)) ||
@@ -250,6 +251,9 @@ static sp_status project_device(sp_ctx* c, const sp_inputs* in, sp_outputs* out,
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_check(c, e, "assembly");
   auto t1 = std::chrono::steady_clock::now();
+  stale = c->S_host->stale_count;
+  if (stale > c->stale_cap)
+    return fail(c, SP_ERR_INVALID_ARG, "too many GFM faces next to unclassified cells");
   if (c->S_host->nonfinite) return fail(c, SP_ERR_NONFINITE, "non-finite value in the assembled system");
   c->has_e = c->S_host->esum > 0.0;
   if ((s = mg_setup(c))) return s;
@@ -271,10 +275,10 @@ static sp_status project_device(sp_ctx* c, const sp_inputs* in, sp_outputs* out,
                        c->stream);
     if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(po_coef)");
   }
-  // checkpoint of the forward pressure for the adjoint
-  e = hipMemcpyAsync(c->x_ckpt, c->x, sizeof(double) * D.N, hipMemcpyDeviceToDevice, c->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream);
+  // checkpoint of the forward pressure for the adjoint: the iterate buffer
+  // becomes the checkpoint and the old checkpoint the next iterate
+  std::swap(c->x, c->x_ckpt);
+  e = hipMemcpyAsync(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_check(c, e, "correction");
   auto t3 = std::chrono::steady_clock::now();

@@ -79,29 +79,60 @@ __device__ inline int gfm_face(const GfmArgs& A, int f, int a, int b, int c, dou
   return -kDry;
 }
 
+// GFM coefficients of the six faces of every cell.  Each thread walks a
+// column up k; the P cell's values and class are formed once per cell.  The
+// arithmetic is gfm_face's, expression for expression.
 __global__ void __launch_bounds__(256) k_gfm(GfmArgs A) {
 #pragma clang fp contract(off)
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y * blockDim.y + threadIdx.y;
-  const int c = blockIdx.z;
-  if (a >= A.I || b >= A.J) return;
-  const int64_t n = (int64_t)a + (int64_t)A.I * ((int64_t)b + (int64_t)A.J * c);
-#pragma unroll 1
-  for (int f = 0; f < 6; ++f) {
-    double beta = 0.0, pref = 0.0;
-    const int r = gfm_face(A, f, a, b, c, beta, pref);
-    double coef = 0.0;
-    if (r > 0) {
-      coef = pref / beta;
-    } else if (r < 0) {
-      const uint32_t slot = atomicAdd(A.stale_count, 1u);
-      if (slot < A.stale_cap) {
-        A.stale[3 * (int64_t)slot + 0] = (int32_t)n;
-        A.stale[3 * (int64_t)slot + 1] = f;
-        A.stale[3 * (int64_t)slot + 2] = -r;
+  const Col3 q = col_of(A.I, A.J, A.K);
+  if (!q.in) return;
+  const int a = q.a, b = q.b;
+  const int64_t IJ = (int64_t)A.I * A.J, SXY = (int64_t)A.SX * A.SY;
+  int64_t n = (int64_t)a + (int64_t)A.I * ((int64_t)b + (int64_t)A.J * q.c0);
+  int64_t h = (int64_t)(a + 1) + (int64_t)A.SX * ((int64_t)(b + 1) + (int64_t)A.SY * (q.c0 + 1));
+  for (int c = q.c0; c < q.c1; ++c, n += IJ, h += SXY) {
+    const double fP = A.phiL[h];
+    const int cP = gfm_class(A.vof[h], A.vofL[h], fP, A.epsi);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      const int dir = f % 3;
+      const int side = f < 3 ? -1 : 1;
+      const int64_t st = dir == 0 ? 1 : (dir == 1 ? (int64_t)A.SX : SXY);
+      const int coord = dir == 0 ? a : (dir == 1 ? b : c);
+      // DGrid: TGrid%d(max(im,1)) with im = i-1 on the minus side (:151-153)
+      const double dgrid = A.dgrid[dir][side < 0 ? (coord > 0 ? coord - 1 : 0) : coord];
+      const double Dp = A.dp[dir][side < 0 ? h - st : h];
+      const double pref = Dp / dgrid * A.betaP * A.betaM;  // :194 / :235
+      double coef = 0.0;
+      if (cP != kNone) {
+        const int64_t hq = h + side * st;
+        const double fQ = A.phiL[hq];
+        const double lam = fabs(fP) / (fabs(fP) + fabs(fQ) + 1.0e-30);  // :155
+        const int cQ = gfm_class(A.vof[hq], A.vofL[hq], fQ, A.epsi);
+        double beta = 0.0;
+        int r;
+        if (cP == kWet) {
+          if (cQ == kDry) { beta = lam * A.betaM + (1.0 - lam) * A.betaP; r = kWet; }  // :176
+          else if (cQ == kWet) { beta = A.betaM; r = kWet; }                          // :181
+          else r = -kWet;
+        } else {
+          if (cQ == kWet) { beta = lam * A.betaP + (1.0 - lam) * A.betaM; r = kDry; }  // :218
+          else if (cQ == kDry) { beta = A.betaP; r = kDry; }                         // :223
+          else r = -kDry;
+        }
+        if (r > 0) {
+          coef = pref / beta;
+        } else {
+          const uint32_t slot = atomicAdd(A.stale_count, 1u);
+          if (slot < A.stale_cap) {
+            A.stale[3 * (int64_t)slot + 0] = (int32_t)n;
+            A.stale[3 * (int64_t)slot + 1] = f;
+            A.stale[3 * (int64_t)slot + 2] = -r;
+          }
+        }
       }
+      A.pocoef[(int64_t)f * A.N + n] = coef;
     }
-    A.pocoef[(int64_t)f * A.N + n] = coef;
   }
 }
 
@@ -109,6 +140,7 @@ __global__ void __launch_bounds__(256) k_gfm(GfmArgs A) {
 // assigns BetaW (kind 1) and BetaD (kind 2), -1 if none.  Columns are the
 // innermost k loop of the reference (ProjectionP.f90:142-144).
 __global__ void __launch_bounds__(256) k_gfm_col_last(GfmArgs A) {
+  if (*A.stale_count == 0u) return;  // no face needs the previous beta
   const int64_t cols = (int64_t)A.I * A.J;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cols * 6) return;
@@ -131,6 +163,7 @@ __global__ void __launch_bounds__(256) k_gfm_col_last(GfmArgs A) {
 // "last assignment before column c" is the maximum over earlier columns).
 // One workgroup of 1024 threads per (face, kind).
 __global__ void __launch_bounds__(1024) k_gfm_col_prefix(GfmArgs A) {
+  if (*A.stale_count == 0u) return;
   const int64_t cols = (int64_t)A.I * A.J;
   const int64_t* src = A.col_last + (int64_t)blockIdx.x * cols;
   int64_t* dst = A.col_prefix + (int64_t)blockIdx.x * cols;
@@ -158,36 +191,38 @@ __global__ void __launch_bounds__(1024) k_gfm_col_prefix(GfmArgs A) {
 
 // Resolve the faces whose neighbour is unclassified: the reference reuses the
 // BetaW/BetaD left by the most recent assigning cell in its i-j-k loop.
-__global__ void __launch_bounds__(256) k_gfm_fixup(GfmArgs A, uint32_t count) {
+__global__ void __launch_bounds__(256) k_gfm_fixup(GfmArgs A) {
 #pragma clang fp contract(off)
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= count) return;
-  const int64_t n = A.stale[3 * (int64_t)t + 0];
-  const int f = A.stale[3 * (int64_t)t + 1];
-  const int kind = A.stale[3 * (int64_t)t + 2];
-  const int a = (int)(n % A.I), b = (int)((n / A.I) % A.J), c = (int)(n / ((int64_t)A.I * A.J));
-  const int64_t cols = (int64_t)A.I * A.J;
-  const int64_t col = (int64_t)a * A.J + b;
-  int64_t pos = -1;
-  for (int cc = c - 1; cc >= 0; --cc) {
-    double beta, pref;
-    if (gfm_face(A, f, a, b, cc, beta, pref) == kind) { pos = col * A.K + cc; break; }
+  const uint32_t count = min(*A.stale_count, A.stale_cap);
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count;
+       t += gridDim.x * blockDim.x) {
+    const int64_t n = A.stale[3 * (int64_t)t + 0];
+    const int f = A.stale[3 * (int64_t)t + 1];
+    const int kind = A.stale[3 * (int64_t)t + 2];
+    const int a = (int)(n % A.I), b = (int)((n / A.I) % A.J), c = (int)(n / ((int64_t)A.I * A.J));
+    const int64_t cols = (int64_t)A.I * A.J;
+    const int64_t col = (int64_t)a * A.J + b;
+    int64_t pos = -1;
+    for (int cc = c - 1; cc >= 0; --cc) {
+      double beta, pref;
+      if (gfm_face(A, f, a, b, cc, beta, pref) == kind) { pos = col * A.K + cc; break; }
+    }
+    if (pos < 0) pos = A.col_prefix[(int64_t)(2 * f + kind - 1) * cols + col];
+    double beta;
+    if (pos >= 0) {
+      const int64_t pc = pos / A.K;
+      const int cc = (int)(pos % A.K);
+      double pref2;
+      gfm_face(A, f, (int)(pc / A.J), (int)(pc % A.J), cc, beta, pref2);
+    } else {
+      // no earlier assignment: the reference's local is undefined here; use the
+      // P cell's own phase
+      beta = kind == kWet ? A.betaM : A.betaP;
+    }
+    double b0, pref;
+    gfm_face(A, f, a, b, c, b0, pref);
+    A.pocoef[(int64_t)f * A.N + n] = pref / beta;
   }
-  if (pos < 0) pos = A.col_prefix[(int64_t)(2 * f + kind - 1) * cols + col];
-  double beta;
-  if (pos >= 0) {
-    const int64_t pc = pos / A.K;
-    const int cc = (int)(pos % A.K);
-    double pref2;
-    gfm_face(A, f, (int)(pc / A.J), (int)(pc % A.J), cc, beta, pref2);
-  } else {
-    // no earlier assignment: the reference's local is undefined here; use the
-    // P cell's own phase
-    beta = kind == kWet ? A.betaM : A.betaP;
-  }
-  double b0, pref;
-  gfm_face(A, f, a, b, c, b0, pref);
-  A.pocoef[(int64_t)f * A.N + n] = pref / beta;
 }
 
 // Boundary faces copy the first interior value (ProjectionP.f90:256-264).
@@ -266,80 +301,90 @@ struct OpArgs {
   Scalars* S;
 };
 
-// Symmetric operator, diagonal-only terms and RHS for one cell.
+// Symmetric operator, diagonal-only terms and RHS.  One cell per thread
+// (kOpK = 1) measured fastest: the body has many independent loads, and
+// walking several cells per thread serialises them.
+constexpr int kOpK = 1;
 __global__ void __launch_bounds__(256) k_operator(OpArgs A) {
 #pragma clang fp contract(off)
-  const Cell3 q = cell_of(A.I, A.J, A.K);
-  const int a = q.a, b = q.b, c = q.c;
+  const Col3 q = col_of<kOpK>(A.I, A.J, A.K);
   const int64_t IJ = (int64_t)A.I * A.J, SXY = (int64_t)A.SX * A.SY, N = A.N;
   // sum b, sum |e|, max |b|/V, active count, non-finite, sum d (as stored)
   double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (q.in) {
-    const double dx = A.dxp[a], dy = A.dyp[b], dz = A.dzp[c];
-    const int64_t n = (int64_t)a + (int64_t)A.I * ((int64_t)b + (int64_t)A.J * c);
-    const int64_t h = (int64_t)(a + 1) + (int64_t)A.SX * ((int64_t)(b + 1) + (int64_t)A.SY * (c + 1));
-    const int ctP = A.ct[n];
-    double cxv = 0, cyv = 0, czv = 0, cW = 0, cS = 0, cB = 0, ex = 0, rhs = 0, dval = 0;
-    bool act = false;
-    if (ctP != 2) {
-      const double W = A.pocoef[n], S = A.pocoef[N + n], B = A.pocoef[2 * N + n];
-      const double E = A.pocoef[3 * N + n], Nn = A.pocoef[4 * N + n], T = A.pocoef[5 * N + n];
-      const bool wS = a > 0 ? A.ct[n - 1] == 2 : A.fl[0] == 0;
-      const bool sS = b > 0 ? A.ct[n - A.I] == 2 : A.fl[2] == 0;
-      const bool bS = c > 0 ? A.ct[n - IJ] == 2 : A.fl[4] == 0;
-      const bool eS = a < A.I - 1 ? A.ct[n + 1] == 2 : A.fl[1] == 0;
-      const bool nS = b < A.J - 1 ? A.ct[n + A.I] == 2 : A.fl[3] == 0;
-      const bool tS = c < A.K - 1 ? A.ct[n + IJ] == 2 : A.fl[5] == 0;
-      // couplings to active neighbours: the lower cell's E/N/T coefficient
-      if (a < A.I - 1 && A.ct[n + 1] != 2) cxv = E * A.ee[h] * dy * dz;
-      if (b < A.J - 1 && A.ct[n + A.I] != 2) cyv = Nn * A.ne[h] * dx * dz;
-      if (c < A.K - 1 && A.ct[n + IJ] != 2) czv = T * A.te[h] * dx * dy;
-      if (a > 0 && A.ct[n - 1] != 2) cW = A.pocoef[3 * N + n - 1] * A.ee[h - 1] * dy * dz;
-      if (b > 0 && A.ct[n - A.I] != 2) cS = A.pocoef[4 * N + n - A.I] * A.ne[h - A.SX] * dx * dz;
-      if (c > 0 && A.ct[n - IJ] != 2) cB = A.pocoef[5 * N + n - IJ] * A.te[h - SXY] * dx * dy;
-      // diagonal-only terms with the row's own coefficient (:379-430)
-      if (wS) ex = ex + W * A.ee[h - 1] * dy * dz;
-      if (sS) ex = ex + S * A.ne[h - A.SX] * dx * dz;
-      if (bS) ex = ex + B * A.te[h - SXY] * dx * dy;
-      if (eS) ex = ex + E * A.ee[h] * dy * dz;
-      if (nS) ex = ex + Nn * A.ne[h] * dx * dz;
-      if (tS) ex = ex + T * A.te[h] * dx * dy;
-      double d = cW;
-      d = d + cS;
-      d = d + cB;
-      d = d + cxv;
-      d = d + cyv;
-      d = d + czv;
-      d = d + ex;
-      act = d > 0.0;
-      dval = act ? d : 0.0;
-      if (!is_finite(d)) v[4] = 1.0;
-      // RHS (:521-546)
-      rhs = -dy * dz * (A.ee[h] * A.u[h] - A.ee[h - 1] * A.u[h - 1]) -
-            dx * dz * (A.ne[h] * A.v[h] - A.ne[h - A.SX] * A.v[h - A.SX]) -
-            dx * dy * (A.te[h] * A.w[h] - A.te[h - SXY] * A.w[h - SXY]);
-      rhs = rhs + (A.vof[h] - A.vofO[h]) * dx * dy * dz / A.dt;
-      if (A.fl[0] == 0 && a == 0) rhs = rhs + W * A.bcv[0][b + (int64_t)A.J * c];
-      if (A.fl[1] == 0 && a == A.I - 1) rhs = rhs + E * A.bcv[1][b + (int64_t)A.J * c];
-      if (A.fl[2] == 0 && b == 0) rhs = rhs + S * A.bcv[2][a + (int64_t)A.I * c];
-      if (A.fl[3] == 0 && b == A.J - 1) rhs = rhs + Nn * A.bcv[3][a + (int64_t)A.I * c];
-      if (A.fl[4] == 0 && c == 0) rhs = rhs + B * A.bcv[4][a + (int64_t)A.I * b];
-      if (A.fl[5] == 0 && c == A.K - 1) rhs = rhs + T * A.bcv[5][a + (int64_t)A.I * b];
-      if (act) {
-        if (!is_finite(rhs)) v[4] = 1.0;
-        v[0] = rhs;
-        v[1] = fabs(ex);
-        v[2] = fabs(rhs) / (dx * dy * dz);
-        v[3] = 1.0;
-        v[5] = (double)(float)dval;
+    const int a = q.a, b = q.b;
+    const double dx = A.dxp[a], dy = A.dyp[b];
+    int64_t n = (int64_t)a + (int64_t)A.I * ((int64_t)b + (int64_t)A.J * q.c0);
+    int64_t h = (int64_t)(a + 1) + (int64_t)A.SX * ((int64_t)(b + 1) + (int64_t)A.SY * (q.c0 + 1));
+#pragma unroll
+    for (int cc = 0; cc < kOpK; ++cc, n += IJ, h += SXY) {
+      const int c = q.c0 + cc;
+      if (c >= q.c1) break;
+      const double dz = A.dzp[c];
+      const int ctP = A.ct[n];
+      double cxv = 0, cyv = 0, czv = 0, cW = 0, cS = 0, cB = 0, ex = 0, rhs = 0, dval = 0;
+      bool act = false;
+      if (ctP != 2) {
+        const double W = A.pocoef[n], S = A.pocoef[N + n], B = A.pocoef[2 * N + n];
+        const double E = A.pocoef[3 * N + n], Nn = A.pocoef[4 * N + n], T = A.pocoef[5 * N + n];
+        const bool wS = a > 0 ? A.ct[n - 1] == 2 : A.fl[0] == 0;
+        const bool sS = b > 0 ? A.ct[n - A.I] == 2 : A.fl[2] == 0;
+        const bool bS = c > 0 ? A.ct[n - IJ] == 2 : A.fl[4] == 0;
+        const bool eS = a < A.I - 1 ? A.ct[n + 1] == 2 : A.fl[1] == 0;
+        const bool nS = b < A.J - 1 ? A.ct[n + A.I] == 2 : A.fl[3] == 0;
+        const bool tS = c < A.K - 1 ? A.ct[n + IJ] == 2 : A.fl[5] == 0;
+        // couplings to active neighbours: the lower cell's E/N/T coefficient
+        if (a < A.I - 1 && A.ct[n + 1] != 2) cxv = E * A.ee[h] * dy * dz;
+        if (b < A.J - 1 && A.ct[n + A.I] != 2) cyv = Nn * A.ne[h] * dx * dz;
+        if (c < A.K - 1 && A.ct[n + IJ] != 2) czv = T * A.te[h] * dx * dy;
+        if (a > 0 && A.ct[n - 1] != 2) cW = A.pocoef[3 * N + n - 1] * A.ee[h - 1] * dy * dz;
+        if (b > 0 && A.ct[n - A.I] != 2) cS = A.pocoef[4 * N + n - A.I] * A.ne[h - A.SX] * dx * dz;
+        if (c > 0 && A.ct[n - IJ] != 2) cB = A.pocoef[5 * N + n - IJ] * A.te[h - SXY] * dx * dy;
+        // diagonal-only terms with the row's own coefficient (:379-430)
+        if (wS) ex = ex + W * A.ee[h - 1] * dy * dz;
+        if (sS) ex = ex + S * A.ne[h - A.SX] * dx * dz;
+        if (bS) ex = ex + B * A.te[h - SXY] * dx * dy;
+        if (eS) ex = ex + E * A.ee[h] * dy * dz;
+        if (nS) ex = ex + Nn * A.ne[h] * dx * dz;
+        if (tS) ex = ex + T * A.te[h] * dx * dy;
+        double d = cW;
+        d = d + cS;
+        d = d + cB;
+        d = d + cxv;
+        d = d + cyv;
+        d = d + czv;
+        d = d + ex;
+        act = d > 0.0;
+        dval = act ? d : 0.0;
+        if (!is_finite(d)) v[4] = 1.0;
+        // RHS (:521-546)
+        rhs = -dy * dz * (A.ee[h] * A.u[h] - A.ee[h - 1] * A.u[h - 1]) -
+              dx * dz * (A.ne[h] * A.v[h] - A.ne[h - A.SX] * A.v[h - A.SX]) -
+              dx * dy * (A.te[h] * A.w[h] - A.te[h - SXY] * A.w[h - SXY]);
+        rhs = rhs + (A.vof[h] - A.vofO[h]) * dx * dy * dz / A.dt;
+        if (A.fl[0] == 0 && a == 0) rhs = rhs + W * A.bcv[0][b + (int64_t)A.J * c];
+        if (A.fl[1] == 0 && a == A.I - 1) rhs = rhs + E * A.bcv[1][b + (int64_t)A.J * c];
+        if (A.fl[2] == 0 && b == 0) rhs = rhs + S * A.bcv[2][a + (int64_t)A.I * c];
+        if (A.fl[3] == 0 && b == A.J - 1) rhs = rhs + Nn * A.bcv[3][a + (int64_t)A.I * c];
+        if (A.fl[4] == 0 && c == 0) rhs = rhs + B * A.bcv[4][a + (int64_t)A.I * b];
+        if (A.fl[5] == 0 && c == A.K - 1) rhs = rhs + T * A.bcv[5][a + (int64_t)A.I * b];
+        if (act) {
+          if (!is_finite(rhs)) v[4] = 1.0;
+          v[0] += rhs;
+          v[1] += fabs(ex);
+          const double bv = fabs(rhs) / (dx * dy * dz);
+          v[2] = (bv > v[2] || bv != bv) ? bv : v[2];
+          v[3] += 1.0;
+          v[5] += (double)(float)dval;
+        }
       }
-    }
-    A.cx[n] = cxv; A.cy[n] = cyv; A.cz[n] = czv; A.e[n] = ex; A.b[n] = rhs;
-    A.active[n] = act ? 1 : 0;
-    A.dw[n] = (float)dval;
-    if (A.cx32) {
-      A.cx32[n] = (float)cxv; A.cy32[n] = (float)cyv; A.cz32[n] = (float)czv;
-      A.e32[n] = (float)ex;
+      A.cx[n] = cxv; A.cy[n] = cyv; A.cz[n] = czv; A.e[n] = ex; A.b[n] = rhs;
+      A.active[n] = act ? 1 : 0;
+      A.dw[n] = (float)dval;
+      if (A.cx32) {
+        A.cx32[n] = (float)cxv; A.cy32[n] = (float)cyv; A.cz32[n] = (float)czv;
+        A.e32[n] = (float)ex;
+      }
     }
   }
   // value 2 (max |b|/V) and 4 (non-finite flag) are maxima
@@ -369,7 +414,6 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_out) {
   const bool mixed = c->prm.precision == SP_PREC_MIXED;
   dim3 blk(64, 4, 1), grd((D.I + 63) / 64, (D.J + 3) / 4, D.K);
   int slot;
-  uint32_t stale = 0;
   if (in.beta_e) {
     BetaArgs B{};
     B.I = D.I; B.J = D.J; B.K = D.K; B.SX = D.SX; B.SY = D.SY; B.N = D.N;
@@ -395,20 +439,14 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_out) {
     G.stale_count = &c->S->stale_count;
     G.col_last = c->col_last;
     G.col_prefix = c->col_prefix;
-    hipError_t e = hipMemsetAsync(&c->S->stale_count, 0, sizeof(uint32_t), c->stream);
-    if (e != hipSuccess) return hip_check(c, e, "hipMemsetAsync");
     timer_begin(c, "gfm", (double)D.N * (3 * 8 + 3 * 8 + 6 * 8), &slot);
-    hipLaunchKernelGGL(k_gfm, grd, blk, 0, c->stream, G);
+    hipLaunchKernelGGL(k_gfm, col_grid(D.I, D.J, D.K), blk, 0, c->stream, G);
     timer_end(c, slot);
-    e = hipMemcpyAsync(&c->S_host->stale_count, &c->S->stale_count, sizeof(uint32_t),
-                       hipMemcpyDeviceToHost, c->stream);
-    if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync");
-    e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return hip_check(c, e, "k_gfm");
-    stale = c->S_host->stale_count;
-    if (stale > c->stale_cap)
-      return fail(c, SP_ERR_INVALID_ARG, "too many GFM faces next to unclassified cells");
-    if (stale > 0) {
+    // faces next to unclassified cells take the previous beta in loop
+    // order; the three kernels read the count on the device and return at
+    // once when there are none (the count is checked against the list's
+    // capacity after the assembly)
+    {
       const int64_t cols = (int64_t)D.I * D.J;
       timer_begin(c, "gfm_col_last", 0, &slot);
       hipLaunchKernelGGL(k_gfm_col_last, dim3((unsigned)((cols * 6 + 255) / 256)), dim3(256), 0,
@@ -416,8 +454,8 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_out) {
       timer_end(c, slot);
       hipLaunchKernelGGL(k_gfm_col_prefix, dim3(12), dim3(1024), 0, c->stream, G);
       timer_begin(c, "gfm_fixup", 0, &slot);
-      hipLaunchKernelGGL(k_gfm_fixup, dim3((stale + 255) / 256), dim3(256), 0, c->stream, G,
-                         stale);
+      const int64_t fb = std::min<int64_t>((c->stale_cap + 255) / 256, 2048);
+      hipLaunchKernelGGL(k_gfm_fixup, dim3((unsigned)fb), dim3(256), 0, c->stream, G);
       timer_end(c, slot);
     }
     int64_t plane = std::max<int64_t>(std::max<int64_t>((int64_t)D.J * D.K, (int64_t)D.I * D.K),
@@ -425,7 +463,7 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_out) {
     hipLaunchKernelGGL(k_gfm_boundary, dim3((unsigned)((plane + 255) / 256), 6), dim3(256), 0,
                        c->stream, G);
   }
-  if (stale_out) *stale_out = stale;
+  if (stale_out) *stale_out = 0;  // counted on the device (Scalars::stale_count)
 
   OpArgs O{};
   O.I = D.I; O.J = D.J; O.K = D.K; O.SX = D.SX; O.SY = D.SY; O.N = D.N;
@@ -451,10 +489,10 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_out) {
     return fail(c, SP_ERR_INVALID_ARG, "grid too large for the reduction buffers");
   timer_begin(c, "operator_rhs",
               (double)D.N * (6 * 8 + 3 * 8 + 4 + 5 * 8 + 4 * 8 + 8 + 1 + 4 + (mixed ? 16 : 0)), &slot);
-  hipLaunchKernelGGL(k_operator, flat_grid(D.I, D.J, D.K), dim3(64, 4, 1), 0, c->stream, O);
+  hipLaunchKernelGGL(k_operator, col_grid_k<kOpK>(D.I, D.J, D.K), dim3(64, 4, 1), 0, c->stream, O);
   timer_end(c, slot);
   hipLaunchKernelGGL(k_operator_fin, dim3(1), dim3(kFinThreads), 0, c->stream, O,
-                     flat_blocks(D.I, D.J, D.K));
+                     col_blocks_k<kOpK>(D.I, D.J, D.K));
   return SP_OK;
 }
 

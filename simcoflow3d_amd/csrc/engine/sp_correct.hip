@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "sp_device.h"
 #include "sp_geom.h"
 #include "sp_internal.h"
@@ -33,11 +35,14 @@ struct CorrArgs {
 
 // shift = x(pin) or mean of x over active cells (singular systems only)
 __global__ void __launch_bounds__(256) k_shift(CorrArgs A) {
-  const Cell3 q = cell_of(A.I, A.J, A.K);
+  const Col3 q = col_of(A.I, A.J, A.K);
   double v[1] = {0.0};
   if (q.in) {
-    const int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c);
-    if (A.active[n]) v[0] = A.x[n];
+    const int64_t IJ = (int64_t)A.I * A.J;
+    int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c0);
+#pragma unroll 4
+    for (int c = q.c0; c < q.c1; ++c, n += IJ)
+      if (A.active[n]) v[0] += A.x[n];
   }
   block_partials<kTileThreads, 1, 0u>(A.site, v);
 }
@@ -60,16 +65,19 @@ __device__ inline double pval(const CorrArgs& A, int64_t n, double shift) {
   return A.active[n] ? A.x[n] - shift : 0.0;
 }
 
+// Correction of the faces on the + side of every cell, and the interior of
+// Proj%Pp (DeltaPressureGetValues, ProjectionP.f90:591-613).  Each thread
+// walks a column of cells up k.
 __global__ void __launch_bounds__(256) k_correct(CorrArgs A) {
 #pragma clang fp contract(off)
-  const Cell3 q = cell_of(A.I, A.J, A.K);
+  const Col3 q = col_of(A.I, A.J, A.K);
   if (!q.in) return;
-  const int a = q.a, b = q.b, c = q.c;
+  const int a = q.a, b = q.b;
   const double shift = A.S->shift;
-  const int64_t IJ = (int64_t)A.I * A.J, N = A.N;
-  {
-    const int64_t n = a + (int64_t)A.I * (b + (int64_t)A.J * c);
-    const int64_t h = (a + 1) + (int64_t)A.SX * ((b + 1) + (int64_t)A.SY * (c + 1));
+  const int64_t IJ = (int64_t)A.I * A.J, N = A.N, SXY = (int64_t)A.SX * A.SY;
+  int64_t n = a + (int64_t)A.I * (b + (int64_t)A.J * q.c0);
+  int64_t h = (a + 1) + (int64_t)A.SX * ((b + 1) + (int64_t)A.SY * (q.c0 + 1));
+  for (int c = q.c0; c < q.c1; ++c, n += IJ, h += SXY) {
     // Pp(i+1)-Pp(i): between two active cells the gauge cancels, so the
     // difference is taken on the solver's iterate, whose stiff-phase values
     // sit near zero; faces touching inactive cells or walls use the output
@@ -100,28 +108,39 @@ __global__ void __launch_bounds__(256) k_correct(CorrArgs A) {
     A.u[h] = uu;
     A.v[h] = vv;
     A.w[h] = ww;
+    A.Pp[h] = pP;
   }
 }
 
-// Proj%Pp over the whole halo array: interior values, ghost faces
-// flag*Pp(inner) (ProjectionP.f90:620-643), edges and corners 0.
-__global__ void __launch_bounds__(256) k_pressure_field(CorrArgs A) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..I+1
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..J+1
-  const int k = blockIdx.z;                             // 0..K+1
-  if (i > A.I + 1 || j > A.J + 1) return;
+// Ghost entries of Proj%Pp: faces flag*Pp(inner) (ProjectionP.f90:620-643),
+// edges and corners 0.  blockIdx.z selects the face: 0/1 bottom/top planes
+// (all i, j), 2/3 south/north (all i, interior k), 4/5 west/east (interior
+// j, k).
+__global__ void __launch_bounds__(256) k_pressure_ghosts(CorrArgs A) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = blockIdx.y * blockDim.y + threadIdx.y;
+  const int f = blockIdx.z;
+  int i, j, k;
+  if (f < 2) {
+    if (u > A.I + 1 || w > A.J + 1) return;
+    i = u; j = w; k = f == 0 ? 0 : A.K + 1;
+  } else if (f < 4) {
+    if (u > A.I + 1 || w >= A.K) return;
+    i = u; j = f == 2 ? 0 : A.J + 1; k = w + 1;
+  } else {
+    if (u >= A.J || w >= A.K) return;
+    i = f == 4 ? 0 : A.I + 1; j = u + 1; k = w + 1;
+  }
   const double shift = A.S->shift;
-  const int64_t h = i + (int64_t)A.SX * (j + (int64_t)A.SY * k);
   const bool ix = i >= 1 && i <= A.I, jx = j >= 1 && j <= A.J, kx = k >= 1 && k <= A.K;
   auto P = [&](int ii, int jj, int kk) {
     return pval(A, (ii - 1) + (int64_t)A.I * ((jj - 1) + (int64_t)A.J * (kk - 1)), shift);
   };
   double v = 0.0;
-  if (ix && jx && kx) v = P(i, j, k);
-  else if (!ix && jx && kx) v = (double)A.fl[i == 0 ? 0 : 1] * P(i == 0 ? 1 : A.I, j, k);
+  if (!ix && jx && kx) v = (double)A.fl[i == 0 ? 0 : 1] * P(i == 0 ? 1 : A.I, j, k);
   else if (ix && !jx && kx) v = (double)A.fl[j == 0 ? 2 : 3] * P(i, j == 0 ? 1 : A.J, k);
   else if (ix && jx && !kx) v = (double)A.fl[k == 0 ? 4 : 5] * P(i, j, k == 0 ? 1 : A.K);
-  A.Pp[h] = v;
+  A.Pp[i + (int64_t)A.SX * (j + (int64_t)A.SY * k)] = v;
 }
 
 // mres of the corrected field (Solver.f90:239-247); the west/south/bottom
@@ -129,22 +148,25 @@ __global__ void __launch_bounds__(256) k_pressure_field(CorrArgs A) {
 // not touch.
 __global__ void __launch_bounds__(256) k_divergence(CorrArgs A) {
 #pragma clang fp contract(off)
-  const Cell3 q = cell_of(A.I, A.J, A.K);
+  const Col3 q = col_of(A.I, A.J, A.K);
   const int64_t SXY = (int64_t)A.SX * A.SY;
   double v[1] = {0.0};
   if (q.in) {
-    const int a = q.a, b = q.b, c = q.c;
-    const double dx = A.dxp[a], dy = A.dyp[b], dz = A.dzp[c];
-    const int64_t h = (a + 1) + (int64_t)A.SX * ((b + 1) + (int64_t)A.SY * (c + 1));
-    const double uw = a > 0 ? A.u[h - 1] : A.us[h - 1];
-    const double vs = b > 0 ? A.v[h - A.SX] : A.vs[h - A.SX];
-    const double wb = c > 0 ? A.w[h - SXY] : A.ws[h - SXY];
-    const double r = dy * dz * (A.ee[h] * A.u[h] - A.ee[h - 1] * uw) +
-                     dx * dz * (A.ne[h] * A.v[h] - A.ne[h - A.SX] * vs) +
-                     dx * dy * (A.te[h] * A.w[h] - A.te[h - SXY] * wb);
-    double m = fabs(r) / (dx * dy * dz);
-    if (!(m <= 1.0e300)) m = 1.0e300;  // NaN / Inf
-    v[0] = m;
+    const int a = q.a, b = q.b;
+    const double dx = A.dxp[a], dy = A.dyp[b];
+    int64_t h = (a + 1) + (int64_t)A.SX * ((b + 1) + (int64_t)A.SY * (q.c0 + 1));
+    for (int c = q.c0; c < q.c1; ++c, h += SXY) {
+      const double dz = A.dzp[c];
+      const double uw = a > 0 ? A.u[h - 1] : A.us[h - 1];
+      const double vs = b > 0 ? A.v[h - A.SX] : A.vs[h - A.SX];
+      const double wb = c > 0 ? A.w[h - SXY] : A.ws[h - SXY];
+      const double r = dy * dz * (A.ee[h] * A.u[h] - A.ee[h - 1] * uw) +
+                       dx * dz * (A.ne[h] * A.v[h] - A.ne[h - A.SX] * vs) +
+                       dx * dy * (A.te[h] * A.w[h] - A.te[h - SXY] * wb);
+      double m = fabs(r) / (dx * dy * dz);
+      if (!(m <= 1.0e300)) m = 1.0e300;  // NaN / Inf
+      v[0] = m > v[0] ? m : v[0];
+    }
   }
   block_partials<kTileThreads, 1, 1u>(A.site, v);
 }
@@ -161,7 +183,7 @@ sp_status correct(sp_ctx* c, const sp_inputs& in, sp_outputs& out, sp_stats* st)
   const Dims& D = c->dm;
   CorrArgs A{};
   A.I = D.I; A.J = D.J; A.K = D.K; A.SX = D.SX; A.SY = D.SY; A.N = D.N;
-  const dim3 fg = flat_grid(D.I, D.J, D.K), fb(64, 4, 1);
+  const dim3 fb(64, 4, 1);
   A.x = c->x; A.active = c->active; A.pocoef = c->pocoef;
   A.ctu = in.cell_type_u; A.ctv = in.cell_type_v; A.ctw = in.cell_type_w;
   A.mu = in.moex_u; A.mv = in.moex_v; A.mw = in.moex_w;
@@ -178,25 +200,30 @@ sp_status correct(sp_ctx* c, const sp_inputs& in, sp_outputs& out, sp_stats* st)
     A.pin = -1;
   A.S = c->S;
   const double NB = (double)D.N;
+  const dim3 cg = col_grid(D.I, D.J, D.K);
+  const uint32_t ncol = col_blocks(D.I, D.J, D.K);
   int slot;
-  A.site = red_site(c, kSiteShift);
-  timer_begin(c, "shift", NB * 9, &slot);
-  hipLaunchKernelGGL(k_shift, fg, fb, 0, c->stream, A);
+  // gauge: only singular systems are shifted (S->shift is 0 otherwise)
+  if (c->S_host->singular) {
+    A.site = red_site(c, kSiteShift);
+    timer_begin(c, "shift", NB * 9, &slot);
+    hipLaunchKernelGGL(k_shift, cg, fb, 0, c->stream, A);
+    timer_end(c, slot);
+    hipLaunchKernelGGL(k_shift_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A, ncol);
+  }
+  // x, mask, masks x6, PoCoef x3, u*, v*, w* -> u, v, w, Pp
+  timer_begin(c, "correct", NB * (8 + 1 + 6 * 4 + 3 * 8 + 3 * 8 + 3 * 8 + 8), &slot);
+  hipLaunchKernelGGL(k_correct, cg, fb, 0, c->stream, A);
   timer_end(c, slot);
-  hipLaunchKernelGGL(k_shift_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A,
-                     flat_blocks(D.I, D.J, D.K));
-  timer_begin(c, "correct", NB * (8 + 1 + 3 * 8 + 6 * 4 + 3 * 8 + 3 * 8), &slot);
-  hipLaunchKernelGGL(k_correct, fg, fb, 0, c->stream, A);
-  timer_end(c, slot);
-  dim3 blk(64, 4, 1), grd((D.I + 2 + 63) / 64, (D.J + 2 + 3) / 4, D.K + 2);
-  timer_begin(c, "pressure_field", (double)D.Nh * 8 + NB * 9, &slot);
-  hipLaunchKernelGGL(k_pressure_field, grd, blk, 0, c->stream, A);
-  timer_end(c, slot);
+  {
+    const int gx = std::max(D.I + 2, D.J), gy = std::max(D.J + 2, D.K);
+    dim3 blk(64, 4, 1), grd((gx + 63) / 64, (gy + 3) / 4, 6);
+    hipLaunchKernelGGL(k_pressure_ghosts, grd, blk, 0, c->stream, A);
+  }
   A.site = red_site(c, kSiteDiv);
   timer_begin(c, "divergence", NB * 6 * 8, &slot);
-  hipLaunchKernelGGL(k_divergence, fg, fb, 0, c->stream, A);
-  hipLaunchKernelGGL(k_divergence_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A,
-                     flat_blocks(D.I, D.J, D.K));
+  hipLaunchKernelGGL(k_divergence, cg, fb, 0, c->stream, A);
+  hipLaunchKernelGGL(k_divergence_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A, ncol);
   timer_end(c, slot);
   (void)st;
   return SP_OK;

@@ -268,16 +268,26 @@ struct Col3 {
   int a, b, c0, c1;  // cells (a, b, c) for c0 <= c < c1
   bool in;
 };
+template <int KC = kColK>
 __device__ inline Col3 col_of(int nx, int ny, int nz) {
   uint32_t bx, by, bz;
   xcd_block3(bx, by, bz);
   Col3 r;
   r.a = (int)(bx * blockDim.x + threadIdx.x);
   r.b = (int)(by * blockDim.y + threadIdx.y);
-  r.c0 = (int)bz * kColK;
-  r.c1 = r.c0 + kColK < nz ? r.c0 + kColK : nz;
+  r.c0 = (int)bz * KC;
+  r.c1 = r.c0 + KC < nz ? r.c0 + KC : nz;
   r.in = r.a < nx && r.b < ny;
   return r;
+}
+// Columns of KC cells (launch and partial count)
+template <int KC>
+inline dim3 col_grid_k(int nx, int ny, int nz) {
+  return dim3((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)((nz + KC - 1) / KC));
+}
+template <int KC>
+inline uint32_t col_blocks_k(int nx, int ny, int nz) {
+  return (uint32_t)((nx + 63) / 64) * (uint32_t)((ny + 3) / 4) * (uint32_t)((nz + KC - 1) / KC);
 }
 
 }  // namespace sp

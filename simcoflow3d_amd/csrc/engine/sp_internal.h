@@ -111,6 +111,7 @@ struct sp_ctx {
   sp_params prm;
   int device = 0;
   int num_cu = 256;  // compute units of the device (workgroup sizing)
+  int desc_state = -1;      // has_e of the uploaded level table, -1: stale
   bool fuse_dots = false;  // the finest post-smoother also forms (r,z), (z,q), (dw,z)
   uint32_t dots_nb = 0;    // workgroups of that post-smoother (its partials)
   hipStream_t stream = nullptr;

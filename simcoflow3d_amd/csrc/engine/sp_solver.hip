@@ -408,25 +408,30 @@ __device__ inline double apply_row(const CgArgs& A, const double* v, int a, int 
 
 // r = b - mean(b) - A x  (x = 0 when from_zero); resets p and q
 __global__ void __launch_bounds__(256) k_cg_residual(CgArgs A, int from_zero) {
-  const Cell3 q = cell_of(A.I, A.J, A.K);
+  const Col3 q = col_of(A.I, A.J, A.K);
   Scalars* S = A.S;
   const double bmean = S->bmean;
   double v[3] = {0.0, 0.0, 0.0};  // max |r|/V, non-finite, sum r
   if (q.in) {
-    const int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c);
-    double rv = 0.0;
-    if (A.active[n]) {
-      rv = A.b[n] - bmean;
-      if (!from_zero) rv -= apply_row(A, A.x, q.a, q.b, q.c, n);
-      v[0] = fabs(rv) / (A.dxp[q.a] * A.dyp[q.b] * A.dzp[q.c]);
-      if (!is_finite(rv)) v[1] = 1.0;
-      v[2] = rv;
+    const int64_t IJ = (int64_t)A.I * A.J;
+    const double dxy = A.dxp[q.a] * A.dyp[q.b];
+    int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c0);
+    for (int c = q.c0; c < q.c1; ++c, n += IJ) {
+      double rv = 0.0;
+      if (A.active[n]) {
+        rv = A.b[n] - bmean;
+        if (!from_zero) rv -= apply_row(A, A.x, q.a, q.b, c, n);
+        const double rr = fabs(rv) / (dxy * A.dzp[c]);
+        v[0] = (rr > v[0] || rr != rr) ? rr : v[0];
+        if (!is_finite(rv)) v[1] = 1.0;
+        v[2] += rv;
+      }
+      A.r[n] = rv;
+      if (A.r32) A.r32[n] = (float)rv;
+      if (from_zero) A.x[n] = 0.0;
+      A.q[n] = 0.0;
+      A.p[n] = 0.0;
     }
-    A.r[n] = rv;
-    if (A.r32) A.r32[n] = (float)rv;
-    if (from_zero) A.x[n] = 0.0;
-    A.q[n] = 0.0;
-    A.p[n] = 0.0;
   }
   block_partials<kTileThreads, 3, 3u>(A.site, v);
 }
@@ -818,15 +823,19 @@ sp_status setup_levels(sp_ctx* c, Level<T>* lv) {
                        view_of(lv[l - 1]), view_of(C), C.cx, C.cy, C.cz, C.e,
                        (T)c->prm.mg_coarse_scale);
   }
-  // level table for the tail kernel
-  std::vector<LevelView<T>> desc(kMaxLevels);
-  for (int l = 0; l < L; ++l) desc[l] = view_of(lv[l]);
-  hipError_t e = hipMemcpyAsync(c->mg_desc, desc.data(), sizeof(LevelView<T>) * kMaxLevels,
-                                hipMemcpyHostToDevice, c->stream);
-  if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(mg_desc)");
-  // the host table must stay valid until the copy has been consumed
-  e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) return hip_check(c, e, "mg setup");
+  // level table for the tail kernel: it changes only with the buffers and
+  // with the presence of diagonal-only terms
+  if (c->desc_state != (c->has_e ? 1 : 0)) {
+    std::vector<LevelView<T>> desc(kMaxLevels);
+    for (int l = 0; l < L; ++l) desc[l] = view_of(lv[l]);
+    hipError_t e = hipMemcpyAsync(c->mg_desc, desc.data(), sizeof(LevelView<T>) * kMaxLevels,
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(mg_desc)");
+    // the host table must stay valid until the copy has been consumed
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "mg setup");
+    c->desc_state = c->has_e ? 1 : 0;
+  }
   return SP_OK;
 }
 
@@ -845,7 +854,7 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st) {
   const Dims& D = c->dm;
   const bool mixed = c->prm.precision == SP_PREC_MIXED;
   const bool mg = c->prm.precond == SP_PRECOND_MG;
-  const dim3 grd = flat_grid(D.I, D.J, D.K), blk(64, 4, 1);
+  const dim3 blk(64, 4, 1);
   if (flat_blocks(D.I, D.J, D.K) > c->red_max_blocks)
     return fail(c, SP_ERR_INVALID_ARG, "grid too large for the reduction buffers");
   CgArgs A{};
@@ -870,15 +879,8 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st) {
   const double eb = c->has_e ? 8.0 : 0.0;
   const double zb = (mixed && mg) ? 4.0 : 8.0;
 
-  const uint32_t nblk = flat_blocks(D.I, D.J, D.K);
   const uint32_t ncol = col_blocks(D.I, D.J, D.K);
   const dim3 cgrd = col_grid(D.I, D.J, D.K);
-  auto launch = [&](auto kern, int site, const char* name, double bytes, auto... extra) {
-    A.site = red_site(c, site);
-    timer_begin(c, name, bytes, &slot);
-    hipLaunchKernelGGL(kern, grd, blk, 0, c->stream, A, extra...);
-    timer_end(c, slot);
-  };
   auto launch_col = [&](auto kern, int site, const char* name, double bytes) {
     A.site = red_site(c, site);
     timer_begin(c, name, bytes, &slot);
@@ -892,9 +894,12 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st) {
     timer_end(c, slot);
   };
   auto enqueue_residual = [&](int from_zero) {
-    launch(k_cg_residual, kSiteResidual, "cg_residual",
-           NB * (from_zero ? 8 + 1 + 8 * 4 : 8 * 5 + eb + 1 + 8 * 4), from_zero);
-    finalize(k_cg_residual_fin, kSiteResidual, nblk);
+    A.site = red_site(c, kSiteResidual);
+    timer_begin(c, "cg_residual", NB * (from_zero ? 8 + 1 + 8 * 4 : 8 * 5 + eb + 1 + 8 * 4),
+                &slot);
+    hipLaunchKernelGGL(k_cg_residual, cgrd, blk, 0, c->stream, A, from_zero);
+    timer_end(c, slot);
+    finalize(k_cg_residual_fin, kSiteResidual, ncol);
   };
   auto precondition = [&]() {
     if (mg) {
