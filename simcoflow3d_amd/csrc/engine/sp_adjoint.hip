@@ -220,7 +220,7 @@ sp_status adjoint(sp_ctx* c, const sp_adj_inputs& in, sp_adj_outputs& out, sp_st
   if (e != hipSuccess) return hip_check(c, e, "adjoint source");
   double rel = c->fwd_initial_div > 0.0 ? c->prm.div_tol / c->fwd_initial_div : 1e-10;
   rel = std::min(std::max(rel, 1e-13), 1e-10);
-  sp_status s = solve(c, c->pbar, rel * c->S_host->src_max, st);
+  sp_status s = solve(c, c->pbar, rel * c->S_host->src_max, st, true, false);
   if (s != SP_OK) return s;
   A.site = red_site(c, kSiteAdj2);
   timer_begin(c, "adj_lam_mean", NB * 9, &slot);

@@ -265,11 +265,24 @@ static sp_status project_device(sp_ctx* c, const sp_inputs* in, sp_outputs* out,
   S->active_cells = c->S_host->active;
   S->stale_faces = stale;
   c->fwd_initial_div = c->S_host->bmax;
-  s = solve(c, c->b, c->prm.div_tol, S);
+  s = solve(c, c->b, c->prm.div_tol, S, false, false);
   if (s != SP_OK && s != SP_ERR_NOT_CONVERGED) return s;
   auto t2 = std::chrono::steady_clock::now();
-  sp_status sc = correct(c, *in, *out, S);
-  if (sc) return sc;
+  // correction and divergence of the corrected field; the divergence is the
+  // true residual of the solve, so a drift of the recursive residual shows
+  // here and the solve resumes from the true residual (at most 3 times)
+  for (int restart = 0;; ++restart) {
+    sp_status sc = correct(c, *in, *out, S);
+    if (sc) return sc;
+    e = hipMemcpyAsync(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "correction");
+    if (c->S_host->divmax <= c->prm.div_tol || c->S_host->nonfinite || restart >= 3 ||
+        c->S_host->iter >= c->prm.max_iter)
+      break;
+    s = solve(c, c->b, c->prm.div_tol, S, false, true);
+    if (s != SP_OK && s != SP_ERR_NOT_CONVERGED) return s;
+  }
   if (out->po_coef) {
     e = hipMemcpyAsync(out->po_coef, c->pocoef, sizeof(double) * 6 * D.N, hipMemcpyDeviceToDevice,
                        c->stream);

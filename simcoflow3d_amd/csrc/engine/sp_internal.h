@@ -208,7 +208,8 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_count);
 
 // ---- solver (sp_solver.hip) ----
 sp_status mg_setup(sp_ctx* c);
-sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st);
+sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st, bool check_true,
+                bool resume);
 
 // ---- correction (sp_correct.hip) ----
 sp_status correct(sp_ctx* c, const sp_inputs& in, sp_outputs& out, sp_stats* st);

@@ -850,7 +850,15 @@ sp_status mg_setup(sp_ctx* c) {
 // Solve A x = rhs - mean (singular) with the multigrid-preconditioned
 // flexible CG until max |r_i| / V_i <= tol.  The operator arrays and c->S
 // (bmean, singular) come from the assembly.  Writes x into c->x.
-sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st) {
+//   check_true: after convergence of the recursive residual, form the true
+//               residual and restart from it (at most 3 times) if it is
+//               above tol.  The forward projection instead measures the
+//               divergence of the corrected field, which is the same
+//               quantity, and resumes when needed.
+//   resume:     continue from the current x (true residual, iteration count
+//               kept) instead of starting from x = 0.
+sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st, bool check_true,
+                bool resume) {
   const Dims& D = c->dm;
   const bool mixed = c->prm.precision == SP_PREC_MIXED;
   const bool mg = c->prm.precond == SP_PRECOND_MG;
@@ -926,8 +934,15 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st) {
   hipEvent_t evs[2];
   hipEventCreateWithFlags(&evs[0], hipEventDisableTiming);
   hipEventCreateWithFlags(&evs[1], hipEventDisableTiming);
-  enqueue_residual(1);
   int it = 0;
+  if (resume) {
+    it = c->S_host->iter;
+    hipError_t e0 = hipMemsetAsync(&c->S->done, 0, sizeof(int32_t), c->stream);
+    if (e0 != hipSuccess) return hip_check(c, e0, "hipMemsetAsync");
+    enqueue_residual(0);
+  } else {
+    enqueue_residual(1);
+  }
   int restarts = 0;
   const int max_iter = std::max(1, c->prm.max_iter);
   bool finished = false;
@@ -964,7 +979,7 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st) {
     e = hipMemcpy(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost);
     if (e != hipSuccess) { hipEventDestroy(evs[0]); hipEventDestroy(evs[1]); return hip_check(c, e, "solve"); }
     const int iters = c->S_host->iter;
-    if (c->S_host->nonfinite) break;
+    if (c->S_host->nonfinite || !check_true) break;
     // true residual of the final iterate
     hipMemsetAsync(&c->S->done, 0, sizeof(int32_t), c->stream);
     enqueue_residual(0);
