@@ -113,7 +113,10 @@ typedef struct sp_outputs {
 
 typedef struct sp_stats {
   int32_t iterations;
-  int32_t converged;
+  int32_t converged;      /* final_div <= div_tol, or the true residual max|b - A p|/V
+                             is: with Dirichlet pressure faces, a vof - vofO source
+                             or masked open faces the reference's divergence of the
+                             corrected field is not the residual of the system */
   int32_t levels;
   int32_t singular;       /* all-Neumann operator (gauge applied) */
   int64_t active_cells;

@@ -285,12 +285,13 @@ class OracleResult:
     d: np.ndarray
 
 
-def project(case, pin=(1, 1, 1), tol=None, rows=True, o=None, u_init=None):
+def project(case, pin=(1, 1, 1), tol=None, rows=True, o=None, u_init=None, bc_val=None):
     """Reference path on the CPU.  ``rows=True`` solves the reference's own
     assembled rows (ProjectionP.f90:330-478); ``rows=False`` the engine's
-    symmetric face operator.  ``tol`` defaults to a machine-precision stop."""
+    symmetric face operator.  ``tol`` defaults to a machine-precision stop.
+    ``bc_val``: face values of the Dirichlet pressure faces (W,E,S,N,B,T)."""
     pc = gfm_coef(case)
-    b = rhs(case, pc)
+    b = rhs(case, pc, bc_val)
     op = sym_operator(case, pc)
     d, _ = sym_diag(case, *op)
     o = o or opts(max_iter=1000, div_tol=tol if tol is not None else 0.0)
