@@ -197,8 +197,7 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
 #pragma unroll
   for (int s = 1; s <= H; ++s) {
     const int o = MODE == 0 ? s - 1 : s;
-    const int p = t - o;
-    if (p < 0 || p >= A.nz) continue;  // uniform over the workgroup
+    // planes outside [0, nz) have d = 0 and stay 0, so no range test
     T v;
     if (MODE == 0 && s == 1) {
       v = kRcp ? F.bw[S(0)] * F.rdw[S(0)] : F.bw[S(0)] / F.dw[S(0)];
@@ -214,6 +213,9 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
       acc += F.czw[S(o)] * F.xw[S(o - 1)];
       v = kRcp ? (F.bw[S(o)] + acc) * F.rdw[S(o)] : (F.bw[S(o)] + acc) / F.dw[S(o)];
     }
+    // keep v in a register on every lane: a select whose operand is only
+    // needed on some lanes would otherwise become an exec-masked branch
+    __asm__ volatile("" : "+v"(v));
     const T xn = (act && F.dw[S(o)] > T(0)) ? v : F.xw[S(o)];
     F.xw[S(o)] = xn;
     xs[Q(o)][ly][lx] = xn;
