@@ -8,6 +8,8 @@ inside FP64 CG) projection solves/s; the all-FP64 solver is timed the same
 way in the same run.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--no-cpu]
+    python bench.py --all-configs     # configs 0-4 at full size, mixed and FP64,
+                                      # plus a forward+adjoint pass (one JSON line each)
 """
 
 import argparse
@@ -40,6 +42,11 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--levels", type=int, default=0)
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the host-array (PCIe-inclusive) timing")
+    ap.add_argument("--all-configs", action="store_true",
+                    help="configs 0-4 at full size: mixed and FP64 solves/s, the mixed "
+                         "vs FP64 pressure error and a timed forward+adjoint pass")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 1 GiB device-to-device copy (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -126,12 +133,117 @@ def kernel_table(P, dev, out, case):
     return table
 
 
+def median(v):
+    v = sorted(v)
+    n = len(v)
+    return 0.5 * (v[(n - 1) // 2] + v[n // 2]) if n else 0.0
+
+
+def host_arrays(P, case, steps=3):
+    """End-to-end rate through the C ABI with host arrays (sp_project): every
+    input is copied host-to-device and p, u, v, w back, from pageable numpy
+    arrays as a Fortran host would pass them.  Not the headline value."""
+    import numpy as np
+    I, J, K = case.shape
+    hs = (I + 2, J + 2, K + 2)
+    fields = {n: case.fields[n] for n in case.fields}
+    out = {n: np.zeros(hs, order="F") for n in ("p", "u", "v", "w")}
+    P.project(fields, case.bc_flag, case.dt, out=out)  # stages the device buffers
+    t0 = time.perf_counter()
+    sts = [P.project(fields, case.bc_flag, case.dt, out=out)[1] for _ in range(steps)]
+    el = time.perf_counter() - t0
+    return dict(solves_per_s=round(steps / el, 3), ms_per_step=round(el / steps * 1e3, 3),
+                device_ms_per_step=round(median([s.t_solve_ms + s.t_assemble_ms + s.t_correct_ms
+                                                 for s in sts]), 3),
+                steps=steps, iterations=sts[-1].iterations, final_div=sts[-1].final_div,
+                note="pageable host arrays in and out of sp_project; includes PCIe copies")
+
+
+def face_masks(case):
+    """1 on the faces a predictor may carry (interior, not wall-normal)."""
+    import torch
+    I, J, K = case.shape
+    hs = (K + 2, J + 2, I + 2)
+    ms = []
+    for d in range(3):
+        m = torch.zeros(hs, dtype=torch.float64, device="cuda")
+        m[1:-1, 1:-1, 1:-1] = 1.0
+        if d == 0:
+            m[:, :, I] = 0.0
+        elif d == 1:
+            m[:, J, :] = 0.0
+        else:
+            m[K, :, :] = 0.0
+        ms.append(m)
+    return ms
+
+
+def all_configs(args):
+    """Configs 0-4 at full size: one JSON line each."""
+    import torch
+
+    from simcoflow3d_amd import configs
+    from simcoflow3d_amd.engine import Projector, SP_PREC_FP64, SP_PREC_MIXED
+
+    steps = max(1, args.steps)
+    for cfg in range(5):
+        case = configs.generate(cfg)
+        I, J, K = case.shape
+        dev = to_device(case)
+        out = make_outputs(case, dev)
+        common = dict(rho_l=case.rho_l, rho_g=case.rho_g, rho_ref=case.rho_l, pin_i=1, pin_j=1,
+                      pin_k=1, div_tol=1e-10)
+        rec = {"config": cfg, "workload": WORKLOADS[cfg], "grid": [I, J, K]}
+        ps = {}
+        for name, prec in (("mixed", SP_PREC_MIXED), ("fp64", SP_PREC_FP64)):
+            P = Projector(case.shape, precision=prec, **common)
+            P.set_grid_1d(case.dxp, case.dyp, case.dzp, case.dxu, case.dyv, case.dzw)
+            run_solves(P, dev, out, case, 2)
+            el, sts, _ = timed(P, dev, out, case, steps)
+            ps[name] = out["p"].clone()
+            s = sts[-1]
+            rec[name] = dict(solves_per_s=round(steps / el, 3), ms_per_step=round(el / steps * 1e3, 3),
+                             median_ms_per_step=round(median([x.t_total_ms for x in sts]), 3),
+                             iterations=s.iterations, final_div=s.final_div,
+                             converged=bool(s.converged))
+            # one forward + adjoint pass (reverse mode of the same projection)
+            gen = torch.Generator(device="cuda").manual_seed(11)
+            g = {n: m * torch.rand(m.shape, generator=gen, device="cuda", dtype=torch.float64)
+                 for n, m in zip(("gu", "gv", "gw"), face_masks(case))}
+            hs = out["p"].shape
+            res = {n: torch.zeros(hs, dtype=torch.float64, device="cuda")
+                   for n in ("gu_star", "gv_star", "gw_star")}
+            for n in ("gk_e", "gk_n", "gk_t"):
+                res[n] = torch.zeros((K, J, I), dtype=torch.float64, device="cuda")
+            P.adjoint_dev(g, res)  # warm
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                P.project_dev(dev, case.bc_flag, case.dt, out)
+                ast = P.adjoint_dev(g, res)
+            torch.cuda.synchronize()
+            ela = time.perf_counter() - t0
+            rec[name]["forward_adjoint_per_s"] = round(steps / ela, 3)
+            rec[name]["adjoint_iterations"] = ast.iterations
+            rec[name]["adjoint_converged"] = bool(ast.converged)
+            P.close()
+        pm, p64 = ps["mixed"], ps["fp64"]
+        rec["mixed_p_rel_err_vs_fp64"] = float((pm - p64).abs().max() / p64.abs().max())
+        rec["mixed_speedup"] = round(rec["mixed"]["solves_per_s"] / rec["fp64"]["solves_per_s"], 3)
+        print(json.dumps(rec), flush=True)
+        del dev, out, ps
+        torch.cuda.empty_cache()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only in this build)", file=sys.stderr)
         sys.exit(2)
+    if args.all_configs:
+        all_configs(args)
+        return
     import numpy as np
     import torch
 
@@ -171,6 +283,8 @@ def main():
     p_mixed = out["p"].cpu().numpy()
     value = args.steps / el
     ms_per_step = el / args.steps * 1e3
+    median_ms = median([st.t_total_ms for st in st_m])
+    host = None if args.no_host else host_arrays(Pm, case)
     Pm.close()
 
     fp64 = {}
@@ -184,6 +298,7 @@ def main():
         P64.close()
         s64 = st_64[-1]
         fp64 = dict(solves_per_s=args.steps / el64, ms_per_step=el64 / args.steps * 1e3,
+                    median_ms_per_step=median([st.t_total_ms for st in st_64]),
                     iterations=s64.iterations, final_div=s64.final_div,
                     dominant=(ktab64[0]["name"] if ktab64 else None))
         err = float(np.max(np.abs(p_mixed - p64)) / max(np.max(np.abs(p64)), 1e-300))
@@ -233,6 +348,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 3),
+        "median_ms_per_step": round(median_ms, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -243,6 +359,7 @@ def main():
                    "iterations": sm.iterations, "final_div": sm.final_div,
                    "converged": bool(sm.converged)},
         "fp64": fp64,
+        "host_arrays": host,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": [dict(name=k["name"], launches=k["launches"], total_ms=round(k["total_ms"], 3),

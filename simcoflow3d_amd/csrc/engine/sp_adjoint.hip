@@ -212,14 +212,16 @@ sp_status adjoint(sp_ctx* c, const sp_adj_inputs& in, sp_adj_outputs& out, sp_st
                      flat_blocks(D.I, D.J, D.K));
   timer_end(c, slot);
   // Solve A lam = pbar (mean-projected when singular) with the forward's
-  // operator and multigrid hierarchy, to the relative reduction the forward
-  // was asked for (div_tol / initial divergence), kept within [1e-13, 1e-10]:
-  // the adjoint source has other units than the divergence.
+  // operator and multigrid hierarchy until max |r|/V <= 1e-10 max |pbar|/V.
+  // The adjoint source has other units than the divergence, so the
+  // tolerance is relative to it; on a stiff two-phase operator the residual
+  // of a source with a large net flux levels off near 1e-12 of the source
+  // (earlier with the FP32 V-cycle than with the FP64 one), and gradients to
+  // 1e-8 need far less.
   hipError_t e = hipMemcpyAsync(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_check(c, e, "adjoint source");
-  double rel = c->fwd_initial_div > 0.0 ? c->prm.div_tol / c->fwd_initial_div : 1e-10;
-  rel = std::min(std::max(rel, 1e-13), 1e-10);
+  const double rel = 1e-10;
   sp_status s = solve(c, c->pbar, rel * c->S_host->src_max, st, true, false);
   if (s != SP_OK) return s;
   A.site = red_site(c, kSiteAdj2);

@@ -264,7 +264,6 @@ static sp_status project_device(sp_ctx* c, const sp_inputs* in, sp_outputs* out,
   S->singular = c->S_host->singular;
   S->active_cells = c->S_host->active;
   S->stale_faces = stale;
-  c->fwd_initial_div = c->S_host->bmax;
   s = solve(c, c->b, c->prm.div_tol, S, false, false);
   if (s != SP_OK && s != SP_ERR_NOT_CONVERGED) return s;
   auto t2 = std::chrono::steady_clock::now();

@@ -173,7 +173,6 @@ struct sp_ctx {
 
   // adjoint work
   double *pbar = nullptr, *lam = nullptr;
-  double fwd_initial_div = 0.0;  // max |b|/V of the last forward (adjoint tolerance)
 
   // bc values staged
   double* bcv[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
