@@ -146,7 +146,8 @@ typedef struct sp_adj_outputs {
  * scale 0.5, 200 iterations, div_tol 1e-10, rho 998.3/1.205, epsi 1e-3. */
 void sp_params_default(sp_params* p);
 
-/* Create a context on HIP device `device`. */
+/* Create a context on HIP device `device`.  The grid needs at least 2 cells
+ * per direction and fewer than 2^29 cells (SP_ERR_INVALID_ARG otherwise). */
 sp_status sp_create(sp_ctx** ctx, const sp_params* params, int device);
 sp_status sp_destroy(sp_ctx* ctx);
 

@@ -190,7 +190,12 @@ static sp_status ensure_buffers(sp_ctx* c) {
 
 static bool check_params(const sp_params* p, std::string& why) {
   if (p->imax < 2 || p->jmax < 2 || p->kmax < 2) { why = "imax, jmax, kmax must be >= 2"; return false; }
-  if ((int64_t)p->imax * p->jmax * p->kmax >= ((int64_t)1 << 31)) { why = "grid too large"; return false; }
+  // the smoother kernels address level arrays with 32-bit byte offsets, and
+  // the FP64 ones reach 8 bytes per cell
+  if ((int64_t)p->imax * p->jmax * p->kmax >= ((int64_t)1 << 29)) {
+    why = "grid too large (at most 2^29 - 1 cells)";
+    return false;
+  }
   if (p->precision != SP_PREC_FP64 && p->precision != SP_PREC_MIXED) { why = "precision"; return false; }
   if (p->precond != SP_PRECOND_MG && p->precond != SP_PRECOND_JACOBI) { why = "precond"; return false; }
   if (p->mg_pre < 1 || p->mg_post < 0 || p->mg_coarse_sweeps < 1) { why = "sweeps"; return false; }

@@ -49,7 +49,9 @@ def test_invalid_arguments_are_rejected_without_a_device():
     L.sp_params_default(ctypes.byref(p))
     p.imax, p.jmax, p.kmax = 1, 8, 8  # grids need at least 2 cells per direction
     assert L.sp_create(ctypes.byref(h), ctypes.byref(p), 0) == engine.SP_ERR_INVALID_ARG
-    p.imax = 8
+    p.imax, p.jmax, p.kmax = 1024, 1024, 512  # 2^29 cells: beyond 32-bit byte offsets of FP64 levels
+    assert L.sp_create(ctypes.byref(h), ctypes.byref(p), 0) == engine.SP_ERR_INVALID_ARG
+    p.imax, p.jmax, p.kmax = 8, 8, 8
     p.precision = 7
     assert L.sp_create(ctypes.byref(h), ctypes.byref(p), 0) == engine.SP_ERR_INVALID_ARG
     assert L.sp_project(None, None, None, None) == engine.SP_ERR_INVALID_ARG

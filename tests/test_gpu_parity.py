@@ -75,6 +75,52 @@ def test_fused_smoother_matches_plain_sweeps(gpu_available, cfg, prec):
     assert H.rel_inf(a["p"], b["p"]) <= 1e-9
 
 
+RAGGED = [(0, (17, 23, 11)), (1, (9, 31, 14)), (4, (33, 7, 19)), (3, (2, 2, 2)), (0, (3, 5, 2)),
+          (1, (40, 3, 6)), (2, (66, 34, 18))]
+
+
+@pytest.mark.parametrize("cfg,size", RAGGED)
+@pytest.mark.parametrize("prec", (0, 1))
+def test_ragged_and_tiny_grids_match_oracle(gpu_available, cfg, size, prec):
+    """Odd sizes (odd coarse levels, partial tiles and k-chunks), the smallest
+    grids the interface accepts (2 cells per direction) and flat ones (3
+    cells): same bars as the configurations."""
+    c = H.case(cfg, size=size)
+    print(f"[ragged] cfg={cfg} size={size} prec={prec}", flush=True)
+    ref = orc.project(c, pin=(1, 1, 1))
+    P = H.projector(c, precision=prec)
+    out, st = P.project(H.fields(c), c.bc_flag, c.dt, po_coef=True)
+    P.close()
+    assert st.converged == 1 and st.final_div <= 1e-10, st.as_dict()
+    assert np.array_equal(out["po_coef"], ref.pocoef)
+    assert H.rel_inf(out["p"], ref.Pp) <= 1e-8
+    umax = max(np.max(np.abs(c.u_star)), np.max(np.abs(c.v_star)), np.max(np.abs(c.w_star)))
+    for n, r in (("u", ref.u), ("v", ref.v), ("w", ref.w)):
+        d = out[n][1:-1, 1:-1, 1:-1] - r[1:-1, 1:-1, 1:-1]
+        assert np.max(np.abs(d)) <= 1e-8 * umax, n
+
+
+@pytest.mark.parametrize("prec", (0, 1))
+def test_all_solid_domain(gpu_available, prec):
+    """No fluid cell at all: nothing to solve; p = 0 and the corrected
+    velocity is 0 on every solid face (ComputePUVW.f90:185-195), as the
+    oracle gives."""
+    c = H.case(0, size=(8, 8, 8))
+    for n in ("cell_type_p", "cell_type_u", "cell_type_v", "cell_type_w"):
+        c.fields[n][...] = 2
+    for n in ("ee_area", "ne_area", "te_area", "vof", "vof_old"):
+        c.fields[n][...] = 0.0
+    ref = orc.project(c, pin=(1, 1, 1))
+    P = H.projector(c, precision=prec)
+    out, st = P.project(H.fields(c), c.bc_flag, c.dt)
+    P.close()
+    assert st.converged == 1 and st.active_cells == 0, st.as_dict()
+    assert st.final_div == 0.0
+    assert np.all(out["p"] == 0.0) and np.all(ref.Pp == 0.0)
+    for n in ("u", "v", "w"):
+        assert np.all(out[n][1:-1, 1:-1, 1:-1] == 0.0), n
+
+
 DIRICHLET = [((0, 1, 1, 1, 1, 1), 0, 0), ((1, 1, 1, 1, 0, 0), 1, 1), ((1, 0, 0, 1, 1, 1), 4, 0)]
 
 
