@@ -518,69 +518,62 @@ __global__ void __launch_bounds__(kFinThreads) k_cg_dots3_fin(CgArgs A, uint32_t
   }
 }
 
-// p = (z - m) + beta p on active cells
-__global__ void __launch_bounds__(256) k_cg_pupdate(CgArgs A) {
-  const Scalars* S = A.S;
-  if (S->done) return;
-  const Col3 q = col_of(A.I, A.J, A.K);
-  if (!q.in) return;
-  const double beta = S->beta, m = S->zshift;
-  const int64_t IJ = (int64_t)A.I * A.J;
-  int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c0);
-#pragma unroll 4
-  for (int c = q.c0; c < q.c1; ++c, n += IJ) {
-    const double z = zval(A, n), po = A.p[n];
-    A.p[n] = A.active[n] ? (z - m) + beta * po : 0.0;
-  }
-}
-
-// q = A p ; pq = (p, q).  Each thread walks its column keeping p and the
-// z-coupling of planes k-1 / k / k+1 in registers; in-plane neighbours come
-// through the cache.  Neighbour indices are clamped at the domain edge,
-// where the coupling is zero.
-__global__ void __launch_bounds__(256) k_cg_spmv(CgArgs A) {
+// The search direction and its image in one pass:
+//   p = (z - m) + beta p_old,   q = A p = A z - m e + beta q_old
+// on active cells (0 elsewhere), with (p, q) and sum q for alpha.  A 1 is the
+// diagonal-only term e on active rows (the face couplings of a row sum to its
+// diagonal minus e), so the stencil is applied to z, which is complete in
+// memory, and p and q need no neighbours.  Each thread walks its column
+// keeping z and the z-coupling of planes k-1 / k / k+1 in registers.
+template <typename Z>
+__global__ void __launch_bounds__(256) k_cg_pspmv(CgArgs A, const Z* zv) {
   Scalars* S = A.S;
   if (S->done) return;
   const Col3 q = col_of(A.I, A.J, A.K);
   double v[2] = {0.0, 0.0};  // (p, q), sum q
   if (q.in) {
+    const double beta = S->beta, m = S->zshift;
     const int64_t I = A.I, IJ = (int64_t)A.I * A.J;
     const int a = q.a, b = q.b;
     const int64_t dW = a > 0 ? 1 : 0, dE = a < A.I - 1 ? 1 : 0;
     const int64_t dS = b > 0 ? I : 0, dN = b < A.J - 1 ? I : 0;
-    const double* P = A.p;
     int64_t n = a + I * (b + (int64_t)A.J * q.c0);
-    double pm = 0.0, czm = 0.0;
+    double zm = 0.0, czm = 0.0;
     if (q.c0 > 0) {
-      pm = P[n - IJ];
+      zm = (double)zv[n - IJ];
       czm = A.cz[n - IJ];
     }
-    double pc = P[n];
+    double zc = (double)zv[n];
     for (int c = q.c0; c < q.c1; ++c, n += IJ) {
-      const double pp = c < A.K - 1 ? P[n + IJ] : 0.0;
+      const double zp = c < A.K - 1 ? (double)zv[n + IJ] : 0.0;
       const double cxm = a > 0 ? A.cx[n - dW] : 0.0;
       const double cym = b > 0 ? A.cy[n - dS] : 0.0;
       const double cx = A.cx[n], cy = A.cy[n], cz = A.cz[n];
-      const double pw = P[n - dW], pe = P[n + dE], ps = P[n - dS], pn = P[n + dN];
+      const double zw = (double)zv[n - dW], ze = (double)zv[n + dE];
+      const double zs = (double)zv[n - dS], zn = (double)zv[n + dN];
+      const double en = A.e ? A.e[n] : 0.0;
       double d = cxm + cym;
       d = d + czm;
       d = d + cx;
       d = d + cy;
       d = d + cz;
-      if (A.e) d = d + A.e[n];
-      double sacc = d * pc;
-      sacc -= cxm * pw;
-      sacc -= cym * ps;
-      sacc -= czm * pm;
-      sacc -= cx * pe;
-      sacc -= cy * pn;
-      sacc -= cz * pp;
-      const double qv = A.active[n] ? sacc : 0.0;
+      d = d + en;
+      double az = d * zc;
+      az -= cxm * zw;
+      az -= cym * zs;
+      az -= czm * zm;
+      az -= cx * ze;
+      az -= cy * zn;
+      az -= cz * zp;
+      const bool act = A.active[n] != 0;
+      const double pv = act ? (zc - m) + beta * A.p[n] : 0.0;
+      const double qv = act ? (az - m * en) + beta * A.q[n] : 0.0;
+      A.p[n] = pv;
       A.q[n] = qv;
-      v[0] += pc * qv;
+      v[0] += pv * qv;
       v[1] += qv;
-      pm = pc;
-      pc = pp;
+      zm = zc;
+      zc = zp;
       czm = cz;
     }
   }
@@ -959,8 +952,16 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st, bool che
         launch_col(k_cg_dots, kSiteDots, "cg_dots", NB * (8 + zb + 8 + 4 + 1));
         finalize(k_cg_dots_fin, kSiteDots, ncol);
       }
-      launch_col(k_cg_pupdate, kSiteDots, "cg_pupdate", NB * (zb + 8 + 8 + 1));
-      launch_col(k_cg_spmv, kSiteSpmv, "cg_spmv", NB * (8 + 24 + eb + 1 + 8));
+      // z (stencil), p, q, three couplings, e, mask -> p, q
+      {
+        A.site = red_site(c, kSiteSpmv);
+        timer_begin(c, "cg_pspmv", NB * (zb + 8 + 8 + 24 + eb + 1 + 8 + 8), &slot);
+        if (A.z32)
+          hipLaunchKernelGGL(k_cg_pspmv<float>, cgrd, blk, 0, c->stream, A, (const float*)A.z32);
+        else
+          hipLaunchKernelGGL(k_cg_pspmv<double>, cgrd, blk, 0, c->stream, A, A.z64);
+        timer_end(c, slot);
+      }
       finalize(k_cg_spmv_fin, kSiteSpmv, ncol);
       launch_col(k_cg_update, kSiteUpdate, "cg_update", NB * (8 * 4 + 8 * 2 + (A.r32 ? 4 : 0)));
       finalize(k_cg_update_fin, kSiteUpdate, ncol);
