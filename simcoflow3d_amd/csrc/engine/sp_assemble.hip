@@ -145,8 +145,11 @@ __global__ void __launch_bounds__(256) k_gfm_col_last(GfmArgs A) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cols * 6) return;
   const int f = (int)(t / cols);
-  const int64_t col = t % cols;          // col = a*J + b (i outer, j inner)
-  const int a = (int)(col / A.J), b = (int)(col % A.J);
+  // consecutive threads take consecutive i (coalesced loads); the column's
+  // loop position is col = a*J + b (i outer, j inner)
+  const int64_t u = t % cols;
+  const int a = (int)(u % A.I), b = (int)(u / A.I);
+  const int64_t col = (int64_t)a * A.J + b;
   int64_t lastW = -1, lastD = -1;
   for (int c = A.K - 1; c >= 0 && (lastW < 0 || lastD < 0); --c) {
     double beta, pref;

@@ -519,17 +519,20 @@ __global__ void __launch_bounds__(kFinThreads) k_cg_dots3_fin(CgArgs A, uint32_t
 }
 
 // The search direction and its image in one pass:
+//   x += alpha p_old            (the x half of the previous CG step)
 //   p = (z - m) + beta p_old,   q = A p = A z - m e + beta q_old
 // on active cells (0 elsewhere), with (p, q) and sum q for alpha.  A 1 is the
 // diagonal-only term e on active rows (the face couplings of a row sum to its
 // diagonal minus e), so the stencil is applied to z, which is complete in
 // memory, and p and q need no neighbours.  Each thread walks its column
-// keeping z and the z-coupling of planes k-1 / k / k+1 in registers.
+// keeping z and the z-coupling of planes k-1 / k / k+1 in registers.  After
+// the last CG step k_cg_xflush applies its x half.
 template <typename Z>
 __global__ void __launch_bounds__(256) k_cg_pspmv(CgArgs A, const Z* zv) {
   Scalars* S = A.S;
   if (S->done) return;
   const Col3 q = col_of(A.I, A.J, A.K);
+  const double alpha = S->alpha;
   double v[2] = {0.0, 0.0};  // (p, q), sum q
   if (q.in) {
     const double beta = S->beta, m = S->zshift;
@@ -566,7 +569,9 @@ __global__ void __launch_bounds__(256) k_cg_pspmv(CgArgs A, const Z* zv) {
       az -= cy * zn;
       az -= cz * zp;
       const bool act = A.active[n] != 0;
-      const double pv = act ? (zc - m) + beta * A.p[n] : 0.0;
+      const double po = A.p[n];
+      A.x[n] += alpha * po;
+      const double pv = act ? (zc - m) + beta * po : 0.0;
       const double qv = act ? (az - m * en) + beta * A.q[n] : 0.0;
       A.p[n] = pv;
       A.q[n] = qv;
@@ -593,7 +598,8 @@ __global__ void __launch_bounds__(kFinThreads) k_cg_spmv_fin(CgArgs A, uint32_t 
   }
 }
 
-// x += alpha p ; r -= alpha q ; rmax = max |r|/V
+// r -= alpha q ; rmax = max |r|/V (x += alpha p follows in the next
+// k_cg_pspmv, or in k_cg_xflush after the last step)
 __global__ void __launch_bounds__(256) k_cg_update(CgArgs A) {
   Scalars* S = A.S;
   if (S->done) return;
@@ -606,7 +612,6 @@ __global__ void __launch_bounds__(256) k_cg_update(CgArgs A) {
     int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c0);
 #pragma unroll 4
     for (int c = q.c0; c < q.c1; ++c, n += IJ) {
-      A.x[n] += alpha * A.p[n];
       const double rv = A.r[n] - alpha * A.q[n];
       A.r[n] = rv;
       if (A.r32) A.r32[n] = (float)rv;
@@ -616,6 +621,20 @@ __global__ void __launch_bounds__(256) k_cg_update(CgArgs A) {
     }
   }
   block_partials<kTileThreads, 2, 1u>(A.site, v);
+}
+
+// x += alpha p of the last CG step (its r update is done; the k_cg_pspmv
+// that would apply it is never launched, or returns once the loop is done).
+// alpha = 0 and p = 0 after a residual that already meets the tolerance.
+__global__ void __launch_bounds__(256) k_cg_xflush(CgArgs A) {
+  const Scalars* S = A.S;
+  const Col3 q = col_of(A.I, A.J, A.K);
+  if (!q.in) return;
+  const double alpha = S->alpha;
+  const int64_t IJ = (int64_t)A.I * A.J;
+  int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c0);
+#pragma unroll 4
+  for (int c = q.c0; c < q.c1; ++c, n += IJ) A.x[n] += alpha * A.p[n];
 }
 
 __global__ void __launch_bounds__(kFinThreads) k_cg_update_fin(CgArgs A, uint32_t nb) {
@@ -955,7 +974,7 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st, bool che
       // z (stencil), p, q, three couplings, e, mask -> p, q
       {
         A.site = red_site(c, kSiteSpmv);
-        timer_begin(c, "cg_pspmv", NB * (zb + 8 + 8 + 24 + eb + 1 + 8 + 8), &slot);
+        timer_begin(c, "cg_pspmv", NB * (zb + 8 + 8 + 8 + 24 + eb + 1 + 8 + 8 + 8), &slot);
         if (A.z32)
           hipLaunchKernelGGL(k_cg_pspmv<float>, cgrd, blk, 0, c->stream, A, (const float*)A.z32);
         else
@@ -963,7 +982,7 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st, bool che
         timer_end(c, slot);
       }
       finalize(k_cg_spmv_fin, kSiteSpmv, ncol);
-      launch_col(k_cg_update, kSiteUpdate, "cg_update", NB * (8 * 4 + 8 * 2 + (A.r32 ? 4 : 0)));
+      launch_col(k_cg_update, kSiteUpdate, "cg_update", NB * (8 * 2 + 8 + (A.r32 ? 4 : 0)));
       finalize(k_cg_update_fin, kSiteUpdate, ncol);
       hipEventRecord(evs[cur], c->stream);
       ++it;
@@ -975,6 +994,8 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st, bool che
       prev_pending = true;
       cur ^= 1;
     }
+    // x += alpha p of the last step
+    launch_col(k_cg_xflush, kSiteUpdate, "cg_xflush", NB * 24);
     e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) { hipEventDestroy(evs[0]); hipEventDestroy(evs[1]); return hip_check(c, e, "solve"); }
     e = hipMemcpy(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost);
