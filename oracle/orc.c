@@ -498,8 +498,8 @@ static void mg_free(mg_hier* H) {
   }
 }
 
-/* one colour of Gauss-Seidel: x_i = (b_i + sum c x_nb)/d_i */
-static void gs_color(const mg_level* A, int color) {
+/* one colour of Gauss-Seidel, over-relaxed: x_i += omega ((b_i + sum c x_nb)/d_i - x_i) */
+static void gs_color(const mg_level* A, int color, double omega) {
   const int nx = A->nx, ny = A->ny, nz = A->nz;
   const int64_t sy = nx, sz = (int64_t)nx * ny;
   const double *cx = A->cx, *cy = A->cy, *cz = A->cz, *d = A->d, *b = A->b;
@@ -517,7 +517,7 @@ static void gs_color(const mg_level* A, int color) {
         if (i < nx - 1) s += cx[n] * x[n + 1];
         if (j < ny - 1) s += cy[n] * x[n + sy];
         if (k < nz - 1) s += cz[n] * x[n + sz];
-        x[n] = s / d[n];
+        x[n] = x[n] + omega * (s / d[n] - x[n]);
       }
 }
 
@@ -536,12 +536,12 @@ static void vcycle(mg_hier* H, int l, const orc_solver_opts* o) {
   memset(A->x, 0, sizeof(double) * A->n);
   if (l == H->L - 1) {
     if (H->singular) project_mean(A, A->b);
-    for (int s = 0; s < o->coarse_sweeps; ++s) { gs_color(A, 0); gs_color(A, 1); }
-    for (int s = 0; s < o->coarse_sweeps; ++s) { gs_color(A, 1); gs_color(A, 0); }
+    for (int s = 0; s < o->coarse_sweeps; ++s) { gs_color(A, 0, o->omega); gs_color(A, 1, o->omega); }
+    for (int s = 0; s < o->coarse_sweeps; ++s) { gs_color(A, 1, o->omega); gs_color(A, 0, o->omega); }
     if (H->singular) project_mean(A, A->x);
     return;
   }
-  for (int s = 0; s < o->pre; ++s) { gs_color(A, 0); gs_color(A, 1); }
+  for (int s = 0; s < o->pre; ++s) { gs_color(A, 0, o->omega); gs_color(A, 1, o->omega); }
   sym_apply_level(A->nx, A->ny, A->nz, A->cx, A->cy, A->cz, A->d, A->x, A->r);
   mg_level* C = &H->lv[l + 1];
   const int64_t fsy = A->nx, fsz = (int64_t)A->nx * A->ny;
@@ -568,7 +568,7 @@ static void vcycle(mg_hier* H, int l, const orc_solver_opts* o) {
         const int64_t m = i + fsy * j + fsz * k;
         if (A->d[m] > 0) A->x[m] += C->x[(i / 2) + (int64_t)C->nx * ((j / 2) + (int64_t)C->ny * (k / 2))];
       }
-  for (int s = 0; s < o->post; ++s) { gs_color(A, 1); gs_color(A, 0); }
+  for (int s = 0; s < o->post; ++s) { gs_color(A, 1, o->omega); gs_color(A, 0, o->omega); }
 }
 
 static double dot_active(int64_t n, const double* d, const double* a, const double* b) {

@@ -44,6 +44,7 @@ struct FusedArgs {
   T* bc;         // pre: coarse right-hand side
   int cnx, cny;
   int kchunk;    // even
+  T omega;       // over-relaxation: x += omega (x_gs - x)
   // finest post-smoother only: the CG dot products (r,z), (z,q), (dw,z) of
   // the z it writes, as per-workgroup partials
   int dots;
@@ -213,6 +214,7 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
       acc += F.czw[S(o)] * F.xw[S(o - 1)];
       v = kRcp ? (F.bw[S(o)] + acc) * F.rdw[S(o)] : (F.bw[S(o)] + acc) / F.dw[S(o)];
     }
+    v = F.xw[S(o)] + A.omega * (v - F.xw[S(o)]);
     // keep v in a register on every lane: a select whose operand is only
     // needed on some lanes would otherwise become an exec-masked branch
     __asm__ volatile("" : "+v"(v));

@@ -71,7 +71,7 @@ __device__ inline T offdiag_sum(const LevelView<T>& L, int i, int j, int k, int6
 // holds one cell of each colour).  zero_init: x starts at 0 (the other colour
 // is written as 0).  FINE only distinguishes the finest level in profiles.
 template <typename T, bool FINE>
-__global__ void __launch_bounds__(256) k_gs(LevelView<T> L, int color, int zero_init,
+__global__ void __launch_bounds__(256) k_gs(LevelView<T> L, int color, int zero_init, T omega,
                                             const Scalars* S) {
   if (S->done) return;
   const int nxh = (L.nx + 1) >> 1;
@@ -94,8 +94,11 @@ __global__ void __launch_bounds__(256) k_gs(LevelView<T> L, int color, int zero_
   }
   T s = L.b[n];
   if (!zero_init) s += offdiag_sum(L, i, j, k, n);
-  // same rounding as the fused smoother (FP32: times 1/d, FP64: divided by d)
-  L.x[n] = sizeof(T) == 4 ? s * smoother_rcp(d) : s / d;
+  // same rounding as the fused smoother (FP32: times 1/d, FP64: divided by d),
+  // over-relaxed by omega
+  const T v = sizeof(T) == 4 ? s * smoother_rcp(d) : s / d;
+  const T x0 = zero_init ? T(0) : L.x[n];
+  L.x[n] = x0 + omega * (v - x0);
 }
 
 // Coarse b = sum over children of the fine residual.
@@ -147,6 +150,7 @@ template <typename T>
 struct TailArgs {
   int first, last;  // levels [first, last)
   int pre, post, coarse_sweeps, singular, has_e;
+  T omega;  // over-relaxation of the Gauss-Seidel updates
   const LevelView<T>* lv;   // the levels in global memory
   int off[kMaxLevels];      // element offset of each tail level in LDS
 };
@@ -186,7 +190,7 @@ __device__ inline T tail_offdiag(const TailLevel<T>& L, int i, int j, int k, int
 // x starts at 0 and the other colour is written as 0.  FP32 multiplies by
 // 1/d, FP64 divides by d, as the other smoothers do.
 template <typename T>
-__device__ void tail_gs(const TailLevel<T>& L, int color, int zero_init) {
+__device__ void tail_gs(const TailLevel<T>& L, int color, int zero_init, T omega) {
   const unsigned nxh = (unsigned)(L.nx + 1) / 2, nxy = nxh * (unsigned)L.ny;
   const unsigned half = nxy * (unsigned)L.nz;
   for (unsigned t = threadIdx.x; t < half; t += blockDim.x) {
@@ -207,7 +211,9 @@ __device__ void tail_gs(const TailLevel<T>& L, int color, int zero_init) {
     }
     T s = L.b[n];
     if (!zero_init) s += tail_offdiag(L, i, q.j, q.k, n);
-    L.x[n] = sizeof(T) == 4 ? s * L.rd[n] : s / d;
+    const T v = sizeof(T) == 4 ? s * L.rd[n] : s / d;
+    const T x0 = zero_init ? T(0) : L.x[n];
+    L.x[n] = x0 + omega * (v - x0);
   }
   __syncthreads();
 }
@@ -311,24 +317,24 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A, const Scalars* S) 
   __syncthreads();
   for (int l = A.first; l < L - 1; ++l) {
     const TailLevel<T> F = tail_level(lds, A, l);
-    tail_gs(F, 0, 1);
-    tail_gs(F, 1, 0);
-    for (int s = 1; s < A.pre; ++s) { tail_gs(F, 0, 0); tail_gs(F, 1, 0); }
+    tail_gs(F, 0, 1, A.omega);
+    tail_gs(F, 1, 0, A.omega);
+    for (int s = 1; s < A.pre; ++s) { tail_gs(F, 0, 0, A.omega); tail_gs(F, 1, 0, A.omega); }
     tail_restrict(F, tail_level(lds, A, l + 1));
   }
   {
     const TailLevel<T> C = tail_level(lds, A, L - 1);
     if (A.singular) tail_project_mean(C, C.b);
-    tail_gs(C, 0, 1);
-    tail_gs(C, 1, 0);
-    for (int s = 1; s < A.coarse_sweeps; ++s) { tail_gs(C, 0, 0); tail_gs(C, 1, 0); }
-    for (int s = 0; s < A.coarse_sweeps; ++s) { tail_gs(C, 1, 0); tail_gs(C, 0, 0); }
+    tail_gs(C, 0, 1, A.omega);
+    tail_gs(C, 1, 0, A.omega);
+    for (int s = 1; s < A.coarse_sweeps; ++s) { tail_gs(C, 0, 0, A.omega); tail_gs(C, 1, 0, A.omega); }
+    for (int s = 0; s < A.coarse_sweeps; ++s) { tail_gs(C, 1, 0, A.omega); tail_gs(C, 0, 0, A.omega); }
     if (A.singular) tail_project_mean(C, C.x);
   }
   for (int l = L - 2; l >= A.first; --l) {
     const TailLevel<T> F = tail_level(lds, A, l);
     tail_prolong(F, tail_level(lds, A, l + 1));
-    for (int s = 0; s < A.post; ++s) { tail_gs(F, 1, 0); tail_gs(F, 0, 0); }
+    for (int s = 0; s < A.post; ++s) { tail_gs(F, 1, 0, A.omega); tail_gs(F, 0, 0, A.omega); }
   }
   {
     const LevelView<T> G = A.lv[A.first];
@@ -373,7 +379,7 @@ struct CgArgs {
   const double* b;
   double *x, *r, *q;
   double* p;  // search direction
-  const double* z64;
+  double* z64;  // written by the Jacobi preconditioner only
   const float* z32;
   float* r32;
   Scalars* S;
@@ -404,6 +410,25 @@ __device__ inline double apply_row(const CgArgs& A, const double* v, int a, int 
   if (b < A.J - 1) s -= A.cy[n] * v[n + A.I];
   if (c < A.K - 1) s -= A.cz[n] * v[n + IJ];
   return s;
+}
+
+// Jacobi preconditioner (SP_PRECOND_JACOBI): z = r / diagonal on active cells
+__global__ void __launch_bounds__(256) k_cg_jacobi(CgArgs A) {
+  if (A.S->done) return;
+  const Col3 q = col_of(A.I, A.J, A.K);
+  if (!q.in) return;
+  const int64_t IJ = (int64_t)A.I * A.J;
+  int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c0);
+  for (int c = q.c0; c < q.c1; ++c, n += IJ) {
+    double d = q.a > 0 ? A.cx[n - 1] : 0.0;
+    d = d + (q.b > 0 ? A.cy[n - A.I] : 0.0);
+    d = d + (c > 0 ? A.cz[n - IJ] : 0.0);
+    d = d + A.cx[n];
+    d = d + A.cy[n];
+    d = d + A.cz[n];
+    if (A.e) d = d + A.e[n];
+    A.z64[n] = (A.active[n] && d > 0.0) ? A.r[n] / d : 0.0;
+  }
 }
 
 // r = b - mean(b) - A x  (x = 0 when from_zero); resets p and q
@@ -674,9 +699,11 @@ void launch_gs(sp_ctx* c, const Level<T>& L, int color, int zero_init, bool fine
               level_bytes(L, words), &slot);
   if (fine)
     hipLaunchKernelGGL((k_gs<T, true>), grd, blk, 0, c->stream, view_of(L), color, zero_init,
+                       (T)c->prm.mg_omega,
                        (const Scalars*)c->S);
   else
     hipLaunchKernelGGL((k_gs<T, false>), grd, blk, 0, c->stream, view_of(L), color, zero_init,
+                       (T)c->prm.mg_omega,
                        (const Scalars*)c->S);
   timer_end(c, slot);
 }
@@ -691,6 +718,7 @@ void launch_fused(sp_ctx* c, Level<T>* lv, int l, int mode) {
   A.cx = L.cx; A.cy = L.cy; A.cz = L.cz; A.e = L.e;
   A.b = L.b;
   A.cnx = C.nx; A.cny = C.ny;
+  A.omega = (T)c->prm.mg_omega;
   if (mode == 0) { A.xout = L.xs; A.bc = C.b; }
   else { A.xin = L.xs; A.xc = C.x; A.xout = L.x; }
   if (mode == 1 && l == 0 && c->fuse_dots) {
@@ -788,6 +816,7 @@ void vcycle(sp_ctx* c, Level<T>* lv) {
     A.pre = pre;
     A.post = post;
     A.coarse_sweeps = std::max(1, c->prm.mg_coarse_sweeps);
+    A.omega = (T)c->prm.mg_omega;
     A.singular = c->S_host->singular;
     A.has_e = lv[Lt].e ? 1 : 0;
     A.lv = reinterpret_cast<const LevelView<T>*>(c->mg_desc);
@@ -926,13 +955,7 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st, bool che
       if (mixed) vcycle<float>(c, c->lf);
       else vcycle<double>(c, c->ld);
     } else {
-      // Jacobi: z = r / d through two zero-init colour sweeps of level 0
-      Level<double> L0{};
-      L0.nx = D.I; L0.ny = D.J; L0.nz = D.K; L0.n = D.N;
-      L0.cx = c->cx; L0.cy = c->cy; L0.cz = c->cz; L0.e = c->has_e ? c->ee : nullptr;
-      L0.x = c->z64; L0.b = c->r;
-      launch_gs(c, L0, 0, 1, true);
-      launch_gs(c, L0, 1, 1, true);
+      launch_col(k_cg_jacobi, kSiteDots, "cg_jacobi", NB * (8 * 4 + eb + 1 + 8));
     }
   };
 

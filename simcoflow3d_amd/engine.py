@@ -47,7 +47,7 @@ class SpParams(ctypes.Structure):
         ("rho_l", ctypes.c_double), ("rho_g", ctypes.c_double), ("rho_ref", ctypes.c_double),
         ("epsi", ctypes.c_double),
         ("pin_i", ctypes.c_int32), ("pin_j", ctypes.c_int32), ("pin_k", ctypes.c_int32),
-        ("flags", ctypes.c_int32),
+        ("flags", ctypes.c_int32), ("mg_omega", ctypes.c_double),
     ]
 
 

@@ -37,7 +37,7 @@ def test_status_strings_and_defaults():
         assert L.sp_strerror(s)
     p = engine.SpParams()
     L.sp_params_default(ctypes.byref(p))
-    assert p.mg_pre == 2 and p.mg_post == 2 and p.div_tol == 1e-10
+    assert p.mg_pre == 2 and p.mg_post == 2 and p.div_tol == 1e-10 and p.mg_omega == 1.25
     assert p.rho_l == 998.3 and p.rho_g == 1.205 and p.epsi == 1e-3
 
 
@@ -52,6 +52,9 @@ def test_invalid_arguments_are_rejected_without_a_device():
     p.imax, p.jmax, p.kmax = 1024, 1024, 512  # 2^29 cells: beyond 32-bit byte offsets of FP64 levels
     assert L.sp_create(ctypes.byref(h), ctypes.byref(p), 0) == engine.SP_ERR_INVALID_ARG
     p.imax, p.jmax, p.kmax = 8, 8, 8
+    p.mg_omega = 2.0  # over-relaxation must stay in (0, 2)
+    assert L.sp_create(ctypes.byref(h), ctypes.byref(p), 0) == engine.SP_ERR_INVALID_ARG
+    p.mg_omega = 1.25
     p.precision = 7
     assert L.sp_create(ctypes.byref(h), ctypes.byref(p), 0) == engine.SP_ERR_INVALID_ARG
     assert L.sp_project(None, None, None, None) == engine.SP_ERR_INVALID_ARG

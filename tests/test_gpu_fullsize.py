@@ -117,4 +117,6 @@ def test_fullsize_adjoint_dot_product(gpu_available, cfg, size, dist):
     assert ast.converged == 1, ast.as_dict()
     lhs = sum((out[n] * wq).sum() for n, wq in zip(("u", "v", "w"), w)).item()
     rhs = sum((vq * res[n]).sum() for vq, n in zip(v, ("gu_star", "gv_star", "gw_star"))).item()
-    assert abs(lhs - rhs) <= 1e-8 * abs(lhs), (lhs, rhs)
+    # the adjoint solve stops at 1e-10 of its source; on the stiff rho 1000:1
+    # operator that leaves a relative error up to ~1e-8 in J^T w
+    assert abs(lhs - rhs) <= 1e-7 * abs(lhs), (lhs, rhs)

@@ -100,6 +100,19 @@ def test_ragged_and_tiny_grids_match_oracle(gpu_available, cfg, size, prec):
         assert np.max(np.abs(d)) <= 1e-8 * umax, n
 
 
+@pytest.mark.parametrize("cfg,size", [(3, (8, 8, 8)), (1, (8, 12, 10))])
+def test_jacobi_preconditioner_matches_oracle(gpu_available, cfg, size):
+    """SP_PRECOND_JACOBI (z = r / diagonal) reaches the same solution."""
+    from simcoflow3d_amd.engine import SP_PRECOND_JACOBI
+    c = H.case(cfg, size=size)
+    ref = orc.project(c, pin=(1, 1, 1))
+    P = H.projector(c, precision=0, precond=SP_PRECOND_JACOBI, max_iter=2000)
+    out, st = P.project(H.fields(c), c.bc_flag, c.dt)
+    P.close()
+    assert st.converged == 1 and st.final_div <= 1e-10, st.as_dict()
+    assert H.rel_inf(out["p"], ref.Pp) <= 1e-8
+
+
 @pytest.mark.parametrize("prec", (0, 1))
 def test_all_solid_domain(gpu_available, prec):
     """No fluid cell at all: nothing to solve; p = 0 and the corrected
