@@ -208,8 +208,13 @@ sp_status adjoint(sp_ctx* c, const sp_adj_inputs& in, sp_adj_outputs& out, sp_st
   A.site = red_site(c, kSiteAdj);
   timer_begin(c, "adj_source", NB * (3 * 8 + 6 * 4 + 3 * 8 + 8 + 1 + 3 * 8 + 3 * 8 + 8), &slot);
   hipLaunchKernelGGL(k_adj_source, fg, fb, 0, c->stream, A);
-  hipLaunchKernelGGL(k_adj_source_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A,
-                     flat_blocks(D.I, D.J, D.K));
+  // one partial per 256 cells: fold them on many workgroups first
+  const uint32_t nb = flat_blocks(D.I, D.J, D.K), chunk = 512, nf = (nb + chunk - 1) / chunk;
+  const RedSite fold = red_site(c, kSiteFold);
+  hipLaunchKernelGGL((k_fold_partials<2, 2u>), dim3(nf), dim3(256), 0, c->stream, A.site, fold, nb,
+                     chunk);
+  A.site = fold;
+  hipLaunchKernelGGL(k_adj_source_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A, nf);
   timer_end(c, slot);
   // Solve A lam = pbar (mean-projected when singular) with the forward's
   // operator and multigrid hierarchy until max |r|/V <= 1e-10 max |pbar|/V.
@@ -227,8 +232,10 @@ sp_status adjoint(sp_ctx* c, const sp_adj_inputs& in, sp_adj_outputs& out, sp_st
   A.site = red_site(c, kSiteAdj2);
   timer_begin(c, "adj_lam_mean", NB * 9, &slot);
   hipLaunchKernelGGL(k_adj_lam_mean, fg, fb, 0, c->stream, A);
-  hipLaunchKernelGGL(k_adj_lam_mean_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A,
-                     flat_blocks(D.I, D.J, D.K));
+  hipLaunchKernelGGL((k_fold_partials<1, 0u>), dim3(nf), dim3(256), 0, c->stream, A.site, fold, nb,
+                     chunk);
+  A.site = fold;
+  hipLaunchKernelGGL(k_adj_lam_mean_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A, nf);
   timer_end(c, slot);
   timer_begin(c, "adj_grad", NB * (8 * 3 + 1 + 3 * 8 + 3 * 8 * 2 + 3 * 8 * 2), &slot);
   hipLaunchKernelGGL(k_adj_grad, fg, fb, 0, c->stream, A);

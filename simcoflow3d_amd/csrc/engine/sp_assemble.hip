@@ -494,8 +494,16 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_out) {
               (double)D.N * (6 * 8 + 3 * 8 + 4 + 5 * 8 + 4 * 8 + 8 + 1 + 4 + (mixed ? 16 : 0)), &slot);
   hipLaunchKernelGGL(k_operator, col_grid_k<kOpK>(D.I, D.J, D.K), dim3(64, 4, 1), 0, c->stream, O);
   timer_end(c, slot);
-  hipLaunchKernelGGL(k_operator_fin, dim3(1), dim3(kFinThreads), 0, c->stream, O,
-                     col_blocks_k<kOpK>(D.I, D.J, D.K));
+  {
+    // one partial per 256 cells: fold them on many workgroups first
+    const uint32_t nb = col_blocks_k<kOpK>(D.I, D.J, D.K), chunk = 512;
+    const uint32_t nf = (nb + chunk - 1) / chunk;
+    const RedSite fold = red_site(c, kSiteFold);
+    hipLaunchKernelGGL((k_fold_partials<6, 0x14u>), dim3(nf), dim3(256), 0, c->stream, O.site, fold,
+                       nb, chunk);
+    O.site = fold;
+    hipLaunchKernelGGL(k_operator_fin, dim3(1), dim3(kFinThreads), 0, c->stream, O, nf);
+  }
   return SP_OK;
 }
 

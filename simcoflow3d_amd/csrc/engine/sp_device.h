@@ -218,6 +218,32 @@ __device__ inline void finalize_partials(const RedSite& R, uint32_t nb, double (
   for (int q = 0; q < NV; ++q) out[q] = tot[q];
 }
 
+// First stage of a reduction over many partials: workgroup g folds the
+// partials [g*chunk, min((g+1)*chunk, nb)) of every value into slot g of
+// `out` (fixed order, so the result stays bitwise reproducible).  The
+// one-workgroup finalize then reads gridDim.x partials instead of nb.
+template <int NV, unsigned MAXMASK>
+__global__ void __launch_bounds__(256) k_fold_partials(RedSite in, RedSite out, uint32_t nb,
+                                                       uint32_t chunk) {
+  const uint32_t g = blockIdx.x;
+  const uint32_t lo = g * chunk, hi = lo + chunk < nb ? lo + chunk : nb;
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+  for (uint32_t k = lo + threadIdx.x; k < hi; k += 256) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const double x = in.part[(size_t)q * in.max_blocks + k];
+      acc[q] = ((MAXMASK >> q) & 1u) ? (x > acc[q] ? x : acc[q]) : acc[q] + x;
+    }
+  }
+  block_reduce_multi<256, NV, MAXMASK>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out.part[(size_t)q * out.max_blocks + g] = acc[q];
+  }
+}
+
 inline RedSite red_site(const sp_ctx* c, int id) {
   RedSite r;
   r.part = c->red_part + (size_t)kSiteValues * c->red_max_blocks * id;

@@ -43,7 +43,7 @@ struct Scalars {
   uint32_t pad2[3];
 };
 
-// Reduction sites (each owns partials and counters, see grid_reduce)
+// Reduction sites (each owns its partials)
 enum RedSiteId : int {
   kSiteOperator = 0,
   kSiteResidual,
@@ -54,6 +54,7 @@ enum RedSiteId : int {
   kSiteDiv,
   kSiteAdj,
   kSiteAdj2,
+  kSiteFold,  // first-stage folds of sites with one partial per 256 cells
   kNumSites
 };
 constexpr int kSiteValues = 8;  // values per site at most
