@@ -164,12 +164,20 @@ __device__ inline TailLevel<T> tail_level(T* lds, const TailArgs<T>& A, int l) {
                       base + 4 * n, base + 5 * n, base + 6 * n};
 }
 
-// (i, j, k) of cell n (32-bit index arithmetic)
+// (i, j, k) of cell n.  Tail levels hold far fewer than 2^24 cells, so the
+// quotients come from a float reciprocal, corrected by one step (the float
+// estimate is within 1 of the integer quotient).
 struct Ijk {
   int i, j, k;
 };
+__device__ inline unsigned small_div(unsigned n, unsigned d) {
+  unsigned q = (unsigned)((float)n * __builtin_amdgcn_rcpf((float)d));
+  if (q * d > n) --q;
+  else if ((q + 1) * d <= n) ++q;
+  return q;
+}
 __device__ inline Ijk ijk_of(unsigned n, unsigned nx, unsigned nxy) {
-  const unsigned k = n / nxy, r = n - k * nxy, j = r / nx;
+  const unsigned k = small_div(n, nxy), r = n - k * nxy, j = small_div(r, nx);
   return Ijk{(int)(r - j * nx), (int)j, (int)k};
 }
 
