@@ -194,14 +194,35 @@ __device__ inline T tail_offdiag(const TailLevel<T>& L, int i, int j, int k, int
   return s;
 }
 
+// The threads that work on a tail level: the whole workgroup, or wave 0
+// alone on the smallest levels, which then need no workgroup barrier (one
+// wave's LDS operations complete in issue order; the fences keep the
+// compiler from moving LDS accesses across the step boundary).
+template <bool WAVE>
+struct TailTeam {
+  __device__ static unsigned id() { return WAVE ? (threadIdx.x & 63u) : threadIdx.x; }
+  __device__ static unsigned size() { return WAVE ? 64u : blockDim.x; }
+  __device__ static void sync() {
+    if (WAVE) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      __syncthreads();
+    }
+  }
+};
+constexpr int kTailWaveCells = 64;  // levels this small run on one wave
+
 // One colour of Gauss-Seidel (one thread per cell pair along i); zero_init:
 // x starts at 0 and the other colour is written as 0.  FP32 multiplies by
 // 1/d, FP64 divides by d, as the other smoothers do.
-template <typename T>
+template <bool WAVE, typename T>
 __device__ void tail_gs(const TailLevel<T>& L, int color, int zero_init, T omega) {
+  using G = TailTeam<WAVE>;
   const unsigned nxh = (unsigned)(L.nx + 1) / 2, nxy = nxh * (unsigned)L.ny;
   const unsigned half = nxy * (unsigned)L.nz;
-  for (unsigned t = threadIdx.x; t < half; t += blockDim.x) {
+  for (unsigned t = G::id(); t < half; t += G::size()) {
     const Ijk q = ijk_of(t, nxh, nxy);
     const int par = (q.j + q.k + color) & 1;
     const int i = 2 * q.i + par;
@@ -223,37 +244,44 @@ __device__ void tail_gs(const TailLevel<T>& L, int color, int zero_init, T omega
     const T x0 = zero_init ? T(0) : L.x[n];
     L.x[n] = x0 + omega * (v - x0);
   }
-  __syncthreads();
+  G::sync();
 }
 
 // v -= mean of v over the cells with d > 0
-template <typename T>
+template <bool WAVE, typename T>
 __device__ void tail_project_mean(const TailLevel<T>& L, T* v) {
+  using G = TailTeam<WAVE>;
   __shared__ double red[2][16];
   double s = 0.0, c = 0.0;
   const int nn = L.nx * L.ny * L.nz;
-  for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+  for (int n = (int)G::id(); n < nn; n += (int)G::size()) {
     if (L.d[n] > T(0)) { s += (double)v[n]; c += 1.0; }
   }
   s = wave_sum(s);
   c = wave_sum(c);
-  if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = s; red[1][threadIdx.x >> 6] = c; }
-  __syncthreads();
   double ts = 0.0, tc = 0.0;
-  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { ts += red[0][w]; tc += red[1][w]; }
+  if (WAVE) {
+    ts = __shfl(s, 0, 64);
+    tc = __shfl(c, 0, 64);
+  } else {
+    if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = s; red[1][threadIdx.x >> 6] = c; }
+    __syncthreads();
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { ts += red[0][w]; tc += red[1][w]; }
+  }
   const T mean = tc > 0 ? (T)(ts / tc) : T(0);
-  __syncthreads();
-  for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+  G::sync();
+  for (int n = (int)G::id(); n < nn; n += (int)G::size()) {
     if (L.d[n] > T(0)) v[n] -= mean;
   }
-  __syncthreads();
+  G::sync();
 }
 
 // coarse b = sum over the 2x2x2 children of the fine residual
-template <typename T>
+template <bool WAVE, typename T>
 __device__ void tail_restrict(const TailLevel<T>& F, const TailLevel<T>& C) {
+  using G = TailTeam<WAVE>;
   const unsigned nxy = (unsigned)C.nx * C.ny, nc = nxy * (unsigned)C.nz;
-  for (unsigned t = threadIdx.x; t < nc; t += blockDim.x) {
+  for (unsigned t = G::id(); t < nc; t += G::size()) {
     const Ijk q = ijk_of(t, (unsigned)C.nx, nxy);
     T s = T(0);
     for (int cc = 0; cc < 2; ++cc)
@@ -267,18 +295,46 @@ __device__ void tail_restrict(const TailLevel<T>& F, const TailLevel<T>& C) {
         }
     C.b[t] = s;
   }
-  __syncthreads();
+  G::sync();
 }
 
-template <typename T>
+template <bool WAVE, typename T>
 __device__ void tail_prolong(const TailLevel<T>& F, const TailLevel<T>& C) {
+  using G = TailTeam<WAVE>;
   const unsigned nxy = (unsigned)F.nx * F.ny, nf = nxy * (unsigned)F.nz;
-  for (unsigned m = threadIdx.x; m < nf; m += blockDim.x) {
+  for (unsigned m = G::id(); m < nf; m += G::size()) {
     const Ijk q = ijk_of(m, (unsigned)F.nx, nxy);
     if (F.d[m] > T(0))
       F.x[m] += C.x[(q.i >> 1) + C.nx * ((q.j >> 1) + C.ny * (q.k >> 1))];
   }
-  __syncthreads();
+  G::sync();
+}
+
+template <bool WAVE, typename T>
+__device__ void tail_pre(T* lds, const TailArgs<T>& A, int l) {
+  const TailLevel<T> F = tail_level(lds, A, l);
+  tail_gs<WAVE>(F, 0, 1, A.omega);
+  tail_gs<WAVE>(F, 1, 0, A.omega);
+  for (int s = 1; s < A.pre; ++s) { tail_gs<WAVE>(F, 0, 0, A.omega); tail_gs<WAVE>(F, 1, 0, A.omega); }
+  tail_restrict<WAVE>(F, tail_level(lds, A, l + 1));
+}
+
+template <bool WAVE, typename T>
+__device__ void tail_coarsest(T* lds, const TailArgs<T>& A) {
+  const TailLevel<T> C = tail_level(lds, A, A.last - 1);
+  if (A.singular) tail_project_mean<WAVE>(C, C.b);
+  tail_gs<WAVE>(C, 0, 1, A.omega);
+  tail_gs<WAVE>(C, 1, 0, A.omega);
+  for (int s = 1; s < A.coarse_sweeps; ++s) { tail_gs<WAVE>(C, 0, 0, A.omega); tail_gs<WAVE>(C, 1, 0, A.omega); }
+  for (int s = 0; s < A.coarse_sweeps; ++s) { tail_gs<WAVE>(C, 1, 0, A.omega); tail_gs<WAVE>(C, 0, 0, A.omega); }
+  if (A.singular) tail_project_mean<WAVE>(C, C.x);
+}
+
+template <bool WAVE, typename T>
+__device__ void tail_post(T* lds, const TailArgs<T>& A, int l) {
+  const TailLevel<T> F = tail_level(lds, A, l);
+  tail_prolong<WAVE>(F, tail_level(lds, A, l + 1));
+  for (int s = 0; s < A.post; ++s) { tail_gs<WAVE>(F, 1, 0, A.omega); tail_gs<WAVE>(F, 0, 0, A.omega); }
 }
 
 // The coarse levels of the V-cycle in one workgroup, resident in LDS: the
@@ -323,27 +379,24 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A, const Scalars* S) 
     }
   }
   __syncthreads();
-  for (int l = A.first; l < L - 1; ++l) {
-    const TailLevel<T> F = tail_level(lds, A, l);
-    tail_gs(F, 0, 1, A.omega);
-    tail_gs(F, 1, 0, A.omega);
-    for (int s = 1; s < A.pre; ++s) { tail_gs(F, 0, 0, A.omega); tail_gs(F, 1, 0, A.omega); }
-    tail_restrict(F, tail_level(lds, A, l + 1));
+  // levels [first, lw) on the whole workgroup, [lw, L) on wave 0 alone
+  int lw = L;
+  for (int l = A.first; l < L; ++l) {
+    const LevelView<T>& G = A.lv[l];
+    if (G.nx * G.ny * G.nz <= kTailWaveCells) { lw = l; break; }
   }
-  {
-    const TailLevel<T> C = tail_level(lds, A, L - 1);
-    if (A.singular) tail_project_mean(C, C.b);
-    tail_gs(C, 0, 1, A.omega);
-    tail_gs(C, 1, 0, A.omega);
-    for (int s = 1; s < A.coarse_sweeps; ++s) { tail_gs(C, 0, 0, A.omega); tail_gs(C, 1, 0, A.omega); }
-    for (int s = 0; s < A.coarse_sweeps; ++s) { tail_gs(C, 1, 0, A.omega); tail_gs(C, 0, 0, A.omega); }
-    if (A.singular) tail_project_mean(C, C.x);
+  for (int l = A.first; l < lw && l < L - 1; ++l) tail_pre<false>(lds, A, l);
+  if (lw >= L) {
+    tail_coarsest<false>(lds, A);
+  } else {
+    if (threadIdx.x < 64) {
+      for (int l = lw; l < L - 1; ++l) tail_pre<true>(lds, A, l);
+      tail_coarsest<true>(lds, A);
+      for (int l = L - 2; l >= lw; --l) tail_post<true>(lds, A, l);
+    }
+    __syncthreads();
   }
-  for (int l = L - 2; l >= A.first; --l) {
-    const TailLevel<T> F = tail_level(lds, A, l);
-    tail_prolong(F, tail_level(lds, A, l + 1));
-    for (int s = 0; s < A.post; ++s) { tail_gs(F, 1, 0, A.omega); tail_gs(F, 0, 0, A.omega); }
-  }
+  for (int l = (lw >= L ? L - 2 : lw - 1); l >= A.first; --l) tail_post<false>(lds, A, l);
   {
     const LevelView<T> G = A.lv[A.first];
     const TailLevel<T> V = tail_level(lds, A, A.first);
