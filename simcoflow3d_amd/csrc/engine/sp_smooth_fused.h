@@ -46,9 +46,10 @@ struct FusedArgs {
   int kchunk;    // even
   T omega;       // over-relaxation: x += omega (x_gs - x)
   // finest post-smoother only: the CG dot products (r,z), (z,q), (dw,z) of
-  // the z it writes, as per-workgroup partials
+  // the z it writes, as per-workgroup partials (r: the FP32 right-hand side
+  // the leg already holds)
   int dots;
-  const double *dr, *dq;
+  const double* dq;
   const float* ddw;
   RedSite dsite;
 };
@@ -95,7 +96,7 @@ template <typename T>
 struct PlaneLoad {
   T b, cx, cy, cz, cxm, cym, e, xin, xc;
   bool vt;
-  double dr, dq;  // dot-product inputs of the step's output plane t - H
+  double dq;  // dot-product inputs of the step's output plane t - H
   float ddw;
 };
 
@@ -135,7 +136,6 @@ __device__ inline void load_plane(PlaneLoad<T>& L, const FusedArgs<T>& A, const 
   if (DOTS && A.dots && C.tile && C.in_ij) {
     const int q = t - H, qc = q < 0 ? 0 : (q >= A.nz ? A.nz - 1 : q);
     const uint32_t e = C.col / (uint32_t)sizeof(T) + (C.plane / (uint32_t)sizeof(T)) * (uint32_t)qc;
-    L.dr = at_off(A.dr, e * 8u);
     L.dq = at_off(A.dq, e * 8u);
     L.ddw = at_off(A.ddw, e * 4u);
   }
@@ -244,7 +244,10 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
     put_off(A.xout, C.col + C.plane * (uint32_t)q, F.xw[S(H)]);
     if (DOTS && A.dots) {
       const double z = (double)F.xw[S(H)];
-      F.dacc[0] += L.dr * z;
+      // (r, z) with the FP32 r the leg smooths with (the b window): an
+      // rz rounded at 1e-7 only perturbs the step length, and the x and r
+      // updates stay consistent (both use the same alpha)
+      F.dacc[0] += (double)F.bw[S(H)] * z;
       F.dacc[1] += z * L.dq;
       F.dacc[2] += (double)L.ddw * z;
     }
