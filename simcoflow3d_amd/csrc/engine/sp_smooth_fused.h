@@ -46,11 +46,10 @@ struct FusedArgs {
   int kchunk;    // even
   T omega;       // over-relaxation: x += omega (x_gs - x)
   // finest post-smoother only: the CG dot products (r,z), (z,q), (dw,z) of
-  // the z it writes, as per-workgroup partials (r: the FP32 right-hand side
-  // the leg already holds)
+  // the z it writes, as per-workgroup partials (r and the diagonal weight:
+  // the FP32 right-hand side and diagonal the leg already holds)
   int dots;
   const double* dq;
-  const float* ddw;
   RedSite dsite;
 };
 
@@ -96,8 +95,7 @@ template <typename T>
 struct PlaneLoad {
   T b, cx, cy, cz, cxm, cym, e, xin, xc;
   bool vt;
-  double dq;  // dot-product inputs of the step's output plane t - H
-  float ddw;
+  double dq;  // dot-product input of the step's output plane t - H
 };
 
 // Per-thread constants of the march.
@@ -137,7 +135,6 @@ __device__ inline void load_plane(PlaneLoad<T>& L, const FusedArgs<T>& A, const 
     const int q = t - H, qc = q < 0 ? 0 : (q >= A.nz ? A.nz - 1 : q);
     const uint32_t e = C.col / (uint32_t)sizeof(T) + (C.plane / (uint32_t)sizeof(T)) * (uint32_t)qc;
     L.dq = at_off(A.dq, e * 8u);
-    L.ddw = at_off(A.ddw, e * 4u);
   }
 }
 
@@ -249,7 +246,7 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
       // updates stay consistent (both use the same alpha)
       F.dacc[0] += (double)F.bw[S(H)] * z;
       F.dacc[1] += z * L.dq;
-      F.dacc[2] += (double)L.ddw * z;
+      F.dacc[2] += (double)F.dw[S(H)] * z;
     }
   }
   lds_barrier();

@@ -785,7 +785,6 @@ void launch_fused(sp_ctx* c, Level<T>* lv, int l, int mode) {
   if (mode == 1 && l == 0 && c->fuse_dots) {
     A.dots = 1;
     A.dq = c->q;
-    A.ddw = c->dw;
     A.dsite = red_site(c, kSiteDots);
   }
   const int gx = (L.nx + Sh::TX - 1) / Sh::TX, gy = (L.ny + Sh::TY - 1) / Sh::TY;
@@ -810,11 +809,11 @@ void launch_fused(sp_ctx* c, Level<T>* lv, int l, int mode) {
   if (A.dots) c->dots_nb = grd.x * grd.y * grd.z;
   // algorithmic words per cell: pre b + 3 faces -> x (+ coarse b);
   // post x_pre + 3 faces + b -> x (+ coarse x); the dot products add q
-  // (FP64) and the diagonal weight (FP32)
+  // (FP64)
   const double words = mode == 0 ? 5.0 + 0.125 : 6.0 + 0.125;
   timer_begin(c, fine ? (mode == 0 ? "mg_pre_fine" : "mg_post_fine")
                       : (mode == 0 ? "mg_pre_coarse" : "mg_post_coarse"),
-              level_bytes(L, words + (L.e ? 1.0 : 0.0)) + (A.dots ? 12.0 * (double)L.n : 0.0),
+              level_bytes(L, words + (L.e ? 1.0 : 0.0)) + (A.dots ? 8.0 * (double)L.n : 0.0),
               &slot);
   const Scalars* S = c->S;
 #define SP_FUSED_LAUNCH(MODE_, FINE_)                                                         \
