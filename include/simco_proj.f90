@@ -40,7 +40,7 @@ module simco_proj
     real(c_double) :: epsi
     integer(c_int32_t) :: pin_i, pin_j, pin_k
     integer(c_int32_t) :: flags
-    real(c_double) :: mg_omega
+    real(c_double) :: mg_omega, mg_omega_coarse
   end type sp_params
 
   type, bind(C) :: sp_inputs

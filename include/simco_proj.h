@@ -74,8 +74,10 @@ typedef struct sp_params {
   int32_t pin_i, pin_j, pin_k;  /* 1-based gauge cell of a singular system;
                                    0,0,0 -> zero mean over fluid cells */
   int32_t flags;                /* SP_FLAG_* bits, 0 by default */
-  double mg_omega;              /* over-relaxation of every red-black Gauss-Seidel
-                                   update, x += omega (x_gs - x); (0, 2) */
+  double mg_omega;              /* over-relaxation of the red-black Gauss-Seidel
+                                   updates on the finest level,
+                                   x += omega (x_gs - x); (0, 2) */
+  double mg_omega_coarse;       /* the same on every coarser level; (0, 2) */
 } sp_params;
 
 /* sp_params.flags bits */
@@ -145,7 +147,8 @@ typedef struct sp_adj_outputs {
 } sp_adj_outputs;
 
 /* Fill the defaults: FP64, multigrid, automatic levels, 2+2 sweeps, coarse
- * scale 0.5, omega 1.25, 200 iterations, div_tol 1e-10, rho 998.3/1.205,
+ * scale 0.5, omega 1.25 (coarse levels 1.5), 200 iterations, div_tol 1e-10,
+ * rho 998.3/1.205,
  * epsi 1e-3. */
 void sp_params_default(sp_params* p);
 

@@ -533,15 +533,16 @@ static void project_mean(const mg_level* A, double* v) {
 
 static void vcycle(mg_hier* H, int l, const orc_solver_opts* o) {
   mg_level* A = &H->lv[l];
+  const double om = l == 0 ? o->omega : o->omega_coarse;
   memset(A->x, 0, sizeof(double) * A->n);
   if (l == H->L - 1) {
     if (H->singular) project_mean(A, A->b);
-    for (int s = 0; s < o->coarse_sweeps; ++s) { gs_color(A, 0, o->omega); gs_color(A, 1, o->omega); }
-    for (int s = 0; s < o->coarse_sweeps; ++s) { gs_color(A, 1, o->omega); gs_color(A, 0, o->omega); }
+    for (int s = 0; s < o->coarse_sweeps; ++s) { gs_color(A, 0, om); gs_color(A, 1, om); }
+    for (int s = 0; s < o->coarse_sweeps; ++s) { gs_color(A, 1, om); gs_color(A, 0, om); }
     if (H->singular) project_mean(A, A->x);
     return;
   }
-  for (int s = 0; s < o->pre; ++s) { gs_color(A, 0, o->omega); gs_color(A, 1, o->omega); }
+  for (int s = 0; s < o->pre; ++s) { gs_color(A, 0, om); gs_color(A, 1, om); }
   sym_apply_level(A->nx, A->ny, A->nz, A->cx, A->cy, A->cz, A->d, A->x, A->r);
   mg_level* C = &H->lv[l + 1];
   const int64_t fsy = A->nx, fsz = (int64_t)A->nx * A->ny;
@@ -568,7 +569,7 @@ static void vcycle(mg_hier* H, int l, const orc_solver_opts* o) {
         const int64_t m = i + fsy * j + fsz * k;
         if (A->d[m] > 0) A->x[m] += C->x[(i / 2) + (int64_t)C->nx * ((j / 2) + (int64_t)C->ny * (k / 2))];
       }
-  for (int s = 0; s < o->post; ++s) { gs_color(A, 1, o->omega); gs_color(A, 0, o->omega); }
+  for (int s = 0; s < o->post; ++s) { gs_color(A, 1, om); gs_color(A, 0, om); }
 }
 
 static double dot_active(int64_t n, const double* d, const double* a, const double* b) {

@@ -96,7 +96,8 @@ typedef struct orc_solver_opts {
   double  div_tol;      /* stop when max |r|/V <= div_tol */
   int32_t use_rows;     /* 1: outer operator = reference rows (vals/cols) */
   int32_t precond;      /* 0 none, 1 Jacobi, 2 multigrid */
-  double  omega;        /* RB-GS over-relaxation: x += omega (x_gs - x) */
+  double  omega;        /* RB-GS over-relaxation on the finest level: x += omega (x_gs - x) */
+  double  omega_coarse; /* the same on every coarser level */
 } orc_solver_opts;
 
 typedef struct orc_solver_stats {

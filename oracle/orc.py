@@ -30,7 +30,7 @@ class OrcOpts(ctypes.Structure):
                 ("coarse_sweeps", ctypes.c_int32), ("max_iter", ctypes.c_int32),
                 ("coarse_scale", ctypes.c_double), ("div_tol", ctypes.c_double),
                 ("use_rows", ctypes.c_int32), ("precond", ctypes.c_int32),
-                ("omega", ctypes.c_double)]
+                ("omega", ctypes.c_double), ("omega_coarse", ctypes.c_double)]
 
 
 class OrcStats(ctypes.Structure):
@@ -108,9 +108,9 @@ def grid_of(case):
 
 
 def opts(levels=0, pre=2, post=2, coarse_sweeps=4, max_iter=200, coarse_scale=1.0,
-         div_tol=1e-10, use_rows=0, precond=2, omega=1.0):
+         div_tol=1e-10, use_rows=0, precond=2, omega=1.0, omega_coarse=1.0):
     return OrcOpts(levels, pre, post, coarse_sweeps, max_iter, coarse_scale, div_tol, use_rows,
-                   precond, omega)
+                   precond, omega, omega_coarse)
 
 
 def gfm_coef(case, rho_l=None, rho_g=None, rho_ref=None, epsi=1.0e-3, phi_l=None, vof_l=None,
@@ -141,7 +141,7 @@ def num_threads():
 
 
 def baseline_project(case, levels=0, pre=2, post=2, coarse_sweeps=4, coarse_scale=0.5,
-                     div_tol=1e-10, max_iter=200, omega=1.25):
+                     div_tol=1e-10, max_iter=200, omega=1.25, omega_coarse=1.5):
     """The CPU baseline: the same path and multigrid-preconditioned CG as the
     engine, FP64, OpenMP over the host cores.  Returns (seconds, info)."""
     import time
@@ -152,7 +152,8 @@ def baseline_project(case, levels=0, pre=2, post=2, coarse_sweeps=4, coarse_scal
     d, _ = sym_diag(case, *op)
     t1 = time.perf_counter()
     o = opts(levels=levels, pre=pre, post=post, coarse_sweeps=coarse_sweeps,
-             coarse_scale=coarse_scale, div_tol=div_tol, max_iter=max_iter, omega=omega)
+             coarse_scale=coarse_scale, div_tol=div_tol, max_iter=max_iter, omega=omega,
+             omega_coarse=omega_coarse)
     x, st, sing = pcg(case, op, b, o)
     t2 = time.perf_counter()
     Pp = pressure_field(case, x)

@@ -203,7 +203,11 @@ static bool check_params(const sp_params* p, std::string& why) {
   if (!(p->div_tol >= 0.0)) { why = "div_tol"; return false; }
   if (!(p->rho_l > 0) || !(p->rho_g > 0) || !(p->rho_ref > 0)) { why = "densities"; return false; }
   if (!(p->mg_coarse_scale > 0)) { why = "mg_coarse_scale"; return false; }
-  if (!(p->mg_omega > 0.0 && p->mg_omega < 2.0)) { why = "mg_omega must be in (0, 2)"; return false; }
+  if (!(p->mg_omega > 0.0 && p->mg_omega < 2.0) ||
+      !(p->mg_omega_coarse > 0.0 && p->mg_omega_coarse < 2.0)) {
+    why = "mg_omega and mg_omega_coarse must be in (0, 2)";
+    return false;
+  }
   return true;
 }
 
@@ -367,6 +371,7 @@ void sp_params_default(sp_params* p) {
   p->mg_coarse_sweeps = 4;
   p->mg_coarse_scale = 0.5;
   p->mg_omega = 1.25;
+  p->mg_omega_coarse = 1.5;
   p->max_iter = 200;
   p->div_tol = 1.0e-10;
   p->rho_l = 998.3;   // row, StateVariables.f90:10

@@ -150,7 +150,7 @@ template <typename T>
 struct TailArgs {
   int first, last;  // levels [first, last)
   int pre, post, coarse_sweeps, singular, has_e;
-  T omega;  // over-relaxation of the Gauss-Seidel updates
+  T omega, omega_c;  // over-relaxation of the Gauss-Seidel updates (level 0, coarser)
   const LevelView<T>* lv;   // the levels in global memory
   int off[kMaxLevels];      // element offset of each tail level in LDS
 };
@@ -313,28 +313,31 @@ __device__ void tail_prolong(const TailLevel<T>& F, const TailLevel<T>& C) {
 template <bool WAVE, typename T>
 __device__ void tail_pre(T* lds, const TailArgs<T>& A, int l) {
   const TailLevel<T> F = tail_level(lds, A, l);
-  tail_gs<WAVE>(F, 0, 1, A.omega);
-  tail_gs<WAVE>(F, 1, 0, A.omega);
-  for (int s = 1; s < A.pre; ++s) { tail_gs<WAVE>(F, 0, 0, A.omega); tail_gs<WAVE>(F, 1, 0, A.omega); }
+  const T om = l == 0 ? A.omega : A.omega_c;
+  tail_gs<WAVE>(F, 0, 1, om);
+  tail_gs<WAVE>(F, 1, 0, om);
+  for (int s = 1; s < A.pre; ++s) { tail_gs<WAVE>(F, 0, 0, om); tail_gs<WAVE>(F, 1, 0, om); }
   tail_restrict<WAVE>(F, tail_level(lds, A, l + 1));
 }
 
 template <bool WAVE, typename T>
 __device__ void tail_coarsest(T* lds, const TailArgs<T>& A) {
   const TailLevel<T> C = tail_level(lds, A, A.last - 1);
+  const T om = A.last - 1 == 0 ? A.omega : A.omega_c;
   if (A.singular) tail_project_mean<WAVE>(C, C.b);
-  tail_gs<WAVE>(C, 0, 1, A.omega);
-  tail_gs<WAVE>(C, 1, 0, A.omega);
-  for (int s = 1; s < A.coarse_sweeps; ++s) { tail_gs<WAVE>(C, 0, 0, A.omega); tail_gs<WAVE>(C, 1, 0, A.omega); }
-  for (int s = 0; s < A.coarse_sweeps; ++s) { tail_gs<WAVE>(C, 1, 0, A.omega); tail_gs<WAVE>(C, 0, 0, A.omega); }
+  tail_gs<WAVE>(C, 0, 1, om);
+  tail_gs<WAVE>(C, 1, 0, om);
+  for (int s = 1; s < A.coarse_sweeps; ++s) { tail_gs<WAVE>(C, 0, 0, om); tail_gs<WAVE>(C, 1, 0, om); }
+  for (int s = 0; s < A.coarse_sweeps; ++s) { tail_gs<WAVE>(C, 1, 0, om); tail_gs<WAVE>(C, 0, 0, om); }
   if (A.singular) tail_project_mean<WAVE>(C, C.x);
 }
 
 template <bool WAVE, typename T>
 __device__ void tail_post(T* lds, const TailArgs<T>& A, int l) {
   const TailLevel<T> F = tail_level(lds, A, l);
+  const T om = l == 0 ? A.omega : A.omega_c;
   tail_prolong<WAVE>(F, tail_level(lds, A, l + 1));
-  for (int s = 0; s < A.post; ++s) { tail_gs<WAVE>(F, 1, 0, A.omega); tail_gs<WAVE>(F, 0, 0, A.omega); }
+  for (int s = 0; s < A.post; ++s) { tail_gs<WAVE>(F, 1, 0, om); tail_gs<WAVE>(F, 0, 0, om); }
 }
 
 // The coarse levels of the V-cycle in one workgroup, resident in LDS: the
@@ -760,11 +763,11 @@ void launch_gs(sp_ctx* c, const Level<T>& L, int color, int zero_init, bool fine
               level_bytes(L, words), &slot);
   if (fine)
     hipLaunchKernelGGL((k_gs<T, true>), grd, blk, 0, c->stream, view_of(L), color, zero_init,
-                       (T)c->prm.mg_omega,
+                       (T)(fine ? c->prm.mg_omega : c->prm.mg_omega_coarse),
                        (const Scalars*)c->S);
   else
     hipLaunchKernelGGL((k_gs<T, false>), grd, blk, 0, c->stream, view_of(L), color, zero_init,
-                       (T)c->prm.mg_omega,
+                       (T)(fine ? c->prm.mg_omega : c->prm.mg_omega_coarse),
                        (const Scalars*)c->S);
   timer_end(c, slot);
 }
@@ -779,7 +782,7 @@ void launch_fused(sp_ctx* c, Level<T>* lv, int l, int mode) {
   A.cx = L.cx; A.cy = L.cy; A.cz = L.cz; A.e = L.e;
   A.b = L.b;
   A.cnx = C.nx; A.cny = C.ny;
-  A.omega = (T)c->prm.mg_omega;
+  A.omega = (T)(l == 0 ? c->prm.mg_omega : c->prm.mg_omega_coarse);
   if (mode == 0) { A.xout = L.xs; A.bc = C.b; }
   else { A.xin = L.xs; A.xc = C.x; A.xout = L.x; }
   if (mode == 1 && l == 0 && c->fuse_dots) {
@@ -876,6 +879,7 @@ void vcycle(sp_ctx* c, Level<T>* lv) {
     A.post = post;
     A.coarse_sweeps = std::max(1, c->prm.mg_coarse_sweeps);
     A.omega = (T)c->prm.mg_omega;
+    A.omega_c = (T)c->prm.mg_omega_coarse;
     A.singular = c->S_host->singular;
     A.has_e = lv[Lt].e ? 1 : 0;
     A.lv = reinterpret_cast<const LevelView<T>*>(c->mg_desc);

@@ -48,6 +48,7 @@ class SpParams(ctypes.Structure):
         ("epsi", ctypes.c_double),
         ("pin_i", ctypes.c_int32), ("pin_j", ctypes.c_int32), ("pin_k", ctypes.c_int32),
         ("flags", ctypes.c_int32), ("mg_omega", ctypes.c_double),
+        ("mg_omega_coarse", ctypes.c_double),
     ]
 
 

@@ -22,6 +22,7 @@ program sp_layout
   print '(a,1x,i0)', 'sp_params.pin_i', off(c_loc(prm%pin_i), c_loc(prm))
   print '(a,1x,i0)', 'sp_params.flags', off(c_loc(prm%flags), c_loc(prm))
   print '(a,1x,i0)', 'sp_params.mg_omega', off(c_loc(prm%mg_omega), c_loc(prm))
+  print '(a,1x,i0)', 'sp_params.mg_omega_coarse', off(c_loc(prm%mg_omega_coarse), c_loc(prm))
   print '(a,1x,i0)', 'sp_inputs.bc_flag', off(c_loc(inp%bc_flag), c_loc(inp))
   print '(a,1x,i0)', 'sp_inputs.bc_val', off(c_loc(inp%bc_val), c_loc(inp))
   print '(a,1x,i0)', 'sp_inputs.dt', off(c_loc(inp%dt), c_loc(inp))

@@ -38,6 +38,7 @@ def test_status_strings_and_defaults():
     p = engine.SpParams()
     L.sp_params_default(ctypes.byref(p))
     assert p.mg_pre == 2 and p.mg_post == 2 and p.div_tol == 1e-10 and p.mg_omega == 1.25
+    assert p.mg_omega_coarse == 1.5
     assert p.rho_l == 998.3 and p.rho_g == 1.205 and p.epsi == 1e-3
 
 
