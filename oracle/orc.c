@@ -862,3 +862,174 @@ int orc_adjoint(const orc_grid* g, const double* pc, const double* ee, const dou
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }
+
+/* ------------------------------------------------------------------ */
+/* Post-correction wall step of UpdatePUVW                             */
+/* ------------------------------------------------------------------ */
+/* TypicalBC (BoundaryFunction.f90:760-781) as BCx%West..Top call it from
+ * BoundaryConditionVarNew: the "inner" argument is a full halo slice such as
+ * Vari%u(1,:,:), passed to an assumed-shape dummy whose element (a,b) is
+ * therefore the halo entry (a-1,b-1) of the slice (ComputePUVW.f90:431-435
+ * passes Vari%u(1,:,:) with the halo, PGrid%dx(1,:,:) without).  The face
+ * value of element (a,b), 1-based, is
+ *   Dirichlet: const;  Neumann: X(src, a-1, b-1) - const*dxyz(a,b)/2
+ * with dxyz = mult * spacing of the boundary cell (BoundaryFunction.f90:83-513
+ * for the multipliers). */
+static void wall_face_values(const orc_grid* g, const double* X, int f, int src, double mult,
+                             int flag, double cst, double* var) {
+  const int I = g->I, J = g->J, K = g->K;
+  const int n1 = f < 2 ? J : I, n2 = f < 4 ? K : J;
+  for (int b = 1; b <= n2; ++b)
+    for (int a = 1; a <= n1; ++a) {
+      double* o = var + (a - 1) + (int64_t)n1 * (b - 1);
+      if (flag == 0) { *o = cst; continue; }
+      int64_t h;
+      double d;
+      if (f < 2) { h = HX(g, src, a - 1, b - 1); d = f == 0 ? g->dxp[0] : g->dxp[I - 1]; }
+      else if (f < 4) { h = HX(g, a - 1, src, b - 1); d = f == 2 ? g->dyp[0] : g->dyp[J - 1]; }
+      else { h = HX(g, a - 1, b - 1, src); d = f == 4 ? g->dzp[0] : g->dzp[K - 1]; }
+      const double dxyz = mult * d;
+      *o = X[h] - cst * dxyz / 2.0;
+    }
+}
+
+/* halo index of element (a,b) (1-based) of the plane `plane` of face f */
+static int64_t wall_plane_index(const orc_grid* g, int f, int plane, int a, int b) {
+  if (f < 2) return HX(g, plane, a, b);
+  if (f < 4) return HX(g, a, plane, b);
+  return HX(g, a, b, plane);
+}
+
+/* BoundaryConditionVarNew (ComputePUVW.f90:418-694) for one variable q
+ * (0 u, 1 v, 2 w, 3 p), faces in the reference's order W,E,S,N,B,T. */
+static void wall_reset_var(const orc_grid* g, int q, double* X, const int32_t* fl,
+                           const double* cst, double* const* var) {
+  const int I = g->I, J = g->J, K = g->K;
+  const int last[6] = {1, I, 1, J, 1, K};       /* boundary cell plane */
+  const int ghost[6] = {0, I + 1, 0, J + 1, 0, K + 1};
+  for (int f = 0; f < 6; ++f) {
+    /* slice the BC routine reads and the spacing multiplier: u E reads
+     * u(Imax-1,:,:) with -2 dx (ComputePUVW.f90:440-444,
+     * BoundaryFunction.f90:102), v N v(:,Jmax-1,:) with -2 dy (:532-537,
+     * :251), w T w(:,:,Kmax-1) with -2 dz (:618-622, :399); every other face
+     * reads the boundary cell's own plane with +dx/dy/dz on W/S/B and
+     * -dx/dy/dz on E/N/T (BoundaryFunction.f90:83-513). */
+    int src = last[f];
+    double mult = (f % 2 == 0) ? 1.0 : -1.0;
+    if ((q == 0 && f == 1) || (q == 1 && f == 3) || (q == 2 && f == 5)) {
+      src = last[f] - 1;
+      mult = -2.0;
+    }
+    const int n1 = f < 2 ? J : I, n2 = f < 4 ? K : J;
+    double* V = var[f];
+    wall_face_values(g, X, f, src, mult, fl[f], cst[f], V);
+    /* direct assignment of the wall-normal face: u(0) and u(Imax) (:437,446),
+     * v(:,0,:) and v(:,Jmax,:) (:529,538), w(:,:,0) and w(:,:,Kmax) (:615,624);
+     * for p the top cell layer p(:,:,Kmax) (:689) */
+    const int assign = (q == 0 && f < 2) || (q == 1 && (f == 2 || f == 3)) ||
+                       (q == 2 && f >= 4) || (q == 3 && f == 5);
+    const int target = f % 2 == 0 ? 0 : last[f];
+    if (assign)
+      for (int b = 1; b <= n2; ++b)
+        for (int a = 1; a <= n1; ++a)
+          X[wall_plane_index(g, f, target, a, b)] = V[(a - 1) + (int64_t)n1 * (b - 1)];
+    /* ComputeGhostVarBoundary (:696-708) on every face except u W, v S, w B */
+    const int no_ghost = (q == 0 && f == 0) || (q == 1 && f == 2) || (q == 2 && f == 4);
+    if (no_ghost) continue;
+    for (int b = 1; b <= n2; ++b)
+      for (int a = 1; a <= n1; ++a) {
+        const double bc = V[(a - 1) + (int64_t)n1 * (b - 1)];
+        const double in = X[wall_plane_index(g, f, last[f], a, b)];
+        X[wall_plane_index(g, f, ghost[f], a, b)] = fl[f] == 1 ? bc - in : 2.0 * bc - in;
+      }
+  }
+}
+
+/* Global mass fix of the first Dirichlet pressure face (ComputePUVW.f90:256-384,
+ * an if/elseif chain W,E,S,N,B,T).  The north branch of the reference loops
+ * j over 1..Jmax with k left at its value after the preceding loops (past
+ * Kmax), so it reads outside the arrays; the restatement takes the branch's
+ * evident intent, the mirror of the south branch at j = Jmax. */
+static void wall_mass_fix(const orc_grid* g, const int32_t* flp, const double* ee, const double* ne,
+                          const double* te, double* u, double* v, double* w) {
+  const int I = g->I, J = g->J, K = g->K;
+  int f = 0;
+  while (f < 6 && flp[f] != 0) ++f;
+  if (f == 6) return;
+  if (f < 2) {
+    const int i = f == 0 ? 1 : I;
+    for (int j = 1; j <= J; ++j)
+      for (int k = 1; k <= K; ++k) {
+        const int64_t h = HX(g, i, j, k);
+        const double dx = g->dxp[i - 1], dy = g->dyp[j - 1], dz = g->dzp[k - 1];
+        if (f == 0)
+          u[HX(g, i - 1, j, k)] = (dx * dz * (ne[h] * v[h] - ne[h] * v[HX(g, i, j - 1, k)]) +
+                                   dx * dy * (te[h] * w[h] - te[h] * w[HX(g, i, j, k - 1)]) +
+                                   dy * dz * ee[h] * u[h]) / ee[h] / dy / dz;
+        else
+          u[h] = (dx * dz * (-ne[h] * v[h] + ne[h] * v[HX(g, i, j - 1, k)]) +
+                  dx * dy * (-te[h] * w[h] + te[h] * w[HX(g, i, j, k - 1)]) +
+                  dy * dz * ee[h] * u[HX(g, i - 1, j, k)]) / ee[h] / dy / dz;
+      }
+  } else if (f < 4) {
+    const int j = f == 2 ? 1 : J;
+    for (int i = 1; i <= I; ++i)
+      for (int k = 1; k <= K; ++k) {
+        const int64_t h = HX(g, i, j, k);
+        const double dx = g->dxp[i - 1], dy = g->dyp[j - 1], dz = g->dzp[k - 1];
+        if (f == 2)
+          v[HX(g, i, j - 1, k)] = (dy * dz * (ee[h] * u[h] - ee[h] * u[HX(g, i - 1, j, k)]) +
+                                   dx * dy * (te[h] * w[h] - te[h] * w[HX(g, i, j, k - 1)]) +
+                                   dx * dz * ne[h] * v[h]) / ne[h] / dx / dz;
+        else
+          v[h] = (-dy * dz * (ee[h] * u[h] - ee[h] * u[HX(g, i - 1, j, k)]) -
+                  dx * dy * (te[h] * w[h] - te[h] * w[HX(g, i, j, k - 1)]) +
+                  dx * dz * ne[h] * v[HX(g, i, j - 1, k)]) / ne[h] / dx / dz;
+      }
+  } else {
+    const int k = f == 4 ? 1 : K;
+    for (int i = 1; i <= I; ++i)
+      for (int j = 1; j <= J; ++j) {
+        const int64_t h = HX(g, i, j, k);
+        const double dx = g->dxp[i - 1], dy = g->dyp[j - 1], dz = g->dzp[k - 1];
+        if (f == 4)
+          w[HX(g, i, j, k - 1)] = (dy * dz * (ee[h] * u[h] - ee[h] * u[HX(g, i - 1, j, k)]) +
+                                   dx * dz * (ne[h] * v[h] - ne[h] * v[HX(g, i, j - 1, k)]) +
+                                   dx * dy * te[h] * w[h]) / te[h] / dx / dy;
+        else
+          w[h] = (-dy * dz * (ee[h] * u[h] - ee[h] * u[HX(g, i - 1, j, k)]) -
+                  dx * dz * (ne[h] * v[h] - ne[h] * v[HX(g, i, j - 1, k)]) +
+                  dx * dy * te[h] * w[HX(g, i, j, k - 1)]) / te[h] / dx / dy;
+      }
+  }
+}
+
+void orc_wall_step(const orc_grid* g, const double* ee, const double* ne, const double* te,
+                   const int32_t* ctp, const int32_t* ctu, const int32_t* ctv, const int32_t* ctw,
+                   const int32_t* fl, const double* cst, double* u, double* v, double* w,
+                   double* p, double* const* var) {
+  const int I = g->I, J = g->J, K = g->K;
+  double* X[4] = {u, v, w, p};
+  const int64_t plane[6] = {(int64_t)J * K, (int64_t)J * K, (int64_t)I * K,
+                            (int64_t)I * K, (int64_t)I * J, (int64_t)I * J};
+  for (int q = 0; q < 4; ++q) {
+    double* V[6];
+    for (int f = 0; f < 6; ++f) V[f] = (double*)malloc(sizeof(double) * plane[f]);
+    wall_reset_var(g, q, X[q], fl + 6 * q, cst + 6 * q, V);
+    for (int f = 0; f < 6; ++f) {
+      if (var && var[6 * q + f]) memcpy(var[6 * q + f], V[f], sizeof(double) * plane[f]);
+      free(V[f]);
+    }
+  }
+  wall_mass_fix(g, fl + 18, ee, ne, te, u, v, w);
+  /* VariablesInternalCellCondition (Solver.f90:467-501) */
+  for (int k = 1; k <= K; ++k)
+    for (int j = 1; j <= J; ++j)
+      for (int i = 1; i <= I; ++i) {
+        const int64_t n = NX(g, i, j, k), h = HX(g, i, j, k);
+        if (ctp[n] == 2) p[h] = 0.0;
+        if (ctu[n] == 2) u[h] = 0.0;
+        if (ctv[n] == 2) v[h] = 0.0;
+        if (ctw[n] == 2) w[h] = 0.0;
+      }
+}

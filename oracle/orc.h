@@ -28,6 +28,11 @@
  *   orc_mres
  *   orc_adjoint         dot-product test of the linear map, central
  *                       differences in u* and in PoCoef(:,4..6)
+ *   orc_wall_step       a transcription of BoundaryConditionVarNew in numpy
+ *                       slice semantics (the assumed-shape halo offset comes
+ *                       out of the slicing itself), Dirichlet faces equal to
+ *                       their constants, ghost rules, zero divergence of the
+ *                       mass-fixed boundary layer for uniform apertures
  * No reference golden vector exists for this path and HYPRE is absent, so the
  * HYPRE iterate itself is not pinned (any converged solve of the pinned,
  * uniquely solvable system is).
@@ -133,6 +138,19 @@ void orc_correct(const orc_grid* g, const double* pocoef, const double* Pp,
 /* mres (Solver.f90:236-250), interior layout; returns max |mres|/V. */
 double orc_mres(const orc_grid* g, const double* ee, const double* ne, const double* te,
                 const double* u, const double* v, const double* w, double* mres);
+
+/* Post-correction wall step of UpdatePUVW, in the reference's order:
+ * BoundaryConditionVarNew (ComputePUVW.f90:418-694) with TypicalBC
+ * (BoundaryFunction.f90:760-781) and ComputeGhostVarBoundary
+ * (ComputePUVW.f90:696-708), the mass fix of the first Dirichlet pressure
+ * face (ComputePUVW.f90:256-384), VariablesInternalCellCondition
+ * (Solver.f90:467-501).  fl/cst: 24 flags and constants, BCu W..T, BCv, BCw,
+ * BCp.  u, v, w, p (halo arrays) are updated in place; var (24 pointers,
+ * each may be NULL) receives the BCx%VarW..VarT face values. */
+void orc_wall_step(const orc_grid* g, const double* ee, const double* ne, const double* te,
+                   const int32_t* ctp, const int32_t* ctu, const int32_t* ctv, const int32_t* ctw,
+                   const int32_t* fl, const double* cst, double* u, double* v, double* w,
+                   double* p, double* const* var);
 
 /* Adjoint of the projection for L with dL/du = gu, dL/dv = gv, dL/dw = gw
  * (halo arrays, read on the corrected faces (1:I,1:J,1:K)).  Gradients are

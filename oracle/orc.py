@@ -83,6 +83,9 @@ def lib():
         L.orc_adjoint.argtypes = ([G, F64P, F64P, F64P, F64P, I32P, I32P] + [I32P] * 6 +
                                   [F64P] * 4 + [ctypes.POINTER(OrcOpts)] + [F64P] * 7)
         L.orc_adjoint.restype = ctypes.c_int
+        L.orc_wall_step.argtypes = [G, F64P, F64P, F64P] + [I32P] * 5 + [F64P] * 5 + [
+            ctypes.POINTER(F64P)]
+        L.orc_wall_step.restype = None
         _lib = L
     return _lib
 
@@ -348,3 +351,26 @@ def adjoint(case, pc, Pp, gu, gv, gw, o=None):
                            _p(np.asfortranarray(gw)), ctypes.byref(o), _p(gus), _p(gvs),
                            _p(gws), _p(gkx), _p(gky), _p(gkz), _p(lam))
     return dict(gus=gus, gvs=gvs, gws=gws, gkx=gkx, gky=gky, gkz=gkz, lam=lam, rc=rc)
+
+
+def wall_plane_shapes(case):
+    I, J, K = case.I, case.J, case.K
+    return [(J, K), (J, K), (I, K), (I, K), (I, J), (I, J)]
+
+
+def wall_step(case, flags, consts, u, v, w, p):
+    """BoundaryConditionVarNew + mass fix + VariablesInternalCellCondition on
+    copies of u, v, w, p.  flags/consts: 24 values (BCu W..T, BCv, BCw, BCp).
+    Returns (u, v, w, p, var) with var the 24 face-value arrays."""
+    u, v, w, p = (np.array(a, dtype=np.float64, order="F", copy=True) for a in (u, v, w, p))
+    fl = np.ascontiguousarray(flags, dtype=np.int32)
+    cs = np.ascontiguousarray(consts, dtype=np.float64)
+    assert fl.shape == (24,) and cs.shape == (24,)
+    shapes = wall_plane_shapes(case)
+    var = [np.zeros(shapes[q % 6], order="F") for q in range(24)]
+    vp = (F64P * 24)(*[_p(a) for a in var])
+    g = grid_of(case)
+    lib().orc_wall_step(ctypes.byref(g), _p(case.ee_area), _p(case.ne_area), _p(case.te_area),
+                        _p(case.cell_type_p), _p(case.cell_type_u), _p(case.cell_type_v),
+                        _p(case.cell_type_w), _p(fl), _p(cs), _p(u), _p(v), _p(w), _p(p), vp)
+    return u, v, w, p, var
