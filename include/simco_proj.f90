@@ -14,10 +14,11 @@ module simco_proj
   use iso_c_binding
   implicit none
   private
-  public :: sp_params, sp_inputs, sp_outputs, sp_stats, sp_adj_inputs, sp_adj_outputs
+  public :: sp_params, sp_inputs, sp_outputs, sp_stats, sp_adj_inputs, sp_adj_outputs, &
+            sp_wall_bc
   public :: sp_params_default, sp_create, sp_destroy, sp_set_params, sp_set_grid, &
             sp_set_grid_1d, sp_project, sp_project_dev, sp_adjoint, sp_adjoint_dev, &
-            sp_strerror_f, sp_last_error_f
+            sp_wall_step, sp_wall_step_dev, sp_strerror_f, sp_last_error_f
   public :: SP_OK, SP_ERR_INVALID_ARG, SP_ERR_HIP, SP_ERR_NOT_CONVERGED, SP_ERR_NONFINITE, &
             SP_ERR_OOM, SP_ERR_GRID, SP_ERR_STATE, SP_ERR_NO_DEVICE, SP_PREC_FP64, &
             SP_PREC_MIXED, SP_PRECOND_MG, SP_PRECOND_JACOBI
@@ -43,6 +44,15 @@ module simco_proj
     real(c_double) :: mg_omega, mg_omega_coarse
   end type sp_params
 
+  ! BCu/BCv/BCw%flag and BCu/BCv/BCw/BCp%const for the wall step after the
+  ! correction (BoundaryConditionVarNew, the Dirichlet-pressure mass fix and
+  ! VariablesInternalCellCondition); the pressure flags are sp_inputs%bc_flag.
+  type, bind(C) :: sp_wall_bc
+    integer(c_int32_t) :: flag_u(6) = 1, flag_v(6) = 1, flag_w(6) = 1
+    real(c_double) :: const_u(6) = 0.0_c_double, const_v(6) = 0.0_c_double
+    real(c_double) :: const_w(6) = 0.0_c_double, const_p(6) = 0.0_c_double
+  end type sp_wall_bc
+
   type, bind(C) :: sp_inputs
     type(c_ptr) :: u_star = c_null_ptr, v_star = c_null_ptr, w_star = c_null_ptr
     type(c_ptr) :: dp_u = c_null_ptr, dp_v = c_null_ptr, dp_w = c_null_ptr
@@ -56,12 +66,15 @@ module simco_proj
     type(c_ptr) :: bc_val(6) = c_null_ptr
     real(c_double) :: dt = 0.0_c_double
     type(c_ptr) :: beta_e = c_null_ptr, beta_n = c_null_ptr, beta_t = c_null_ptr
+    type(c_ptr) :: wall = c_null_ptr      ! c_loc of a type(sp_wall_bc), or null
   end type sp_inputs
 
   type, bind(C) :: sp_outputs
     type(c_ptr) :: p = c_null_ptr
     type(c_ptr) :: u = c_null_ptr, v = c_null_ptr, w = c_null_ptr
     type(c_ptr) :: po_coef = c_null_ptr
+    ! BCu%VarW..VarT, BCv, BCw, BCp after the wall step (null entries skipped)
+    type(c_ptr) :: bc_var(24) = c_null_ptr
   end type sp_outputs
 
   type, bind(C) :: sp_stats
@@ -69,6 +82,7 @@ module simco_proj
     integer(c_int64_t) :: active_cells, stale_faces
     real(c_double) :: initial_div, final_div, residual_div
     real(c_double) :: t_assemble_ms, t_solve_ms, t_correct_ms, t_total_ms
+    real(c_double) :: wall_div
   end type sp_stats
 
   type, bind(C) :: sp_adj_inputs
@@ -151,6 +165,22 @@ module simco_proj
       type(sp_adj_outputs), intent(in) :: outp
       type(sp_stats), intent(out) :: st
     end function sp_adjoint_dev
+
+    integer(c_int) function sp_wall_step(ctx, inp, outp, st) bind(C, name="sp_wall_step")
+      import :: c_ptr, c_int, sp_inputs, sp_outputs, sp_stats
+      type(c_ptr), value :: ctx
+      type(sp_inputs), intent(in) :: inp
+      type(sp_outputs), intent(in) :: outp
+      type(sp_stats), intent(out) :: st
+    end function sp_wall_step
+
+    integer(c_int) function sp_wall_step_dev(ctx, inp, outp, st) bind(C, name="sp_wall_step_dev")
+      import :: c_ptr, c_int, sp_inputs, sp_outputs, sp_stats
+      type(c_ptr), value :: ctx
+      type(sp_inputs), intent(in) :: inp
+      type(sp_outputs), intent(in) :: outp
+      type(sp_stats), intent(out) :: st
+    end function sp_wall_step_dev
 
     type(c_ptr) function sp_strerror_c(s) bind(C, name="sp_strerror")
       import :: c_ptr, c_int

@@ -88,6 +88,24 @@ enum {
 
 typedef struct sp_ctx sp_ctx;
 
+/* The host's velocity and pressure boundary conditions (BCu, BCv, BCw, BCp:
+ * %flag(1..6) and %const(1..6), BoundaryInterface.f90:7-21, set in the
+ * drivers as Main.f90:92-105 does) for the wall step that follows the
+ * correction in UpdatePUVW:
+ *   BoundaryConditionVarNew (ComputePUVW.f90:255, 418-694): TypicalBC face
+ *     values (BoundaryFunction.f90:760-781), the wall-normal faces
+ *     u(0), u(Imax), v(:,0,:), v(:,Jmax,:), w(:,:,0), w(:,:,Kmax) and the
+ *     top pressure layer p(:,:,Kmax) set to them, ghosts by
+ *     ComputeGhostVarBoundary (:696-708);
+ *   the mass fix of the first Dirichlet pressure face (:256-384);
+ *   VariablesInternalCellCondition (Solver.f90:223, 467-501).
+ * The pressure flags are sp_inputs.bc_flag.  Flags: 0 Dirichlet, 1 Neumann,
+ * faces W,E,S,N,B,T. */
+typedef struct sp_wall_bc {
+  int32_t flag_u[6], flag_v[6], flag_w[6];
+  double const_u[6], const_v[6], const_w[6], const_p[6];
+} sp_wall_bc;
+
 /* Inputs of one projection.  Pointers are host memory for sp_project and
  * device memory for sp_project_dev. */
 typedef struct sp_inputs {
@@ -107,12 +125,18 @@ typedef struct sp_inputs {
    * PoCoef(:,4..6) = Dp/delta * beta; W,S,B faces take the neighbour's value.
    * This is the parametrisation the adjoint differentiates. */
   const double *beta_e, *beta_n, *beta_t;
+  /* Optional: the wall step after the correction (NULL: the host keeps it). */
+  const sp_wall_bc *wall;
 } sp_inputs;
 
 typedef struct sp_outputs {
   double *p;          /* Proj%Pp with ghosts (halo array, every entry written) */
   double *u, *v, *w;  /* TVar%u,v,w: entries (1:I,1:J,1:K) written, others kept */
   double *po_coef;    /* optional PoCoef (cell array x6), NULL to skip */
+  /* Optional face values after the wall step, each NULL to skip:
+   * BCu%VarW,VarE,VarS,VarN,VarB,VarT, then BCv, BCw, BCp (24 arrays of
+   * (Jmax,Kmax) for W/E, (Imax,Kmax) for S/N, (Imax,Jmax) for B/T). */
+  double *bc_var[24];
 } sp_outputs;
 
 typedef struct sp_stats {
@@ -129,6 +153,8 @@ typedef struct sp_stats {
   double final_div;       /* max |div u| of the corrected field */
   double residual_div;    /* max |A p - b|/V of the recursive residual */
   double t_assemble_ms, t_solve_ms, t_correct_ms, t_total_ms;
+  double wall_div;        /* max |mres|/V after the wall step (Solver.f90:236-250),
+                             0 when no wall step ran */
 } sp_stats;
 
 /* Adjoint inputs: dL/du, dL/dv, dL/dw on the corrected faces (halo arrays,
@@ -175,10 +201,19 @@ sp_status sp_project(sp_ctx* ctx, const sp_inputs* in, sp_outputs* out, sp_stats
 /* Same with device arrays (no host<->device copies). */
 sp_status sp_project_dev(sp_ctx* ctx, const sp_inputs* in, sp_outputs* out, sp_stats* stats);
 
+/* The wall step alone (sp_wall_bc) on out->u, v, w, p, updated in place from
+ * their current values.  Reads ee_area, ne_area, te_area, cell_type_p/u/v/w,
+ * bc_flag and wall of `in` (the other input pointers may be NULL); writes
+ * out->bc_var and stats->wall_div.  Host and device variants. */
+sp_status sp_wall_step(sp_ctx* ctx, const sp_inputs* in, sp_outputs* out, sp_stats* stats);
+sp_status sp_wall_step_dev(sp_ctx* ctx, const sp_inputs* in, sp_outputs* out, sp_stats* stats);
+
 /* Reverse-mode derivative of the most recent projection on this context.
  * Reuses the multigrid-preconditioned CG on the backward source (the
  * operator is symmetric); only the forward pressure is kept from the
- * forward call.  Host and device variants. */
+ * forward call.  It differentiates the projection without the wall step:
+ * after a forward call with sp_inputs.wall set it returns SP_ERR_STATE.
+ * Host and device variants. */
 sp_status sp_adjoint(sp_ctx* ctx, const sp_adj_inputs* in, sp_adj_outputs* out, sp_stats* stats);
 sp_status sp_adjoint_dev(sp_ctx* ctx, const sp_adj_inputs* in, sp_adj_outputs* out,
                          sp_stats* stats);

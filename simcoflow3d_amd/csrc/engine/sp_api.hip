@@ -181,6 +181,10 @@ static sp_status ensure_buffers(sp_ctx* c) {
     const int64_t m = std::max<int64_t>({(int64_t)D.J * D.K, (int64_t)D.I * D.K, (int64_t)D.I * D.J});
     if ((s = dmalloc(c, &c->bcv[f], m))) return s;
   }
+  // face values of the wall step: u, v, w, p x the six boundary planes
+  if ((s = dmalloc(c, &c->wall_var,
+                   8 * ((int64_t)D.J * D.K + (int64_t)D.I * D.K + (int64_t)D.I * D.J))))
+    return s;
   c->allocated_for = (size_t)N;
   c->allocated_prec = prec;
   c->allocated_precond = pc;
@@ -211,6 +215,13 @@ static bool check_params(const sp_params* p, std::string& why) {
   return true;
 }
 
+static bool check_wall(const sp_wall_bc* w) {
+  for (int f = 0; f < 6; ++f)
+    for (int32_t v : {w->flag_u[f], w->flag_v[f], w->flag_w[f]})
+      if (v != 0 && v != 1) return false;
+  return true;
+}
+
 static sp_status check_inputs(sp_ctx* c, const sp_inputs* in, const sp_outputs* out) {
   if (!in || !out) return fail(c, SP_ERR_INVALID_ARG, "null inputs/outputs");
   const void* req[] = {in->u_star, in->v_star, in->w_star, in->dp_u, in->dp_v, in->dp_w,
@@ -232,6 +243,8 @@ static sp_status check_inputs(sp_ctx* c, const sp_inputs* in, const sp_outputs* 
   }
   if (!(in->dt > 0.0) || !std::isfinite(in->dt)) return fail(c, SP_ERR_INVALID_ARG, "dt must be > 0");
   if (!c->grid_set) return fail(c, SP_ERR_GRID, "sp_set_grid has not been called");
+  if (in->wall && !check_wall(in->wall))
+    return fail(c, SP_ERR_INVALID_ARG, "sp_wall_bc flags must be 0 (Dirichlet) or 1 (Neumann)");
   return SP_OK;
 }
 
@@ -307,6 +320,9 @@ static sp_status project_device(sp_ctx* c, const sp_inputs* in, sp_outputs* out,
                        c->stream);
     if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(po_coef)");
   }
+  // the wall step of UpdatePUVW on the corrected field (the convergence
+  // test above stays on the projection's own output)
+  if (in->wall && (s = wall_step(c, *in, *out, host_bc))) return s;
   // checkpoint of the forward pressure for the adjoint: the iterate buffer
   // becomes the checkpoint and the old checkpoint the next iterate
   std::swap(c->x, c->x_ckpt);
@@ -315,12 +331,14 @@ static sp_status project_device(sp_ctx* c, const sp_inputs* in, sp_outputs* out,
   if (e != hipSuccess) return hip_check(c, e, "correction");
   auto t3 = std::chrono::steady_clock::now();
   S->final_div = c->S_host->divmax;
+  S->wall_div = in->wall ? c->S_host->walldiv : 0.0;
   S->converged = !c->S_host->nonfinite && (S->final_div <= c->prm.div_tol || res_converged);
   S->t_assemble_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   S->t_solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
   S->t_correct_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
   S->t_total_ms = std::chrono::duration<double, std::milli>(t3 - t0).count();
   c->last_in = *in;
+  c->last_wall = in->wall != nullptr;
   c->have_forward = true;
   timers_collect(c);
   return S->converged ? SP_OK : SP_ERR_NOT_CONVERGED;
@@ -538,6 +556,7 @@ sp_status sp_project(sp_ctx* c, const sp_inputs* in, sp_outputs* out, sp_stats* 
   }
   o.p = outs_d[0]; o.u = outs_d[1]; o.v = outs_d[2]; o.w = outs_d[3];
   o.po_coef = nullptr;
+  for (int q = 0; q < 24; ++q) o.bc_var[q] = out->bc_var[q];  // host memory
   c->last_in_host = true;
   sp_status r = project_device(c, &d, &o, st, true);
   if (r != SP_OK && r != SP_ERR_NOT_CONVERGED) return r;
@@ -580,6 +599,8 @@ sp_status sp_adjoint_dev(sp_ctx* c, const sp_adj_inputs* in, sp_adj_outputs* out
   if (!c || !in || !out) return SP_ERR_INVALID_ARG;
   hipSetDevice(c->device);
   if (!c->have_forward) return fail(c, SP_ERR_STATE, "no forward projection to differentiate");
+  if (c->last_wall)
+    return fail(c, SP_ERR_STATE, "the adjoint covers the projection without the wall step");
   if (!in->gu || !in->gv || !in->gw || !out->gu_star || !out->gv_star || !out->gw_star ||
       !out->gk_e || !out->gk_n || !out->gk_t)
     return fail(c, SP_ERR_INVALID_ARG, "null adjoint pointer");
@@ -592,6 +613,8 @@ sp_status sp_adjoint(sp_ctx* c, const sp_adj_inputs* in, sp_adj_outputs* out, sp
   if (!c->have_forward) return fail(c, SP_ERR_STATE, "no forward projection to differentiate");
   if (!c->last_in_host)
     return fail(c, SP_ERR_STATE, "forward ran on device arrays; use sp_adjoint_dev");
+  if (c->last_wall)
+    return fail(c, SP_ERR_STATE, "the adjoint covers the projection without the wall step");
   if (!in->gu || !in->gv || !in->gw || !out->gu_star || !out->gv_star || !out->gw_star ||
       !out->gk_e || !out->gk_n || !out->gk_t)
     return fail(c, SP_ERR_INVALID_ARG, "null adjoint pointer");
@@ -627,6 +650,82 @@ sp_status sp_adjoint(sp_ctx* c, const sp_adj_inputs* in, sp_adj_outputs* out, sp
   hipError_t e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_check(c, e, "sp_adjoint");
   return r;
+}
+
+static sp_status check_wall_call(sp_ctx* c, const sp_inputs* in, const sp_outputs* out) {
+  if (!in || !out || !in->wall) return fail(c, SP_ERR_INVALID_ARG, "null inputs/outputs/wall");
+  const void* req[] = {in->ee_area, in->ne_area, in->te_area, in->cell_type_p, in->cell_type_u,
+                       in->cell_type_v, in->cell_type_w, out->u, out->v, out->w, out->p};
+  for (const void* p : req)
+    if (!p) return fail(c, SP_ERR_INVALID_ARG, "required input/output pointer is null");
+  for (int f = 0; f < 6; ++f)
+    if (in->bc_flag[f] != 0 && in->bc_flag[f] != 1)
+      return fail(c, SP_ERR_INVALID_ARG, "bc_flag must be 0 (Dirichlet) or 1 (Neumann)");
+  if (!check_wall(in->wall))
+    return fail(c, SP_ERR_INVALID_ARG, "sp_wall_bc flags must be 0 (Dirichlet) or 1 (Neumann)");
+  if (!c->grid_set) return fail(c, SP_ERR_GRID, "sp_set_grid has not been called");
+  return ensure_buffers(c);
+}
+
+static sp_status wall_finish(sp_ctx* c, sp_stats* st) {
+  hipError_t e = hipMemcpyAsync(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "wall step");
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->wall_div = c->S_host->walldiv;
+    st->converged = 1;
+  }
+  timers_collect(c);
+  return SP_OK;
+}
+
+sp_status sp_wall_step_dev(sp_ctx* c, const sp_inputs* in, sp_outputs* out, sp_stats* st) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  sp_status s = check_wall_call(c, in, out);
+  if (s) return s;
+  if ((s = wall_step(c, *in, *out, false))) return s;
+  return wall_finish(c, st);
+}
+
+sp_status sp_wall_step(sp_ctx* c, const sp_inputs* in, sp_outputs* out, sp_stats* st) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  sp_status s = check_wall_call(c, in, out);
+  if (s) return s;
+  const Dims& D = c->dm;
+  const size_t Nh = (size_t)D.Nh, N = (size_t)D.N;
+  // staging slots past the forward's and the adjoint's, so a later adjoint
+  // still finds the forward's inputs
+  size_t k = 60;
+  sp_inputs d{};
+  std::memcpy(d.bc_flag, in->bc_flag, sizeof(d.bc_flag));
+  d.wall = in->wall;
+  if ((s = upload(c, k++, in->ee_area, Nh, &d.ee_area)) ||
+      (s = upload(c, k++, in->ne_area, Nh, &d.ne_area)) ||
+      (s = upload(c, k++, in->te_area, Nh, &d.te_area)) ||
+      (s = upload(c, k++, in->cell_type_p, N, &d.cell_type_p)) ||
+      (s = upload(c, k++, in->cell_type_u, N, &d.cell_type_u)) ||
+      (s = upload(c, k++, in->cell_type_v, N, &d.cell_type_v)) ||
+      (s = upload(c, k++, in->cell_type_w, N, &d.cell_type_w)))
+    return s;
+  sp_outputs o{};
+  double* hs[4] = {out->u, out->v, out->w, out->p};
+  double* ds[4];
+  for (int q = 0; q < 4; ++q) {
+    const double* p;
+    if ((s = upload(c, k++, static_cast<const double*>(hs[q]), Nh, &p))) return s;
+    ds[q] = const_cast<double*>(p);
+  }
+  o.u = ds[0]; o.v = ds[1]; o.w = ds[2]; o.p = ds[3];
+  for (int q = 0; q < 24; ++q) o.bc_var[q] = out->bc_var[q];  // host memory
+  if ((s = wall_step(c, d, o, true))) return s;
+  for (int q = 0; q < 4; ++q) {
+    hipError_t e = hipMemcpyAsync(hs[q], ds[q], Nh * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(D2H)");
+  }
+  return wall_finish(c, st);
 }
 
 void* sp_stream(sp_ctx* c) { return c ? (void*)c->stream : nullptr; }

@@ -26,6 +26,7 @@ struct Scalars {
   double rmax;      // max |r_i| / V_i of the recursive residual
   double bmean;     // mean of b over active cells (singular systems)
   double bsum, esum, bmax, shift, divmax;
+  double walldiv;   // max |mres|/V after the wall step
   double dsum;      // sum of the diagonal over active cells
   double zshift;    // diagonal-weighted mean of z removed in singular systems
   double src_max;   // max |pbar_i| / V_i of the adjoint source
@@ -178,6 +179,10 @@ struct sp_ctx {
   // bc values staged
   double* bcv[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 
+  // wall step: face values of u, v, w, p (4 blocks of the six boundary planes)
+  double* wall_var = nullptr;
+  bool last_wall = false;  // the last forward applied the wall step
+
   // profiling
   bool profiling = false;
   std::string prof_filter;  // time only launches of this kernel (empty: all)
@@ -213,6 +218,10 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st, bool che
 
 // ---- correction (sp_correct.hip) ----
 sp_status correct(sp_ctx* c, const sp_inputs& in, sp_outputs& out, sp_stats* st);
+
+// ---- wall step after the correction (sp_wall.hip) ----
+// Updates out.u, v, w, p in place; `host_var`: out.bc_var are host pointers.
+sp_status wall_step(sp_ctx* c, const sp_inputs& in, const sp_outputs& out, bool host_var);
 
 // ---- adjoint (sp_adjoint.hip) ----
 sp_status adjoint(sp_ctx* c, const sp_adj_inputs& in, sp_adj_outputs& out, sp_stats* st);

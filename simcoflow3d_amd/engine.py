@@ -33,7 +33,7 @@ EXPORTED_SYMBOLS = (
     "sp_params_default", "sp_create", "sp_destroy", "sp_set_params", "sp_set_grid",
     "sp_set_grid_1d", "sp_project", "sp_project_dev", "sp_adjoint", "sp_adjoint_dev",
     "sp_stream", "sp_kernel_times", "sp_set_profiling", "sp_set_profiling_filter",
-    "sp_strerror", "sp_last_error",
+    "sp_strerror", "sp_last_error", "sp_wall_step", "sp_wall_step_dev",
 )
 
 
@@ -52,17 +52,38 @@ class SpParams(ctypes.Structure):
     ]
 
 
+class SpWallBc(ctypes.Structure):
+    """BCu/BCv/BCw flags and BCu/BCv/BCw/BCp constants of the wall step
+    (faces W,E,S,N,B,T; 0 Dirichlet, 1 Neumann)."""
+    _fields_ = [("flag_u", ctypes.c_int32 * 6), ("flag_v", ctypes.c_int32 * 6),
+                ("flag_w", ctypes.c_int32 * 6), ("const_u", ctypes.c_double * 6),
+                ("const_v", ctypes.c_double * 6), ("const_w", ctypes.c_double * 6),
+                ("const_p", ctypes.c_double * 6)]
+
+
+def make_wall(flags, consts):
+    """flags: 18 values (BCu W..T, BCv, BCw); consts: 24 (BCu, BCv, BCw, BCp)."""
+    w = SpWallBc()
+    for f in range(6):
+        w.flag_u[f], w.flag_v[f], w.flag_w[f] = (int(flags[6 * q + f]) for q in range(3))
+        w.const_u[f], w.const_v[f], w.const_w[f], w.const_p[f] = (
+            float(consts[6 * q + f]) for q in range(4))
+    return w
+
+
 class SpInputs(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in (
         "u_star", "v_star", "w_star", "dp_u", "dp_v", "dp_w", "ee_area", "ne_area", "te_area",
         "vof", "vof_old", "vof_l", "phi_l", "cell_type_p", "cell_type_u", "cell_type_v",
         "cell_type_w", "moex_u", "moex_v", "moex_w")] + [
         ("bc_flag", ctypes.c_int32 * 6), ("bc_val", ctypes.c_void_p * 6), ("dt", ctypes.c_double),
-        ("beta_e", ctypes.c_void_p), ("beta_n", ctypes.c_void_p), ("beta_t", ctypes.c_void_p)]
+        ("beta_e", ctypes.c_void_p), ("beta_n", ctypes.c_void_p), ("beta_t", ctypes.c_void_p),
+        ("wall", ctypes.c_void_p)]
 
 
 class SpOutputs(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p) for n in ("p", "u", "v", "w", "po_coef")]
+    _fields_ = [(n, ctypes.c_void_p) for n in ("p", "u", "v", "w", "po_coef")] + [
+        ("bc_var", ctypes.c_void_p * 24)]
 
 
 class SpStats(ctypes.Structure):
@@ -73,6 +94,7 @@ class SpStats(ctypes.Structure):
         ("final_div", ctypes.c_double), ("residual_div", ctypes.c_double),
         ("t_assemble_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double),
         ("t_correct_ms", ctypes.c_double), ("t_total_ms", ctypes.c_double),
+        ("wall_div", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -131,6 +153,10 @@ def load():
     for name in ("sp_adjoint", "sp_adjoint_dev"):
         f = getattr(L, name)
         f.argtypes = [V, P(SpAdjInputs), P(SpAdjOutputs), P(SpStats)]
+        f.restype = ctypes.c_int
+    for name in ("sp_wall_step", "sp_wall_step_dev"):
+        f = getattr(L, name)
+        f.argtypes = [V, P(SpInputs), P(SpOutputs), P(SpStats)]
         f.restype = ctypes.c_int
     L.sp_stream.argtypes = [V]
     L.sp_stream.restype = V
@@ -215,8 +241,9 @@ class Projector:
                     "sp_set_grid")
 
     @staticmethod
-    def make_inputs(f, bc_flag, dt, bc_val=None, beta=None):
-        """f: mapping with the sp_inputs field names (numpy arrays or torch tensors)."""
+    def make_inputs(f, bc_flag, dt, bc_val=None, beta=None, wall=None):
+        """f: mapping with the sp_inputs field names (numpy arrays or torch tensors);
+        wall: an SpWallBc (kept alive by the caller) or None."""
         inp = SpInputs()
         for n in ("u_star", "v_star", "w_star", "dp_u", "dp_v", "dp_w", "ee_area", "ne_area",
                   "te_area", "vof", "vof_old", "vof_l", "phi_l", "cell_type_p", "cell_type_u",
@@ -228,18 +255,32 @@ class Projector:
         inp.dt = dt
         if beta is not None:
             inp.beta_e, inp.beta_n, inp.beta_t = (_addr(b) for b in beta)
+        if wall is not None:
+            inp.wall = ctypes.addressof(wall)
         return inp
 
-    def project(self, fields, bc_flag, dt, out=None, bc_val=None, beta=None, po_coef=False):
-        """Host-array projection (sp_project).  Returns (outputs dict, stats)."""
+    def _bc_var_arrays(self):
+        I, J, K = self.shape
+        shapes = [(J, K), (J, K), (I, K), (I, K), (I, J), (I, J)]
+        return [np.zeros(shapes[q % 6], order="F") for q in range(24)]
+
+    def project(self, fields, bc_flag, dt, out=None, bc_val=None, beta=None, po_coef=False,
+                wall=None):
+        """Host-array projection (sp_project).  Returns (outputs dict, stats).
+        With ``wall`` (SpWallBc) the wall step runs after the correction and
+        out["bc_var"] holds the 24 face values."""
         I, J, K = self.shape
         hs = (I + 2, J + 2, K + 2)
-        inp = self.make_inputs(fields, bc_flag, dt, bc_val, beta)
+        inp = self.make_inputs(fields, bc_flag, dt, bc_val, beta, wall)
         if out is None:
             out = {n: np.zeros(hs, order="F") for n in ("p", "u", "v", "w")}
         if po_coef:
             out["po_coef"] = np.zeros((I, J, K, 6), order="F")
         o = SpOutputs(*(_addr(out.get(n)) for n in ("p", "u", "v", "w", "po_coef")))
+        if wall is not None:
+            out["bc_var"] = self._bc_var_arrays()
+            for q in range(24):
+                o.bc_var[q] = _addr(out["bc_var"][q])
         st = SpStats()
         rc = self.lib.sp_project(self.h, ctypes.byref(inp), ctypes.byref(o), ctypes.byref(st))
         self._check(rc, "sp_project", allow=(SP_ERR_NOT_CONVERGED,))
@@ -253,6 +294,21 @@ class Projector:
         rc = self.lib.sp_project_dev(self.h, ctypes.byref(inp), ctypes.byref(o), ctypes.byref(st))
         self._check(rc, "sp_project_dev", allow=(SP_ERR_NOT_CONVERGED,))
         return st
+
+    def wall_step(self, fields, bc_flag, wall, u, v, w, p):
+        """The wall step alone (sp_wall_step) on copies of the host arrays
+        u, v, w, p.  Returns (dict u, v, w, p, bc_var; stats)."""
+        inp = self.make_inputs(fields, bc_flag, 1.0, wall=wall)
+        out = {n: np.array(a, dtype=np.float64, order="F", copy=True)
+               for n, a in (("u", u), ("v", v), ("w", w), ("p", p))}
+        out["bc_var"] = self._bc_var_arrays()
+        o = SpOutputs(*(_addr(out.get(n)) for n in ("p", "u", "v", "w", "po_coef")))
+        for q in range(24):
+            o.bc_var[q] = _addr(out["bc_var"][q])
+        st = SpStats()
+        rc = self.lib.sp_wall_step(self.h, ctypes.byref(inp), ctypes.byref(o), ctypes.byref(st))
+        self._check(rc, "sp_wall_step")
+        return out, st
 
     def adjoint(self, gu, gv, gw, beta_grad=False):
         I, J, K = self.shape

@@ -10,12 +10,19 @@ program sp_layout
   type(sp_stats), target :: st
   type(sp_adj_inputs), target :: ai
   type(sp_adj_outputs), target :: ao
+  type(sp_wall_bc), target :: wb
   print '(a,1x,i0)', 'sp_params', c_sizeof(prm)
   print '(a,1x,i0)', 'sp_inputs', c_sizeof(inp)
   print '(a,1x,i0)', 'sp_outputs', c_sizeof(outp)
   print '(a,1x,i0)', 'sp_stats', c_sizeof(st)
   print '(a,1x,i0)', 'sp_adj_inputs', c_sizeof(ai)
   print '(a,1x,i0)', 'sp_adj_outputs', c_sizeof(ao)
+  print '(a,1x,i0)', 'sp_wall_bc', c_sizeof(wb)
+  print '(a,1x,i0)', 'sp_wall_bc.const_u', off(c_loc(wb%const_u), c_loc(wb))
+  print '(a,1x,i0)', 'sp_wall_bc.const_p', off(c_loc(wb%const_p), c_loc(wb))
+  print '(a,1x,i0)', 'sp_inputs.wall', off(c_loc(inp%wall), c_loc(inp))
+  print '(a,1x,i0)', 'sp_outputs.bc_var', off(c_loc(outp%bc_var), c_loc(outp))
+  print '(a,1x,i0)', 'sp_stats.wall_div', off(c_loc(st%wall_div), c_loc(st))
   print '(a,1x,i0)', 'sp_params.mg_coarse_scale', off(c_loc(prm%mg_coarse_scale), c_loc(prm))
   print '(a,1x,i0)', 'sp_params.max_iter', off(c_loc(prm%max_iter), c_loc(prm))
   print '(a,1x,i0)', 'sp_params.div_tol', off(c_loc(prm%div_tol), c_loc(prm))

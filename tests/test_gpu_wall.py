@@ -1,0 +1,113 @@
+"""GPU parity of the wall step that follows the correction (sp_wall_step,
+sp_inputs.wall): BoundaryConditionVarNew (ComputePUVW.f90:418-694), the mass
+fix of the first Dirichlet pressure face (:256-384) and
+VariablesInternalCellCondition (Solver.f90:467-501), against the oracle on
+the same inputs.  Bar: bit-identical u, v, w, p (every halo entry) and face
+values; NaN where the reference divides by a zero aperture."""
+
+import numpy as np
+import pytest
+
+from oracle import orc
+from simcoflow3d_amd import configs
+from simcoflow3d_amd.engine import SP_ERR_STATE, SpError, make_wall
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+PFLAGS = [(0, 1, 1, 1, 1, 1), (1, 0, 1, 1, 1, 1), (1, 1, 0, 1, 1, 1), (1, 1, 1, 0, 1, 1),
+          (1, 1, 1, 1, 0, 1), (1, 1, 1, 1, 1, 0), (1, 1, 1, 1, 1, 1)]
+
+
+def random_bc(seed, pflags):
+    rng = np.random.default_rng(1000 + seed)
+    fl = rng.integers(0, 2, 24).astype(np.int32)
+    fl[18:24] = pflags
+    return fl, rng.uniform(-2.0, 2.0, 24)
+
+
+def same(a, b):
+    return np.array_equal(a, b, equal_nan=True)
+
+
+@pytest.mark.parametrize("pflags", PFLAGS)
+@pytest.mark.parametrize("cfg,size,seed", [(0, (17, 9, 6), 0), (1, (6, 13, 8), 1),
+                                           (4, (40, 6, 7), 2), (3, (2, 3, 2), 3)])
+def test_wall_step_matches_oracle(gpu_available, cfg, size, seed, pflags):
+    c = configs.generate(cfg, size=size, seed=seed)
+    print(f"[wall] cfg={cfg} size={size} pflags={pflags}", flush=True)
+    rng = np.random.default_rng(seed)
+    u, v, w, p = (np.asfortranarray(rng.uniform(-1.0, 1.0, c.halo_shape)) for _ in range(4))
+    fl, cs = random_bc(seed, pflags)
+    exp = orc.wall_step(c, fl, cs, u, v, w, p)
+    P = H.projector(c)
+    bcf = np.array(pflags, np.int32)
+    wall = make_wall(fl[:18], cs)
+    out, st = P.wall_step(H.fields(c), bcf, wall, u, v, w, p)
+    P.close()
+    for n, e in zip("uvwp", exp[:4]):
+        assert same(out[n], e), n
+    for q in range(24):
+        assert same(out["bc_var"][q], exp[4][q]), q
+    if all(np.isfinite(a).all() for a in exp[:3]):
+        _, m = orc.mres(c, *exp[:3])
+        assert st.wall_div == m
+
+
+@pytest.mark.parametrize("cfg,prec", [(0, 1), (1, 0), (4, 1)])
+def test_projection_with_wall_step(gpu_available, cfg, prec):
+    """sp_project with sp_inputs.wall = the oracle's wall step applied to the
+    engine's own projection without it (the projection is deterministic)."""
+    c = H.case(cfg)
+    print(f"[wall-project] cfg={cfg} prec={prec}", flush=True)
+    fl, cs = random_bc(10 + cfg, (1, 1, 1, 1, 1, 1))
+    hs = c.halo_shape
+    rng = np.random.default_rng(cfg)
+    init = {n: np.asfortranarray(rng.uniform(-1, 1, hs)) for n in ("u", "v", "w")}
+    P = H.projector(c, precision=prec)
+    out0 = {n: a.copy(order="F") for n, a in init.items()}
+    out0["p"] = np.zeros(hs, order="F")
+    out0, st0 = P.project(H.fields(c), c.bc_flag, c.dt, out=out0)
+    out1 = {n: a.copy(order="F") for n, a in init.items()}
+    out1["p"] = np.zeros(hs, order="F")
+    wall = make_wall(fl[:18], cs)
+    out1, st1 = P.project(H.fields(c), c.bc_flag, c.dt, out=out1, wall=wall)
+    # the adjoint covers the projection without the wall step
+    with pytest.raises(SpError) as ei:
+        P.adjoint(*(np.zeros(hs, order="F") for _ in range(3)))
+    assert ei.value.status == SP_ERR_STATE
+    P.close()
+    assert st0.converged == 1 and st1.converged == 1
+    assert st0.iterations == st1.iterations and st1.final_div == st0.final_div
+    fl[18:24] = c.bc_flag
+    exp = orc.wall_step(c, fl, cs, out0["u"], out0["v"], out0["w"], out0["p"])
+    for n, e in zip("uvwp", exp[:4]):
+        assert same(out1[n], e), n
+    for q in range(24):
+        assert same(out1["bc_var"][q], exp[4][q]), q
+    _, m = orc.mres(c, *exp[:3])
+    assert st1.wall_div == m
+
+
+def test_inflow_outflow_step_is_divergence_free(gpu_available):
+    """The reference drivers' boundary conditions (Main.f90:92-105): pressure
+    Dirichlet at the east outlet, u = 1 inflow at the west, u Neumann at the
+    outlet, no-slip elsewhere.  After the projection and the wall step (reset,
+    mass fix at the outlet, solid cells) the field is divergence free."""
+    c = H.case(0)
+    I, J, K = c.shape
+    c.bc_flag[:] = (1, 0, 1, 1, 1, 1)
+    c.fields["u_star"][0, 1:-1, 1:-1] = 1.0     # inflow face as the predictor sets it
+    bcv = [None, np.zeros((J, K), order="F"), None, None, None, None]
+    flags = [0, 1, 0, 0, 0, 0] * 3
+    consts = [1.0, 0, 0, 0, 0, 0] + [0.0] * 18
+    wall = make_wall(flags, consts)
+    print("[wall-inflow] cfg=0", flush=True)
+    P = H.projector(c, precision=1)
+    out, st = P.project(H.fields(c), c.bc_flag, c.dt, bc_val=bcv, wall=wall)
+    P.close()
+    assert st.converged == 1, st.as_dict()
+    assert st.wall_div <= 1e-10, st.as_dict()
+    assert np.all(out["u"][0, 1:-1, 1:-1] == 1.0)
+    assert np.all(out["v"][1:-1, 0, 1:-1] == 0.0) and np.all(out["v"][1:-1, J, 1:-1] == 0.0)
+    assert np.all(out["w"][1:-1, 1:-1, 0] == 0.0) and np.all(out["w"][1:-1, 1:-1, K] == 0.0)
