@@ -232,11 +232,17 @@ class Projector:
 
     def set_grid_1d(self, dxp, dyp, dzp, dxu, dyv, dzw):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (dxp, dyp, dzp, dxu, dyv, dzw)]
+        I, J, K = self.shape
+        if [a.size for a in arrs] != [I, J, K, I, J, K]:
+            raise ValueError("sp_set_grid_1d takes spacings of lengths Imax, Jmax, Kmax")
         self._check(self.lib.sp_set_grid_1d(self.h, *[a.ctypes.data_as(F64P) for a in arrs]),
                     "sp_set_grid_1d")
 
     def set_grid(self, dxp, dyp, dzp, dxu, dyv, dzw):
         arrs = [np.asfortranarray(a, dtype=np.float64) for a in (dxp, dyp, dzp, dxu, dyv, dzw)]
+        n = self.shape[0] * self.shape[1] * self.shape[2]
+        if any(a.size != n for a in arrs):
+            raise ValueError("sp_set_grid takes (Imax, Jmax, Kmax) arrays; use set_grid_1d for spacings")
         self._check(self.lib.sp_set_grid(self.h, *[a.ctypes.data_as(F64P) for a in arrs]),
                     "sp_set_grid")
 
@@ -286,10 +292,15 @@ class Projector:
         self._check(rc, "sp_project", allow=(SP_ERR_NOT_CONVERGED,))
         return out, st
 
-    def project_dev(self, fields, bc_flag, dt, out, bc_val=None, beta=None):
-        """Device-array projection (sp_project_dev) on torch tensors."""
-        inp = self.make_inputs(fields, bc_flag, dt, bc_val, beta)
+    def project_dev(self, fields, bc_flag, dt, out, bc_val=None, beta=None, wall=None):
+        """Device-array projection (sp_project_dev) on torch tensors; with
+        ``wall`` (SpWallBc) the wall step follows, and out["bc_var"] (a list
+        of 24 device tensors or None entries), when present, receives the
+        face values."""
+        inp = self.make_inputs(fields, bc_flag, dt, bc_val, beta, wall)
         o = SpOutputs(*(_addr(out.get(n)) for n in ("p", "u", "v", "w", "po_coef")))
+        for q, a in enumerate(out.get("bc_var") or ()):
+            o.bc_var[q] = _addr(a)
         st = SpStats()
         rc = self.lib.sp_project_dev(self.h, ctypes.byref(inp), ctypes.byref(o), ctypes.byref(st))
         self._check(rc, "sp_project_dev", allow=(SP_ERR_NOT_CONVERGED,))

@@ -6,6 +6,10 @@
   old solid vof, variables, predictor, d-coefficients, pressure BC,
   PoCoef, projection, dt), same outputs written in place (PoCoef, Proj%Pp,
   TVar%u/v/w/p).
+* :func:`boundary_condition_var_new` follows the lines after the correction
+  in ``UpdatePUVW``: ``BoundaryConditionVarNew`` (ComputePUVW.f90:418-694),
+  the mass fix of the first Dirichlet pressure face (:256-384) and
+  ``VariablesInternalCellCondition`` (Solver.f90:467-501), on the engine.
 * :class:`DeviceProjection` runs the projection on torch tensors resident on
   the GPU and keeps the forward pressure for the adjoint.
 * :class:`ProjectionFunction` is the ``torch.autograd.Function`` whose
@@ -71,6 +75,7 @@ class PoissonCoefficient:  # PredictUVW.f90:39-41
 @dataclasses.dataclass
 class BCBase:  # BoundaryInterface.f90:7-21
     flag: np.ndarray  # W,E,S,N,B,T: 0 Dirichlet, 1 Neumann
+    Const: Optional[np.ndarray] = None  # %Const(1..6) of TypicalBC
     VarW: Optional[np.ndarray] = None
     VarE: Optional[np.ndarray] = None
     VarS: Optional[np.ndarray] = None
@@ -130,6 +135,37 @@ def poisson_equation_solver(PGrid, UGrid, VGrid, WGrid, PCell, UCell, VCell, WCe
                         po_coef=True)
     PoCoef[...] = res["po_coef"]
     TVar.p[1:-1, 1:-1, 1:-1] = Proj.Pp[1:-1, 1:-1, 1:-1]
+    return st
+
+
+def boundary_condition_var_new(PGrid, PCell, UCell, VCell, WCell, TVar, BCp, BCu, BCv, BCw):
+    """BoundaryConditionVarNew + the Dirichlet-pressure mass fix +
+    VariablesInternalCellCondition on the engine (sp_wall_step), with the
+    reference's arguments.  TVar.u/v/w/p are updated in place; BCu/BCv/BCw/BCp
+    VarW..VarT receive the new face values.  Returns the engine statistics
+    (wall_div = max |mres|/V of the result, Solver.f90:236-250)."""
+    I, J, K = PCell.Cell_Type.shape
+    for a in (PGrid.dx, PGrid.dy, PGrid.dz):
+        if np.shape(a) != (I, J, K):
+            raise ProjectionError("PGrid%dx, dy, dz must be (Imax, Jmax, Kmax) arrays")
+    P = _projector((I, J, K), dict(pin_i=0, pin_j=0, pin_k=0))
+    # the wall step reads only the P-cell spacings; the staggered ones of
+    # this context are never used
+    P.set_grid(PGrid.dx, PGrid.dy, PGrid.dz, PGrid.dx, PGrid.dy, PGrid.dz)
+    f = dict(ee_area=PCell.EEArea, ne_area=PCell.NEArea, te_area=PCell.TEArea,
+             cell_type_p=PCell.Cell_Type, cell_type_u=UCell.Cell_Type,
+             cell_type_v=VCell.Cell_Type, cell_type_w=WCell.Cell_Type)
+    f = {k: np.asfortranarray(v) for k, v in f.items()}
+    flags = np.concatenate([np.asarray(b.flag, dtype=np.int32) for b in (BCu, BCv, BCw)])
+    consts = np.concatenate([np.asarray(b.Const, dtype=np.float64) for b in (BCu, BCv, BCw, BCp)])
+    wall = _eng.make_wall(flags, consts)
+    res, st = P.wall_step(f, np.asarray(BCp.flag, dtype=np.int32), wall, TVar.u, TVar.v, TVar.w,
+                          TVar.p)
+    for n in ("u", "v", "w", "p"):
+        getattr(TVar, n)[...] = res[n]
+    for q, b in enumerate((BCu, BCv, BCw, BCp)):
+        for f_, name in enumerate(("VarW", "VarE", "VarS", "VarN", "VarB", "VarT")):
+            setattr(b, name, res["bc_var"][6 * q + f_])
     return st
 
 

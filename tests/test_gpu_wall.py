@@ -111,3 +111,31 @@ def test_inflow_outflow_step_is_divergence_free(gpu_available):
     assert np.all(out["u"][0, 1:-1, 1:-1] == 1.0)
     assert np.all(out["v"][1:-1, 0, 1:-1] == 0.0) and np.all(out["v"][1:-1, J, 1:-1] == 0.0)
     assert np.all(out["w"][1:-1, 1:-1, 0] == 0.0) and np.all(out["w"][1:-1, 1:-1, K] == 0.0)
+
+
+def test_reference_interface_mirror(gpu_available):
+    """boundary_condition_var_new with the reference's argument list gives
+    the oracle's wall step and fills BCx%VarW..VarT."""
+    from simcoflow3d_amd import projection as pj
+    c = configs.generate(1, size=(10, 12, 9), seed=5)
+    rng = np.random.default_rng(5)
+    u, v, w, p = (np.asfortranarray(rng.uniform(-1.0, 1.0, c.halo_shape)) for _ in range(4))
+    fl, cs = random_bc(5, (1, 0, 1, 1, 1, 1))
+    exp = orc.wall_step(c, fl, cs, u, v, w, p)
+    ones = np.ones(c.shape)
+    grid = pj.Grid(np.asfortranarray(c.dxp[:, None, None] * ones),
+                   np.asfortranarray(c.dyp[None, :, None] * ones),
+                   np.asfortranarray(c.dzp[None, None, :] * ones))
+    PCell = pj.Cell(c.cell_type_p, EEArea=c.ee_area, NEArea=c.ne_area, TEArea=c.te_area)
+    UCell, VCell, WCell = (pj.Cell(getattr(c, "cell_type_" + n)) for n in "uvw")
+    TVar = pj.Variables(u.copy(order="F"), v.copy(order="F"), w.copy(order="F"), p.copy(order="F"))
+    BC = [pj.BCBase(fl[6 * q:6 * q + 6], Const=cs[6 * q:6 * q + 6]) for q in range(4)]
+    print("[wall-mirror]", flush=True)
+    st = pj.boundary_condition_var_new(grid, PCell, UCell, VCell, WCell, TVar, BC[3], BC[0],
+                                       BC[1], BC[2])
+    for n, e in zip("uvwp", exp[:4]):
+        assert same(getattr(TVar, n), e), n
+    for q, b in enumerate((BC[0], BC[1], BC[2], BC[3])):
+        for f, name in enumerate(("VarW", "VarE", "VarS", "VarN", "VarB", "VarT")):
+            assert same(getattr(b, name), exp[4][6 * q + f]), (q, name)
+    assert st.wall_div >= 0.0
