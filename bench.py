@@ -10,6 +10,8 @@ way in the same run.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--no-cpu]
     python bench.py --all-configs     # configs 0-4 at full size, mixed and FP64,
                                       # plus a forward+adjoint pass (one JSON line each)
+    python bench.py --config 4 --adjoint   # adds a timed forward+adjoint pass and its
+                                           # per-kernel GB/s and fraction of HBM peak
 """
 
 import argparse
@@ -47,6 +49,9 @@ def parse():
     ap.add_argument("--all-configs", action="store_true",
                     help="configs 0-4 at full size: mixed and FP64 solves/s, the mixed "
                          "vs FP64 pressure error and a timed forward+adjoint pass")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time forward + adjoint passes of the mixed solver and report the "
+                         "per-kernel GB/s of one profiled pair")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 1 GiB device-to-device copy (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -178,6 +183,62 @@ def face_masks(case):
     return ms
 
 
+def adjoint_buffers(case, out):
+    """Cotangents U(0,1) on the corrected faces (a source with a large net
+    flux) and the gradient outputs, as device tensors."""
+    import torch
+    I, J, K = case.shape
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    g = {n: m * torch.rand(m.shape, generator=gen, device="cuda", dtype=torch.float64)
+         for n, m in zip(("gu", "gv", "gw"), face_masks(case))}
+    hs = out["p"].shape
+    res = {n: torch.zeros(hs, dtype=torch.float64, device="cuda")
+           for n in ("gu_star", "gv_star", "gw_star")}
+    for n in ("gk_e", "gk_n", "gk_t"):
+        res[n] = torch.zeros((K, J, I), dtype=torch.float64, device="cuda")
+    return g, res
+
+
+def forward_adjoint(P, dev, out, case, steps):
+    """`steps` timed forward + adjoint pairs, then one profiled pair whose
+    engine HIP-event timers give each kernel's device time, algorithmic
+    GB/s and fraction of the HBM peak."""
+    import torch
+    g, res = adjoint_buffers(case, out)
+    P.project_dev(dev, case.bc_flag, case.dt, out)
+    P.adjoint_dev(g, res)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = P.project_dev(dev, case.bc_flag, case.dt, out)
+        ast = P.adjoint_dev(g, res)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    P.set_profiling(True)
+    P.project_dev(dev, case.bc_flag, case.dt, out)
+    P.adjoint_dev(g, res)
+    rows = P.kernel_times()
+    P.set_profiling(False)
+    # Each of the two solves enqueues one CG iteration past convergence whose
+    # kernels return at once; kernels launched a whole number of times per
+    # iteration count those no-op launches' bytes, which are taken out here.
+    work = st.iterations + ast.iterations
+    table = []
+    for r in sorted(rows, key=lambda r: -r["total_ms"]):
+        if not r["launches"]:
+            continue
+        nbytes = r["bytes"]
+        if r["launches"] % (work + 2) == 0:
+            nbytes *= work / (work + 2)
+        gbs = nbytes / (r["total_ms"] * 1e-3) / 1e9 if nbytes and r["total_ms"] > 0 else 0.0
+        table.append(dict(name=r["name"], launches=r["launches"], total_ms=round(r["total_ms"], 3),
+                          avg_us=round(r["total_ms"] * 1e3 / r["launches"], 2),
+                          gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4)))
+    return dict(pairs_per_s=round(steps / el, 3), ms_per_pair=round(el / steps * 1e3, 3),
+                forward_iterations=st.iterations, adjoint_iterations=ast.iterations,
+                adjoint_converged=bool(ast.converged), kernels=table)
+
+
 def all_configs(args):
     """Configs 0-4 at full size: one JSON line each."""
     import torch
@@ -207,14 +268,7 @@ def all_configs(args):
                              iterations=s.iterations, final_div=s.final_div,
                              converged=bool(s.converged))
             # one forward + adjoint pass (reverse mode of the same projection)
-            gen = torch.Generator(device="cuda").manual_seed(11)
-            g = {n: m * torch.rand(m.shape, generator=gen, device="cuda", dtype=torch.float64)
-                 for n, m in zip(("gu", "gv", "gw"), face_masks(case))}
-            hs = out["p"].shape
-            res = {n: torch.zeros(hs, dtype=torch.float64, device="cuda")
-                   for n in ("gu_star", "gv_star", "gw_star")}
-            for n in ("gk_e", "gk_n", "gk_t"):
-                res[n] = torch.zeros((K, J, I), dtype=torch.float64, device="cuda")
+            g, res = adjoint_buffers(case, out)
             P.adjoint_dev(g, res)  # warm
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -285,6 +339,7 @@ def main():
     ms_per_step = el / args.steps * 1e3
     median_ms = median([st.t_total_ms for st in st_m])
     host = None if args.no_host else host_arrays(Pm, case)
+    fwd_adj = forward_adjoint(Pm, dev, out, case, max(1, args.steps // 2)) if args.adjoint else None
     Pm.close()
 
     fp64 = {}
@@ -360,6 +415,7 @@ def main():
                    "converged": bool(sm.converged)},
         "fp64": fp64,
         "host_arrays": host,
+        "forward_adjoint": fwd_adj,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": [dict(name=k["name"], launches=k["launches"], total_ms=round(k["total_ms"], 3),
