@@ -137,7 +137,8 @@ static sp_status ensure_buffers(sp_ctx* c) {
       (s = dmalloc(c, &c->cz, N)) || (s = dmalloc(c, &c->ee, N)) || (s = dmalloc(c, &c->b, N)) ||
       (s = dmalloc(c, &c->active, N)) || (s = dmalloc(c, &c->dw, N)) || (s = dmalloc(c, &c->x, N)) || (s = dmalloc(c, &c->r, N)) ||
       (s = dmalloc(c, &c->q, N)) || (s = dmalloc(c, &c->p0, N)) || (s = dmalloc(c, &c->p1, N)) ||
-      (s = dmalloc(c, &c->pbar, N)) || (s = dmalloc(c, &c->x_ckpt, N)))
+      (s = dmalloc(c, &c->pbar, N)) || (s = dmalloc(c, &c->x_ckpt, N)) ||
+      (s = dmalloc(c, &c->xd, N)) || (s = dmalloc(c, &c->xe, N)))
     return s;
   const bool mixed_mg = prec == SP_PREC_MIXED && pc == SP_PRECOND_MG;
   if (mixed_mg) {
@@ -268,6 +269,7 @@ static sp_status project_device(sp_ctx* c, const sp_inputs* in, sp_outputs* out,
     if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(bc)");
   }
   c->have_forward = false;
+  c->xd_active = false;
   int64_t stale = 0;
   if ((s = assemble(c, *in, &stale))) return s;
   e = hipMemcpyAsync(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream);
@@ -307,6 +309,22 @@ static sp_status project_device(sp_ctx* c, const sp_inputs* in, sp_outputs* out,
     if (c->S_host->divmax <= c->prm.div_tol || c->S_host->nonfinite || restart >= 3 ||
         c->S_host->iter >= c->prm.max_iter)
       break;
+    if (!c->S_host->singular) {
+      // Non-singular systems (a Dirichlet face fixes the pressure level):
+      // refine the pressure as x + xd, so that the corrected velocity is
+      // not limited by the rounding of that level
+      double rmax = 0.0;
+      int it_ref = 0;
+      const int32_t it_before = S->iterations;
+      s = refine(c, c->prm.div_tol, &rmax, &it_ref);
+      if (s != SP_OK && s != SP_ERR_NOT_CONVERGED) return s;
+      S->iterations = it_before + it_ref;
+      if (!(rmax > c->prm.div_tol)) {
+        res_converged = true;  // the residual is below tol: the inputs carry the rest
+        break;
+      }
+      continue;
+    }
     const int32_t it_before = c->S_host->iter;
     s = solve(c, c->b, c->prm.div_tol, S, false, true);
     if (s != SP_OK && s != SP_ERR_NOT_CONVERGED) return s;
@@ -324,7 +342,9 @@ static sp_status project_device(sp_ctx* c, const sp_inputs* in, sp_outputs* out,
   // test above stays on the projection's own output)
   if (in->wall && (s = wall_step(c, *in, *out, host_bc))) return s;
   // checkpoint of the forward pressure for the adjoint: the iterate buffer
-  // becomes the checkpoint and the old checkpoint the next iterate
+  // (with a refinement folded in) becomes the checkpoint and the old
+  // checkpoint the next iterate
+  if (c->xd_active && (s = fold_refinement(c))) return s;
   std::swap(c->x, c->x_ckpt);
   e = hipMemcpyAsync(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

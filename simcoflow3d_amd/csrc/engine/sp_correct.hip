@@ -20,6 +20,7 @@ struct CorrArgs {
   int I, J, K, SX, SY;
   int64_t N;
   const double* x;
+  const double* xd;  // refinement of x (non-singular systems), or nullptr
   const uint8_t* active;
   const double* pocoef;
   const int32_t *ctu, *ctv, *ctw, *mu, *mv, *mw;
@@ -62,7 +63,8 @@ __global__ void __launch_bounds__(kFinThreads) k_shift_fin(CorrArgs A, uint32_t 
 }
 
 __device__ inline double pval(const CorrArgs& A, int64_t n, double shift) {
-  return A.active[n] ? A.x[n] - shift : 0.0;
+  if (!A.active[n]) return 0.0;
+  return A.xd ? (A.x[n] - shift) + A.xd[n] : A.x[n] - shift;
 }
 
 // Correction of the faces on the + side of every cell, and the interior of
@@ -85,7 +87,8 @@ __global__ void __launch_bounds__(256) k_correct(CorrArgs A) {
     const bool aP = A.active[n] != 0;
     const double pP = pval(A, n, shift);
     auto dpres = [&](bool inside, int64_t m, int fl) {
-      if (inside && aP && A.active[m]) return A.x[m] - A.x[n];
+      if (inside && aP && A.active[m])
+        return A.xd ? (A.x[m] - A.x[n]) + (A.xd[m] - A.xd[n]) : A.x[m] - A.x[n];
       const double pN = inside ? pval(A, m, shift) : (double)fl * pP;
       return pN - pP;
     };
@@ -185,6 +188,7 @@ sp_status correct(sp_ctx* c, const sp_inputs& in, sp_outputs& out, sp_stats* st)
   A.I = D.I; A.J = D.J; A.K = D.K; A.SX = D.SX; A.SY = D.SY; A.N = D.N;
   const dim3 fb(64, 4, 1);
   A.x = c->x; A.active = c->active; A.pocoef = c->pocoef;
+  A.xd = c->xd_active ? c->xd : nullptr;
   A.ctu = in.cell_type_u; A.ctv = in.cell_type_v; A.ctw = in.cell_type_w;
   A.mu = in.moex_u; A.mv = in.moex_v; A.mw = in.moex_w;
   for (int f = 0; f < 6; ++f) A.fl[f] = in.bc_flag[f];

@@ -138,6 +138,10 @@ struct sp_ctx {
 
   // CG vectors (FP64)
   double *x = nullptr, *r = nullptr, *q = nullptr, *p0 = nullptr, *p1 = nullptr, *z64 = nullptr;
+  // iterative refinement of non-singular systems: pressure = x + xd, xe the
+  // increment being solved for, p1 its right-hand side
+  double *xd = nullptr, *xe = nullptr;
+  bool xd_active = false;
   float *r32 = nullptr, *z32 = nullptr;
 
   // multigrid hierarchies (only one is populated, by precision)
@@ -215,6 +219,8 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_count);
 sp_status mg_setup(sp_ctx* c);
 sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st, bool check_true,
                 bool resume);
+sp_status refine(sp_ctx* c, double tol, double* rmax, int* iters);
+sp_status fold_refinement(sp_ctx* c);
 
 // ---- correction (sp_correct.hip) ----
 sp_status correct(sp_ctx* c, const sp_inputs& in, sp_outputs& out, sp_stats* st);

@@ -458,22 +458,59 @@ __device__ inline double zval(const CgArgs& A, int64_t n) {
 
 __device__ inline double apply_row(const CgArgs& A, const double* v, int a, int b, int c,
                                    int64_t n) {
+  // (A v)_n in difference form: sum of c_f (v_n - v_nb) plus the
+  // diagonal-only terms.  Neighbouring values of an iterate with a large
+  // level (a pressure fixed by a Dirichlet face) are close, so their
+  // differences are exact and the result is accurate to the size of the
+  // fluxes rather than to that of the level.
   const int64_t IJ = (int64_t)A.I * A.J;
-  double d = a > 0 ? A.cx[n - 1] : 0.0;
-  d = d + (b > 0 ? A.cy[n - A.I] : 0.0);
-  d = d + (c > 0 ? A.cz[n - IJ] : 0.0);
-  d = d + A.cx[n];
-  d = d + A.cy[n];
-  d = d + A.cz[n];
-  if (A.e) d = d + A.e[n];
-  double s = d * v[n];
-  if (a > 0) s -= A.cx[n - 1] * v[n - 1];
-  if (b > 0) s -= A.cy[n - A.I] * v[n - A.I];
-  if (c > 0) s -= A.cz[n - IJ] * v[n - IJ];
-  if (a < A.I - 1) s -= A.cx[n] * v[n + 1];
-  if (b < A.J - 1) s -= A.cy[n] * v[n + A.I];
-  if (c < A.K - 1) s -= A.cz[n] * v[n + IJ];
+  const double vn = v[n];
+  double s = A.e ? A.e[n] * vn : 0.0;
+  if (a > 0) s += A.cx[n - 1] * (vn - v[n - 1]);
+  if (b > 0) s += A.cy[n - A.I] * (vn - v[n - A.I]);
+  if (c > 0) s += A.cz[n - IJ] * (vn - v[n - IJ]);
+  s += a < A.I - 1 ? A.cx[n] * (vn - v[n + 1]) : A.cx[n] * vn;
+  s += b < A.J - 1 ? A.cy[n] * (vn - v[n + A.I]) : A.cy[n] * vn;
+  s += c < A.K - 1 ? A.cz[n] * (vn - v[n + IJ]) : A.cz[n] * vn;
   return s;
+}
+
+// Residual of the refined pressure x + xd of a non-singular system:
+// rt = b - A x - A xd on active cells, in difference form; max |rt|/V.
+__global__ void __launch_bounds__(256) k_refine_residual(CgArgs A, const double* xd, double* rt) {
+  const Col3 q = col_of(A.I, A.J, A.K);
+  double v[1] = {0.0};
+  if (q.in) {
+    const int64_t IJ = (int64_t)A.I * A.J;
+    const double dxy = A.dxp[q.a] * A.dyp[q.b];
+    int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c0);
+    for (int c = q.c0; c < q.c1; ++c, n += IJ) {
+      double rv = 0.0;
+      if (A.active[n]) {
+        rv = A.b[n] - apply_row(A, A.x, q.a, q.b, c, n);
+        rv -= apply_row(A, xd, q.a, q.b, c, n);
+        const double rr = fabs(rv) / (dxy * A.dzp[c]);
+        v[0] = (rr > v[0] || rr != rr) ? rr : v[0];
+      }
+      rt[n] = rv;
+    }
+  }
+  block_partials<kTileThreads, 1, 1u>(A.site, v);
+}
+
+__global__ void __launch_bounds__(kFinThreads) k_refine_residual_fin(CgArgs A, uint32_t nb) {
+  double out[1];
+  finalize_partials<kFinThreads, 1, 1u>(A.site, nb, out);
+  if (flat_tid() == 0) A.S->npart[0] = out[0];
+}
+
+// y += x
+__global__ void __launch_bounds__(256) k_refine_add(CgArgs A, const double* x, double* y) {
+  const Col3 q = col_of(A.I, A.J, A.K);
+  if (!q.in) return;
+  const int64_t IJ = (int64_t)A.I * A.J;
+  int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c0);
+  for (int c = q.c0; c < q.c1; ++c, n += IJ) y[n] += x[n];
 }
 
 // Jacobi preconditioner (SP_PRECOND_JACOBI): z = r / diagonal on active cells
@@ -1111,6 +1148,70 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st, bool che
   }
   if (c->S_host->nonfinite) return fail(c, SP_ERR_NONFINITE, "non-finite value in the solve");
   return SP_OK;
+}
+
+// Iterative refinement of a non-singular system's pressure, kept as
+// x + xd: the residual of x + xd in difference form, then, when it is above
+// tol, a solve of A e = residual from e = 0 and xd += e.  The corrected
+// velocity takes the differences of x and of xd separately, so it is not
+// limited by the rounding of a pressure level that a Dirichlet face fixes.
+// *rmax: max |residual|/V before the solve; *iters: its CG iterations.
+sp_status refine(sp_ctx* c, double tol, double* rmax, int* iters) {
+  const Dims& D = c->dm;
+  CgArgs A{};
+  A.I = D.I; A.J = D.J; A.K = D.K;
+  A.cx = c->cx; A.cy = c->cy; A.cz = c->cz; A.e = c->has_e ? c->ee : nullptr;
+  A.active = c->active;
+  A.dxp = c->dxp; A.dyp = c->dyp; A.dzp = c->dzp;
+  A.b = c->b;
+  A.x = c->x;
+  A.S = c->S;
+  const dim3 blk(64, 4, 1);
+  const dim3 cgrd = col_grid(D.I, D.J, D.K);
+  const uint32_t ncol = col_blocks(D.I, D.J, D.K);
+  hipError_t e;
+  if (!c->xd_active) {
+    e = hipMemsetAsync(c->xd, 0, sizeof(double) * D.N, c->stream);
+    if (e != hipSuccess) return hip_check(c, e, "hipMemsetAsync(xd)");
+    c->xd_active = true;
+  }
+  int slot;
+  A.site = red_site(c, kSiteResidual);
+  const double eb = c->has_e ? 8.0 : 0.0;
+  timer_begin(c, "refine_residual", (double)D.N * (8 * 4 + 24 + eb + 1 + 8), &slot);
+  hipLaunchKernelGGL(k_refine_residual, cgrd, blk, 0, c->stream, A, (const double*)c->xd, c->p1);
+  hipLaunchKernelGGL(k_refine_residual_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A, ncol);
+  timer_end(c, slot);
+  e = hipMemcpyAsync(c->S_host, c->S, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "refine");
+  *rmax = c->S_host->npart[0];
+  *iters = 0;
+  if (!(*rmax > tol)) return SP_OK;
+  const int32_t iter_before = c->S_host->iter;  // the device count runs on across solves
+  // solve A e = rt into the spare iterate, then xd += e
+  std::swap(c->x, c->xe);
+  sp_stats st{};
+  sp_status s = solve(c, c->p1, tol, &st, false, false);
+  std::swap(c->x, c->xe);
+  if (s != SP_OK) return s;
+  *iters = st.iterations - iter_before;
+  timer_begin(c, "refine_add", (double)D.N * 24, &slot);
+  hipLaunchKernelGGL(k_refine_add, cgrd, blk, 0, c->stream, A, (const double*)c->xe, c->xd);
+  timer_end(c, slot);
+  return SP_OK;
+}
+
+// x += xd (the refined pressure, rounded, for the adjoint's checkpoint).
+sp_status fold_refinement(sp_ctx* c) {
+  const Dims& D = c->dm;
+  CgArgs A{};
+  A.I = D.I; A.J = D.J; A.K = D.K;
+  hipLaunchKernelGGL(k_refine_add, col_grid(D.I, D.J, D.K), dim3(64, 4, 1), 0, c->stream, A,
+                     (const double*)c->xd, c->x);
+  c->xd_active = false;
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SP_OK : hip_check(c, e, "fold_refinement");
 }
 
 }  // namespace sp

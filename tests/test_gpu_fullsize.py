@@ -120,3 +120,33 @@ def test_fullsize_adjoint_dot_product(gpu_available, cfg, size, dist):
     # the adjoint solve stops at 1e-10 of its source; on the stiff rho 1000:1
     # operator that leaves a relative error up to ~1e-8 in J^T w
     assert abs(lhs - rhs) <= 1e-7 * abs(lhs), (lhs, rhs)
+
+
+@pytest.mark.parametrize("size", [(128, 128, 128), (256, 256, 256)])
+def test_dirichlet_outlet_reaches_tolerance(gpu_available, size):
+    """The reference drivers' outlet (pressure Dirichlet on the east face,
+    Main.f90:92) with an inflow u*(0) = 1 fixes the pressure level at a few
+    hundred: rounding of that level in one FP64 value per cell limits the
+    divergence of the corrected field near 3.5e-10 at 256^3.  The engine
+    refines the pressure as x + xd and reaches max |div u| <= 1e-10; both
+    precisions give the same pressure."""
+    import numpy as np
+    c = H.case(2, size=size)
+    I, J, K = c.shape
+    c.bc_flag[:] = (1, 0, 1, 1, 1, 1)
+    c.fields["u_star"][0, 1:-1, 1:-1] = 1.0
+    print(f"[outlet] size={size}", flush=True)
+    dev, torch = _dev(c)
+    bcv = [None, torch.zeros((K, J), dtype=torch.float64, device="cuda"), None, None, None, None]
+    hs = (K + 2, J + 2, I + 2)
+    ps = []
+    for prec in (0, 1):
+        P = H.projector(c, precision=prec)
+        out = {n: torch.zeros(hs, dtype=torch.float64, device="cuda") for n in ("p", "u", "v", "w")}
+        st = P.project_dev(dev, c.bc_flag, c.dt, out, bc_val=bcv)
+        torch.cuda.synchronize()
+        P.close()
+        assert st.converged == 1 and st.final_div <= 1e-10, st.as_dict()
+        assert st.singular == 0
+        ps.append(out["p"].cpu().numpy())
+    assert np.max(np.abs(ps[0] - ps[1])) <= 1e-8 * np.max(np.abs(ps[0]))
