@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--adjoint", action="store_true",
                     help="time forward + adjoint passes of the mixed solver and report the "
                          "per-kernel GB/s of one profiled pair")
+    ap.add_argument("--vtr", type=str, default=None, metavar="DIR",
+                    help="after the timed run, write the mixed solution as PrintResultVTR3D's "
+                         "VTR file under DIR and report the writer's time")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 1 GiB device-to-device copy (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -239,6 +242,36 @@ def forward_adjoint(P, dev, out, case, steps):
                 adjoint_converged=bool(ast.converged), kernels=table)
 
 
+def write_vtr(P, dev, out, case, directory):
+    """sp_write_vtr of the device-resident solution: wall time of the call and
+    the GPU formatting kernel's algorithmic GB/s (8 B read, 17.2 B of text
+    written per value)."""
+    import numpy as np
+    import torch
+    x = torch.from_numpy(np.cumsum(case.dxp) - 0.5 * case.dxp).cuda()
+    y = torch.from_numpy(np.cumsum(case.dyp) - 0.5 * case.dyp).cuda()
+    z = torch.from_numpy(np.cumsum(case.dzp) - 0.5 * case.dzp).cuda()
+    I, J, K = case.shape
+    mres = torch.zeros((K, J, I), dtype=torch.float64, device="cuda")
+    fields = dict(x=x, y=y, z=z, u=out["u"], v=out["v"], w=out["w"], p=out["p"],
+                  phi=dev["phi_l"], phi_l=dev["phi_l"], vof=dev["vof"], vof_l=dev["vof_l"],
+                  mres=mres)
+    P.set_profiling(True)
+    P.set_profiling_filter("vtr_format")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    path = P.write_vtr(directory, "case", 1, fields, device=True)
+    el = time.perf_counter() - t0
+    rows = [r for r in P.kernel_times() if r["name"] == "vtr_format"]
+    P.set_profiling_filter(None)
+    P.set_profiling(False)
+    r = rows[0]
+    size = os.path.getsize(path)
+    return dict(path=path, bytes=size, seconds=round(el, 3), mb_per_s=round(size / el / 1e6, 1),
+                format_launches=r["launches"], format_ms=round(r["total_ms"], 3),
+                format_gbs=round(r["bytes"] / (r["total_ms"] * 1e-3) / 1e9, 1))
+
+
 def all_configs(args):
     """Configs 0-4 at full size: one JSON line each."""
     import torch
@@ -340,6 +373,7 @@ def main():
     median_ms = median([st.t_total_ms for st in st_m])
     host = None if args.no_host else host_arrays(Pm, case)
     fwd_adj = forward_adjoint(Pm, dev, out, case, max(1, args.steps // 2)) if args.adjoint else None
+    vtr = write_vtr(Pm, dev, out, case, args.vtr) if args.vtr else None
     Pm.close()
 
     fp64 = {}
@@ -416,6 +450,7 @@ def main():
         "fp64": fp64,
         "host_arrays": host,
         "forward_adjoint": fwd_adj,
+        "vtr": vtr,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": [dict(name=k["name"], launches=k["launches"], total_ms=round(k["total_ms"], 3),

@@ -218,6 +218,33 @@ sp_status sp_adjoint(sp_ctx* ctx, const sp_adj_inputs* in, sp_adj_outputs* out, 
 sp_status sp_adjoint_dev(sp_ctx* ctx, const sp_adj_inputs* in, sp_adj_outputs* out,
                          sp_stats* stats);
 
+/* Fields of PrintResultVTR3D (PrintResult.f90:97-147).  x, y, z are the
+ * cell-centre coordinates TGrid%x(:,1,1), %y(1,:,1), %z(1,1,:) (lengths Imax,
+ * Jmax, Kmax); the nine fields are read on (1:Imax,1:Jmax,1:Kmax).  Each field
+ * is either a halo array (0:Imax+1, 0:Jmax+1, 0:Kmax+1) or a cell array
+ * (1:Imax, 1:Jmax, 1:Kmax): bit q of halo_mask set marks field q (in the
+ * order u, v, w, p, phi, phi_l, vof, vof_l, mres) as a halo array. */
+typedef struct sp_vtr_fields {
+  const double *x, *y, *z;
+  const double *u, *v, *w, *p;   /* TVar%u, v, w, p */
+  const double *phi, *phi_l;     /* TCell%phi, phiL (FluidLvs, LiquidLvs) */
+  const double *vof, *vof_l;     /* TCell%vof, vofL (FluidVof, LiquidVof) */
+  const double *mres;            /* TVar%mres (Mass error) */
+  uint32_t halo_mask;
+} sp_vtr_fields;
+
+/* Write <dir>Paraview/<prefix>_<itt as 8 digits>.vtr, the file
+ * PrintResultVTR3D writes through VTR_mod.f90 (file name VTR_mod.f90:116-130):
+ * an ASCII VTK RectilinearGrid with the same elements, array names and order
+ * (Velocity, X_/Y_/Z_Velocity, Pressure, FluidLvs, LiquidLvs, FluidVof,
+ * LiquidVof, Mass error).  Every value is the field rounded to float32, as
+ * real(., kind=4) does, printed with 10 significant digits so that a reader
+ * gets back exactly that float32; the numbers are formatted on the GPU.
+ * `dir` ends with '/'; missing folders of <dir>Paraview/ are created.  `device`: 1
+ * when the array pointers are device memory, 0 for host memory. */
+sp_status sp_write_vtr(sp_ctx* ctx, const char* dir, const char* prefix, int64_t itt,
+                       const sp_vtr_fields* fields, int device);
+
 /* HIP stream the context runs on (hipStream_t as void*). */
 void* sp_stream(sp_ctx* ctx);
 

@@ -748,6 +748,16 @@ sp_status sp_wall_step(sp_ctx* c, const sp_inputs* in, sp_outputs* out, sp_stats
   return wall_finish(c, st);
 }
 
+sp_status sp_write_vtr(sp_ctx* c, const char* dir, const char* prefix, int64_t itt,
+                       const sp_vtr_fields* f, int device) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  if (!dir || !prefix || !f) return fail(c, SP_ERR_INVALID_ARG, "null VTR argument");
+  const sp_status s = write_vtr(c, dir, prefix, itt, *f, device != 0);
+  timers_collect(c);
+  return s;
+}
+
 void* sp_stream(sp_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 void sp_set_profiling(sp_ctx* c, int on) {

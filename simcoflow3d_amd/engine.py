@@ -33,7 +33,7 @@ EXPORTED_SYMBOLS = (
     "sp_params_default", "sp_create", "sp_destroy", "sp_set_params", "sp_set_grid",
     "sp_set_grid_1d", "sp_project", "sp_project_dev", "sp_adjoint", "sp_adjoint_dev",
     "sp_stream", "sp_kernel_times", "sp_set_profiling", "sp_set_profiling_filter",
-    "sp_strerror", "sp_last_error", "sp_wall_step", "sp_wall_step_dev",
+    "sp_strerror", "sp_last_error", "sp_wall_step", "sp_wall_step_dev", "sp_write_vtr",
 )
 
 
@@ -69,6 +69,14 @@ def make_wall(flags, consts):
         w.const_u[f], w.const_v[f], w.const_w[f], w.const_p[f] = (
             float(consts[6 * q + f]) for q in range(4))
     return w
+
+
+VTR_FIELDS = ("u", "v", "w", "p", "phi", "phi_l", "vof", "vof_l", "mres")
+
+
+class SpVtrFields(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("x", "y", "z") + VTR_FIELDS] + [
+        ("halo_mask", ctypes.c_uint32)]
 
 
 class SpInputs(ctypes.Structure):
@@ -158,6 +166,9 @@ def load():
         f = getattr(L, name)
         f.argtypes = [V, P(SpInputs), P(SpOutputs), P(SpStats)]
         f.restype = ctypes.c_int
+    L.sp_write_vtr.argtypes = [V, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64,
+                               P(SpVtrFields), ctypes.c_int]
+    L.sp_write_vtr.restype = ctypes.c_int
     L.sp_stream.argtypes = [V]
     L.sp_stream.restype = V
     L.sp_kernel_times.argtypes = [V, P(SpKernelTime), ctypes.c_int]
@@ -320,6 +331,33 @@ class Projector:
         rc = self.lib.sp_wall_step(self.h, ctypes.byref(inp), ctypes.byref(o), ctypes.byref(st))
         self._check(rc, "sp_wall_step")
         return out, st
+
+    def write_vtr(self, directory, prefix, itt, fields, device=False):
+        """PrintResultVTR3D's file (sp_write_vtr): fields maps x, y, z (1-D) and
+        u, v, w, p, phi, phi_l, vof, vof_l, mres (halo or cell arrays; numpy
+        arrays, or torch device tensors with device=True)."""
+        I, J, K = self.shape
+        f = SpVtrFields()
+        keep = []
+        mask = 0
+        for n in ("x", "y", "z") + VTR_FIELDS:
+            a = fields[n]
+            if isinstance(a, np.ndarray):
+                a = np.asfortranarray(a, dtype=np.float64)
+                keep.append(a)
+            setattr(f, n, _addr(a))
+            if n in VTR_FIELDS:
+                shape = tuple(a.shape) if isinstance(a, np.ndarray) else tuple(a.shape)[::-1]
+                if shape == (I + 2, J + 2, K + 2):
+                    mask |= 1 << VTR_FIELDS.index(n)
+                elif shape != (I, J, K):
+                    raise ValueError(f"{n}: shape {shape} is neither a halo nor a cell array")
+        f.halo_mask = mask
+        d = directory if directory.endswith("/") else directory + "/"
+        rc = self.lib.sp_write_vtr(self.h, d.encode(), prefix.encode(), int(itt), ctypes.byref(f),
+                                   1 if device else 0)
+        self._check(rc, "sp_write_vtr")
+        return os.path.join(d, "Paraview", f"{prefix}_{int(itt):08d}.vtr")
 
     def adjoint(self, gu, gv, gw, beta_grad=False):
         I, J, K = self.shape
