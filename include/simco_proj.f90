@@ -15,10 +15,10 @@ module simco_proj
   implicit none
   private
   public :: sp_params, sp_inputs, sp_outputs, sp_stats, sp_adj_inputs, sp_adj_outputs, &
-            sp_wall_bc
+            sp_wall_bc, sp_vtr_fields
   public :: sp_params_default, sp_create, sp_destroy, sp_set_params, sp_set_grid, &
             sp_set_grid_1d, sp_project, sp_project_dev, sp_adjoint, sp_adjoint_dev, &
-            sp_wall_step, sp_wall_step_dev, sp_strerror_f, sp_last_error_f
+            sp_wall_step, sp_wall_step_dev, sp_write_vtr, sp_strerror_f, sp_last_error_f
   public :: SP_OK, SP_ERR_INVALID_ARG, SP_ERR_HIP, SP_ERR_NOT_CONVERGED, SP_ERR_NONFINITE, &
             SP_ERR_OOM, SP_ERR_GRID, SP_ERR_STATE, SP_ERR_NO_DEVICE, SP_PREC_FP64, &
             SP_PREC_MIXED, SP_PRECOND_MG, SP_PRECOND_JACOBI
@@ -84,6 +84,18 @@ module simco_proj
     real(c_double) :: t_assemble_ms, t_solve_ms, t_correct_ms, t_total_ms
     real(c_double) :: wall_div
   end type sp_stats
+
+  ! PrintResultVTR3D's fields (c_loc of the host arrays); bit q of halo_mask
+  ! marks field q (u, v, w, p, phi, phi_l, vof, vof_l, mres) as a
+  ! (0:Imax+1, ...) array, otherwise (1:Imax, ...)
+  type, bind(C) :: sp_vtr_fields
+    type(c_ptr) :: x = c_null_ptr, y = c_null_ptr, z = c_null_ptr
+    type(c_ptr) :: u = c_null_ptr, v = c_null_ptr, w = c_null_ptr, p = c_null_ptr
+    type(c_ptr) :: phi = c_null_ptr, phi_l = c_null_ptr
+    type(c_ptr) :: vof = c_null_ptr, vof_l = c_null_ptr
+    type(c_ptr) :: mres = c_null_ptr
+    integer(c_int32_t) :: halo_mask = 0
+  end type sp_vtr_fields
 
   type, bind(C) :: sp_adj_inputs
     type(c_ptr) :: gu = c_null_ptr, gv = c_null_ptr, gw = c_null_ptr
@@ -181,6 +193,17 @@ module simco_proj
       type(sp_outputs), intent(in) :: outp
       type(sp_stats), intent(out) :: st
     end function sp_wall_step_dev
+
+    ! dir and prefix: NUL-terminated (trim(dir)//c_null_char)
+    integer(c_int) function sp_write_vtr(ctx, dir, prefix, itt, fields, device) &
+        bind(C, name="sp_write_vtr")
+      import :: c_ptr, c_int, c_char, c_int64_t, sp_vtr_fields
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: dir(*), prefix(*)
+      integer(c_int64_t), value :: itt
+      type(sp_vtr_fields), intent(in) :: fields
+      integer(c_int), value :: device
+    end function sp_write_vtr
 
     type(c_ptr) function sp_strerror_c(s) bind(C, name="sp_strerror")
       import :: c_ptr, c_int

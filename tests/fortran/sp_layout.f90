@@ -11,6 +11,7 @@ program sp_layout
   type(sp_adj_inputs), target :: ai
   type(sp_adj_outputs), target :: ao
   type(sp_wall_bc), target :: wb
+  type(sp_vtr_fields), target :: vf
   print '(a,1x,i0)', 'sp_params', c_sizeof(prm)
   print '(a,1x,i0)', 'sp_inputs', c_sizeof(inp)
   print '(a,1x,i0)', 'sp_outputs', c_sizeof(outp)
@@ -18,6 +19,9 @@ program sp_layout
   print '(a,1x,i0)', 'sp_adj_inputs', c_sizeof(ai)
   print '(a,1x,i0)', 'sp_adj_outputs', c_sizeof(ao)
   print '(a,1x,i0)', 'sp_wall_bc', c_sizeof(wb)
+  print '(a,1x,i0)', 'sp_vtr_fields', c_sizeof(vf)
+  print '(a,1x,i0)', 'sp_vtr_fields.mres', off(c_loc(vf%mres), c_loc(vf))
+  print '(a,1x,i0)', 'sp_vtr_fields.halo_mask', off(c_loc(vf%halo_mask), c_loc(vf))
   print '(a,1x,i0)', 'sp_wall_bc.const_u', off(c_loc(wb%const_u), c_loc(wb))
   print '(a,1x,i0)', 'sp_wall_bc.const_p', off(c_loc(wb%const_p), c_loc(wb))
   print '(a,1x,i0)', 'sp_inputs.wall', off(c_loc(inp%wall), c_loc(inp))
