@@ -13,6 +13,7 @@ import sys
 from ._paths import ENGINE_LIB, INCLUDE_DIR, LIB_DIR, PKG_DIR
 
 CSRC = os.path.join(PKG_DIR, "csrc")
+FLANG = "/opt/rocm/lib/llvm/bin/flang"
 
 # gfx950 only.  Packed FP32 instructions are disabled for every HIP object.
 HIP_FLAGS = [
@@ -78,11 +79,12 @@ def build_engine(jobs=8):
 
 
 def build_oracle():
+    """The CPU oracle; where the reference tree and amdflang are present, also
+    the reference's own VTR writer with its test host (oracle/_ref)."""
     repo = os.path.dirname(PKG_DIR)
     _run(["make", "-s", "-C", os.path.join(repo, "oracle")])
-
-
-FLANG = "/opt/rocm/lib/llvm/bin/flang"
+    if os.path.isdir("/root/reference/src") and os.path.exists(FLANG):
+        _run(["make", "-s", "-C", os.path.join(repo, "oracle"), "ref"])
 
 
 def build_fortran():
