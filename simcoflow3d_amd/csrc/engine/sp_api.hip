@@ -386,8 +386,7 @@ static sp_status upload(sp_ctx* c, size_t idx, const T* src, size_t count, const
   void* d;
   sp_status s = stage(c, idx, count * sizeof(T), &d);
   if (s) return s;
-  hipError_t e = hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream);
-  if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(H2D)");
+  if ((s = pipe_upload(c, d, src, count * sizeof(T)))) return s;
   *dst = static_cast<const T*>(d);
   return SP_OK;
 }
@@ -467,6 +466,7 @@ sp_status sp_destroy(sp_ctx* c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   free_buffers(c);
+  pipe_destroy(c);
   for (auto& s : c->stage) if (s.ptr) hipFree(s.ptr);
   c->stage.clear();
   for (auto& p : c->ev_pool) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
@@ -569,10 +569,7 @@ sp_status sp_project(sp_ctx* c, const sp_inputs* in, sp_outputs* out, sp_stats* 
     void* p;
     if ((s = stage(c, k++, Nh * sizeof(double), &p))) return s;
     outs_d[q] = static_cast<double*>(p);
-    if (q > 0) {
-      hipError_t e = hipMemcpyAsync(p, outs_h[q], Nh * sizeof(double), hipMemcpyHostToDevice, c->stream);
-      if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(H2D)");
-    }
+    if (q > 0 && (s = pipe_upload(c, p, outs_h[q], Nh * sizeof(double)))) return s;
   }
   o.p = outs_d[0]; o.u = outs_d[1]; o.v = outs_d[2]; o.w = outs_d[3];
   o.po_coef = nullptr;
@@ -580,14 +577,10 @@ sp_status sp_project(sp_ctx* c, const sp_inputs* in, sp_outputs* out, sp_stats* 
   c->last_in_host = true;
   sp_status r = project_device(c, &d, &o, st, true);
   if (r != SP_OK && r != SP_ERR_NOT_CONVERGED) return r;
-  for (int q = 0; q < 4; ++q) {
-    hipError_t e = hipMemcpyAsync(outs_h[q], outs_d[q], Nh * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(D2H)");
-  }
-  if (out->po_coef) {
-    hipError_t e = hipMemcpyAsync(out->po_coef, c->pocoef, 6 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(po_coef)");
-  }
+  for (int q = 0; q < 4; ++q)
+    if ((s = pipe_download(c, outs_h[q], outs_d[q], Nh * sizeof(double)))) return s;
+  if (out->po_coef && (s = pipe_download(c, out->po_coef, c->pocoef, 6 * N * sizeof(double))))
+    return s;
   hipError_t e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_check(c, e, "sp_project");
   return r;
@@ -664,8 +657,7 @@ sp_status sp_adjoint(sp_ctx* c, const sp_adj_inputs* in, sp_adj_outputs* out, sp
   if (r != SP_OK && r != SP_ERR_NOT_CONVERGED) return r;
   for (int q = 0; q < 9; ++q) {
     if (!hs[q]) continue;
-    hipError_t e = hipMemcpyAsync(hs[q], ds[q], (q < 3 ? Nh : N) * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(D2H)");
+    if ((s = pipe_download(c, hs[q], ds[q], (q < 3 ? Nh : N) * sizeof(double)))) return s;
   }
   hipError_t e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_check(c, e, "sp_adjoint");
@@ -741,10 +733,8 @@ sp_status sp_wall_step(sp_ctx* c, const sp_inputs* in, sp_outputs* out, sp_stats
   o.u = ds[0]; o.v = ds[1]; o.w = ds[2]; o.p = ds[3];
   for (int q = 0; q < 24; ++q) o.bc_var[q] = out->bc_var[q];  // host memory
   if ((s = wall_step(c, d, o, true))) return s;
-  for (int q = 0; q < 4; ++q) {
-    hipError_t e = hipMemcpyAsync(hs[q], ds[q], Nh * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    if (e != hipSuccess) return hip_check(c, e, "hipMemcpyAsync(D2H)");
-  }
+  for (int q = 0; q < 4; ++q)
+    if ((s = pipe_download(c, hs[q], ds[q], Nh * sizeof(double)))) return s;
   return wall_finish(c, st);
 }
 

@@ -107,6 +107,8 @@ struct DeviceBuf {
   size_t bytes = 0;
 };
 
+struct HostPipe;  // pinned staging ring for the host-array entry points
+
 }  // namespace sp
 
 struct sp_ctx {
@@ -127,6 +129,7 @@ struct sp_ctx {
 
   // staged inputs for the host-pointer entry points (device copies)
   std::vector<sp::DeviceBuf> stage;
+  sp::HostPipe* pipe = nullptr;  // created on the first host-array call
 
   // assembled operator (FP64, interior layout)
   double* pocoef = nullptr;  // 6*N, faces W,S,B,E,N,T
@@ -211,6 +214,12 @@ sp_status fail(sp_ctx* c, sp_status s, const std::string& msg);
 sp_status hip_check(sp_ctx* c, hipError_t e, const char* what);
 void timer_begin(sp_ctx* c, const char* name, double bytes, int* slot);
 void timer_end(sp_ctx* c, int slot);
+
+// ---- host <-> device copies of pageable arrays (sp_hostio.hip) ----
+sp_status pipe_upload(sp_ctx* c, void* dst, const void* src, size_t bytes);
+// synchronous: dst holds the data on return
+sp_status pipe_download(sp_ctx* c, void* dst, const void* src, size_t bytes);
+void pipe_destroy(sp_ctx* c);
 
 // ---- assembly (sp_assemble.hip) ----
 sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_count);
