@@ -328,7 +328,11 @@ __global__ void __launch_bounds__(256) k_operator(OpArgs A) {
       double cxv = 0, cyv = 0, czv = 0, cW = 0, cS = 0, cB = 0, ex = 0, rhs = 0, dval = 0;
       bool act = false;
       if (ctP != 2) {
-        const double W = A.pocoef[n], S = A.pocoef[N + n], B = A.pocoef[2 * N + n];
+        // the W/S/B coefficients enter only at walls and faces towards solid
+        // cells: loaded there alone (a third of the PoCoef traffic otherwise)
+        const double* pcW = A.pocoef;
+        const double* pcS = A.pocoef + N;
+        const double* pcB = A.pocoef + 2 * N;
         const double E = A.pocoef[3 * N + n], Nn = A.pocoef[4 * N + n], T = A.pocoef[5 * N + n];
         const bool wS = a > 0 ? A.ct[n - 1] == 2 : A.fl[0] == 0;
         const bool sS = b > 0 ? A.ct[n - A.I] == 2 : A.fl[2] == 0;
@@ -344,9 +348,9 @@ __global__ void __launch_bounds__(256) k_operator(OpArgs A) {
         if (b > 0 && A.ct[n - A.I] != 2) cS = A.pocoef[4 * N + n - A.I] * A.ne[h - A.SX] * dx * dz;
         if (c > 0 && A.ct[n - IJ] != 2) cB = A.pocoef[5 * N + n - IJ] * A.te[h - SXY] * dx * dy;
         // diagonal-only terms with the row's own coefficient (:379-430)
-        if (wS) ex = ex + W * A.ee[h - 1] * dy * dz;
-        if (sS) ex = ex + S * A.ne[h - A.SX] * dx * dz;
-        if (bS) ex = ex + B * A.te[h - SXY] * dx * dy;
+        if (wS) ex = ex + pcW[n] * A.ee[h - 1] * dy * dz;
+        if (sS) ex = ex + pcS[n] * A.ne[h - A.SX] * dx * dz;
+        if (bS) ex = ex + pcB[n] * A.te[h - SXY] * dx * dy;
         if (eS) ex = ex + E * A.ee[h] * dy * dz;
         if (nS) ex = ex + Nn * A.ne[h] * dx * dz;
         if (tS) ex = ex + T * A.te[h] * dx * dy;
@@ -365,11 +369,11 @@ __global__ void __launch_bounds__(256) k_operator(OpArgs A) {
               dx * dz * (A.ne[h] * A.v[h] - A.ne[h - A.SX] * A.v[h - A.SX]) -
               dx * dy * (A.te[h] * A.w[h] - A.te[h - SXY] * A.w[h - SXY]);
         rhs = rhs + (A.vof[h] - A.vofO[h]) * dx * dy * dz / A.dt;
-        if (A.fl[0] == 0 && a == 0) rhs = rhs + W * A.bcv[0][b + (int64_t)A.J * c];
+        if (A.fl[0] == 0 && a == 0) rhs = rhs + pcW[n] * A.bcv[0][b + (int64_t)A.J * c];
         if (A.fl[1] == 0 && a == A.I - 1) rhs = rhs + E * A.bcv[1][b + (int64_t)A.J * c];
-        if (A.fl[2] == 0 && b == 0) rhs = rhs + S * A.bcv[2][a + (int64_t)A.I * c];
+        if (A.fl[2] == 0 && b == 0) rhs = rhs + pcS[n] * A.bcv[2][a + (int64_t)A.I * c];
         if (A.fl[3] == 0 && b == A.J - 1) rhs = rhs + Nn * A.bcv[3][a + (int64_t)A.I * c];
-        if (A.fl[4] == 0 && c == 0) rhs = rhs + B * A.bcv[4][a + (int64_t)A.I * b];
+        if (A.fl[4] == 0 && c == 0) rhs = rhs + pcB[n] * A.bcv[4][a + (int64_t)A.I * b];
         if (A.fl[5] == 0 && c == A.K - 1) rhs = rhs + T * A.bcv[5][a + (int64_t)A.I * b];
         if (act) {
           if (!is_finite(rhs)) v[4] = 1.0;
@@ -490,8 +494,11 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_out) {
   O.S = c->S;
   if (flat_blocks(D.I, D.J, D.K) > c->red_max_blocks)
     return fail(c, SP_ERR_INVALID_ARG, "grid too large for the reduction buffers");
+  // per cell: PoCoef E,N,T (W,S,B only at walls and solid faces), apertures,
+  // cell type, u*, v*, w*, vof, vofO -> cx, cy, cz, e, b, mask, diagonal
+  // weight (and the FP32 couplings)
   timer_begin(c, "operator_rhs",
-              (double)D.N * (6 * 8 + 3 * 8 + 4 + 5 * 8 + 4 * 8 + 8 + 1 + 4 + (mixed ? 16 : 0)), &slot);
+              (double)D.N * (3 * 8 + 3 * 8 + 4 + 5 * 8 + 4 * 8 + 8 + 1 + 4 + (mixed ? 16 : 0)), &slot);
   hipLaunchKernelGGL(k_operator, col_grid_k<kOpK>(D.I, D.J, D.K), dim3(64, 4, 1), 0, c->stream, O);
   timer_end(c, slot);
   {
