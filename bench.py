@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--vtr", type=str, default=None, metavar="DIR",
                     help="after the timed run, write the mixed solution as PrintResultVTR3D's "
                          "VTR file under DIR and report the writer's time")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="measure the practical HBM bandwidth of plain streaming kernels "
+                         "(torch copy and a*x+y on 2 GiB FP64 arrays) and report it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 1 GiB device-to-device copy (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -242,6 +245,33 @@ def forward_adjoint(P, dev, out, case, steps):
                 adjoint_converged=bool(ast.converged), kernels=table)
 
 
+def stream_probe(reps=10):
+    """Practical HBM bandwidth on this GPU: device-side time of torch's copy
+    (16 B moved per element) and y = a*x + y (24 B) on 2^28-element FP64
+    arrays, the best of `reps` launches each."""
+    import torch
+    n = 1 << 28
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.rand(n, dtype=torch.float64, device="cuda")
+    z = torch.empty_like(x)
+    res = {}
+    for name, fn, nbytes in (("copy", lambda: z.copy_(x), 16 * n),
+                             ("axpy", lambda: y.add_(x, alpha=0.5), 24 * n)):
+        best = float("inf")
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1) * 1e-3)
+        res[name + "_gbs"] = round(nbytes / best / 1e9, 1)
+    del x, y, z
+    torch.cuda.empty_cache()
+    return res
+
+
 def write_vtr(P, dev, out, case, directory):
     """sp_write_vtr of the device-resident solution: wall time of the call and
     the GPU formatting kernel's algorithmic GB/s (8 B read, 17.2 B of text
@@ -374,6 +404,7 @@ def main():
     host = None if args.no_host else host_arrays(Pm, case)
     fwd_adj = forward_adjoint(Pm, dev, out, case, max(1, args.steps // 2)) if args.adjoint else None
     vtr = write_vtr(Pm, dev, out, case, args.vtr) if args.vtr else None
+    stream = stream_probe() if args.stream_probe else None
     Pm.close()
 
     fp64 = {}
@@ -451,6 +482,7 @@ def main():
         "host_arrays": host,
         "forward_adjoint": fwd_adj,
         "vtr": vtr,
+        "stream_probe": stream,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": [dict(name=k["name"], launches=k["launches"], total_ms=round(k["total_ms"], 3),
