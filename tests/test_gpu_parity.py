@@ -162,3 +162,34 @@ def test_dirichlet_walls_match_oracle(gpu_available, flags, cfg, prec):
     for n, r in (("u", ref.u), ("v", ref.v), ("w", ref.w)):
         d = out[n][1:-1, 1:-1, 1:-1] - r[1:-1, 1:-1, 1:-1]
         assert np.max(np.abs(d)) <= 1e-8 * umax, n
+
+
+def test_iteration_limit_returns_outputs(gpu_available):
+    """Like the reference, which continues after a HYPRE failure
+    (ProjectionP.f90:105-106), a solve stopped by max_iter writes every
+    output and reports SP_ERR_NOT_CONVERGED with converged = 0."""
+    c = H.case(1, size=(24, 24, 32))
+    print("[limits] max_iter=3", flush=True)
+    P = H.projector(c, precision=1, max_iter=3)
+    out, st = P.project(H.fields(c), c.bc_flag, c.dt)
+    P.close()
+    assert st.converged == 0 and st.iterations <= 4, st.as_dict()
+    assert st.final_div > 1e-10
+    for n in ("p", "u", "v", "w"):
+        assert np.all(np.isfinite(out[n])), n
+    assert np.max(np.abs(out["p"])) > 0.0
+
+
+def test_nonfinite_input_is_reported(gpu_available):
+    """A NaN in u* is reported as SP_ERR_NONFINITE instead of propagating."""
+    from simcoflow3d_amd.engine import SP_ERR_NONFINITE, SpError
+    c = H.case(0, size=(16, 16, 16))
+    f = H.fields(c)
+    f["u_star"] = f["u_star"].copy(order="F")
+    f["u_star"][5, 6, 7] = np.nan
+    print("[limits] NaN input", flush=True)
+    P = H.projector(c)
+    with pytest.raises(SpError) as ei:
+        P.project(f, c.bc_flag, c.dt)
+    P.close()
+    assert ei.value.status == SP_ERR_NONFINITE
