@@ -193,3 +193,40 @@ def test_nonfinite_input_is_reported(gpu_available):
         P.project(f, c.bc_flag, c.dt)
     P.close()
     assert ei.value.status == SP_ERR_NONFINITE
+
+
+def _random_cases(n=8, seed=2024):
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(n):
+        cfg = int(rng.integers(0, 5))
+        size = tuple(int(v) for v in rng.integers(2, 41, 3))
+        flags = tuple(int(v) for v in (rng.random(6) < 0.75))  # mostly Neumann
+        out.append((cfg, size, flags, q % 2))
+    return out
+
+
+@pytest.mark.parametrize("cfg,size,flags,prec", _random_cases())
+def test_random_grids_and_walls_match_oracle(gpu_available, cfg, size, flags, prec):
+    """Seeded random sizes (2..40 per direction), configurations, pressure
+    wall flags and Dirichlet face values: same bars as the configurations."""
+    c = H.case(cfg, size=size, seed=sum(size))
+    c.bc_flag[:] = flags
+    I, J, K = c.shape
+    rng = np.random.default_rng(sum(size) + 7)
+    dims = [(J, K), (J, K), (I, K), (I, K), (I, J), (I, J)]
+    bcv = [np.asfortranarray(rng.uniform(-1.0, 1.0, dims[f])) if flags[f] == 0 else None
+           for f in range(6)]
+    print(f"[random] cfg={cfg} size={size} flags={flags} prec={prec}", flush=True)
+    ref = orc.project(c, pin=(1, 1, 1), bc_val=bcv)
+    P = H.projector(c, precision=prec)
+    out, st = P.project(H.fields(c), c.bc_flag, c.dt, bc_val=bcv, po_coef=True)
+    P.close()
+    assert st.converged == 1, st.as_dict()
+    assert np.array_equal(out["po_coef"], ref.pocoef)
+    if np.max(np.abs(ref.Pp)) > 0:
+        assert H.rel_inf(out["p"], ref.Pp) <= 1e-8
+    umax = max(np.max(np.abs(c.u_star)), np.max(np.abs(c.v_star)), np.max(np.abs(c.w_star)))
+    for n, r in (("u", ref.u), ("v", ref.v), ("w", ref.w)):
+        d = out[n][1:-1, 1:-1, 1:-1] - r[1:-1, 1:-1, 1:-1]
+        assert np.max(np.abs(d)) <= 1e-8 * max(umax, 1e-300), n
