@@ -383,9 +383,11 @@ def main():
     gen_s = time.perf_counter() - tg
     I, J, K = case.shape
     N = I * J * K
+    print(f"[bench] generated {I}x{J}x{K} in {gen_s:.1f} s", file=sys.stderr, flush=True)
     dev = to_device(case)
     out = make_outputs(case, dev)
     torch.cuda.synchronize()
+    print("[bench] inputs on the device", file=sys.stderr, flush=True)
 
     common = dict(rho_l=case.rho_l, rho_g=case.rho_g, rho_ref=case.rho_l, pin_i=1, pin_j=1,
                   pin_k=1, div_tol=1e-10, mg_levels=args.levels)
@@ -395,6 +397,7 @@ def main():
     # per-kernel table from one untimed profiled solve: names the dominant kernel
     ktab = [] if args.no_profile else kernel_table(Pm, dev, out, case)
     dom_name = ktab[0]["name"] if ktab else None
+    print("[bench] warm-up done", file=sys.stderr, flush=True)
     el, st_m, live = timed(Pm, dev, out, case, args.steps, dom_name)
     sm = st_m[-1]
     p_mixed = out["p"].cpu().numpy()
@@ -474,7 +477,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 outer CG + f32 V-cycle",
         "data": "synthetic (seeded generator, inputs resident in HBM)",
-        "config": {"workload": WORKLOADS.get(cfg, f"config {cfg}"), "grid": [I, J, K],
+        "config": {"workload": (WORKLOADS.get(cfg, f"config {cfg}") if size is None else
+                                f"{I}x{J}x{K} variant of: " + WORKLOADS.get(cfg, f"config {cfg}")),
+                   "grid": [I, J, K],
                    "cells": N, "precision": "mixed", "mg_levels": sm.levels,
                    "iterations": sm.iterations, "final_div": sm.final_div,
                    "converged": bool(sm.converged)},
