@@ -245,6 +245,38 @@ typedef struct sp_vtr_fields {
 sp_status sp_write_vtr(sp_ctx* ctx, const char* dir, const char* prefix, int64_t itt,
                        const sp_vtr_fields* fields, int device);
 
+/* One staggered cell grid (PGrid/PCell, UGrid/UCell, VGrid/VCell or
+ * WGrid/WCell) as GridPreprocess (CutCell.f90:62-325) reads and writes it.
+ * Cell arrays are (1:Imax,1:Jmax,1:Kmax); ee/ne/te_area are halo arrays;
+ * fce/fcn/fct are TCell%FCE/FCN/FCT (0:Imax+1,0:Jmax+1,0:Kmax+1,3), the
+ * component index slowest; cell_cent is TCell%Cell_Cent (1:Imax,..,3).
+ * Allocated as in AllocateVar (Main.f90:229-399). */
+typedef struct sp_cell_geom {
+  const double *dx, *dy, *dz;         /* TGrid%dx, dy, dz (cell arrays) */
+  const double *phi, *nx, *ny, *nz;   /* TCell%phi, nx, ny, nz: solid level set, normal */
+  double *vof, *vof_l;                /* TCell%vof, vofL: read, then set by DefineCell */
+  int32_t* cell_type;                 /* TCell%Cell_Type: written (kept where vof is NaN) */
+  double *ee_area, *ne_area, *te_area;/* TCell%EEArea, NEArea, TEArea: written */
+  double *fce, *fcn, *fct;            /* updated; entries the reference never sets keep
+                                         their values */
+  double* cell_cent;                  /* TCell%Cell_Cent: written */
+  double* wllh;                       /* TCell%WlLh: written */
+  double* delh;                       /* TCell%delh: updated where the reference sets it */
+} sp_cell_geom;
+
+/* GridPreprocess(PGrid,UGrid,VGrid,WGrid, PCell,UCell,VCell,WCell)
+ * (CutCell.f90:62-325): DefineCell and CellGeoCal on the four grids (the
+ * MaxFace demotion on P only, whose MoExCell is never allocated), then the
+ * U/V/W and P face and cell centre corrections.  Called at start-up
+ * (Main.f90:203) and after each solid move (SolidBody.f90:815).  Every
+ * output is bit-identical to the reference's loops.  The grid size is the
+ * context's (sp_create); sp_set_grid is not needed.  Host and device
+ * variants. */
+sp_status sp_grid_preprocess(sp_ctx* ctx, const sp_cell_geom* p, const sp_cell_geom* u,
+                             const sp_cell_geom* v, const sp_cell_geom* w);
+sp_status sp_grid_preprocess_dev(sp_ctx* ctx, const sp_cell_geom* p, const sp_cell_geom* u,
+                                 const sp_cell_geom* v, const sp_cell_geom* w);
+
 /* HIP stream the context runs on (hipStream_t as void*). */
 void* sp_stream(sp_ctx* ctx);
 

@@ -1033,3 +1033,350 @@ void orc_wall_step(const orc_grid* g, const double* ee, const double* ne, const 
         if (ctw[n] == 2) w[h] = 0.0;
       }
 }
+
+/* ------------------------------------------------------------------ */
+/* Cut-cell geometry: GridPreprocess (CutCell.f90:62-325) with         */
+/* DefineCell (:329-365), CellGeoCal (:369-671), FaceGeoCal (:883-981). */
+/* ------------------------------------------------------------------ */
+
+/* FaceGeoCal (CutCell.f90:883-981): the part of the rectangle
+ * [-d1/2,d1/2] x [-d2/2,d2/2] (at offset d3 along the third axis) where
+ * n1*x + n2*y + n3*d3 + phi > 0, as an area fraction and a centroid. */
+void orc_face_geo(double d1, double d2, double n1, double n2, double n3, double d3, double phi,
+                  double* area, double cent[3]) {
+  const double d11 = d1 / 2.0, d22 = d2 / 2.0;
+  /* node(6,2) in the reference; 8 + 1 here because separately rounded corner
+   * values can alternate in sign when the normal is ~1e-17 (the reference
+   * then writes past node's end) */
+  double nx_[9] = {0}, ny_[9] = {0};
+  int t = 0;
+  double dpt[4];
+  dpt[0] = d11 * n1 - d22 * n2 + d3 * n3 + phi;   /* :900-903 */
+  dpt[1] = -d11 * n1 - d22 * n2 + d3 * n3 + phi;
+  dpt[2] = -d11 * n1 + d22 * n2 + d3 * n3 + phi;
+  dpt[3] = d11 * n1 + d22 * n2 + d3 * n3 + phi;
+  if (dpt[0] > 0.0) { nx_[t] = d11; ny_[t] = -d22; ++t; }                                  /* :905 */
+  if (dpt[0] * dpt[1] < 0.0) { nx_[t] = (d22 * n2 - d3 * n3 - phi) / n1; ny_[t] = -d22; ++t; }
+  if (dpt[1] > 0.0) { nx_[t] = -d11; ny_[t] = -d22; ++t; }
+  if (dpt[1] * dpt[2] < 0.0) { nx_[t] = -d11; ny_[t] = (d11 * n1 - d3 * n3 - phi) / n2; ++t; }
+  if (dpt[2] > 0.0) { nx_[t] = -d11; ny_[t] = d22; ++t; }
+  if (dpt[2] * dpt[3] < 0.0) { nx_[t] = (-d22 * n2 - d3 * n3 - phi) / n1; ny_[t] = d22; ++t; }
+  if (dpt[3] > 0.0) { nx_[t] = d11; ny_[t] = d22; ++t; }
+  if (dpt[3] * dpt[0] < 0.0) { nx_[t] = d11; ny_[t] = (-d11 * n1 - d3 * n3 - phi) / n2; ++t; }
+  nx_[t] = nx_[0]; ny_[t] = ny_[0];                                                           /* :958 */
+  double vol = 0.0;
+  for (int k = 0; k < t; ++k) vol = vol + 0.5 * (nx_[k] * ny_[k + 1] - nx_[k + 1] * ny_[k]);
+  *area = fabs(vol / (d1 * d2));
+  cent[0] = 0.0; cent[1] = 0.0;
+  if (*area > 0.0) {
+    for (int k = 0; k < t; ++k) {
+      const double cr = nx_[k] * ny_[k + 1] - nx_[k + 1] * ny_[k];
+      cent[0] = cent[0] + 1.0 / 6.0 / vol * (nx_[k] + nx_[k + 1]) * cr;
+      cent[1] = cent[1] + 1.0 / 6.0 / vol * (ny_[k] + ny_[k + 1]) * cr;
+    }
+  }
+  cent[2] = d3;
+}
+
+#define CG_EPSI 1.0e-3  /* CutCell.f90:11 */
+
+/* DefineCell (CutCell.f90:329-365) */
+static void define_cell(int I, int J, int K, orc_cell* c) {
+  const int64_t N = (int64_t)I * J * K;
+  for (int64_t q = 0; q < N; ++q) {
+    if (c->vof[q] >= 1.0 - CG_EPSI) {
+      c->type[q] = 0;
+      c->vofl[q] = c->vofl[q] / c->vof[q];
+      c->vof[q] = 1.0;
+    }
+    if (c->vof[q] <= CG_EPSI) {
+      c->type[q] = 2;
+      c->vof[q] = 0.0;
+      c->vofl[q] = 0.0;
+    }
+    if (c->vof[q] >= CG_EPSI && c->vof[q] <= 1.0 - CG_EPSI) c->type[q] = 1;
+  }
+}
+
+/* CellGeoCal (CutCell.f90:369-671).  `demote`: the MaxFace < 1e-2 rule of
+ * :577-582, which applies where TCell%MoExCell is not allocated (the P grid:
+ * Main.f90:377-399 allocates it for U, V, W only). */
+static void cell_geo_cal(int I, int J, int K, orc_cell* c, int demote) {
+  const orc_grid G = {.I = I, .J = J, .K = K};
+  const orc_grid* g = &G;
+  const int64_t N = (int64_t)I * J * K, Nh = (int64_t)(I + 2) * (J + 2) * (K + 2);
+  const double tol = 1.0e-20;
+  double* WE = dalloc(N); double* SE = dalloc(N); double* BE = dalloc(N);
+  double* Ce[6];  /* E, W, N, S, T, B; 3 components each */
+  for (int f = 0; f < 6; ++f) Ce[f] = dalloc(3 * N);
+  for (int64_t q = 0; q < N; ++q) { WE[q] = 1.0; SE[q] = 1.0; BE[q] = 1.0; c->wllh[q] = 0.0; }
+  for (int64_t q = 0; q < Nh; ++q) { c->ee[q] = 1.0; c->ne[q] = 1.0; c->te[q] = 1.0; }
+  double* fce[3] = {c->fce, c->fce + Nh, c->fce + 2 * Nh};
+  double* fcn[3] = {c->fcn, c->fcn + Nh, c->fcn + 2 * Nh};
+  double* fct[3] = {c->fct, c->fct + Nh, c->fct + 2 * Nh};
+#define CE(f, q, m) Ce[f][(int64_t)(m) * N + (q)]
+  for (int i = 1; i <= I; ++i)
+    for (int j = 1; j <= J; ++j)
+      for (int k = 1; k <= K; ++k) {
+        const int64_t q = NX(g, i, j, k), h = HX(g, i, j, k);
+        const double dx = c->dx[q], dy = c->dy[q], dz = c->dz[q], ph = c->phi[q];
+        const double nx = c->nx[q], ny = c->ny[q], nz = c->nz[q];
+        double F[3];
+        for (int m = 0; m < 3; ++m) c->cent[(int64_t)m * N + q] = 0.0;
+        orc_face_geo(dx, dy, nx, ny, nz, dz / 2.0, ph, &c->te[h], F);          /* :414 */
+        CE(4, q, 0) = F[0]; CE(4, q, 1) = F[1]; CE(4, q, 2) = F[2];
+        orc_face_geo(dx, dy, nx, ny, nz, -dz / 2.0, ph, &BE[q], F);            /* :424 */
+        CE(5, q, 0) = F[0]; CE(5, q, 1) = F[1]; CE(5, q, 2) = F[2];
+        orc_face_geo(dy, dz, ny, nz, nx, dx / 2.0, ph, &c->ee[h], F);          /* :434 */
+        CE(0, q, 0) = F[2]; CE(0, q, 1) = F[0]; CE(0, q, 2) = F[1];
+        orc_face_geo(dy, dz, ny, nz, nx, -dx / 2.0, ph, &WE[q], F);            /* :444 */
+        CE(1, q, 0) = F[2]; CE(1, q, 1) = F[0]; CE(1, q, 2) = F[1];
+        orc_face_geo(dz, dx, nz, nx, ny, dy / 2.0, ph, &c->ne[h], F);          /* :454 */
+        CE(2, q, 0) = F[1]; CE(2, q, 1) = F[2]; CE(2, q, 2) = F[0];
+        orc_face_geo(dz, dx, nz, nx, ny, -dy / 2.0, ph, &SE[q], F);            /* :464 */
+        CE(3, q, 0) = F[1]; CE(3, q, 1) = F[2]; CE(3, q, 2) = F[0];
+        const double a = (c->ee[h] - WE[q]) * dy * dz;                         /* :477-482 */
+        const double b = (c->ne[h] - SE[q]) * dx * dz;
+        const double e = (c->te[h] - BE[q]) * dx * dy;
+        c->wllh[q] = sqrt(a * a + b * b + e * e);
+      }
+  /* shared faces (:487-555) */
+  for (int j = 1; j <= J; ++j)
+    for (int k = 1; k <= K; ++k)
+      for (int i = 2; i <= I; ++i) {
+        const int64_t q = NX(g, i, j, k), qm = NX(g, i - 1, j, k), hm = HX(g, i - 1, j, k);
+        const double w = WE[q], e = c->ee[hm];
+        const double avg = (w * w + e * e) / (w + e + tol);
+        fce[1][hm] = (CE(1, q, 1) * w + CE(0, qm, 1) * e) / (w + e + tol);
+        fce[2][hm] = (CE(1, q, 2) * w + CE(0, qm, 2) * e) / (w + e + tol);
+        fce[0][hm] = CE(0, qm, 0);
+        c->ee[hm] = avg;
+      }
+  for (int i = 1; i <= I; ++i)
+    for (int k = 1; k <= K; ++k)
+      for (int j = 2; j <= J; ++j) {
+        const int64_t q = NX(g, i, j, k), qm = NX(g, i, j - 1, k), hm = HX(g, i, j - 1, k);
+        const double s = SE[q], n = c->ne[hm];
+        const double avg = (s * s + n * n) / (s + n + tol);
+        fcn[0][hm] = (CE(3, q, 0) * s + CE(2, qm, 0) * n) / (s + n + tol);
+        fcn[2][hm] = (CE(3, q, 2) * s + CE(2, qm, 2) * n) / (s + n + tol);
+        fcn[1][hm] = CE(2, qm, 1);
+        c->ne[hm] = avg;
+      }
+  for (int i = 1; i <= I; ++i)
+    for (int j = 1; j <= J; ++j)
+      for (int k = 2; k <= K; ++k) {
+        const int64_t q = NX(g, i, j, k), qm = NX(g, i, j, k - 1), hm = HX(g, i, j, k - 1);
+        const double b = BE[q], t = c->te[hm];
+        const double avg = (b * b + t * t) / (b + t + tol);
+        fct[0][hm] = (CE(5, q, 0) * b + CE(4, qm, 0) * t) / (b + t + tol);
+        /* :541-543 weighs the bottom face's first component here too */
+        fct[1][hm] = (CE(5, q, 0) * b + CE(4, qm, 1) * t) / (b + t + tol);
+        fct[2][hm] = CE(4, qm, 2);
+        c->te[hm] = avg;
+      }
+#undef CE
+  if (demote)                                                                  /* :557-569 */
+    for (int i = 2; i <= I; ++i)
+      for (int j = 2; j <= J; ++j)
+        for (int k = 2; k <= K; ++k) {
+          double mf = c->ee[HX(g, i - 1, j, k)];
+          const double o[5] = {c->ee[HX(g, i, j, k)], c->ne[HX(g, i, j - 1, k)],
+                               c->ne[HX(g, i, j, k)], c->te[HX(g, i, j, k - 1)],
+                               c->te[HX(g, i, j, k)]};
+          for (int m = 0; m < 5; ++m) mf = o[m] > mf ? o[m] : mf;
+          if (mf < 1.0e-2) c->type[NX(g, i, j, k)] = 2;
+        }
+  for (int i = 1; i <= I; ++i)                                                 /* :571-599 */
+    for (int j = 1; j <= J; ++j)
+      for (int k = 1; k <= K; ++k) {
+        const int t = c->type[NX(g, i, j, k)];
+        if (t == 2) {
+          c->ee[HX(g, i - 1, j, k)] = 0.0; c->ee[HX(g, i, j, k)] = 0.0;
+          c->ne[HX(g, i, j - 1, k)] = 0.0; c->ne[HX(g, i, j, k)] = 0.0;
+          c->te[HX(g, i, j, k - 1)] = 0.0; c->te[HX(g, i, j, k)] = 0.0;
+        }
+        if (t == 1) {
+          if (i + 1 <= I && c->type[NX(g, i + 1, j, k)] == 2) c->ee[HX(g, i, j, k)] = 0.0;
+          if (j + 1 <= J && c->type[NX(g, i, j + 1, k)] == 2) c->ne[HX(g, i, j, k)] = 0.0;
+          if (k + 1 <= K && c->type[NX(g, i, j, k + 1)] == 2) c->te[HX(g, i, j, k)] = 0.0;
+        }
+      }
+  /* boundary copies of the face centres, in the order of :601-640 */
+  for (int m = 0; m < 3; ++m)
+    for (int k = 0; k <= K + 1; ++k)
+      for (int j = 0; j <= J + 1; ++j) {
+        fce[m][HX(g, 0, j, k)] = fce[m][HX(g, 1, j, k)];
+        fcn[m][HX(g, 0, j, k)] = fcn[m][HX(g, 1, j, k)];
+        fct[m][HX(g, 0, j, k)] = fct[m][HX(g, 1, j, k)];
+      }
+  for (int m = 0; m < 3; ++m)
+    for (int k = 0; k <= K + 1; ++k)
+      for (int j = 0; j <= J + 1; ++j) fce[m][HX(g, I, j, k)] = fce[m][HX(g, I - 1, j, k)];
+  for (int m = 0; m < 3; ++m)
+    for (int k = 0; k <= K + 1; ++k)
+      for (int i = 0; i <= I + 1; ++i) {
+        fce[m][HX(g, i, 0, k)] = fce[m][HX(g, i, 1, k)];
+        fcn[m][HX(g, i, 0, k)] = fcn[m][HX(g, i, 1, k)];
+        fct[m][HX(g, i, 0, k)] = fct[m][HX(g, i, 1, k)];
+      }
+  for (int m = 0; m < 3; ++m)
+    for (int k = 0; k <= K + 1; ++k)
+      for (int i = 0; i <= I + 1; ++i) fcn[m][HX(g, i, J, k)] = fcn[m][HX(g, i, J - 1, k)];
+  for (int m = 0; m < 3; ++m)
+    for (int j = 0; j <= J + 1; ++j)
+      for (int i = 0; i <= I + 1; ++i) {
+        fce[m][HX(g, i, j, 0)] = fce[m][HX(g, i, j, 1)];
+        fcn[m][HX(g, i, j, 0)] = fcn[m][HX(g, i, j, 1)];
+        fct[m][HX(g, i, j, 0)] = fct[m][HX(g, i, j, 1)];
+      }
+  for (int m = 0; m < 3; ++m)
+    for (int j = 0; j <= J + 1; ++j)
+      for (int i = 0; i <= I + 1; ++i) fct[m][HX(g, i, j, K)] = fct[m][HX(g, i, j, K - 1)];
+  for (int i = 2; i <= I; ++i)                                                 /* :642-666 */
+    for (int j = 2; j <= J; ++j)
+      for (int k = 2; k <= K; ++k) {
+        const int64_t q = NX(g, i, j, k);
+        if (!(fabs(c->vof[q] - 1.0) < CG_EPSI)) continue;
+        const double dx = c->dx[q], dy = c->dy[q], dz = c->dz[q];
+        if (c->ee[HX(g, i - 1, j, k)] == 0.0 || c->ee[HX(g, i, j, k)] == 0.0) {
+          c->wllh[q] = dy * dz; c->delh[q] = dx / 2.0;
+        }
+        if (c->ne[HX(g, i, j - 1, k)] == 0.0 || c->ne[HX(g, i, j, k)] == 0.0) {
+          c->wllh[q] = dx * dz; c->delh[q] = dy / 2.0;
+        }
+        if (c->te[HX(g, i, j, k - 1)] == 0.0 || c->te[HX(g, i, j, k)] == 0.0) {
+          c->wllh[q] = dx * dy; c->delh[q] = dz / 2.0;
+        }
+      }
+  free(WE); free(SE); free(BE);
+  for (int f = 0; f < 6; ++f) free(Ce[f]);
+}
+
+void orc_grid_preprocess(int I, int J, int K, orc_cell* P, orc_cell* U, orc_cell* V,
+                         orc_cell* W) {
+  const orc_grid G = {.I = I, .J = J, .K = K};
+  const orc_grid* g = &G;
+  const int64_t N = (int64_t)I * J * K, Nh = (int64_t)(I + 2) * (J + 2) * (K + 2);
+  orc_cell* cs[4] = {P, U, V, W};
+  for (int m = 0; m < 4; ++m) define_cell(I, J, K, cs[m]);                     /* :73-76 */
+  for (int m = 0; m < 4; ++m) cell_geo_cal(I, J, K, cs[m], m == 0);            /* :80-83 */
+#define F3(a, c_, h) ((a)[(int64_t)(c_) * Nh + (h)])
+  /* face centres of U, V, W on stretched spacing (:89-159) */
+  for (int j = 1; j <= J; ++j)
+    for (int k = 1; k <= K; ++k) {
+      for (int i = 1; i <= I - 1; ++i)
+        if (U->type[NX(g, i, j, k)] != 1 && U->type[NX(g, i + 1, j, k)] != 1 &&
+            fabs(U->dx[NX(g, i, j, k)] - U->dx[NX(g, i + 1, j, k)]) > 1.0e-10) {
+          F3(U->fce, 0, HX(g, i, j, k)) = 0.5 * P->dx[NX(g, i + 1, j, k)];
+          F3(U->fcn, 0, HX(g, i, j, k)) = 0.0;
+          F3(U->fct, 0, HX(g, i, j, k)) = 0.0;
+        }
+      F3(U->fcn, 0, HX(g, I, j, k)) = 0.0;
+      F3(U->fct, 0, HX(g, I, j, k)) = 0.0;
+    }
+  for (int i = 1; i <= I; ++i)
+    for (int k = 1; k <= K; ++k) {
+      for (int j = 1; j <= J - 1; ++j)
+        if (V->type[NX(g, i, j, k)] != 1 && V->type[NX(g, i, j + 1, k)] != 1 &&
+            fabs(V->dy[NX(g, i, j, k)] - V->dy[NX(g, i, j + 1, k)]) > 1.0e-10) {
+          F3(V->fcn, 1, HX(g, i, j, k)) = 0.5 * P->dy[NX(g, i, j + 1, k)];
+          F3(V->fce, 1, HX(g, i, j, k)) = 0.0;
+          F3(V->fct, 1, HX(g, i, j, k)) = 0.0;
+        }
+      F3(V->fce, 1, HX(g, i, J, k)) = 0.0;
+      F3(V->fct, 1, HX(g, i, J, k)) = 0.0;
+    }
+  for (int i = 1; i <= I; ++i)
+    for (int j = 1; j <= J; ++j) {
+      for (int k = 1; k <= K - 1; ++k)
+        if (W->type[NX(g, i, j, k)] != 1 && W->type[NX(g, i, j, k + 1)] != 1 &&
+            fabs(W->dz[NX(g, i, j, k)] - W->dz[NX(g, i, j, k + 1)]) > 1.0e-10) {
+          F3(W->fct, 2, HX(g, i, j, k)) = 0.5 * P->dz[NX(g, i, j, k + 1)];
+          F3(W->fce, 2, HX(g, i, j, k)) = 0.0;
+          F3(W->fcn, 2, HX(g, i, j, k)) = 0.0;
+        }
+      F3(W->fce, 2, HX(g, i, j, K)) = 0.0;
+      F3(W->fcn, 2, HX(g, i, j, K)) = 0.0;
+    }
+  /* U/V/W faces through the P-cell centre (:163-223) */
+  for (int i = 1; i <= I; ++i)
+    for (int j = 1; j <= J; ++j)
+      for (int k = 1; k <= K; ++k) {
+        const int64_t q = NX(g, i, j, k);
+        const int64_t hu = HX(g, i - 1, j, k), hv = HX(g, i, j - 1, k), hw = HX(g, i, j, k - 1);
+        for (int m = 0; m < 3; ++m) P->cent[(int64_t)m * N + q] = 0.0;
+        const double dx = P->dx[q], dy = P->dy[q], dz = P->dz[q], vf = P->vof[q];
+        if (vf < 1.0 && vf > 0.0) {
+          double F[3];
+          orc_face_geo(dy, dz, P->ny[q], P->nz[q], P->nx[q], 0.0, P->phi[q], &U->ee[hu], F);
+          F3(U->fce, 0, hu) = F[2] + dx / 2.0;
+          F3(U->fce, 1, hu) = F[0]; F3(U->fce, 2, hu) = F[1];
+          orc_face_geo(dz, dx, P->nz[q], P->nx[q], P->ny[q], 0.0, P->phi[q], &V->ne[hv], F);
+          F3(V->fcn, 1, hv) = F[2] + dy / 2.0;
+          F3(V->fcn, 0, hv) = F[1]; F3(V->fcn, 2, hv) = F[0];
+          orc_face_geo(dx, dy, P->nx[q], P->ny[q], P->nz[q], 0.0, P->phi[q], &W->te[hw], F);
+          F3(W->fct, 2, hw) = F[2] + dz / 2.0;
+          F3(W->fct, 0, hw) = F[0]; F3(W->fct, 1, hw) = F[1];
+        } else if (vf > 0.5) {
+          F3(U->fce, 0, hu) = dx / 2.0; F3(U->fce, 1, hu) = 0.0; F3(U->fce, 2, hu) = 0.0;
+          U->ee[hu] = 1.0;
+          F3(V->fcn, 1, hv) = dy / 2.0; F3(V->fcn, 0, hv) = 0.0; F3(V->fcn, 2, hv) = 0.0;
+          V->ne[hv] = 1.0;
+          F3(W->fct, 2, hw) = dz / 2.0; F3(W->fct, 0, hw) = 0.0; F3(W->fct, 1, hw) = 0.0;
+          W->te[hw] = 1.0;
+        } else {
+          U->ee[hu] = 0.0; V->ne[hv] = 0.0; W->te[hw] = 0.0;
+          for (int m = 0; m < 3; ++m) {
+            F3(U->fce, m, hu) = 0.0; F3(V->fcn, m, hv) = 0.0; F3(W->fct, m, hw) = 0.0;
+          }
+        }
+      }
+  /* P faces through the U/V/W cell centres and their cell centres (:225-323) */
+  for (int i = 1; i <= I; ++i)
+    for (int j = 1; j <= J; ++j)
+      for (int k = 1; k <= K; ++k) {
+        const int64_t q = NX(g, i, j, k), h = HX(g, i, j, k);
+        double F[3], A;
+        if (U->vof[q] < 1.0 && U->vof[q] > 0.0) {
+          orc_face_geo(U->dy[q], U->dz[q], U->ny[q], U->nz[q], U->nx[q], 0.0, U->phi[q], &A, F);
+          F3(P->fce, 0, h) = F[2] + P->dx[q] / 2.0;
+          F3(P->fce, 1, h) = F[0]; F3(P->fce, 2, h) = F[1];
+          U->cent[q] = F[2]; U->cent[N + q] = F[0]; U->cent[2 * N + q] = F[1];
+        } else if (U->vof[q] > 0.5) {
+          P->ee[h] = 1.0;
+          F3(P->fce, 0, h) = P->dx[q] / 2.0; F3(P->fce, 1, h) = 0.0; F3(P->fce, 2, h) = 0.0;
+          for (int m = 0; m < 3; ++m) U->cent[(int64_t)m * N + q] = 0.0;
+        } else {
+          P->ee[h] = 0.0;
+          for (int m = 0; m < 3; ++m) { F3(P->fce, m, h) = 0.0; U->cent[(int64_t)m * N + q] = 0.0; }
+        }
+        if (V->vof[q] < 1.0 && V->vof[q] > 0.0) {
+          orc_face_geo(V->dz[q], V->dx[q], V->nz[q], V->nx[q], V->ny[q], 0.0, V->phi[q], &A, F);
+          F3(P->fcn, 0, h) = F[1]; F3(P->fcn, 2, h) = F[0];
+          F3(P->fcn, 1, h) = F[2] + P->dy[q] / 2.0;
+          V->cent[q] = F[1]; V->cent[N + q] = F[2]; V->cent[2 * N + q] = F[0];
+        } else if (V->vof[q] > 0.5) {
+          P->ne[h] = 1.0;
+          F3(P->fcn, 0, h) = 0.0; F3(P->fcn, 2, h) = 0.0; F3(P->fcn, 1, h) = P->dy[q] / 2.0;
+          for (int m = 0; m < 3; ++m) V->cent[(int64_t)m * N + q] = 0.0;
+        } else {
+          P->ne[h] = 0.0;
+          for (int m = 0; m < 3; ++m) { F3(P->fcn, m, h) = 0.0; V->cent[(int64_t)m * N + q] = 0.0; }
+        }
+        if (W->vof[q] < 1.0 && W->vof[q] > 0.0) {
+          orc_face_geo(W->dx[q], W->dy[q], W->nx[q], W->ny[q], W->nz[q], 0.0, W->phi[q], &A, F);
+          F3(P->fct, 0, h) = F[0]; F3(P->fct, 1, h) = F[1];
+          F3(P->fct, 2, h) = F[2] + P->dz[q] / 2.0;
+          W->cent[q] = F[0]; W->cent[N + q] = F[1]; W->cent[2 * N + q] = F[2];
+        } else if (W->vof[q] > 0.5) {
+          P->te[h] = 1.0;
+          F3(P->fct, 0, h) = 0.0; F3(P->fct, 1, h) = 0.0; F3(P->fct, 2, h) = P->dz[q] / 2.0;
+          for (int m = 0; m < 3; ++m) W->cent[(int64_t)m * N + q] = 0.0;
+        } else {
+          P->te[h] = 0.0;
+          for (int m = 0; m < 3; ++m) { F3(P->fct, m, h) = 0.0; W->cent[(int64_t)m * N + q] = 0.0; }
+        }
+      }
+#undef F3
+}

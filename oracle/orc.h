@@ -33,6 +33,16 @@
  *                       out of the slicing itself), Dirichlet faces equal to
  *                       their constants, ghost rules, zero divergence of the
  *                       mass-fixed boundary layer for uniform apertures
+ *   orc_face_geo        exact rational clipping of the face rectangle by the
+ *                       half-plane (an independent algorithm), closed forms of
+ *                       full, empty, half and corner-triangle faces
+ *   orc_grid_preprocess planar level sets: shared faces of neighbours agree,
+ *                       WlLh equals the exact cut area of the plane in the
+ *                       cell, U/V/W faces through the P centre equal the exact
+ *                       mid-plane fraction; every face next to a solid cell is
+ *                       zero; DefineCell classes and vofL scaling; the
+ *                       stretched-grid face centre rule and the FCT(:,2)
+ *                       quirk on hand-built cells
  * No reference golden vector exists for this path and HYPRE is absent, so the
  * HYPRE iterate itself is not pinned (any converged solve of the pinned,
  * uniquely solvable system is).
@@ -166,6 +176,25 @@ int orc_adjoint(const orc_grid* g, const double* pocoef, const double* ee, const
 
 /* OpenMP threads the parallel functions use. */
 int orc_num_threads(void);
+
+/* One staggered cell grid of GridPreprocess (CutCell.f90:62-325): TGrid%dx,
+ * dy, dz and the TCell members it reads or writes.  Cell arrays are
+ * (1:I,1:J,1:K); ee/ne/te are halo arrays; fce/fcn/fct are halo arrays x 3
+ * components (component slowest); cent is a cell array x 3. */
+typedef struct orc_cell {
+  const double *dx, *dy, *dz, *phi, *nx, *ny, *nz;
+  double *vof, *vofl;
+  int32_t* type;
+  double *ee, *ne, *te, *fce, *fcn, *fct, *cent, *wllh, *delh;
+} orc_cell;
+
+/* FaceGeoCal (CutCell.f90:883-981). */
+void orc_face_geo(double d1, double d2, double n1, double n2, double n3, double d3, double phi,
+                  double* area, double cent[3]);
+/* GridPreprocess (CutCell.f90:62-325): DefineCell and CellGeoCal on the four
+ * grids, then the face and cell centre corrections of U, V, W and P. */
+void orc_grid_preprocess(int I, int J, int K, orc_cell* P, orc_cell* U, orc_cell* V,
+                         orc_cell* W);
 
 #ifdef __cplusplus
 }

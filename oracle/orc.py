@@ -39,6 +39,16 @@ class OrcStats(ctypes.Structure):
                 ("final_div", ctypes.c_double), ("seconds", ctypes.c_double)]
 
 
+GEOM_FIELDS = ("dx", "dy", "dz", "phi", "nx", "ny", "nz", "vof", "vof_l", "cell_type",
+               "ee_area", "ne_area", "te_area", "fce", "fcn", "fct", "cell_cent", "wllh",
+               "delh")
+GEOM_INPUTS = GEOM_FIELDS[:7]
+
+
+class OrcCell(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in GEOM_FIELDS]
+
+
 _lib = None
 
 
@@ -86,6 +96,11 @@ def lib():
         L.orc_wall_step.argtypes = [G, F64P, F64P, F64P] + [I32P] * 5 + [F64P] * 5 + [
             ctypes.POINTER(F64P)]
         L.orc_wall_step.restype = None
+        L.orc_face_geo.argtypes = [ctypes.c_double] * 7 + [F64P, F64P]
+        L.orc_face_geo.restype = None
+        C = ctypes.POINTER(OrcCell)
+        L.orc_grid_preprocess.argtypes = [ctypes.c_int] * 3 + [C] * 4
+        L.orc_grid_preprocess.restype = None
         _lib = L
     return _lib
 
@@ -374,3 +389,43 @@ def wall_step(case, flags, consts, u, v, w, p):
                         _p(case.cell_type_p), _p(case.cell_type_u), _p(case.cell_type_v),
                         _p(case.cell_type_w), _p(fl), _p(cs), _p(u), _p(v), _p(w), _p(p), vp)
     return u, v, w, p, var
+
+
+def face_geo(d1, d2, n1, n2, n3, d3, phi):
+    """FaceGeoCal: (area fraction, centroid[3])."""
+    a = ctypes.c_double()
+    c = (ctypes.c_double * 3)()
+    lib().orc_face_geo(d1, d2, n1, n2, n3, d3, phi, ctypes.byref(a), c)
+    return a.value, np.array(c[:])
+
+
+def geom_shapes(I, J, K):
+    cell, halo = (I, J, K), (I + 2, J + 2, K + 2)
+    sh = {n: cell for n in GEOM_FIELDS}
+    for n in ("ee_area", "ne_area", "te_area"):
+        sh[n] = halo
+    for n in ("fce", "fcn", "fct"):
+        sh[n] = halo + (3,)
+    sh["cell_cent"] = cell + (3,)
+    return sh
+
+
+def grid_preprocess(cells):
+    """GridPreprocess on copies of four dicts of host arrays (P, U, V, W,
+    keyed like sp_cell_geom).  Returns four dicts with every field."""
+    I, J, K = cells[0]["dx"].shape
+    sh = geom_shapes(I, J, K)
+    outs, structs = [], []
+    for c in cells:
+        o = {}
+        for n in GEOM_FIELDS:
+            dt = np.int32 if n == "cell_type" else np.float64
+            if n in c:
+                o[n] = np.array(c[n], dtype=dt, order="F", copy=True)
+            else:
+                o[n] = np.zeros(sh[n], dtype=dt, order="F")
+            assert o[n].shape == sh[n], (n, o[n].shape)
+        outs.append(o)
+        structs.append(OrcCell(*(o[n].ctypes.data for n in GEOM_FIELDS)))
+    lib().orc_grid_preprocess(I, J, K, *(ctypes.byref(s) for s in structs))
+    return outs

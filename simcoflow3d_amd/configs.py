@@ -152,3 +152,130 @@ def generate(cfg, size=None, seed=0, dt=1.0e-3):
                           dyv=arrs["dyv"], dzw=arrs["dzw"], xc=arrs["xc"], yc=arrs["yc"],
                           zc=arrs["zc"], fields=fields,
                           bc_flag=np.ones(6, dtype=np.int32))
+
+
+# ---------------------------------------------------------------------------
+# Cut-cell geometry inputs (GridPreprocess, CutCell.f90:62-325)
+# ---------------------------------------------------------------------------
+
+GEOM_SHAPES = ("sphere", "plane", "cylinder", "bodies", "axis_plane")
+
+
+def _centres(d):
+    return np.cumsum(d) - 0.5 * d
+
+
+def staggered_spacings(dxp, dyp, dzp):
+    """Cell centres and 3-D spacing arrays of the P, U, V, W grids as the
+    reference builds them (Mesh.f90:160-247): a U cell is centred on the
+    east face of its P cell, its width is the distance to the next P centre
+    (the P width in the last cell); V and W likewise."""
+    xc, yc, zc = _centres(dxp), _centres(dyp), _centres(dzp)
+    I, J, K = len(dxp), len(dyp), len(dzp)
+
+    def stag(c, d):
+        w = np.empty_like(d)
+        w[:-1] = c[1:] - c[:-1]
+        w[-1] = d[-1]
+        return c + 0.5 * d, w
+
+    xu, dxu = stag(xc, dxp)
+    yv, dyv = stag(yc, dyp)
+    zw, dzw = stag(zc, dzp)
+    grids = []
+    for (x, dx), (y, dy), (z, dz) in (((xc, dxp), (yc, dyp), (zc, dzp)),
+                                      ((xu, dxu), (yc, dyp), (zc, dzp)),
+                                      ((xc, dxp), (yv, dyv), (zc, dzp)),
+                                      ((xc, dxp), (yc, dyp), (zw, dzw))):
+        X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+        DX, DY, DZ = np.meshgrid(dx, dy, dz, indexing="ij")
+        grids.append(dict(x=np.asfortranarray(X), y=np.asfortranarray(Y), z=np.asfortranarray(Z),
+                          dx=np.asfortranarray(DX), dy=np.asfortranarray(DY),
+                          dz=np.asfortranarray(DZ)))
+    assert all(g["dx"].shape == (I, J, K) for g in grids)
+    return grids
+
+
+def solid_level_set(shape, X, Y, Z, rng):
+    """Signed distance to a solid (positive in the fluid) and its unit
+    normal (the gradient, pointing into the fluid)."""
+    if shape == "sphere":
+        c = np.array([0.5, 0.5, 0.45]) + rng.uniform(-0.05, 0.05, 3)
+        r = 0.3
+        dX, dY, dZ = X - c[0], Y - c[1], Z - c[2]
+        rho = np.sqrt(dX * dX + dY * dY + dZ * dZ)
+        rho = np.where(rho > 0.0, rho, 1.0)
+        return rho - r, dX / rho, dY / rho, dZ / rho
+    if shape in ("plane", "axis_plane"):
+        if shape == "plane":
+            n = rng.normal(size=3)
+            n /= np.linalg.norm(n)
+        else:
+            n = np.zeros(3)
+            n[rng.integers(0, 3)] = 1.0 if rng.uniform() < 0.5 else -1.0
+        d = float(n @ np.array([0.5, 0.5, 0.5])) + rng.uniform(-0.1, 0.1)
+        phi = n[0] * X + n[1] * Y + n[2] * Z - d
+        ones = np.ones_like(X)
+        return phi, n[0] * ones, n[1] * ones, n[2] * ones
+    if shape == "cylinder":
+        yc, zc = 0.5 + rng.uniform(-0.05, 0.05, 2)
+        dY, dZ = Y - yc, Z - zc
+        rho = np.sqrt(dY * dY + dZ * dZ)
+        rho = np.where(rho > 0.0, rho, 1.0)
+        return rho - 0.2, np.zeros_like(X), dY / rho, dZ / rho
+    if shape == "bodies":
+        best = None
+        for _ in range(3):
+            c = rng.uniform(0.2, 0.8, 3)
+            r = rng.uniform(0.08, 0.2)
+            dX, dY, dZ = X - c[0], Y - c[1], Z - c[2]
+            rho = np.sqrt(dX * dX + dY * dY + dZ * dZ)
+            rho = np.where(rho > 0.0, rho, 1.0)
+            cand = (rho - r, dX / rho, dY / rho, dZ / rho)
+            if best is None:
+                best = cand
+            else:
+                take = cand[0] < best[0]
+                best = tuple(np.where(take, a, b) for a, b in zip(cand, best))
+        return best
+    raise ValueError(f"unknown solid shape {shape}")
+
+
+def geometry_case(size, shape="sphere", seed=0, stretch=False, edge_values=False):
+    """Inputs of GridPreprocess for the P, U, V, W grids on [0,1]^3: a list
+    of four dicts keyed like sp_cell_geom (Fortran-ordered arrays).  The
+    solid vof is a smooth function of the level set; the arrays the
+    reference updates in place (vof, vof_l, cell_type, fce/fcn/fct, delh)
+    hold seeded values.  ``stretch`` grows dz by 2% per cell;
+    ``edge_values`` sets some vof to the DefineCell thresholds, 0, 1 and NaN."""
+    I, J, K = size
+    rng = np.random.default_rng(seed)
+    dxp = np.full(I, 1.0 / I)
+    dyp = np.full(J, 1.0 / J)
+    dzp = np.full(K, 1.0 / K)
+    if stretch:
+        dzp = 1.02 ** np.arange(K)
+        dzp /= dzp.sum()
+    grids = staggered_spacings(dxp, dyp, dzp)
+    hs = (I + 2, J + 2, K + 2)
+    cells = []
+    shape_seed = int(rng.integers(0, 2**31))
+    for g in grids:
+        phi, nx, ny, nz = solid_level_set(shape, g["x"], g["y"], g["z"],
+                                          np.random.default_rng(shape_seed))
+        h = np.abs(nx) * g["dx"] + np.abs(ny) * g["dy"] + np.abs(nz) * g["dz"]
+        vof = np.clip(0.5 + phi / h, 0.0, 1.0)
+        if edge_values:
+            sel = rng.uniform(size=vof.shape) < 0.02
+            vals = np.array([1.0e-3, 1.0 - 1.0e-3, 0.0, 1.0, 0.5, np.nan])
+            vof = np.where(sel, vals[rng.integers(0, len(vals), vof.shape)], vof)
+        c = dict(dx=g["dx"], dy=g["dy"], dz=g["dz"])
+        for n, a in (("phi", phi), ("nx", nx), ("ny", ny), ("nz", nz), ("vof", vof)):
+            c[n] = np.asfortranarray(a, dtype=np.float64)
+        c["vof_l"] = np.asfortranarray(vof * rng.uniform(0.0, 1.0, vof.shape))
+        c["cell_type"] = np.asfortranarray(rng.integers(0, 3, (I, J, K)).astype(np.int32))
+        for n in ("fce", "fcn", "fct"):
+            c[n] = np.asfortranarray(rng.uniform(-1.0, 1.0, hs + (3,)))
+        c["delh"] = np.asfortranarray(rng.uniform(0.0, 1.0, (I, J, K)))
+        cells.append(c)
+    return cells

@@ -748,6 +748,103 @@ sp_status sp_write_vtr(sp_ctx* c, const char* dir, const char* prefix, int64_t i
   return s;
 }
 
+static sp_status check_geom(sp_ctx* c, const sp_cell_geom* const g[4]) {
+  for (int m = 0; m < 4; ++m) {
+    if (!g[m]) return fail(c, SP_ERR_INVALID_ARG, "null sp_cell_geom");
+    const sp_cell_geom& a = *g[m];
+    const void* req[] = {a.dx, a.dy, a.dz, a.phi, a.nx, a.ny, a.nz, a.vof, a.vof_l,
+                         a.cell_type, a.ee_area, a.ne_area, a.te_area, a.fce, a.fcn, a.fct,
+                         a.cell_cent, a.wllh, a.delh};
+    for (const void* p : req)
+      if (!p) return fail(c, SP_ERR_INVALID_ARG, "sp_cell_geom member is null");
+  }
+  return SP_OK;
+}
+
+sp_status sp_grid_preprocess_dev(sp_ctx* c, const sp_cell_geom* p, const sp_cell_geom* u,
+                                 const sp_cell_geom* v, const sp_cell_geom* w) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  const sp_cell_geom* g[4] = {p, u, v, w};
+  sp_status s = check_geom(c, g);
+  if (s) return s;
+  if ((s = grid_preprocess(c, g))) return s;
+  const hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "sp_grid_preprocess");
+  timers_collect(c);
+  return SP_OK;
+}
+
+sp_status sp_grid_preprocess(sp_ctx* c, const sp_cell_geom* p, const sp_cell_geom* u,
+                             const sp_cell_geom* v, const sp_cell_geom* w) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  const sp_cell_geom* g[4] = {p, u, v, w};
+  sp_status s = check_geom(c, g);
+  if (s) return s;
+  const size_t N = (size_t)c->dm.N, Nh = (size_t)c->dm.Nh;
+  // staging slots past those of the projection, adjoint and wall step
+  size_t k = 100;
+  sp_cell_geom d[4];
+  for (int m = 0; m < 4; ++m) {
+    const sp_cell_geom& h = *g[m];
+    sp_cell_geom& o = d[m];
+    const double* t;
+    const int32_t* ti;
+    if ((s = upload(c, k++, h.dx, N, &o.dx)) || (s = upload(c, k++, h.dy, N, &o.dy)) ||
+        (s = upload(c, k++, h.dz, N, &o.dz)) || (s = upload(c, k++, h.phi, N, &o.phi)) ||
+        (s = upload(c, k++, h.nx, N, &o.nx)) || (s = upload(c, k++, h.ny, N, &o.ny)) ||
+        (s = upload(c, k++, h.nz, N, &o.nz)))
+      return s;
+    // read-write arrays: uploaded, written back below
+    if ((s = upload(c, k++, static_cast<const double*>(h.vof), N, &t))) return s;
+    o.vof = const_cast<double*>(t);
+    if ((s = upload(c, k++, static_cast<const double*>(h.vof_l), N, &t))) return s;
+    o.vof_l = const_cast<double*>(t);
+    if ((s = upload(c, k++, static_cast<const int32_t*>(h.cell_type), N, &ti))) return s;
+    o.cell_type = const_cast<int32_t*>(ti);
+    if ((s = upload(c, k++, static_cast<const double*>(h.fce), 3 * Nh, &t))) return s;
+    o.fce = const_cast<double*>(t);
+    if ((s = upload(c, k++, static_cast<const double*>(h.fcn), 3 * Nh, &t))) return s;
+    o.fcn = const_cast<double*>(t);
+    if ((s = upload(c, k++, static_cast<const double*>(h.fct), 3 * Nh, &t))) return s;
+    o.fct = const_cast<double*>(t);
+    if ((s = upload(c, k++, static_cast<const double*>(h.delh), N, &t))) return s;
+    o.delh = const_cast<double*>(t);
+    // write-only arrays
+    void* b;
+    if ((s = stage(c, k++, Nh * sizeof(double), &b))) return s;
+    o.ee_area = static_cast<double*>(b);
+    if ((s = stage(c, k++, Nh * sizeof(double), &b))) return s;
+    o.ne_area = static_cast<double*>(b);
+    if ((s = stage(c, k++, Nh * sizeof(double), &b))) return s;
+    o.te_area = static_cast<double*>(b);
+    if ((s = stage(c, k++, 3 * N * sizeof(double), &b))) return s;
+    o.cell_cent = static_cast<double*>(b);
+    if ((s = stage(c, k++, N * sizeof(double), &b))) return s;
+    o.wllh = static_cast<double*>(b);
+  }
+  const sp_cell_geom* dg[4] = {&d[0], &d[1], &d[2], &d[3]};
+  if ((s = grid_preprocess(c, dg))) return s;
+  for (int m = 0; m < 4; ++m) {
+    const sp_cell_geom& h = *g[m];
+    const sp_cell_geom& o = d[m];
+    const struct { void* dst; const void* src; size_t bytes; } back[] = {
+        {h.vof, o.vof, N * 8},           {h.vof_l, o.vof_l, N * 8},
+        {h.cell_type, o.cell_type, N * 4}, {h.ee_area, o.ee_area, Nh * 8},
+        {h.ne_area, o.ne_area, Nh * 8},  {h.te_area, o.te_area, Nh * 8},
+        {h.fce, o.fce, 3 * Nh * 8},      {h.fcn, o.fcn, 3 * Nh * 8},
+        {h.fct, o.fct, 3 * Nh * 8},      {h.cell_cent, o.cell_cent, 3 * N * 8},
+        {h.wllh, o.wllh, N * 8},         {h.delh, o.delh, N * 8}};
+    for (const auto& x : back)
+      if ((s = pipe_download(c, x.dst, x.src, x.bytes))) return s;
+  }
+  const hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "sp_grid_preprocess");
+  timers_collect(c);
+  return SP_OK;
+}
+
 void* sp_stream(sp_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 void sp_set_profiling(sp_ctx* c, int on) {

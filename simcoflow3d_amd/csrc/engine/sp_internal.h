@@ -238,6 +238,8 @@ sp_status correct(sp_ctx* c, const sp_inputs& in, sp_outputs& out, sp_stats* st)
 // Updates out.u, v, w, p in place; `host_var`: out.bc_var are host pointers.
 sp_status wall_step(sp_ctx* c, const sp_inputs& in, const sp_outputs& out, bool host_var);
 
+// GridPreprocess on the four device grids P, U, V, W (sp_cutcell.hip).
+sp_status grid_preprocess(sp_ctx* c, const sp_cell_geom* const g[4]);
 // ---- VTR output (sp_vtr.hip) ----
 sp_status write_vtr(sp_ctx* c, const char* dir, const char* prefix, int64_t itt,
                     const sp_vtr_fields& f, bool device);

@@ -34,7 +34,28 @@ EXPORTED_SYMBOLS = (
     "sp_set_grid_1d", "sp_project", "sp_project_dev", "sp_adjoint", "sp_adjoint_dev",
     "sp_stream", "sp_kernel_times", "sp_set_profiling", "sp_set_profiling_filter",
     "sp_strerror", "sp_last_error", "sp_wall_step", "sp_wall_step_dev", "sp_write_vtr",
+    "sp_grid_preprocess", "sp_grid_preprocess_dev",
 )
+
+# members of sp_cell_geom, in order
+GEOM_FIELDS = ("dx", "dy", "dz", "phi", "nx", "ny", "nz", "vof", "vof_l", "cell_type",
+               "ee_area", "ne_area", "te_area", "fce", "fcn", "fct", "cell_cent", "wllh",
+               "delh")
+GEOM_INPUTS = GEOM_FIELDS[:7]
+GEOM_UPDATED = ("vof", "vof_l", "cell_type", "fce", "fcn", "fct", "delh")
+GEOM_WRITTEN = ("ee_area", "ne_area", "te_area", "cell_cent", "wllh")
+
+
+def geom_shapes(I, J, K):
+    """Fortran shapes of the sp_cell_geom arrays for an I x J x K grid."""
+    cell, halo = (I, J, K), (I + 2, J + 2, K + 2)
+    sh = {n: cell for n in GEOM_FIELDS}
+    for n in ("ee_area", "ne_area", "te_area"):
+        sh[n] = halo
+    for n in ("fce", "fcn", "fct"):
+        sh[n] = halo + (3,)
+    sh["cell_cent"] = cell + (3,)
+    return sh
 
 
 class SpParams(ctypes.Structure):
@@ -123,6 +144,10 @@ class SpKernelTime(ctypes.Structure):
                 ("total_ms", ctypes.c_double), ("bytes", ctypes.c_double)]
 
 
+class SpCellGeom(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in GEOM_FIELDS]
+
+
 class SpError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{msg} (status {status})")
@@ -169,6 +194,10 @@ def load():
     L.sp_write_vtr.argtypes = [V, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64,
                                P(SpVtrFields), ctypes.c_int]
     L.sp_write_vtr.restype = ctypes.c_int
+    for name in ("sp_grid_preprocess", "sp_grid_preprocess_dev"):
+        f = getattr(L, name)
+        f.argtypes = [V] + [P(SpCellGeom)] * 4
+        f.restype = ctypes.c_int
     L.sp_stream.argtypes = [V]
     L.sp_stream.restype = V
     L.sp_kernel_times.argtypes = [V, P(SpKernelTime), ctypes.c_int]
@@ -331,6 +360,44 @@ class Projector:
         rc = self.lib.sp_wall_step(self.h, ctypes.byref(inp), ctypes.byref(o), ctypes.byref(st))
         self._check(rc, "sp_wall_step")
         return out, st
+
+    def grid_preprocess(self, cells, device=False):
+        """GridPreprocess (sp_grid_preprocess) on the P, U, V, W grids.
+        ``cells``: four dicts keyed like sp_cell_geom.  Host (numpy) inputs
+        are copied; the result is four dicts with the updated and written
+        arrays.  With device=True every member is a torch tensor on the
+        context's device (Fortran shape reversed, C-contiguous), updated in
+        place; nothing is returned."""
+        I, J, K = self.shape
+        sh = geom_shapes(I, J, K)
+        structs = []
+        if device:
+            for c in cells:
+                for n in GEOM_FIELDS:
+                    t = c[n]
+                    if tuple(t.shape) != sh[n][::-1] or not t.is_contiguous():
+                        raise ValueError(f"{n}: expected a contiguous tensor of shape "
+                                         f"{sh[n][::-1]}, got {tuple(t.shape)}")
+                structs.append(SpCellGeom(*(_addr(c[n]) for n in GEOM_FIELDS)))
+            rc = self.lib.sp_grid_preprocess_dev(self.h, *(ctypes.byref(s) for s in structs))
+            self._check(rc, "sp_grid_preprocess_dev")
+            return None
+        outs = []
+        for c in cells:
+            o = {}
+            for n in GEOM_FIELDS:
+                dt = np.int32 if n == "cell_type" else np.float64
+                if n in GEOM_WRITTEN and n not in c:
+                    o[n] = np.zeros(sh[n], dtype=dt, order="F")
+                else:
+                    o[n] = np.array(c[n], dtype=dt, order="F", copy=True)
+                if o[n].shape != sh[n]:
+                    raise ValueError(f"{n}: expected shape {sh[n]}, got {o[n].shape}")
+            outs.append(o)
+            structs.append(SpCellGeom(*(_addr(o[n]) for n in GEOM_FIELDS)))
+        rc = self.lib.sp_grid_preprocess(self.h, *(ctypes.byref(s) for s in structs))
+        self._check(rc, "sp_grid_preprocess")
+        return outs
 
     def write_vtr(self, directory, prefix, itt, fields, device=False):
         """PrintResultVTR3D's file (sp_write_vtr): fields maps x, y, z (1-D) and
