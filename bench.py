@@ -58,6 +58,11 @@ def parse():
     ap.add_argument("--stream-probe", action="store_true",
                     help="measure the practical HBM bandwidth of plain streaming kernels "
                          "(torch copy and a*x+y on 2 GiB FP64 arrays) and report it")
+    ap.add_argument("--geometry", action="store_true",
+                    help="also time GridPreprocess (cut-cell geometry of the P, U, V, W grids) "
+                         "at the same grid size on a three-sphere solid, with its CPU baseline")
+    ap.add_argument("--geometry-only", action="store_true",
+                    help="time only GridPreprocess (no projection) and print its record")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 1 GiB device-to-device copy (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -302,6 +307,99 @@ def write_vtr(P, dev, out, case, directory):
                 format_gbs=round(r["bytes"] / (r["total_ms"] * 1e-3) / 1e9, 1))
 
 
+def geometry(shape, steps, warmup, cpu=True):
+    """GridPreprocess (sp_grid_preprocess_dev) on device-resident arrays of
+    the four staggered grids, a stretched grid with three spherical solids.
+    Timed calls of the whole preprocessing (all launches of the four grids);
+    per-kernel GB/s from one profiled call; the oracle timed on a bounded
+    sample of the same workload as the CPU baseline."""
+    import numpy as np
+    import torch
+
+    from simcoflow3d_amd import configs
+    from simcoflow3d_amd.engine import GEOM_FIELDS, Projector, geom_shapes
+
+    I, J, K = shape
+    tg = time.perf_counter()
+    cells = configs.geometry_case(shape, "bodies", seed=5, stretch=True)
+    sh = geom_shapes(I, J, K)
+    dev = []
+    for c in cells:
+        d = {}
+        for n in GEOM_FIELDS:
+            if n in c:
+                d[n] = torch.from_numpy(np.ascontiguousarray(c[n].T)).to("cuda")
+            else:
+                dt = torch.int32 if n == "cell_type" else torch.float64
+                d[n] = torch.zeros(sh[n][::-1], dtype=dt, device="cuda")
+        dev.append(d)
+    # pristine copies of the arrays the call updates, restored before each call
+    keep = [{n: d[n].clone() for n in ("vof", "vof_l", "cell_type")} for d in dev]
+    del cells
+    torch.cuda.synchronize()
+    print(f"[bench] geometry inputs on the device ({time.perf_counter() - tg:.1f} s)",
+          file=sys.stderr, flush=True)
+
+    def restore():
+        for d, k in zip(dev, keep):
+            for n, t in k.items():
+                d[n].copy_(t)
+
+    P = Projector(shape)
+    for _ in range(warmup):
+        restore()
+        P.grid_preprocess(dev, device=True)
+    times = []
+    for _ in range(steps):
+        restore()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        P.grid_preprocess(dev, device=True)  # synchronises its stream
+        times.append(time.perf_counter() - t0)
+    restore()
+    P.set_profiling(True)
+    P.grid_preprocess(dev, device=True)
+    rows = P.kernel_times()
+    P.set_profiling(False)
+    P.close()
+    N = I * J * K
+    kern = []
+    for r in rows:
+        ms = r["total_ms"]
+        gbs = r["bytes"] / (ms * 1e-3) / 1e9 if ms > 0 and r["bytes"] > 0 else None
+        kern.append(dict(name=r["name"], launches=r["launches"], total_ms=round(ms, 3),
+                         avg_us=round(ms / max(r["launches"], 1) * 1e3, 1),
+                         bytes_per_launch=r["bytes"] / max(r["launches"], 1),
+                         gbs=round(gbs, 1) if gbs else None))
+    kern.sort(key=lambda t: -t["total_ms"])
+    el = sum(times) / len(times)
+    rec = dict(workload=f"GridPreprocess of the P, U, V, W grids, {I}x{J}x{K}, three spheres, "
+                        "z stretched 2% per cell",
+               calls=steps, ms_per_call=round(el * 1e3, 3),
+               median_ms=round(median(times) * 1e3, 3),
+               cells_per_s=round(N / el, 1), kernels=kern)
+    del dev, keep
+    torch.cuda.empty_cache()
+    if cpu:
+        from oracle import orc
+        # bounded sample: the oracle on a 96^3 case of the same workload,
+        # repeated until about 10 s of CPU work
+        sample = (96, 96, 96)
+        sc = configs.geometry_case(sample, "bodies", seed=5, stretch=True)
+        runs, secs = 0, 0.0
+        while runs == 0 or (secs < 10.0 and runs < 20):
+            t0 = time.perf_counter()
+            orc.grid_preprocess(sc)
+            secs += time.perf_counter() - t0
+            runs += 1
+        ns = sample[0] * sample[1] * sample[2]
+        rec["cpu_baseline"] = dict(value=round(runs * ns / secs, 1), unit="cells/s", cores=1,
+                                   kind="port", seconds=round(secs, 3),
+                                   sample=f"{runs} oracle GridPreprocess calls on the same "
+                                          f"workload at {sample[0]}^3")
+    return rec
+
+
 def all_configs(args):
     """Configs 0-4 at full size: one JSON line each."""
     import torch
@@ -378,6 +476,11 @@ def main():
 
     cfg = args.config
     size = tuple(args.size) if args.size else None
+    if args.geometry_only:
+        gshape = size or configs.config_dims(cfg)
+        print(json.dumps(dict(geometry=geometry(gshape, args.steps, args.warmup,
+                                                cpu=not args.no_cpu))), flush=True)
+        return
     tg = time.perf_counter()
     case = configs.generate(cfg, size=size)
     gen_s = time.perf_counter() - tg
@@ -409,6 +512,13 @@ def main():
     vtr = write_vtr(Pm, dev, out, case, args.vtr) if args.vtr else None
     stream = stream_probe() if args.stream_probe else None
     Pm.close()
+    geo = None
+    if args.geometry:
+        del dev, out
+        torch.cuda.empty_cache()
+        geo = geometry(case.shape, args.steps, args.warmup, cpu=not args.no_cpu)
+        dev = to_device(case)
+        out = make_outputs(case, dev)
 
     fp64 = {}
     if not args.no_fp64:
@@ -488,6 +598,7 @@ def main():
         "forward_adjoint": fwd_adj,
         "vtr": vtr,
         "stream_probe": stream,
+        "geometry": geo,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": [dict(name=k["name"], launches=k["launches"], total_ms=round(k["total_ms"], 3),

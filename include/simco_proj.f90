@@ -15,10 +15,11 @@ module simco_proj
   implicit none
   private
   public :: sp_params, sp_inputs, sp_outputs, sp_stats, sp_adj_inputs, sp_adj_outputs, &
-            sp_wall_bc, sp_vtr_fields
+            sp_wall_bc, sp_vtr_fields, sp_cell_geom
   public :: sp_params_default, sp_create, sp_destroy, sp_set_params, sp_set_grid, &
             sp_set_grid_1d, sp_project, sp_project_dev, sp_adjoint, sp_adjoint_dev, &
-            sp_wall_step, sp_wall_step_dev, sp_write_vtr, sp_strerror_f, sp_last_error_f
+            sp_wall_step, sp_wall_step_dev, sp_write_vtr, sp_grid_preprocess, &
+            sp_grid_preprocess_dev, sp_strerror_f, sp_last_error_f
   public :: SP_OK, SP_ERR_INVALID_ARG, SP_ERR_HIP, SP_ERR_NOT_CONVERGED, SP_ERR_NONFINITE, &
             SP_ERR_OOM, SP_ERR_GRID, SP_ERR_STATE, SP_ERR_NO_DEVICE, SP_PREC_FP64, &
             SP_PREC_MIXED, SP_PRECOND_MG, SP_PRECOND_JACOBI
@@ -96,6 +97,19 @@ module simco_proj
     type(c_ptr) :: mres = c_null_ptr
     integer(c_int32_t) :: halo_mask = 0
   end type sp_vtr_fields
+
+  ! One staggered grid of GridPreprocess (CutCell.f90:62-325): c_loc of
+  ! TGrid%dx, dy, dz and of the TCell members (allocated as in AllocateVar,
+  ! Main.f90:229-399).  fce/fcn/fct are TCell%FCE/FCN/FCT(0:Imax+1,0:Jmax+1,
+  ! 0:Kmax+1,3), cell_cent is TCell%Cell_Cent(Imax,Jmax,Kmax,3).
+  type, bind(C) :: sp_cell_geom
+    type(c_ptr) :: dx = c_null_ptr, dy = c_null_ptr, dz = c_null_ptr
+    type(c_ptr) :: phi = c_null_ptr, nx = c_null_ptr, ny = c_null_ptr, nz = c_null_ptr
+    type(c_ptr) :: vof = c_null_ptr, vof_l = c_null_ptr, cell_type = c_null_ptr
+    type(c_ptr) :: ee_area = c_null_ptr, ne_area = c_null_ptr, te_area = c_null_ptr
+    type(c_ptr) :: fce = c_null_ptr, fcn = c_null_ptr, fct = c_null_ptr
+    type(c_ptr) :: cell_cent = c_null_ptr, wllh = c_null_ptr, delh = c_null_ptr
+  end type sp_cell_geom
 
   type, bind(C) :: sp_adj_inputs
     type(c_ptr) :: gu = c_null_ptr, gv = c_null_ptr, gw = c_null_ptr
@@ -204,6 +218,20 @@ module simco_proj
       type(sp_vtr_fields), intent(in) :: fields
       integer(c_int), value :: device
     end function sp_write_vtr
+
+    integer(c_int) function sp_grid_preprocess(ctx, p, u, v, w) &
+        bind(C, name="sp_grid_preprocess")
+      import :: c_ptr, c_int, sp_cell_geom
+      type(c_ptr), value :: ctx
+      type(sp_cell_geom), intent(in) :: p, u, v, w
+    end function sp_grid_preprocess
+
+    integer(c_int) function sp_grid_preprocess_dev(ctx, p, u, v, w) &
+        bind(C, name="sp_grid_preprocess_dev")
+      import :: c_ptr, c_int, sp_cell_geom
+      type(c_ptr), value :: ctx
+      type(sp_cell_geom), intent(in) :: p, u, v, w
+    end function sp_grid_preprocess_dev
 
     type(c_ptr) function sp_strerror_c(s) bind(C, name="sp_strerror")
       import :: c_ptr, c_int

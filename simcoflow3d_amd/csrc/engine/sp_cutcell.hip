@@ -116,8 +116,8 @@ __device__ inline double face_geo(double d1, double d2, double n1, double n2, do
   return area;
 }
 
-// The west, south and bottom faces of cell q (used by the cell itself and
-// by its -1 neighbour for the shared-face average).
+// The west and south faces of cell q, as its -1 neighbour needs them for
+// the shared-face average.
 __device__ inline double west_face(const GeoGrid& G, int64_t q, double F[3]) {
 #pragma clang fp contract(off)
   return face_geo(G.dy[q], G.dz[q], G.ny[q], G.nz[q], G.nx[q], -G.dx[q] / 2.0, G.phi[q], F);
@@ -126,87 +126,115 @@ __device__ inline double south_face(const GeoGrid& G, int64_t q, double F[3]) {
 #pragma clang fp contract(off)
   return face_geo(G.dz[q], G.dx[q], G.nz[q], G.nx[q], G.ny[q], -G.dy[q] / 2.0, G.phi[q], F);
 }
-__device__ inline double bottom_face(const GeoGrid& G, int64_t q, double F[3]) {
-#pragma clang fp contract(off)
-  return face_geo(G.dx[q], G.dy[q], G.nx[q], G.ny[q], G.nz[q], -G.dz[q] / 2.0, G.phi[q], F);
+
+// Cells per thread along k in k_geo_cells.
+constexpr int kGeoChunk = 32;
+
+struct CellIn {
+  double dx, dy, dz, phi, n1, n2, n3;
+};
+
+__device__ inline CellIn load_cell(const GeoGrid& G, int64_t q) {
+  return CellIn{G.dx[q], G.dy[q], G.dz[q], G.phi[q], G.nx[q], G.ny[q], G.nz[q]};
 }
 
-// DefineCell and the face pass of CellGeoCal (:387-555).  One thread per
-// entry of the halo array: halo entries get the initial aperture 1
-// (:394-396); an interior cell classifies itself, computes its six faces
-// and WlLh, and writes the averaged aperture and centre of its E, N, T
-// faces from its own side and its +1 neighbour's W, S, B side.  The last
+// Initial aperture 1 on the halo entries of EEArea/NEArea/TEArea (:394-396):
+// blockIdx.z picks one of the six boundary planes of the halo box (edges
+// and corners are written by two or three planes, with the same value).
+__global__ void __launch_bounds__(256) k_geo_halo(GeoDims D, GeoGrid G) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y * blockDim.y + threadIdx.y;
+  const int f = blockIdx.z;
+  const int ax = f >> 1;
+  const int na = ax == 0 ? D.SY : D.SX;
+  const int nb = ax == 2 ? D.SY : D.K + 2;
+  if (a >= na || b >= nb) return;
+  const int pl = (f & 1) ? (ax == 0 ? D.I + 1 : (ax == 1 ? D.J + 1 : D.K + 1)) : 0;
+  const int64_t h = ax == 0 ? hx(D, pl, a, b) : (ax == 1 ? hx(D, a, pl, b) : hx(D, a, b, pl));
+  G.ee[h] = 1.0;
+  G.ne[h] = 1.0;
+  G.te[h] = 1.0;
+}
+
+// DefineCell and the face pass of CellGeoCal (:387-555) for the interior.
+// A thread walks kGeoChunk cells up a k column: each cell classifies
+// itself, computes its faces and WlLh, and writes the averaged aperture and
+// centre of its E, N, T faces from its own side and its +1 neighbour's W,
+// S, B side.  The +1 neighbour along k is the next cell of the walk, so its
+// inputs are loaded once and its bottom face, computed for the average, is
+// the next cell's own bottom face (8 face clippings per cell).  The last
 // cell of a line keeps its raw E (N, T) aperture: the averaging loops start
 // at 2 and write index i-1.
 __global__ void __launch_bounds__(256) k_geo_cells(GeoDims D, GeoGrid G) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  if (i > D.I + 1 || j > D.J + 1) return;
-  const int64_t h = hx(D, i, j, k);
-  if (i == 0 || j == 0 || k == 0 || i == D.I + 1 || j == D.J + 1 || k == D.K + 1) {
-    G.ee[h] = 1.0;
-    G.ne[h] = 1.0;
-    G.te[h] = 1.0;
-    return;
-  }
+  // XCD-aware block order: the blocks of one XCD cover a contiguous range
+  // of rows, so the j+1 neighbour row and the cache lines shared by two
+  // blocks of a row stay in that XCD's L2
+  uint32_t bx, by, bz;
+  xcd_block3(bx, by, bz);
+  const int i = bx * blockDim.x + threadIdx.x + 1;
+  const int j = by * blockDim.y + threadIdx.y + 1;
+  const int k0 = bz * kGeoChunk + 1;
+  if (i > D.I || j > D.J) return;
+  const int k1 = min(D.K, k0 + kGeoChunk - 1);
   const double tol = 1.0e-20;
-  const int64_t q = nx_(D, i, j, k), N = D.N, Nh = D.Nh;
-  // DefineCell
-  double v = G.vof[q], vl = G.vofl[q];
-  int32_t ty = G.type[q];
-  if (v >= 1.0 - kEpsi) { ty = 0; vl = vl / v; v = 1.0; }
-  if (v <= kEpsi) { ty = 2; v = 0.0; vl = 0.0; }
-  if (v >= kEpsi && v <= 1.0 - kEpsi) ty = 1;
-  G.vof[q] = v;
-  G.vofl[q] = vl;
-  G.type[q] = ty;
-  G.cent[q] = 0.0;
-  G.cent[N + q] = 0.0;
-  G.cent[2 * N + q] = 0.0;
-  const double dx = G.dx[q], dy = G.dy[q], dz = G.dz[q], ph = G.phi[q];
-  const double n1 = G.nx[q], n2 = G.ny[q], n3 = G.nz[q];
-  double Ft[3], Fb[3], Fe[3], Fw[3], Fn[3], Fs[3];
-  const double TE = face_geo(dx, dy, n1, n2, n3, dz / 2.0, ph, Ft);
-  const double BE = face_geo(dx, dy, n1, n2, n3, -dz / 2.0, ph, Fb);
-  const double EE = face_geo(dy, dz, n2, n3, n1, dx / 2.0, ph, Fe);
-  const double WE = face_geo(dy, dz, n2, n3, n1, -dx / 2.0, ph, Fw);
-  const double NE = face_geo(dz, dx, n3, n1, n2, dy / 2.0, ph, Fn);
-  const double SE = face_geo(dz, dx, n3, n1, n2, -dy / 2.0, ph, Fs);
-  const double a = (EE - WE) * dy * dz;
-  const double b = (NE - SE) * dx * dz;
-  const double e = (TE - BE) * dx * dy;
-  G.wllh[q] = sqrt(a * a + b * b + e * e);
-  if (i < D.I) {  // CeE = (F3, F1, F2), CeW of the neighbour likewise
-    double F[3];
-    const double w = west_face(G, q + 1, F);
-    G.ee[h] = (w * w + EE * EE) / (w + EE + tol);
-    G.fce[Nh + h] = (F[0] * w + Fe[0] * EE) / (w + EE + tol);
-    G.fce[2 * Nh + h] = (F[1] * w + Fe[1] * EE) / (w + EE + tol);
-    G.fce[h] = Fe[2];
-  } else {
-    G.ee[h] = EE;
-  }
-  if (j < D.J) {  // CeN = (F2, F3, F1)
-    double F[3];
-    const double s = south_face(G, q + D.I, F);
-    G.ne[h] = (s * s + NE * NE) / (s + NE + tol);
-    G.fcn[h] = (F[1] * s + Fn[1] * NE) / (s + NE + tol);
-    G.fcn[2 * Nh + h] = (F[0] * s + Fn[0] * NE) / (s + NE + tol);
-    G.fcn[Nh + h] = Fn[2];
-  } else {
-    G.ne[h] = NE;
-  }
-  if (k < D.K) {  // CeT = (F1, F2, F3); :541-543 weighs CeB(:,1) in both sums
-    double F[3];
-    const double bb = bottom_face(G, q + (int64_t)D.I * D.J, F);
-    G.te[h] = (bb * bb + TE * TE) / (bb + TE + tol);
-    G.fct[h] = (F[0] * bb + Ft[0] * TE) / (bb + TE + tol);
-    G.fct[Nh + h] = (F[0] * bb + Ft[1] * TE) / (bb + TE + tol);
-    G.fct[2 * Nh + h] = Ft[2];
-  } else {
-    G.te[h] = TE;
+  const int64_t N = D.N, Nh = D.Nh, IJ = (int64_t)D.I * D.J, SXY = (int64_t)D.SX * D.SY;
+  int64_t q = nx_(D, i, j, k0), h = hx(D, i, j, k0);
+  CellIn c = load_cell(G, q);
+  double F[3];
+  double BE = face_geo(c.dx, c.dy, c.n1, c.n2, c.n3, -c.dz / 2.0, c.phi, F);
+  for (int k = k0; k <= k1; ++k, q += IJ, h += SXY) {
+    // DefineCell
+    double v = G.vof[q], vl = G.vofl[q];
+    int32_t ty = G.type[q];
+    if (v >= 1.0 - kEpsi) { ty = 0; vl = vl / v; v = 1.0; }
+    if (v <= kEpsi) { ty = 2; v = 0.0; vl = 0.0; }
+    if (v >= kEpsi && v <= 1.0 - kEpsi) ty = 1;
+    G.vof[q] = v;
+    G.vofl[q] = vl;
+    G.type[q] = ty;
+    G.cent[q] = 0.0;
+    G.cent[N + q] = 0.0;
+    G.cent[2 * N + q] = 0.0;
+    double Ft[3], Fe[3], Fn[3];
+    const double TE = face_geo(c.dx, c.dy, c.n1, c.n2, c.n3, c.dz / 2.0, c.phi, Ft);
+    const double EE = face_geo(c.dy, c.dz, c.n2, c.n3, c.n1, c.dx / 2.0, c.phi, Fe);
+    const double WE = face_geo(c.dy, c.dz, c.n2, c.n3, c.n1, -c.dx / 2.0, c.phi, F);
+    const double NE = face_geo(c.dz, c.dx, c.n3, c.n1, c.n2, c.dy / 2.0, c.phi, Fn);
+    const double SE = face_geo(c.dz, c.dx, c.n3, c.n1, c.n2, -c.dy / 2.0, c.phi, F);
+    const double a = (EE - WE) * c.dy * c.dz;
+    const double b = (NE - SE) * c.dx * c.dz;
+    const double e = (TE - BE) * c.dx * c.dy;
+    G.wllh[q] = sqrt(a * a + b * b + e * e);
+    if (i < D.I) {  // CeE = (F3, F1, F2), CeW of the neighbour likewise
+      const double w = west_face(G, q + 1, F);
+      G.ee[h] = (w * w + EE * EE) / (w + EE + tol);
+      G.fce[Nh + h] = (F[0] * w + Fe[0] * EE) / (w + EE + tol);
+      G.fce[2 * Nh + h] = (F[1] * w + Fe[1] * EE) / (w + EE + tol);
+      G.fce[h] = Fe[2];
+    } else {
+      G.ee[h] = EE;
+    }
+    if (j < D.J) {  // CeN = (F2, F3, F1)
+      const double s = south_face(G, q + D.I, F);
+      G.ne[h] = (s * s + NE * NE) / (s + NE + tol);
+      G.fcn[h] = (F[1] * s + Fn[1] * NE) / (s + NE + tol);
+      G.fcn[2 * Nh + h] = (F[0] * s + Fn[0] * NE) / (s + NE + tol);
+      G.fcn[Nh + h] = Fn[2];
+    } else {
+      G.ne[h] = NE;
+    }
+    if (k < D.K) {  // CeT = (F1, F2, F3); :541-543 weighs CeB(:,1) in both sums
+      c = load_cell(G, q + IJ);
+      const double bb = face_geo(c.dx, c.dy, c.n1, c.n2, c.n3, -c.dz / 2.0, c.phi, F);
+      G.te[h] = (bb * bb + TE * TE) / (bb + TE + tol);
+      G.fct[h] = (F[0] * bb + Ft[0] * TE) / (bb + TE + tol);
+      G.fct[Nh + h] = (F[0] * bb + Ft[1] * TE) / (bb + TE + tol);
+      G.fct[2 * Nh + h] = Ft[2];
+      BE = bb;
+    } else {
+      G.te[h] = TE;
+    }
   }
 }
 
@@ -451,8 +479,10 @@ sp_status grid_preprocess(sp_ctx* c, const sp_cell_geom* const g[4]) {
   GeoGrid G[4];
   for (int m = 0; m < 4; ++m) G[m] = to_grid(*g[m]);
   const dim3 blk(64, 4, 1);
-  const dim3 halo_grid((unsigned)((D.I + 2 + 63) / 64), (unsigned)((D.J + 2 + 3) / 4),
-                       (unsigned)(D.K + 2));
+  const int sab = std::max(std::max(D.SX, D.SY), D.K + 2);
+  const dim3 halo_planes((unsigned)((sab + 63) / 64), (unsigned)((sab + 3) / 4), 6);
+  const dim3 col_chunks((unsigned)((D.I + 63) / 64), (unsigned)((D.J + 3) / 4),
+                        (unsigned)((D.K + kGeoChunk - 1) / kGeoChunk));
   const dim3 cell_grid((unsigned)((D.I + 63) / 64), (unsigned)((D.J + 3) / 4), (unsigned)D.K);
   const dim3 inner_grid((unsigned)((std::max(D.I - 1, 1) + 63) / 64),
                         (unsigned)((std::max(D.J - 1, 1) + 3) / 4),
@@ -470,7 +500,8 @@ sp_status grid_preprocess(sp_ctx* c, const sp_cell_geom* const g[4]) {
     // 3 centre, 3 apertures, 9 face-centre components, WlLh written; halo
     // entries write 3 apertures
     timer_begin(c, "geo_cells", N * (9 * 8 + 4) + N * (15 * 8 + 8 + 4) + (Nh - N) * 24, &slot);
-    hipLaunchKernelGGL(k_geo_cells, halo_grid, blk, 0, c->stream, D, T);
+    hipLaunchKernelGGL(k_geo_halo, halo_planes, blk, 0, c->stream, D, T);
+    hipLaunchKernelGGL(k_geo_cells, col_chunks, blk, 0, c->stream, D, T);
     timer_end(c, slot);
     timer_begin(c, "geo_faces", 0.0, &slot);
     if (m == 0 && inner) hipLaunchKernelGGL(k_geo_demote, inner_grid, blk, 0, c->stream, D, T);

@@ -10,6 +10,9 @@
   in ``UpdatePUVW``: ``BoundaryConditionVarNew`` (ComputePUVW.f90:418-694),
   the mass fix of the first Dirichlet pressure face (:256-384) and
   ``VariablesInternalCellCondition`` (Solver.f90:467-501), on the engine.
+* :func:`grid_preprocess` follows ``GridPreprocess`` (CutCell.f90:62-325):
+  the cut-cell geometry of the four staggered grids, written into the
+  Cell containers in place.
 * :class:`DeviceProjection` runs the projection on torch tensors resident on
   the GPU and keeps the forward pressure for the adjoint.
 * :class:`ProjectionFunction` is the ``torch.autograd.Function`` whose
@@ -50,6 +53,16 @@ class Cell:  # CutCell.f90:13-39
     NEArea: Optional[np.ndarray] = None
     TEArea: Optional[np.ndarray] = None
     MoExCell: Optional[np.ndarray] = None
+    phi: Optional[np.ndarray] = None         # solid level set and normal
+    nx: Optional[np.ndarray] = None
+    ny: Optional[np.ndarray] = None
+    nz: Optional[np.ndarray] = None
+    FCE: Optional[np.ndarray] = None         # (0:I+1, 0:J+1, 0:K+1, 3)
+    FCN: Optional[np.ndarray] = None
+    FCT: Optional[np.ndarray] = None
+    Cell_Cent: Optional[np.ndarray] = None   # (I, J, K, 3)
+    WlLh: Optional[np.ndarray] = None
+    delh: Optional[np.ndarray] = None
 
 
 @dataclasses.dataclass
@@ -167,6 +180,54 @@ def boundary_condition_var_new(PGrid, PCell, UCell, VCell, WCell, TVar, BCp, BCu
         for f_, name in enumerate(("VarW", "VarE", "VarS", "VarN", "VarB", "VarT")):
             setattr(b, name, res["bc_var"][6 * q + f_])
     return st
+
+
+# sp_cell_geom member -> Cell member
+_GEOM_CELL = dict(phi="phi", nx="nx", ny="ny", nz="nz", vof="vof", vof_l="vofL",
+                  cell_type="Cell_Type", ee_area="EEArea", ne_area="NEArea", te_area="TEArea",
+                  fce="FCE", fcn="FCN", fct="FCT", cell_cent="Cell_Cent", wllh="WlLh",
+                  delh="delh")
+
+
+def grid_preprocess(PGrid, UGrid, VGrid, WGrid, PCell, UCell, VCell, WCell):
+    """GridPreprocess (CutCell.f90:62-325) on the engine (sp_grid_preprocess),
+    with the reference's arguments.  Grids carry 3-D dx, dy, dz; each Cell
+    needs phi, nx, ny, nz, vof, vofL and delh (and FCE/FCN/FCT where their
+    untouched entries matter); Cell_Type, vof, vofL, EEArea, NEArea,
+    TEArea, FCE, FCN, FCT, Cell_Cent, WlLh and delh are written back (the
+    ones that are None are allocated)."""
+    I, J, K = np.shape(PCell.phi)
+    sh = _eng.geom_shapes(I, J, K)
+    cells = []
+    for G, C in ((PGrid, PCell), (UGrid, UCell), (VGrid, VCell), (WGrid, WCell)):
+        d = {}
+        for n in ("dx", "dy", "dz"):
+            a = getattr(G, n)
+            if np.shape(a) != (I, J, K):
+                raise ProjectionError(f"TGrid%{n} must be an (Imax, Jmax, Kmax) array")
+            d[n] = a
+        for n, m in _GEOM_CELL.items():
+            a = getattr(C, m)
+            if a is None:
+                if n in ("phi", "nx", "ny", "nz", "vof", "vof_l", "delh"):
+                    raise ProjectionError(f"TCell%{m} is required")
+                a = np.zeros(sh[n], dtype=np.int32 if n == "cell_type" else np.float64,
+                             order="F")
+            if np.shape(a) != sh[n]:
+                raise ProjectionError(f"TCell%{m} must have shape {sh[n]}")
+            d[n] = a
+        cells.append(d)
+    P = _projector((I, J, K), dict(pin_i=0, pin_j=0, pin_k=0))
+    outs = P.grid_preprocess(cells)
+    for C, o in zip((PCell, UCell, VCell, WCell), outs):
+        for n, m in _GEOM_CELL.items():
+            if n in ("phi", "nx", "ny", "nz"):
+                continue
+            cur = getattr(C, m)
+            if cur is None or not isinstance(cur, np.ndarray) or cur.dtype != o[n].dtype:
+                setattr(C, m, o[n])
+            else:
+                cur[...] = o[n]
 
 
 # ---------------------------------------------------------------------------

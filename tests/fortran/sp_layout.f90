@@ -12,6 +12,7 @@ program sp_layout
   type(sp_adj_outputs), target :: ao
   type(sp_wall_bc), target :: wb
   type(sp_vtr_fields), target :: vf
+  type(sp_cell_geom), target :: cg
   print '(a,1x,i0)', 'sp_params', c_sizeof(prm)
   print '(a,1x,i0)', 'sp_inputs', c_sizeof(inp)
   print '(a,1x,i0)', 'sp_outputs', c_sizeof(outp)
@@ -20,6 +21,10 @@ program sp_layout
   print '(a,1x,i0)', 'sp_adj_outputs', c_sizeof(ao)
   print '(a,1x,i0)', 'sp_wall_bc', c_sizeof(wb)
   print '(a,1x,i0)', 'sp_vtr_fields', c_sizeof(vf)
+  print '(a,1x,i0)', 'sp_cell_geom', c_sizeof(cg)
+  print '(a,1x,i0)', 'sp_cell_geom.cell_type', off(c_loc(cg%cell_type), c_loc(cg))
+  print '(a,1x,i0)', 'sp_cell_geom.fct', off(c_loc(cg%fct), c_loc(cg))
+  print '(a,1x,i0)', 'sp_cell_geom.delh', off(c_loc(cg%delh), c_loc(cg))
   print '(a,1x,i0)', 'sp_vtr_fields.mres', off(c_loc(vf%mres), c_loc(vf))
   print '(a,1x,i0)', 'sp_vtr_fields.halo_mask', off(c_loc(vf%halo_mask), c_loc(vf))
   print '(a,1x,i0)', 'sp_wall_bc.const_u', off(c_loc(wb%const_u), c_loc(wb))

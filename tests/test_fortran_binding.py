@@ -39,7 +39,8 @@ def test_fortran_types_match_c_layout():
     structs = {"sp_params": engine.SpParams, "sp_inputs": engine.SpInputs,
                "sp_outputs": engine.SpOutputs, "sp_stats": engine.SpStats,
                "sp_adj_inputs": engine.SpAdjInputs, "sp_adj_outputs": engine.SpAdjOutputs,
-               "sp_wall_bc": engine.SpWallBc, "sp_vtr_fields": engine.SpVtrFields}
+               "sp_wall_bc": engine.SpWallBc, "sp_vtr_fields": engine.SpVtrFields,
+               "sp_cell_geom": engine.SpCellGeom}
     for name, cls in structs.items():
         assert got[name] == ctypes.sizeof(cls), name
     for key, v in got.items():

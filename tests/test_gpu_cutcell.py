@@ -91,3 +91,28 @@ def test_null_member_is_rejected(gpu_available):
         P._check(rc, "sp_grid_preprocess")
     assert "null" in str(ei.value)
     P.close()
+
+
+def test_reference_interface_mirror(gpu_available):
+    """grid_preprocess(PGrid, UGrid, VGrid, WGrid, PCell, UCell, VCell, WCell)
+    fills the Cell containers with the oracle's GridPreprocess result."""
+    from simcoflow3d_amd import projection as pj
+    size = (11, 9, 7)
+    print("[geometry] reference interface mirror", flush=True)
+    cells = configs.geometry_case(size, "cylinder", seed=4, stretch=True, edge_values=True)
+    exp = orc.grid_preprocess(cells)
+    grids = [pj.Grid(c["dx"], c["dy"], c["dz"]) for c in cells]
+    objs = []
+    for c in cells:
+        objs.append(pj.Cell(c["cell_type"].copy(order="F"), vof=c["vof"].copy(order="F"),
+                            vofL=c["vof_l"].copy(order="F"), phi=c["phi"], nx=c["nx"],
+                            ny=c["ny"], nz=c["nz"], FCE=c["fce"].copy(order="F"),
+                            FCN=c["fcn"].copy(order="F"), FCT=c["fct"].copy(order="F"),
+                            delh=c["delh"].copy(order="F")))
+    pj.grid_preprocess(*grids, *objs)
+    names = dict(vof="vof", vof_l="vofL", cell_type="Cell_Type", ee_area="EEArea",
+                 ne_area="NEArea", te_area="TEArea", fce="FCE", fcn="FCN", fct="FCT",
+                 cell_cent="Cell_Cent", wllh="WlLh", delh="delh")
+    for o, e in zip(objs, exp):
+        for n, m in names.items():
+            assert np.array_equal(getattr(o, m), e[n], equal_nan=True), m
