@@ -19,7 +19,8 @@ module simco_proj
   public :: sp_params_default, sp_create, sp_destroy, sp_set_params, sp_set_grid, &
             sp_set_grid_1d, sp_project, sp_project_dev, sp_adjoint, sp_adjoint_dev, &
             sp_wall_step, sp_wall_step_dev, sp_write_vtr, sp_grid_preprocess, &
-            sp_grid_preprocess_dev, sp_strerror_f, sp_last_error_f
+            sp_grid_preprocess_dev, sp_define_momentum_exchange_cell, &
+            sp_number_external_cell, sp_strerror_f, sp_last_error_f
   public :: SP_OK, SP_ERR_INVALID_ARG, SP_ERR_HIP, SP_ERR_NOT_CONVERGED, SP_ERR_NONFINITE, &
             SP_ERR_OOM, SP_ERR_GRID, SP_ERR_STATE, SP_ERR_NO_DEVICE, SP_PREC_FP64, &
             SP_PREC_MIXED, SP_PRECOND_MG, SP_PRECOND_JACOBI
@@ -232,6 +233,26 @@ module simco_proj
       type(c_ptr), value :: ctx
       type(sp_cell_geom), intent(in) :: p, u, v, w
     end function sp_grid_preprocess_dev
+
+    ! c_loc of PCell%Cell_Type, UCell/VCell/WCell%Cell_Type and of
+    ! UCell/VCell/WCell%MoExCell; device = 0 for host arrays
+    integer(c_int) function sp_define_momentum_exchange_cell(ctx, ctp, ctu, ctv, ctw, mu, &
+        mv, mw, device) bind(C, name="sp_define_momentum_exchange_cell")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, ctp, ctu, ctv, ctw, mu, mv, mw
+      integer(c_int), value :: device
+    end function sp_define_momentum_exchange_cell
+
+    ! TCell%PosNu numbering; ext_cell receives TCell%ExtCell
+    integer(c_int) function sp_number_external_cell(ctx, iu, iv, iw, cell_type, posnu, &
+        ext_cell, device) bind(C, name="sp_number_external_cell")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: iu, iv, iw
+      type(c_ptr), value :: cell_type, posnu
+      integer(c_int32_t), intent(out) :: ext_cell
+      integer(c_int), value :: device
+    end function sp_number_external_cell
 
     type(c_ptr) function sp_strerror_c(s) bind(C, name="sp_strerror")
       import :: c_ptr, c_int

@@ -277,6 +277,27 @@ sp_status sp_grid_preprocess(sp_ctx* ctx, const sp_cell_geom* p, const sp_cell_g
 sp_status sp_grid_preprocess_dev(sp_ctx* ctx, const sp_cell_geom* p, const sp_cell_geom* u,
                                  const sp_cell_geom* v, const sp_cell_geom* w);
 
+/* DefineMomentumExchangeCell(PCell, UCell, VCell, WCell)
+ * (CutCell.f90:673-751): a cut U (V, W) cell whose P cell or +1 P neighbour
+ * along its axis is solid (its own P cell only, in the last cell of a line)
+ * gets MoExCell = 1; the other MoExCell entries keep their values.  Cell
+ * arrays (1:Imax,1:Jmax,1:Kmax).  `device`: 1 when the pointers are device
+ * memory, 0 for host memory. */
+sp_status sp_define_momentum_exchange_cell(sp_ctx* ctx, const int32_t* cell_type_p,
+                                           const int32_t* cell_type_u,
+                                           const int32_t* cell_type_v,
+                                           const int32_t* cell_type_w, int32_t* moex_u,
+                                           int32_t* moex_v, int32_t* moex_w, int device);
+
+/* NumberExternalCell(iu, iv, iw, TCell) (CutCell.f90:753-786): posnu
+ * (TCell%PosNu, shape (Imax-iu, Jmax-iv, Kmax-iw)) numbers the non-solid
+ * cells (Cell_Type /= 2) from 0 in the loop order i, j, k with k
+ * innermost, -1 for solid cells; *ext_cell (host memory) receives
+ * TCell%ExtCell = count - 1.  iu, iv, iw are 0 or 1.  `device` as above,
+ * for cell_type and posnu. */
+sp_status sp_number_external_cell(sp_ctx* ctx, int iu, int iv, int iw, const int32_t* cell_type,
+                                  int32_t* posnu, int32_t* ext_cell, int device);
+
 /* HIP stream the context runs on (hipStream_t as void*). */
 void* sp_stream(sp_ctx* ctx);
 

@@ -1380,3 +1380,58 @@ void orc_grid_preprocess(int I, int J, int K, orc_cell* P, orc_cell* U, orc_cell
       }
 #undef F3
 }
+
+/* DefineMomentumExchangeCell (CutCell.f90:673-751): a cut U (V, W) cell next
+ * to a solid P cell (either P cell of its face, only the own one in the last
+ * cell of a line) gets MoExCell = 1; other entries keep their values. */
+void orc_define_moex(int I, int J, int K, const int32_t* ctp, const int32_t* ctu,
+                     const int32_t* ctv, const int32_t* ctw, int32_t* mu, int32_t* mv,
+                     int32_t* mw) {
+  const orc_grid G = {.I = I, .J = J, .K = K};
+  const orc_grid* g = &G;
+  for (int i = 1; i <= I; ++i)
+    for (int j = 1; j <= J; ++j)
+      for (int k = 1; k <= K; ++k) {
+        const int64_t q = NX(g, i, j, k);
+        if (ctu[q] == 1) {
+          if (i < I) {
+            if (ctp[q] == 2 || ctp[NX(g, i + 1, j, k)] == 2) mu[q] = 1;
+          } else if (ctp[q] == 2) {
+            mu[q] = 1;
+          }
+        }
+        if (ctv[q] == 1) {
+          if (j < J) {
+            if (ctp[q] == 2 || ctp[NX(g, i, j + 1, k)] == 2) mv[q] = 1;
+          } else if (ctp[q] == 2) {
+            mv[q] = 1;
+          }
+        }
+        if (ctw[q] == 1) {
+          if (k < K) {
+            if (ctp[q] == 2 || ctp[NX(g, i, j, k + 1)] == 2) mw[q] = 1;
+          } else if (ctp[q] == 2) {
+            mw[q] = 1;
+          }
+        }
+      }
+}
+
+/* NumberExternalCell (CutCell.f90:753-786): PosNu (Imax-iu, Jmax-iv,
+ * Kmax-iw) numbers the non-solid cells from 0 in the loop order i, j, k (k
+ * innermost), -1 for solid cells; returns ExtCell = count - 1. */
+int32_t orc_number_external(int I, int J, int K, int iu, int iv, int iw, const int32_t* ct,
+                            int32_t* posnu) {
+  const orc_grid G = {.I = I, .J = J, .K = K};
+  const orc_grid* g = &G;
+  const int64_t I2 = I - iu, J2 = J - iv;
+  int32_t ctr = 0;
+  for (int i = 1; i <= I - iu; ++i)
+    for (int j = 1; j <= J - iv; ++j)
+      for (int k = 1; k <= K - iw; ++k) {
+        const int64_t p = (i - 1) + I2 * ((j - 1) + J2 * (int64_t)(k - 1));
+        if (ct[NX(g, i, j, k)] != 2) posnu[p] = ctr++;
+        else posnu[p] = -1;
+      }
+  return ctr - 1;
+}

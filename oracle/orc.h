@@ -43,6 +43,8 @@
  *                       zero; DefineCell classes and vofL scaling; the
  *                       stretched-grid face centre rule and the FCT(:,2)
  *                       quirk on hand-built cells
+ *   orc_define_moex,    vectorised restatements over shifted arrays (numpy),
+ *   orc_number_external a cumulative count in C order of the (i,j,k) array
  * No reference golden vector exists for this path and HYPRE is absent, so the
  * HYPRE iterate itself is not pinned (any converged solve of the pinned,
  * uniquely solvable system is).
@@ -195,6 +197,14 @@ void orc_face_geo(double d1, double d2, double n1, double n2, double n3, double 
  * grids, then the face and cell centre corrections of U, V, W and P. */
 void orc_grid_preprocess(int I, int J, int K, orc_cell* P, orc_cell* U, orc_cell* V,
                          orc_cell* W);
+
+/* DefineMomentumExchangeCell (CutCell.f90:673-751). */
+void orc_define_moex(int I, int J, int K, const int32_t* ctp, const int32_t* ctu,
+                     const int32_t* ctv, const int32_t* ctw, int32_t* mu, int32_t* mv,
+                     int32_t* mw);
+/* NumberExternalCell (CutCell.f90:753-786); returns ExtCell. */
+int32_t orc_number_external(int I, int J, int K, int iu, int iv, int iw, const int32_t* ct,
+                            int32_t* posnu);
 
 #ifdef __cplusplus
 }

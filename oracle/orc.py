@@ -101,6 +101,10 @@ def lib():
         C = ctypes.POINTER(OrcCell)
         L.orc_grid_preprocess.argtypes = [ctypes.c_int] * 3 + [C] * 4
         L.orc_grid_preprocess.restype = None
+        L.orc_define_moex.argtypes = [ctypes.c_int] * 3 + [I32P] * 7
+        L.orc_define_moex.restype = None
+        L.orc_number_external.argtypes = [ctypes.c_int] * 6 + [I32P, I32P]
+        L.orc_number_external.restype = ctypes.c_int32
         _lib = L
     return _lib
 
@@ -429,3 +433,21 @@ def grid_preprocess(cells):
         structs.append(OrcCell(*(o[n].ctypes.data for n in GEOM_FIELDS)))
     lib().orc_grid_preprocess(I, J, K, *(ctypes.byref(s) for s in structs))
     return outs
+
+
+def define_moex(ctp, ctu, ctv, ctw, mu, mv, mw):
+    """DefineMomentumExchangeCell on copies of MoExCell of U, V, W."""
+    I, J, K = ctp.shape
+    cts = [np.asfortranarray(a, dtype=np.int32) for a in (ctp, ctu, ctv, ctw)]
+    ms = [np.array(a, dtype=np.int32, order="F", copy=True) for a in (mu, mv, mw)]
+    lib().orc_define_moex(I, J, K, *(_p(a) for a in cts), *(_p(a) for a in ms))
+    return ms
+
+
+def number_external(ct, iu, iv, iw):
+    """NumberExternalCell: (PosNu of shape (I-iu, J-iv, K-iw), ExtCell)."""
+    I, J, K = ct.shape
+    ct = np.asfortranarray(ct, dtype=np.int32)
+    pos = np.zeros((I - iu, J - iv, K - iw), dtype=np.int32, order="F")
+    ext = lib().orc_number_external(I, J, K, iu, iv, iw, _p(ct), _p(pos))
+    return pos, int(ext)

@@ -845,6 +845,74 @@ sp_status sp_grid_preprocess(sp_ctx* c, const sp_cell_geom* p, const sp_cell_geo
   return SP_OK;
 }
 
+sp_status sp_define_momentum_exchange_cell(sp_ctx* c, const int32_t* ctp, const int32_t* ctu,
+                                           const int32_t* ctv, const int32_t* ctw, int32_t* mu,
+                                           int32_t* mv, int32_t* mw, int device) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  if (!ctp || !ctu || !ctv || !ctw || !mu || !mv || !mw)
+    return fail(c, SP_ERR_INVALID_ARG, "null Cell_Type or MoExCell pointer");
+  const size_t N = (size_t)c->dm.N;
+  sp_status s;
+  if (device) {
+    if ((s = define_moex(c, ctp, ctu, ctv, ctw, mu, mv, mw))) return s;
+  } else {
+    size_t k = 130;  // staging slots past those of the other host entry points
+    const int32_t *d[4], *m[3];
+    const int32_t* hin[4] = {ctp, ctu, ctv, ctw};
+    int32_t* hm[3] = {mu, mv, mw};
+    for (int q = 0; q < 4; ++q)
+      if ((s = upload(c, k++, hin[q], N, &d[q]))) return s;
+    for (int q = 0; q < 3; ++q)
+      if ((s = upload(c, k++, static_cast<const int32_t*>(hm[q]), N, &m[q]))) return s;
+    if ((s = define_moex(c, d[0], d[1], d[2], d[3], const_cast<int32_t*>(m[0]),
+                         const_cast<int32_t*>(m[1]), const_cast<int32_t*>(m[2]))))
+      return s;
+    for (int q = 0; q < 3; ++q)
+      if ((s = pipe_download(c, hm[q], m[q], N * sizeof(int32_t)))) return s;
+  }
+  const hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "sp_define_momentum_exchange_cell");
+  timers_collect(c);
+  return SP_OK;
+}
+
+sp_status sp_number_external_cell(sp_ctx* c, int iu, int iv, int iw, const int32_t* ct,
+                                  int32_t* posnu, int32_t* ext_cell, int device) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  if (!ct || !posnu || !ext_cell)
+    return fail(c, SP_ERR_INVALID_ARG, "null Cell_Type, PosNu or ExtCell pointer");
+  if ((iu != 0 && iu != 1) || (iv != 0 && iv != 1) || (iw != 0 && iw != 1))
+    return fail(c, SP_ERR_INVALID_ARG, "iu, iv, iw must be 0 or 1");
+  const Dims& D = c->dm;
+  const size_t N = (size_t)D.N;
+  const size_t NP = (size_t)(D.I - iu) * (D.J - iv) * (D.K - iw);
+  const size_t NC = (size_t)(D.I - iu) * (D.J - iv);
+  void *cnt, *ext;
+  sp_status s;
+  if ((s = stage(c, 140, NC * sizeof(int32_t), &cnt)) ||
+      (s = stage(c, 141, sizeof(int32_t), &ext)))
+    return s;
+  const int32_t* dct = ct;
+  int32_t* dpos = posnu;
+  if (!device) {
+    void* p;
+    if ((s = upload(c, 142, ct, N, &dct)) || (s = stage(c, 143, NP * sizeof(int32_t), &p)))
+      return s;
+    dpos = static_cast<int32_t*>(p);
+  }
+  if ((s = number_external(c, iu, iv, iw, dct, dpos, static_cast<int32_t*>(cnt),
+                           static_cast<int32_t*>(ext))))
+    return s;
+  if (!device && (s = pipe_download(c, posnu, dpos, NP * sizeof(int32_t)))) return s;
+  hipError_t e = hipMemcpyAsync(ext_cell, ext, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "sp_number_external_cell");
+  timers_collect(c);
+  return SP_OK;
+}
+
 void* sp_stream(sp_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 void sp_set_profiling(sp_ctx* c, int on) {

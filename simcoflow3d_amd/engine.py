@@ -34,7 +34,8 @@ EXPORTED_SYMBOLS = (
     "sp_set_grid_1d", "sp_project", "sp_project_dev", "sp_adjoint", "sp_adjoint_dev",
     "sp_stream", "sp_kernel_times", "sp_set_profiling", "sp_set_profiling_filter",
     "sp_strerror", "sp_last_error", "sp_wall_step", "sp_wall_step_dev", "sp_write_vtr",
-    "sp_grid_preprocess", "sp_grid_preprocess_dev",
+    "sp_grid_preprocess", "sp_grid_preprocess_dev", "sp_define_momentum_exchange_cell",
+    "sp_number_external_cell",
 )
 
 # members of sp_cell_geom, in order
@@ -198,6 +199,10 @@ def load():
         f = getattr(L, name)
         f.argtypes = [V] + [P(SpCellGeom)] * 4
         f.restype = ctypes.c_int
+    L.sp_define_momentum_exchange_cell.argtypes = [V] + [V] * 7 + [ctypes.c_int]
+    L.sp_define_momentum_exchange_cell.restype = ctypes.c_int
+    L.sp_number_external_cell.argtypes = [V] + [ctypes.c_int] * 3 + [V, V, I32P, ctypes.c_int]
+    L.sp_number_external_cell.restype = ctypes.c_int
     L.sp_stream.argtypes = [V]
     L.sp_stream.restype = V
     L.sp_kernel_times.argtypes = [V, P(SpKernelTime), ctypes.c_int]
@@ -398,6 +403,44 @@ class Projector:
         rc = self.lib.sp_grid_preprocess(self.h, *(ctypes.byref(s) for s in structs))
         self._check(rc, "sp_grid_preprocess")
         return outs
+
+    def define_moex(self, ctp, ctu, ctv, ctw, mu, mv, mw, device=False):
+        """DefineMomentumExchangeCell (sp_define_momentum_exchange_cell).  Host:
+        returns updated copies of mu, mv, mw; device (torch int32 tensors):
+        updates mu, mv, mw in place."""
+        if device:
+            args = [_addr(a) for a in (ctp, ctu, ctv, ctw, mu, mv, mw)]
+            rc = self.lib.sp_define_momentum_exchange_cell(self.h, *args, 1)
+            self._check(rc, "sp_define_momentum_exchange_cell")
+            return None
+        shape = self.shape
+        cts = [np.asfortranarray(a, dtype=np.int32) for a in (ctp, ctu, ctv, ctw)]
+        ms = [np.array(a, dtype=np.int32, order="F", copy=True) for a in (mu, mv, mw)]
+        for a in cts + ms:
+            if a.shape != shape:
+                raise ValueError(f"expected cell arrays of shape {shape}, got {a.shape}")
+        rc = self.lib.sp_define_momentum_exchange_cell(self.h, *(_addr(a) for a in cts + ms), 0)
+        self._check(rc, "sp_define_momentum_exchange_cell")
+        return ms
+
+    def number_external(self, cell_type, iu, iv, iw, posnu=None, device=False):
+        """NumberExternalCell (sp_number_external_cell): (PosNu, ExtCell).
+        Device: cell_type and posnu are torch int32 tensors (posnu written)."""
+        I, J, K = self.shape
+        ext = ctypes.c_int32()
+        if device:
+            rc = self.lib.sp_number_external_cell(self.h, iu, iv, iw, _addr(cell_type),
+                                                  _addr(posnu), ctypes.byref(ext), 1)
+            self._check(rc, "sp_number_external_cell")
+            return posnu, ext.value
+        ct = np.asfortranarray(cell_type, dtype=np.int32)
+        if ct.shape != (I, J, K):
+            raise ValueError(f"expected Cell_Type of shape {(I, J, K)}, got {ct.shape}")
+        pos = np.zeros((I - iu, J - iv, K - iw), dtype=np.int32, order="F")
+        rc = self.lib.sp_number_external_cell(self.h, iu, iv, iw, _addr(ct), _addr(pos),
+                                              ctypes.byref(ext), 0)
+        self._check(rc, "sp_number_external_cell")
+        return pos, ext.value
 
     def write_vtr(self, directory, prefix, itt, fields, device=False):
         """PrintResultVTR3D's file (sp_write_vtr): fields maps x, y, z (1-D) and

@@ -116,3 +116,37 @@ def test_reference_interface_mirror(gpu_available):
     for o, e in zip(objs, exp):
         for n, m in names.items():
             assert np.array_equal(getattr(o, m), e[n], equal_nan=True), m
+
+
+@pytest.mark.parametrize("size", [(9, 7, 5), (2, 2, 2), (130, 3, 4), (64, 64, 64)])
+def test_moex_and_numbering_match_oracle(gpu_available, size):
+    """sp_define_momentum_exchange_cell and sp_number_external_cell (host and
+    device variants) on the types GridPreprocess produced."""
+    import torch
+    print(f"[geometry] MoExCell / PosNu size={size}", flush=True)
+    cells = configs.geometry_case(size, "bodies", seed=7, stretch=True, edge_values=True)
+    out = orc.grid_preprocess(cells)
+    cts = [o["cell_type"] for o in out]
+    rng = np.random.default_rng(1)
+    ms = [np.asfortranarray(rng.integers(0, 2, size).astype(np.int32)) for _ in range(3)]
+    exp = orc.define_moex(*cts, *ms)
+    P = Projector(size)
+    got = P.define_moex(*cts, *ms)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+    tdev = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).to("cuda:0")
+    dm = [tdev(m) for m in ms]
+    P.define_moex(*(tdev(c) for c in cts), *dm, device=True)
+    for g, e in zip(dm, exp):
+        assert np.array_equal(g.cpu().numpy().T, e)
+    for q, (iu, iv, iw) in enumerate(((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1))):
+        epos, eext = orc.number_external(cts[q], iu, iv, iw)
+        pos, ext = P.number_external(cts[q], iu, iv, iw)
+        assert ext == eext and np.array_equal(pos, epos), (iu, iv, iw)
+        dpos = torch.empty(epos.shape[::-1], dtype=torch.int32, device="cuda:0")
+        _, ext2 = P.number_external(tdev(cts[q]), iu, iv, iw, posnu=dpos, device=True)
+        assert ext2 == eext and np.array_equal(dpos.cpu().numpy().T, epos)
+    # every cell solid: ExtCell = -1
+    pos, ext = P.number_external(np.full(size, 2, np.int32), 0, 0, 0)
+    assert ext == -1 and np.all(pos == -1)
+    P.close()

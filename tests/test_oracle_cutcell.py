@@ -294,3 +294,35 @@ def test_fct_second_component_uses_bottom_first_component():
                 if abs(fb[0] - fb[1]) > 1e-9 and be > 1e-3:
                     hit += 1
     assert hit > 0
+
+
+def test_define_moex_and_numbering_restated_over_arrays():
+    """DefineMomentumExchangeCell and NumberExternalCell against vectorised
+    restatements: shifted-array masks, and a cumulative count of the
+    non-solid cells in C order of the (i, j, k) array (k fastest)."""
+    rng = np.random.default_rng(3)
+    I, J, K = 7, 6, 5
+    cts = [np.asfortranarray(rng.integers(0, 3, (I, J, K)).astype(np.int32)) for _ in range(4)]
+    ms = [np.asfortranarray(rng.integers(0, 2, (I, J, K)).astype(np.int32)) for _ in range(3)]
+    got = orc.define_moex(*cts, *ms)
+    ctp = cts[0]
+    for ax, (ct, m0, g) in enumerate(zip(cts[1:], ms, got)):
+        solid = ctp == 2
+        nb = np.zeros_like(solid)
+        sl = [slice(None)] * 3
+        sl2 = [slice(None)] * 3
+        sl[ax] = slice(0, -1)
+        sl2[ax] = slice(1, None)
+        nb[tuple(sl)] = solid[tuple(sl2)]
+        exp = np.where((ct == 1) & (solid | nb), 1, m0)
+        assert np.array_equal(g, exp), ax
+    for iu, iv, iw in ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)):
+        ct = cts[iu + 2 * iv + 3 * iw]
+        pos, ext = orc.number_external(ct, iu, iv, iw)
+        sub = ct[:I - iu, :J - iv, :K - iw]
+        flat = (sub != 2).ravel(order="C")
+        num = np.where(flat, np.cumsum(flat) - 1, -1).reshape(sub.shape, order="C")
+        assert np.array_equal(pos, num)
+        assert ext == int(flat.sum()) - 1
+    pos, ext = orc.number_external(np.full((3, 3, 3), 2, np.int32), 0, 0, 0)
+    assert ext == -1 and np.all(pos == -1)
