@@ -15,11 +15,11 @@ module simco_proj
   implicit none
   private
   public :: sp_params, sp_inputs, sp_outputs, sp_stats, sp_adj_inputs, sp_adj_outputs, &
-            sp_wall_bc, sp_vtr_fields, sp_cell_geom
+            sp_wall_bc, sp_vtr_fields, sp_cell_geom, sp_cell_interp
   public :: sp_params_default, sp_create, sp_destroy, sp_set_params, sp_set_grid, &
             sp_set_grid_1d, sp_project, sp_project_dev, sp_adjoint, sp_adjoint_dev, &
             sp_wall_step, sp_wall_step_dev, sp_write_vtr, sp_grid_preprocess, &
-            sp_grid_preprocess_dev, sp_define_momentum_exchange_cell, &
+            sp_grid_preprocess_dev, sp_new_cell_face, sp_define_momentum_exchange_cell, &
             sp_number_external_cell, sp_strerror_f, sp_last_error_f
   public :: SP_OK, SP_ERR_INVALID_ARG, SP_ERR_HIP, SP_ERR_NOT_CONVERGED, SP_ERR_NONFINITE, &
             SP_ERR_OOM, SP_ERR_GRID, SP_ERR_STATE, SP_ERR_NO_DEVICE, SP_PREC_FP64, &
@@ -111,6 +111,16 @@ module simco_proj
     type(c_ptr) :: fce = c_null_ptr, fcn = c_null_ptr, fct = c_null_ptr
     type(c_ptr) :: cell_cent = c_null_ptr, wllh = c_null_ptr, delh = c_null_ptr
   end type sp_cell_geom
+
+  ! NewCellFace members of one U/V/W cell besides sp_cell_geom: c_loc of
+  ! TCell%MoExCell, MsCe(Imax,Jmax,Kmax,3) and SxE, EtaE, AlE, DAlE, SyN,
+  ! EtaN, AlN, DAlN, SzT, EtaT, AlT, DAlT (halo arrays)
+  type, bind(C) :: sp_cell_interp
+    type(c_ptr) :: moex = c_null_ptr, msce = c_null_ptr
+    type(c_ptr) :: sxe = c_null_ptr, etae = c_null_ptr, ale = c_null_ptr, dale = c_null_ptr
+    type(c_ptr) :: syn = c_null_ptr, etan = c_null_ptr, aln = c_null_ptr, daln = c_null_ptr
+    type(c_ptr) :: szt = c_null_ptr, etat = c_null_ptr, alt = c_null_ptr, dalt = c_null_ptr
+  end type sp_cell_interp
 
   type, bind(C) :: sp_adj_inputs
     type(c_ptr) :: gu = c_null_ptr, gv = c_null_ptr, gw = c_null_ptr
@@ -233,6 +243,15 @@ module simco_proj
       type(c_ptr), value :: ctx
       type(sp_cell_geom), intent(in) :: p, u, v, w
     end function sp_grid_preprocess_dev
+
+    integer(c_int) function sp_new_cell_face(ctx, p, u, v, w, iu, iv, iw, device) &
+        bind(C, name="sp_new_cell_face")
+      import :: c_ptr, c_int, sp_cell_geom, sp_cell_interp
+      type(c_ptr), value :: ctx
+      type(sp_cell_geom), intent(in) :: p, u, v, w
+      type(sp_cell_interp), intent(in) :: iu, iv, iw
+      integer(c_int), value :: device
+    end function sp_new_cell_face
 
     ! c_loc of PCell%Cell_Type, UCell/VCell/WCell%Cell_Type and of
     ! UCell/VCell/WCell%MoExCell; device = 0 for host arrays

@@ -277,6 +277,33 @@ sp_status sp_grid_preprocess(sp_ctx* ctx, const sp_cell_geom* p, const sp_cell_g
 sp_status sp_grid_preprocess_dev(sp_ctx* ctx, const sp_cell_geom* p, const sp_cell_geom* u,
                                  const sp_cell_geom* v, const sp_cell_geom* w);
 
+/* The members of one U/V/W cell that NewCellFace reads or writes besides
+ * those of sp_cell_geom: TCell%MoExCell (read), TCell%MsCe (1:Imax,..,3,
+ * written for linked cells) and the interpolation data of the E, N, T
+ * faces, TCell%SxE, EtaE, AlE, DAlE, SyN, ..., DAlT (halo arrays, written
+ * where the reference writes them). */
+typedef struct sp_cell_interp {
+  const int32_t* moex;
+  int32_t* msce;
+  double *sxe, *etae, *ale, *dale;
+  double *syn, *etan, *aln, *daln;
+  double *szt, *etat, *alt, *dalt;
+} sp_cell_interp;
+
+/* NewCellFace(PGrid,UGrid,VGrid,WGrid, PCell,UCell,VCell,WCell)
+ * (CutCell.f90:788-881): the U/V/W cell centres from the P face centres,
+ * CellLinking (:1269-1355) of the MoExCell cells with the reference's
+ * sequential semantics, delh, and East/North/TopFaceInterpolationInf
+ * (:983-1267) for the three grids.  Reads P's dx, dy, dz, ee/ne/te_area
+ * and fce/fcn/fct, and U/V/W's dx, dy, dz, phi, nx, ny, nz, cell_type,
+ * ee/ne/te_area and fce/fcn/fct; updates U/V/W's cell_cent and delh.  A
+ * link whose master would lie at index 0 (outside the grid: the reference
+ * reads outside its arrays there) leaves the cell unlinked.  `device`: 1
+ * when the pointers are device memory, 0 for host memory. */
+sp_status sp_new_cell_face(sp_ctx* ctx, const sp_cell_geom* p, const sp_cell_geom* u,
+                           const sp_cell_geom* v, const sp_cell_geom* w, const sp_cell_interp* iu,
+                           const sp_cell_interp* iv, const sp_cell_interp* iw, int device);
+
 /* DefineMomentumExchangeCell(PCell, UCell, VCell, WCell)
  * (CutCell.f90:673-751): a cut U (V, W) cell whose P cell or +1 P neighbour
  * along its axis is solid (its own P cell only, in the last cell of a line)

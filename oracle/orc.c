@@ -1435,3 +1435,198 @@ int32_t orc_number_external(int I, int J, int K, int iu, int iv, int iw, const i
       }
   return ctr - 1;
 }
+
+/* ------------------------------------------------------------------ */
+/* NewCellFace (CutCell.f90:788-881) with CellLinking (:1269-1355) and */
+/* East/North/TopFaceInterpolationInf (:983-1267).                      */
+/* ------------------------------------------------------------------ */
+
+/* The interpolation data of one face direction `ax` (0 E, 1 N, 2 T) of grid
+ * T, with B the grid whose spacing crosses the face and `ib` the offset of
+ * its index (the iu/iv/iw argument). */
+static void face_interp(int I, int J, int K, int ax, int ib, const orc_cell* T,
+                        const double* bd, const int32_t* ct, double* S, double* Eta,
+                        double* Al, double* DAl) {
+  const orc_grid G = {.I = I, .J = J, .K = K};
+  const orc_grid* g = &G;
+  const int64_t N = (int64_t)I * J * K, Nh = (int64_t)(I + 2) * (J + 2) * (K + 2);
+  const double* cent = T->cent;
+  const double* FC = ax == 0 ? T->fce : (ax == 1 ? T->fcn : T->fct);
+  const double* area = ax == 0 ? T->ee : (ax == 1 ? T->ne : T->te);
+  const double* td[3] = {T->dx, T->dy, T->dz};
+  const double* tn[3] = {T->nx, T->ny, T->nz};
+  const int L = ax == 0 ? I : (ax == 1 ? J : K);
+  const int o1 = ax == 0 ? 1 : 0, o2 = ax == 2 ? 1 : 2;  /* the two transverse axes */
+  for (int i = 1; i <= I; ++i)
+    for (int j = 1; j <= J; ++j)
+      for (int k = 1; k <= K; ++k) {
+        const int idx[3] = {i, j, k};
+        if (idx[ax] > L - 1) continue;
+        int n3[3] = {i, j, k};
+        n3[ax] += 1;
+        const int64_t q = NX(g, i, j, k), qn = NX(g, n3[0], n3[1], n3[2]), h = HX(g, i, j, k);
+        int b3[3] = {i, j, k};
+        b3[ax] += ib;
+        const double dB = bd[NX(g, b3[0], b3[1], b3[2])];
+        if (ct[q] != 2 && ct[qn] != 2) {
+          const double ca = cent[ax * N + q], can = cent[ax * N + qn];
+          S[h] = can + dB - ca;
+          const double s1 = cent[o1 * N + qn] - cent[o1 * N + q];
+          const double s2 = cent[o2 * N + qn] - cent[o2 * N + q];
+          Eta[h] = fabs((FC[ax * Nh + h] - ca) / S[h]);
+          if (Eta[h] > 1.0) Eta[h] = 0.5;
+          double f[3];
+          for (int m = 0; m < 3; ++m) {
+            const double a = cent[m * N + q], b = cent[m * N + qn];
+            f[m] = m == ax ? (1.0 - Eta[h]) * a + Eta[h] * (b + dB) : (1.0 - Eta[h]) * a + Eta[h] * b;
+          }
+          /* East tests > (:1013-1014), North and Top >= (:1107, :1202) */
+          const int tr = ax == 0 ? (fabs(s1) > 1.0e-4 * td[o1][q] || fabs(s2) > 1.0e-4 * td[o2][q])
+                                 : (fabs(s1) >= 1.0e-4 * td[o1][q] || fabs(s2) >= 1.0e-4 * td[o2][q]);
+          if (tr) {
+            const double nf1 = 0.5 * (tn[o1][q] + tn[o1][qn]);
+            const double nf2 = 0.5 * (tn[o2][q] + tn[o2][qn]);
+            const double nx0 = T->nx[q], ny0 = T->ny[q], nz0 = T->nz[q], ph0 = T->phi[q];
+            const double nx1 = T->nx[qn], ny1 = T->ny[qn], nz1 = T->nz[qn], ph1 = T->phi[qn];
+            const double delh1 = f[0] * nx0 + f[1] * ny0 + f[2] * nz0 + ph0;
+            double g1[3] = {f[0], f[1], f[2]};
+            g1[ax] = f[ax] - dB;
+            const double delh2 = g1[0] * nx1 + g1[1] * ny1 + g1[2] * nz1 + ph1;
+            double delh = 0.5 * (delh1 + delh2);
+            const double F1 = FC[h], F2 = FC[Nh + h], F3 = FC[2 * Nh + h];
+            double delhec1, delhec2;
+            if (ax == 0) {          /* :1032-1037 */
+              delhec1 = F1 * nx0 + F2 * ny0 + F3 * nz0 + ph0;
+              delhec2 = (F1 - dB) * nx1 + F2 * ny1 + F3 * nz1 + ph1;
+            } else if (ax == 1) {   /* :1127-1132, BGrid%dy(i,j+1,k) whatever iv is */
+              const double dB1 = bd[NX(g, i, j + 1, k)];
+              delhec1 = F1 * nx0 + F2 * ny0 + nz0 * F3 + ph0;
+              delhec2 = F1 * nx1 + (F2 - dB1) * ny1 + nz1 * F3 + ph1;
+            } else {                /* :1222-1227 */
+              delhec1 = F1 * nx0 + F2 * ny0 + nz0 * F3 + ph0;
+              delhec2 = F1 * nx1 + F2 * ny1 + nz1 * (F3 - dB) + ph1;
+            }
+            const double delhec = 0.5 * (delhec1 + delhec2);
+            if (delh < CG_EPSI && area[h] > 0.0) delh = delhec;
+            Al[h] = fabs(delhec) / fabs(delh);
+            DAl[h] = (s1 * nf1 + s2 * nf2) / (S[h] * fabs(delh));
+          } else {
+            DAl[h] = 0.0;
+            Al[h] = 1.0;
+          }
+        } else {
+          Al[h] = 1.0;
+          DAl[h] = 0.0;
+          S[h] = dB;
+          Eta[h] = 0.5;
+        }
+      }
+  /* the entry after each line, index Imax (Jmax, Kmax), copies the last one
+   * (the loop variable after the loop, :1067-1070), then plane 0 copies
+   * plane 1 over the whole halo range (:1073-1076) */
+  double* arrs[4] = {S, Eta, Al, DAl};
+  for (int i = 1; i <= I; ++i)
+    for (int j = 1; j <= J; ++j)
+      for (int k = 1; k <= K; ++k) {
+        const int idx[3] = {i, j, k};
+        if (idx[ax] != L) continue;
+        int p3[3] = {i, j, k};
+        p3[ax] -= 1;
+        for (int m = 0; m < 4; ++m) arrs[m][HX(g, i, j, k)] = arrs[m][HX(g, p3[0], p3[1], p3[2])];
+      }
+  for (int m = 0; m < 4; ++m)
+    for (int k = 0; k <= K + 1; ++k)
+      for (int j = 0; j <= J + 1; ++j)
+        for (int i = 0; i <= I + 1; ++i) {
+          const int idx[3] = {i, j, k};
+          if (idx[ax] != 0) continue;
+          int p3[3] = {i, j, k};
+          p3[ax] = 1;
+          arrs[m][HX(g, i, j, k)] = arrs[m][HX(g, p3[0], p3[1], p3[2])];
+        }
+}
+
+/* CellLinking (CutCell.f90:1269-1355) of cell (i,j,k) of grid T: the cell
+ * centre moves to its master neighbour's, shifted by that cell's width (less
+ * 1e-8 of it), along the dominant normal component.  A master outside the
+ * grid (index 0, which the reference reads outside its arrays) leaves the
+ * cell unchanged. */
+static void cell_linking(int I, int J, int K, int i, int j, int k, const orc_cell* T,
+                         double* cent, int32_t* msce) {
+  const orc_grid G = {.I = I, .J = J, .K = K};
+  const orc_grid* g = &G;
+  const int64_t N = (int64_t)I * J * K, q = NX(g, i, j, k);
+  const double tol = 1.0e-8;
+  const double n[3] = {T->nx[q], T->ny[q], T->nz[q]};
+  const double* d[3] = {T->dx, T->dy, T->dz};
+  double nmax = fabs(n[0]);
+  if (fabs(n[1]) > nmax) nmax = fabs(n[1]);
+  if (fabs(n[2]) > nmax) nmax = fabs(n[2]);
+  int ax = -1;
+  for (int a = 0; a < 3 && ax < 0; ++a)
+    if (fabs(fabs(n[a]) - nmax) < 1.0e-10) ax = a;
+  if (ax < 0) return;
+  const int idx[3] = {i, j, k};
+  int m3[3] = {i, j, k};
+  m3[ax] += n[ax] < 0.0 ? -1 : 1;
+  if (m3[ax] < 1) return;
+  const int64_t qm = NX(g, m3[0], m3[1], m3[2]);
+  const double dm = d[ax][qm];
+  double c[3];
+  for (int a = 0; a < 3; ++a) c[a] = cent[a * N + qm];
+  if (n[ax] < 0.0) c[ax] = c[ax] - dm + tol * dm;
+  else c[ax] = c[ax] + dm - tol * dm;
+  for (int a = 0; a < 3; ++a) {
+    cent[a * N + q] = c[a];
+    msce[a * N + q] = a == ax ? m3[a] : idx[a];
+  }
+}
+
+void orc_new_cell_face(int I, int J, int K, const orc_cell* P, orc_cell* U, orc_cell* V,
+                       orc_cell* W, orc_interp* iu, orc_interp* iv, orc_interp* iw) {
+  const orc_grid G = {.I = I, .J = J, .K = K};
+  const orc_grid* g = &G;
+  const int64_t N = (int64_t)I * J * K, Nh = (int64_t)(I + 2) * (J + 2) * (K + 2);
+  orc_cell* C[3] = {U, V, W};
+  orc_interp* X[3] = {iu, iv, iw};
+  /* cell centres of U/V/W from the P face centres (:798-822) */
+  for (int i = 1; i <= I; ++i)
+    for (int j = 1; j <= J; ++j)
+      for (int k = 1; k <= K; ++k) {
+        const int64_t q = NX(g, i, j, k), h = HX(g, i, j, k);
+        if (P->ee[h] > 0.0) {
+          U->cent[q] = 0.0; U->cent[N + q] = P->fce[Nh + h]; U->cent[2 * N + q] = P->fce[2 * Nh + h];
+        }
+        if (P->ne[h] > 0.0) {
+          V->cent[q] = P->fcn[h]; V->cent[N + q] = 0.0; V->cent[2 * N + q] = P->fcn[2 * Nh + h];
+        }
+        if (P->te[h] > 0.0) {
+          W->cent[q] = P->fct[h]; W->cent[N + q] = P->fct[Nh + h]; W->cent[2 * N + q] = 0.0;
+        }
+      }
+  /* CellLinking of the momentum-exchange cells, in the reference's order
+   * (:824-842) */
+  for (int i = 1; i <= I - 1; ++i)
+    for (int j = 1; j <= J - 1; ++j)
+      for (int k = 1; k <= K - 1; ++k) {
+        const int64_t q = NX(g, i, j, k);
+        for (int m = 0; m < 3; ++m)
+          if (X[m]->moex[q] == 1) cell_linking(I, J, K, i, j, k, C[m], C[m]->cent, X[m]->msce);
+      }
+  /* distance of the cell centre to the solid surface (:844-858) */
+  for (int m = 0; m < 3; ++m)
+    for (int64_t q = 0; q < N; ++q) {
+      const orc_cell* T = C[m];
+      T->delh[q] = fabs(T->cent[q] * T->nx[q] + T->cent[N + q] * T->ny[q] +
+                        T->cent[2 * N + q] * T->nz[q] + T->phi[q]);
+    }
+  /* interpolation data (:862-872): face ax of grid m crosses grid B, which
+   * is P along the grid's own axis (offset 1) and U/V/W otherwise */
+  for (int m = 0; m < 3; ++m)
+    for (int ax = 0; ax < 3; ++ax) {
+      const orc_cell* B = ax == m ? P : C[ax];
+      const double* bd = ax == 0 ? B->dx : (ax == 1 ? B->dy : B->dz);
+      face_interp(I, J, K, ax, ax == m ? 1 : 0, C[m], bd, C[m]->type, X[m]->s[ax],
+                  X[m]->eta[ax], X[m]->al[ax], X[m]->dal[ax]);
+    }
+}

@@ -190,6 +190,16 @@ typedef struct orc_cell {
   double *ee, *ne, *te, *fce, *fcn, *fct, *cent, *wllh, *delh;
 } orc_cell;
 
+/* NewCellFace data of one U/V/W grid: MoExCell (read), MsCe (cell array x
+ * 3, written for linked cells) and the interpolation arrays (halo arrays)
+ * of the E, N, T faces: s = SxE/SyN/SzT, eta = EtaE/N/T, al = AlE/N/T,
+ * dal = DAlE/N/T. */
+typedef struct orc_interp {
+  const int32_t* moex;
+  int32_t* msce;
+  double *s[3], *eta[3], *al[3], *dal[3];
+} orc_interp;
+
 /* FaceGeoCal (CutCell.f90:883-981). */
 void orc_face_geo(double d1, double d2, double n1, double n2, double n3, double d3, double phi,
                   double* area, double cent[3]);
@@ -205,6 +215,12 @@ void orc_define_moex(int I, int J, int K, const int32_t* ctp, const int32_t* ctu
 /* NumberExternalCell (CutCell.f90:753-786); returns ExtCell. */
 int32_t orc_number_external(int I, int J, int K, int iu, int iv, int iw, const int32_t* ct,
                             int32_t* posnu);
+
+/* NewCellFace (CutCell.f90:788-881): U/V/W cell centres from the P face
+ * centres, CellLinking of the MoExCell cells in the reference's sequential
+ * order, delh, and the nine Face*InterpolationInf calls. */
+void orc_new_cell_face(int I, int J, int K, const orc_cell* P, orc_cell* U, orc_cell* V,
+                       orc_cell* W, orc_interp* iu, orc_interp* iv, orc_interp* iw);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,15 @@ class OrcCell(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in GEOM_FIELDS]
 
 
+INTERP_FIELDS = ("sxe", "syn", "szt", "etae", "etan", "etat", "ale", "aln", "alt", "dale",
+                 "daln", "dalt")
+
+
+class OrcInterp(ctypes.Structure):
+    _fields_ = [("moex", ctypes.c_void_p), ("msce", ctypes.c_void_p)] + [
+        (n, ctypes.c_void_p) for n in INTERP_FIELDS]
+
+
 _lib = None
 
 
@@ -101,6 +110,8 @@ def lib():
         C = ctypes.POINTER(OrcCell)
         L.orc_grid_preprocess.argtypes = [ctypes.c_int] * 3 + [C] * 4
         L.orc_grid_preprocess.restype = None
+        L.orc_new_cell_face.argtypes = [ctypes.c_int] * 3 + [C] * 4 + [ctypes.POINTER(OrcInterp)] * 3
+        L.orc_new_cell_face.restype = None
         L.orc_define_moex.argtypes = [ctypes.c_int] * 3 + [I32P] * 7
         L.orc_define_moex.restype = None
         L.orc_number_external.argtypes = [ctypes.c_int] * 6 + [I32P, I32P]
@@ -451,3 +462,31 @@ def number_external(ct, iu, iv, iw):
     pos = np.zeros((I - iu, J - iv, K - iw), dtype=np.int32, order="F")
     ext = lib().orc_number_external(I, J, K, iu, iv, iw, _p(ct), _p(pos))
     return pos, int(ext)
+
+
+def new_cell_face(cells, interp):
+    """NewCellFace on copies: cells = four dicts (P, U, V, W) keyed like
+    sp_cell_geom (GridPreprocess output), interp = three dicts (U, V, W)
+    with moex (and optionally msce and the interpolation arrays' start
+    values).  Returns (cells, interp) with the updated copies."""
+    I, J, K = cells[0]["dx"].shape
+    sh = geom_shapes(I, J, K)
+    cs, structs = [], []
+    for c in cells:
+        o = {n: np.array(c[n], dtype=np.int32 if n == "cell_type" else np.float64, order="F",
+                         copy=True) for n in GEOM_FIELDS}
+        cs.append(o)
+        structs.append(OrcCell(*(o[n].ctypes.data for n in GEOM_FIELDS)))
+    halo = (I + 2, J + 2, K + 2)
+    xs, xstructs = [], []
+    for d in interp:
+        o = {"moex": np.array(d["moex"], dtype=np.int32, order="F", copy=True)}
+        o["msce"] = np.array(d.get("msce", np.zeros((I, J, K, 3), np.int32)), dtype=np.int32,
+                             order="F", copy=True)
+        for n in INTERP_FIELDS:
+            o[n] = np.array(d.get(n, np.zeros(halo)), dtype=np.float64, order="F", copy=True)
+        xs.append(o)
+        xstructs.append(OrcInterp(*(o[n].ctypes.data for n in ("moex", "msce") + INTERP_FIELDS)))
+    lib().orc_new_cell_face(I, J, K, *(ctypes.byref(s) for s in structs),
+                            *(ctypes.byref(s) for s in xstructs))
+    return cs, xs

@@ -845,6 +845,119 @@ sp_status sp_grid_preprocess(sp_ctx* c, const sp_cell_geom* p, const sp_cell_geo
   return SP_OK;
 }
 
+sp_status sp_new_cell_face(sp_ctx* c, const sp_cell_geom* p, const sp_cell_geom* u,
+                           const sp_cell_geom* v, const sp_cell_geom* w, const sp_cell_interp* iu,
+                           const sp_cell_interp* iv, const sp_cell_interp* iw, int device) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  const sp_cell_geom* g[4] = {p, u, v, w};
+  const sp_cell_interp* x[3] = {iu, iv, iw};
+  for (int m = 0; m < 4; ++m) {
+    if (!g[m]) return fail(c, SP_ERR_INVALID_ARG, "null sp_cell_geom");
+    const sp_cell_geom& a = *g[m];
+    // P's spacing is read too: the face along a grid's own axis crosses a P cell
+    const void* req[] = {a.dx, a.dy, a.dz, a.ee_area, a.ne_area, a.te_area, a.fce, a.fcn, a.fct};
+    for (const void* q : req)
+      if (!q) return fail(c, SP_ERR_INVALID_ARG, "sp_cell_geom member is null");
+    if (m == 0) continue;
+    const void* req2[] = {a.phi, a.nx, a.ny, a.nz, a.cell_type, a.cell_cent, a.delh};
+    for (const void* q : req2)
+      if (!q) return fail(c, SP_ERR_INVALID_ARG, "sp_cell_geom member is null");
+  }
+  for (int m = 0; m < 3; ++m) {
+    if (!x[m]) return fail(c, SP_ERR_INVALID_ARG, "null sp_cell_interp");
+    const sp_cell_interp& b = *x[m];
+    const void* req[] = {b.moex, b.msce, b.sxe, b.etae, b.ale, b.dale, b.syn, b.etan,
+                         b.aln, b.daln, b.szt, b.etat, b.alt, b.dalt};
+    for (const void* q : req)
+      if (!q) return fail(c, SP_ERR_INVALID_ARG, "sp_cell_interp member is null");
+  }
+  const size_t N = (size_t)c->dm.N, Nh = (size_t)c->dm.Nh;
+  void* scratch;  // the U/V/W centres before CellLinking
+  sp_status s = stage(c, 150, 9 * N * sizeof(double), &scratch);
+  if (s) return s;
+  if (device) {
+    if ((s = new_cell_face(c, g, x, static_cast<double*>(scratch)))) return s;
+  } else {
+    size_t k = 151;
+    sp_cell_geom d[4];
+    sp_cell_interp xi[3];
+    auto up = [&](const double* h, size_t n, const double** o) { return upload(c, k++, h, n, o); };
+    for (int m = 0; m < 4; ++m) {
+      const sp_cell_geom& h = *g[m];
+      sp_cell_geom& o = d[m];
+      o = sp_cell_geom{};
+      const double* t;
+      if ((s = up(h.ee_area, Nh, &t))) return s;
+      o.ee_area = const_cast<double*>(t);
+      if ((s = up(h.ne_area, Nh, &t))) return s;
+      o.ne_area = const_cast<double*>(t);
+      if ((s = up(h.te_area, Nh, &t))) return s;
+      o.te_area = const_cast<double*>(t);
+      if ((s = up(h.fce, 3 * Nh, &t))) return s;
+      o.fce = const_cast<double*>(t);
+      if ((s = up(h.fcn, 3 * Nh, &t))) return s;
+      o.fcn = const_cast<double*>(t);
+      if ((s = up(h.fct, 3 * Nh, &t))) return s;
+      o.fct = const_cast<double*>(t);
+      if ((s = up(h.dx, N, &o.dx)) || (s = up(h.dy, N, &o.dy)) || (s = up(h.dz, N, &o.dz)))
+        return s;
+      if (m == 0) continue;
+      if ((s = up(h.phi, N, &o.phi)) || (s = up(h.nx, N, &o.nx)) || (s = up(h.ny, N, &o.ny)) ||
+          (s = up(h.nz, N, &o.nz)))
+        return s;
+      const int32_t* ti;
+      if ((s = upload(c, k++, static_cast<const int32_t*>(h.cell_type), N, &ti))) return s;
+      o.cell_type = const_cast<int32_t*>(ti);
+      if ((s = up(h.cell_cent, 3 * N, &t))) return s;
+      o.cell_cent = const_cast<double*>(t);
+      void* b;
+      if ((s = stage(c, k++, N * sizeof(double), &b))) return s;
+      o.delh = static_cast<double*>(b);
+    }
+    for (int m = 0; m < 3; ++m) {
+      const sp_cell_interp& h = *x[m];
+      sp_cell_interp& o = xi[m];
+      const int32_t* ti;
+      if ((s = upload(c, k++, h.moex, N, &o.moex))) return s;
+      if ((s = upload(c, k++, static_cast<const int32_t*>(h.msce), 3 * N, &ti))) return s;
+      o.msce = const_cast<int32_t*>(ti);
+      double* const hs[12] = {h.sxe, h.etae, h.ale, h.dale, h.syn, h.etan,
+                              h.aln, h.daln, h.szt, h.etat, h.alt, h.dalt};
+      double** os[12] = {&o.sxe, &o.etae, &o.ale, &o.dale, &o.syn, &o.etan,
+                         &o.aln, &o.daln, &o.szt, &o.etat, &o.alt, &o.dalt};
+      for (int q = 0; q < 12; ++q) {
+        const double* t;
+        if ((s = up(hs[q], Nh, &t))) return s;
+        *os[q] = const_cast<double*>(t);
+      }
+    }
+    const sp_cell_geom* dg[4] = {&d[0], &d[1], &d[2], &d[3]};
+    const sp_cell_interp* dxi[3] = {&xi[0], &xi[1], &xi[2]};
+    if ((s = new_cell_face(c, dg, dxi, static_cast<double*>(scratch)))) return s;
+    for (int m = 0; m < 3; ++m) {
+      const sp_cell_geom& h = *g[1 + m];
+      const sp_cell_geom& o = d[1 + m];
+      if ((s = pipe_download(c, h.cell_cent, o.cell_cent, 3 * N * sizeof(double))) ||
+          (s = pipe_download(c, h.delh, o.delh, N * sizeof(double))))
+        return s;
+      const sp_cell_interp& hx = *x[m];
+      const sp_cell_interp& ox = xi[m];
+      if ((s = pipe_download(c, hx.msce, ox.msce, 3 * N * sizeof(int32_t)))) return s;
+      double* const hs[12] = {hx.sxe, hx.etae, hx.ale, hx.dale, hx.syn, hx.etan,
+                              hx.aln, hx.daln, hx.szt, hx.etat, hx.alt, hx.dalt};
+      double* const os[12] = {ox.sxe, ox.etae, ox.ale, ox.dale, ox.syn, ox.etan,
+                              ox.aln, ox.daln, ox.szt, ox.etat, ox.alt, ox.dalt};
+      for (int q = 0; q < 12; ++q)
+        if ((s = pipe_download(c, hs[q], os[q], Nh * sizeof(double)))) return s;
+    }
+  }
+  const hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "sp_new_cell_face");
+  timers_collect(c);
+  return SP_OK;
+}
+
 sp_status sp_define_momentum_exchange_cell(sp_ctx* c, const int32_t* ctp, const int32_t* ctu,
                                            const int32_t* ctv, const int32_t* ctw, int32_t* mu,
                                            int32_t* mv, int32_t* mw, int device) {
@@ -857,7 +970,9 @@ sp_status sp_define_momentum_exchange_cell(sp_ctx* c, const int32_t* ctp, const 
   if (device) {
     if ((s = define_moex(c, ctp, ctu, ctv, ctw, mu, mv, mw))) return s;
   } else {
-    size_t k = 130;  // staging slots past those of the other host entry points
+    // staging slots past those the projection, adjoint and wall step keep
+    // between calls; the geometry entry points share theirs within a call
+    size_t k = 130;
     const int32_t *d[4], *m[3];
     const int32_t* hin[4] = {ctp, ctu, ctv, ctw};
     int32_t* hm[3] = {mu, mv, mw};

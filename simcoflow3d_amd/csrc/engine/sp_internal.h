@@ -246,6 +246,9 @@ sp_status define_moex(sp_ctx* c, const int32_t* ctp, const int32_t* ctu, const i
                       const int32_t* ctw, int32_t* mu, int32_t* mv, int32_t* mw);
 sp_status number_external(sp_ctx* c, int iu, int iv, int iw, const int32_t* ct, int32_t* pos,
                           int32_t* cnt, int32_t* ext);
+// NewCellFace (sp_newcellface.hip); scratch holds 9 N doubles.
+sp_status new_cell_face(sp_ctx* c, const sp_cell_geom* const g[4],
+                        const sp_cell_interp* const x[3], double* scratch);
 // ---- VTR output (sp_vtr.hip) ----
 sp_status write_vtr(sp_ctx* c, const char* dir, const char* prefix, int64_t itt,
                     const sp_vtr_fields& f, bool device);

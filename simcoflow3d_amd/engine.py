@@ -35,8 +35,12 @@ EXPORTED_SYMBOLS = (
     "sp_stream", "sp_kernel_times", "sp_set_profiling", "sp_set_profiling_filter",
     "sp_strerror", "sp_last_error", "sp_wall_step", "sp_wall_step_dev", "sp_write_vtr",
     "sp_grid_preprocess", "sp_grid_preprocess_dev", "sp_define_momentum_exchange_cell",
-    "sp_number_external_cell",
+    "sp_number_external_cell", "sp_new_cell_face",
 )
+
+# members of sp_cell_interp, in order
+INTERP_FIELDS = ("moex", "msce", "sxe", "etae", "ale", "dale", "syn", "etan", "aln", "daln",
+                 "szt", "etat", "alt", "dalt")
 
 # members of sp_cell_geom, in order
 GEOM_FIELDS = ("dx", "dy", "dz", "phi", "nx", "ny", "nz", "vof", "vof_l", "cell_type",
@@ -149,6 +153,10 @@ class SpCellGeom(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in GEOM_FIELDS]
 
 
+class SpCellInterp(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in INTERP_FIELDS]
+
+
 class SpError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{msg} (status {status})")
@@ -199,6 +207,9 @@ def load():
         f = getattr(L, name)
         f.argtypes = [V] + [P(SpCellGeom)] * 4
         f.restype = ctypes.c_int
+    L.sp_new_cell_face.argtypes = [V] + [P(SpCellGeom)] * 4 + [P(SpCellInterp)] * 3 + [
+        ctypes.c_int]
+    L.sp_new_cell_face.restype = ctypes.c_int
     L.sp_define_momentum_exchange_cell.argtypes = [V] + [V] * 7 + [ctypes.c_int]
     L.sp_define_momentum_exchange_cell.restype = ctypes.c_int
     L.sp_number_external_cell.argtypes = [V] + [ctypes.c_int] * 3 + [V, V, I32P, ctypes.c_int]
@@ -403,6 +414,53 @@ class Projector:
         rc = self.lib.sp_grid_preprocess(self.h, *(ctypes.byref(s) for s in structs))
         self._check(rc, "sp_grid_preprocess")
         return outs
+
+    def new_cell_face(self, cells, interp, device=False):
+        """NewCellFace (sp_new_cell_face).  cells: four dicts (P, U, V, W)
+        keyed like sp_cell_geom; interp: three dicts (U, V, W) keyed like
+        sp_cell_interp (moex required; msce and the interpolation arrays
+        start values, zeros when missing).  Host: returns updated copies
+        (cells, interp); device (torch tensors): updates in place."""
+        I, J, K = self.shape
+        sh = geom_shapes(I, J, K)
+        ish = {n: (I + 2, J + 2, K + 2) for n in INTERP_FIELDS}
+        ish["moex"] = (I, J, K)
+        ish["msce"] = (I, J, K, 3)
+        if device:
+            gs = [SpCellGeom(*(_addr(c.get(n)) for n in GEOM_FIELDS)) for c in cells]
+            xs = [SpCellInterp(*(_addr(d.get(n)) for n in INTERP_FIELDS)) for d in interp]
+            rc = self.lib.sp_new_cell_face(self.h, *(ctypes.byref(a) for a in gs),
+                                           *(ctypes.byref(a) for a in xs), 1)
+            self._check(rc, "sp_new_cell_face")
+            return None
+        cs, gs = [], []
+        for c in cells:
+            o = {}
+            for n in GEOM_FIELDS:
+                dt = np.int32 if n == "cell_type" else np.float64
+                a = c.get(n)
+                o[n] = (np.zeros(sh[n], dtype=dt, order="F") if a is None else
+                        np.array(a, dtype=dt, order="F", copy=True))
+                if o[n].shape != sh[n]:
+                    raise ValueError(f"{n}: expected shape {sh[n]}, got {o[n].shape}")
+            cs.append(o)
+            gs.append(SpCellGeom(*(_addr(o[n]) for n in GEOM_FIELDS)))
+        xs, xstructs = [], []
+        for d in interp:
+            o = {}
+            for n in INTERP_FIELDS:
+                dt = np.int32 if n in ("moex", "msce") else np.float64
+                a = d.get(n)
+                o[n] = (np.zeros(ish[n], dtype=dt, order="F") if a is None else
+                        np.array(a, dtype=dt, order="F", copy=True))
+                if o[n].shape != ish[n]:
+                    raise ValueError(f"{n}: expected shape {ish[n]}, got {o[n].shape}")
+            xs.append(o)
+            xstructs.append(SpCellInterp(*(_addr(o[n]) for n in INTERP_FIELDS)))
+        rc = self.lib.sp_new_cell_face(self.h, *(ctypes.byref(a) for a in gs),
+                                       *(ctypes.byref(a) for a in xstructs), 0)
+        self._check(rc, "sp_new_cell_face")
+        return cs, xs
 
     def define_moex(self, ctp, ctu, ctv, ctw, mu, mv, mw, device=False):
         """DefineMomentumExchangeCell (sp_define_momentum_exchange_cell).  Host:

@@ -13,6 +13,7 @@ program sp_layout
   type(sp_wall_bc), target :: wb
   type(sp_vtr_fields), target :: vf
   type(sp_cell_geom), target :: cg
+  type(sp_cell_interp), target :: ci
   print '(a,1x,i0)', 'sp_params', c_sizeof(prm)
   print '(a,1x,i0)', 'sp_inputs', c_sizeof(inp)
   print '(a,1x,i0)', 'sp_outputs', c_sizeof(outp)
@@ -22,6 +23,9 @@ program sp_layout
   print '(a,1x,i0)', 'sp_wall_bc', c_sizeof(wb)
   print '(a,1x,i0)', 'sp_vtr_fields', c_sizeof(vf)
   print '(a,1x,i0)', 'sp_cell_geom', c_sizeof(cg)
+  print '(a,1x,i0)', 'sp_cell_interp', c_sizeof(ci)
+  print '(a,1x,i0)', 'sp_cell_interp.sxe', off(c_loc(ci%sxe), c_loc(ci))
+  print '(a,1x,i0)', 'sp_cell_interp.dalt', off(c_loc(ci%dalt), c_loc(ci))
   print '(a,1x,i0)', 'sp_cell_geom.cell_type', off(c_loc(cg%cell_type), c_loc(cg))
   print '(a,1x,i0)', 'sp_cell_geom.fct', off(c_loc(cg%fct), c_loc(cg))
   print '(a,1x,i0)', 'sp_cell_geom.delh', off(c_loc(cg%delh), c_loc(cg))

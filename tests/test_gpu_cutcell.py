@@ -150,3 +150,86 @@ def test_moex_and_numbering_match_oracle(gpu_available, size):
     pos, ext = P.number_external(np.full(size, 2, np.int32), 0, 0, 0)
     assert ext == -1 and np.all(pos == -1)
     P.close()
+
+
+def _ncf_inputs(size, shape, seed, extra_links):
+    """GridPreprocess + DefineMomentumExchangeCell outputs of a seeded case;
+    with extra_links a third of all U/V/W cells is flagged too, which makes
+    long CellLinking chains."""
+    cells = configs.geometry_case(size, shape, seed=seed, stretch=True)
+    out = orc.grid_preprocess(cells)
+    rng = np.random.default_rng(seed)
+    zeros = [np.zeros(size, np.int32, order="F") for _ in range(3)]
+    moex = orc.define_moex(*(o["cell_type"] for o in out), *zeros)
+    interp = []
+    for m in range(3):
+        mo = moex[m]
+        if extra_links:
+            mo = np.asfortranarray(np.where(rng.uniform(size=size) < 0.33, 1, mo).astype(np.int32))
+        d = dict(moex=mo, msce=np.asfortranarray(rng.integers(-5, 5, size + (3,)).astype(np.int32)))
+        for n in orc.INTERP_FIELDS:
+            d[n] = np.asfortranarray(rng.uniform(-1, 1, tuple(s + 2 for s in size)))
+        interp.append(d)
+    return out, interp
+
+
+def _check_ncf(cs, xs, ecs, exs):
+    for g in range(1, 4):
+        for n in ("cell_cent", "delh"):
+            assert np.array_equal(cs[g][n], ecs[g][n], equal_nan=True), (g, n)
+    for m in range(3):
+        for n in ("msce",) + orc.INTERP_FIELDS:
+            assert np.array_equal(xs[m][n], exs[m][n], equal_nan=True), (m, n)
+
+
+@pytest.mark.parametrize("shape", ["sphere", "bodies", "plane", "cylinder"])
+@pytest.mark.parametrize("size,extra", [((12, 10, 9), False), ((9, 8, 11), True),
+                                        ((40, 6, 5), True)])
+def test_new_cell_face_matches_oracle(gpu_available, shape, size, extra):
+    print(f"[geometry] NewCellFace shape={shape} size={size} extra_links={extra}", flush=True)
+    out, interp = _ncf_inputs(size, shape, sum(size), extra)
+    ecs, exs = orc.new_cell_face(out, interp)
+    P = Projector(size)
+    cs, xs = P.new_cell_face(out, interp)
+    P.close()
+    _check_ncf(cs, xs, ecs, exs)
+
+
+def test_new_cell_face_device_and_96_cubed(gpu_available):
+    import torch
+    size = (96, 96, 96)
+    print("[geometry] NewCellFace 96^3 host and device", flush=True)
+    out, interp = _ncf_inputs(size, "bodies", 3, True)
+    ecs, exs = orc.new_cell_face(out, interp)
+    P = Projector(size)
+    cs, xs = P.new_cell_face(out, interp)
+    _check_ncf(cs, xs, ecs, exs)
+    tdev = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).to("cuda:0")
+    dc = [{n: tdev(c[n]) for n in GEOM_FIELDS} for c in out]
+    dx = [{n: tdev(d[n]) for n in ("moex", "msce") + orc.INTERP_FIELDS} for d in interp]
+    P.new_cell_face(dc, dx, device=True)
+    P.close()
+    got_c = [{n: d[n].cpu().numpy().T for n in GEOM_FIELDS} for d in dc]
+    got_x = [{n: d[n].cpu().numpy().T for n in ("msce",) + orc.INTERP_FIELDS} for d in dx]
+    _check_ncf(got_c, got_x, ecs, exs)
+
+
+def test_new_cell_face_requires_p_spacing(gpu_available):
+    """The faces along a grid's own axis cross P cells, so NewCellFace reads
+    PGrid%dx, dy, dz: a missing P spacing is rejected before any launch."""
+    size = (6, 5, 4)
+    print("[geometry] NewCellFace without P spacing", flush=True)
+    out, interp = _ncf_inputs(size, "sphere", 2, False)
+    p = dict(out[0])
+    del p["dx"]
+    P = Projector(size)
+    import ctypes
+    from simcoflow3d_amd.engine import SpCellGeom, SpCellInterp, INTERP_FIELDS, _addr
+    keep = [p] + out[1:] + interp
+    gs = [SpCellGeom(*(_addr(c.get(n)) for n in GEOM_FIELDS)) for c in [p] + out[1:]]
+    xs = [SpCellInterp(*(_addr(d.get(n)) for n in INTERP_FIELDS)) for d in interp]
+    rc = P.lib.sp_new_cell_face(P.h, *(ctypes.byref(a) for a in gs),
+                                *(ctypes.byref(a) for a in xs), 0)
+    assert rc == SP_ERR_INVALID_ARG
+    P.close()
+    del keep

@@ -326,3 +326,120 @@ def test_define_moex_and_numbering_restated_over_arrays():
         assert ext == int(flat.sum()) - 1
     pos, ext = orc.number_external(np.full((3, 3, 3), 2, np.int32), 0, 0, 0)
     assert ext == -1 and np.all(pos == -1)
+
+
+def _after_preprocess(cells):
+    out = orc.grid_preprocess(cells)
+    I, J, K = cells[0]["dx"].shape
+    interp = [dict(moex=np.zeros((I, J, K), np.int32, order="F")) for _ in range(3)]
+    return out, interp
+
+
+def test_new_cell_face_without_solid_closed_forms():
+    """No solid on a stretched grid: U/V/W centres stay at their cells'
+    centres, every face interpolates at Eta = the fraction of the distance
+    between the two centres where the face lies, with Al = 1, DAl = 0."""
+    size = (6, 5, 7)
+    cells, grids, _ = _plane_case(size, 0.0, -10.0, stretch=True)   # solid far outside
+    out, interp = _after_preprocess(cells)
+    cs, xs = orc.new_cell_face(out, interp)
+    I, J, K = size
+    for m in range(3):
+        assert np.all(cs[1 + m]["cell_cent"] == 0.0)
+        assert np.array_equal(cs[1 + m]["delh"], np.abs(cells[1 + m]["phi"]))
+    # the W grid along z: face k of W sits on P centre k+1, W centres at P faces
+    W = grids[3]
+    for k in range(1, K):
+        dzu = W["z"][0, 0, k] - W["z"][0, 0, k - 1]
+        eta = cells[0]["dz"][0, 0, k] / 2.0 / dzu   # the face is half a P cell ahead
+        assert xs[2]["szt"][3, 3, k] == pytest.approx(dzu, rel=1e-14)
+        assert xs[2]["etat"][3, 3, k] == pytest.approx(eta, rel=1e-12)
+    for x in xs:
+        for n in ("ale", "aln", "alt"):
+            assert np.all(x[n][1:I, 1:J, 1:K] == 1.0)
+        for n in ("dale", "daln", "dalt"):
+            assert np.all(x[n][1:I, 1:J, 1:K] == 0.0)
+    # boundary copies: plane 0 equals plane 1, the last entry the one before
+    assert np.array_equal(xs[0]["sxe"][0], xs[0]["sxe"][1])
+    assert np.array_equal(xs[0]["sxe"][I, 1:J + 1, 1:K + 1], xs[0]["sxe"][I - 1, 1:J + 1, 1:K + 1])
+
+
+@pytest.mark.parametrize("theta,d", [(0.4, 0.52), (2.5, -0.3)])
+def test_new_cell_face_interpolation_in_global_frame(theta, d):
+    """Planar solid (fluid where n.x > d): Al is the ratio of the distances
+    of the face centroid and of the interpolation point to the plane, and
+    DAl the transverse offset of the two centroids along the normal over
+    S * distance, all computed from global coordinates."""
+    size = (10, 9, 4)
+    cells, grids, (n1, n2) = _plane_case(size, theta, d)
+    out, interp = _after_preprocess(cells)
+    cs, xs = orc.new_cell_face(out, interp)
+    I, J, K = size
+    checked = 0
+    for m in range(3):  # U, V, W
+        g, c, x = grids[1 + m], cs[1 + m], xs[m]
+        for i in range(1, I):
+            for j in range(1, J + 1):
+                for k in range(1, K + 1):
+                    q, qn = (i - 1, j - 1, k - 1), (i, j - 1, k - 1)
+                    if c["cell_type"][q] == 2 or c["cell_type"][qn] == 2:
+                        continue
+                    p0 = np.array([g["x"][q], g["y"][q], g["z"][q]]) + c["cell_cent"][q]
+                    p1 = np.array([g["x"][qn], g["y"][qn], g["z"][qn]]) + c["cell_cent"][qn]
+                    S = p1[0] - p0[0]
+                    assert x["sxe"][i, j, k] == pytest.approx(S, rel=1e-12)
+                    eta = abs((c["fce"][i, j, k, 0] - c["cell_cent"][q][0]) / S)
+                    if eta > 1.0:
+                        continue
+                    xf = (1 - eta) * p0 + eta * p1
+                    dist = lambda p: n1 * p[0] + n2 * p[1] - d
+                    sy, sz = p1[1] - p0[1], p1[2] - p0[2]
+                    if not (abs(sy) > 1e-4 * g["dy"][q] or abs(sz) > 1e-4 * g["dz"][q]):
+                        assert x["ale"][i, j, k] == 1.0 and x["dale"][i, j, k] == 0.0
+                        continue
+                    dh = dist(xf)
+                    fc = np.array([g["x"][q], g["y"][q], g["z"][q]]) + c["fce"][i, j, k]
+                    dhec = dist(fc)
+                    if dh < 1e-3 and c["ee_area"][i, j, k] > 0.0:
+                        dh = dhec
+                    if abs(dh) < 1e-6:
+                        continue
+                    assert x["ale"][i, j, k] == pytest.approx(abs(dhec) / abs(dh), rel=1e-8)
+                    assert x["dale"][i, j, k] == pytest.approx(sy * n2 / (S * abs(dh)),
+                                                               rel=1e-8, abs=1e-10)
+                    checked += 1
+    assert checked > 10
+
+
+def test_cell_linking_chains_follow_the_loop_order():
+    """CellLinking in the reference's order: a minus-direction link reads
+    the master's already linked centre, a plus-direction link the master's
+    centre before linking."""
+    size = (8, 5, 5)
+    cells = configs.geometry_case(size, "sphere", seed=1)
+    out, interp = _after_preprocess(cells)
+    U = out[1]
+    j, k = 1, 1   # 0-based (j, k) = Fortran (2, 2)
+    for i in (2, 3, 4):         # Fortran i = 3, 4, 5: normal -x, chained
+        U["nx"][i, j, k], U["ny"][i, j, k], U["nz"][i, j, k] = -1.0, 0.0, 0.0
+        interp[0]["moex"][i, j, k] = 1
+    for i in (5, 6):            # Fortran i = 6, 7: normal +x
+        U["nx"][i, j, k], U["ny"][i, j, k], U["nz"][i, j, k] = 1.0, 0.0, 0.0
+        interp[0]["moex"][i, j, k] = 1
+    base, _ = orc.new_cell_face(out, [dict(moex=np.zeros(size, np.int32, order="F"))] * 3)
+    orig = base[1]["cell_cent"]
+    cs, xs = orc.new_cell_face(out, interp)
+    cent, dx = cs[1]["cell_cent"], U["dx"]
+    tol = 1e-8
+    prev = orig[1, j, k].copy()
+    for i in (2, 3, 4):
+        exp = prev.copy()
+        exp[0] = prev[0] - dx[i - 1, j, k] + tol * dx[i - 1, j, k]
+        assert np.array_equal(cent[i, j, k], exp), i
+        assert list(xs[0]["msce"][i, j, k]) == [i, j + 1, k + 1]   # Fortran i-1, j, k
+        prev = exp
+    for i in (5, 6):
+        exp = orig[i + 1, j, k].copy()
+        exp[0] = orig[i + 1, j, k][0] + dx[i + 1, j, k] - tol * dx[i + 1, j, k]
+        assert np.array_equal(cent[i, j, k], exp), i
+        assert list(xs[0]["msce"][i, j, k]) == [i + 2, j + 1, k + 1]
