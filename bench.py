@@ -310,8 +310,9 @@ def write_vtr(P, dev, out, case, directory):
 def geometry(shape, steps, warmup, cpu=True):
     """GridPreprocess (sp_grid_preprocess_dev) on device-resident arrays of
     the four staggered grids, a stretched grid with three spherical solids.
-    Timed calls of the whole preprocessing (all launches of the four grids);
-    per-kernel GB/s from one profiled call; the oracle timed on a bounded
+    Timed calls of the whole preprocessing (all launches of the four grids),
+    then of the whole geometry update after a solid move; per-kernel GB/s
+    from one profiled update; the oracle's GridPreprocess timed on a bounded
     sample of the same workload as the CPU baseline."""
     import numpy as np
     import torch
@@ -356,12 +357,45 @@ def geometry(shape, steps, warmup, cpu=True):
         t0 = time.perf_counter()
         P.grid_preprocess(dev, device=True)  # synchronises its stream
         times.append(time.perf_counter() - t0)
+    # the whole geometry update after a solid move (SolidBody.f90:815-824):
+    # GridPreprocess, DefineMomentumExchangeCell, NumberExternalCell x 4,
+    # NewCellFace, all on device-resident arrays
+    from simcoflow3d_amd.engine import INTERP_FIELDS
+    zi = lambda shp: torch.zeros(shp, dtype=torch.int32, device="cuda")
+    hshape = (K + 2, J + 2, I + 2)
+    interp = [dict(moex=zi((K, J, I)), msce=zi((3, K, J, I)),
+                   **{n: torch.zeros(hshape, dtype=torch.float64, device="cuda")
+                      for n in INTERP_FIELDS[2:]}) for _ in range(3)]
+    posnu = [zi((K - iw, J - iv, I - iu)) for iu, iv, iw in
+             ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1))]
+
+    def update():
+        P.grid_preprocess(dev, device=True)
+        for x in interp:
+            x["moex"].zero_()
+        P.define_moex(*(d["cell_type"] for d in dev), *(x["moex"] for x in interp), device=True)
+        for q, (iu, iv, iw) in enumerate(((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1))):
+            P.number_external(dev[q]["cell_type"], iu, iv, iw, posnu=posnu[q], device=True)
+        P.new_cell_face(dev, interp, device=True)
+
+    for _ in range(max(1, warmup)):
+        restore()
+        update()
+    utimes = []
+    for _ in range(steps):
+        restore()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        update()
+        torch.cuda.synchronize()
+        utimes.append(time.perf_counter() - t0)
     restore()
     P.set_profiling(True)
-    P.grid_preprocess(dev, device=True)
+    update()
     rows = P.kernel_times()
     P.set_profiling(False)
     P.close()
+    del interp, posnu
     N = I * J * K
     kern = []
     for r in rows:
@@ -374,10 +408,15 @@ def geometry(shape, steps, warmup, cpu=True):
     kern.sort(key=lambda t: -t["total_ms"])
     el = sum(times) / len(times)
     rec = dict(workload=f"GridPreprocess of the P, U, V, W grids, {I}x{J}x{K}, three spheres, "
-                        "z stretched 2% per cell",
+                        "z stretched 2% per cell (kernel table: one profiled solid update)",
                calls=steps, ms_per_call=round(el * 1e3, 3),
                median_ms=round(median(times) * 1e3, 3),
-               cells_per_s=round(N / el, 1), kernels=kern)
+               cells_per_s=round(N / el, 1),
+               solid_update=dict(what="GridPreprocess + DefineMomentumExchangeCell + "
+                                      "NumberExternalCell x 4 + NewCellFace",
+                                 ms_per_call=round(sum(utimes) / len(utimes) * 1e3, 3),
+                                 median_ms=round(median(utimes) * 1e3, 3)),
+               kernels=kern)
     del dev, keep
     torch.cuda.empty_cache()
     if cpu:
