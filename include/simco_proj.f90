@@ -15,11 +15,12 @@ module simco_proj
   implicit none
   private
   public :: sp_params, sp_inputs, sp_outputs, sp_stats, sp_adj_inputs, sp_adj_outputs, &
-            sp_wall_bc, sp_vtr_fields, sp_cell_geom, sp_cell_interp
+            sp_wall_bc, sp_vtr_fields, sp_cell_geom, sp_cell_interp, sp_liquid_field
   public :: sp_params_default, sp_create, sp_destroy, sp_set_params, sp_set_grid, &
             sp_set_grid_1d, sp_project, sp_project_dev, sp_adjoint, sp_adjoint_dev, &
             sp_wall_step, sp_wall_step_dev, sp_write_vtr, sp_grid_preprocess, &
             sp_grid_preprocess_dev, sp_new_cell_face, sp_define_momentum_exchange_cell, &
+            sp_compute_uvw_liquid_field, &
             sp_number_external_cell, sp_strerror_f, sp_last_error_f
   public :: SP_OK, SP_ERR_INVALID_ARG, SP_ERR_HIP, SP_ERR_NOT_CONVERGED, SP_ERR_NONFINITE, &
             SP_ERR_OOM, SP_ERR_GRID, SP_ERR_STATE, SP_ERR_NO_DEVICE, SP_PREC_FP64, &
@@ -121,6 +122,14 @@ module simco_proj
     type(c_ptr) :: syn = c_null_ptr, etan = c_null_ptr, aln = c_null_ptr, daln = c_null_ptr
     type(c_ptr) :: szt = c_null_ptr, etat = c_null_ptr, alt = c_null_ptr, dalt = c_null_ptr
   end type sp_cell_interp
+
+  ! ComputeUVWLiquidField fields of one grid: c_loc of TGrid%dx, dy, dz and
+  ! of TCell%phiL, nxL, nyL, nzL, vofL
+  type, bind(C) :: sp_liquid_field
+    type(c_ptr) :: dx = c_null_ptr, dy = c_null_ptr, dz = c_null_ptr
+    type(c_ptr) :: phi_l = c_null_ptr, nx_l = c_null_ptr, ny_l = c_null_ptr
+    type(c_ptr) :: nz_l = c_null_ptr, vof_l = c_null_ptr
+  end type sp_liquid_field
 
   type, bind(C) :: sp_adj_inputs
     type(c_ptr) :: gu = c_null_ptr, gv = c_null_ptr, gw = c_null_ptr
@@ -243,6 +252,14 @@ module simco_proj
       type(c_ptr), value :: ctx
       type(sp_cell_geom), intent(in) :: p, u, v, w
     end function sp_grid_preprocess_dev
+
+    integer(c_int) function sp_compute_uvw_liquid_field(ctx, p, u, v, w, device) &
+        bind(C, name="sp_compute_uvw_liquid_field")
+      import :: c_ptr, c_int, sp_liquid_field
+      type(c_ptr), value :: ctx
+      type(sp_liquid_field), intent(in) :: p, u, v, w
+      integer(c_int), value :: device
+    end function sp_compute_uvw_liquid_field
 
     integer(c_int) function sp_new_cell_face(ctx, p, u, v, w, iu, iv, iw, device) &
         bind(C, name="sp_new_cell_face")

@@ -325,6 +325,27 @@ sp_status sp_define_momentum_exchange_cell(sp_ctx* ctx, const int32_t* cell_type
 sp_status sp_number_external_cell(sp_ctx* ctx, int iu, int iv, int iw, const int32_t* cell_type,
                                   int32_t* posnu, int32_t* ext_cell, int device);
 
+/* The liquid fields of one grid for ComputeUVWLiquidField: TGrid%dx, dy,
+ * dz and TCell%phiL, nxL, nyL, nzL, vofL (cell arrays (1:Imax,1:Jmax,
+ * 1:Kmax)). */
+typedef struct sp_liquid_field {
+  const double *dx, *dy, *dz;
+  double *phi_l, *nx_l, *ny_l, *nz_l, *vof_l;
+} sp_liquid_field;
+
+/* ComputeUVWLiquidField(PGrid,PCell,UCell,VCell,WCell,UGrid,VGrid,WGrid)
+ * (Clsvof.f90:174-298): U/V/W phiL, nxL, nyL, nzL averaged from the P cells
+ * (DirectionAverageArray, :328-353) and U/V/W vofL from the liquid volume
+ * of the two P half cells of each face (Volume_Fraction_Calc, :1475-1550).
+ * Reads P's members and U/V/W's dx, dy, dz; writes U/V/W's phi_l, nx_l,
+ * ny_l, nz_l, vof_l.  The reference's x**3.d0 is a pow call, here as
+ * there, so vof_l can differ from a host build by the last bits of pow;
+ * everything else is bit-identical.  `device`: 1 when the pointers are
+ * device memory, 0 for host memory. */
+sp_status sp_compute_uvw_liquid_field(sp_ctx* ctx, const sp_liquid_field* p,
+                                      const sp_liquid_field* u, const sp_liquid_field* v,
+                                      const sp_liquid_field* w, int device);
+
 /* HIP stream the context runs on (hipStream_t as void*). */
 void* sp_stream(sp_ctx* ctx);
 

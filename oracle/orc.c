@@ -1630,3 +1630,146 @@ void orc_new_cell_face(int I, int J, int K, const orc_cell* P, orc_cell* U, orc_
                   X[m]->eta[ax], X[m]->al[ax], X[m]->dal[ax]);
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* ComputeUVWLiquidField (Clsvof.f90:174-298) with                      */
+/* ComputeVelocityCellLiquidField (:300-326), DirectionAverageArray     */
+/* (:328-353), Volume_Fraction_Calc (:1475-1550) and                    */
+/* ConvertTheCoordinateSystem (:2458-2465).                             */
+/* ------------------------------------------------------------------ */
+
+/* ConvertTheCoordinateSystem: the level set measured from the cell corner */
+static double orc_corner_dist(double dx, double dy, double dz, double nx, double ny, double nz,
+                              double phi) {
+  return phi + 0.5 * (fabs(nx) * dx + fabs(ny) * dy + fabs(nz) * dz);
+}
+
+/* Volume_Fraction_Calc (Griebel and Klitz): the volume of the box on the
+ * corner side of the plane at distance s.  x**3.d0 is pow(x, 3.0) (a real
+ * exponent); x**2.d0 is x*x. */
+double orc_volume_fraction(double dx, double dy, double dz, double nxx, double nyy, double nzz,
+                           double s) {
+  const double nx1 = fabs(nxx), ny1 = fabs(nyy), nz1 = fabs(nzz);
+  const double a = nx1 * dx, b = ny1 * dy, c = nz1 * dz;
+  double dx1 = a, dz1 = a;
+  if (b > dx1) dx1 = b;
+  if (c > dx1) dx1 = c;
+  if (b < dz1) dz1 = b;
+  if (c < dz1) dz1 = c;
+  const double dy1 = (a + b + c) - dx1 - dz1;
+  const double sm = dx1 + dy1 + dz1;
+  const double sc = s < sm - s ? s : sm - s;
+  if (s <= 0.0) return 0.0;
+  if (s >= sm) return 1.0 * dx * dy * dz;
+  double fc;
+  if (sc <= dz1) {
+    fc = pow(sc, 3.0) / (6.0 * dx1 * dy1 * dz1);
+  } else if (sc <= dy1) {
+    fc = (sc * sc - dz1 * sc + dz1 * dz1 / 3.0) / (2.0 * dx1 * dy1);
+  } else if (dx1 >= dy1 + dz1) {
+    if (sc >= dy1 + dz1) {
+      fc = (2.0 * sc - dy1 - dz1) / (2.0 * dx1);
+    } else {
+      fc = (pow(sc, 3.0) - pow(sc - dz1, 3.0) - pow(sc - dy1, 3.0));
+      fc = fc / (6.0 * dx1 * dy1 * dz1);
+    }
+  } else {
+    if (sc > dx1) fc = (pow(sc, 3.0) - pow(sc - dz1, 3.0) - pow(sc - dy1, 3.0) - pow(sc - dx1, 3.0));
+    else fc = (pow(sc, 3.0) - pow(sc - dz1, 3.0) - pow(sc - dy1, 3.0));
+    fc = fc / (6.0 * dx1 * dy1 * dz1);
+  }
+  if (s <= 0.5 * sm) return fc * dx * dy * dz;
+  return (1.0 - fc) * dx * dy * dz;
+}
+
+/* the liquid volume of the part (d1, d2, d3) of a P cell whose level set,
+ * moved to that part as the reference moves it, is phin */
+static double liquid_part(double d1, double d2, double d3, double nx, double ny, double nz,
+                          double phin) {
+  const double dist = orc_corner_dist(d1, d2, d3, nx, ny, nz, phin);
+  const double v = orc_volume_fraction(d1, d2, d3, nx, ny, nz, dist);
+  return d1 * d2 * d3 - v;
+}
+
+void orc_uvw_liquid_field(int I, int J, int K, const orc_liquid* P, orc_liquid* U, orc_liquid* V,
+                          orc_liquid* W) {
+  const orc_grid G = {.I = I, .J = J, .K = K};
+  const orc_grid* g = &G;
+  orc_liquid* C[3] = {U, V, W};
+  const double* pd[3] = {P->dx, P->dy, P->dz};
+  /* DirectionAverageArray of phiL, nxL, nyL, nzL (:311-315, :328-353) */
+  for (int m = 0; m < 3; ++m) {
+    const double* in[4] = {P->phi, P->nx, P->ny, P->nz};
+    double* out[4] = {C[m]->phi, C[m]->nx, C[m]->ny, C[m]->nz};
+    const int iu = m == 0, iv = m == 1, iw = m == 2;
+    for (int f = 0; f < 4; ++f) {
+      for (int i = 1; i <= I - iu; ++i)
+        for (int j = 1; j <= J - iv; ++j)
+          for (int k = 1; k <= K - iw; ++k) {
+            const int64_t q = NX(g, i, j, k), qn = NX(g, i + iu, j + iv, k + iw);
+            const double lamda = pd[m][qn] / (pd[m][q] + pd[m][qn]);
+            out[f][q] = lamda * in[f][q] + (1.0 - lamda) * in[f][qn];
+          }
+      for (int i = 1; i <= I; ++i)
+        for (int j = 1; j <= J; ++j)
+          for (int k = 1; k <= K; ++k) {
+            const int idx[3] = {i, j, k};
+            const int L = m == 0 ? I : (m == 1 ? J : K);
+            if (idx[m] != L) continue;
+            int p3[3] = {i, j, k};
+            p3[m] -= 1;
+            out[f][NX(g, i, j, k)] = out[f][NX(g, p3[0], p3[1], p3[2])];
+          }
+    }
+  }
+  /* U (:189-216) */
+  for (int j = 1; j <= J; ++j)
+    for (int k = 1; k <= K; ++k) {
+      for (int i = 1; i <= I - 1; ++i) {
+        const int64_t q = NX(g, i, j, k), qn = NX(g, i + 1, j, k);
+        double dc = P->dx[q] / 2.0;
+        double phin = P->phi[q] + dc * P->nx[q];
+        const double vl = liquid_part(dc, P->dy[q], P->dz[q], P->nx[q], P->ny[q], P->nz[q], phin);
+        dc = P->dx[qn] / 2.0;
+        phin = P->phi[qn] - dc * P->nx[qn];
+        const double vr = liquid_part(dc, P->dy[qn], P->dz[qn], P->nx[qn], P->ny[qn], P->nz[qn], phin);
+        U->vof[q] = (vl + vr) / U->dx[q] / U->dy[q] / U->dz[q];
+      }
+      U->vof[NX(g, I, j, k)] = P->vof[NX(g, I, j, k)];
+    }
+  /* V (:218-257): the south part uses phiL itself with the full dy for the
+   * corner distance; the north part moves phiL(i,j,k) by nyL(i,j+1,k) */
+  for (int i = 1; i <= I; ++i)
+    for (int k = 1; k <= K; ++k) {
+      for (int j = 1; j <= J - 1; ++j) {
+        const int64_t q = NX(g, i, j, k), qn = NX(g, i, j + 1, k);
+        double dc = P->dy[q] / 2.0;
+        double dist = orc_corner_dist(P->dx[q], P->dy[q], P->dz[q], P->nx[q], P->ny[q], P->nz[q],
+                                      P->phi[q]);
+        double v = orc_volume_fraction(P->dx[q], dc, P->dz[q], P->nx[q], P->ny[q], P->nz[q], dist);
+        const double vl = P->dx[q] * dc * P->dz[q] - v;
+        dc = P->dy[qn] / 2.0;
+        const double phin = P->phi[q] + dc * P->ny[qn];
+        dist = orc_corner_dist(P->dx[qn], dc, P->dz[qn], P->nx[qn], P->ny[qn], P->nz[qn], phin);
+        v = orc_volume_fraction(P->dx[qn], dc, P->dz[qn], P->nx[qn], P->ny[qn], P->nz[qn], dist);
+        const double vr = P->dx[qn] * dc * P->dz[qn] - v;
+        V->vof[q] = (vl + vr) / V->dx[q] / V->dy[q] / V->dz[q];
+      }
+      V->vof[NX(g, i, J, k)] = P->vof[NX(g, i, J, k)];
+    }
+  /* W (:259-296): the top part moves phiL(k+1) by +dz/2 nzL(k+1) */
+  for (int i = 1; i <= I; ++i)
+    for (int j = 1; j <= J; ++j) {
+      for (int k = 1; k <= K - 1; ++k) {
+        const int64_t q = NX(g, i, j, k), qn = NX(g, i, j, k + 1);
+        double dc = P->dz[q] / 2.0;
+        double phin = P->phi[q] + dc * P->nz[q];
+        const double vl = liquid_part(P->dx[q], P->dy[q], dc, P->nx[q], P->ny[q], P->nz[q], phin);
+        dc = P->dz[qn] / 2.0;
+        phin = P->phi[qn] + dc * P->nz[qn];
+        const double vr = liquid_part(P->dx[qn], P->dy[qn], dc, P->nx[qn], P->ny[qn], P->nz[qn], phin);
+        W->vof[q] = (vl + vr) / W->dx[q] / W->dy[q] / W->dz[q];
+      }
+      W->vof[NX(g, i, j, K)] = P->vof[NX(g, i, j, K)];
+    }
+}

@@ -43,6 +43,12 @@
  *                       zero; DefineCell classes and vofL scaling; the
  *                       stretched-grid face centre rule and the FCT(:,2)
  *                       quirk on hand-built cells
+ *   orc_volume_fraction the exact volume of a plane-cut box by independent
+ *                       integration (numpy) and closed forms (empty, full,
+ *                       corner tetrahedron, axis-aligned slab)
+ *   orc_uvw_liquid_field  planar liquid level sets: U vofL equals the exact
+ *                       liquid fraction of the U cell; lambda-weighted
+ *                       averages; the V/W quirks on hand-built columns
  *   orc_define_moex,    vectorised restatements over shifted arrays (numpy),
  *   orc_number_external a cumulative count in C order of the (i,j,k) array
  * No reference golden vector exists for this path and HYPRE is absent, so the
@@ -221,6 +227,20 @@ int32_t orc_number_external(int I, int J, int K, int iu, int iv, int iw, const i
  * order, delh, and the nine Face*InterpolationInf calls. */
 void orc_new_cell_face(int I, int J, int K, const orc_cell* P, orc_cell* U, orc_cell* V,
                        orc_cell* W, orc_interp* iu, orc_interp* iv, orc_interp* iw);
+
+/* The liquid fields of one grid for ComputeUVWLiquidField: TGrid%dx, dy,
+ * dz and TCell%phiL, nxL, nyL, nzL, vofL (cell arrays). */
+typedef struct orc_liquid {
+  const double *dx, *dy, *dz;
+  double *phi, *nx, *ny, *nz, *vof;
+} orc_liquid;
+/* Volume_Fraction_Calc (Clsvof.f90:1475-1550). */
+double orc_volume_fraction(double dx, double dy, double dz, double nxx, double nyy, double nzz,
+                           double s);
+/* ComputeUVWLiquidField (Clsvof.f90:174-298): U/V/W phiL, nxL, nyL, nzL
+ * by DirectionAverageArray and vofL from the two halves of the P cells. */
+void orc_uvw_liquid_field(int I, int J, int K, const orc_liquid* P, orc_liquid* U, orc_liquid* V,
+                          orc_liquid* W);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,13 @@ class OrcInterp(ctypes.Structure):
         (n, ctypes.c_void_p) for n in INTERP_FIELDS]
 
 
+LIQUID_FIELDS = ("dx", "dy", "dz", "phi_l", "nx_l", "ny_l", "nz_l", "vof_l")
+
+
+class OrcLiquid(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in LIQUID_FIELDS]
+
+
 _lib = None
 
 
@@ -112,6 +119,11 @@ def lib():
         L.orc_grid_preprocess.restype = None
         L.orc_new_cell_face.argtypes = [ctypes.c_int] * 3 + [C] * 4 + [ctypes.POINTER(OrcInterp)] * 3
         L.orc_new_cell_face.restype = None
+        L.orc_volume_fraction.argtypes = [ctypes.c_double] * 7
+        L.orc_volume_fraction.restype = ctypes.c_double
+        Lq = ctypes.POINTER(OrcLiquid)
+        L.orc_uvw_liquid_field.argtypes = [ctypes.c_int] * 3 + [Lq] * 4
+        L.orc_uvw_liquid_field.restype = None
         L.orc_define_moex.argtypes = [ctypes.c_int] * 3 + [I32P] * 7
         L.orc_define_moex.restype = None
         L.orc_number_external.argtypes = [ctypes.c_int] * 6 + [I32P, I32P]
@@ -490,3 +502,26 @@ def new_cell_face(cells, interp):
     lib().orc_new_cell_face(I, J, K, *(ctypes.byref(s) for s in structs),
                             *(ctypes.byref(s) for s in xstructs))
     return cs, xs
+
+
+def volume_fraction(dx, dy, dz, nx, ny, nz, s):
+    """Volume_Fraction_Calc: the volume on the corner side of the plane."""
+    return lib().orc_volume_fraction(dx, dy, dz, nx, ny, nz, s)
+
+
+def uvw_liquid_field(cells):
+    """ComputeUVWLiquidField on copies: cells = four dicts (P, U, V, W) keyed
+    by LIQUID_FIELDS (cell arrays); P's are read, U/V/W's phi_l, nx_l,
+    ny_l, nz_l and vof_l returned in copies."""
+    I, J, K = cells[0]["dx"].shape
+    outs, structs = [], []
+    for c in cells:
+        o = {}
+        for n in LIQUID_FIELDS:
+            a = c.get(n)
+            o[n] = (np.zeros((I, J, K), order="F") if a is None else
+                    np.array(a, dtype=np.float64, order="F", copy=True))
+        outs.append(o)
+        structs.append(OrcLiquid(*(o[n].ctypes.data for n in LIQUID_FIELDS)))
+    lib().orc_uvw_liquid_field(I, J, K, *(ctypes.byref(s) for s in structs))
+    return outs

@@ -958,6 +958,61 @@ sp_status sp_new_cell_face(sp_ctx* c, const sp_cell_geom* p, const sp_cell_geom*
   return SP_OK;
 }
 
+sp_status sp_compute_uvw_liquid_field(sp_ctx* c, const sp_liquid_field* p,
+                                      const sp_liquid_field* u, const sp_liquid_field* v,
+                                      const sp_liquid_field* w, int device) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  const sp_liquid_field* g[4] = {p, u, v, w};
+  for (int m = 0; m < 4; ++m) {
+    if (!g[m]) return fail(c, SP_ERR_INVALID_ARG, "null sp_liquid_field");
+    const sp_liquid_field& a = *g[m];
+    const void* req[] = {a.dx, a.dy, a.dz, a.phi_l, a.nx_l, a.ny_l, a.nz_l, a.vof_l};
+    for (const void* q : req)
+      if (!q) return fail(c, SP_ERR_INVALID_ARG, "sp_liquid_field member is null");
+  }
+  const size_t N = (size_t)c->dm.N;
+  sp_status s;
+  if (device) {
+    if ((s = uvw_liquid_field(c, g))) return s;
+  } else {
+    size_t k = 250;  // staging slots shared with the other geometry entry points' range
+    sp_liquid_field d[4];
+    for (int m = 0; m < 4; ++m) {
+      const sp_liquid_field& h = *g[m];
+      sp_liquid_field& o = d[m];
+      if ((s = upload(c, k++, h.dx, N, &o.dx)) || (s = upload(c, k++, h.dy, N, &o.dy)) ||
+          (s = upload(c, k++, h.dz, N, &o.dz)))
+        return s;
+      double* const hs[5] = {h.phi_l, h.nx_l, h.ny_l, h.nz_l, h.vof_l};
+      double** os[5] = {&o.phi_l, &o.nx_l, &o.ny_l, &o.nz_l, &o.vof_l};
+      for (int f = 0; f < 5; ++f) {
+        if (m == 0) {
+          const double* t;
+          if ((s = upload(c, k++, static_cast<const double*>(hs[f]), N, &t))) return s;
+          *os[f] = const_cast<double*>(t);
+        } else {
+          void* b;
+          if ((s = stage(c, k++, N * sizeof(double), &b))) return s;
+          *os[f] = static_cast<double*>(b);
+        }
+      }
+    }
+    const sp_liquid_field* dg[4] = {&d[0], &d[1], &d[2], &d[3]};
+    if ((s = uvw_liquid_field(c, dg))) return s;
+    for (int m = 1; m < 4; ++m) {
+      double* const hs[5] = {g[m]->phi_l, g[m]->nx_l, g[m]->ny_l, g[m]->nz_l, g[m]->vof_l};
+      double* const os[5] = {d[m].phi_l, d[m].nx_l, d[m].ny_l, d[m].nz_l, d[m].vof_l};
+      for (int f = 0; f < 5; ++f)
+        if ((s = pipe_download(c, hs[f], os[f], N * sizeof(double)))) return s;
+    }
+  }
+  const hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "sp_compute_uvw_liquid_field");
+  timers_collect(c);
+  return SP_OK;
+}
+
 sp_status sp_define_momentum_exchange_cell(sp_ctx* c, const int32_t* ctp, const int32_t* ctu,
                                            const int32_t* ctv, const int32_t* ctw, int32_t* mu,
                                            int32_t* mv, int32_t* mw, int device) {

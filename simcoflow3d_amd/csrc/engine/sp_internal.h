@@ -249,6 +249,8 @@ sp_status number_external(sp_ctx* c, int iu, int iv, int iw, const int32_t* ct, 
 // NewCellFace (sp_newcellface.hip); scratch holds 9 N doubles.
 sp_status new_cell_face(sp_ctx* c, const sp_cell_geom* const g[4],
                         const sp_cell_interp* const x[3], double* scratch);
+// ComputeUVWLiquidField (sp_liquid.hip) on device arrays P, U, V, W.
+sp_status uvw_liquid_field(sp_ctx* c, const sp_liquid_field* const g[4]);
 // ---- VTR output (sp_vtr.hip) ----
 sp_status write_vtr(sp_ctx* c, const char* dir, const char* prefix, int64_t itt,
                     const sp_vtr_fields& f, bool device);

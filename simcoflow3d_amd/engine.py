@@ -35,8 +35,11 @@ EXPORTED_SYMBOLS = (
     "sp_stream", "sp_kernel_times", "sp_set_profiling", "sp_set_profiling_filter",
     "sp_strerror", "sp_last_error", "sp_wall_step", "sp_wall_step_dev", "sp_write_vtr",
     "sp_grid_preprocess", "sp_grid_preprocess_dev", "sp_define_momentum_exchange_cell",
-    "sp_number_external_cell", "sp_new_cell_face",
+    "sp_number_external_cell", "sp_new_cell_face", "sp_compute_uvw_liquid_field",
 )
+
+# members of sp_liquid_field, in order
+LIQUID_FIELDS = ("dx", "dy", "dz", "phi_l", "nx_l", "ny_l", "nz_l", "vof_l")
 
 # members of sp_cell_interp, in order
 INTERP_FIELDS = ("moex", "msce", "sxe", "etae", "ale", "dale", "syn", "etan", "aln", "daln",
@@ -157,6 +160,10 @@ class SpCellInterp(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in INTERP_FIELDS]
 
 
+class SpLiquidField(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in LIQUID_FIELDS]
+
+
 class SpError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{msg} (status {status})")
@@ -207,6 +214,8 @@ def load():
         f = getattr(L, name)
         f.argtypes = [V] + [P(SpCellGeom)] * 4
         f.restype = ctypes.c_int
+    L.sp_compute_uvw_liquid_field.argtypes = [V] + [P(SpLiquidField)] * 4 + [ctypes.c_int]
+    L.sp_compute_uvw_liquid_field.restype = ctypes.c_int
     L.sp_new_cell_face.argtypes = [V] + [P(SpCellGeom)] * 4 + [P(SpCellInterp)] * 3 + [
         ctypes.c_int]
     L.sp_new_cell_face.restype = ctypes.c_int
@@ -414,6 +423,34 @@ class Projector:
         rc = self.lib.sp_grid_preprocess(self.h, *(ctypes.byref(s) for s in structs))
         self._check(rc, "sp_grid_preprocess")
         return outs
+
+    def uvw_liquid_field(self, cells, device=False):
+        """ComputeUVWLiquidField (sp_compute_uvw_liquid_field).  cells: four
+        dicts (P, U, V, W) keyed like sp_liquid_field (cell arrays).  Host:
+        returns the U, V, W dicts with new phi_l, nx_l, ny_l, nz_l, vof_l;
+        device (torch tensors): writes them in place."""
+        I, J, K = self.shape
+        if device:
+            gs = [SpLiquidField(*(_addr(c.get(n)) for n in LIQUID_FIELDS)) for c in cells]
+            rc = self.lib.sp_compute_uvw_liquid_field(self.h, *(ctypes.byref(a) for a in gs), 1)
+            self._check(rc, "sp_compute_uvw_liquid_field")
+            return None
+        outs, gs = [], []
+        for m, c in enumerate(cells):
+            o = {}
+            for n in LIQUID_FIELDS:
+                a = c.get(n)
+                if a is None and (m == 0 or n in ("dx", "dy", "dz")):
+                    raise ValueError(f"{n} is required")
+                o[n] = (np.zeros((I, J, K), order="F") if a is None else
+                        np.array(a, dtype=np.float64, order="F", copy=True))
+                if o[n].shape != (I, J, K):
+                    raise ValueError(f"{n}: expected shape {(I, J, K)}, got {o[n].shape}")
+            outs.append(o)
+            gs.append(SpLiquidField(*(_addr(o[n]) for n in LIQUID_FIELDS)))
+        rc = self.lib.sp_compute_uvw_liquid_field(self.h, *(ctypes.byref(a) for a in gs), 0)
+        self._check(rc, "sp_compute_uvw_liquid_field")
+        return outs[1:]
 
     def new_cell_face(self, cells, interp, device=False):
         """NewCellFace (sp_new_cell_face).  cells: four dicts (P, U, V, W)

@@ -14,6 +14,7 @@ program sp_layout
   type(sp_vtr_fields), target :: vf
   type(sp_cell_geom), target :: cg
   type(sp_cell_interp), target :: ci
+  type(sp_liquid_field), target :: lf
   print '(a,1x,i0)', 'sp_params', c_sizeof(prm)
   print '(a,1x,i0)', 'sp_inputs', c_sizeof(inp)
   print '(a,1x,i0)', 'sp_outputs', c_sizeof(outp)
@@ -24,6 +25,8 @@ program sp_layout
   print '(a,1x,i0)', 'sp_vtr_fields', c_sizeof(vf)
   print '(a,1x,i0)', 'sp_cell_geom', c_sizeof(cg)
   print '(a,1x,i0)', 'sp_cell_interp', c_sizeof(ci)
+  print '(a,1x,i0)', 'sp_liquid_field', c_sizeof(lf)
+  print '(a,1x,i0)', 'sp_liquid_field.vof_l', off(c_loc(lf%vof_l), c_loc(lf))
   print '(a,1x,i0)', 'sp_cell_interp.sxe', off(c_loc(ci%sxe), c_loc(ci))
   print '(a,1x,i0)', 'sp_cell_interp.dalt', off(c_loc(ci%dalt), c_loc(ci))
   print '(a,1x,i0)', 'sp_cell_geom.cell_type', off(c_loc(cg%cell_type), c_loc(cg))
