@@ -419,6 +419,7 @@ def geometry(shape, steps, warmup, cpu=True):
                kernels=kern)
     del dev, keep
     torch.cuda.empty_cache()
+    rec["liquid_field"] = liquid_field(shape, steps, warmup, cpu)
     if cpu:
         from oracle import orc
         # bounded sample: the oracle on a 96^3 case of the same workload,
@@ -436,6 +437,74 @@ def geometry(shape, steps, warmup, cpu=True):
                                    kind="port", seconds=round(secs, 3),
                                    sample=f"{runs} oracle GridPreprocess calls on the same "
                                           f"workload at {sample[0]}^3")
+    return rec
+
+
+def liquid_field(shape, steps, warmup, cpu=True):
+    """ComputeUVWLiquidField (sp_compute_uvw_liquid_field, device arrays) on
+    the same grid with a liquid level set of three spheres: timed calls,
+    the kernel's algorithmic GB/s, and the oracle on a bounded sample."""
+    import numpy as np
+    import torch
+
+    from simcoflow3d_amd import configs
+    from simcoflow3d_amd.engine import LIQUID_FIELDS, Projector
+
+    I, J, K = shape
+
+    def case(sz):
+        dzp = 1.02 ** np.arange(sz[2])
+        grids = configs.staggered_spacings(np.full(sz[0], 1.0 / sz[0]),
+                                           np.full(sz[1], 1.0 / sz[1]), dzp / dzp.sum())
+        g = grids[0]
+        phi, nx, ny, nz = configs.solid_level_set("bodies", g["x"], g["y"], g["z"],
+                                                  np.random.default_rng(5))
+        P = dict(dx=g["dx"], dy=g["dy"], dz=g["dz"], phi_l=phi, nx_l=nx, ny_l=ny, nz_l=nz,
+                 vof_l=np.random.default_rng(6).uniform(size=sz))
+        return [P] + [dict(dx=q["dx"], dy=q["dy"], dz=q["dz"]) for q in grids[1:]]
+
+    cells = case(shape)
+    dev = [{n: (torch.from_numpy(np.ascontiguousarray(np.asarray(c[n]).T)).to("cuda")
+                if n in c else torch.zeros((K, J, I), dtype=torch.float64, device="cuda"))
+            for n in LIQUID_FIELDS} for c in cells]
+    del cells
+    P = Projector(shape)
+    for _ in range(max(1, warmup)):
+        P.uvw_liquid_field(dev, device=True)
+    times = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        P.uvw_liquid_field(dev, device=True)
+        times.append(time.perf_counter() - t0)
+    P.set_profiling(True)
+    P.uvw_liquid_field(dev, device=True)
+    r = [x for x in P.kernel_times() if x["name"] == "liquid_uvw"][0]
+    P.set_profiling(False)
+    P.close()
+    del dev
+    torch.cuda.empty_cache()
+    N = I * J * K
+    el = sum(times) / len(times)
+    rec = dict(workload=f"ComputeUVWLiquidField, {I}x{J}x{K}, three spherical drops, "
+                        "z stretched 2% per cell",
+               calls=steps, ms_per_call=round(el * 1e3, 3), cells_per_s=round(N / el, 1),
+               kernel_us=round(r["total_ms"] * 1e3 / r["launches"], 1),
+               kernel_gbs=round(r["bytes"] / (r["total_ms"] * 1e-3) / 1e9, 1),
+               bytes_per_cell=256)
+    if cpu:
+        from oracle import orc
+        sample = (96, 96, 96)
+        sc = case(sample)
+        runs, secs = 0, 0.0
+        while runs == 0 or (secs < 10.0 and runs < 40):
+            t0 = time.perf_counter()
+            orc.uvw_liquid_field(sc)
+            secs += time.perf_counter() - t0
+            runs += 1
+        rec["cpu_baseline"] = dict(value=round(runs * 96 ** 3 / secs, 1), unit="cells/s",
+                                   cores=1, kind="port", seconds=round(secs, 3),
+                                   sample=f"{runs} oracle calls on the same workload at 96^3")
     return rec
 
 
