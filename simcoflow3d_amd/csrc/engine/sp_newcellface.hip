@@ -163,14 +163,10 @@ struct InterpArgs {
   int ax, ib;
 };
 
-// Face*InterpolationInf for face direction ax of one grid: one thread per
-// face (i,j,k) with index < Imax (Jmax, Kmax) along ax.
-__global__ void __launch_bounds__(256) k_ncf_interp(NcfDims D, InterpArgs A) {
+// Face*InterpolationInf for face direction A.ax of one grid at face
+// (i,j,k), index < Imax (Jmax, Kmax) along ax.
+__device__ inline void interp_face(const NcfDims& D, const InterpArgs& A, int i, int j, int k) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  if (i > D.I || j > D.J) return;
   const int ax = A.ax;
   const int idx[3] = {i, j, k};
   const int L = ax == 0 ? D.I : (ax == 1 ? D.J : D.K);
@@ -236,6 +232,20 @@ __global__ void __launch_bounds__(256) k_ncf_interp(NcfDims D, InterpArgs A) {
   if (delh < kEpsiI && A.area[h] > 0.0) delh = delhec;
   A.Al[h] = fabs(delhec) / fabs(delh);
   A.DAl[h] = (s1 * nf1 + s2 * nf2) / (S * fabs(delh));
+}
+
+// The three face directions of one grid in one launch: a thread handles
+// the E, N and T faces of its cell, so the cell's own centre, normal,
+// level set and type are read once for the three.
+__global__ void __launch_bounds__(256) k_ncf_interp(NcfDims D, InterpArgs AE, InterpArgs AN,
+                                                    InterpArgs AT) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > D.I || j > D.J) return;
+  interp_face(D, AE, i, j, k);
+  interp_face(D, AN, i, j, k);
+  interp_face(D, AT, i, j, k);
 }
 
 // The entry at index Imax (Jmax, Kmax) along ax copies the one before it
@@ -309,10 +319,11 @@ sp_status new_cell_face(sp_ctx* c, const sp_cell_geom* const g[4],
     double* Eta[3] = {X.etae, X.etan, X.etat};
     double* Al[3] = {X.ale, X.aln, X.alt};
     double* DAl[3] = {X.dale, X.daln, X.dalt};
+    InterpArgs AA[3];
     for (int ax = 0; ax < 3; ++ax) {
       // the grid crossed by the face: P along the grid's own axis, else U/V/W
       const sp_cell_geom& B = ax == m ? P : *g[1 + ax];
-      InterpArgs A;
+      InterpArgs& A = AA[ax];
       A.d[0] = T.dx; A.d[1] = T.dy; A.d[2] = T.dz;
       A.n[0] = T.nx; A.n[1] = T.ny; A.n[2] = T.nz;
       A.phi = T.phi; A.cent = T.cell_cent;
@@ -323,7 +334,10 @@ sp_status new_cell_face(sp_ctx* c, const sp_cell_geom* const g[4],
       A.S = S[ax]; A.Eta = Eta[ax]; A.Al = Al[ax]; A.DAl = DAl[ax];
       A.ax = ax;
       A.ib = ax == m ? 1 : 0;
-      hipLaunchKernelGGL(k_ncf_interp, cells, blk, 0, c->stream, D, A);
+    }
+    hipLaunchKernelGGL(k_ncf_interp, cells, blk, 0, c->stream, D, AA[0], AA[1], AA[2]);
+    for (int ax = 0; ax < 3; ++ax) {
+      const InterpArgs& A = AA[ax];
       const int na = ax == 0 ? D.J : D.I, nb = ax == 2 ? D.J : D.K;
       hipLaunchKernelGGL(k_ncf_last, dim3((unsigned)((na + 63) / 64), (unsigned)((nb + 3) / 4)),
                          blk, 0, c->stream, D, A);
