@@ -1058,7 +1058,7 @@ sp_status sp_number_external_cell(sp_ctx* c, int iu, int iv, int iw, const int32
   const Dims& D = c->dm;
   const size_t N = (size_t)D.N;
   const size_t NP = (size_t)(D.I - iu) * (D.J - iv) * (D.K - iw);
-  const size_t NC = (size_t)(D.I - iu) * (D.J - iv);
+  const size_t NC = number_scratch(D.I - iu, D.J - iv, D.K - iw);
   void *cnt, *ext;
   sp_status s;
   if ((s = stage(c, 140, NC * sizeof(int32_t), &cnt)) ||

@@ -241,11 +241,13 @@ sp_status wall_step(sp_ctx* c, const sp_inputs& in, const sp_outputs& out, bool 
 // GridPreprocess on the four device grids P, U, V, W (sp_cutcell.hip).
 sp_status grid_preprocess(sp_ctx* c, const sp_cell_geom* const g[4]);
 // DefineMomentumExchangeCell and NumberExternalCell (sp_celllinks.hip);
-// cnt holds (Imax-iu)*(Jmax-iv) counts, ext one int32 (device memory).
+// cnt holds number_scratch(Imax-iu, Jmax-iv, Kmax-iw) int32, ext one int32
+// (device memory).
 sp_status define_moex(sp_ctx* c, const int32_t* ctp, const int32_t* ctu, const int32_t* ctv,
                       const int32_t* ctw, int32_t* mu, int32_t* mv, int32_t* mw);
 sp_status number_external(sp_ctx* c, int iu, int iv, int iw, const int32_t* ct, int32_t* pos,
                           int32_t* cnt, int32_t* ext);
+size_t number_scratch(int I2, int J2, int K2);
 // NewCellFace (sp_newcellface.hip); scratch holds 9 N doubles.
 sp_status new_cell_face(sp_ctx* c, const sp_cell_geom* const g[4],
                         const sp_cell_interp* const x[3], double* scratch);

@@ -233,3 +233,19 @@ def test_new_cell_face_requires_p_spacing(gpu_available):
     assert rc == SP_ERR_INVALID_ARG
     P.close()
     del keep
+
+
+@pytest.mark.parametrize("size", [(300, 200, 130), (5, 3, 200)])
+def test_numbering_large_and_deep(gpu_available, size):
+    """NumberExternalCell where the piece counts span several scan
+    workgroups (300x200 columns of three 64-plane pieces) and where one
+    column holds several pieces (200 planes)."""
+    print(f"[geometry] PosNu size={size}", flush=True)
+    rng = np.random.default_rng(3)
+    ct = np.asfortranarray(rng.integers(0, 3, size).astype(np.int32))
+    P = Projector(size)
+    for iu, iv, iw in ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)):
+        epos, eext = orc.number_external(ct, iu, iv, iw)
+        pos, ext = P.number_external(ct, iu, iv, iw)
+        assert ext == eext and np.array_equal(pos, epos), (iu, iv, iw)
+    P.close()
