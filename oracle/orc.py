@@ -186,9 +186,10 @@ def num_threads():
 
 
 def baseline_project(case, levels=0, pre=2, post=2, coarse_sweeps=4, coarse_scale=0.5,
-                     div_tol=1e-10, max_iter=200, omega=1.25, omega_coarse=1.5):
+                     div_tol=1e-10, max_iter=200, omega=1.25, omega_coarse=1.5, pin=(1, 1, 1)):
     """The CPU baseline: the same path and multigrid-preconditioned CG as the
-    engine, FP64, OpenMP over the host cores.  Returns (seconds, info)."""
+    engine, FP64, OpenMP over the host cores.  Returns (seconds, info); info
+    holds the pressure field Pp (gauge ``pin``) of the solve."""
     import time
     t0 = time.perf_counter()
     pc = gfm_coef_fast(case)
@@ -200,13 +201,15 @@ def baseline_project(case, levels=0, pre=2, post=2, coarse_sweeps=4, coarse_scal
              coarse_scale=coarse_scale, div_tol=div_tol, max_iter=max_iter, omega=omega,
              omega_coarse=omega_coarse)
     x, st, sing = pcg(case, op, b, o)
+    x = gauge(case, d, sing, x, pin)
     t2 = time.perf_counter()
     Pp = pressure_field(case, x)
     u, v, w = correct(case, pc, Pp)
     _, dinf = mres(case, u, v, w)
     t3 = time.perf_counter()
     return t3 - t0, dict(assemble_s=t1 - t0, solve_s=t2 - t1, correct_s=t3 - t2,
-                         iterations=st.iterations, converged=st.converged, div_inf=dinf)
+                         iterations=st.iterations, converged=st.converged, div_inf=dinf,
+                         Pp=Pp)
 
 
 def matrix_rows(case, pc):
@@ -332,16 +335,25 @@ class OracleResult:
     d: np.ndarray
 
 
-def project(case, pin=(1, 1, 1), tol=None, rows=True, o=None, u_init=None, bc_val=None):
+def project(case, pin=(1, 1, 1), tol=None, rows=True, o=None, u_init=None, bc_val=None,
+            gfm_fast=False, max_iter=40):
     """Reference path on the CPU.  ``rows=True`` solves the reference's own
     assembled rows (ProjectionP.f90:330-478); ``rows=False`` the engine's
-    symmetric face operator.  ``tol`` defaults to a machine-precision stop.
-    ``bc_val``: face values of the Dirichlet pressure faces (W,E,S,N,B,T)."""
-    pc = gfm_coef(case)
+    symmetric face operator.  ``tol`` defaults to a machine-precision stop
+    (repeated solves until the divergence stops falling); a given ``tol``
+    runs one solve to max |r|/V <= tol or ``max_iter`` iterations, with the
+    over-relaxed multigrid of the CPU baseline.  ``gfm_fast``: the parallel
+    GFM restatement (bit-identical to the serial one, pinned on every
+    configuration).  ``bc_val``: face values of the Dirichlet pressure faces
+    (W,E,S,N,B,T)."""
+    pc = gfm_coef_fast(case) if gfm_fast else gfm_coef(case)
     b = rhs(case, pc, bc_val)
     op = sym_operator(case, pc)
     d, _ = sym_diag(case, *op)
-    o = o or opts(max_iter=1000, div_tol=tol if tol is not None else 0.0)
+    if o is None:
+        o = (opts(max_iter=1000, div_tol=0.0) if tol is None else
+             opts(max_iter=max_iter, div_tol=tol, coarse_scale=0.5, omega=1.25,
+                  omega_coarse=1.5))
     R = matrix_rows(case, pc) if rows else None
     if tol is None:
         # converge until the residual stops improving: run at a tiny target

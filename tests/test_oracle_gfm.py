@@ -103,7 +103,8 @@ def test_stale_beta_reuses_previous_loop_cell():
 
 
 @pytest.mark.parametrize("cfg,size", [(0, (16, 16, 16)), (1, (12, 14, 18)), (3, (10, 10, 10)),
-                                      (4, (32, 16, 16)), (2, (16, 16, 16))])
+                                      (4, (32, 16, 16)), (2, (16, 16, 16)), (1, None),
+                                      (4, (128, 64, 64))])
 def test_parallel_restatement_is_bit_identical(cfg, size):
     c = configs.generate(cfg, size=size)
     a = orc.gfm_coef(c)
