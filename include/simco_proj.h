@@ -325,6 +325,24 @@ sp_status sp_define_momentum_exchange_cell(sp_ctx* ctx, const int32_t* cell_type
 sp_status sp_number_external_cell(sp_ctx* ctx, int iu, int iv, int iw, const int32_t* cell_type,
                                   int32_t* posnu, int32_t* ext_cell, int device);
 
+/* Allocation-free variant of sp_number_external_cell on device arrays.  It
+ * has no counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  cell_type, posnu and ext_cell
+ * are device memory; the same numbering as sp_number_external_cell is
+ * written to posnu and ExtCell to *ext_cell.  The scan's scratch is a
+ * static 16 MiB device array (2^22 int32) that each GPU the library runs on
+ * holds until the process ends, shared by every call on that GPU: calls
+ * that overlap on one GPU (other host threads, other contexts' streams)
+ * race on it.  The call only enqueues work on sp_stream(ctx) and returns;
+ * the outputs are valid once that stream has synchronized.  A grid whose
+ * numbering needs more scratch (more than about 4.19 M pieces of 64 planes,
+ * (Imax-iu)*(Jmax-iv)*ceil((Kmax-iw)/64)) returns SP_ERR_INVALID_ARG.  No
+ * hipMalloc, hipFree or allocating library routine is called. */
+sp_status sp_number_external_cell_noalloc(sp_ctx* ctx, int iu, int iv, int iw,
+                                          const int32_t* cell_type, int32_t* posnu,
+                                          int32_t* ext_cell);
+
 /* The liquid fields of one grid for ComputeUVWLiquidField: TGrid%dx, dy,
  * dz and TCell%phiL, nxL, nyL, nzL, vofL (cell arrays (1:Imax,1:Jmax,
  * 1:Kmax)). */

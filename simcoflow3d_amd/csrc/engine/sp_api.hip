@@ -434,6 +434,12 @@ sp_status sp_create(sp_ctx** out, const sp_params* params, int device) {
     delete c;
     return SP_ERR_HIP;
   }
+  // the allocation-free entry points' scratch address on this device
+  if (noalloc_init(device) != hipSuccess) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    return SP_ERR_HIP;
+  }
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
       ncu > 0)

@@ -158,7 +158,7 @@ sp_status define_moex(sp_ctx* c, const int32_t* ctp, const int32_t* ctu, const i
 }
 
 sp_status number_external(sp_ctx* c, int iu, int iv, int iw, const int32_t* ct, int32_t* pos,
-                          int32_t* cnt, int32_t* ext) {
+                          int32_t* cnt, int32_t* ext, bool timed) {
   const Dims& D = c->dm;
   const int I2 = D.I - iu, J2 = D.J - iv, K2 = D.K - iw;
   const int nc = number_chunks(K2);
@@ -167,8 +167,8 @@ sp_status number_external(sp_ctx* c, int iu, int iv, int iw, const int32_t* ct, 
   int32_t* bsum = cnt + n;
   const dim3 blk(64, 4, 1);
   const dim3 grid((unsigned)((I2 + 63) / 64), (unsigned)((J2 + 3) / 4), (unsigned)nc);
-  int slot;
-  timer_begin(c, "number_cells", (double)I2 * J2 * K2 * 8, &slot);
+  int slot = -1;
+  if (timed) timer_begin(c, "number_cells", (double)I2 * J2 * K2 * 8, &slot);
   hipLaunchKernelGGL(k_number_count, grid, blk, 0, c->stream, D.I, D.J, I2, J2, K2, nc, ct, cnt);
   hipLaunchKernelGGL(k_number_scan, dim3((unsigned)nb), dim3(kScanThreads), 0, c->stream, n, cnt,
                      bsum);

@@ -246,8 +246,12 @@ sp_status grid_preprocess(sp_ctx* c, const sp_cell_geom* const g[4]);
 sp_status define_moex(sp_ctx* c, const int32_t* ctp, const int32_t* ctu, const int32_t* ctv,
                       const int32_t* ctw, int32_t* mu, int32_t* mv, int32_t* mw);
 sp_status number_external(sp_ctx* c, int iu, int iv, int iw, const int32_t* ct, int32_t* pos,
-                          int32_t* cnt, int32_t* ext);
+                          int32_t* cnt, int32_t* ext, bool timed = true);
 size_t number_scratch(int I2, int J2, int K2);
+// The allocation-free entry points' static device scratch (sp_noalloc.hip):
+// its address on `device` (the current device) is looked up once per device
+// by sp_create.
+hipError_t noalloc_init(int device);
 // NewCellFace (sp_newcellface.hip); scratch holds 9 N doubles.
 sp_status new_cell_face(sp_ctx* c, const sp_cell_geom* const g[4],
                         const sp_cell_interp* const x[3], double* scratch);

@@ -36,6 +36,7 @@ EXPORTED_SYMBOLS = (
     "sp_strerror", "sp_last_error", "sp_wall_step", "sp_wall_step_dev", "sp_write_vtr",
     "sp_grid_preprocess", "sp_grid_preprocess_dev", "sp_define_momentum_exchange_cell",
     "sp_number_external_cell", "sp_new_cell_face", "sp_compute_uvw_liquid_field",
+    "sp_number_external_cell_noalloc",
 )
 
 # members of sp_liquid_field, in order
@@ -223,6 +224,8 @@ def load():
     L.sp_define_momentum_exchange_cell.restype = ctypes.c_int
     L.sp_number_external_cell.argtypes = [V] + [ctypes.c_int] * 3 + [V, V, I32P, ctypes.c_int]
     L.sp_number_external_cell.restype = ctypes.c_int
+    L.sp_number_external_cell_noalloc.argtypes = [V] + [ctypes.c_int] * 3 + [V, V, V]
+    L.sp_number_external_cell_noalloc.restype = ctypes.c_int
     L.sp_stream.argtypes = [V]
     L.sp_stream.restype = V
     L.sp_kernel_times.argtypes = [V, P(SpKernelTime), ctypes.c_int]
@@ -536,6 +539,18 @@ class Projector:
                                               ctypes.byref(ext), 0)
         self._check(rc, "sp_number_external_cell")
         return pos, ext.value
+
+    def number_external_noalloc(self, cell_type, iu, iv, iw, posnu, ext_cell):
+        """NumberExternalCell without device allocation
+        (sp_number_external_cell_noalloc): cell_type, posnu and ext_cell
+        (one int32) are torch tensors on the context's device; the work is
+        enqueued on the context's stream; this wrapper then synchronizes the
+        device (every stream on it, the context's included)."""
+        import torch
+        rc = self.lib.sp_number_external_cell_noalloc(self.h, iu, iv, iw, _addr(cell_type),
+                                                      _addr(posnu), _addr(ext_cell))
+        self._check(rc, "sp_number_external_cell_noalloc")
+        torch.cuda.synchronize()
 
     def write_vtr(self, directory, prefix, itt, fields, device=False):
         """PrintResultVTR3D's file (sp_write_vtr): fields maps x, y, z (1-D) and
