@@ -23,13 +23,14 @@ module simco_proj
             sp_compute_uvw_liquid_field, &
             sp_number_external_cell, sp_strerror_f, sp_last_error_f
   public :: SP_OK, SP_ERR_INVALID_ARG, SP_ERR_HIP, SP_ERR_NOT_CONVERGED, SP_ERR_NONFINITE, &
-            SP_ERR_OOM, SP_ERR_GRID, SP_ERR_STATE, SP_ERR_NO_DEVICE, SP_PREC_FP64, &
+            SP_ERR_OOM, SP_ERR_GRID, SP_ERR_STATE, SP_ERR_NO_DEVICE, SP_ERR_CAPACITY, &
+            SP_PREC_FP64, &
             SP_PREC_MIXED, SP_PRECOND_MG, SP_PRECOND_JACOBI
 
   integer(c_int), parameter :: SP_OK = 0, SP_ERR_INVALID_ARG = -1, SP_ERR_HIP = -2, &
                                SP_ERR_NOT_CONVERGED = -3, SP_ERR_NONFINITE = -4, &
                                SP_ERR_OOM = -5, SP_ERR_GRID = -6, SP_ERR_STATE = -7, &
-                               SP_ERR_NO_DEVICE = -8
+                               SP_ERR_NO_DEVICE = -8, SP_ERR_CAPACITY = -9
   integer(c_int32_t), parameter :: SP_PREC_FP64 = 0, SP_PREC_MIXED = 1
   integer(c_int32_t), parameter :: SP_PRECOND_MG = 0, SP_PRECOND_JACOBI = 1
 
@@ -70,6 +71,8 @@ module simco_proj
     real(c_double) :: dt = 0.0_c_double
     type(c_ptr) :: beta_e = c_null_ptr, beta_n = c_null_ptr, beta_t = c_null_ptr
     type(c_ptr) :: wall = c_null_ptr      ! c_loc of a type(sp_wall_bc), or null
+    ! 1: geometry arrays unchanged since the previous sp_project call (no re-copy)
+    integer(c_int32_t) :: static_geometry = 0
   end type sp_inputs
 
   type, bind(C) :: sp_outputs

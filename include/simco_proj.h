@@ -43,7 +43,9 @@ typedef enum sp_status {
   SP_ERR_OOM = -5,          /* device allocation failed */
   SP_ERR_GRID = -6,         /* grid missing or not a tensor-product grid */
   SP_ERR_STATE = -7,        /* adjoint without a preceding forward call */
-  SP_ERR_NO_DEVICE = -8     /* no usable gfx950 device */
+  SP_ERR_NO_DEVICE = -8,    /* no usable gfx950 device */
+  SP_ERR_CAPACITY = -9      /* an engine work list is full: more GFM faces next to
+                               solid cells than min(6N, max(2^20, N/2)) */
 } sp_status;
 
 typedef enum sp_precision {
@@ -127,6 +129,15 @@ typedef struct sp_inputs {
   const double *beta_e, *beta_n, *beta_t;
   /* Optional: the wall step after the correction (NULL: the host keeps it). */
   const sp_wall_bc *wall;
+  /* sp_project only (host arrays): 1 when ee/ne/te_area, vof, vof_old, the
+   * four Cell_Type arrays and the three MoExCell arrays hold the same values
+   * as in the previous sp_project call on this context (the solid has not
+   * moved): the copies already on the device are reused instead of being
+   * sent over PCIe again.  Ignored (everything is copied) when the previous
+   * call on the context was not an sp_project call that copied them; 0
+   * always copies.  The caller answers for the claim: stale geometry is not
+   * detected. */
+  int32_t static_geometry;
 } sp_inputs;
 
 typedef struct sp_outputs {

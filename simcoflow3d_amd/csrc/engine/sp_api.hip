@@ -278,7 +278,9 @@ static sp_status project_device(sp_ctx* c, const sp_inputs* in, sp_outputs* out,
   auto t1 = std::chrono::steady_clock::now();
   stale = c->S_host->stale_count;
   if (stale > c->stale_cap)
-    return fail(c, SP_ERR_INVALID_ARG, "too many GFM faces next to unclassified cells");
+    return fail(c, SP_ERR_CAPACITY,
+                "too many GFM faces next to unclassified (solid) cells for the stale-face "
+                "list (" + std::to_string(stale) + " > " + std::to_string(c->stale_cap) + ")");
   if (c->S_host->nonfinite) return fail(c, SP_ERR_NONFINITE, "non-finite value in the assembled system");
   c->has_e = c->S_host->esum > 0.0;
   if ((s = mg_setup(c))) return s;
@@ -556,18 +558,36 @@ sp_status sp_project(sp_ctx* c, const sp_inputs* in, sp_outputs* out, sp_stats* 
   const size_t Nh = (size_t)D.Nh, N = (size_t)D.N;
   sp_inputs d = *in;
   size_t k = 0;
-  const double** hp[] = {&d.u_star, &d.v_star, &d.w_star, &d.dp_u, &d.dp_v, &d.dp_w, &d.ee_area,
-                         &d.ne_area, &d.te_area, &d.vof, &d.vof_old, &d.vof_l, &d.phi_l};
+  const double** hp[] = {&d.u_star, &d.v_star, &d.w_star, &d.dp_u, &d.dp_v, &d.dp_w,
+                         &d.vof_l, &d.phi_l};
   for (const double** p : hp)
     if ((s = upload(c, k++, *p, Nh, p))) return s;
-  const int32_t** ip[] = {&d.cell_type_p, &d.cell_type_u, &d.cell_type_v, &d.cell_type_w,
+  // The geometry (apertures, vof, vofO, cell types, MoExCell) keeps staging
+  // slots of its own, used by no other entry point, so a caller whose solid
+  // has not moved can leave it on the device (sp_inputs.static_geometry).
+  const double** gh[] = {&d.ee_area, &d.ne_area, &d.te_area, &d.vof, &d.vof_old};
+  const int32_t** gc[] = {&d.cell_type_p, &d.cell_type_u, &d.cell_type_v, &d.cell_type_w,
                           &d.moex_u, &d.moex_v, &d.moex_w};
-  for (const int32_t** p : ip)
-    if ((s = upload(c, k++, *p, N, p))) return s;
+  const size_t kGeo = 200;
+  if (in->static_geometry && c->geom_staged) {
+    for (size_t q = 0; q < 5; ++q) *gh[q] = static_cast<const double*>(c->stage[kGeo + q].ptr);
+    for (size_t q = 0; q < 7; ++q) *gc[q] = static_cast<const int32_t*>(c->stage[kGeo + 5 + q].ptr);
+  } else {
+    c->geom_staged = false;
+    for (size_t q = 0; q < 5; ++q)
+      if ((s = upload(c, kGeo + q, *gh[q], Nh, gh[q]))) return s;
+    for (size_t q = 0; q < 7; ++q)
+      if ((s = upload(c, kGeo + 5 + q, *gc[q], N, gc[q]))) return s;
+    c->geom_staged = true;
+  }
   const double** bp[] = {&d.beta_e, &d.beta_n, &d.beta_t};
   for (const double** p : bp)
     if ((s = upload(c, k++, *p, N, p))) return s;
-  // outputs: upload the caller's u,v,w so entries outside (1:I,1:J,1:K) are kept
+  // Outputs.  Without the wall step only the entries (1:I,1:J,1:K) of u, v,
+  // w are written (and only those are copied back, so the caller's other
+  // entries are kept); the wall step also writes boundary entries, so then
+  // the caller's u, v, w are uploaded first and copied back whole.
+  const bool whole = in->wall != nullptr;
   sp_outputs o{};
   double* outs_h[4] = {out->p, out->u, out->v, out->w};
   double* outs_d[4];
@@ -575,7 +595,7 @@ sp_status sp_project(sp_ctx* c, const sp_inputs* in, sp_outputs* out, sp_stats* 
     void* p;
     if ((s = stage(c, k++, Nh * sizeof(double), &p))) return s;
     outs_d[q] = static_cast<double*>(p);
-    if (q > 0 && (s = pipe_upload(c, p, outs_h[q], Nh * sizeof(double)))) return s;
+    if (whole && q > 0 && (s = pipe_upload(c, p, outs_h[q], Nh * sizeof(double)))) return s;
   }
   o.p = outs_d[0]; o.u = outs_d[1]; o.v = outs_d[2]; o.w = outs_d[3];
   o.po_coef = nullptr;
@@ -583,8 +603,11 @@ sp_status sp_project(sp_ctx* c, const sp_inputs* in, sp_outputs* out, sp_stats* 
   c->last_in_host = true;
   sp_status r = project_device(c, &d, &o, st, true);
   if (r != SP_OK && r != SP_ERR_NOT_CONVERGED) return r;
-  for (int q = 0; q < 4; ++q)
-    if ((s = pipe_download(c, outs_h[q], outs_d[q], Nh * sizeof(double)))) return s;
+  for (int q = 0; q < 4; ++q) {
+    s = (whole || q == 0) ? pipe_download(c, outs_h[q], outs_d[q], Nh * sizeof(double))
+                          : pipe_download_interior(c, outs_h[q], outs_d[q]);
+    if (s) return s;
+  }
   if (out->po_coef && (s = pipe_download(c, out->po_coef, c->pocoef, 6 * N * sizeof(double))))
     return s;
   hipError_t e = hipStreamSynchronize(c->stream);
@@ -1129,6 +1152,7 @@ const char* sp_strerror(sp_status s) {
     case SP_ERR_GRID: return "grid not set or not a tensor-product grid";
     case SP_ERR_STATE: return "no forward projection to differentiate";
     case SP_ERR_NO_DEVICE: return "no usable HIP device";
+    case SP_ERR_CAPACITY: return "an engine work list is full";
   }
   return "unknown status";
 }

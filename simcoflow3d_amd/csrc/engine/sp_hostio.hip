@@ -20,6 +20,28 @@
 
 namespace sp {
 
+// Rows of a halo array (0:I+1, 0:J+1, 0:K+1) of doubles: a download with a
+// RowSpec copies only the entries (1:I, 1:J, 1:K) into the caller's array and
+// leaves the others as the caller had them.
+struct RowSpec {
+  size_t sx, sy;  // elements per row, rows per plane
+  int I, J, K;
+};
+
+// The interior entries of the element range [off + lo, off + hi) of an array
+// (byte offsets, multiples of 8) from `chunk`, which holds bytes [off, ...).
+static void copy_interior(char* dst, const char* chunk, size_t off, size_t lo, size_t hi,
+                          const RowSpec& r) {
+  const size_t e0 = (off + lo) / 8, e1 = (off + hi) / 8;
+  if (e1 <= e0) return;
+  for (size_t R = e0 / r.sx; R <= (e1 - 1) / r.sx; ++R) {
+    const size_t j = R % r.sy, k = R / r.sy;
+    if (j < 1 || j > (size_t)r.J || k < 1 || k > (size_t)r.K) continue;
+    const size_t a = std::max(e0, R * r.sx + 1), b = std::min(e1, R * r.sx + 1 + (size_t)r.I);
+    if (a < b) std::memcpy(dst + a * 8, chunk + (a * 8 - off), (b - a) * 8);
+  }
+}
+
 struct HostPipe {
   static constexpr int kBufs = 4;
   static constexpr size_t kChunk = size_t(32) << 20;
@@ -38,6 +60,8 @@ struct HostPipe {
   char* job_dst = nullptr;
   const char* job_src = nullptr;
   size_t job_len = 0;
+  size_t job_off = 0;
+  const RowSpec* job_rows = nullptr;  // null: plain copy of [0, len)
 
   // part p of n of [0, len), on 4 KiB boundaries
   static void part(size_t len, int p, int n, size_t* lo, size_t* hi) {
@@ -51,37 +75,50 @@ struct HostPipe {
     for (;;) {
       char* dst;
       const char* src;
-      size_t len;
+      size_t len, off;
+      const RowSpec* rows;
       {
         std::unique_lock<std::mutex> lk(m);
         cv_job.wait(lk, [&] { return stop || generation != seen; });
         if (stop) return;
         seen = generation;
-        dst = job_dst; src = job_src; len = job_len;
+        dst = job_dst; src = job_src; len = job_len; off = job_off; rows = job_rows;
       }
       size_t lo, hi;
       part(len, id + 1, (int)workers.size() + 1, &lo, &hi);
-      if (hi > lo) std::memcpy(dst + lo, src + lo, hi - lo);
+      if (hi > lo) run(dst, src, off, lo, hi, rows);
       std::lock_guard<std::mutex> lk(m);
       if (--remaining == 0) cv_done.notify_one();
     }
   }
 
-  void copy(char* dst, const char* src, size_t len) {
+  // plain: dst[lo, hi) = src[lo, hi); rows: the interior entries of bytes
+  // [off + lo, off + hi) of the array at dst from the chunk src
+  static void run(char* dst, const char* src, size_t off, size_t lo, size_t hi,
+                  const RowSpec* rows) {
+    if (rows) copy_interior(dst, src, off, lo, hi, *rows);
+    else std::memcpy(dst + lo, src + lo, hi - lo);
+  }
+
+  // Copy `len` bytes from src to dst on the calling thread and the workers.
+  // With `rows`, dst is the whole array, src holds its bytes [off, off + len)
+  // and only interior entries are written.
+  void copy(char* dst, const char* src, size_t len, size_t off = 0,
+            const RowSpec* rows = nullptr) {
     if (workers.empty() || len < (size_t(1) << 20)) {
-      std::memcpy(dst, src, len);
+      run(dst, src, off, 0, len, rows);
       return;
     }
     {
       std::lock_guard<std::mutex> lk(m);
-      job_dst = dst; job_src = src; job_len = len;
+      job_dst = dst; job_src = src; job_len = len; job_off = off; job_rows = rows;
       remaining = (int)workers.size();
       ++generation;
     }
     cv_job.notify_all();
     size_t lo, hi;
     part(len, 0, (int)workers.size() + 1, &lo, &hi);
-    if (hi > lo) std::memcpy(dst + lo, src + lo, hi - lo);
+    if (hi > lo) run(dst, src, off, lo, hi, rows);
     std::unique_lock<std::mutex> lk(m);
     cv_done.wait(lk, [&] { return remaining == 0; });
   }
@@ -151,7 +188,8 @@ sp_status pipe_upload(sp_ctx* c, void* dst, const void* src, size_t bytes) {
   return SP_OK;
 }
 
-sp_status pipe_download(sp_ctx* c, void* dst, const void* src, size_t bytes) {
+static sp_status download(sp_ctx* c, void* dst, const void* src, size_t bytes,
+                          const RowSpec* rows) {
   sp_status s = pipe_init(c);
   if (s) return s;
   HostPipe& P = *c->pipe;
@@ -183,9 +221,20 @@ sp_status pipe_download(sp_ctx* c, void* dst, const void* src, size_t bytes) {
     if (e != hipSuccess) return hip_check(c, e, "pinned staging");
     P.pending[b] = false;
     const size_t off = q * HostPipe::kChunk, len = std::min(HostPipe::kChunk, bytes - off);
-    P.copy(static_cast<char*>(dst) + off, P.buf[b], len);
+    if (rows) P.copy(static_cast<char*>(dst), P.buf[b], len, off, rows);
+    else P.copy(static_cast<char*>(dst) + off, P.buf[b], len);
   }
   return SP_OK;
+}
+
+sp_status pipe_download(sp_ctx* c, void* dst, const void* src, size_t bytes) {
+  return download(c, dst, src, bytes, nullptr);
+}
+
+sp_status pipe_download_interior(sp_ctx* c, double* dst, const double* src) {
+  const Dims& D = c->dm;
+  const RowSpec r{(size_t)D.SX, (size_t)D.SY, D.I, D.J, D.K};
+  return download(c, dst, src, (size_t)D.Nh * sizeof(double), &r);
 }
 
 }  // namespace sp

@@ -179,6 +179,7 @@ struct sp_ctx {
   // or the caller's device pointers)
   sp_inputs last_in{};
   bool last_in_host = false;
+  bool geom_staged = false;  // sp_project's geometry staging slots hold the last host geometry
 
   // adjoint work
   double *pbar = nullptr, *lam = nullptr;
@@ -219,6 +220,9 @@ void timer_end(sp_ctx* c, int slot);
 sp_status pipe_upload(sp_ctx* c, void* dst, const void* src, size_t bytes);
 // synchronous: dst holds the data on return
 sp_status pipe_download(sp_ctx* c, void* dst, const void* src, size_t bytes);
+// Download of a halo array (0:I+1, 0:J+1, 0:K+1) of doubles that writes only
+// its entries (1:I, 1:J, 1:K) into dst.
+sp_status pipe_download_interior(sp_ctx* c, double* dst, const double* src);
 void pipe_destroy(sp_ctx* c);
 
 // ---- assembly (sp_assemble.hip) ----

@@ -23,6 +23,7 @@ SP_ERR_OOM = -5
 SP_ERR_GRID = -6
 SP_ERR_STATE = -7
 SP_ERR_NO_DEVICE = -8
+SP_ERR_CAPACITY = -9
 
 SP_PREC_FP64 = 0
 SP_PREC_MIXED = 1
@@ -116,7 +117,7 @@ class SpInputs(ctypes.Structure):
         "cell_type_w", "moex_u", "moex_v", "moex_w")] + [
         ("bc_flag", ctypes.c_int32 * 6), ("bc_val", ctypes.c_void_p * 6), ("dt", ctypes.c_double),
         ("beta_e", ctypes.c_void_p), ("beta_n", ctypes.c_void_p), ("beta_t", ctypes.c_void_p),
-        ("wall", ctypes.c_void_p)]
+        ("wall", ctypes.c_void_p), ("static_geometry", ctypes.c_int32)]
 
 
 class SpOutputs(ctypes.Structure):
@@ -339,13 +340,16 @@ class Projector:
         return [np.zeros(shapes[q % 6], order="F") for q in range(24)]
 
     def project(self, fields, bc_flag, dt, out=None, bc_val=None, beta=None, po_coef=False,
-                wall=None):
+                wall=None, static_geometry=False):
         """Host-array projection (sp_project).  Returns (outputs dict, stats).
         With ``wall`` (SpWallBc) the wall step runs after the correction and
-        out["bc_var"] holds the 24 face values."""
+        out["bc_var"] holds the 24 face values.  ``static_geometry``: the
+        apertures, vof, vofO, cell types and MoExCell equal the previous
+        call's, so their device copies are reused."""
         I, J, K = self.shape
         hs = (I + 2, J + 2, K + 2)
         inp = self.make_inputs(fields, bc_flag, dt, bc_val, beta, wall)
+        inp.static_geometry = 1 if static_geometry else 0
         if out is None:
             out = {n: np.zeros(hs, order="F") for n in ("p", "u", "v", "w")}
         if po_coef:

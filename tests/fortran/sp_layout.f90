@@ -37,6 +37,7 @@ program sp_layout
   print '(a,1x,i0)', 'sp_wall_bc.const_u', off(c_loc(wb%const_u), c_loc(wb))
   print '(a,1x,i0)', 'sp_wall_bc.const_p', off(c_loc(wb%const_p), c_loc(wb))
   print '(a,1x,i0)', 'sp_inputs.wall', off(c_loc(inp%wall), c_loc(inp))
+  print '(a,1x,i0)', 'sp_inputs.static_geometry', off(c_loc(inp%static_geometry), c_loc(inp))
   print '(a,1x,i0)', 'sp_outputs.bc_var', off(c_loc(outp%bc_var), c_loc(outp))
   print '(a,1x,i0)', 'sp_stats.wall_div', off(c_loc(st%wall_div), c_loc(st))
   print '(a,1x,i0)', 'sp_params.mg_coarse_scale', off(c_loc(prm%mg_coarse_scale), c_loc(prm))

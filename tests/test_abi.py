@@ -33,7 +33,8 @@ def test_library_exports_every_declared_function():
 
 def test_status_strings_and_defaults():
     L = engine.load()
-    for s in range(0, -9, -1):
+    for s in range(0, -10, -1):
+        assert L.sp_strerror(s) != b"unknown status"
         assert L.sp_strerror(s)
     p = engine.SpParams()
     L.sp_params_default(ctypes.byref(p))
