@@ -375,6 +375,58 @@ sp_status sp_compute_uvw_liquid_field(sp_ctx* ctx, const sp_liquid_field* p,
                                       const sp_liquid_field* u, const sp_liquid_field* v,
                                       const sp_liquid_field* w, int device);
 
+/* Inputs of one momentum predictor step, PredictingUVW(PGrid, UGrid, VGrid,
+ * WGrid, PCell, UCell, VCell, WCell, PCellO, UCellO, VCellO, WCellO,
+ * FluxDivOld, TVar_n, TVar, BCu, BCv, BCw, PU, PV, PW, Pred, dt, itt)
+ * (PredictUVW.f90:49-803), with SpaceOrder 1 (first-order upwind convection,
+ * :1338-1794; every driver's input file selects it).  The grid is the
+ * context's (sp_set_grid); the cells are the host's own arrays as the
+ * geometry update writes them (sp_cell_geom / sp_cell_interp layouts). */
+typedef struct sp_pred_inputs {
+  /* PCellO%EEArea, NEArea, TEArea (halo) */
+  const double *p_ee_area, *p_ne_area, *p_te_area;
+  /* UCellO, VCellO, WCellO: phi, nx, ny, nz, vof, cell_type, ee/ne/te_area,
+   * fce/fcn/fct and cell_cent are read */
+  const sp_cell_geom *u_old, *v_old, *w_old;
+  /* their moex, sxe, syn, szt, etae, etan, etat, ale, aln, alt are read */
+  const sp_cell_interp *u_old_x, *v_old_x, *w_old_x;
+  /* UCell, VCell, WCell: vof, vof_l, cell_type, wllh, delh are read */
+  const sp_cell_geom *u_cur, *v_cur, *w_cur;
+  const double *u, *v, *w;        /* TVar%u, v, w (halo) */
+  const double *u_n, *v_n, *w_n;  /* TVar_n%u, v, w (halo); read for TimeOrder 2 only */
+  double* flux_div_old;           /* FluxDivOld (Imax,Jmax,Kmax,3): read for TimeOrder 3
+                                     after the first step, updated where the reference
+                                     updates it */
+  int32_t bc_flag[18];            /* BCu%flag(1:6), BCv%flag, BCw%flag: W,E,S,N,B,T */
+  const double* bc_var[18];       /* BCu%VarW..VarT, BCv%..., BCw%... face arrays */
+  double dt;
+  int64_t itt;                    /* time step number (1: first-order start) */
+  int32_t time_order;             /* TimeOrder: 1 Euler, 2 extrapolated velocity, 3
+                                     Adams-Bashforth (StateVariables.f90:13) */
+  int32_t space_order;            /* SpaceOrder: only 1 is built */
+  double mu_w_ref, mu_a_ref;      /* muWRef = muw/muref, muARef = mua/muref */
+  double rey, fr;                 /* Rey, Fr (StateVariables.f90:12) */
+  double gx, gy, gz, g;           /* gravity components and g */
+} sp_pred_inputs;
+
+/* Outputs: Pred%u, v, w (halo arrays: the entries the reference writes are
+ * written, the others kept) and PU%Dp, PV%Dp, PW%Dp (halo arrays, every
+ * entry written). */
+typedef struct sp_pred_outputs {
+  double *u, *v, *w;
+  double *dp_u, *dp_v, *dp_w;
+} sp_pred_outputs;
+
+/* PredictingUVW (PredictUVW.f90:49-803) for SpaceOrder 1.  DiffusiveFlux
+ * zeroes its coefficients (:3672-3673), so the three HYPRE momentum solves
+ * (:692-759) are diagonal: u* = rhs / aP.  Every output equals a sequential
+ * evaluation of the reference's expressions bit for bit; two cut-cell tests
+ * that read a cell array at index Jmax + 1 (:1416-1430, :1744-1751) read
+ * "no such neighbour".  Returns SP_ERR_INVALID_ARG for SpaceOrder /= 1.
+ * `device`: 1 when every pointer is device memory, 0 for host memory. */
+sp_status sp_predict_uvw(sp_ctx* ctx, const sp_pred_inputs* in, sp_pred_outputs* out,
+                         int device);
+
 /* HIP stream the context runs on (hipStream_t as void*). */
 void* sp_stream(sp_ctx* ctx);
 

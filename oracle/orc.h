@@ -242,6 +242,45 @@ double orc_volume_fraction(double dx, double dy, double dz, double nxx, double n
 void orc_uvw_liquid_field(int I, int J, int K, const orc_liquid* P, orc_liquid* U, orc_liquid* V,
                           orc_liquid* W);
 
+/* ---- Momentum predictor (PredictUVW.f90), SpaceOrder 1 (orc_pred.c) ---- */
+/* The members of one staggered cell set (UCell, VCell or WCell) that
+ * PredictingUVW reads: cell arrays (1:I,1:J,1:K) and halo arrays
+ * (0:I+1,0:J+1,0:K+1); fce/fcn/fct halo x3, cent cell x3. */
+typedef struct orc_pred_cell {
+  const double *phi, *nx, *ny, *nz, *vof, *vof_l;
+  const int32_t *cell_type, *moex;
+  const double *ee, *ne, *te, *fce, *fcn, *fct, *cent;
+  const double *sxe, *syn, *szt, *etae, *etan, *etat, *ale, *aln, *alt;
+  const double *wllh, *delh;
+} orc_pred_cell;
+
+typedef struct orc_pred_params {
+  double dt;
+  int64_t itt;
+  int32_t time_order, space_order;
+  double rho_a_ref, rho_w_ref, mu_a_ref, mu_w_ref, rey, fr, gx, gy, gz, g;
+} orc_pred_params;
+
+/* ModifiedConvectiveFluxFirstOrder x3 (PredictUVW.f90:1338-1794): CFEW
+ * (0:I+1,J,K,3), CFNS (I,0:J+1,K,3), CFTB (I,J,0:K+1,3); bcv: BCu%VarW..VarT,
+ * BCv%..., BCw%... (18 face arrays). */
+void orc_conv_flux_first(const orc_grid* g, const double* pee, const double* pne,
+                         const double* pte, const orc_pred_cell* U, const orc_pred_cell* V,
+                         const orc_pred_cell* W, const double* un, const double* vn,
+                         const double* wn, const double* const* bcv, double* cfew, double* cfns,
+                         double* cftb);
+
+/* PredictingUVW (PredictUVW.f90:49-803) for SpaceOrder 1: pred = Pred%u,v,w
+ * (halo, updated where the reference writes), dp = PU/PV/PW%Dp (halo, every
+ * entry written), flux_div_old (I,J,K,3) read and updated.  old = UCellO..,
+ * cur = UCell..; vel = TVar%u,v,w, vel_n = TVar_n%u,v,w; bc_flag: BCu%flag,
+ * BCv%flag, BCw%flag (18).  Returns -1 for SpaceOrder /= 1. */
+int orc_predict(const orc_grid* g, const double* pee, const double* pne, const double* pte,
+                const orc_pred_cell* const old[3], const orc_pred_cell* const cur[3],
+                const double* const vel[3], const double* const vel_n[3], double* flux_div_old,
+                const int32_t* bc_flag, const double* const* bcv, const orc_pred_params* p,
+                double* const pred[3], double* const dp[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -537,3 +537,115 @@ def uvw_liquid_field(cells):
         structs.append(OrcLiquid(*(o[n].ctypes.data for n in LIQUID_FIELDS)))
     lib().orc_uvw_liquid_field(I, J, K, *(ctypes.byref(s) for s in structs))
     return outs
+
+
+# ---------------------------------------------------------------------------
+# Momentum predictor (PredictUVW.f90), SpaceOrder 1
+# ---------------------------------------------------------------------------
+
+PRED_CELL_FIELDS = ("phi", "nx", "ny", "nz", "vof", "vof_l", "cell_type", "moex", "ee", "ne",
+                    "te", "fce", "fcn", "fct", "cent", "sxe", "syn", "szt", "etae", "etan",
+                    "etat", "ale", "aln", "alt", "wllh", "delh")
+
+
+class OrcPredCell(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in PRED_CELL_FIELDS]
+
+
+class OrcPredParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("itt", ctypes.c_int64), ("time_order", ctypes.c_int32),
+                ("space_order", ctypes.c_int32)] + [
+        (n, ctypes.c_double) for n in ("rho_a_ref", "rho_w_ref", "mu_a_ref", "mu_w_ref", "rey",
+                                       "fr", "gx", "gy", "gz", "g")]
+
+
+def _pred_lib():
+    L = lib()
+    if not getattr(L, "_pred_ready", False):
+        C = ctypes.POINTER(OrcPredCell)
+        G = ctypes.POINTER(OrcGrid)
+        L.orc_conv_flux_first.argtypes = [G, F64P, F64P, F64P, C, C, C, F64P, F64P, F64P,
+                                          ctypes.POINTER(F64P), F64P, F64P, F64P]
+        L.orc_conv_flux_first.restype = None
+        L.orc_predict.argtypes = [G, F64P, F64P, F64P, ctypes.POINTER(C), ctypes.POINTER(C),
+                                  ctypes.POINTER(F64P), ctypes.POINTER(F64P), F64P, I32P,
+                                  ctypes.POINTER(F64P), ctypes.POINTER(OrcPredParams),
+                                  ctypes.POINTER(F64P), ctypes.POINTER(F64P)]
+        L.orc_predict.restype = ctypes.c_int
+        L._pred_ready = True
+    return L
+
+
+def _pred_cell(d, keep):
+    arrs = {}
+    for n in PRED_CELL_FIELDS:
+        a = d[n]
+        dt = np.int32 if n in ("cell_type", "moex") else np.float64
+        a = np.asfortranarray(a, dtype=dt)
+        keep.append(a)
+        arrs[n] = a.ctypes.data
+    return OrcPredCell(*(arrs[n] for n in PRED_CELL_FIELDS))
+
+
+def _f64_array(arrs, keep):
+    out = (F64P * len(arrs))()
+    for q, a in enumerate(arrs):
+        if a is None:
+            continue
+        a = np.asfortranarray(a, dtype=np.float64)
+        keep.append(a)
+        out[q] = _p(a)
+    return out
+
+
+def conv_flux_first(grid, p_areas, cells, vel, bcv):
+    """ModifiedConvectiveFluxFirstOrder x3: (CFEW, CFNS, CFTB) with shapes
+    (I+2,J,K,3), (I,J+2,K,3), (I,J,K+2,3) (index 0 of the extended axis is
+    the reference's 0)."""
+    L = _pred_lib()
+    I, J, K = grid.I, grid.J, grid.K
+    keep = []
+    cs = [_pred_cell(c, keep) for c in cells]
+    pa = [np.asfortranarray(a, dtype=np.float64) for a in p_areas]
+    vv = [np.asfortranarray(a, dtype=np.float64) for a in vel]
+    cfew = np.zeros((I + 2, J, K, 3), order="F")
+    cfns = np.zeros((I, J + 2, K, 3), order="F")
+    cftb = np.zeros((I, J, K + 2, 3), order="F")
+    g = grid_of(grid)
+    L.orc_conv_flux_first(ctypes.byref(g), *(_p(a) for a in pa),
+                          *(ctypes.byref(c) for c in cs), *(_p(a) for a in vv),
+                          _f64_array(bcv, keep), _p(cfew), _p(cfns), _p(cftb))
+    return cfew, cfns, cftb
+
+
+def predict(grid, p_areas, old, cur, vel, vel_n, flux_div_old, bc_flag, bcv, params, pred=None):
+    """PredictingUVW on copies: returns (pred (u, v, w halo arrays), dp (PU/PV/PW%Dp),
+    flux_div_old (I,J,K,3)).  ``old``/``cur``: three dicts keyed by
+    PRED_CELL_FIELDS (UCellO.. and UCell..); ``vel``/``vel_n``: TVar and TVar_n
+    velocities; ``bc_flag``: 18 flags (BCu, BCv, BCw); ``bcv``: 18 face arrays;
+    ``params``: dict of OrcPredParams fields; ``pred``: the Pred arrays before
+    the call (entries the predictor does not write are kept), zeros if None."""
+    L = _pred_lib()
+    I, J, K = grid.I, grid.J, grid.K
+    hs = (I + 2, J + 2, K + 2)
+    keep = []
+    co = (OrcPredCell * 3)(*(_pred_cell(c, keep) for c in old))
+    cc = (OrcPredCell * 3)(*(_pred_cell(c, keep) for c in cur))
+    po = (ctypes.POINTER(OrcPredCell) * 3)(*(ctypes.pointer(co[q]) for q in range(3)))
+    pc = (ctypes.POINTER(OrcPredCell) * 3)(*(ctypes.pointer(cc[q]) for q in range(3)))
+    pa = [np.asfortranarray(a, dtype=np.float64) for a in p_areas]
+    fdo = np.array(flux_div_old, dtype=np.float64, order="F", copy=True)
+    assert fdo.shape == (I, J, K, 3)
+    outs = [np.zeros(hs, order="F") if pred is None else
+            np.array(pred[q], dtype=np.float64, order="F", copy=True) for q in range(3)]
+    dps = [np.zeros(hs, order="F") for _ in range(3)]
+    fl = np.ascontiguousarray(bc_flag, dtype=np.int32)
+    assert fl.shape == (18,)
+    prm = OrcPredParams(**params)
+    g = grid_of(grid)
+    rc = L.orc_predict(ctypes.byref(g), *(_p(a) for a in pa), po, pc, _f64_array(vel, keep),
+                       _f64_array(vel_n, keep), _p(fdo), _p(fl), _f64_array(bcv, keep),
+                       ctypes.byref(prm), _f64_array(outs, keep), _f64_array(dps, keep))
+    if rc != 0:
+        raise ValueError("orc_predict: only SpaceOrder 1 is restated")
+    return outs, dps, fdo

@@ -279,3 +279,54 @@ def geometry_case(size, shape="sphere", seed=0, stretch=False, edge_values=False
         c["delh"] = np.asfortranarray(rng.uniform(0.0, 1.0, (I, J, K)))
         cells.append(c)
     return cells
+
+
+# ---------------------------------------------------------------------------
+# Momentum predictor inputs (PredictingUVW, PredictUVW.f90:49-803)
+# ---------------------------------------------------------------------------
+
+# members of one U/V/W cell set the predictor reads, and where they come
+# from in the geometry update's outputs (sp_cell_geom / sp_cell_interp names)
+PRED_CELL_FIELDS = ("phi", "nx", "ny", "nz", "vof", "vof_l", "cell_type", "moex", "ee", "ne",
+                    "te", "fce", "fcn", "fct", "cent", "sxe", "syn", "szt", "etae", "etan",
+                    "etat", "ale", "aln", "alt", "wllh", "delh")
+_PRED_SOURCE = {"ee": "ee_area", "ne": "ne_area", "te": "te_area", "cent": "cell_cent"}
+
+# reference constants (StateVariables.f90:9-10) in the drivers' scaling:
+# densities and viscosities relative to the liquid's
+PRED_PARAMS = dict(dt=1.0e-3, itt=1, time_order=1, space_order=1,
+                   rho_a_ref=1.205 / 998.3, rho_w_ref=1.0, mu_a_ref=1.506e-5 / 1.002e-3,
+                   mu_w_ref=1.0, rey=150.0, fr=0.5, gx=0.0, gy=0.0, gz=-9.80665, g=9.80665)
+
+
+def pred_cells(geo, interp):
+    """The predictor's view of the U, V, W cells: three dicts keyed by
+    PRED_CELL_FIELDS, from the geometry update's four cell dicts (P, U, V,
+    W) and the three interpolation dicts (U, V, W)."""
+    out = []
+    for c, x in zip(geo[1:], interp):
+        d = {}
+        for n in PRED_CELL_FIELDS:
+            src = _PRED_SOURCE.get(n, n)
+            d[n] = c[src] if src in c else x[src]
+        out.append(d)
+    return out
+
+
+def predictor_fields(size, seed=0, time_order=1, itt=1, walls=None):
+    """The non-geometry inputs of one predictor step on a grid of ``size``:
+    TVar and TVar_n velocities (U(-1,1), halo arrays), FluxDivOld (I,J,K,3),
+    the 18 velocity BC flags (BCu, BCv, BCw; W,E,S,N,B,T; ``walls`` or
+    seeded) and face values, and the parameters."""
+    I, J, K = size
+    hs = (I + 2, J + 2, K + 2)
+    vel = [uniform(seed, 80 + q, -1.0, 1.0, hs) for q in range(3)]
+    vel_n = [uniform(seed, 83 + q, -1.0, 1.0, hs) for q in range(3)]
+    fdo = uniform(seed, 86, -1.0e-3, 1.0e-3, (I, J, K, 3))
+    rng = np.random.default_rng(seed + 17)
+    flags = (np.asarray(walls, dtype=np.int32) if walls is not None else
+             rng.integers(0, 2, 18).astype(np.int32))
+    dims = [(J, K), (J, K), (I, K), (I, K), (I, J), (I, J)]
+    bcv = [np.asfortranarray(rng.uniform(-0.5, 0.5, dims[q % 6])) for q in range(18)]
+    params = dict(PRED_PARAMS, itt=itt, time_order=time_order)
+    return dict(vel=vel, vel_n=vel_n, flux_div_old=fdo, bc_flag=flags, bcv=bcv, params=params)

@@ -568,7 +568,7 @@ sp_status sp_project(sp_ctx* c, const sp_inputs* in, sp_outputs* out, sp_stats* 
   const double** gh[] = {&d.ee_area, &d.ne_area, &d.te_area, &d.vof, &d.vof_old};
   const int32_t** gc[] = {&d.cell_type_p, &d.cell_type_u, &d.cell_type_v, &d.cell_type_w,
                           &d.moex_u, &d.moex_v, &d.moex_w};
-  const size_t kGeo = 200;
+  const size_t kGeo = 400;  // no other entry point stages at or above 400
   if (in->static_geometry && c->geom_staged) {
     for (size_t q = 0; q < 5; ++q) *gh[q] = static_cast<const double*>(c->stage[kGeo + q].ptr);
     for (size_t q = 0; q < 7; ++q) *gc[q] = static_cast<const int32_t*>(c->stage[kGeo + 5 + q].ptr);
@@ -1108,6 +1108,159 @@ sp_status sp_number_external_cell(sp_ctx* c, int iu, int iv, int iw, const int32
   hipError_t e = hipMemcpyAsync(ext_cell, ext, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_check(c, e, "sp_number_external_cell");
+  timers_collect(c);
+  return SP_OK;
+}
+
+static sp_status check_pred(sp_ctx* c, const sp_pred_inputs* in, const sp_pred_outputs* out) {
+  if (!in || !out) return fail(c, SP_ERR_INVALID_ARG, "null predictor inputs/outputs");
+  if (in->space_order != 1)
+    return fail(c, SP_ERR_INVALID_ARG, "only SpaceOrder 1 (first-order upwind) is built");
+  if (in->time_order < 1 || in->time_order > 3)
+    return fail(c, SP_ERR_INVALID_ARG, "time_order must be 1, 2 or 3");
+  if (!(in->dt > 0.0) || !std::isfinite(in->dt))
+    return fail(c, SP_ERR_INVALID_ARG, "dt must be > 0");
+  if (!c->grid_set) return fail(c, SP_ERR_GRID, "sp_set_grid has not been called");
+  const void* req[] = {in->p_ee_area, in->p_ne_area, in->p_te_area, in->u_old, in->v_old,
+                       in->w_old, in->u_old_x, in->v_old_x, in->w_old_x, in->u_cur, in->v_cur,
+                       in->w_cur, in->u, in->v, in->w, in->flux_div_old, out->u, out->v, out->w,
+                       out->dp_u, out->dp_v, out->dp_w};
+  for (const void* p : req)
+    if (!p) return fail(c, SP_ERR_INVALID_ARG, "required predictor pointer is null");
+  if (in->time_order == 2 && (!in->u_n || !in->v_n || !in->w_n))
+    return fail(c, SP_ERR_INVALID_ARG, "TimeOrder 2 needs TVar_n (u_n, v_n, w_n)");
+  for (int q = 0; q < 18; ++q) {
+    if (in->bc_flag[q] != 0 && in->bc_flag[q] != 1)
+      return fail(c, SP_ERR_INVALID_ARG, "predictor bc_flag must be 0 or 1");
+    if (!in->bc_var[q]) return fail(c, SP_ERR_INVALID_ARG, "predictor bc_var entry is null");
+  }
+  const sp_cell_geom* og[3] = {in->u_old, in->v_old, in->w_old};
+  const sp_cell_interp* ox[3] = {in->u_old_x, in->v_old_x, in->w_old_x};
+  const sp_cell_geom* cg[3] = {in->u_cur, in->v_cur, in->w_cur};
+  for (int q = 0; q < 3; ++q) {
+    const sp_cell_geom& g = *og[q];
+    const void* rg[] = {g.phi, g.nx, g.ny, g.nz, g.vof, g.cell_type, g.ee_area, g.ne_area,
+                        g.te_area, g.fce, g.fcn, g.fct, g.cell_cent};
+    for (const void* p : rg)
+      if (!p) return fail(c, SP_ERR_INVALID_ARG, "a member of an old U/V/W cell is null");
+    const sp_cell_interp& x = *ox[q];
+    const void* rx[] = {x.moex, x.sxe, x.syn, x.szt, x.etae, x.etan, x.etat, x.ale, x.aln, x.alt};
+    for (const void* p : rx)
+      if (!p) return fail(c, SP_ERR_INVALID_ARG, "a member of an old U/V/W cell's interpolation data is null");
+    const sp_cell_geom& h = *cg[q];
+    const void* rc[] = {h.vof, h.vof_l, h.cell_type, h.wllh, h.delh};
+    for (const void* p : rc)
+      if (!p) return fail(c, SP_ERR_INVALID_ARG, "a member of a current U/V/W cell is null");
+  }
+  return SP_OK;
+}
+
+sp_status sp_predict_uvw(sp_ctx* c, const sp_pred_inputs* in, sp_pred_outputs* out, int device) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  sp_status s = check_pred(c, in, out);
+  if (s) return s;
+  const Dims& D = c->dm;
+  const size_t N = (size_t)D.N, Nh = (size_t)D.Nh;
+  void* faces;
+  if ((s = stage(c, 299, predict_face_doubles(D) * sizeof(double), &faces))) return s;
+  if (device) {
+    if ((s = predict_uvw(c, *in, *out, static_cast<double*>(faces)))) return s;
+  } else {
+    // staging slots 300..399 belong to the predictor alone
+    size_t k = 300;
+    sp_pred_inputs d = *in;
+    auto upd = [&](const double* h, size_t n, const double** o) { return upload(c, k++, h, n, o); };
+    auto upi = [&](const int32_t* h, size_t n, const int32_t** o) { return upload(c, k++, h, n, o); };
+    if ((s = upd(in->p_ee_area, Nh, &d.p_ee_area)) || (s = upd(in->p_ne_area, Nh, &d.p_ne_area)) ||
+        (s = upd(in->p_te_area, Nh, &d.p_te_area)))
+      return s;
+    sp_cell_geom og[3], cg[3];
+    sp_cell_interp ox[3];
+    const sp_cell_geom* hg[3] = {in->u_old, in->v_old, in->w_old};
+    const sp_cell_interp* hx[3] = {in->u_old_x, in->v_old_x, in->w_old_x};
+    const sp_cell_geom* hc[3] = {in->u_cur, in->v_cur, in->w_cur};
+    for (int q = 0; q < 3; ++q) {
+      const sp_cell_geom& h = *hg[q];
+      sp_cell_geom& o = og[q];
+      o = sp_cell_geom{};
+      const double* t;
+      const int32_t* ti;
+      if ((s = upd(h.phi, N, &o.phi)) || (s = upd(h.nx, N, &o.nx)) || (s = upd(h.ny, N, &o.ny)) ||
+          (s = upd(h.nz, N, &o.nz)))
+        return s;
+      if ((s = upd(h.vof, N, &t))) return s;
+      o.vof = const_cast<double*>(t);
+      if ((s = upi(h.cell_type, N, &ti))) return s;
+      o.cell_type = const_cast<int32_t*>(ti);
+      double* const hv[7] = {h.ee_area, h.ne_area, h.te_area, h.fce, h.fcn, h.fct, h.cell_cent};
+      double** ov[7] = {&o.ee_area, &o.ne_area, &o.te_area, &o.fce, &o.fcn, &o.fct, &o.cell_cent};
+      const size_t nv[7] = {Nh, Nh, Nh, 3 * Nh, 3 * Nh, 3 * Nh, 3 * N};
+      for (int m = 0; m < 7; ++m) {
+        if ((s = upd(hv[m], nv[m], &t))) return s;
+        *ov[m] = const_cast<double*>(t);
+      }
+      const sp_cell_interp& x = *hx[q];
+      sp_cell_interp& ou = ox[q];
+      ou = sp_cell_interp{};
+      if ((s = upi(x.moex, N, &ou.moex))) return s;
+      double* const xv[9] = {x.sxe, x.syn, x.szt, x.etae, x.etan, x.etat, x.ale, x.aln, x.alt};
+      double** xo[9] = {&ou.sxe, &ou.syn, &ou.szt, &ou.etae, &ou.etan, &ou.etat,
+                        &ou.ale, &ou.aln, &ou.alt};
+      for (int m = 0; m < 9; ++m) {
+        if ((s = upd(xv[m], Nh, &t))) return s;
+        *xo[m] = const_cast<double*>(t);
+      }
+      const sp_cell_geom& hcq = *hc[q];
+      sp_cell_geom& oc = cg[q];
+      oc = sp_cell_geom{};
+      double* const cv[4] = {hcq.vof, hcq.vof_l, hcq.wllh, hcq.delh};
+      double** co[4] = {&oc.vof, &oc.vof_l, &oc.wllh, &oc.delh};
+      for (int m = 0; m < 4; ++m) {
+        if ((s = upd(cv[m], N, &t))) return s;
+        *co[m] = const_cast<double*>(t);
+      }
+      if ((s = upi(hcq.cell_type, N, &ti))) return s;
+      oc.cell_type = const_cast<int32_t*>(ti);
+    }
+    d.u_old = &og[0]; d.v_old = &og[1]; d.w_old = &og[2];
+    d.u_old_x = &ox[0]; d.v_old_x = &ox[1]; d.w_old_x = &ox[2];
+    d.u_cur = &cg[0]; d.v_cur = &cg[1]; d.w_cur = &cg[2];
+    if ((s = upd(in->u, Nh, &d.u)) || (s = upd(in->v, Nh, &d.v)) || (s = upd(in->w, Nh, &d.w)))
+      return s;
+    if (in->time_order == 2 &&
+        ((s = upd(in->u_n, Nh, &d.u_n)) || (s = upd(in->v_n, Nh, &d.v_n)) ||
+         (s = upd(in->w_n, Nh, &d.w_n))))
+      return s;
+    const double* fd;
+    if ((s = upd(in->flux_div_old, 3 * N, &fd))) return s;
+    d.flux_div_old = const_cast<double*>(fd);
+    const size_t bl[6] = {(size_t)D.J * D.K, (size_t)D.J * D.K, (size_t)D.I * D.K,
+                          (size_t)D.I * D.K, (size_t)D.I * D.J, (size_t)D.I * D.J};
+    for (int q = 0; q < 18; ++q)
+      if ((s = upd(in->bc_var[q], bl[q % 6], &d.bc_var[q]))) return s;
+    // Pred keeps the entries it does not write: upload, then copy back whole
+    sp_pred_outputs o{};
+    double* ho[6] = {out->u, out->v, out->w, out->dp_u, out->dp_v, out->dp_w};
+    double** oo[6] = {&o.u, &o.v, &o.w, &o.dp_u, &o.dp_v, &o.dp_w};
+    for (int q = 0; q < 6; ++q) {
+      const double* t;
+      if (q < 3) {
+        if ((s = upd(ho[q], Nh, &t))) return s;
+      } else {
+        void* b;
+        if ((s = stage(c, k++, Nh * sizeof(double), &b))) return s;
+        t = static_cast<const double*>(b);
+      }
+      *oo[q] = const_cast<double*>(t);
+    }
+    if ((s = predict_uvw(c, d, o, static_cast<double*>(faces)))) return s;
+    for (int q = 0; q < 6; ++q)
+      if ((s = pipe_download(c, ho[q], *oo[q], Nh * sizeof(double)))) return s;
+    if ((s = pipe_download(c, in->flux_div_old, d.flux_div_old, 3 * N * sizeof(double)))) return s;
+  }
+  const hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_check(c, e, "sp_predict_uvw");
   timers_collect(c);
   return SP_OK;
 }

@@ -1,0 +1,657 @@
+// The momentum predictor PredictingUVW (PredictUVW.f90:49-803) for
+// SpaceOrder 1: u*, v*, w* and the d-coefficients Dp that the projection
+// reads.
+//
+// DiffusiveFlux computes its coefficients and then zeroes them
+// (PredictUVW.f90:3672-3673), so every neighbour coefficient of the three
+// momentum systems vanishes and each HYPRE solve (:692-759) is the division
+// rhs / aP of a diagonal system.  What remains is:
+//
+//   k_pred_faces  the first-order upwind convective fluxes with the cut-cell
+//                 corrections, ModifiedConvectiveFluxFirstOrder
+//                 (:1338-1794): one thread per (i,j,k) writes the face
+//                 fluxes CFEW(i,j,k,1:3), CFNS(i,j,k,1:3), CFTB(i,j,k,1:3)
+//                 it owns (x, y and z faces at index i, j, k);
+//   k_pred_cells  per U, V, W cell: the flux divergence, Adams-Bashforth
+//                 for TimeOrder 3 (:320-337), FluxDivOld, the wall friction
+//                 Sp (:404-410), aP and Dp (:1025-1037), the right-hand
+//                 side and Dp scaling (:1206-1241) and u* = rhs / aP;
+//   k_pred_dp_halo  the Dp halo copies (:778-800): the sequence of plane
+//                 copies leaves every halo entry equal to the interior
+//                 value at the clamped index (the component's own last
+//                 plane copied from the one before it);
+//   k_pred_bc     PredictorVelocityBoundaryCondition (:3796-3894) and then
+//                 PredictorVelocityInternalCellCondition (:3896-3916) on the
+//                 last plane of each component (the cells kernel zeroes the
+//                 solid cells of the others).
+//
+// Every expression keeps the reference's operation order with contraction
+// off, so the results equal a sequential evaluation bit for bit (the
+// oracle's).  Two cut-cell tests of the reference read a cell array at
+// index Jmax + 1 (:1416-1430, :1744-1751), outside the array; they are
+// read as "no such neighbour", as the sibling branches that guard the index
+// do.
+
+#include <hip/hip_runtime.h>
+
+#include "sp_device.h"
+#include "sp_internal.h"
+
+namespace sp {
+namespace {
+
+// The members of one U/V/W cell set the predictor reads (device pointers).
+struct PredCell {
+  const double *phi, *nx, *ny, *nz, *vof, *vof_l;
+  const int32_t *ct, *moex;
+  const double *ee, *ne, *te, *fce, *fcn, *fct, *cent;
+  const double *sxe, *syn, *szt, *etae, *etan, *etat, *ale, *aln, *alt;
+  const double *wllh, *delh;
+};
+
+struct PredArgs {
+  int I, J, K, SX, SY;
+  int64_t N, Nh;
+  const double *dxp, *dyp, *dzp, *dxu, *dyv, *dzw;
+  const double *pee, *pne, *pte;       // PCellO apertures (halo)
+  PredCell old[3], cur[3];
+  const double *vel[3], *veln[3];      // TVar, TVar_n (halo)
+  int extrapolate;                     // TimeOrder 2: un12 = 1.5 TVar - 0.5 TVar_n
+  const double* bcv[18];
+  int32_t flag[18];
+  double* fdo;                          // FluxDivOld (I,J,K,3)
+  double *cfew, *cfns, *cftb;           // face fluxes
+  double *pred[3], *dp[3];
+  double dt, inv_fr2, gx, gy, gz, g, mu_w, mu_a, rey;
+  int ab2;                              // Adams-Bashforth source (TimeOrder 3, itt > 1)
+};
+
+constexpr double kEps = 1.0e-20;  // local epsi of the flux routine (:1362)
+
+struct Ix {
+  int I, J, K, SX, SY;
+  int64_t N, Nh;
+  __device__ int64_t h(int i, int j, int k) const { return i + (int64_t)SX * (j + (int64_t)SY * k); }
+  __device__ int64_t c(int i, int j, int k) const {
+    return (i - 1) + (int64_t)I * ((j - 1) + (int64_t)J * (k - 1));
+  }
+  __device__ int64_t ew(int i, int j, int k, int q) const {
+    return i + (int64_t)(I + 2) * ((j - 1) + (int64_t)J * (k - 1)) + (int64_t)(I + 2) * J * K * q;
+  }
+  __device__ int64_t ns(int i, int j, int k, int q) const {
+    return (i - 1) + (int64_t)I * (j + (int64_t)(J + 2) * (k - 1)) + (int64_t)I * (J + 2) * K * q;
+  }
+  __device__ int64_t tb(int i, int j, int k, int q) const {
+    return (i - 1) + (int64_t)I * ((j - 1) + (int64_t)J * k) + (int64_t)I * J * (K + 2) * q;
+  }
+};
+
+__device__ inline Ix ix_of(const PredArgs& A) { return Ix{A.I, A.J, A.K, A.SX, A.SY, A.N, A.Nh}; }
+
+// un12 of component q at halo index h
+__device__ inline double v12(const PredArgs& A, int q, int64_t h) {
+#pragma clang fp contract(off)
+  return A.extrapolate ? 1.5 * A.vel[q][h] - 0.5 * A.veln[q][h] : A.vel[q][h];
+}
+
+__device__ inline double bcval(const PredArgs& A, int q, int a, int b) {
+  const int f = q % 6;
+  const int n1 = f < 2 ? A.J : A.I;
+  return A.bcv[q][(a - 1) + (int64_t)n1 * (b - 1)];
+}
+
+__device__ inline double plane_dist(const Ix& X, const PredCell& C, int i, int j, int k) {
+#pragma clang fp contract(off)
+  const int64_t n = X.c(i, j, k);
+  return fabs(C.cent[n] * C.nx[n] + C.cent[n + X.N] * C.ny[n] + C.cent[n + 2 * X.N] * C.nz[n] +
+              C.phi[n]);
+}
+
+__device__ inline bool usable(const Ix& X, const PredCell& C, int i, int j, int k) {
+  const int64_t n = X.c(i, j, k);
+  return C.moex[n] != 1 && C.ct[n] != 2;
+}
+
+// |FC . n + phi| halves of a cut face: the face centre (halo x3 array fc at
+// (fi,fj,fk)) against cell a's plane and, moved by `shift` along axis `ax`,
+// against cell b's plane
+__device__ inline double cut_dhec(const Ix& X, const PredCell& C, const double* fc, int fi, int fj,
+                                  int fk, int ai, int aj, int ak, int bi, int bj, int bk, int ax,
+                                  double shift) {
+#pragma clang fp contract(off)
+  const int64_t f = X.h(fi, fj, fk);
+  const double f1 = fc[f], f2 = fc[f + X.Nh], f3 = fc[f + 2 * X.Nh];
+  const int64_t na = X.c(ai, aj, ak), nb = X.c(bi, bj, bk);
+  const double d1 = f1 * C.nx[na] + f2 * C.ny[na] + f3 * C.nz[na] + C.phi[na];
+  const double g1 = ax == 0 ? f1 - shift : f1;
+  const double g2 = ax == 1 ? f2 - shift : f2;
+  const double d2 = g1 * C.nx[nb] + g2 * C.ny[nb] + f3 * C.nz[nb] + C.phi[nb];
+  return 0.5 * (d1 + d2);
+}
+
+__device__ inline double up_p(double a) {
+#pragma clang fp contract(off)
+  return 0.5 * (a + fabs(a));
+}
+__device__ inline double up_n(double a) {
+#pragma clang fp contract(off)
+  return 0.5 * (a - fabs(a));
+}
+
+// ---- x faces: CFEW(i,j,k,1:3), i = 1..I+1 (:1366-1525) ----
+__device__ void flux_x(const PredArgs& A, const Ix& X, int i, int j, int k) {
+#pragma clang fp contract(off)
+  const PredCell &U = A.old[0], &V = A.old[1], &W = A.old[2];
+  const int I = A.I, J = A.J, K = A.K;
+  const double dyu = A.dyp[j - 1], dzu = A.dzp[k - 1];
+  double uw, uwp, uwn, eta, flux;
+  // u momentum, convective u
+  {
+    const double ui = v12(A, 0, X.h(i, j, k)), um = v12(A, 0, X.h(i - 1, j, k));
+    uw = 0.5 * (ui + um);
+    uwp = up_p(uw);
+    uwn = up_n(uw);
+    if (i == 1) {
+      flux = (uwn * ui + uwp * um) * U.ee[X.h(i, j, k)] * dyu * dzu;
+    } else if (i >= I) {
+      flux = (uwn * ui + uwp * um) * U.ee[X.h(i - 1, j, k)] * dyu * dzu;
+    } else {
+      eta = U.etae[X.h(i - 1, j, k)];
+      uw = (1.0 - eta) * um + eta * ui;
+      uw = (uw - 0.0) * U.ale[X.h(i - 1, j, k)] + 0.0;
+      uwp = up_p(uw);
+      uwn = up_n(uw);
+      flux = (uwn * ui + uwp * um) * U.ee[X.h(i - 1, j, k)] * dyu * dzu;
+    }
+    A.cfew[X.ew(i, j, k, 0)] = flux;
+  }
+  // v momentum, convective u (V grid: dy = dyv, dz = dzp)
+  {
+    const double dyv = A.dyv[j - 1], dzv = A.dzp[k - 1];
+    uw = 0.5 * (v12(A, 0, X.h(i - 1, j + 1, k)) + v12(A, 0, X.h(i - 1, j, k)));
+    if (i > I) {
+      flux = uw * bcval(A, 7, j, k) * V.ee[X.h(i - 1, j, k)] * dyv * dzv;
+    } else if (i == 1) {
+      flux = uw * bcval(A, 6, j, k) * V.ee[X.h(i, j, k)] * dyv * dzv;
+    } else {
+      if (A.pee[X.h(i - 1, j, k)] <= 1.0e-5 || A.pee[X.h(i - 1, j + 1, k)] <= 1.0e-5) {
+        const double dhec = cut_dhec(X, V, V.fce, i - 1, j, k, i - 1, j, k, i, j, k, 0,
+                                     A.dxu[i - 2]);
+        double dh;
+        if (j < J && usable(X, U, i - 1, j + 1, k) &&
+            U.vof[X.c(i - 1, j + 1, k)] >= U.vof[X.c(i - 1, j, k)]) {
+          dh = plane_dist(X, U, i - 1, j + 1, k);
+          uw = v12(A, 0, X.h(i - 1, j + 1, k));
+        } else if (j < J && usable(X, U, i - 1, j, k) &&
+                   U.vof[X.c(i - 1, j, k)] > U.vof[X.c(i - 1, j + 1, k)]) {
+          dh = plane_dist(X, U, i - 1, j, k);
+          uw = v12(A, 0, X.h(i - 1, j, k));
+        } else {
+          dh = dhec;
+        }
+        uw = (uw - 0.0) * dhec / (dh + kEps) + 0.0;
+      } else {
+        const double Sy = U.syn[X.h(i - 1, j, k)];
+        eta = fabs(V.fce[X.h(i - 1, j, k) + X.Nh] + dyv / 2.0 - U.cent[X.c(i - 1, j, k) + X.N]) / Sy;
+        if (fabs(eta) >= 1.0) eta = 0.5;
+        uw = (1.0 - eta) * v12(A, 0, X.h(i - 1, j, k)) + eta * v12(A, 0, X.h(i - 1, j + 1, k));
+      }
+      uwp = up_p(uw);
+      uwn = up_n(uw);
+      flux = (uwp * v12(A, 1, X.h(i - 1, j, k)) + uwn * v12(A, 1, X.h(i, j, k))) *
+             V.ee[X.h(i - 1, j, k)] * dyv * dzv;
+    }
+    A.cfew[X.ew(i, j, k, 1)] = flux;
+  }
+  // w momentum, convective u (W grid: dy = dyp, dz = dzw)
+  {
+    const double dyw = A.dyp[j - 1], dzw = A.dzw[k - 1];
+    uw = 0.5 * (v12(A, 0, X.h(i - 1, j, k + 1)) + v12(A, 0, X.h(i - 1, j, k)));
+    if (i >= I) {
+      flux = uw * bcval(A, 13, j, k) * W.ee[X.h(i - 1, j, k)] * dyw * dzw;
+    } else if (i == 1) {
+      flux = uw * bcval(A, 12, j, k) * W.ee[X.h(i, j, k)] * dyw * dzw;
+    } else {
+      if (A.pee[X.h(i - 1, j, k)] <= 1.0e-5 || A.pee[X.h(i - 1, j, k + 1)] <= 1.0e-5) {
+        const double dhec = cut_dhec(X, W, W.fce, i - 1, j, k, i - 1, j, k, i, j, k, 0,
+                                     A.dxu[i - 2]);
+        double dh;
+        if (k < K) {
+          if (usable(X, U, i - 1, j, k + 1) &&
+              U.vof[X.c(i - 1, j, k + 1)] >= U.vof[X.c(i - 1, j, k)]) {
+            dh = plane_dist(X, U, i - 1, j, k + 1);
+            uw = v12(A, 0, X.h(i - 1, j, k + 1));
+          } else if (usable(X, U, i - 1, j, k) &&
+                     U.vof[X.c(i - 1, j, k)] > U.vof[X.c(i - 1, j, k + 1)]) {
+            dh = plane_dist(X, U, i - 1, j, k);
+            uw = v12(A, 0, X.h(i - 1, j, k));
+          } else {
+            dh = dhec;
+          }
+        } else {
+          dh = dhec;
+        }
+        // the scaling is applied twice (:1492-1494)
+        uw = uw * dhec / (dh + kEps);
+        uw = (uw - 0.0) * dhec / (dh + kEps) + 0.0;
+      } else {
+        const double Sz = U.szt[X.h(i - 1, j, k)];
+        eta = fabs(W.fce[X.h(i - 1, j, k) + 2 * X.Nh] + dzw / 2.0 -
+                   U.cent[X.c(i - 1, j, k) + 2 * X.N]) /
+              Sz;
+        if (fabs(eta) >= 1.0) eta = 0.5;
+        uw = (1.0 - eta) * v12(A, 0, X.h(i - 1, j, k)) + eta * v12(A, 0, X.h(i - 1, j, k + 1));
+      }
+      uwp = up_p(uw);
+      uwn = up_n(uw);
+      flux = (uwp * v12(A, 2, X.h(i - 1, j, k)) + uwn * v12(A, 2, X.h(i, j, k))) *
+             W.ee[X.h(i - 1, j, k)] * dyw * dzw;
+    }
+    A.cfew[X.ew(i, j, k, 2)] = flux;
+  }
+}
+
+// ---- y faces: CFNS(i,j,k,1:3), j = 1..J+1 (:1527-1643) ----
+__device__ void flux_y(const PredArgs& A, const Ix& X, int i, int j, int k) {
+#pragma clang fp contract(off)
+  const PredCell &U = A.old[0], &V = A.old[1], &W = A.old[2];
+  const int I = A.I, J = A.J, K = A.K;
+  double vs, vsp, vsn, eta, flux;
+  // u momentum, convective v (U grid: dx = dxu, dz = dzp)
+  {
+    const double dxu = A.dxu[i - 1], dzu = A.dzp[k - 1];
+    vs = 0.5 * (v12(A, 1, X.h(i, j - 1, k)) + v12(A, 1, X.h(i + 1, j - 1, k)));
+    if (j >= J) {
+      flux = vs * bcval(A, 3, i, k) * U.ne[X.h(i, j - 1, k)] * dxu * dzu;
+    } else if (j == 1) {
+      flux = vs * bcval(A, 2, i, k) * U.ne[X.h(i, j, k)] * dxu * dzu;
+    } else {
+      if (A.pne[X.h(i, j - 1, k)] <= 1.0e-5 || A.pne[X.h(i + 1, j - 1, k)] <= 1.0e-5) {
+        const double dhec = cut_dhec(X, U, U.fcn, i, j - 1, k, i, j - 1, k, i, j, k, 1,
+                                     A.dyv[j - 2]);
+        double dh;
+        if (i < I) {
+          if (usable(X, V, i, j - 1, k) && V.vof[X.c(i, j - 1, k)] >= V.vof[X.c(i + 1, j - 1, k)]) {
+            dh = plane_dist(X, V, i, j - 1, k);
+            vs = v12(A, 1, X.h(i, j - 1, k));
+          } else if (usable(X, V, i + 1, j - 1, k) &&
+                     V.vof[X.c(i + 1, j - 1, k)] > V.vof[X.c(i, j - 1, k)]) {
+            dh = plane_dist(X, V, i + 1, j - 1, k);
+            vs = v12(A, 1, X.h(i + 1, j - 1, k));
+          } else {
+            dh = dhec;
+          }
+        } else {
+          dh = dhec;
+        }
+        vs = (vs - 0.0) * dhec / (dh + kEps) + 0.0;
+      } else {
+        const double Sx = V.sxe[X.h(i, j - 1, k)];
+        eta = fabs(U.fcn[X.h(i, j - 1, k)] + 0.5 * dxu - V.cent[X.c(i, j - 1, k)]) / Sx;
+        if (fabs(eta) >= 1.0) eta = 0.5;
+        vs = (1.0 - eta) * v12(A, 1, X.h(i, j - 1, k)) + eta * v12(A, 1, X.h(i + 1, j - 1, k));
+      }
+      vsp = up_p(vs);
+      vsn = up_n(vs);
+      flux = (vsn * v12(A, 0, X.h(i, j, k)) + vsp * v12(A, 0, X.h(i, j - 1, k))) *
+             U.ne[X.h(i, j - 1, k)] * dxu * dzu;
+    }
+    A.cfns[X.ns(i, j, k, 0)] = flux;
+  }
+  // v momentum, convective v: the interior branch's interpolated vs is not
+  // used, the flux keeps the plain average's parts (:1569-1585)
+  {
+    const double dxv = A.dxp[i - 1], dzv = A.dzp[k - 1];
+    const double vj = v12(A, 1, X.h(i, j, k)), vm = v12(A, 1, X.h(i, j - 1, k));
+    vs = 0.5 * (vj + vm);
+    vsp = up_p(vs);
+    vsn = up_n(vs);
+    if (j >= J) {
+      flux = (vsn * vj + vsp * vm) * V.ne[X.h(i, j - 1, k)] * dxv * dzv;
+    } else if (j == 1) {
+      flux = (vsn * vj + vsp * vm) * V.ne[X.h(i, j, k)] * dxv * dzv;
+    } else {
+      flux = (vsn * vj + vsp * vm) * V.ne[X.h(i, j - 1, k)] * dxv * dzv;
+    }
+    A.cfns[X.ns(i, j, k, 1)] = flux;
+  }
+  // w momentum, convective v (W grid: dx = dxp, dz = dzw)
+  {
+    const double dxw = A.dxp[i - 1], dzw = A.dzw[k - 1];
+    vs = 0.5 * (v12(A, 1, X.h(i, j - 1, k)) + v12(A, 1, X.h(i, j - 1, k + 1)));
+    if (j >= J) {
+      flux = vs * bcval(A, 15, i, k) * W.ne[X.h(i, j - 1, k)] * dxw * dzw;
+    } else if (j == 1) {
+      flux = vs * bcval(A, 14, i, k) * W.ne[X.h(i, j, k)] * dxw * dzw;
+    } else {
+      if (A.pne[X.h(i, j - 1, k)] <= 1.0e-5 || A.pne[X.h(i, j - 1, k + 1)] <= 1.0e-5) {
+        const double dhec = cut_dhec(X, W, W.fcn, i, j - 1, k, i, j - 1, k, i, j, k, 1,
+                                     A.dyp[j - 2]);
+        double dh;
+        if (k < K) {
+          if (usable(X, V, i, j - 1, k) && V.vof[X.c(i, j - 1, k)] >= V.vof[X.c(i, j - 1, k + 1)]) {
+            dh = plane_dist(X, V, i, j - 1, k);
+            vs = v12(A, 1, X.h(i, j - 1, k));
+          } else if (usable(X, V, i, j - 1, k + 1) &&
+                     V.vof[X.c(i, j - 1, k + 1)] > V.vof[X.c(i, j - 1, k)]) {
+            dh = plane_dist(X, V, i, j - 1, k + 1);
+            vs = v12(A, 1, X.h(i, j - 1, k + 1));
+          } else {
+            dh = dhec;
+          }
+        } else {
+          dh = dhec;
+        }
+        vs = (vs - 0.0) * dhec / (dh + kEps) + 0.0;
+      } else {
+        const double Sz = V.szt[X.h(i, j - 1, k)];
+        eta = fabs(W.fcn[X.h(i, j - 1, k) + 2 * X.Nh] + 0.5 * dzw -
+                   W.cent[X.c(i, j - 1, k) + 2 * X.N]) /
+              Sz;
+        if (fabs(eta) >= 1.0) eta = 0.5;
+        vs = (1.0 - eta) * v12(A, 1, X.h(i, j - 1, k)) + eta * v12(A, 1, X.h(i, j - 1, k + 1));
+      }
+      vsp = up_p(vs);
+      vsn = up_n(vs);
+      flux = (vsp * v12(A, 2, X.h(i, j - 1, k)) + vsn * v12(A, 2, X.h(i, j, k))) *
+             W.ne[X.h(i, j - 1, k)] * dxw * dzw;
+    }
+    A.cfns[X.ns(i, j, k, 2)] = flux;
+  }
+}
+
+// ---- z faces: CFTB(i,j,k,1:3), k = 1..K+1 (:1644-1790) ----
+__device__ void flux_z(const PredArgs& A, const Ix& X, int i, int j, int k) {
+#pragma clang fp contract(off)
+  const PredCell &U = A.old[0], &V = A.old[1], &W = A.old[2];
+  const int I = A.I, J = A.J, K = A.K;
+  double wb, wbp, wbn, eta, flux;
+  // u momentum, convective w: the average takes vn12(i+1,j,k-1) (:1647),
+  // and the bottom boundary BCu%VarT (:1654)
+  {
+    const double dxu = A.dxu[i - 1], dyu = A.dyp[j - 1];
+    wb = 0.5 * (v12(A, 2, X.h(i, j, k - 1)) + v12(A, 1, X.h(i + 1, j, k - 1)));
+    if (k >= K) {
+      flux = wb * bcval(A, 5, i, j) * U.te[X.h(i, j, k - 1)] * dxu * dyu;
+    } else if (k == 1) {
+      flux = wb * bcval(A, 5, i, j) * U.te[X.h(i, j, k)] * dxu * dyu;
+    } else {
+      if (A.pte[X.h(i, j, k - 1)] <= 1.0e-5 || A.pte[X.h(i + 1, j, k - 1)] <= 1.0e-5) {
+        // (FCT(:,2) - dz) * ny, as :1662-1663 write it
+        const double dhec = cut_dhec(X, U, U.fct, i, j, k - 1, i, j, k - 1, i, j, k, 1,
+                                     A.dzw[k - 2]);
+        double dh;
+        if (i < I) {
+          if (usable(X, W, i, j, k - 1) && W.vof[X.c(i, j, k - 1)] >= W.vof[X.c(i + 1, j, k - 1)]) {
+            dh = plane_dist(X, W, i, j, k - 1);
+            wb = v12(A, 2, X.h(i, j, k - 1));
+          } else if (usable(X, W, i + 1, j, k - 1) &&
+                     W.vof[X.c(i + 1, j, k - 1)] > W.vof[X.c(i, j, k - 1)]) {
+            dh = plane_dist(X, W, i + 1, j, k - 1);
+            wb = v12(A, 2, X.h(i + 1, j, k - 1));
+          } else {
+            dh = dhec;
+          }
+        } else {
+          dh = dhec;
+        }
+        wb = (wb - 0.0) * dhec / (dh + kEps) + 0.0;
+      } else {
+        const double Sx = W.sxe[X.h(i, j, k - 1)];
+        eta = fabs(U.fct[X.h(i, j, k - 1)] + 0.5 * dxu - W.cent[X.c(i, j, k - 1)]) / Sx;
+        if (fabs(eta) >= 1.0) eta = 0.5;
+        wb = (1.0 - eta) * v12(A, 2, X.h(i, j, k - 1)) + eta * v12(A, 2, X.h(i + 1, j, k - 1));
+      }
+      wbp = up_p(wb);
+      wbn = up_n(wb);
+      flux = (wbp * v12(A, 0, X.h(i, j, k - 1)) + wbn * v12(A, 0, X.h(i, j, k))) *
+             U.te[X.h(i, j, k - 1)] * dxu * dyu;
+    }
+    A.cftb[X.tb(i, j, k, 0)] = flux;
+  }
+  // v momentum, convective w (V grid: dx = dxp, dy = dyv)
+  {
+    const double dxv = A.dxp[i - 1], dyv = A.dyv[j - 1];
+    wb = 0.5 * (v12(A, 2, X.h(i, j + 1, k - 1)) + v12(A, 2, X.h(i, j, k - 1)));
+    if (k >= K) {
+      flux = wb * bcval(A, 11, i, j) * V.te[X.h(i, j, k - 1)] * dxv * dyv;
+    } else if (k == 1) {
+      flux = wb * bcval(A, 10, i, j) * V.te[X.h(i, j, k)] * dxv * dyv;
+    } else {
+      if (A.pte[X.h(i, j, k - 1)] <= 1.0e-5 || A.pte[X.h(i, j + 1, k - 1)] <= 1.0e-5) {
+        // (FCT(:,1) - dz) * nx, as :1722-1723 write it
+        const double dhec = cut_dhec(X, V, V.fct, i, j, k - 1, i, j, k - 1, i, j, k, 0,
+                                     A.dzw[k - 2]);
+        double dh = dhec;
+        if (j < J && usable(X, W, i, j + 1, k - 1) &&
+            W.vof[X.c(i, j + 1, k - 1)] >= W.vof[X.c(i, j, k - 1)]) {
+          dh = plane_dist(X, W, i, j + 1, k - 1);
+          wb = v12(A, 2, X.h(i, j + 1, k - 1));
+        }
+        if (j < J && usable(X, W, i, j, k - 1) &&
+            W.vof[X.c(i, j, k - 1)] > W.vof[X.c(i, j + 1, k - 1)]) {
+          dh = plane_dist(X, W, i, j, k - 1);
+          wb = v12(A, 2, X.h(i, j, k - 1));
+        }
+        wb = (wb - 0.0) * dhec / (dh + kEps) + 0.0;
+      } else {
+        const double Sy = W.syn[X.h(i, j, k - 1)];
+        eta = fabs(V.fct[X.h(i, j, k - 1) + X.Nh] + 0.5 * dyv - W.cent[X.c(i, j, k - 1) + X.N]) /
+              Sy;
+        if (fabs(eta) >= 1.0) eta = 0.5;
+        wb = (1.0 - eta) * v12(A, 2, X.h(i, j, k - 1)) + eta * v12(A, 2, X.h(i, j + 1, k - 1));
+      }
+      wbp = up_p(wb);
+      wbn = up_n(wb);
+      flux = (wbp * v12(A, 1, X.h(i, j, k - 1)) + wbn * v12(A, 1, X.h(i, j, k))) *
+             V.te[X.h(i, j, k - 1)] * dxv * dyv;
+    }
+    A.cftb[X.tb(i, j, k, 1)] = flux;
+  }
+  // w momentum, convective w (W grid: dx = dxp, dy = dyp)
+  {
+    const double dxw = A.dxp[i - 1], dyw = A.dyp[j - 1];
+    const double wk = v12(A, 2, X.h(i, j, k)), wm = v12(A, 2, X.h(i, j, k - 1));
+    wb = 0.5 * (wk + wm);
+    wbp = up_p(wb);
+    wbn = up_n(wb);
+    if (k >= K) {
+      flux = (wbp * wm + wbn * wk) * W.te[X.h(i, j, k - 1)] * dxw * dyw;
+    } else if (k == 1) {
+      flux = (wbp * wm + wbn * wk) * W.te[X.h(i, j, k)] * dxw * dyw;
+    } else {
+      eta = W.etat[X.h(i, j, k - 1)];
+      wb = (1.0 - eta) * wm + eta * wk;
+      wb = (wb - 0.0) * W.alt[X.h(i, j, k - 1)] + 0.0;
+      wbp = up_p(wb);
+      wbn = up_n(wb);
+      flux = (wbp * wm + wbn * wk) * W.te[X.h(i, j, k - 1)] * dxw * dyw;
+    }
+    A.cftb[X.tb(i, j, k, 2)] = flux;
+  }
+}
+
+// One thread per (i, j, k) in [1, I+1] x [1, J+1] x [1, K+1]: the x face
+// fluxes at i (j <= J, k <= K), the y faces at j (i <= I, k <= K), the z
+// faces at k (i <= I, j <= J).
+__global__ void __launch_bounds__(256) k_pred_faces(PredArgs A) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > A.I + 1 || j > A.J + 1 || k > A.K + 1) return;
+  const Ix X = ix_of(A);
+  if (j <= A.J && k <= A.K) flux_x(A, X, i, j, k);
+  if (i <= A.I && k <= A.K) flux_y(A, X, i, j, k);
+  if (i <= A.I && j <= A.J) flux_z(A, X, i, j, k);
+}
+
+// One thread per cell (i, j, k): the U, V, W cells it owns.
+__global__ void __launch_bounds__(256) k_pred_cells(PredArgs A) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > A.I || j > A.J || k > A.K) return;
+  const Ix X = ix_of(A);
+  const int64_t n = X.c(i, j, k), h = X.h(i, j, k);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if ((c == 0 && i == A.I) || (c == 1 && j == A.J) || (c == 2 && k == A.K)) continue;
+    const double Fe = A.cfew[X.ew(i + 1, j, k, c)], Fw = A.cfew[X.ew(i, j, k, c)];
+    const double Fn = A.cfns[X.ns(i, j + 1, k, c)], Fs = A.cfns[X.ns(i, j, k, c)];
+    const double Ft = A.cftb[X.tb(i, j, k + 1, c)], Fb = A.cftb[X.tb(i, j, k, c)];
+    const double flux0 = Fe - Fw + Fn - Fs + Ft - Fb;
+    double* fdo = A.fdo + n + X.N * c;
+    const double fdiv = A.ab2 ? 1.5 * flux0 - 0.5 * *fdo : flux0;
+    *fdo = flux0;
+    const PredCell& C = A.cur[c];
+    double* dp = A.dp[c];
+    double* pr = A.pred[c];
+    if (C.ct[n] == 2) {
+      dp[h] = 0.0;
+      pr[h] = 0.0;
+      continue;
+    }
+    const double dx = c == 0 ? A.dxu[i - 1] : A.dxp[i - 1];
+    const double dy = c == 1 ? A.dyv[j - 1] : A.dyp[j - 1];
+    const double dz = c == 2 ? A.dzw[k - 1] : A.dzp[k - 1];
+    const double vof = C.vof[n], vofl = C.vof_l[n];
+    // wall friction (:404-410)
+    const double fr =
+        (vofl / vof * A.mu_w + (1.0 - vofl / vof) * A.mu_a) / A.rey * C.wllh[n] / C.delh[n];
+    // aP with zero neighbour coefficients (:1025-1037)
+    const double aP = vof * dx * dy * dz / A.dt + fr;
+    double dpv = 1.0 / aP;
+    // right-hand side (:1206-1241)
+    const double s = (double)(c == 0) * A.vel[0][h] + (double)(c == 1) * A.vel[1][h] +
+                     (double)(c == 2) * A.vel[2][h];
+    const double gc = A.gx * (double)(c == 0) + A.gy * (double)(c == 1) + A.gz * (double)(c == 2);
+    double rhs = s * vof * dx * dy * dz / A.dt - s * (vof - A.old[c].vof[n]) * dx * dy * dz / A.dt +
+                 A.inv_fr2 * vof * dx * dy * dz * gc / A.g;
+    dpv = vof * dx * dy * dz * dpv;
+    rhs = rhs - fdiv;
+    dp[h] = dpv;
+    pr[h] = rhs / aP;
+  }
+}
+
+// Dp halo: every entry outside the component's computed range takes the
+// value at the clamped index (the component's last plane from the one
+// before it).
+__global__ void __launch_bounds__(256) k_pred_dp_halo(PredArgs A) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i > A.I + 1 || j > A.J + 1 || k > A.K + 1) return;
+  const Ix X = ix_of(A);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int ti = c == 0 ? A.I - 1 : A.I, tj = c == 1 ? A.J - 1 : A.J, tk = c == 2 ? A.K - 1 : A.K;
+    if (i >= 1 && i <= ti && j >= 1 && j <= tj && k >= 1 && k <= tk) continue;
+    const int si = i < 1 ? 1 : (i > ti ? ti : i);
+    const int sj = j < 1 ? 1 : (j > tj ? tj : j);
+    const int sk = k < 1 ? 1 : (k > tk ? tk : k);
+    A.dp[c][X.h(i, j, k)] = A.dp[c][X.h(si, sj, sk)];
+  }
+}
+
+// Predictor boundary conditions and the solid-cell condition of the
+// component's last plane (:3796-3916).  Thread (a, b) of face set q:
+// q = 0: u at i = 0, I, I+1 for (j, k) = (a, b); q = 1: v at j = 0, J, J+1
+// for (i, k); q = 2: w at k = 0, K, K+1 for (i, j).
+__global__ void __launch_bounds__(256) k_pred_bc(PredArgs A) {
+#pragma clang fp contract(off)
+  const int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int b = blockIdx.y + 1;
+  const int q = blockIdx.z;
+  const Ix X = ix_of(A);
+  const int na = q == 0 ? A.J : A.I, nb = q == 2 ? A.J : A.K;
+  if (a > na || b > nb) return;
+  const int L = q == 0 ? A.I : (q == 1 ? A.J : A.K);
+  auto at = [&](int m) -> int64_t {
+    return q == 0 ? X.h(m, a, b) : (q == 1 ? X.h(a, m, b) : X.h(a, b, m));
+  };
+  double* p = A.pred[q];
+  const int fw = 6 * q + 2 * q, fe = fw + 1;  // W/E of u, S/N of v, B/T of w
+  const double ve = bcval(A, fe, a, b);
+  p[at(0)] = A.flag[fw] == 1 ? p[at(1)] : bcval(A, fw, a, b);
+  const double before = p[at(L - 1)];
+  p[at(L)] = A.flag[fe] == 1 ? before : ve;
+  p[at(L + 1)] = 2.0 * ve - before;
+  const int64_t n = q == 0 ? X.c(L, a, b) : (q == 1 ? X.c(a, L, b) : X.c(a, b, L));
+  if (A.cur[q].ct[n] == 2) p[at(L)] = 0.0;
+}
+
+}  // namespace
+
+static PredCell pred_cell(const sp_cell_geom& g, const sp_cell_interp& x) {
+  PredCell c{};
+  c.phi = g.phi; c.nx = g.nx; c.ny = g.ny; c.nz = g.nz; c.vof = g.vof; c.vof_l = g.vof_l;
+  c.ct = g.cell_type; c.moex = x.moex;
+  c.ee = g.ee_area; c.ne = g.ne_area; c.te = g.te_area;
+  c.fce = g.fce; c.fcn = g.fcn; c.fct = g.fct; c.cent = g.cell_cent;
+  c.sxe = x.sxe; c.syn = x.syn; c.szt = x.szt;
+  c.etae = x.etae; c.etan = x.etan; c.etat = x.etat;
+  c.ale = x.ale; c.aln = x.aln; c.alt = x.alt;
+  c.wllh = g.wllh; c.delh = g.delh;
+  return c;
+}
+
+sp_status predict_uvw(sp_ctx* c, const sp_pred_inputs& in, const sp_pred_outputs& out,
+                      double* faces) {
+  const Dims& D = c->dm;
+  PredArgs A{};
+  A.I = D.I; A.J = D.J; A.K = D.K; A.SX = D.SX; A.SY = D.SY; A.N = D.N; A.Nh = D.Nh;
+  A.dxp = c->dxp; A.dyp = c->dyp; A.dzp = c->dzp; A.dxu = c->dxu; A.dyv = c->dyv; A.dzw = c->dzw;
+  A.pee = in.p_ee_area; A.pne = in.p_ne_area; A.pte = in.p_te_area;
+  const sp_cell_geom* og[3] = {in.u_old, in.v_old, in.w_old};
+  const sp_cell_interp* ox[3] = {in.u_old_x, in.v_old_x, in.w_old_x};
+  const sp_cell_geom* cg[3] = {in.u_cur, in.v_cur, in.w_cur};
+  for (int q = 0; q < 3; ++q) {
+    A.old[q] = pred_cell(*og[q], *ox[q]);
+    sp_cell_interp none{};
+    A.cur[q] = pred_cell(*cg[q], none);
+  }
+  A.vel[0] = in.u; A.vel[1] = in.v; A.vel[2] = in.w;
+  A.veln[0] = in.u_n; A.veln[1] = in.v_n; A.veln[2] = in.w_n;
+  A.extrapolate = in.time_order == 2;
+  A.ab2 = in.itt != 1 && in.time_order != 1 && in.time_order != 2;
+  for (int q = 0; q < 18; ++q) { A.bcv[q] = in.bc_var[q]; A.flag[q] = in.bc_flag[q]; }
+  A.fdo = in.flux_div_old;
+  const int64_t few = (int64_t)(D.I + 2) * D.J * D.K * 3, fns = (int64_t)D.I * (D.J + 2) * D.K * 3;
+  A.cfew = faces; A.cfns = faces + few; A.cftb = faces + few + fns;
+  A.pred[0] = out.u; A.pred[1] = out.v; A.pred[2] = out.w;
+  A.dp[0] = out.dp_u; A.dp[1] = out.dp_v; A.dp[2] = out.dp_w;
+  A.dt = in.dt;
+  A.inv_fr2 = 1.0 / (in.fr * in.fr);  // 1.d0/Fr**2.d0
+  A.gx = in.gx; A.gy = in.gy; A.gz = in.gz; A.g = in.g;
+  A.mu_w = in.mu_w_ref; A.mu_a = in.mu_a_ref; A.rey = in.rey;
+  const dim3 blk(64, 4, 1);
+  int slot;
+  const double NB = (double)D.N;
+  // per cell: old velocities (x3, x2 for TimeOrder 2), P apertures, the
+  // old cells' apertures, interpolation and centre data read by the
+  // non-cut branches -> 9 face fluxes
+  timer_begin(c, "pred_faces", NB * (8.0 * (3 + 3 + 9 + 3 + 3 + 6 + 6 + 6) + 72.0), &slot);
+  hipLaunchKernelGGL(k_pred_faces, dim3((D.I + 1 + 63) / 64, (D.J + 1 + 3) / 4, D.K + 1), blk, 0,
+                     c->stream, A);
+  timer_end(c, slot);
+  // per cell: 9 face fluxes, FluxDivOld, current vof/vofL/type/WlLh/delh,
+  // old vof, velocities -> FluxDivOld, Dp, u*
+  timer_begin(c, "pred_cells", NB * (72.0 + 48.0 + 3 * (8 + 8 + 4 + 8 + 8) + 24 + 24 + 48), &slot);
+  hipLaunchKernelGGL(k_pred_cells, dim3((D.I + 63) / 64, (D.J + 3) / 4, D.K), blk, 0, c->stream, A);
+  timer_end(c, slot);
+  hipLaunchKernelGGL(k_pred_dp_halo, dim3((D.I + 2 + 63) / 64, (D.J + 2 + 3) / 4, D.K + 2), blk, 0,
+                     c->stream, A);
+  const int amax = std::max(D.I, D.J), bmax = std::max(D.J, D.K);
+  hipLaunchKernelGGL(k_pred_bc, dim3((amax + 255) / 256, bmax, 3), dim3(256), 0, c->stream, A);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SP_OK : hip_check(c, e, "predictor launch");
+}
+
+size_t predict_face_doubles(const Dims& D) {
+  return (size_t)(D.I + 2) * D.J * D.K * 3 + (size_t)D.I * (D.J + 2) * D.K * 3 +
+         (size_t)D.I * D.J * (D.K + 2) * 3;
+}
+
+}  // namespace sp

@@ -37,7 +37,7 @@ EXPORTED_SYMBOLS = (
     "sp_strerror", "sp_last_error", "sp_wall_step", "sp_wall_step_dev", "sp_write_vtr",
     "sp_grid_preprocess", "sp_grid_preprocess_dev", "sp_define_momentum_exchange_cell",
     "sp_number_external_cell", "sp_new_cell_face", "sp_compute_uvw_liquid_field",
-    "sp_number_external_cell_noalloc",
+    "sp_number_external_cell_noalloc", "sp_predict_uvw",
 )
 
 # members of sp_liquid_field, in order
@@ -166,6 +166,26 @@ class SpLiquidField(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in LIQUID_FIELDS]
 
 
+class SpPredInputs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "p_ee_area", "p_ne_area", "p_te_area", "u_old", "v_old", "w_old", "u_old_x", "v_old_x",
+        "w_old_x", "u_cur", "v_cur", "w_cur", "u", "v", "w", "u_n", "v_n", "w_n",
+        "flux_div_old")] + [
+        ("bc_flag", ctypes.c_int32 * 18), ("bc_var", ctypes.c_void_p * 18),
+        ("dt", ctypes.c_double), ("itt", ctypes.c_int64), ("time_order", ctypes.c_int32),
+        ("space_order", ctypes.c_int32)] + [
+        (n, ctypes.c_double) for n in ("mu_w_ref", "mu_a_ref", "rey", "fr", "gx", "gy", "gz", "g")]
+
+
+class SpPredOutputs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("u", "v", "w", "dp_u", "dp_v", "dp_w")]
+
+
+# predictor cell members (configs.PRED_CELL_FIELDS) -> sp_cell_geom / sp_cell_interp
+_PRED_GEOM = {"ee": "ee_area", "ne": "ne_area", "te": "te_area", "cent": "cell_cent"}
+_PRED_INTERP = ("moex", "sxe", "syn", "szt", "etae", "etan", "etat", "ale", "aln", "alt")
+
+
 class SpError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{msg} (status {status})")
@@ -227,6 +247,9 @@ def load():
     L.sp_number_external_cell.restype = ctypes.c_int
     L.sp_number_external_cell_noalloc.argtypes = [V] + [ctypes.c_int] * 3 + [V, V, V]
     L.sp_number_external_cell_noalloc.restype = ctypes.c_int
+    L.sp_predict_uvw.argtypes = [V, ctypes.POINTER(SpPredInputs), ctypes.POINTER(SpPredOutputs),
+                                 ctypes.c_int]
+    L.sp_predict_uvw.restype = ctypes.c_int
     L.sp_stream.argtypes = [V]
     L.sp_stream.restype = V
     L.sp_kernel_times.argtypes = [V, P(SpKernelTime), ctypes.c_int]
@@ -543,6 +566,76 @@ class Projector:
                                               ctypes.byref(ext), 0)
         self._check(rc, "sp_number_external_cell")
         return pos, ext.value
+
+    def predict_uvw(self, p_areas, old, cur, fields, pred, dp, device=False):
+        """PredictingUVW (sp_predict_uvw) with SpaceOrder 1.  ``p_areas``:
+        PCellO%EEArea, NEArea, TEArea; ``old``/``cur``: three dicts (U, V, W)
+        keyed by configs.PRED_CELL_FIELDS (UCellO.. / UCell..); ``fields``: the
+        dict of configs.predictor_fields (vel, vel_n, flux_div_old, bc_flag,
+        bcv, params).  ``pred`` (Pred%u, v, w) and ``dp`` (PU/PV/PW%Dp) are
+        written in place, and fields["flux_div_old"] is updated in place:
+        numpy arrays in the host's layouts, or torch tensors with
+        device=True."""
+        keep = []
+
+        def ptr(a, dt=None):
+            if a is None:
+                return None
+            if not device:
+                a = np.asfortranarray(a, dtype=dt or a.dtype)
+                keep.append(a)
+            return _addr(a)
+
+        def geom(d):
+            g = SpCellGeom(*(None for _ in GEOM_FIELDS))
+            for n in d:
+                if n in _PRED_INTERP:
+                    continue
+                setattr(g, _PRED_GEOM.get(n, n),
+                        ptr(d[n], np.int32 if n == "cell_type" else np.float64))
+            keep.append(g)
+            return g
+
+        def interp(d):
+            x = SpCellInterp(*(None for _ in INTERP_FIELDS))
+            for n in _PRED_INTERP:
+                if n in d:
+                    setattr(x, n, ptr(d[n], np.int32 if n == "moex" else np.float64))
+            keep.append(x)
+            return x
+
+        inp = SpPredInputs()
+        inp.p_ee_area, inp.p_ne_area, inp.p_te_area = (ptr(a, np.float64) for a in p_areas)
+        og = [geom(d) for d in old]
+        ox = [interp(d) for d in old]
+        cg = [geom(d) for d in cur]
+        inp.u_old, inp.v_old, inp.w_old = (ctypes.addressof(g) for g in og)
+        inp.u_old_x, inp.v_old_x, inp.w_old_x = (ctypes.addressof(x) for x in ox)
+        inp.u_cur, inp.v_cur, inp.w_cur = (ctypes.addressof(g) for g in cg)
+        inp.u, inp.v, inp.w = (ptr(a, np.float64) for a in fields["vel"])
+        if fields.get("vel_n") is not None:
+            inp.u_n, inp.v_n, inp.w_n = (ptr(a, np.float64) for a in fields["vel_n"])
+        fdo = fields["flux_div_old"]
+        if not device and not (isinstance(fdo, np.ndarray) and fdo.flags.f_contiguous and
+                               fdo.dtype == np.float64):
+            raise ValueError("flux_div_old must be a Fortran-ordered float64 array (updated in place)")
+        inp.flux_div_old = _addr(fdo)
+        for q in range(18):
+            inp.bc_flag[q] = int(fields["bc_flag"][q])
+            inp.bc_var[q] = ptr(fields["bcv"][q], np.float64)
+        p = fields["params"]
+        inp.dt, inp.itt = p["dt"], p["itt"]
+        inp.time_order, inp.space_order = p["time_order"], p["space_order"]
+        for n in ("mu_w_ref", "mu_a_ref", "rey", "fr", "gx", "gy", "gz", "g"):
+            setattr(inp, n, p[n])
+        for a in list(pred) + list(dp):
+            if not device and not (isinstance(a, np.ndarray) and a.flags.f_contiguous and
+                                   a.dtype == np.float64):
+                raise ValueError("pred and dp must be Fortran-ordered float64 arrays")
+        out = SpPredOutputs(*(_addr(a) for a in list(pred) + list(dp)))
+        rc = self.lib.sp_predict_uvw(self.h, ctypes.byref(inp), ctypes.byref(out),
+                                     1 if device else 0)
+        self._check(rc, "sp_predict_uvw")
 
     def number_external_noalloc(self, cell_type, iu, iv, iw, posnu, ext_cell):
         """NumberExternalCell without device allocation
