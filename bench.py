@@ -63,6 +63,11 @@ def parse():
     ap.add_argument("--geometry", action="store_true",
                     help="also time GridPreprocess (cut-cell geometry of the P, U, V, W grids) "
                          "at the same grid size on a three-sphere solid, with its CPU baseline")
+    ap.add_argument("--predictor", action="store_true",
+                    help="also time the momentum predictor (sp_predict_uvw) at the same grid "
+                         "size, with its CPU baseline")
+    ap.add_argument("--predictor-only", action="store_true",
+                    help="time only the momentum predictor and print its record")
     ap.add_argument("--geometry-only", action="store_true",
                     help="time only GridPreprocess (no projection) and print its record")
     ap.add_argument("--copy-probe", action="store_true",
@@ -331,6 +336,129 @@ def write_vtr(P, dev, out, case, directory):
     return dict(path=path, bytes=size, seconds=round(el, 3), mb_per_s=round(size / el / 1e6, 1),
                 format_launches=r["launches"], format_ms=round(r["total_ms"], 3),
                 format_gbs=round(r["bytes"] / (r["total_ms"] * 1e-3) / 1e9, 1))
+
+
+def predictor(shape, steps, warmup, cpu=True):
+    """The momentum predictor (sp_predict_uvw, PredictingUVW with SpaceOrder
+    1 and TimeOrder 1) on device-resident arrays: the geometry of three
+    spherical solids on a z-stretched grid is built on the GPU by the
+    engine's own geometry update, velocities and BC values are seeded.
+    Timed calls of the whole step (four launches), per-kernel GB/s from HIP
+    events over the timed calls, and the oracle's predictor on the same
+    inputs (copied to the host) as the CPU baseline."""
+    import numpy as np
+    import torch
+
+    from simcoflow3d_amd import configs
+    from simcoflow3d_amd.engine import GEOM_FIELDS, INTERP_FIELDS, Projector, geom_shapes
+
+    I, J, K = shape
+    tg = time.perf_counter()
+    cells = configs.geometry_case(shape, "bodies", seed=5, stretch=True)
+    P = Projector(shape)
+    P0 = cells[0]
+    P.set_grid_1d(P0["dx"][:, 0, 0], P0["dy"][0, :, 0], P0["dz"][0, 0, :],
+                  cells[1]["dx"][:, 0, 0], cells[2]["dy"][0, :, 0], cells[3]["dz"][0, 0, :])
+    sh = geom_shapes(I, J, K)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).to("cuda")  # noqa: E731
+    dev = []
+    for c in cells:
+        d = {}
+        for n in GEOM_FIELDS:
+            if n in c:
+                d[n] = T(c[n])
+            else:
+                dt = torch.int32 if n == "cell_type" else torch.float64
+                d[n] = torch.zeros(sh[n][::-1], dtype=dt, device="cuda")
+        dev.append(d)
+    del cells
+    zi = lambda shp: torch.zeros(shp, dtype=torch.int32, device="cuda")  # noqa: E731
+    hshape = (K + 2, J + 2, I + 2)
+    interp = [dict(moex=zi((K, J, I)), msce=zi((3, K, J, I)),
+                   **{n: torch.zeros(hshape, dtype=torch.float64, device="cuda")
+                      for n in INTERP_FIELDS[2:]}) for _ in range(3)]
+    P.grid_preprocess(dev, device=True)
+    P.define_moex(*(d["cell_type"] for d in dev), *(x["moex"] for x in interp), device=True)
+    P.new_cell_face(dev, interp, device=True)
+    f = configs.predictor_fields(shape, seed=3, time_order=1, itt=2)
+    geo_np = None
+    if cpu:
+        back = lambda t: np.asfortranarray(t.cpu().numpy().T)  # noqa: E731
+        geo_np = ([{n: back(t) for n, t in d.items()} for d in dev],
+                  [{n: back(t) for n, t in x.items()} for x in interp])
+    # the predictor's view of the cells (device tensors): UCellO = UCell here
+    pc = [dict() for _ in range(3)]
+    for q in range(3):
+        for n in configs.PRED_CELL_FIELDS:
+            src = {"ee": "ee_area", "ne": "ne_area", "te": "te_area", "cent": "cell_cent"}.get(n, n)
+            pc[q][n] = dev[q + 1][src] if src in dev[q + 1] else interp[q][src]
+    pa = [dev[0]["ee_area"], dev[0]["ne_area"], dev[0]["te_area"]]
+    fd = dict(f)
+    fd["vel"] = [T(a) for a in f["vel"]]
+    fd["vel_n"] = None
+    fd["flux_div_old"] = T(f["flux_div_old"])
+    fd["bcv"] = [T(a) for a in f["bcv"]]
+    pred = [torch.zeros(hshape, dtype=torch.float64, device="cuda") for _ in range(3)]
+    dp = [torch.zeros(hshape, dtype=torch.float64, device="cuda") for _ in range(3)]
+    torch.cuda.synchronize()
+    print(f"[bench] predictor inputs on the device ({time.perf_counter() - tg:.1f} s)",
+          file=sys.stderr, flush=True)
+    for _ in range(max(1, warmup)):
+        P.predict_uvw(pa, pc, pc, fd, pred, dp, device=True)
+    P.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        P.predict_uvw(pa, pc, pc, fd, pred, dp, device=True)  # synchronises its stream
+    el = time.perf_counter() - t0
+    rows = P.kernel_times()
+    P.set_profiling(False)
+    P.close()
+    N = I * J * K
+    kern = []
+    for r in rows:
+        ms = r["total_ms"]
+        gbs = r["bytes"] / (ms * 1e-3) / 1e9 if ms > 0 and r["bytes"] > 0 else None
+        kern.append(dict(name=r["name"], launches=r["launches"],
+                         avg_us=round(ms / max(r["launches"], 1) * 1e3, 1),
+                         bytes_per_launch=r["bytes"] / max(r["launches"], 1),
+                         gbs=round(gbs, 1) if gbs else None,
+                         frac=round(gbs / HBM_PEAK_GBS, 4) if gbs else None))
+    kern.sort(key=lambda t: -(t["avg_us"] * t["launches"]))
+    rec = dict(workload=f"PredictingUVW, SpaceOrder 1, TimeOrder 1, {I}x{J}x{K}, three spheres, "
+                        "z stretched 2% per cell, geometry from the engine's geometry update",
+               calls=steps, ms_per_call=round(el / steps * 1e3, 3), cells_per_s=round(N * steps / el, 1),
+               kernels=kern)
+    del dev, interp, pc, pa, fd, pred, dp
+    torch.cuda.empty_cache()
+    if cpu:
+        from oracle import orc
+
+        class _G:
+            pass
+        g = _G()
+        g.I, g.J, g.K = I, J, K
+        geo, xs = geo_np
+        g.dxp, g.dyp, g.dzp = (np.ascontiguousarray(geo[0][n][:, 0, 0] if n == "dx" else
+                                                    (geo[0][n][0, :, 0] if n == "dy" else
+                                                     geo[0][n][0, 0, :])) for n in ("dx", "dy", "dz"))
+        g.dxu = np.ascontiguousarray(geo[1]["dx"][:, 0, 0])
+        g.dyv = np.ascontiguousarray(geo[2]["dy"][0, :, 0])
+        g.dzw = np.ascontiguousarray(geo[3]["dz"][0, 0, :])
+        cc = configs.pred_cells(geo, xs)
+        pa_h = [geo[0]["ee_area"], geo[0]["ne_area"], geo[0]["te_area"]]
+        runs, secs = 0, 0.0
+        while runs == 0 or (secs < 10.0 and runs < 20):
+            t0 = time.perf_counter()
+            orc.predict(g, pa_h, cc, cc, f["vel"], f["vel_n"], f["flux_div_old"], f["bc_flag"],
+                        f["bcv"], f["params"])
+            secs += time.perf_counter() - t0
+            runs += 1
+        rec["cpu_baseline"] = dict(value=round(runs * N / secs, 1), unit="cells/s",
+                                   cores=orc.num_threads(), kind="port",
+                                   sample=f"{runs} x the same {I}x{J}x{K} step",
+                                   seconds=round(secs, 3))
+    return rec
 
 
 def geometry(shape, steps, warmup, cpu=True):
@@ -610,6 +738,11 @@ def main():
 
     cfg = args.config
     size = tuple(args.size) if args.size else None
+    if args.predictor_only:
+        pshape = size or configs.config_dims(cfg)
+        print(json.dumps(dict(predictor=predictor(pshape, args.steps, args.warmup,
+                                                  cpu=not args.no_cpu))), flush=True)
+        return
     if args.geometry_only:
         gshape = size or configs.config_dims(cfg)
         print(json.dumps(dict(geometry=geometry(gshape, args.steps, args.warmup,
@@ -646,11 +779,14 @@ def main():
     vtr = write_vtr(Pm, dev, out, case, args.vtr) if args.vtr else None
     stream = stream_probe() if args.stream_probe else None
     Pm.close()
-    geo = None
-    if args.geometry:
+    geo = pred_rec = None
+    if args.geometry or args.predictor:
         del dev, out
         torch.cuda.empty_cache()
-        geo = geometry(case.shape, args.steps, args.warmup, cpu=not args.no_cpu)
+        if args.geometry:
+            geo = geometry(case.shape, args.steps, args.warmup, cpu=not args.no_cpu)
+        if args.predictor:
+            pred_rec = predictor(case.shape, args.steps, args.warmup, cpu=not args.no_cpu)
         dev = to_device(case)
         out = make_outputs(case, dev)
 
@@ -757,6 +893,7 @@ def main():
         "vtr": vtr,
         "stream_probe": stream,
         "geometry": geo,
+        "predictor": pred_rec,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": [dict(name=k["name"], launches=k["launches"], work_launches=k["work_launches"],

@@ -15,12 +15,13 @@ module simco_proj
   implicit none
   private
   public :: sp_params, sp_inputs, sp_outputs, sp_stats, sp_adj_inputs, sp_adj_outputs, &
-            sp_wall_bc, sp_vtr_fields, sp_cell_geom, sp_cell_interp, sp_liquid_field
+            sp_wall_bc, sp_vtr_fields, sp_cell_geom, sp_cell_interp, sp_liquid_field, &
+            sp_pred_inputs, sp_pred_outputs
   public :: sp_params_default, sp_create, sp_destroy, sp_set_params, sp_set_grid, &
             sp_set_grid_1d, sp_project, sp_project_dev, sp_adjoint, sp_adjoint_dev, &
             sp_wall_step, sp_wall_step_dev, sp_write_vtr, sp_grid_preprocess, &
             sp_grid_preprocess_dev, sp_new_cell_face, sp_define_momentum_exchange_cell, &
-            sp_compute_uvw_liquid_field, &
+            sp_compute_uvw_liquid_field, sp_predict_uvw, sp_number_external_cell_noalloc, &
             sp_number_external_cell, sp_strerror_f, sp_last_error_f
   public :: SP_OK, SP_ERR_INVALID_ARG, SP_ERR_HIP, SP_ERR_NOT_CONVERGED, SP_ERR_NONFINITE, &
             SP_ERR_OOM, SP_ERR_GRID, SP_ERR_STATE, SP_ERR_NO_DEVICE, SP_ERR_CAPACITY, &
@@ -133,6 +134,33 @@ module simco_proj
     type(c_ptr) :: phi_l = c_null_ptr, nx_l = c_null_ptr, ny_l = c_null_ptr
     type(c_ptr) :: nz_l = c_null_ptr, vof_l = c_null_ptr
   end type sp_liquid_field
+
+  ! PredictingUVW inputs: c_loc of PCellO%EEArea.., of type(sp_cell_geom) /
+  ! type(sp_cell_interp) variables describing UCellO.. and UCell.., of TVar
+  ! and TVar_n velocities, FluxDivOld, the BCu/BCv/BCw face arrays
+  type, bind(C) :: sp_pred_inputs
+    type(c_ptr) :: p_ee_area = c_null_ptr, p_ne_area = c_null_ptr, p_te_area = c_null_ptr
+    type(c_ptr) :: u_old = c_null_ptr, v_old = c_null_ptr, w_old = c_null_ptr
+    type(c_ptr) :: u_old_x = c_null_ptr, v_old_x = c_null_ptr, w_old_x = c_null_ptr
+    type(c_ptr) :: u_cur = c_null_ptr, v_cur = c_null_ptr, w_cur = c_null_ptr
+    type(c_ptr) :: u = c_null_ptr, v = c_null_ptr, w = c_null_ptr
+    type(c_ptr) :: u_n = c_null_ptr, v_n = c_null_ptr, w_n = c_null_ptr
+    type(c_ptr) :: flux_div_old = c_null_ptr
+    integer(c_int32_t) :: bc_flag(18) = 1
+    type(c_ptr) :: bc_var(18) = c_null_ptr
+    real(c_double) :: dt = 0.0_c_double
+    integer(c_int64_t) :: itt = 1
+    integer(c_int32_t) :: time_order = 1, space_order = 1
+    real(c_double) :: mu_w_ref = 1.0_c_double, mu_a_ref = 1.0_c_double
+    real(c_double) :: rey = 1.0_c_double, fr = 1.0_c_double
+    real(c_double) :: gx = 0.0_c_double, gy = 0.0_c_double, gz = 0.0_c_double, g = 1.0_c_double
+  end type sp_pred_inputs
+
+  ! c_loc of Pred%u, v, w and PU%Dp, PV%Dp, PW%Dp
+  type, bind(C) :: sp_pred_outputs
+    type(c_ptr) :: u = c_null_ptr, v = c_null_ptr, w = c_null_ptr
+    type(c_ptr) :: dp_u = c_null_ptr, dp_v = c_null_ptr, dp_w = c_null_ptr
+  end type sp_pred_outputs
 
   type, bind(C) :: sp_adj_inputs
     type(c_ptr) :: gu = c_null_ptr, gv = c_null_ptr, gw = c_null_ptr
@@ -263,6 +291,25 @@ module simco_proj
       type(sp_liquid_field), intent(in) :: p, u, v, w
       integer(c_int), value :: device
     end function sp_compute_uvw_liquid_field
+
+    ! PredictingUVW with SpaceOrder 1; device = 0 for host arrays
+    integer(c_int) function sp_predict_uvw(ctx, inp, outp, device) bind(C, name="sp_predict_uvw")
+      import :: c_ptr, c_int, sp_pred_inputs, sp_pred_outputs
+      type(c_ptr), value :: ctx
+      type(sp_pred_inputs), intent(in) :: inp
+      type(sp_pred_outputs), intent(inout) :: outp
+      integer(c_int), value :: device
+    end function sp_predict_uvw
+
+    ! NumberExternalCell without device allocation: cell_type, posnu and
+    ! ext_cell are device memory (c_ptr); the call only enqueues work
+    integer(c_int) function sp_number_external_cell_noalloc(ctx, iu, iv, iw, cell_type, posnu, &
+        ext_cell) bind(C, name="sp_number_external_cell_noalloc")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: iu, iv, iw
+      type(c_ptr), value :: cell_type, posnu, ext_cell
+    end function sp_number_external_cell_noalloc
 
     integer(c_int) function sp_new_cell_face(ctx, p, u, v, w, iu, iv, iw, device) &
         bind(C, name="sp_new_cell_face")

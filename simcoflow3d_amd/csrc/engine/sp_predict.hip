@@ -632,7 +632,7 @@ sp_status predict_uvw(sp_ctx* c, const sp_pred_inputs& in, const sp_pred_outputs
   // per cell: old velocities (x3, x2 for TimeOrder 2), P apertures, the
   // old cells' apertures, interpolation and centre data read by the
   // non-cut branches -> 9 face fluxes
-  timer_begin(c, "pred_faces", NB * (8.0 * (3 + 3 + 9 + 3 + 3 + 6 + 6 + 6) + 72.0), &slot);
+  timer_begin(c, "pred_faces", NB * (8.0 * (3 + 3 + 9 + 4 + 6 + 6 + 6 + (A.extrapolate ? 3 : 0)) + 72.0), &slot);
   hipLaunchKernelGGL(k_pred_faces, dim3((D.I + 1 + 63) / 64, (D.J + 1 + 3) / 4, D.K + 1), blk, 0,
                      c->stream, A);
   timer_end(c, slot);

@@ -15,6 +15,8 @@ program sp_layout
   type(sp_cell_geom), target :: cg
   type(sp_cell_interp), target :: ci
   type(sp_liquid_field), target :: lf
+  type(sp_pred_inputs), target :: pin
+  type(sp_pred_outputs), target :: pout
   print '(a,1x,i0)', 'sp_params', c_sizeof(prm)
   print '(a,1x,i0)', 'sp_inputs', c_sizeof(inp)
   print '(a,1x,i0)', 'sp_outputs', c_sizeof(outp)
@@ -26,6 +28,14 @@ program sp_layout
   print '(a,1x,i0)', 'sp_cell_geom', c_sizeof(cg)
   print '(a,1x,i0)', 'sp_cell_interp', c_sizeof(ci)
   print '(a,1x,i0)', 'sp_liquid_field', c_sizeof(lf)
+  print '(a,1x,i0)', 'sp_pred_inputs', c_sizeof(pin)
+  print '(a,1x,i0)', 'sp_pred_outputs', c_sizeof(pout)
+  print '(a,1x,i0)', 'sp_pred_inputs.bc_flag', off(c_loc(pin%bc_flag), c_loc(pin))
+  print '(a,1x,i0)', 'sp_pred_inputs.bc_var', off(c_loc(pin%bc_var), c_loc(pin))
+  print '(a,1x,i0)', 'sp_pred_inputs.itt', off(c_loc(pin%itt), c_loc(pin))
+  print '(a,1x,i0)', 'sp_pred_inputs.space_order', off(c_loc(pin%space_order), c_loc(pin))
+  print '(a,1x,i0)', 'sp_pred_inputs.g', off(c_loc(pin%g), c_loc(pin))
+  print '(a,1x,i0)', 'sp_pred_outputs.dp_w', off(c_loc(pout%dp_w), c_loc(pout))
   print '(a,1x,i0)', 'sp_liquid_field.vof_l', off(c_loc(lf%vof_l), c_loc(lf))
   print '(a,1x,i0)', 'sp_cell_interp.sxe', off(c_loc(ci%sxe), c_loc(ci))
   print '(a,1x,i0)', 'sp_cell_interp.dalt', off(c_loc(ci%dalt), c_loc(ci))

@@ -41,7 +41,8 @@ def test_fortran_types_match_c_layout():
                "sp_adj_inputs": engine.SpAdjInputs, "sp_adj_outputs": engine.SpAdjOutputs,
                "sp_wall_bc": engine.SpWallBc, "sp_vtr_fields": engine.SpVtrFields,
                "sp_cell_geom": engine.SpCellGeom, "sp_cell_interp": engine.SpCellInterp,
-               "sp_liquid_field": engine.SpLiquidField}
+               "sp_liquid_field": engine.SpLiquidField, "sp_pred_inputs": engine.SpPredInputs,
+               "sp_pred_outputs": engine.SpPredOutputs}
     for name, cls in structs.items():
         assert got[name] == ctypes.sizeof(cls), name
     for key, v in got.items():
