@@ -56,7 +56,8 @@ def parse():
                          "per-kernel GB/s of one profiled pair")
     ap.add_argument("--vtr", type=str, default=None, metavar="DIR",
                     help="after the timed run, write the mixed solution as PrintResultVTR3D's "
-                         "VTR file under DIR and report the writer's time")
+                         "VTR file and PrintResultVTK's binary file under DIR and report "
+                         "the writers' times")
     ap.add_argument("--stream-probe", action="store_true",
                     help="measure the practical HBM bandwidth of plain streaming kernels "
                          "(torch copy and a*x+y on 2 GiB FP64 arrays) and report it")
@@ -309,9 +310,9 @@ def stream_probe(reps=10):
 
 
 def write_vtr(P, dev, out, case, directory):
-    """sp_write_vtr of the device-resident solution: wall time of the call and
-    the GPU formatting kernel's algorithmic GB/s (8 B read, 17.2 B of text
-    written per value)."""
+    """sp_write_vtr and sp_write_vtk of the device-resident solution: wall time
+    of each call and the GPU kernel's algorithmic GB/s (VTR: 8 B read, 17.2 B
+    of text written per value)."""
     import numpy as np
     import torch
     x = torch.from_numpy(np.cumsum(case.dxp) - 0.5 * case.dxp).cuda()
@@ -333,9 +334,34 @@ def write_vtr(P, dev, out, case, directory):
     P.set_profiling(False)
     r = rows[0]
     size = os.path.getsize(path)
-    return dict(path=path, bytes=size, seconds=round(el, 3), mb_per_s=round(size / el / 1e6, 1),
-                format_launches=r["launches"], format_ms=round(r["total_ms"], 3),
-                format_gbs=round(r["bytes"] / (r["total_ms"] * 1e-3) / 1e9, 1))
+    res = dict(path=path, bytes=size, seconds=round(el, 3), mb_per_s=round(size / el / 1e6, 1),
+               format_launches=r["launches"], format_ms=round(r["total_ms"], 3),
+               format_gbs=round(r["bytes"] / (r["total_ms"] * 1e-3) / 1e9, 1))
+    # PrintResultVTK's binary XML file of the same fields (sp_write_vtk):
+    # base64 kernel GB/s over 6 B of doubles read and 4 B of text written per
+    # 3 stream bytes
+    dx, dy, dz = (torch.from_numpy(np.ascontiguousarray(a)).cuda()
+                  for a in (case.dxp, case.dyp, case.dzp))
+    kfields = dict(x=x, y=y, z=z, dx=dx, dy=dy, dz=dz, p=out["p"], u=out["u"], v=out["v"],
+                   w=out["w"], phi_l=dev["phi_l"], phi=dev["phi_l"], vof_l=dev["vof_l"],
+                   vof=dev["vof"])
+    P.set_profiling(True)
+    P.set_profiling_filter("vtk_base64")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kpath = P.write_vtk(directory if directory.endswith("/") else directory + "/", 1, kfields,
+                        device=True)
+    el = time.perf_counter() - t0
+    rows = [r for r in P.kernel_times() if r["name"] == "vtk_base64"]
+    P.set_profiling_filter(None)
+    P.set_profiling(False)
+    r = rows[0]
+    size = os.path.getsize(kpath)
+    res["vtk"] = dict(path=kpath, bytes=size, seconds=round(el, 3),
+                      mb_per_s=round(size / el / 1e6, 1), base64_launches=r["launches"],
+                      base64_ms=round(r["total_ms"], 3),
+                      base64_gbs=round(r["bytes"] / (r["total_ms"] * 1e-3) / 1e9, 1))
+    return res
 
 
 def predictor(shape, steps, warmup, cpu=True):

@@ -15,11 +15,11 @@ module simco_proj
   implicit none
   private
   public :: sp_params, sp_inputs, sp_outputs, sp_stats, sp_adj_inputs, sp_adj_outputs, &
-            sp_wall_bc, sp_vtr_fields, sp_cell_geom, sp_cell_interp, sp_liquid_field, &
+            sp_wall_bc, sp_vtr_fields, sp_vtk_fields, sp_solid_body, sp_lvs_grid, sp_cell_geom, sp_cell_interp, sp_liquid_field, &
             sp_pred_inputs, sp_pred_outputs
   public :: sp_params_default, sp_create, sp_destroy, sp_set_params, sp_set_grid, &
             sp_set_grid_1d, sp_project, sp_project_dev, sp_adjoint, sp_adjoint_dev, &
-            sp_wall_step, sp_wall_step_dev, sp_write_vtr, sp_grid_preprocess, &
+            sp_wall_step, sp_wall_step_dev, sp_write_vtr, sp_write_vtk, sp_interpolate_lvs, sp_grid_preprocess, &
             sp_grid_preprocess_dev, sp_new_cell_face, sp_define_momentum_exchange_cell, &
             sp_compute_uvw_liquid_field, sp_predict_uvw, sp_number_external_cell_noalloc, &
             sp_number_external_cell, sp_strerror_f, sp_last_error_f
@@ -103,6 +103,39 @@ module simco_proj
     type(c_ptr) :: mres = c_null_ptr
     integer(c_int32_t) :: halo_mask = 0
   end type sp_vtr_fields
+
+  ! PrintResultVTK's fields (c_loc of the host arrays): cell centres x, y, z
+  ! and widths dx, dy, dz (TGrid%x(:,1,1), TGrid%dx(:,1,1), ...); bit q of
+  ! halo_mask marks field q (p, u, v, w, phi_l, phi, vof_l, vof) as a
+  ! (0:Imax+1, ...) array, otherwise (1:Imax, ...)
+  type, bind(C) :: sp_vtk_fields
+    type(c_ptr) :: x = c_null_ptr, y = c_null_ptr, z = c_null_ptr
+    type(c_ptr) :: dx = c_null_ptr, dy = c_null_ptr, dz = c_null_ptr
+    type(c_ptr) :: p = c_null_ptr, u = c_null_ptr, v = c_null_ptr, w = c_null_ptr
+    type(c_ptr) :: phi_l = c_null_ptr, phi = c_null_ptr
+    type(c_ptr) :: vof_l = c_null_ptr, vof = c_null_ptr
+    integer(c_int32_t) :: halo_mask = 0
+  end type sp_vtk_fields
+
+  ! A solid body of SolidBody.f90:25-54: local mesh size, uniform spacings,
+  ! c_loc of this%x, y, z and this%phi, this%CMpoint, this%rotMatLtoG
+  type, bind(C) :: sp_solid_body
+    integer(c_int32_t) :: imax = 0, jmax = 0, kmax = 0
+    real(c_double) :: dx = 0, dy = 0, dz = 0
+    type(c_ptr) :: x = c_null_ptr, y = c_null_ptr, z = c_null_ptr
+    type(c_ptr) :: phi = c_null_ptr
+    real(c_double) :: cm(3) = 0
+    real(c_double) :: rot(3,3) = 0
+  end type sp_solid_body
+
+  ! One grid of interpolateLvS: c_loc of TGrid%x, y, z, dx, dy, dz and of
+  ! TCell%phi, vof, nx, ny, nz
+  type, bind(C) :: sp_lvs_grid
+    type(c_ptr) :: x = c_null_ptr, y = c_null_ptr, z = c_null_ptr
+    type(c_ptr) :: dx = c_null_ptr, dy = c_null_ptr, dz = c_null_ptr
+    type(c_ptr) :: phi = c_null_ptr, vof = c_null_ptr
+    type(c_ptr) :: nx = c_null_ptr, ny = c_null_ptr, nz = c_null_ptr
+  end type sp_lvs_grid
 
   ! One staggered grid of GridPreprocess (CutCell.f90:62-325): c_loc of
   ! TGrid%dx, dy, dz and of the TCell members (allocated as in AllocateVar,
@@ -269,6 +302,26 @@ module simco_proj
       type(sp_vtr_fields), intent(in) :: fields
       integer(c_int), value :: device
     end function sp_write_vtr
+
+    ! dir: NUL-terminated, "" for the working folder or ending with '/'
+    integer(c_int) function sp_write_vtk(ctx, dir, itt, fields, device) &
+        bind(C, name="sp_write_vtk")
+      import :: c_ptr, c_int, c_char, c_int64_t, sp_vtk_fields
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: dir(*)
+      integer(c_int64_t), value :: itt
+      type(sp_vtk_fields), intent(in) :: fields
+      integer(c_int), value :: device
+    end function sp_write_vtk
+
+    integer(c_int) function sp_interpolate_lvs(ctx, body, grid, device) &
+        bind(C, name="sp_interpolate_lvs")
+      import :: c_ptr, c_int, sp_solid_body, sp_lvs_grid
+      type(c_ptr), value :: ctx
+      type(sp_solid_body), intent(in) :: body
+      type(sp_lvs_grid), intent(in) :: grid
+      integer(c_int), value :: device
+    end function sp_interpolate_lvs
 
     integer(c_int) function sp_grid_preprocess(ctx, p, u, v, w) &
         bind(C, name="sp_grid_preprocess")

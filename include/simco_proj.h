@@ -256,6 +256,37 @@ typedef struct sp_vtr_fields {
 sp_status sp_write_vtr(sp_ctx* ctx, const char* dir, const char* prefix, int64_t itt,
                        const sp_vtr_fields* fields, int device);
 
+/* Fields of PrintResultVTK (PrintResult.f90:328-420).  x, y, z are the
+ * cell-centre coordinates TGrid%x(:,1,1), %y(1,:,1), %z(1,1,:) and dx, dy, dz
+ * the cell widths TGrid%dx(:,1,1), %dy(1,:,1), %dz(1,1,:) (lengths Imax, Jmax,
+ * Kmax); the eight fields are read on (1:Imax,1:Jmax,1:Kmax), each a halo
+ * array (0:Imax+1, 0:Jmax+1, 0:Kmax+1) or a cell array (1:Imax, 1:Jmax,
+ * 1:Kmax): bit q of halo_mask set marks field q (in the order p, u, v, w,
+ * phi_l, phi, vof_l, vof) as a halo array. */
+typedef struct sp_vtk_fields {
+  const double *x, *y, *z;
+  const double *dx, *dy, *dz;
+  const double *p, *u, *v, *w;   /* TVar%p, u, v, w */
+  const double *phi_l, *phi;     /* TCell%phiL, phi (LiquidLvs, FluidLvs) */
+  const double *vof_l, *vof;     /* TCell%vofL, vof (LiquidVof, FluidVof) */
+  uint32_t halo_mask;
+} sp_vtk_fields;
+
+/* Write <dir>PCell_Binary_3D<itt as i8.8>.vtr, the file PrintResultVTK writes
+ * through VTKFortran's binary XML writer, byte for byte: a RectilinearGrid
+ * with WholeExtent/Extent "+1 +Imax+1 +1 +Jmax+1 +1 +Kmax+1", Coordinates X,
+ * Y, Z at the cell faces (centre -/+ half width, PrintResult.f90:347-361) and
+ * CellData p, u, v, w, LiquidLvs, FluidLvs, LiquidVof, FluidVof, each array
+ * the base64 code of its int32 byte count followed by its float32 values
+ * (i fastest).  The base64 text is produced on the GPU.  itt outside
+ * [0, 99999999] gives "********" in the name, as the i8.8 edit descriptor
+ * does.  `dir` is "" (the working folder, as in the reference) or ends with
+ * '/'; missing folders are created.  `device`: 1 when the array pointers are
+ * device memory, 0 for host memory.  Grids of 2^29 cells or more are rejected
+ * (SP_ERR_INVALID_ARG): the byte count is a 32-bit integer. */
+sp_status sp_write_vtk(sp_ctx* ctx, const char* dir, int64_t itt, const sp_vtk_fields* fields,
+                       int device);
+
 /* One staggered cell grid (PGrid/PCell, UGrid/UCell, VGrid/VCell or
  * WGrid/WCell) as GridPreprocess (CutCell.f90:62-325) reads and writes it.
  * Cell arrays are (1:Imax,1:Jmax,1:Kmax); ee/ne/te_area are halo arrays;
@@ -374,6 +405,46 @@ typedef struct sp_liquid_field {
 sp_status sp_compute_uvw_liquid_field(sp_ctx* ctx, const sp_liquid_field* p,
                                       const sp_liquid_field* u, const sp_liquid_field* v,
                                       const sp_liquid_field* w, int device);
+
+/* A solid body (TSolidBody, SolidBody.f90:25-54): its uniform local mesh of
+ * imax x jmax x kmax cells (centres this%x, y, z; spacings this%dx, dy, dz),
+ * its level set this%phi (imax, jmax, kmax), the centre of mass
+ * this%CMpoint and the rotation this%rotMatLtoG (3x3, column-major as in
+ * Fortran). */
+typedef struct sp_solid_body {
+  int32_t imax, jmax, kmax;
+  double dx, dy, dz;
+  const double *x, *y, *z;
+  const double* phi;
+  double cm[3];
+  double rot[9];
+} sp_solid_body;
+
+/* One grid for interpolateLvS: TGrid%x, y, z (cell centres) and dx, dy, dz
+ * (cell arrays (1:Imax,1:Jmax,1:Kmax)); TCell%phi, vof (halo arrays,
+ * written over the whole array) and nx, ny, nz (halo arrays, written where
+ * the reference writes them and kept elsewhere). */
+typedef struct sp_lvs_grid {
+  const double *x, *y, *z;
+  const double *dx, *dy, *dz;
+  double *phi, *vof;
+  double *nx, *ny, *nz;
+} sp_lvs_grid;
+
+/* TSolidBody%interpolateLvS(grid, cell) (SolidBody.f90:646-742), the first
+ * step of the moving-solid geometry update (SolidBody.f90:810-813): phi =
+ * 1e4 and vof = 1 everywhere, then at every cell centre that falls inside
+ * the body mesh the trilinear interpolation of the body's level set; in the
+ * band |phi| <= 4 max(dx, dy, dz) the normal (Normal_Vector_Irre,
+ * Clsvof.f90:2361-2456, or clamped differences on the outer layer),
+ * normalised, and vof from Volume_Fraction_Calc.  Outside the band vof stays
+ * 1 (inside the body too) and the normals keep their values, as in the
+ * reference.  phi and the normals are bit-identical to the reference's
+ * loops; vof can differ by the last bits of pow (x**3.d0).  Call once per
+ * grid (P, U, V, W).  `device`: 1 when the pointers (body arrays included)
+ * are device memory, 0 for host memory. */
+sp_status sp_interpolate_lvs(sp_ctx* ctx, const sp_solid_body* body, const sp_lvs_grid* grid,
+                             int device);
 
 /* Inputs of one momentum predictor step, PredictingUVW(PGrid, UGrid, VGrid,
  * WGrid, PCell, UCell, VCell, WCell, PCellO, UCellO, VCellO, WCellO,

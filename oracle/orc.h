@@ -281,6 +281,32 @@ int orc_predict(const orc_grid* g, const double* pee, const double* pne, const d
                 const int32_t* bc_flag, const double* const* bcv, const orc_pred_params* p,
                 double* const pred[3], double* const dp[3]);
 
+/* ---- Interface transport and solid level set (orc_clsvof.c) ---- */
+/* A solid body of SolidBody.f90:25-54: its uniform local mesh (cell
+ * centres x, y, z, spacings dx, dy, dz), its level set phi (imax, jmax,
+ * kmax), the centre of mass and rotMatLtoG (column-major). */
+typedef struct orc_solid_body {
+  int32_t imax, jmax, kmax;
+  double dx, dy, dz;
+  const double *x, *y, *z;
+  const double* phi;
+  double cm[3];
+  double rot[9];
+} orc_solid_body;
+
+/* Normal_Vector_Irre (Clsvof.f90:2361-2456) at interior cell (i, j, k):
+ * phi a halo array, x, y, z cell arrays of the grid's cell centres. */
+void orc_normal_vector_irre(int I, int J, int K, const double* x, const double* y,
+                            const double* z, const double* phi, int i, int j, int k, double* nxx,
+                            double* nyy, double* nzz);
+/* interpolateLvS (SolidBody.f90:646-742) onto one grid: gx..gdz cell
+ * arrays (TGrid%x, y, z, dx, dy, dz), phi/vof written over the whole halo
+ * array, nx/ny/nz (halo) written in the band |phi| <= 4 max(d). */
+void orc_interpolate_lvs(const orc_solid_body* B, int I, int J, int K, const double* gx,
+                         const double* gy, const double* gz, const double* gdx, const double* gdy,
+                         const double* gdz, double* phi, double* vof, double* nx, double* ny,
+                         double* nz);
+
 #ifdef __cplusplus
 }
 #endif

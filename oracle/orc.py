@@ -649,3 +649,76 @@ def predict(grid, p_areas, old, cur, vel, vel_n, flux_div_old, bc_flag, bcv, par
     if rc != 0:
         raise ValueError("orc_predict: only SpaceOrder 1 is restated")
     return outs, dps, fdo
+
+
+# ---------------------------------------------------------------------------
+# Solid level set and interface transport (orc_clsvof.c)
+# ---------------------------------------------------------------------------
+
+class OrcSolidBody(ctypes.Structure):
+    _fields_ = [("imax", ctypes.c_int32), ("jmax", ctypes.c_int32), ("kmax", ctypes.c_int32),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+                ("x", ctypes.c_void_p), ("y", ctypes.c_void_p), ("z", ctypes.c_void_p),
+                ("phi", ctypes.c_void_p), ("cm", ctypes.c_double * 3),
+                ("rot", ctypes.c_double * 9)]
+
+
+def _lvs_lib():
+    L = lib()
+    if not getattr(L, "_lvs_ready", False):
+        L.orc_interpolate_lvs.argtypes = ([ctypes.POINTER(OrcSolidBody)] + [ctypes.c_int] * 3
+                                          + [F64P] * 11)
+        L.orc_interpolate_lvs.restype = None
+        L.orc_normal_vector_irre.argtypes = ([ctypes.c_int] * 3 + [F64P] * 4 + [ctypes.c_int] * 3
+                                             + [ctypes.POINTER(ctypes.c_double)] * 3)
+        L.orc_normal_vector_irre.restype = None
+        L._lvs_ready = True
+    return L
+
+
+def solid_body(body, keep):
+    """OrcSolidBody from a dict: x, y, z (local centres), dx, dy, dz, phi
+    (imax, jmax, kmax), cm (3), rot (3x3 rotMatLtoG)."""
+    arrs = {}
+    for n in ("x", "y", "z", "phi"):
+        a = np.asfortranarray(body[n], dtype=np.float64)
+        keep.append(a)
+        arrs[n] = a.ctypes.data
+    I, J, K = body["phi"].shape
+    rot = np.asarray(body["rot"], dtype=np.float64).reshape(3, 3).ravel(order="F")
+    return OrcSolidBody(I, J, K, float(body["dx"]), float(body["dy"]), float(body["dz"]),
+                        arrs["x"], arrs["y"], arrs["z"], arrs["phi"],
+                        (ctypes.c_double * 3)(*np.asarray(body["cm"], dtype=np.float64)),
+                        (ctypes.c_double * 9)(*rot))
+
+
+def interpolate_lvs(body, grid, nx=None, ny=None, nz=None):
+    """interpolateLvS of `body` onto one grid: grid maps x, y, z, dx, dy, dz
+    to cell arrays (I, J, K); nx, ny, nz are the normals before the call
+    (halo arrays, default zero).  Returns phi, vof, nx, ny, nz (halo)."""
+    L = _lvs_lib()
+    keep = []
+    I, J, K = grid["dx"].shape
+    g = [np.asfortranarray(grid[n], dtype=np.float64) for n in ("x", "y", "z", "dx", "dy", "dz")]
+    hs = (I + 2, J + 2, K + 2)
+    phi = np.zeros(hs, order="F")
+    vof = np.zeros(hs, order="F")
+    n = [np.zeros(hs, order="F") if a is None else np.array(a, dtype=np.float64, order="F",
+                                                             copy=True) for a in (nx, ny, nz)]
+    B = solid_body(body, keep)
+    L.orc_interpolate_lvs(ctypes.byref(B), I, J, K, *(_p(a) for a in g), _p(phi), _p(vof),
+                          *(_p(a) for a in n))
+    return phi, vof, n[0], n[1], n[2]
+
+
+def normal_vector_irre(grid, phi, i, j, k):
+    """Normal_Vector_Irre at interior cell (i, j, k) (1-based): grid x, y, z
+    cell arrays, phi a halo array."""
+    L = _lvs_lib()
+    I, J, K = grid["x"].shape
+    g = [np.asfortranarray(grid[n], dtype=np.float64) for n in ("x", "y", "z")]
+    ph = np.asfortranarray(phi, dtype=np.float64)
+    out = [ctypes.c_double() for _ in range(3)]
+    L.orc_normal_vector_irre(I, J, K, *(_p(a) for a in g), _p(ph), i, j, k,
+                             *(ctypes.byref(o) for o in out))
+    return tuple(o.value for o in out)

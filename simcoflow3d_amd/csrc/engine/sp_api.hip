@@ -777,6 +777,16 @@ sp_status sp_write_vtr(sp_ctx* c, const char* dir, const char* prefix, int64_t i
   return s;
 }
 
+sp_status sp_write_vtk(sp_ctx* c, const char* dir, int64_t itt, const sp_vtk_fields* f,
+                       int device) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  if (!dir || !f) return fail(c, SP_ERR_INVALID_ARG, "null VTK argument");
+  const sp_status s = write_vtk(c, dir, itt, *f, device != 0);
+  timers_collect(c);
+  return s;
+}
+
 static sp_status check_geom(sp_ctx* c, const sp_cell_geom* const g[4]) {
   for (int m = 0; m < 4; ++m) {
     if (!g[m]) return fail(c, SP_ERR_INVALID_ARG, "null sp_cell_geom");
@@ -985,6 +995,16 @@ sp_status sp_new_cell_face(sp_ctx* c, const sp_cell_geom* p, const sp_cell_geom*
   if (e != hipSuccess) return hip_check(c, e, "sp_new_cell_face");
   timers_collect(c);
   return SP_OK;
+}
+
+sp_status sp_interpolate_lvs(sp_ctx* c, const sp_solid_body* body, const sp_lvs_grid* grid,
+                             int device) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  if (!body || !grid) return fail(c, SP_ERR_INVALID_ARG, "null sp_solid_body or sp_lvs_grid");
+  const sp_status s = interpolate_lvs(c, *body, *grid, device != 0);
+  timers_collect(c);
+  return s;
 }
 
 sp_status sp_compute_uvw_liquid_field(sp_ctx* c, const sp_liquid_field* p,

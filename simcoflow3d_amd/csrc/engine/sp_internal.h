@@ -261,6 +261,8 @@ sp_status new_cell_face(sp_ctx* c, const sp_cell_geom* const g[4],
                         const sp_cell_interp* const x[3], double* scratch);
 // ComputeUVWLiquidField (sp_liquid.hip) on device arrays P, U, V, W.
 sp_status uvw_liquid_field(sp_ctx* c, const sp_liquid_field* const g[4]);
+// interpolateLvS (sp_clsvof.hip) of a solid body onto one grid
+sp_status interpolate_lvs(sp_ctx* c, const sp_solid_body& b, const sp_lvs_grid& g, bool device);
 // Momentum predictor (sp_predict.hip) on device arrays; faces holds
 // predict_face_doubles(D) doubles.
 sp_status predict_uvw(sp_ctx* c, const sp_pred_inputs& in, const sp_pred_outputs& out,
@@ -269,6 +271,8 @@ size_t predict_face_doubles(const Dims& D);
 // ---- VTR output (sp_vtr.hip) ----
 sp_status write_vtr(sp_ctx* c, const char* dir, const char* prefix, int64_t itt,
                     const sp_vtr_fields& f, bool device);
+sp_status write_vtk(sp_ctx* c, const char* dir, int64_t itt, const sp_vtk_fields& f,
+                    bool device);
 
 // ---- adjoint (sp_adjoint.hip) ----
 sp_status adjoint(sp_ctx* c, const sp_adj_inputs& in, sp_adj_outputs& out, sp_stats* st);

@@ -26,6 +26,7 @@
 
 #include "sp_device.h"
 #include "sp_internal.h"
+#include "sp_vof.h"
 
 namespace sp {
 namespace {
@@ -36,50 +37,6 @@ struct LiquidArgs {
   const double *vd[3][3];  // U/V/W spacing dx, dy, dz
   double *out[3][5];       // U/V/W phiL, nxL, nyL, nzL, vofL
 };
-
-__device__ inline double corner_dist(double dx, double dy, double dz, double nx, double ny,
-                                     double nz, double phi) {
-#pragma clang fp contract(off)
-  return phi + 0.5 * (fabs(nx) * dx + fabs(ny) * dy + fabs(nz) * dz);
-}
-
-__device__ double volume_fraction(double dx, double dy, double dz, double nxx, double nyy,
-                                  double nzz, double s) {
-#pragma clang fp contract(off)
-  const double nx1 = fabs(nxx), ny1 = fabs(nyy), nz1 = fabs(nzz);
-  const double a = nx1 * dx, b = ny1 * dy, c = nz1 * dz;
-  double dx1 = a, dz1 = a;
-  if (b > dx1) dx1 = b;
-  if (c > dx1) dx1 = c;
-  if (b < dz1) dz1 = b;
-  if (c < dz1) dz1 = c;
-  const double dy1 = (a + b + c) - dx1 - dz1;
-  const double sm = dx1 + dy1 + dz1;
-  const double sc = s < sm - s ? s : sm - s;
-  if (s <= 0.0) return 0.0;
-  if (s >= sm) return 1.0 * dx * dy * dz;
-  double fc;
-  if (sc <= dz1) {
-    fc = pow(sc, 3.0) / (6.0 * dx1 * dy1 * dz1);
-  } else if (sc <= dy1) {
-    fc = (sc * sc - dz1 * sc + dz1 * dz1 / 3.0) / (2.0 * dx1 * dy1);
-  } else if (dx1 >= dy1 + dz1) {
-    if (sc >= dy1 + dz1) {
-      fc = (2.0 * sc - dy1 - dz1) / (2.0 * dx1);
-    } else {
-      fc = (pow(sc, 3.0) - pow(sc - dz1, 3.0) - pow(sc - dy1, 3.0));
-      fc = fc / (6.0 * dx1 * dy1 * dz1);
-    }
-  } else {
-    if (sc > dx1)
-      fc = (pow(sc, 3.0) - pow(sc - dz1, 3.0) - pow(sc - dy1, 3.0) - pow(sc - dx1, 3.0));
-    else
-      fc = (pow(sc, 3.0) - pow(sc - dz1, 3.0) - pow(sc - dy1, 3.0));
-    fc = fc / (6.0 * dx1 * dy1 * dz1);
-  }
-  if (s <= 0.5 * sm) return fc * dx * dy * dz;
-  return (1.0 - fc) * dx * dy * dz;
-}
 
 // liquid volume of a box (d1, d2, d3) whose centre level set is phin
 __device__ inline double liquid_part(double d1, double d2, double d3, double nx, double ny,

@@ -299,4 +299,216 @@ sp_status write_vtr(sp_ctx* c, const char* dir, const char* prefix, int64_t itt,
   return s;
 }
 
+// ---------------------------------------------------------------------------
+// PrintResultVTK (PrintResult.f90:328-420): VTKFortran's binary XML
+// RectilinearGrid.  Each DataArray's content is the base64 code of the byte
+// stream [int32 4 n, float32 value 0 .. n-1]; thread g encodes stream bytes
+// 3g .. 3g+2 into characters 4g .. 4g+3, reading the (at most two) values
+// those bytes come from, so the text of every array is made in one pass with
+// no scan.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int64_t kGroups = int64_t(1) << 22;  // base64 groups per launch (16 MiB of text)
+
+__constant__ char kB64[64] = {
+    'A', 'B', 'C', 'D', 'E', 'F', 'G', 'H', 'I', 'J', 'K', 'L', 'M', 'N', 'O', 'P',
+    'Q', 'R', 'S', 'T', 'U', 'V', 'W', 'X', 'Y', 'Z', 'a', 'b', 'c', 'd', 'e', 'f',
+    'g', 'h', 'i', 'j', 'k', 'l', 'm', 'n', 'o', 'p', 'q', 'r', 's', 't', 'u', 'v',
+    'w', 'x', 'y', 'z', '0', '1', '2', '3', '4', '5', '6', '7', '8', '9', '+', '/'};
+
+struct VtkBlock {
+  const double* src;    // cell field, or cell centres for a coordinate array
+  const double* width;  // cell widths for a coordinate array, else null
+  int halo;             // src is a (0:I+1, ...) halo array
+  int I, J, K;
+  int64_t n;            // float32 values in the array
+  int64_t g0, ng;       // this launch: groups [g0, g0 + ng)
+  char* out;            // text of group g0
+};
+
+// value m of the array, rounded to float32 as real(., kind=R4P) does
+__device__ inline float vtk_value(const VtkBlock& B, int64_t m) {
+  if (B.width) {
+    // faces: x(i) = xc(i) - dx(i)/2, x(n) = xc(n-1) + dx(n-1)/2 (0-based)
+    const int64_t last = B.n - 1;
+    const double x = m < last ? B.src[m] - B.width[m] / 2.0
+                              : B.src[last - 1] + B.width[last - 1] / 2.0;
+    return (float)x;
+  }
+  const int64_t IJ = (int64_t)B.I * B.J;
+  const int k = (int)(m / IJ);
+  const int64_t rem = m - (int64_t)k * IJ;
+  const int j = (int)(rem / B.I), i = (int)(rem - (int64_t)j * B.I);
+  const int64_t idx =
+      B.halo ? (i + 1) + (int64_t)(B.I + 2) * ((j + 1) + (int64_t)(B.J + 2) * (k + 1)) : m;
+  return (float)B.src[idx];
+}
+
+// 32-bit word w of the stream (little endian): the byte count, then the values
+__device__ inline uint32_t vtk_word(const VtkBlock& B, int64_t w) {
+  if (w == 0) return (uint32_t)(4 * B.n);
+  if (w > B.n) return 0u;
+  return __float_as_uint(vtk_value(B, w - 1));
+}
+
+__global__ void __launch_bounds__(256) k_vtk_base64(VtkBlock B) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= B.ng) return;
+  const int64_t L = 4 * (B.n + 1);  // stream bytes
+  const int64_t b0 = 3 * (B.g0 + r);
+  const int64_t w0 = b0 >> 2;
+  const int s = (int)(b0 & 3);
+  // bytes s .. s+2 of words w0, w0+1 (the second is needed only when s >= 2)
+  uint64_t pair = vtk_word(B, w0);
+  if (s >= 2) pair |= (uint64_t)vtk_word(B, w0 + 1) << 32;
+  const uint32_t n1 = b0 + 1 < L, n2 = b0 + 2 < L;  // bytes 2 and 3 of the group exist
+  const uint32_t c0 = (uint32_t)(pair >> (8 * s)) & 255u;
+  const uint32_t c1 = n1 ? (uint32_t)(pair >> (8 * s + 8)) & 255u : 0u;
+  const uint32_t c2 = n2 ? (uint32_t)(pair >> (8 * s + 16)) & 255u : 0u;
+  const uint32_t v = c0 << 16 | c1 << 8 | c2;
+  const uint32_t a0 = (uint8_t)kB64[v >> 18];
+  const uint32_t a1 = (uint8_t)kB64[(v >> 12) & 63];
+  const uint32_t a2 = n1 ? (uint8_t)kB64[(v >> 6) & 63] : (uint32_t)'=';
+  const uint32_t a3 = n2 ? (uint8_t)kB64[v & 63] : (uint32_t)'=';
+  reinterpret_cast<uint32_t*>(B.out)[r] = a0 | a1 << 8 | a2 << 16 | a3 << 24;
+}
+
+// PENF's str for an integer: '+' before a non-negative value
+std::string penf_str(int64_t n) { return (n >= 0 ? "+" : "") + std::to_string(n); }
+
+}  // namespace
+
+sp_status write_vtk(sp_ctx* c, const char* dir, int64_t itt, const sp_vtk_fields& f,
+                    bool device) {
+  const Dims& D = c->dm;
+  const int I = D.I, J = D.J, K = D.K;
+  const int64_t N = D.N, Nh = D.Nh;
+  if (4 * (N + 1) > (int64_t)INT32_MAX)
+    return fail(c, SP_ERR_INVALID_ARG, "VTK binary arrays carry a 32-bit byte count: grid too large");
+  const double* cell[8] = {f.p, f.u, f.v, f.w, f.phi_l, f.phi, f.vof_l, f.vof};
+  const double* coord[6] = {f.x, f.y, f.z, f.dx, f.dy, f.dz};
+  for (const double* p : cell)
+    if (!p) return fail(c, SP_ERR_INVALID_ARG, "null VTK field");
+  for (const double* p : coord)
+    if (!p) return fail(c, SP_ERR_INVALID_ARG, "null VTK coordinate or width");
+  int halo[8];
+  for (int q = 0; q < 8; ++q) halo[q] = (f.halo_mask >> q) & 1u;
+
+  // 'PCell_Binary_3D'//curd//'.vtr', curd = itt in i8.8
+  char curd[16];
+  if (itt >= 0 && itt <= 99999999) std::snprintf(curd, sizeof(curd), "%08lld", (long long)itt);
+  else std::snprintf(curd, sizeof(curd), "********");
+  const std::string sdir(dir);
+  if (!sdir.empty() && !make_dirs(sdir)) return fail(c, SP_ERR_INVALID_ARG, "cannot create " + sdir);
+  const std::string path = sdir + "PCell_Binary_3D" + curd + ".vtr";
+
+  // device copies of host arrays
+  const double* dcell[8];
+  const double* dcoord[6];
+  std::vector<void*> tmp;
+  auto release = [&]() { for (void* p : tmp) hipFree(p); tmp.clear(); };
+  if (device) {
+    for (int q = 0; q < 8; ++q) dcell[q] = cell[q];
+    for (int q = 0; q < 6; ++q) dcoord[q] = coord[q];
+  } else {
+    for (int q = 0; q < 14; ++q) {
+      const double* src = q < 8 ? cell[q] : coord[q - 8];
+      const int axis = (q - 8) % 3;
+      const int64_t n = q < 8 ? (halo[q] ? Nh : N) : (axis == 0 ? I : (axis == 1 ? J : K));
+      void* d = nullptr;
+      hipError_t e = hipMalloc(&d, sizeof(double) * (size_t)n);
+      if (e != hipSuccess) { release(); return hip_check(c, e, "hipMalloc(VTK staging)"); }
+      tmp.push_back(d);
+      e = hipMemcpyAsync(d, src, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream);
+      if (e != hipSuccess) { release(); return hip_check(c, e, "hipMemcpyAsync(VTK)"); }
+      if (q < 8) dcell[q] = static_cast<const double*>(d);
+      else dcoord[q - 8] = static_cast<const double*>(d);
+    }
+  }
+
+  char* dtext = nullptr;
+  char* htext = nullptr;
+  const size_t cap = (size_t)(4 * kGroups);
+  hipError_t e = hipMalloc(&dtext, cap);
+  if (e == hipSuccess) e = hipHostMalloc(&htext, cap, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    if (dtext) hipFree(dtext);
+    release();
+    return hip_check(c, e, "VTK buffers");
+  }
+  FILE* fh = std::fopen(path.c_str(), "wb");
+  sp_status s = SP_OK;
+  if (!fh) s = fail(c, SP_ERR_INVALID_ARG, "cannot open " + path);
+  bool ok = true;
+  auto put = [&](const std::string& t) {
+    if (ok && std::fwrite(t.data(), 1, t.size(), fh) != t.size()) ok = false;
+  };
+  // one DataArray tag: start tag, content line, end tag (FoXy's layout)
+  auto data_array = [&](const std::string& name, const VtkBlock& b0) {
+    put("        <DataArray type=\"Float32\" NumberOfComponents=\"1\" Name=\"" + name +
+        "\" format=\"binary\">\n          ");
+    const int64_t G = (4 * (b0.n + 1) + 2) / 3;
+    for (int64_t g0 = 0; ok && !s && g0 < G; g0 += kGroups) {
+      VtkBlock b = b0;
+      b.g0 = g0;
+      b.ng = G - g0 < kGroups ? G - g0 : kGroups;
+      b.out = dtext;
+      int slot;
+      // algorithmic bytes: 3/4 of a source double (two for a coordinate) read
+      // and 4 characters written per group
+      timer_begin(c, "vtk_base64", (double)b.ng * ((b.width ? 12.0 : 6.0) + 4.0), &slot);
+      hipLaunchKernelGGL(k_vtk_base64, dim3((unsigned)((b.ng + 255) / 256)), dim3(256), 0,
+                         c->stream, b);
+      timer_end(c, slot);
+      hipError_t e2 = hipGetLastError();
+      if (e2 == hipSuccess)
+        e2 = hipMemcpyAsync(htext, dtext, (size_t)(4 * b.ng), hipMemcpyDeviceToHost, c->stream);
+      if (e2 == hipSuccess) e2 = hipStreamSynchronize(c->stream);
+      if (e2 != hipSuccess) { s = hip_check(c, e2, "VTK base64"); break; }
+      if (std::fwrite(htext, 1, (size_t)(4 * b.ng), fh) != (size_t)(4 * b.ng)) ok = false;
+    }
+    put("\n        </DataArray>\n");
+  };
+  if (!s) {
+    const std::string ext = penf_str(1) + " " + penf_str(I + 1) + " " + penf_str(1) + " " +
+                            penf_str(J + 1) + " " + penf_str(1) + " " + penf_str(K + 1);
+    put("<?xml version=\"1.0\"?>\n"
+        "<VTKFile type=\"RectilinearGrid\" version=\"1.0\" byte_order=\"LittleEndian\">\n");
+    put("  <RectilinearGrid WholeExtent=\"" + ext + "\">\n");
+    put("    <Piece Extent=\"" + ext + "\">\n");
+    put("      <Coordinates>\n");
+    const char* cname[3] = {"X", "Y", "Z"};
+    const int cn[3] = {I, J, K};
+    for (int q = 0; q < 3 && ok && !s; ++q) {
+      VtkBlock b{};
+      b.src = dcoord[q];
+      b.width = dcoord[3 + q];
+      b.n = cn[q] + 1;
+      data_array(cname[q], b);
+    }
+    put("      </Coordinates>\n");
+    put("      <CellData>\n");
+    const char* name[8] = {"p", "u", "v", "w", "LiquidLvs", "FluidLvs", "LiquidVof", "FluidVof"};
+    for (int q = 0; q < 8 && ok && !s; ++q) {
+      VtkBlock b{};
+      b.src = dcell[q];
+      b.halo = halo[q];
+      b.I = I; b.J = J; b.K = K;
+      b.n = N;
+      data_array(name[q], b);
+    }
+    put("      </CellData>\n");
+    put("    </Piece>\n");
+    put("  </RectilinearGrid>\n");
+    put("</VTKFile>\n");
+    if (!ok && !s) s = fail(c, SP_ERR_INVALID_ARG, "write failed: " + path);
+  }
+  if (fh && std::fclose(fh) != 0 && !s) s = fail(c, SP_ERR_INVALID_ARG, "close failed: " + path);
+  hipFree(dtext);
+  hipHostFree(htext);
+  release();
+  return s;
+}
+
 }  // namespace sp

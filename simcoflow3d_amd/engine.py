@@ -35,9 +35,9 @@ EXPORTED_SYMBOLS = (
     "sp_set_grid_1d", "sp_project", "sp_project_dev", "sp_adjoint", "sp_adjoint_dev",
     "sp_stream", "sp_kernel_times", "sp_set_profiling", "sp_set_profiling_filter",
     "sp_strerror", "sp_last_error", "sp_wall_step", "sp_wall_step_dev", "sp_write_vtr",
-    "sp_grid_preprocess", "sp_grid_preprocess_dev", "sp_define_momentum_exchange_cell",
+    "sp_write_vtk", "sp_grid_preprocess", "sp_grid_preprocess_dev", "sp_define_momentum_exchange_cell",
     "sp_number_external_cell", "sp_new_cell_face", "sp_compute_uvw_liquid_field",
-    "sp_number_external_cell_noalloc", "sp_predict_uvw",
+    "sp_number_external_cell_noalloc", "sp_predict_uvw", "sp_interpolate_lvs",
 )
 
 # members of sp_liquid_field, in order
@@ -108,6 +108,29 @@ VTR_FIELDS = ("u", "v", "w", "p", "phi", "phi_l", "vof", "vof_l", "mres")
 class SpVtrFields(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in ("x", "y", "z") + VTR_FIELDS] + [
         ("halo_mask", ctypes.c_uint32)]
+
+
+VTK_FIELDS = ("p", "u", "v", "w", "phi_l", "phi", "vof_l", "vof")
+
+
+class SpVtkFields(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("x", "y", "z", "dx", "dy", "dz") + VTK_FIELDS] + [
+        ("halo_mask", ctypes.c_uint32)]
+
+
+class SpSolidBody(ctypes.Structure):
+    _fields_ = [("imax", ctypes.c_int32), ("jmax", ctypes.c_int32), ("kmax", ctypes.c_int32),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+                ("x", ctypes.c_void_p), ("y", ctypes.c_void_p), ("z", ctypes.c_void_p),
+                ("phi", ctypes.c_void_p), ("cm", ctypes.c_double * 3),
+                ("rot", ctypes.c_double * 9)]
+
+
+LVS_GRID_FIELDS = ("x", "y", "z", "dx", "dy", "dz", "phi", "vof", "nx", "ny", "nz")
+
+
+class SpLvsGrid(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in LVS_GRID_FIELDS]
 
 
 class SpInputs(ctypes.Structure):
@@ -232,6 +255,10 @@ def load():
     L.sp_write_vtr.argtypes = [V, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64,
                                P(SpVtrFields), ctypes.c_int]
     L.sp_write_vtr.restype = ctypes.c_int
+    L.sp_write_vtk.argtypes = [V, ctypes.c_char_p, ctypes.c_int64, P(SpVtkFields), ctypes.c_int]
+    L.sp_write_vtk.restype = ctypes.c_int
+    L.sp_interpolate_lvs.argtypes = [V, P(SpSolidBody), P(SpLvsGrid), ctypes.c_int]
+    L.sp_interpolate_lvs.restype = ctypes.c_int
     for name in ("sp_grid_preprocess", "sp_grid_preprocess_dev"):
         f = getattr(L, name)
         f.argtypes = [V] + [P(SpCellGeom)] * 4
@@ -675,6 +702,68 @@ class Projector:
                                    1 if device else 0)
         self._check(rc, "sp_write_vtr")
         return os.path.join(d, "Paraview", f"{prefix}_{int(itt):08d}.vtr")
+
+    def write_vtk(self, directory, itt, fields, device=False):
+        """PrintResultVTK's file (sp_write_vtk): fields maps x, y, z (cell
+        centres) and dx, dy, dz (cell widths), 1-D, and p, u, v, w, phi_l,
+        phi, vof_l, vof (halo or cell arrays); numpy arrays, or torch device
+        tensors with device=True.  directory "" writes into the working
+        folder.  Returns the path of the file."""
+        I, J, K = self.shape
+        f = SpVtkFields()
+        keep = []
+        mask = 0
+        for n in ("x", "y", "z", "dx", "dy", "dz") + VTK_FIELDS:
+            a = fields[n]
+            if isinstance(a, np.ndarray):
+                a = np.asfortranarray(a, dtype=np.float64)
+                keep.append(a)
+            setattr(f, n, _addr(a))
+            if n in VTK_FIELDS:
+                shape = tuple(a.shape) if isinstance(a, np.ndarray) else tuple(a.shape)[::-1]
+                if shape == (I + 2, J + 2, K + 2):
+                    mask |= 1 << VTK_FIELDS.index(n)
+                elif shape != (I, J, K):
+                    raise ValueError(f"{n}: shape {shape} is neither a halo nor a cell array")
+        f.halo_mask = mask
+        d = directory if (not directory or directory.endswith("/")) else directory + "/"
+        rc = self.lib.sp_write_vtk(self.h, d.encode(), int(itt), ctypes.byref(f),
+                                   1 if device else 0)
+        self._check(rc, "sp_write_vtk")
+        itt = int(itt)
+        curd = f"{itt:08d}" if 0 <= itt <= 99999999 else "*" * 8
+        return d + f"PCell_Binary_3D{curd}.vtr"
+
+    def interpolate_lvs(self, body, grid, device=False):
+        """interpolateLvS of a solid body onto one grid (sp_interpolate_lvs).
+        body maps x, y, z (local centres), dx, dy, dz, phi (imax, jmax, kmax),
+        cm (3) and rot (3x3 rotMatLtoG); grid maps x, y, z, dx, dy, dz (cell
+        arrays) and phi, vof, nx, ny, nz (halo arrays, updated in place).
+        numpy arrays, or torch device tensors with device=True (body arrays
+        included)."""
+        keep = []
+
+        def addr(a, out=False):
+            if isinstance(a, np.ndarray):
+                if out:
+                    if not (a.flags.f_contiguous and a.dtype == np.float64):
+                        raise ValueError("output arrays must be Fortran-ordered float64")
+                else:
+                    a = np.asfortranarray(a, dtype=np.float64)
+                    keep.append(a)
+            return _addr(a)
+        phi = body["phi"]
+        bshape = tuple(phi.shape) if isinstance(phi, np.ndarray) else tuple(phi.shape)[::-1]
+        rot = np.asarray(body["rot"], dtype=np.float64).reshape(3, 3).ravel(order="F")
+        b = SpSolidBody(*bshape, float(body["dx"]), float(body["dy"]), float(body["dz"]),
+                        addr(body["x"]), addr(body["y"]), addr(body["z"]), addr(phi),
+                        (ctypes.c_double * 3)(*np.asarray(body["cm"], dtype=np.float64)),
+                        (ctypes.c_double * 9)(*rot))
+        g = SpLvsGrid(*(addr(grid[n], out=n in ("phi", "vof", "nx", "ny", "nz"))
+                        for n in LVS_GRID_FIELDS))
+        rc = self.lib.sp_interpolate_lvs(self.h, ctypes.byref(b), ctypes.byref(g),
+                                         1 if device else 0)
+        self._check(rc, "sp_interpolate_lvs")
 
     def adjoint(self, gu, gv, gw, beta_grad=False):
         I, J, K = self.shape

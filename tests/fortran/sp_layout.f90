@@ -12,6 +12,9 @@ program sp_layout
   type(sp_adj_outputs), target :: ao
   type(sp_wall_bc), target :: wb
   type(sp_vtr_fields), target :: vf
+  type(sp_vtk_fields), target :: kf
+  type(sp_solid_body), target :: sb
+  type(sp_lvs_grid), target :: lg
   type(sp_cell_geom), target :: cg
   type(sp_cell_interp), target :: ci
   type(sp_liquid_field), target :: lf
@@ -25,6 +28,9 @@ program sp_layout
   print '(a,1x,i0)', 'sp_adj_outputs', c_sizeof(ao)
   print '(a,1x,i0)', 'sp_wall_bc', c_sizeof(wb)
   print '(a,1x,i0)', 'sp_vtr_fields', c_sizeof(vf)
+  print '(a,1x,i0)', 'sp_vtk_fields', c_sizeof(kf)
+  print '(a,1x,i0)', 'sp_solid_body', c_sizeof(sb)
+  print '(a,1x,i0)', 'sp_lvs_grid', c_sizeof(lg)
   print '(a,1x,i0)', 'sp_cell_geom', c_sizeof(cg)
   print '(a,1x,i0)', 'sp_cell_interp', c_sizeof(ci)
   print '(a,1x,i0)', 'sp_liquid_field', c_sizeof(lf)
@@ -44,6 +50,14 @@ program sp_layout
   print '(a,1x,i0)', 'sp_cell_geom.delh', off(c_loc(cg%delh), c_loc(cg))
   print '(a,1x,i0)', 'sp_vtr_fields.mres', off(c_loc(vf%mres), c_loc(vf))
   print '(a,1x,i0)', 'sp_vtr_fields.halo_mask', off(c_loc(vf%halo_mask), c_loc(vf))
+  print '(a,1x,i0)', 'sp_vtk_fields.dz', off(c_loc(kf%dz), c_loc(kf))
+  print '(a,1x,i0)', 'sp_vtk_fields.vof', off(c_loc(kf%vof), c_loc(kf))
+  print '(a,1x,i0)', 'sp_vtk_fields.halo_mask', off(c_loc(kf%halo_mask), c_loc(kf))
+  print '(a,1x,i0)', 'sp_solid_body.dx', off(c_loc(sb%dx), c_loc(sb))
+  print '(a,1x,i0)', 'sp_solid_body.phi', off(c_loc(sb%phi), c_loc(sb))
+  print '(a,1x,i0)', 'sp_solid_body.cm', off(c_loc(sb%cm), c_loc(sb))
+  print '(a,1x,i0)', 'sp_solid_body.rot', off(c_loc(sb%rot), c_loc(sb))
+  print '(a,1x,i0)', 'sp_lvs_grid.nz', off(c_loc(lg%nz), c_loc(lg))
   print '(a,1x,i0)', 'sp_wall_bc.const_u', off(c_loc(wb%const_u), c_loc(wb))
   print '(a,1x,i0)', 'sp_wall_bc.const_p', off(c_loc(wb%const_p), c_loc(wb))
   print '(a,1x,i0)', 'sp_inputs.wall', off(c_loc(inp%wall), c_loc(inp))
