@@ -307,6 +307,40 @@ void orc_interpolate_lvs(const orc_solid_body* B, int I, int J, int K, const dou
                          const double* gdz, double* phi, double* vof, double* nx, double* ny,
                          double* nz);
 
+/* The inputs of Clsvof_Scheme (Clsvof.f90:355-798): TVar%u, v, w (halo),
+ * PGrid%x, y, z, dx, dy, dz (cell arrays), PCell%nx, ny, nz (halo, the solid
+ * normals used for the level-set ghosts), BCu%VarW (J,K), BCv%VarS (I,K),
+ * BCw%VarB (I,J), BCLvs and BCVof %flag and %const (W,E,S,N,B,T), dt. */
+typedef struct orc_clsvof_in {
+  const double *u, *v, *w;
+  const double *x, *y, *z, *dx, *dy, *dz;
+  const double *nx, *ny, *nz;
+  const double *bcu_w, *bcv_s, *bcw_b;
+  int32_t lvs_flag[6];
+  double lvs_const[6];
+  int32_t vof_flag[6];
+  double vof_const[6];
+  double dt;
+} orc_clsvof_in;
+/* The state it updates: PCell%vofL, phiL, nxL, nyL, nzL (halo) and the face
+ * values BCLvs%VarW..VarT, BCVof%VarW..VarT (null: not kept). */
+typedef struct orc_clsvof_state {
+  double *vof, *phi;
+  double *nx, *ny, *nz;
+  double* lvs_var[6];
+  double* vof_var[6];
+} orc_clsvof_state;
+/* Find_Distance (Clsvof.f90:1311-1408). */
+double orc_find_distance(double dx, double dy, double dz, double nxx, double nyy, double nzz,
+                         double f);
+/* One candidate distance of Redistance_Levelset for the cell at offset
+ * (ii, jj, kk) from an interface cell (plane n, diss; size d); phit is the
+ * target's level set. */
+double orc_redist_candidate(double nxx, double nyy, double nzz, double diss, double dx, double dy,
+                            double dz, int ii, int jj, int kk, double phit);
+/* Clsvof_Scheme; returns -1 when a dimension is below 3. */
+int orc_clsvof_scheme(int I, int J, int K, const orc_clsvof_in* in, orc_clsvof_state* st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -722,3 +722,83 @@ def normal_vector_irre(grid, phi, i, j, k):
     L.orc_normal_vector_irre(I, J, K, *(_p(a) for a in g), _p(ph), i, j, k,
                              *(ctypes.byref(o) for o in out))
     return tuple(o.value for o in out)
+
+
+CLSVOF_IN_PTRS = ("u", "v", "w", "x", "y", "z", "dx", "dy", "dz", "nx", "ny", "nz", "bcu_w",
+                  "bcv_s", "bcw_b")
+
+
+class OrcClsvofIn(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in CLSVOF_IN_PTRS] + [
+        ("lvs_flag", ctypes.c_int32 * 6), ("lvs_const", ctypes.c_double * 6),
+        ("vof_flag", ctypes.c_int32 * 6), ("vof_const", ctypes.c_double * 6),
+        ("dt", ctypes.c_double)]
+
+
+class OrcClsvofState(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("vof", "phi", "nx", "ny", "nz")] + [
+        ("lvs_var", ctypes.c_void_p * 6), ("vof_var", ctypes.c_void_p * 6)]
+
+
+def _clsvof_lib():
+    L = _lvs_lib()
+    if not getattr(L, "_clsvof_ready", False):
+        L.orc_clsvof_scheme.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(OrcClsvofIn),
+                                                             ctypes.POINTER(OrcClsvofState)]
+        L.orc_clsvof_scheme.restype = ctypes.c_int
+        L.orc_find_distance.argtypes = [ctypes.c_double] * 7
+        L.orc_find_distance.restype = ctypes.c_double
+        L.orc_redist_candidate.argtypes = [ctypes.c_double] * 7 + [ctypes.c_int] * 3 + [
+            ctypes.c_double]
+        L.orc_redist_candidate.restype = ctypes.c_double
+        L._clsvof_ready = True
+    return L
+
+
+def find_distance(dx, dy, dz, nx, ny, nz, f):
+    return _clsvof_lib().orc_find_distance(dx, dy, dz, nx, ny, nz, f)
+
+
+def redist_candidate(nx, ny, nz, dis, dx, dy, dz, ii, jj, kk, phit):
+    return _clsvof_lib().orc_redist_candidate(nx, ny, nz, dis, dx, dy, dz, ii, jj, kk, phit)
+
+
+def clsvof_face_shapes(shape):
+    """Shapes of the W, E, S, N, B, T face arrays: (J,K), (J,K), (I,K), (I,K),
+    (I,J), (I,J)."""
+    I, J, K = shape
+    return [(J, K), (J, K), (I, K), (I, K), (I, J), (I, J)]
+
+
+def clsvof_scheme(inp, state):
+    """Clsvof_Scheme on copies.  inp maps u, v, w, nx, ny, nz (halo), x, y,
+    z, dx, dy, dz (cell), bcu_w, bcv_s, bcw_b (face arrays), lvs_flag,
+    lvs_const, vof_flag, vof_const (6 each) and dt; state maps vof_l, phi_l,
+    nx_l, ny_l, nz_l (halo) and lvs_var, vof_var (six face arrays each).
+    Returns the updated state."""
+    L = _clsvof_lib()
+    I, J, K = inp["dx"].shape
+    keep = []
+    ptr = {}
+    for n in CLSVOF_IN_PTRS:
+        a = np.asfortranarray(inp[n], dtype=np.float64)
+        keep.append(a)
+        ptr[n] = a.ctypes.data
+    ci = OrcClsvofIn(*(ptr[n] for n in CLSVOF_IN_PTRS),
+                     (ctypes.c_int32 * 6)(*inp["lvs_flag"]), (ctypes.c_double * 6)(*inp["lvs_const"]),
+                     (ctypes.c_int32 * 6)(*inp["vof_flag"]), (ctypes.c_double * 6)(*inp["vof_const"]),
+                     float(inp["dt"]))
+    out = {n: np.array(state[n], dtype=np.float64, order="F", copy=True)
+           for n in ("vof_l", "phi_l", "nx_l", "ny_l", "nz_l")}
+    shapes = clsvof_face_shapes((I, J, K))
+    out["lvs_var"] = [np.array(a, dtype=np.float64, order="F", copy=True).reshape(s, order="F")
+                      for a, s in zip(state["lvs_var"], shapes)]
+    out["vof_var"] = [np.array(a, dtype=np.float64, order="F", copy=True).reshape(s, order="F")
+                      for a, s in zip(state["vof_var"], shapes)]
+    cs = OrcClsvofState(*(out[n].ctypes.data for n in ("vof_l", "phi_l", "nx_l", "ny_l", "nz_l")),
+                        (ctypes.c_void_p * 6)(*(a.ctypes.data for a in out["lvs_var"])),
+                        (ctypes.c_void_p * 6)(*(a.ctypes.data for a in out["vof_var"])))
+    rc = L.orc_clsvof_scheme(I, J, K, ctypes.byref(ci), ctypes.byref(cs))
+    if rc != 0:
+        raise ValueError(f"orc_clsvof_scheme: {rc}")
+    return out
