@@ -69,6 +69,11 @@ def parse():
                          "size, with its CPU baseline")
     ap.add_argument("--predictor-only", action="store_true",
                     help="time only the momentum predictor and print its record")
+    ap.add_argument("--clsvof", action="store_true",
+                    help="also time the interface transport (sp_clsvof_scheme) at the same grid "
+                         "size, with its CPU baseline")
+    ap.add_argument("--clsvof-only", action="store_true",
+                    help="time only the interface transport and print its record")
     ap.add_argument("--geometry-only", action="store_true",
                     help="time only GridPreprocess (no projection) and print its record")
     ap.add_argument("--copy-probe", action="store_true",
@@ -487,6 +492,130 @@ def predictor(shape, steps, warmup, cpu=True):
     return rec
 
 
+def clsvof_fields(shape):
+    """A sphere of liquid (radius 0.15 at (0.35, 0.35, 0.35) in the unit
+    cube; volume fractions from 2^3 sub-samples, level set its signed
+    distance) in the deformation field of the reference's AdvectionTest
+    (Test/AdvectionTest/AdvectionTest.f90:186-196) at t = 0, with Neumann
+    level-set and vof boundary conditions.  numpy arrays in the layouts of
+    sp_clsvof_scheme."""
+    import numpy as np
+    I, J, K = shape
+    e = [np.linspace(0.0, 1.0, n + 1) for n in shape]
+    c = [0.5 * (q[1:] + q[:-1]) for q in e]
+    d = [np.diff(q) for q in e]
+    X, Y, Z = np.meshgrid(*c, indexing="ij")
+    DX, DY, DZ = np.meshgrid(*d, indexing="ij")
+    f = np.asfortranarray
+    hs = (I + 2, J + 2, K + 2)
+
+    def halo(a):
+        h = np.zeros(hs, order="F")
+        h[1:-1, 1:-1, 1:-1] = a
+        return h
+    r = np.sqrt((X - 0.35) ** 2 + (Y - 0.35) ** 2 + (Z - 0.35) ** 2)
+    vof = np.zeros(shape)
+    for a in (-0.25, 0.25):
+        for b in (-0.25, 0.25):
+            for g in (-0.25, 0.25):
+                rr = np.sqrt((X + a * DX - 0.35) ** 2 + (Y + b * DY - 0.35) ** 2
+                             + (Z + g * DZ - 0.35) ** 2)
+                vof += (rr < 0.15) * 0.125
+    hv = halo(vof)
+    hv[0], hv[-1] = hv[1], hv[-2]
+    hv[:, 0], hv[:, -1] = hv[:, 1], hv[:, -2]
+    hv[:, :, 0], hv[:, :, -1] = hv[:, :, 1], hv[:, :, -2]
+    u = 2.0 * np.sin(np.pi * X) ** 2 * np.sin(2 * np.pi * Y) * np.sin(2 * np.pi * Z)
+    v = -np.sin(2 * np.pi * X) * np.sin(np.pi * Y) ** 2 * np.sin(2 * np.pi * Z)
+    w = -np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y) * np.sin(np.pi * Z) ** 2
+    hu, hvv, hw = halo(u), halo(v), halo(w)
+    dt = 0.4 * d[0][0] / 2.0
+    faces = [(J, K), (J, K), (I, K), (I, K), (I, J), (I, J)]
+    inp = dict(u=hu, v=hvv, w=hw, x=f(X), y=f(Y), z=f(Z), dx=f(DX), dy=f(DY), dz=f(DZ),
+               nx=np.zeros(hs, order="F"), ny=np.zeros(hs, order="F"),
+               nz=np.ones(hs, order="F"), bcu_w=f(hu[0, 1:-1, 1:-1]),
+               bcv_s=f(hvv[1:-1, 0, 1:-1]), bcw_b=f(hw[1:-1, 1:-1, 0]),
+               lvs_flag=[1] * 6, lvs_const=[0.0] * 6, vof_flag=[1] * 6, vof_const=[0.0] * 6,
+               dt=dt)
+    state = dict(vof_l=hv, phi_l=halo(r - 0.15), nx_l=np.zeros(hs, order="F"),
+                 ny_l=np.zeros(hs, order="F"), nz_l=np.zeros(hs, order="F"),
+                 lvs_var=[np.zeros(q, order="F") for q in faces],
+                 vof_var=[np.zeros(q, order="F") for q in faces])
+    return inp, state
+
+
+def clsvof(shape, steps, warmup, cpu=True):
+    """Clsvof_Scheme (sp_clsvof_scheme) on device-resident arrays: one call is
+    the whole step of six sub-steps (60 launches).  Timed calls, per-kernel
+    GB/s from HIP events over the timed calls, and the oracle's
+    Clsvof_Scheme on a bounded sample (the same workload at 96^3 at most) as
+    the CPU baseline."""
+    import numpy as np
+    import torch
+
+    from simcoflow3d_amd.engine import Projector
+
+    I, J, K = shape
+    tg = time.perf_counter()
+    inp, st = clsvof_fields(shape)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).to("cuda")  # noqa: E731
+    dinp = {n: (T(a) if isinstance(a, np.ndarray) else a) for n, a in inp.items()}
+    dst = {n: T(st[n]) for n in ("vof_l", "phi_l", "nx_l", "ny_l", "nz_l")}
+    dst["lvs_var"] = [T(a) for a in st["lvs_var"]]
+    dst["vof_var"] = [T(a) for a in st["vof_var"]]
+    del inp, st
+    P = Projector(shape)
+    torch.cuda.synchronize()
+    print(f"[bench] clsvof inputs on the device ({time.perf_counter() - tg:.1f} s)",
+          file=sys.stderr, flush=True)
+    for _ in range(max(1, warmup)):
+        P.clsvof_scheme(dinp, dst, device=True)
+    P.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        P.clsvof_scheme(dinp, dst, device=True)  # synchronises its stream
+    el = time.perf_counter() - t0
+    rows = P.kernel_times()
+    P.set_profiling(False)
+    P.close()
+    N = I * J * K
+    kern = []
+    for r in rows:
+        ms = r["total_ms"]
+        gbs = r["bytes"] / (ms * 1e-3) / 1e9 if ms > 0 and r["bytes"] > 0 else None
+        kern.append(dict(name=r["name"], launches=r["launches"],
+                         avg_us=round(ms / max(r["launches"], 1) * 1e3, 1),
+                         bytes_per_launch=r["bytes"] / max(r["launches"], 1),
+                         gbs=round(gbs, 1) if gbs else None,
+                         frac=round(gbs / HBM_PEAK_GBS, 4) if gbs else None))
+    kern.sort(key=lambda t: -(t["avg_us"] * t["launches"]))
+    interface = float((dst["vof_l"] > 0).logical_and(dst["vof_l"] < 1).sum().item())
+    rec = dict(workload=f"Clsvof_Scheme, {I}x{J}x{K}, a sphere of radius 0.15 in the "
+                        "AdvectionTest deformation field, CFL 0.4, six sub-steps per call",
+               calls=steps, ms_per_call=round(el / steps * 1e3, 3),
+               cells_per_s=round(N * steps / el, 1), interface_cells=int(interface),
+               kernels=kern)
+    del dinp, dst
+    torch.cuda.empty_cache()
+    if cpu:
+        from oracle import orc
+        cs = tuple(min(n, 96) for n in shape)
+        inp, st = clsvof_fields(cs)
+        runs, secs = 0, 0.0
+        while runs == 0 or (secs < 10.0 and runs < 20):
+            t0 = time.perf_counter()
+            st = orc.clsvof_scheme(inp, st)
+            secs += time.perf_counter() - t0
+            runs += 1
+        n = cs[0] * cs[1] * cs[2]
+        rec["cpu_baseline"] = dict(value=round(runs * n / secs, 1), unit="cells/s",
+                                   cores=orc.num_threads(), kind="port",
+                                   sample=f"{runs} consecutive steps at {cs[0]}x{cs[1]}x{cs[2]}",
+                                   seconds=round(secs, 3))
+    return rec
+
+
 def geometry(shape, steps, warmup, cpu=True):
     """GridPreprocess (sp_grid_preprocess_dev) on device-resident arrays of
     the four staggered grids, a stretched grid with three spherical solids.
@@ -769,6 +898,11 @@ def main():
         print(json.dumps(dict(predictor=predictor(pshape, args.steps, args.warmup,
                                                   cpu=not args.no_cpu))), flush=True)
         return
+    if args.clsvof_only:
+        cshape = size or configs.config_dims(cfg)
+        print(json.dumps(dict(clsvof=clsvof(cshape, args.steps, args.warmup,
+                                            cpu=not args.no_cpu))), flush=True)
+        return
     if args.geometry_only:
         gshape = size or configs.config_dims(cfg)
         print(json.dumps(dict(geometry=geometry(gshape, args.steps, args.warmup,
@@ -805,14 +939,16 @@ def main():
     vtr = write_vtr(Pm, dev, out, case, args.vtr) if args.vtr else None
     stream = stream_probe() if args.stream_probe else None
     Pm.close()
-    geo = pred_rec = None
-    if args.geometry or args.predictor:
+    geo = pred_rec = cls_rec = None
+    if args.geometry or args.predictor or args.clsvof:
         del dev, out
         torch.cuda.empty_cache()
         if args.geometry:
             geo = geometry(case.shape, args.steps, args.warmup, cpu=not args.no_cpu)
         if args.predictor:
             pred_rec = predictor(case.shape, args.steps, args.warmup, cpu=not args.no_cpu)
+        if args.clsvof:
+            cls_rec = clsvof(case.shape, args.steps, args.warmup, cpu=not args.no_cpu)
         dev = to_device(case)
         out = make_outputs(case, dev)
 
@@ -920,6 +1056,7 @@ def main():
         "stream_probe": stream,
         "geometry": geo,
         "predictor": pred_rec,
+        "clsvof": cls_rec,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": [dict(name=k["name"], launches=k["launches"], work_launches=k["work_launches"],

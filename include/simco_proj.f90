@@ -15,11 +15,11 @@ module simco_proj
   implicit none
   private
   public :: sp_params, sp_inputs, sp_outputs, sp_stats, sp_adj_inputs, sp_adj_outputs, &
-            sp_wall_bc, sp_vtr_fields, sp_vtk_fields, sp_solid_body, sp_lvs_grid, sp_cell_geom, sp_cell_interp, sp_liquid_field, &
+            sp_wall_bc, sp_vtr_fields, sp_vtk_fields, sp_solid_body, sp_lvs_grid, sp_clsvof_inputs, sp_clsvof_state, sp_cell_geom, sp_cell_interp, sp_liquid_field, &
             sp_pred_inputs, sp_pred_outputs
   public :: sp_params_default, sp_create, sp_destroy, sp_set_params, sp_set_grid, &
             sp_set_grid_1d, sp_project, sp_project_dev, sp_adjoint, sp_adjoint_dev, &
-            sp_wall_step, sp_wall_step_dev, sp_write_vtr, sp_write_vtk, sp_interpolate_lvs, sp_grid_preprocess, &
+            sp_wall_step, sp_wall_step_dev, sp_write_vtr, sp_write_vtk, sp_interpolate_lvs, sp_clsvof_scheme, sp_grid_preprocess, &
             sp_grid_preprocess_dev, sp_new_cell_face, sp_define_momentum_exchange_cell, &
             sp_compute_uvw_liquid_field, sp_predict_uvw, sp_number_external_cell_noalloc, &
             sp_number_external_cell, sp_strerror_f, sp_last_error_f
@@ -136,6 +136,30 @@ module simco_proj
     type(c_ptr) :: phi = c_null_ptr, vof = c_null_ptr
     type(c_ptr) :: nx = c_null_ptr, ny = c_null_ptr, nz = c_null_ptr
   end type sp_lvs_grid
+
+  ! Clsvof_Scheme inputs: c_loc of TVar%u, v, w, PGrid%x..dz, PCell%nx, ny,
+  ! nz, BCu%VarW, BCv%VarS, BCw%VarB; BCLvs and BCVof %flag and %const; dt
+  type, bind(C) :: sp_clsvof_inputs
+    type(c_ptr) :: u = c_null_ptr, v = c_null_ptr, w = c_null_ptr
+    type(c_ptr) :: x = c_null_ptr, y = c_null_ptr, z = c_null_ptr
+    type(c_ptr) :: dx = c_null_ptr, dy = c_null_ptr, dz = c_null_ptr
+    type(c_ptr) :: nx = c_null_ptr, ny = c_null_ptr, nz = c_null_ptr
+    type(c_ptr) :: bcu_w = c_null_ptr, bcv_s = c_null_ptr, bcw_b = c_null_ptr
+    integer(c_int32_t) :: lvs_flag(6) = 1
+    real(c_double) :: lvs_const(6) = 0
+    integer(c_int32_t) :: vof_flag(6) = 1
+    real(c_double) :: vof_const(6) = 0
+    real(c_double) :: dt = 0
+  end type sp_clsvof_inputs
+
+  ! Clsvof_Scheme state: c_loc of PCell%vofL, phiL, nxL, nyL, nzL and of
+  ! BCLvs%VarW..VarT, BCVof%VarW..VarT
+  type, bind(C) :: sp_clsvof_state
+    type(c_ptr) :: vof_l = c_null_ptr, phi_l = c_null_ptr
+    type(c_ptr) :: nx_l = c_null_ptr, ny_l = c_null_ptr, nz_l = c_null_ptr
+    type(c_ptr) :: lvs_var(6) = c_null_ptr
+    type(c_ptr) :: vof_var(6) = c_null_ptr
+  end type sp_clsvof_state
 
   ! One staggered grid of GridPreprocess (CutCell.f90:62-325): c_loc of
   ! TGrid%dx, dy, dz and of the TCell members (allocated as in AllocateVar,
@@ -322,6 +346,15 @@ module simco_proj
       type(sp_lvs_grid), intent(in) :: grid
       integer(c_int), value :: device
     end function sp_interpolate_lvs
+
+    integer(c_int) function sp_clsvof_scheme(ctx, inputs, state, device) &
+        bind(C, name="sp_clsvof_scheme")
+      import :: c_ptr, c_int, sp_clsvof_inputs, sp_clsvof_state
+      type(c_ptr), value :: ctx
+      type(sp_clsvof_inputs), intent(in) :: inputs
+      type(sp_clsvof_state), intent(inout) :: state
+      integer(c_int), value :: device
+    end function sp_clsvof_scheme
 
     integer(c_int) function sp_grid_preprocess(ctx, p, u, v, w) &
         bind(C, name="sp_grid_preprocess")

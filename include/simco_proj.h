@@ -446,6 +446,55 @@ typedef struct sp_lvs_grid {
 sp_status sp_interpolate_lvs(sp_ctx* ctx, const sp_solid_body* body, const sp_lvs_grid* grid,
                              int device);
 
+/* Inputs of Clsvof_Scheme(PGrid,PCell,TVar,BCu,BCv,BCw,BCLvs,BCvof,Time,dt,
+ * itt) (Clsvof.f90:355-798).  Halo arrays: TVar%u, v, w (face velocities)
+ * and PCell%nx, ny, nz (the solid normals, which set the level-set ghosts);
+ * cell arrays: PGrid%x, y, z, dx, dy, dz; BCu%VarW (Jmax,Kmax), BCv%VarS
+ * (Imax,Kmax), BCw%VarB (Imax,Jmax) (the inflow velocities of the level-set
+ * fluxes); BCLvs%flag/const and BCVof%flag/const, faces W,E,S,N,B,T (0
+ * Dirichlet, 1 Neumann). */
+typedef struct sp_clsvof_inputs {
+  const double *u, *v, *w;
+  const double *x, *y, *z;
+  const double *dx, *dy, *dz;
+  const double *nx, *ny, *nz;
+  const double *bcu_w, *bcv_s, *bcw_b;
+  int32_t lvs_flag[6];
+  double lvs_const[6];
+  int32_t vof_flag[6];
+  double vof_const[6];
+  double dt;
+} sp_clsvof_inputs;
+
+/* The state Clsvof_Scheme advances: PCell%vofL, phiL (halo, interior and
+ * face ghosts written), PCell%nxL, nyL, nzL (halo, interior written; read
+ * by the first sub-step's boundary values), BCLvs%VarW..VarT (read: VarW,
+ * VarS, VarB feed the first sweeps; all written) and BCVof%VarW..VarT
+ * (written; NULL entries are skipped). */
+typedef struct sp_clsvof_state {
+  double *vof_l, *phi_l;
+  double *nx_l, *ny_l, *nz_l;
+  double* lvs_var[6];
+  double* vof_var[6];
+} sp_clsvof_state;
+
+/* Clsvof_Scheme (Clsvof.f90:355-798): six sub-steps of dt/6, each three
+ * directional sweeps of the liquid volume fraction (geometric fluxes of the
+ * reconstructed plane, East_Flux..Bottom_Flux) and level set (the
+ * reference's Lax-Wendroff fluxes), rotating the sweep order, then
+ * BoundaryConditionLvsVof, Interface_Reconstruct (Normal_Vector_Irre,
+ * Find_Distance) and Redistance_Levelset.  The reference's loops are kept
+ * with their boundary forms (the level-set face 1 of the x sweep divides by
+ * 2 and multiplies by dx(2), the z sweep's face 1 adds and its face kmax
+ * divides by dy, the north ghost subtracts the normal term).  Values are
+ * those of the reference's loops except where pow (x**3.d0 in
+ * Volume_Fraction_Calc and Newton_Raphson, x**(1/3) in Find_Distance) rounds
+ * differently on the GPU.  Newton_Raphson stops after 200 iterations (the
+ * reference has no limit).  Every dimension must be at least 3.  `device`:
+ * 1 when the pointers are device memory, 0 for host memory. */
+sp_status sp_clsvof_scheme(sp_ctx* ctx, const sp_clsvof_inputs* in, sp_clsvof_state* state,
+                           int device);
+
 /* Inputs of one momentum predictor step, PredictingUVW(PGrid, UGrid, VGrid,
  * WGrid, PCell, UCell, VCell, WCell, PCellO, UCellO, VCellO, WCellO,
  * FluxDivOld, TVar_n, TVar, BCu, BCv, BCw, PU, PV, PW, Pred, dt, itt)

@@ -1007,6 +1007,81 @@ sp_status sp_interpolate_lvs(sp_ctx* c, const sp_solid_body* body, const sp_lvs_
   return s;
 }
 
+sp_status sp_clsvof_scheme(sp_ctx* c, const sp_clsvof_inputs* in, sp_clsvof_state* st,
+                           int device) {
+  if (!c) return SP_ERR_INVALID_ARG;
+  hipSetDevice(c->device);
+  if (!in || !st) return fail(c, SP_ERR_INVALID_ARG, "null sp_clsvof_inputs or sp_clsvof_state");
+  const Dims& D = c->dm;
+  if (D.I < 3 || D.J < 3 || D.K < 3)
+    return fail(c, SP_ERR_INVALID_ARG, "Clsvof_Scheme needs at least 3 cells in each direction");
+  if (D.K > 65535 || (D.J + 3) / 4 > 65535)
+    return fail(c, SP_ERR_INVALID_ARG, "Clsvof_Scheme: Kmax or Jmax beyond the launch grid");
+  const void* req[] = {in->u, in->v, in->w, in->x, in->y, in->z, in->dx, in->dy, in->dz,
+                       in->nx, in->ny, in->nz, in->bcu_w, in->bcv_s, in->bcw_b,
+                       st->vof_l, st->phi_l, st->nx_l, st->ny_l, st->nz_l};
+  for (const void* q : req)
+    if (!q) return fail(c, SP_ERR_INVALID_ARG, "required Clsvof_Scheme array is null");
+  for (int f = 0; f < 6; ++f) {
+    if (!st->lvs_var[f]) return fail(c, SP_ERR_INVALID_ARG, "sp_clsvof_state.lvs_var entry is null");
+    if ((in->lvs_flag[f] != 0 && in->lvs_flag[f] != 1) || (in->vof_flag[f] != 0 && in->vof_flag[f] != 1))
+      return fail(c, SP_ERR_INVALID_ARG, "BCLvs/BCVof flags must be 0 (Dirichlet) or 1 (Neumann)");
+  }
+  if (!(in->dt >= 0.0)) return fail(c, SP_ERR_INVALID_ARG, "dt must be non-negative");
+  const size_t N = (size_t)D.N, Nh = (size_t)D.Nh;
+  const size_t nface[6] = {(size_t)D.J * D.K, (size_t)D.J * D.K, (size_t)D.I * D.K,
+                           (size_t)D.I * D.K, (size_t)D.I * D.J, (size_t)D.I * D.J};
+  void* work;
+  sp_status s = stage(c, 500, clsvof_work_doubles(D) * sizeof(double), &work);
+  if (s) return s;
+  if (device) {
+    s = clsvof_scheme(c, *in, *st, static_cast<double*>(work));
+    if (s) return s;
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    timers_collect(c);
+    return e == hipSuccess ? SP_OK : hip_check(c, e, "Clsvof_Scheme");
+  }
+  // host arrays: staging slots 510..559 belong to Clsvof_Scheme alone
+  size_t k = 510;
+  sp_clsvof_inputs d = *in;
+  auto upd = [&](const double* h, size_t n, const double** o) { return upload(c, k++, h, n, o); };
+  const double** ins[15] = {&d.u, &d.v, &d.w, &d.x, &d.y, &d.z, &d.dx, &d.dy, &d.dz,
+                            &d.nx, &d.ny, &d.nz, &d.bcu_w, &d.bcv_s, &d.bcw_b};
+  const size_t nin[15] = {Nh, Nh, Nh, N, N, N, N, N, N, Nh, Nh, Nh, nface[0], nface[2], nface[4]};
+  for (int q = 0; q < 15; ++q)
+    if ((s = upd(*ins[q], nin[q], ins[q]))) return s;
+  sp_clsvof_state ds = *st;
+  double** sts[5] = {&ds.vof_l, &ds.phi_l, &ds.nx_l, &ds.ny_l, &ds.nz_l};
+  for (int q = 0; q < 5; ++q) {
+    const double* t;
+    if ((s = upd(*sts[q], Nh, &t))) return s;
+    *sts[q] = const_cast<double*>(t);
+  }
+  for (int f = 0; f < 6; ++f) {
+    const double* t;
+    if ((s = upd(st->lvs_var[f], nface[f], &t))) return s;
+    ds.lvs_var[f] = const_cast<double*>(t);
+    if (st->vof_var[f]) {
+      void* o;
+      if ((s = stage(c, k++, nface[f] * sizeof(double), &o))) return s;
+      ds.vof_var[f] = static_cast<double*>(o);
+    }
+  }
+  if ((s = clsvof_scheme(c, d, ds, static_cast<double*>(work)))) return s;
+  double* const hs[5] = {st->vof_l, st->phi_l, st->nx_l, st->ny_l, st->nz_l};
+  const double* const dsrc[5] = {ds.vof_l, ds.phi_l, ds.nx_l, ds.ny_l, ds.nz_l};
+  for (int q = 0; q < 5; ++q)
+    if ((s = pipe_download(c, hs[q], dsrc[q], Nh * sizeof(double)))) return s;
+  for (int f = 0; f < 6; ++f) {
+    if ((s = pipe_download(c, st->lvs_var[f], ds.lvs_var[f], nface[f] * sizeof(double)))) return s;
+    if (st->vof_var[f] &&
+        (s = pipe_download(c, st->vof_var[f], ds.vof_var[f], nface[f] * sizeof(double))))
+      return s;
+  }
+  timers_collect(c);
+  return SP_OK;
+}
+
 sp_status sp_compute_uvw_liquid_field(sp_ctx* c, const sp_liquid_field* p,
                                       const sp_liquid_field* u, const sp_liquid_field* v,
                                       const sp_liquid_field* w, int device) {

@@ -263,6 +263,11 @@ sp_status new_cell_face(sp_ctx* c, const sp_cell_geom* const g[4],
 sp_status uvw_liquid_field(sp_ctx* c, const sp_liquid_field* const g[4]);
 // interpolateLvS (sp_clsvof.hip) of a solid body onto one grid
 sp_status interpolate_lvs(sp_ctx* c, const sp_solid_body& b, const sp_lvs_grid& g, bool device);
+// Clsvof_Scheme (sp_clsvof.hip) on device arrays; work holds
+// clsvof_work_doubles(D) doubles.
+sp_status clsvof_scheme(sp_ctx* c, const sp_clsvof_inputs& in, const sp_clsvof_state& st,
+                        double* work);
+size_t clsvof_work_doubles(const Dims& D);
 // Momentum predictor (sp_predict.hip) on device arrays; faces holds
 // predict_face_doubles(D) doubles.
 sp_status predict_uvw(sp_ctx* c, const sp_pred_inputs& in, const sp_pred_outputs& out,

@@ -38,6 +38,7 @@ EXPORTED_SYMBOLS = (
     "sp_write_vtk", "sp_grid_preprocess", "sp_grid_preprocess_dev", "sp_define_momentum_exchange_cell",
     "sp_number_external_cell", "sp_new_cell_face", "sp_compute_uvw_liquid_field",
     "sp_number_external_cell_noalloc", "sp_predict_uvw", "sp_interpolate_lvs",
+    "sp_clsvof_scheme",
 )
 
 # members of sp_liquid_field, in order
@@ -131,6 +132,25 @@ LVS_GRID_FIELDS = ("x", "y", "z", "dx", "dy", "dz", "phi", "vof", "nx", "ny", "n
 
 class SpLvsGrid(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in LVS_GRID_FIELDS]
+
+
+CLSVOF_IN_PTRS = ("u", "v", "w", "x", "y", "z", "dx", "dy", "dz", "nx", "ny", "nz", "bcu_w",
+                  "bcv_s", "bcw_b")
+
+
+class SpClsvofInputs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in CLSVOF_IN_PTRS] + [
+        ("lvs_flag", ctypes.c_int32 * 6), ("lvs_const", ctypes.c_double * 6),
+        ("vof_flag", ctypes.c_int32 * 6), ("vof_const", ctypes.c_double * 6),
+        ("dt", ctypes.c_double)]
+
+
+CLSVOF_STATE_PTRS = ("vof_l", "phi_l", "nx_l", "ny_l", "nz_l")
+
+
+class SpClsvofState(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in CLSVOF_STATE_PTRS] + [
+        ("lvs_var", ctypes.c_void_p * 6), ("vof_var", ctypes.c_void_p * 6)]
 
 
 class SpInputs(ctypes.Structure):
@@ -259,6 +279,8 @@ def load():
     L.sp_write_vtk.restype = ctypes.c_int
     L.sp_interpolate_lvs.argtypes = [V, P(SpSolidBody), P(SpLvsGrid), ctypes.c_int]
     L.sp_interpolate_lvs.restype = ctypes.c_int
+    L.sp_clsvof_scheme.argtypes = [V, P(SpClsvofInputs), P(SpClsvofState), ctypes.c_int]
+    L.sp_clsvof_scheme.restype = ctypes.c_int
     for name in ("sp_grid_preprocess", "sp_grid_preprocess_dev"):
         f = getattr(L, name)
         f.argtypes = [V] + [P(SpCellGeom)] * 4
@@ -764,6 +786,39 @@ class Projector:
         rc = self.lib.sp_interpolate_lvs(self.h, ctypes.byref(b), ctypes.byref(g),
                                          1 if device else 0)
         self._check(rc, "sp_interpolate_lvs")
+
+    def clsvof_scheme(self, inputs, state, device=False):
+        """Clsvof_Scheme (sp_clsvof_scheme).  inputs maps u, v, w, nx, ny, nz
+        (halo), x, y, z, dx, dy, dz (cell), bcu_w, bcv_s, bcw_b (face
+        arrays), lvs_flag, lvs_const, vof_flag, vof_const (6 each) and dt;
+        state maps vof_l, phi_l, nx_l, ny_l, nz_l (halo) and lvs_var,
+        vof_var (six face arrays each, vof_var entries may be None), all
+        updated in place.  numpy arrays (Fortran order), or torch device
+        tensors with device=True."""
+        keep = []
+
+        def inp(a):
+            if isinstance(a, np.ndarray):
+                a = np.asfortranarray(a, dtype=np.float64)
+                keep.append(a)
+            return _addr(a)
+
+        def out(a):
+            if isinstance(a, np.ndarray) and not (a.flags.f_contiguous and a.dtype == np.float64):
+                raise ValueError("state arrays must be Fortran-ordered float64")
+            return _addr(a)
+        ci = SpClsvofInputs(*(inp(inputs[n]) for n in CLSVOF_IN_PTRS),
+                            (ctypes.c_int32 * 6)(*inputs["lvs_flag"]),
+                            (ctypes.c_double * 6)(*inputs["lvs_const"]),
+                            (ctypes.c_int32 * 6)(*inputs["vof_flag"]),
+                            (ctypes.c_double * 6)(*inputs["vof_const"]), float(inputs["dt"]))
+        vv = list(state.get("vof_var") or [None] * 6)
+        cs = SpClsvofState(*(out(state[n]) for n in CLSVOF_STATE_PTRS),
+                           (ctypes.c_void_p * 6)(*(out(a) for a in state["lvs_var"])),
+                           (ctypes.c_void_p * 6)(*(out(a) for a in vv)))
+        rc = self.lib.sp_clsvof_scheme(self.h, ctypes.byref(ci), ctypes.byref(cs),
+                                       1 if device else 0)
+        self._check(rc, "sp_clsvof_scheme")
 
     def adjoint(self, gu, gv, gw, beta_grad=False):
         I, J, K = self.shape

@@ -15,6 +15,8 @@ program sp_layout
   type(sp_vtk_fields), target :: kf
   type(sp_solid_body), target :: sb
   type(sp_lvs_grid), target :: lg
+  type(sp_clsvof_inputs), target :: csi
+  type(sp_clsvof_state), target :: css
   type(sp_cell_geom), target :: cg
   type(sp_cell_interp), target :: ci
   type(sp_liquid_field), target :: lf
@@ -31,6 +33,8 @@ program sp_layout
   print '(a,1x,i0)', 'sp_vtk_fields', c_sizeof(kf)
   print '(a,1x,i0)', 'sp_solid_body', c_sizeof(sb)
   print '(a,1x,i0)', 'sp_lvs_grid', c_sizeof(lg)
+  print '(a,1x,i0)', 'sp_clsvof_inputs', c_sizeof(csi)
+  print '(a,1x,i0)', 'sp_clsvof_state', c_sizeof(css)
   print '(a,1x,i0)', 'sp_cell_geom', c_sizeof(cg)
   print '(a,1x,i0)', 'sp_cell_interp', c_sizeof(ci)
   print '(a,1x,i0)', 'sp_liquid_field', c_sizeof(lf)
@@ -58,6 +62,11 @@ program sp_layout
   print '(a,1x,i0)', 'sp_solid_body.cm', off(c_loc(sb%cm), c_loc(sb))
   print '(a,1x,i0)', 'sp_solid_body.rot', off(c_loc(sb%rot), c_loc(sb))
   print '(a,1x,i0)', 'sp_lvs_grid.nz', off(c_loc(lg%nz), c_loc(lg))
+  print '(a,1x,i0)', 'sp_clsvof_inputs.lvs_flag', off(c_loc(csi%lvs_flag), c_loc(csi))
+  print '(a,1x,i0)', 'sp_clsvof_inputs.vof_const', off(c_loc(csi%vof_const), c_loc(csi))
+  print '(a,1x,i0)', 'sp_clsvof_inputs.dt', off(c_loc(csi%dt), c_loc(csi))
+  print '(a,1x,i0)', 'sp_clsvof_state.lvs_var', off(c_loc(css%lvs_var), c_loc(css))
+  print '(a,1x,i0)', 'sp_clsvof_state.vof_var', off(c_loc(css%vof_var), c_loc(css))
   print '(a,1x,i0)', 'sp_wall_bc.const_u', off(c_loc(wb%const_u), c_loc(wb))
   print '(a,1x,i0)', 'sp_wall_bc.const_p', off(c_loc(wb%const_p), c_loc(wb))
   print '(a,1x,i0)', 'sp_inputs.wall', off(c_loc(inp%wall), c_loc(inp))

@@ -41,7 +41,8 @@ def test_fortran_types_match_c_layout():
                "sp_adj_inputs": engine.SpAdjInputs, "sp_adj_outputs": engine.SpAdjOutputs,
                "sp_wall_bc": engine.SpWallBc, "sp_vtr_fields": engine.SpVtrFields,
                "sp_vtk_fields": engine.SpVtkFields, "sp_solid_body": engine.SpSolidBody,
-               "sp_lvs_grid": engine.SpLvsGrid,
+               "sp_lvs_grid": engine.SpLvsGrid, "sp_clsvof_inputs": engine.SpClsvofInputs,
+               "sp_clsvof_state": engine.SpClsvofState,
                "sp_cell_geom": engine.SpCellGeom, "sp_cell_interp": engine.SpCellInterp,
                "sp_liquid_field": engine.SpLiquidField, "sp_pred_inputs": engine.SpPredInputs,
                "sp_pred_outputs": engine.SpPredOutputs}
