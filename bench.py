@@ -69,6 +69,9 @@ def parse():
                          "size, with its CPU baseline")
     ap.add_argument("--predictor-only", action="store_true",
                     help="time only the momentum predictor and print its record")
+    ap.add_argument("--chebyshev", action="store_true",
+                    help="also time the headline solve with the Chebyshev-Jacobi smoother "
+                         "(degrees 2 and 4) instead of red-black Gauss-Seidel")
     ap.add_argument("--clsvof", action="store_true",
                     help="also time the interface transport (sp_clsvof_scheme) at the same grid "
                          "size, with its CPU baseline")
@@ -988,6 +991,32 @@ def main():
                    ms_per_step=round(el4 / len(st_4) * 1e3, 3), iterations=st_4[-1].iterations,
                    final_div=st_4[-1].final_div, converged=bool(st_4[-1].converged))
 
+    # the Chebyshev-Jacobi smoother (SP_FLAG_CHEBYSHEV) in the same V-cycle,
+    # degrees 2 and 4, beside the fused red-black Gauss-Seidel headline
+    cheb = None
+    if args.chebyshev:
+        cheb = []
+        for deg in (2, 4):
+            cc = dict(common, flags=2, mg_pre=deg, mg_post=deg)
+            PC = Projector(case.shape, precision=SP_PREC_MIXED, **cc)
+            PC.set_grid_1d(case.dxp, case.dyp, case.dzp, case.dxu, case.dyv, case.dzw)
+            run_solves(PC, dev, out, case, max(1, args.warmup))
+            elc, st_c, _ = timed(PC, dev, out, case, max(1, args.steps // 2))
+            PC.set_profiling(True)
+            run_solves(PC, dev, out, case, 1)
+            rows = PC.kernel_times()
+            PC.set_profiling(False)
+            PC.close()
+            fine = [r for r in rows if r["name"] == "mg_cheb_fine"]
+            rec = dict(degree=deg, solves_per_s=round(len(st_c) / elc, 3),
+                       ms_per_step=round(elc / len(st_c) * 1e3, 3), iterations=st_c[-1].iterations,
+                       final_div=st_c[-1].final_div, converged=bool(st_c[-1].converged))
+            if fine and fine[0]["total_ms"] > 0:
+                f = fine[0]
+                rec["cheb_fine_avg_us"] = round(f["total_ms"] / f["launches"] * 1e3, 1)
+                rec["cheb_fine_gbs"] = round(f["bytes"] / (f["total_ms"] * 1e-3) / 1e9, 1)
+            cheb.append(rec)
+
     roof = None
     if live:
         dom = live
@@ -1050,6 +1079,7 @@ def main():
                    "converged": bool(sm.converged)},
         "fp64": fp64,
         "levels4": lv4,
+        "chebyshev": cheb,
         "host_arrays": host,
         "forward_adjoint": fwd_adj,
         "vtr": vtr,

@@ -84,8 +84,17 @@ typedef struct sp_params {
 
 /* sp_params.flags bits */
 enum {
-  SP_FLAG_UNFUSED_SMOOTHER = 1  /* one kernel per half-sweep instead of the
+  SP_FLAG_UNFUSED_SMOOTHER = 1, /* one kernel per half-sweep instead of the
                                    temporally blocked smoother (same result) */
+  SP_FLAG_CHEBYSHEV = 2         /* Chebyshev-Jacobi smoothing instead of
+                                   red-black Gauss-Seidel on the levels above
+                                   the single-workgroup tail: mg_pre / mg_post
+                                   steps of the Chebyshev semi-iteration for
+                                   D^-1 A on [lambda_max / 8, lambda_max],
+                                   lambda_max = 2 (Gershgorin bound of a
+                                   diagonally dominant operator); mg_omega is
+                                   not used there.  Every lane updates every
+                                   cell (no colours); one kernel per step. */
 };
 
 typedef struct sp_ctx sp_ctx;

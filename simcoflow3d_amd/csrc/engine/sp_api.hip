@@ -95,6 +95,7 @@ static void free_buffers(sp_ctx* c) {
   c->allocated_prec = -1;
   c->allocated_precond = -1;
   c->allocated_levels = -1;
+  c->allocated_cheb = -1;
   c->have_forward = false;
 }
 
@@ -110,6 +111,9 @@ static sp_status alloc_levels(sp_ctx* c, Level<T>* lv, bool level0_owned) {
     L.n = (int64_t)L.nx * L.ny * L.nz;
     sp_status s;
     if ((s = dmalloc(c, &L.xs, L.n))) return s;
+    // Chebyshev step buffer only where that smoother is selected
+    L.cd = nullptr;
+    if ((c->prm.flags & SP_FLAG_CHEBYSHEV) && (s = dmalloc(c, &L.cd, L.n))) return s;
     if (l == 0 && !level0_owned) continue;
     if ((s = dmalloc(c, &L.cx, L.n)) || (s = dmalloc(c, &L.cy, L.n)) || (s = dmalloc(c, &L.cz, L.n)) ||
         (s = dmalloc(c, &L.e_store, L.n)) || (s = dmalloc(c, &L.b, L.n)) || (s = dmalloc(c, &L.x, L.n)))
@@ -125,8 +129,9 @@ static sp_status ensure_buffers(sp_ctx* c) {
   int L = auto_levels(D.I, D.J, D.K);
   if (c->prm.mg_levels > 0) L = std::min(L, c->prm.mg_levels);
   if (pc != SP_PRECOND_MG) L = 1;
+  const int cheb = (c->prm.flags & SP_FLAG_CHEBYSHEV) ? 1 : 0;
   if (c->allocated_for == (size_t)D.N && c->allocated_prec == prec && c->allocated_precond == pc &&
-      c->allocated_levels == L)
+      c->allocated_levels == L && c->allocated_cheb == cheb)
     return SP_OK;
   free_buffers(c);
   c->levels = L;
@@ -190,6 +195,7 @@ static sp_status ensure_buffers(sp_ctx* c) {
   c->allocated_prec = prec;
   c->allocated_precond = pc;
   c->allocated_levels = L;
+  c->allocated_cheb = cheb;
   return SP_OK;
 }
 

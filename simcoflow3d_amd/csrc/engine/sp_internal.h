@@ -86,7 +86,9 @@ struct Level {
   T* e;        // diagonal-only terms, nullptr when they are all zero
   T* e_store;  // storage for e
   T *x, *b;
-  T* xs;       // pre-smoothed x (input of the fused post-smoother)
+  T* xs;       // pre-smoothed x (input of the fused post-smoother); the
+               // Chebyshev smoother's second iterate buffer
+  T* cd;       // the Chebyshev smoother's direction
 };
 
 struct Dims {
@@ -203,6 +205,7 @@ struct sp_ctx {
   int allocated_prec = -1;
   int allocated_precond = -1;
   int allocated_levels = -1;
+  int allocated_cheb = -1;  // Chebyshev step buffers allocated (flag at allocation)
   std::vector<void*> allocs;    // every device allocation of the buffers above
   int32_t* hflags = nullptr;    // pinned convergence flags (2 slots)
 };

@@ -101,6 +101,40 @@ __global__ void __launch_bounds__(256) k_gs(LevelView<T> L, int color, int zero_
   L.x[n] = x0 + omega * (v - x0);
 }
 
+// One step of the Chebyshev semi-iteration for D^-1 A (Saad, Iterative
+// Methods for Sparse Linear Systems, 2nd ed., Algorithm 12.1):
+//   d <- c1 d + c2 D^-1 (b - A x_in),  x_out = x_in + d
+// (c1 unused on the first step; x_in == nullptr stands for x_in = 0).
+// x_out is another buffer than x_in: every lane reads its neighbours' x_in.
+template <typename T, bool FINE>
+__global__ void __launch_bounds__(256) k_cheb(LevelView<T> L, const T* __restrict__ xin,
+                                              T* __restrict__ xout, T* __restrict__ dv, T c1,
+                                              T c2, int first, const Scalars* S) {
+  if (S->done) return;
+  const Cell3 q = cell_of(L.nx, L.ny, L.nz);
+  if (!q.in) return;
+  const int i = q.a, j = q.b, k = q.c;
+  const int64_t n = i + (int64_t)L.nx * (j + (int64_t)L.ny * k);
+  const T d = diag_of(L, i, j, k, n);
+  if (!(d > T(0))) {
+    xout[n] = T(0);
+    dv[n] = T(0);
+    return;
+  }
+  T r = L.b[n];
+  T x0 = T(0);
+  if (xin) {
+    LevelView<T> V = L;
+    V.x = const_cast<T*>(xin);
+    x0 = xin[n];
+    r = r - (d * x0 - offdiag_sum(V, i, j, k, n));
+  }
+  const T z = sizeof(T) == 4 ? r * smoother_rcp(d) : r / d;
+  const T dn = first ? c2 * z : c1 * dv[n] + c2 * z;
+  dv[n] = dn;
+  xout[n] = x0 + dn;
+}
+
 // Coarse b = sum over children of the fine residual.
 template <typename T, bool FINE>
 __global__ void __launch_bounds__(256) k_resid_restrict(LevelView<T> F, LevelView<T> C,
@@ -809,6 +843,47 @@ void launch_gs(sp_ctx* c, const Level<T>& L, int color, int zero_init, bool fine
   timer_end(c, slot);
 }
 
+// m steps of the Chebyshev smoother on level L, from zero (pre) or from L.x
+// (post); the result ends in L.x (L.xs is the other iterate buffer).
+template <typename T>
+void launch_cheb(sp_ctx* c, const Level<T>& L, int m, bool from_zero, bool fine) {
+  constexpr double lmax = 2.0, lmin = lmax / 8.0;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+  dim3 blk(64, 4, 1), grd = flat_grid(L.nx, L.ny, L.nz);
+  const LevelView<T> V = view_of(L);
+  // the last step writes L.x; a step never writes the buffer it reads
+  T* buf[2] = {L.x, L.xs};
+  int w = from_zero ? (m - 1) % 2 : 1;  // buffer of the first step's output
+  const T* in = from_zero ? nullptr : L.x;
+  double rho = 1.0 / sigma;
+  int slot;
+  for (int s = 0; s < m; ++s) {
+    double c1 = 0.0, c2 = 1.0 / theta;
+    if (s > 0) {
+      const double rn = 1.0 / (2.0 * sigma - rho);
+      c1 = rn * rho;
+      c2 = 2.0 * rn / delta;
+      rho = rn;
+    }
+    // words per cell: b and the three face arrays read, x in read (not on a
+    // step from zero), d read after the first step, e read where present;
+    // x out and d written
+    const double words = 6.0 + (in ? 1.0 : 0.0) + (s > 0 ? 1.0 : 0.0) + (L.e ? 1.0 : 0.0);
+    timer_begin(c, fine ? "mg_cheb_fine" : "mg_cheb_coarse", level_bytes(L, words), &slot);
+    if (fine)
+      hipLaunchKernelGGL((k_cheb<T, true>), grd, blk, 0, c->stream, V, in, buf[w], L.cd, (T)c1,
+                         (T)c2, s == 0 ? 1 : 0, (const Scalars*)c->S);
+    else
+      hipLaunchKernelGGL((k_cheb<T, false>), grd, blk, 0, c->stream, V, in, buf[w], L.cd, (T)c1,
+                         (T)c2, s == 0 ? 1 : 0, (const Scalars*)c->S);
+    timer_end(c, slot);
+    in = buf[w];
+    w ^= 1;
+  }
+  if (in != L.x)  // post with an odd m: the result is in L.xs
+    hipMemcpyAsync(L.x, in, sizeof(T) * (size_t)L.n, hipMemcpyDeviceToDevice, c->stream);
+}
+
 template <typename T, int NU>
 void launch_fused(sp_ctx* c, Level<T>* lv, int l, int mode) {
   using Sh = FusedShape<NU>;
@@ -877,7 +952,8 @@ void launch_fused(sp_ctx* c, Level<T>* lv, int l, int mode) {
 // The V-cycle uses the temporally blocked smoother legs
 bool fused_vcycle(const sp_ctx* c) {
   const int pre = std::max(1, c->prm.mg_pre), post = c->prm.mg_post;
-  return pre == post && (pre == 1 || pre == 2) && !(c->prm.flags & SP_FLAG_UNFUSED_SMOOTHER);
+  return pre == post && (pre == 1 || pre == 2) &&
+         !(c->prm.flags & (SP_FLAG_UNFUSED_SMOOTHER | SP_FLAG_CHEBYSHEV));
 }
 
 template <typename T>
@@ -885,6 +961,7 @@ void vcycle(sp_ctx* c, Level<T>* lv) {
   const int L = c->levels, Lt = c->tail_level;
   const int pre = std::max(1, c->prm.mg_pre), post = c->prm.mg_post;
   const bool fused = fused_vcycle(c);
+  const bool cheb = (c->prm.flags & SP_FLAG_CHEBYSHEV) != 0;
   dim3 blk(64, 4, 1);
   int slot;
   for (int l = 0; l < Lt; ++l) {
@@ -894,9 +971,13 @@ void vcycle(sp_ctx* c, Level<T>* lv) {
       else launch_fused<T, 1>(c, lv, l, 0);
       continue;
     }
-    launch_gs(c, lv[l], 0, 1, fine);
-    launch_gs(c, lv[l], 1, 0, fine);
-    for (int s = 1; s < pre; ++s) { launch_gs(c, lv[l], 0, 0, fine); launch_gs(c, lv[l], 1, 0, fine); }
+    if (cheb) {
+      launch_cheb(c, lv[l], pre, true, fine);
+    } else {
+      launch_gs(c, lv[l], 0, 1, fine);
+      launch_gs(c, lv[l], 1, 0, fine);
+      for (int s = 1; s < pre; ++s) { launch_gs(c, lv[l], 0, 0, fine); launch_gs(c, lv[l], 1, 0, fine); }
+    }
     dim3 grd = flat_grid(lv[l + 1].nx, lv[l + 1].ny, lv[l + 1].nz);
     timer_begin(c, fine ? "mg_restrict_fine" : "mg_restrict_coarse",
                 level_bytes(lv[l], 5.0 + (lv[l].e ? 1.0 : 0.0)) + level_bytes(lv[l + 1], 1.0), &slot);
@@ -949,7 +1030,11 @@ void vcycle(sp_ctx* c, Level<T>* lv) {
       hipLaunchKernelGGL((k_prolong<T, false>), grd, blk, 0, c->stream, view_of(lv[l], act),
                          view_of(lv[l + 1]), (const Scalars*)c->S);
     timer_end(c, slot);
-    for (int s = 0; s < post; ++s) { launch_gs(c, lv[l], 1, 0, fine); launch_gs(c, lv[l], 0, 0, fine); }
+    if (cheb) {
+      if (post > 0) launch_cheb(c, lv[l], post, false, fine);
+    } else {
+      for (int s = 0; s < post; ++s) { launch_gs(c, lv[l], 1, 0, fine); launch_gs(c, lv[l], 0, 0, fine); }
+    }
   }
 }
 

@@ -29,6 +29,8 @@ SP_PREC_FP64 = 0
 SP_PREC_MIXED = 1
 SP_PRECOND_MG = 0
 SP_PRECOND_JACOBI = 1
+SP_FLAG_UNFUSED_SMOOTHER = 1
+SP_FLAG_CHEBYSHEV = 2
 
 EXPORTED_SYMBOLS = (
     "sp_params_default", "sp_create", "sp_destroy", "sp_set_params", "sp_set_grid",
