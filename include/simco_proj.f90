@@ -221,6 +221,7 @@ module simco_proj
 
   type, bind(C) :: sp_adj_inputs
     type(c_ptr) :: gu = c_null_ptr, gv = c_null_ptr, gw = c_null_ptr
+    type(c_ptr) :: gp = c_null_ptr  ! dL/dPp, or null
   end type sp_adj_inputs
 
   type, bind(C) :: sp_adj_outputs

@@ -178,9 +178,14 @@ typedef struct sp_stats {
 } sp_stats;
 
 /* Adjoint inputs: dL/du, dL/dv, dL/dw on the corrected faces (halo arrays,
- * entries (1:I,1:J,1:K) read). */
+ * entries (1:I,1:J,1:K) read) and, optionally, dL/dPp for the output
+ * pressure Proj%Pp (halo array: the interior and the face ghosts are read,
+ * since each ghost is flag * Pp of its inner cell, ProjectionP.f90:620-643;
+ * the gradient follows the gauge of the forward call, pin cell or zero mean).
+ * gp = NULL: L does not depend on the pressure. */
 typedef struct sp_adj_inputs {
   const double *gu, *gv, *gw;
+  const double* gp;
 } sp_adj_inputs;
 
 /* Adjoint outputs: dL/du*, dL/dv*, dL/dw* (halo arrays, every entry written)

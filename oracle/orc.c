@@ -773,8 +773,9 @@ int orc_adjoint(const orc_grid* g, const double* pc, const double* ee, const dou
                 const double* te, const int32_t* ct, const int32_t* fl, const int32_t* ctu,
                 const int32_t* ctv, const int32_t* ctw, const int32_t* mu, const int32_t* mv,
                 const int32_t* mw, const double* Pp, const double* gu, const double* gv,
-                const double* gw, const orc_solver_opts* o, double* gus, double* gvs,
-                double* gws, double* gkx, double* gky, double* gkz, double* lambda_out) {
+                const double* gw, const double* gp, int pin_i, int pin_j, int pin_k,
+                const orc_solver_opts* o, double* gus, double* gvs, double* gws, double* gkx,
+                double* gky, double* gkz, double* lambda_out) {
   const int I = g->I, J = g->J, K = g->K;
   const int64_t N = (int64_t)I * J * K, Nh = (int64_t)(I + 2) * (J + 2) * (K + 2);
   const double *P4 = pc + 3 * N, *P5 = pc + 4 * N, *P6 = pc + 5 * N;
@@ -804,6 +805,41 @@ int orc_adjoint(const orc_grid* g, const double* pc, const double* ee, const dou
         if (k < K) { if (d[NX(g, i, j, k + 1)] > 0) pbar[NX(g, i, j, k + 1)] -= tw * P6[n]; }
         else if (d[n] > 0) pbar[n] -= (double)fl[5] * tw * P6[n];
       }
+  /* pressure cotangent: Pp = x - shift on active cells, the face ghosts
+   * flag * Pp of the inner cell (orc_pressure_field), the shift x(pin) or
+   * the active-cell mean of x (orc_gauge) in singular systems */
+  if (gp) {
+    double esum = 0.0;
+    for (int64_t q = 0; q < N; ++q) esum += e[q];
+    double qsum = 0.0;
+    int64_t nact = 0;
+    for (int k = 1; k <= K; ++k)
+      for (int j = 1; j <= J; ++j)
+        for (int i = 1; i <= I; ++i) {
+          const int64_t n = NX(g, i, j, k);
+          if (!(d[n] > 0)) continue;
+          double q = gp[HX(g, i, j, k)];
+          if (i == 1) q += (double)fl[0] * gp[HX(g, 0, j, k)];
+          if (i == I) q += (double)fl[1] * gp[HX(g, I + 1, j, k)];
+          if (j == 1) q += (double)fl[2] * gp[HX(g, i, 0, k)];
+          if (j == J) q += (double)fl[3] * gp[HX(g, i, J + 1, k)];
+          if (k == 1) q += (double)fl[4] * gp[HX(g, i, j, 0)];
+          if (k == K) q += (double)fl[5] * gp[HX(g, i, j, K + 1)];
+          pbar[n] += q;
+          qsum += q;
+          ++nact;
+        }
+    if (esum == 0.0 && nact > 0) {
+      const int pinned = pin_i >= 1 && pin_j >= 1 && pin_k >= 1 && pin_i <= I && pin_j <= J &&
+                         pin_k <= K && d[NX(g, pin_i, pin_j, pin_k)] > 0;
+      if (pinned) {
+        pbar[NX(g, pin_i, pin_j, pin_k)] -= qsum;
+      } else {
+        for (int64_t n = 0; n < N; ++n)
+          if (d[n] > 0) pbar[n] -= qsum / (double)nact;
+      }
+    }
+  }
   /* lambda = A^{-1} pbar */
   double* lam = dalloc(N);
   orc_solver_stats st;

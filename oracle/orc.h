@@ -171,14 +171,17 @@ void orc_wall_step(const orc_grid* g, const double* ee, const double* ne, const 
                    double* p, double* const* var);
 
 /* Adjoint of the projection for L with dL/du = gu, dL/dv = gv, dL/dw = gw
- * (halo arrays, read on the corrected faces (1:I,1:J,1:K)).  Gradients are
- * returned for u*,v*,w* (halo arrays, written on the faces the forward reads)
- * and for the E/N/T face coefficients K = PoCoef(:,4..6) (interior arrays). */
+ * (halo arrays, read on the corrected faces (1:I,1:J,1:K)) and dL/dPp = gp
+ * (halo array with its face ghosts, or NULL) for the pressure in the gauge
+ * pin_i, pin_j, pin_k of orc_gauge.  Gradients are returned for u*,v*,w*
+ * (halo arrays, written on the faces the forward reads) and for the E/N/T
+ * face coefficients K = PoCoef(:,4..6) (interior arrays). */
 int orc_adjoint(const orc_grid* g, const double* pocoef, const double* ee, const double* ne,
                 const double* te, const int32_t* ct, const int32_t* bc_flag,
                 const int32_t* ctu, const int32_t* ctv, const int32_t* ctw,
                 const int32_t* mu, const int32_t* mv, const int32_t* mw,
                 const double* Pp, const double* gu, const double* gv, const double* gw,
+                const double* gp, int pin_i, int pin_j, int pin_k,
                 const orc_solver_opts* o, double* gus, double* gvs, double* gws,
                 double* gkx, double* gky, double* gkz, double* lambda_out);
 

@@ -107,7 +107,8 @@ def lib():
         L.orc_mres.argtypes = [G, F64P, F64P, F64P, F64P, F64P, F64P, F64P]
         L.orc_mres.restype = ctypes.c_double
         L.orc_adjoint.argtypes = ([G, F64P, F64P, F64P, F64P, I32P, I32P] + [I32P] * 6 +
-                                  [F64P] * 4 + [ctypes.POINTER(OrcOpts)] + [F64P] * 7)
+                                  [F64P] * 5 + [ctypes.c_int] * 3 +
+                                  [ctypes.POINTER(OrcOpts)] + [F64P] * 7)
         L.orc_adjoint.restype = ctypes.c_int
         L.orc_wall_step.argtypes = [G, F64P, F64P, F64P] + [I32P] * 5 + [F64P] * 5 + [
             ctypes.POINTER(F64P)]
@@ -390,8 +391,15 @@ def _pcg_to_floor(case, op, b, o, R):
     return x, st, sing
 
 
-def adjoint(case, pc, Pp, gu, gv, gw, o=None):
+def adjoint(case, pc, Pp, gu, gv, gw, o=None, gp=None, pin=None):
+    """Gradients of L for dL/du, dL/dv, dL/dw and, optionally, dL/dPp = gp
+    (halo array) for the pressure in the gauge `pin` (a cell (i, j, k),
+    1-based, or (0, 0, 0) for the zero mean); pin is required with gp."""
     o = o or opts(max_iter=1000, div_tol=1e-300)
+    if gp is not None:
+        assert pin is not None, "the pressure cotangent needs the forward's gauge"
+        gp = np.asfortranarray(gp, dtype=np.float64)
+    pi, pj, pk = pin if pin is not None else (0, 0, 0)
     hs = case.halo_shape
     gus, gvs, gws = (np.zeros(hs, order="F") for _ in range(3))
     gkx, gky, gkz = (np.zeros((case.I, case.J, case.K), order="F") for _ in range(3))
@@ -402,7 +410,8 @@ def adjoint(case, pc, Pp, gu, gv, gw, o=None):
                            _p(case.cell_type_u), _p(case.cell_type_v), _p(case.cell_type_w),
                            _p(case.moex_u), _p(case.moex_v), _p(case.moex_w), _p(Pp),
                            _p(np.asfortranarray(gu)), _p(np.asfortranarray(gv)),
-                           _p(np.asfortranarray(gw)), ctypes.byref(o), _p(gus), _p(gvs),
+                           _p(np.asfortranarray(gw)), _p(gp), pi, pj, pk, ctypes.byref(o),
+                           _p(gus), _p(gvs),
                            _p(gws), _p(gkx), _p(gky), _p(gkz), _p(lam))
     return dict(gus=gus, gvs=gvs, gws=gws, gkx=gkx, gky=gky, gkz=gkz, lam=lam, rc=rc)
 

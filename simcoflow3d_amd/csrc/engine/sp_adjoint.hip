@@ -5,8 +5,10 @@
 //   b   = G^T diag(a) u*            (SetPoissonVectors, ProjectionP.f90:521-526)
 //   A p = b,  A = G^T diag(a K) G    (symmetric, SetPoissonMatrix)
 //   u   = m (u* - K G p)             (ComputePUVW.f90:185-219)
-// Backward for dL/du = g:
+// Backward for dL/du = g and dL/dPp = gp:
 //   pbar = -G^T (m K g)              (with the pressure ghost rule at walls)
+//        + Q^T gp                    (Q: the output field Pp of the iterate x,
+//                                     its gauge shift and face ghosts)
 //   A lam = pbar                     (same multigrid PCG: A is self-adjoint)
 //   dL/du* = m g + a G lam
 //   dL/dK  = -m g (G p) - a (G lam)(G p)   (second term on faces between active cells)
@@ -34,6 +36,9 @@ struct AdjArgs {
   const int32_t *ctu, *ctv, *ctw, *mu, *mv, *mw;
   int32_t fl[6];
   const double *gu, *gv, *gw;
+  const double* gp;       // dL/dPp (halo array) or nullptr
+  int64_t pin;            // interior index of the gauge cell, or -1 (mean gauge)
+  int pin_a, pin_b, pin_c;
   const double *ee, *ne, *te;
   const double *dxp, *dyp, *dzp;
   const double *dp[3];
@@ -64,7 +69,7 @@ __global__ void __launch_bounds__(256) k_adj_source(AdjArgs A) {
   const Cell3 cq = cell_of(A.I, A.J, A.K);
   const int a = cq.a, b = cq.b, c = cq.c;
   const int64_t N = A.N;
-  double psum = 0.0, pmax = 0.0;
+  double psum = 0.0, pmax = 0.0, qsum = 0.0;
   if (cq.in) {
     {
       const int64_t n = nidx(A, a, b, c), h = hidx(A, a, b, c);
@@ -92,24 +97,47 @@ __global__ void __launch_bounds__(256) k_adj_source(AdjArgs A) {
           pb -= tface(A, d, m, h - hs[d]) * A.pocoef[(3 + d) * N + m];
         }
       }
+      if (A.gp && A.active[n]) {
+        // Pp(P) = x(P) - shift, and each face ghost is flag * Pp of its
+        // inner cell (edges and corners 0)
+        double q = A.gp[h];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          if (co[d] == 0) q += (double)A.fl[2 * d] * A.gp[h - hs[d]];
+          if (co[d] == lim[d] - 1) q += (double)A.fl[2 * d + 1] * A.gp[h + hs[d]];
+        }
+        pb += q;
+        qsum = q;
+      }
       pb = A.active[n] ? pb : 0.0;
       A.pbar[n] = pb;
       psum += pb;
       pmax = fabs(pb) / (A.dxp[a] * A.dyp[b] * A.dzp[c]);
     }
   }
-  double v[2] = {psum, pmax};
-  block_partials<kTileThreads, 2, 2u>(A.site, v);
+  double v[3] = {psum, pmax, qsum};
+  block_partials<kTileThreads, 3, 2u>(A.site, v);
 }
 
 __global__ void __launch_bounds__(kFinThreads) k_adj_source_fin(AdjArgs A, uint32_t nb) {
-  double t[2];
-  finalize_partials<kFinThreads, 2, 2u>(A.site, nb, t);
+  double t[3];
+  finalize_partials<kFinThreads, 3, 2u>(A.site, nb, t);
   if (flat_tid() == 0) {
     Scalars* S = A.S;
-    S->bsum = t[0];
-    S->src_max = t[1];
-    S->bmean = (S->singular && S->active > 0) ? t[0] / (double)S->active : 0.0;
+    double bsum = t[0], smax = t[1];
+    if (S->singular && A.pin >= 0 && A.active[A.pin]) {
+      // pin gauge Pp = x - x(pin): its transpose takes the sum of the
+      // pressure cotangent off the pin cell.  (The mean gauge's transpose is
+      // the mean projection the solver applies to every singular source.)
+      const double pb = A.pbar[A.pin] - t[2];
+      A.pbar[A.pin] = pb;
+      bsum -= t[2];
+      const double r = fabs(pb) / (A.dxp[A.pin_a] * A.dyp[A.pin_b] * A.dzp[A.pin_c]);
+      smax = r > smax ? r : smax;
+    }
+    S->bsum = bsum;
+    S->src_max = smax;
+    S->bmean = (S->singular && S->active > 0) ? bsum / (double)S->active : 0.0;
   }
 }
 
@@ -188,6 +216,14 @@ sp_status adjoint(sp_ctx* c, const sp_adj_inputs& in, sp_adj_outputs& out, sp_st
   A.mu = fin.moex_u; A.mv = fin.moex_v; A.mw = fin.moex_w;
   for (int f = 0; f < 6; ++f) A.fl[f] = fin.bc_flag[f];
   A.gu = in.gu; A.gv = in.gv; A.gw = in.gw;
+  A.gp = in.gp;
+  const sp_params& P = c->prm;
+  A.pin = -1;
+  if (P.pin_i >= 1 && P.pin_j >= 1 && P.pin_k >= 1 && P.pin_i <= D.I && P.pin_j <= D.J &&
+      P.pin_k <= D.K) {
+    A.pin_a = P.pin_i - 1; A.pin_b = P.pin_j - 1; A.pin_c = P.pin_k - 1;
+    A.pin = A.pin_a + (int64_t)D.I * (A.pin_b + (int64_t)D.J * A.pin_c);
+  }
   A.ee = fin.ee_area; A.ne = fin.ne_area; A.te = fin.te_area;
   A.dxp = c->dxp; A.dyp = c->dyp; A.dzp = c->dzp;
   A.dp[0] = fin.dp_u; A.dp[1] = fin.dp_v; A.dp[2] = fin.dp_w;
@@ -206,12 +242,13 @@ sp_status adjoint(sp_ctx* c, const sp_adj_inputs& in, sp_adj_outputs& out, sp_st
   const double NB = (double)D.N;
   int slot;
   A.site = red_site(c, kSiteAdj);
-  timer_begin(c, "adj_source", NB * (3 * 8 + 6 * 4 + 3 * 8 + 8 + 1 + 3 * 8 + 3 * 8 + 8), &slot);
+  timer_begin(c, "adj_source",
+              NB * (3 * 8 + 6 * 4 + 3 * 8 + 8 + 1 + 3 * 8 + 3 * 8 + 8 + (in.gp ? 8 : 0)), &slot);
   hipLaunchKernelGGL(k_adj_source, fg, fb, 0, c->stream, A);
   // one partial per 256 cells: fold them on many workgroups first
   const uint32_t nb = flat_blocks(D.I, D.J, D.K), chunk = 512, nf = (nb + chunk - 1) / chunk;
   const RedSite fold = red_site(c, kSiteFold);
-  hipLaunchKernelGGL((k_fold_partials<2, 2u>), dim3(nf), dim3(256), 0, c->stream, A.site, fold, nb,
+  hipLaunchKernelGGL((k_fold_partials<3, 2u>), dim3(nf), dim3(256), 0, c->stream, A.site, fold, nb,
                      chunk);
   A.site = fold;
   hipLaunchKernelGGL(k_adj_source_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A, nf);

@@ -674,6 +674,8 @@ sp_status sp_adjoint(sp_ctx* c, const sp_adj_inputs* in, sp_adj_outputs* out, sp
   if ((s = upload(c, k++, in->gu, Nh, &di.gu)) || (s = upload(c, k++, in->gv, Nh, &di.gv)) ||
       (s = upload(c, k++, in->gw, Nh, &di.gw)))
     return s;
+  // slots 43-51 hold the outputs below, 52 the pressure cotangent
+  if (in->gp && (s = upload(c, 52, in->gp, Nh, &di.gp))) return s;
   sp_adj_outputs dout{};
   double* hs[9] = {out->gu_star, out->gv_star, out->gw_star, out->gk_e, out->gk_n, out->gk_t,
                    out->gbeta_e, out->gbeta_n, out->gbeta_t};

@@ -186,7 +186,7 @@ class SpStats(ctypes.Structure):
 
 
 class SpAdjInputs(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p) for n in ("gu", "gv", "gw")]
+    _fields_ = [(n, ctypes.c_void_p) for n in ("gu", "gv", "gw", "gp")]
 
 
 class SpAdjOutputs(ctypes.Structure):
@@ -822,11 +822,13 @@ class Projector:
                                        1 if device else 0)
         self._check(rc, "sp_clsvof_scheme")
 
-    def adjoint(self, gu, gv, gw, beta_grad=False):
+    def adjoint(self, gu, gv, gw, beta_grad=False, gp=None):
+        """dL/du*, dL/dK (and dL/dbeta) for dL/du, dL/dv, dL/dw and, when gp
+        is given, dL/dPp (halo arrays)."""
         I, J, K = self.shape
         hs = (I + 2, J + 2, K + 2)
-        ai = SpAdjInputs(*(_addr(np.asfortranarray(g)) for g in (gu, gv, gw)))
         keep = [np.asfortranarray(g) for g in (gu, gv, gw)]
+        keep.append(None if gp is None else np.asfortranarray(gp))
         ai = SpAdjInputs(*(_addr(g) for g in keep))
         res = {n: np.zeros(hs, order="F") for n in ("gu_star", "gv_star", "gw_star")}
         for n in ("gk_e", "gk_n", "gk_t") + (("gbeta_e", "gbeta_n", "gbeta_t") if beta_grad else ()):
@@ -839,8 +841,9 @@ class Projector:
         return res, st
 
     def adjoint_dev(self, g, res):
-        """g: dict gu,gv,gw (device tensors); res: dict of device output tensors."""
-        ai = SpAdjInputs(*(_addr(g[n]) for n in ("gu", "gv", "gw")))
+        """g: dict gu, gv, gw and optionally gp (device tensors); res: dict of
+        device output tensors."""
+        ai = SpAdjInputs(*(_addr(g.get(n)) for n in ("gu", "gv", "gw", "gp")))
         ao = SpAdjOutputs(*(_addr(res.get(n)) for n in (
             "gu_star", "gv_star", "gw_star", "gk_e", "gk_n", "gk_t", "gbeta_e", "gbeta_n", "gbeta_t")))
         st = SpStats()

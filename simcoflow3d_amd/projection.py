@@ -16,8 +16,9 @@
 * :class:`DeviceProjection` runs the projection on torch tensors resident on
   the GPU and keeps the forward pressure for the adjoint.
 * :class:`ProjectionFunction` is the ``torch.autograd.Function`` whose
-  backward is the engine's hand-written adjoint: gradients with respect to
-  u*, v*, w* and to the face coefficients beta = 1/rho' (E, N, T faces).
+  backward is the engine's hand-written adjoint: gradients of a loss on the
+  corrected velocities and the pressure with respect to u*, v*, w* and to
+  the face coefficients beta = 1/rho' (E, N, T faces).
 
 Array layouts are the host's Fortran arrays.  On the torch side a Fortran
 (I+2, J+2, K+2) halo array is a C-contiguous tensor of shape (K+2, J+2, I+2)
@@ -303,13 +304,17 @@ class DeviceProjection:
         bt = pc[5] * dzw.view(K, 1, 1) / dp[2]
         return [be.contiguous(), bn.contiguous(), bt.contiguous()]
 
-    def adjoint(self, gu, gv, gw):
+    def adjoint(self, gu, gv, gw, gp=None):
+        """Gradients for dL/du, dL/dv, dL/dw and, if given, dL/dPp (device
+        halo tensors)."""
         torch = self.torch
         res = {n: torch.zeros(self.hs, dtype=torch.float64, device=gu.device)
                for n in ("gu_star", "gv_star", "gw_star")}
         for n in ("gk_e", "gk_n", "gk_t", "gbeta_e", "gbeta_n", "gbeta_t"):
             res[n] = torch.zeros(self.cs, dtype=torch.float64, device=gu.device)
         g = {"gu": gu.contiguous(), "gv": gv.contiguous(), "gw": gw.contiguous()}
+        if gp is not None:
+            g["gp"] = gp.contiguous()
         torch.cuda.synchronize()
         st = self.P.adjoint_dev(g, res)
         return res, st
@@ -331,7 +336,6 @@ def _ProjectionFunction():
             ctx.dp_engine = dp_engine
             ctx.fid = dp_engine.forward_id
             ctx.save_for_backward(us, vs, ws, be, bn, bt)
-            ctx.mark_non_differentiable(out["p"])
             return out["u"], out["v"], out["w"], out["p"]
 
         @staticmethod
@@ -342,7 +346,7 @@ def _ProjectionFunction():
                 E.forward(us, vs, ws, beta=[be, bn, bt])
             z = torch.zeros_like(us)
             res, _ = E.adjoint(gu if gu is not None else z, gv if gv is not None else z,
-                               gw if gw is not None else z)
+                               gw if gw is not None else z, gp=gp)
             return (None, res["gu_star"], res["gv_star"], res["gw_star"], res["gbeta_e"],
                     res["gbeta_n"], res["gbeta_t"])
 
