@@ -235,11 +235,13 @@ def test_new_cell_face_requires_p_spacing(gpu_available):
     del keep
 
 
-@pytest.mark.parametrize("size", [(300, 200, 130), (5, 3, 200)])
+@pytest.mark.parametrize("size", [(300, 200, 130), (5, 3, 200), (3000, 3000, 2)])
 def test_numbering_large_and_deep(gpu_available, size):
     """NumberExternalCell where the piece counts span several scan
-    workgroups (300x200 columns of three 64-plane pieces) and where one
-    column holds several pieces (200 planes)."""
+    workgroups (300x200 columns of three 64-plane pieces), where one column
+    holds several pieces (200 planes), and where there are more workgroup
+    totals than threads in the top-level scan (3000x3000 columns, 9 M
+    pieces, 1099 totals: each scan thread sums a segment of two)."""
     print(f"[geometry] PosNu size={size}", flush=True)
     rng = np.random.default_rng(3)
     ct = np.asfortranarray(rng.integers(0, 3, size).astype(np.int32))
