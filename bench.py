@@ -374,7 +374,7 @@ def write_vtr(P, dev, out, case, directory):
 
 def predictor(shape, steps, warmup, cpu=True):
     """The momentum predictor (sp_predict_uvw, PredictingUVW with SpaceOrder
-    1 and TimeOrder 1) on device-resident arrays: the geometry of three
+    1 and TimeOrder 1, then SpaceOrder 2) on device-resident arrays: the geometry of three
     spherical solids on a z-stretched grid is built on the GPU by the
     engine's own geometry update, velocities and BC values are seeded.
     Timed calls of the whole step (four launches), per-kernel GB/s from HIP
@@ -437,32 +437,46 @@ def predictor(shape, steps, warmup, cpu=True):
     torch.cuda.synchronize()
     print(f"[bench] predictor inputs on the device ({time.perf_counter() - tg:.1f} s)",
           file=sys.stderr, flush=True)
-    for _ in range(max(1, warmup)):
-        P.predict_uvw(pa, pc, pc, fd, pred, dp, device=True)
-    P.set_profiling(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        P.predict_uvw(pa, pc, pc, fd, pred, dp, device=True)  # synchronises its stream
-    el = time.perf_counter() - t0
-    rows = P.kernel_times()
-    P.set_profiling(False)
-    P.close()
     N = I * J * K
-    kern = []
-    for r in rows:
-        ms = r["total_ms"]
-        gbs = r["bytes"] / (ms * 1e-3) / 1e9 if ms > 0 and r["bytes"] > 0 else None
-        kern.append(dict(name=r["name"], launches=r["launches"],
-                         avg_us=round(ms / max(r["launches"], 1) * 1e3, 1),
-                         bytes_per_launch=r["bytes"] / max(r["launches"], 1),
-                         gbs=round(gbs, 1) if gbs else None,
-                         frac=round(gbs / HBM_PEAK_GBS, 4) if gbs else None))
-    kern.sort(key=lambda t: -(t["avg_us"] * t["launches"]))
+
+    def timed(space_order):
+        fd["params"] = dict(f["params"], space_order=space_order)
+        for _ in range(max(1, warmup)):
+            P.predict_uvw(pa, pc, pc, fd, pred, dp, device=True)
+        P.set_profiling(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            P.predict_uvw(pa, pc, pc, fd, pred, dp, device=True)  # synchronises its stream
+        el = time.perf_counter() - t0
+        rows = P.kernel_times()
+        P.set_profiling(False)
+        kern = []
+        for r in rows:
+            ms = r["total_ms"]
+            gbs = r["bytes"] / (ms * 1e-3) / 1e9 if ms > 0 and r["bytes"] > 0 else None
+            kern.append(dict(name=r["name"], launches=r["launches"],
+                             avg_us=round(ms / max(r["launches"], 1) * 1e3, 1),
+                             bytes_per_launch=r["bytes"] / max(r["launches"], 1),
+                             gbs=round(gbs, 1) if gbs else None,
+                             frac=round(gbs / HBM_PEAK_GBS, 4) if gbs else None))
+        kern.sort(key=lambda t: -(t["avg_us"] * t["launches"]))
+        # device time of the timed kernels per call (the wall time adds the
+        # host's launch and synchronisation overhead)
+        dev_ms = sum(r["total_ms"] for r in rows) / steps
+        return el, kern, dev_ms
+
+    el, kern, dev_ms = timed(1)
     rec = dict(workload=f"PredictingUVW, SpaceOrder 1, TimeOrder 1, {I}x{J}x{K}, three spheres, "
                         "z stretched 2% per cell, geometry from the engine's geometry update",
                calls=steps, ms_per_call=round(el / steps * 1e3, 3), cells_per_s=round(N * steps / el, 1),
-               kernels=kern)
+               timed_kernel_ms_per_call=round(dev_ms, 3), kernels=kern)
+    # SpaceOrder 2 (ModifiedConvectiveFlux) on the same inputs, second step
+    el2, kern2, dev_ms2 = timed(2)
+    rec["space_order_2"] = dict(calls=steps, ms_per_call=round(el2 / steps * 1e3, 3),
+                                cells_per_s=round(N * steps / el2, 1),
+                                timed_kernel_ms_per_call=round(dev_ms2, 3), kernels=kern2)
+    P.close()
     del dev, interp, pc, pa, fd, pred, dp
     torch.cuda.empty_cache()
     if cpu:
@@ -492,6 +506,19 @@ def predictor(shape, steps, warmup, cpu=True):
                                    cores=orc.num_threads(), kind="port",
                                    sample=f"{runs} x the same {I}x{J}x{K} step",
                                    seconds=round(secs, 3))
+        # SpaceOrder 2: the flux loop carries delh in loop order, one thread
+        p2 = dict(f["params"], space_order=2)
+        runs, secs = 0, 0.0
+        while runs == 0 or (secs < 10.0 and runs < 20):
+            t0 = time.perf_counter()
+            orc.predict(g, pa_h, cc, cc, f["vel"], f["vel_n"], f["flux_div_old"], f["bc_flag"],
+                        f["bcv"], p2)
+            secs += time.perf_counter() - t0
+            runs += 1
+        rec["space_order_2"]["cpu_baseline"] = dict(
+            value=round(runs * N / secs, 1), unit="cells/s", cores=orc.num_threads(),
+            kind="port", sample=f"{runs} x the same {I}x{J}x{K} step (fluxes on one thread)",
+            seconds=round(secs, 3))
     return rec
 
 

@@ -379,7 +379,7 @@ module simco_proj
       integer(c_int), value :: device
     end function sp_compute_uvw_liquid_field
 
-    ! PredictingUVW with SpaceOrder 1; device = 0 for host arrays
+    ! PredictingUVW with SpaceOrder 1 or 2; device = 0 for host arrays
     integer(c_int) function sp_predict_uvw(ctx, inp, outp, device) bind(C, name="sp_predict_uvw")
       import :: c_ptr, c_int, sp_pred_inputs, sp_pred_outputs
       type(c_ptr), value :: ctx

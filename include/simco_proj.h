@@ -513,7 +513,8 @@ sp_status sp_clsvof_scheme(sp_ctx* ctx, const sp_clsvof_inputs* in, sp_clsvof_st
  * WGrid, PCell, UCell, VCell, WCell, PCellO, UCellO, VCellO, WCellO,
  * FluxDivOld, TVar_n, TVar, BCu, BCv, BCw, PU, PV, PW, Pred, dt, itt)
  * (PredictUVW.f90:49-803), with SpaceOrder 1 (first-order upwind convection,
- * :1338-1794; every driver's input file selects it).  The grid is the
+ * :1338-1794; every driver's input file selects it) or 2 (central,
+ * ModifiedConvectiveFlux :2253-2644).  The grid is the
  * context's (sp_set_grid); the cells are the host's own arrays as the
  * geometry update writes them (sp_cell_geom / sp_cell_interp layouts). */
 typedef struct sp_pred_inputs {
@@ -537,7 +538,9 @@ typedef struct sp_pred_inputs {
   int64_t itt;                    /* time step number (1: first-order start) */
   int32_t time_order;             /* TimeOrder: 1 Euler, 2 extrapolated velocity, 3
                                      Adams-Bashforth (StateVariables.f90:13) */
-  int32_t space_order;            /* SpaceOrder: only 1 is built */
+  int32_t space_order;            /* SpaceOrder: 1 or 2 (3 is not built); the first
+                                     step (itt = 1) takes the first-order fluxes
+                                     either way (:172-185) */
   double mu_w_ref, mu_a_ref;      /* muWRef = muw/muref, muARef = mua/muref */
   double rey, fr;                 /* Rey, Fr (StateVariables.f90:12) */
   double gx, gy, gz, g;           /* gravity components and g */
@@ -551,12 +554,17 @@ typedef struct sp_pred_outputs {
   double *dp_u, *dp_v, *dp_w;
 } sp_pred_outputs;
 
-/* PredictingUVW (PredictUVW.f90:49-803) for SpaceOrder 1.  DiffusiveFlux
- * zeroes its coefficients (:3672-3673), so the three HYPRE momentum solves
- * (:692-759) are diagonal: u* = rhs / aP.  Every output equals a sequential
- * evaluation of the reference's expressions bit for bit; two cut-cell tests
- * that read a cell array at index Jmax + 1 (:1416-1430, :1744-1751) read
- * "no such neighbour".  Returns SP_ERR_INVALID_ARG for SpaceOrder /= 1.
+/* PredictingUVW (PredictUVW.f90:49-803) for SpaceOrder 1 and 2.
+ * DiffusiveFlux zeroes its coefficients (:3672-3673), so the three HYPRE
+ * momentum solves (:692-759) are diagonal: u* = rhs / aP.  Every output
+ * equals a sequential evaluation of the reference's expressions bit for
+ * bit; cut-cell tests that read a cell array one past its end (index
+ * Jmax + 1 at :1416-1430 and :1744-1751, and their SpaceOrder 2 siblings)
+ * read "no such neighbour".  SpaceOrder 2 keeps the reference's carry of
+ * its local delh from one cut face to the next in loop order (i outermost,
+ * k innermost); before the first cut face of a call that sets it, the
+ * face's own delhec is taken (the reference reads an undefined value
+ * there).  Returns SP_ERR_INVALID_ARG for SpaceOrder 3.
  * `device`: 1 when every pointer is device memory, 0 for host memory. */
 sp_status sp_predict_uvw(sp_ctx* ctx, const sp_pred_inputs* in, sp_pred_outputs* out,
                          int device);

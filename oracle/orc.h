@@ -245,7 +245,7 @@ double orc_volume_fraction(double dx, double dy, double dz, double nxx, double n
 void orc_uvw_liquid_field(int I, int J, int K, const orc_liquid* P, orc_liquid* U, orc_liquid* V,
                           orc_liquid* W);
 
-/* ---- Momentum predictor (PredictUVW.f90), SpaceOrder 1 (orc_pred.c) ---- */
+/* ---- Momentum predictor (PredictUVW.f90), SpaceOrder 1 and 2 (orc_pred.c) ---- */
 /* The members of one staggered cell set (UCell, VCell or WCell) that
  * PredictingUVW reads: cell arrays (1:I,1:J,1:K) and halo arrays
  * (0:I+1,0:J+1,0:K+1); fce/fcn/fct halo x3, cent cell x3. */
@@ -273,11 +273,25 @@ void orc_conv_flux_first(const orc_grid* g, const double* pee, const double* pne
                          const double* wn, const double* const* bcv, double* cfew, double* cfns,
                          double* cftb);
 
-/* PredictingUVW (PredictUVW.f90:49-803) for SpaceOrder 1: pred = Pred%u,v,w
+/* ModifiedConvectiveFlux x3 (PredictUVW.f90:2253-2644), SpaceOrder 2: the
+ * same arrays; the local delh carried between loop iterations as the
+ * reference's loop leaves it. */
+void orc_conv_flux_second(const orc_grid* g, const double* pee, const double* pne,
+                          const double* pte, const orc_pred_cell* U, const orc_pred_cell* V,
+                          const orc_pred_cell* W, const double* un, const double* vn,
+                          const double* wn, const double* const* bcv, double* cfew, double* cfns,
+                          double* cftb);
+
+/* Of the last orc_conv_flux_second call: cut faces that read a carried
+ * delh, and cut faces with none carried yet (test instrumentation). */
+void orc_conv_flux_second_carries(int64_t* carried, int64_t* uncarried);
+
+/* PredictingUVW (PredictUVW.f90:49-803) for SpaceOrder 1 and 2 (the first
+ * step, itt = 1, takes the first-order fluxes either way): pred = Pred%u,v,w
  * (halo, updated where the reference writes), dp = PU/PV/PW%Dp (halo, every
  * entry written), flux_div_old (I,J,K,3) read and updated.  old = UCellO..,
  * cur = UCell..; vel = TVar%u,v,w, vel_n = TVar_n%u,v,w; bc_flag: BCu%flag,
- * BCv%flag, BCw%flag (18).  Returns -1 for SpaceOrder /= 1. */
+ * BCv%flag, BCw%flag (18).  Returns -1 for SpaceOrder 3 (not restated). */
 int orc_predict(const orc_grid* g, const double* pee, const double* pne, const double* pte,
                 const orc_pred_cell* const old[3], const orc_pred_cell* const cur[3],
                 const double* const vel[3], const double* const vel_n[3], double* flux_div_old,

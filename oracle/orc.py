@@ -576,6 +576,8 @@ def _pred_lib():
         L.orc_conv_flux_first.argtypes = [G, F64P, F64P, F64P, C, C, C, F64P, F64P, F64P,
                                           ctypes.POINTER(F64P), F64P, F64P, F64P]
         L.orc_conv_flux_first.restype = None
+        L.orc_conv_flux_second.argtypes = L.orc_conv_flux_first.argtypes
+        L.orc_conv_flux_second.restype = None
         L.orc_predict.argtypes = [G, F64P, F64P, F64P, ctypes.POINTER(C), ctypes.POINTER(C),
                                   ctypes.POINTER(F64P), ctypes.POINTER(F64P), F64P, I32P,
                                   ctypes.POINTER(F64P), ctypes.POINTER(OrcPredParams),
@@ -607,8 +609,9 @@ def _f64_array(arrs, keep):
     return out
 
 
-def conv_flux_first(grid, p_areas, cells, vel, bcv):
-    """ModifiedConvectiveFluxFirstOrder x3: (CFEW, CFNS, CFTB) with shapes
+def conv_flux_first(grid, p_areas, cells, vel, bcv, second=False):
+    """ModifiedConvectiveFluxFirstOrder x3 (ModifiedConvectiveFlux x3, the
+    SpaceOrder 2 fluxes, with ``second``): (CFEW, CFNS, CFTB) with shapes
     (I+2,J,K,3), (I,J+2,K,3), (I,J,K+2,3) (index 0 of the extended axis is
     the reference's 0)."""
     L = _pred_lib()
@@ -621,10 +624,25 @@ def conv_flux_first(grid, p_areas, cells, vel, bcv):
     cfns = np.zeros((I, J + 2, K, 3), order="F")
     cftb = np.zeros((I, J, K + 2, 3), order="F")
     g = grid_of(grid)
-    L.orc_conv_flux_first(ctypes.byref(g), *(_p(a) for a in pa),
-                          *(ctypes.byref(c) for c in cs), *(_p(a) for a in vv),
-                          _f64_array(bcv, keep), _p(cfew), _p(cfns), _p(cftb))
+    f = L.orc_conv_flux_second if second else L.orc_conv_flux_first
+    f(ctypes.byref(g), *(_p(a) for a in pa), *(ctypes.byref(c) for c in cs),
+      *(_p(a) for a in vv), _f64_array(bcv, keep), _p(cfew), _p(cfns), _p(cftb))
     return cfew, cfns, cftb
+
+
+def conv_flux_second(grid, p_areas, cells, vel, bcv):
+    """ModifiedConvectiveFlux x3 (SpaceOrder 2), shapes as conv_flux_first."""
+    return conv_flux_first(grid, p_areas, cells, vel, bcv, second=True)
+
+
+def second_order_carries():
+    """(carried, uncarried) of the last SpaceOrder 2 flux evaluation: cut
+    faces whose neighbour tests both failed and that read the delh of an
+    earlier face, or had none to read."""
+    L = _pred_lib()
+    a, b = ctypes.c_int64(), ctypes.c_int64()
+    L.orc_conv_flux_second_carries(ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
 
 
 def predict(grid, p_areas, old, cur, vel, vel_n, flux_div_old, bc_flag, bcv, params, pred=None):
@@ -656,7 +674,7 @@ def predict(grid, p_areas, old, cur, vel, vel_n, flux_div_old, bc_flag, bcv, par
                        _f64_array(vel_n, keep), _p(fdo), _p(fl), _f64_array(bcv, keep),
                        ctypes.byref(prm), _f64_array(outs, keep), _f64_array(dps, keep))
     if rc != 0:
-        raise ValueError("orc_predict: only SpaceOrder 1 is restated")
+        raise ValueError("orc_predict: SpaceOrder 1 and 2 are restated")
     return outs, dps, fdo
 
 

@@ -1,6 +1,6 @@
 /*
  * CPU FP64 restatement of the momentum predictor PredictingUVW
- * (src/PredictUVW.f90:49-803) for SpaceOrder = 1 -- TEST INFRASTRUCTURE ONLY
+ * (src/PredictUVW.f90:49-803) for SpaceOrder 1 and 2 -- TEST INFRASTRUCTURE ONLY
  * (see orc.h).  Compiled with -ffp-contract=off: every expression keeps the
  * reference's operation order, so the engine's kernels can match it bit for
  * bit.
@@ -467,6 +467,314 @@ void orc_conv_flux_first(const orc_grid* g, const double* pee, const double* pne
 #undef H
 }
 
+/* ---- SpaceOrder 2: ModifiedConvectiveFlux (PredictUVW.f90:2253-2644) ----
+ *
+ * Central interpolation with the cut-cell corrections.  The reference loops
+ * i outermost and k innermost (:2277-2279) and keeps its local delh between
+ * iterations: a cut-face branch whose two neighbour tests both fail
+ * (:2326-2341 and its siblings) divides by the delh of the last branch that
+ * set one in this call.  That carry is restated here as the loop does it;
+ * before any branch of the call has set delh the reference reads an
+ * undefined value, for which this restatement takes the face's own delhec
+ * (the first-order routine's choice, :1415).  x**2.d0 is x*x (the exact
+ * square GCC makes of pow(x, 2.0)). */
+
+typedef struct delh_carry {
+  int have;
+  double v;
+} delh_carry;
+
+/* how often the last call read a carried delh, and how often it had none
+ * to read (test instrumentation) */
+static int64_t g_carried = 0, g_uncarried = 0;
+void orc_conv_flux_second_carries(int64_t* carried, int64_t* uncarried) {
+  *carried = g_carried;
+  *uncarried = g_uncarried;
+}
+
+/* the two neighbour tests of a cut face: cell a first (>=), then cell b
+ * (>), each setting delh and the convective velocity; `avail` false where
+ * the cells lie beyond the arrays (read as no neighbour, as the first-order
+ * routine's guards do) */
+static void cut_tests(const orc_grid* g, const orc_pred_cell* C, int avail, int ai, int aj, int ak,
+                      int bi, int bj, int bk, const double* vel, double* cv, double dhec,
+                      delh_carry* dc, int init_to_dhec) {
+  double delh;
+  int set = 0;
+  if (init_to_dhec) { delh = dhec; set = 1; }
+  if (avail && usable(g, C, ai, aj, ak) && vofc(g, C, ai, aj, ak) >= vofc(g, C, bi, bj, bk)) {
+    delh = plane_dist(g, C, ai, aj, ak);
+    *cv = vel[HX(g, ai, aj, ak)];
+    set = 1;
+  }
+  if (avail && usable(g, C, bi, bj, bk) && vofc(g, C, bi, bj, bk) > vofc(g, C, ai, aj, ak)) {
+    delh = plane_dist(g, C, bi, bj, bk);
+    *cv = vel[HX(g, bi, bj, bk)];
+    set = 1;
+  }
+  if (set) {
+    dc->have = 1;
+    dc->v = delh;
+  } else {
+    delh = dc->have ? dc->v : dhec;
+    if (dc->have) ++g_carried;
+    else ++g_uncarried;
+  }
+  *cv = *cv * dhec / (delh + kEps);
+}
+
+/* |eta| >= 1 -> 0.5 */
+static inline double clamp_eta(double eta) { return fabs(eta) >= 1.0 ? 0.5 : eta; }
+
+void orc_conv_flux_second(const orc_grid* g, const double* pee, const double* pne,
+                          const double* pte, const orc_pred_cell* U, const orc_pred_cell* V,
+                          const orc_pred_cell* W, const double* un, const double* vn,
+                          const double* wn, const double* const* bcv, double* cfew, double* cfns,
+                          double* cftb) {
+  const int I = g->I, J = g->J, K = g->K;
+  g_carried = g_uncarried = 0;
+#define H(i, j, k) HX(g, i, j, k)
+  /* ---- iu = 1 (:2280-2408) ---- */
+  {
+    delh_carry dc = {0, 0.0};
+    for (int i = 1; i <= I + 1; ++i)
+      for (int j = 1; j <= J; ++j)
+        for (int k = 1; k <= K; ++k) {
+          double uw, eta, flux;
+          /* u momentum, convective u (:2281-2298) */
+          uw = 0.5 * (un[H(i, j, k)] + un[H(i - 1, j, k)]);
+          if (i == 1) {
+            flux = uw * uw * U->ee[H(i, j, k)] * UDY(g, j) * UDZ(g, k);
+          } else if (i >= I) {
+            flux = uw * uw * U->ee[H(i - 1, j, k)] * UDY(g, j) * UDZ(g, k);
+          } else {
+            eta = U->etae[H(i - 1, j, k)];
+            uw = (1.0 - eta) * un[H(i - 1, j, k)] + eta * un[H(i, j, k)];
+            const double a = uw * U->ale[H(i - 1, j, k)];
+            flux = a * a * U->ee[H(i - 1, j, k)] * UDY(g, j) * UDZ(g, k);
+          }
+          cfew[ew_ix(g, i, j, k, 1)] = flux;
+          /* v momentum, convective u (:2299-2357) */
+          uw = 0.5 * (un[H(i - 1, j + 1, k)] + un[H(i - 1, j, k)]);
+          if (i > I) {
+            flux = uw * bcval(g, bcv, 6 + 1, j, k) * V->ee[H(i - 1, j, k)] * VDY(g, j) * VDZ(g, k);
+          } else if (i == 1) {
+            flux = uw * bcval(g, bcv, 6 + 0, j, k) * V->ee[H(i, j, k)] * VDY(g, j) * VDZ(g, k);
+          } else {
+            if (pee[H(i - 1, j, k)] <= 1.0e-5 || pee[H(i - 1, j + 1, k)] <= 1.0e-5) {
+              const double d1 = f3(g, V->fce, i - 1, j, k, 1) * V->nx[NX(g, i - 1, j, k)] +
+                                f3(g, V->fce, i - 1, j, k, 2) * V->ny[NX(g, i - 1, j, k)] +
+                                f3(g, V->fce, i - 1, j, k, 3) * V->nz[NX(g, i - 1, j, k)] +
+                                V->phi[NX(g, i - 1, j, k)];
+              const double d2 = (f3(g, V->fce, i - 1, j, k, 1) - UDX(g, i - 1)) * V->nx[NX(g, i, j, k)] +
+                                f3(g, V->fce, i - 1, j, k, 2) * V->ny[NX(g, i, j, k)] +
+                                f3(g, V->fce, i - 1, j, k, 3) * V->nz[NX(g, i, j, k)] +
+                                V->phi[NX(g, i, j, k)];
+              cut_tests(g, U, j < J, i - 1, j + 1, k, i - 1, j, k, un, &uw, 0.5 * (d1 + d2), &dc, 0);
+            } else {
+              const double Sy = U->syn[H(i - 1, j, k)];
+              eta = clamp_eta(fabs(f3(g, V->fce, i - 1, j, k, 2) + VDY(g, j) / 2.0 -
+                                   cent(g, U, i - 1, j, k, 2)) / Sy);
+              uw = (1.0 - eta) * un[H(i - 1, j, k)] + eta * un[H(i - 1, j + 1, k)];
+            }
+            eta = V->etae[H(i - 1, j, k)];
+            const double vw = (1.0 - eta) * vn[H(i - 1, j, k)] + eta * vn[H(i, j, k)];
+            flux = uw * vw * V->ale[H(i - 1, j, k)] * V->ee[H(i - 1, j, k)] * VDY(g, j) * VDZ(g, k);
+          }
+          cfew[ew_ix(g, i, j, k, 2)] = flux;
+          /* w momentum, convective u (:2358-2407) */
+          uw = 0.5 * (un[H(i - 1, j, k + 1)] + un[H(i - 1, j, k)]);
+          if (i > I) {
+            flux = uw * bcval(g, bcv, 12 + 1, j, k) * W->ee[H(i - 1, j, k)] * WDY(g, j) * WDZ(g, k);
+          } else if (i == 1) {
+            flux = uw * bcval(g, bcv, 12 + 0, j, k) * W->ee[H(i, j, k)] * WDY(g, j) * WDZ(g, k);
+          } else {
+            if (pee[H(i - 1, j, k)] <= 1.0e-5 || pee[H(i - 1, j, k + 1)] <= 1.0e-5) {
+              const double d1 = f3(g, W->fce, i - 1, j, k, 1) * W->nx[NX(g, i - 1, j, k)] +
+                                f3(g, W->fce, i - 1, j, k, 2) * W->ny[NX(g, i - 1, j, k)] +
+                                f3(g, W->fce, i - 1, j, k, 3) * W->nz[NX(g, i - 1, j, k)] +
+                                W->phi[NX(g, i - 1, j, k)];
+              const double d2 = (f3(g, W->fce, i - 1, j, k, 1) - UDX(g, i - 1)) * W->nx[NX(g, i, j, k)] +
+                                f3(g, W->fce, i - 1, j, k, 2) * W->ny[NX(g, i, j, k)] +
+                                f3(g, W->fce, i - 1, j, k, 3) * W->nz[NX(g, i, j, k)] +
+                                W->phi[NX(g, i, j, k)];
+              cut_tests(g, U, k < K, i - 1, j, k + 1, i - 1, j, k, un, &uw, 0.5 * (d1 + d2), &dc, 0);
+            } else {
+              const double Sz = U->szt[H(i - 1, j, k)];
+              eta = clamp_eta(fabs(f3(g, W->fce, i - 1, j, k, 3) + WDZ(g, k) / 2.0 -
+                                   cent(g, U, i - 1, j, k, 3)) / Sz);
+              uw = (1.0 - eta) * un[H(i - 1, j, k)] + eta * un[H(i - 1, j, k + 1)];
+            }
+            eta = W->etae[H(i - 1, j, k)];
+            const double ww = (1.0 - eta) * wn[H(i - 1, j, k)] + eta * wn[H(i, j, k)];
+            flux = uw * ww * W->ale[H(i - 1, j, k)] * W->ee[H(i - 1, j, k)] * WDY(g, j) * WDZ(g, k);
+          }
+          cfew[ew_ix(g, i, j, k, 3)] = flux;
+        }
+  }
+  /* ---- iv = 1 (:2409-2520) ---- */
+  {
+    delh_carry dc = {0, 0.0};
+    for (int i = 1; i <= I; ++i)
+      for (int j = 1; j <= J + 1; ++j)
+        for (int k = 1; k <= K; ++k) {
+          double vs, eta, flux;
+          /* u momentum, convective v (:2410-2457) */
+          vs = 0.5 * (vn[H(i, j - 1, k)] + vn[H(i + 1, j - 1, k)]);
+          if (j > J) {
+            flux = vs * bcval(g, bcv, 3, i, k) * U->ne[H(i, j - 1, k)] * UDX(g, i) * UDZ(g, k);
+          } else if (j == 1) {
+            flux = vs * bcval(g, bcv, 2, i, k) * U->ne[H(i, j, k)] * UDX(g, i) * UDZ(g, k);
+          } else {
+            if (pne[H(i, j - 1, k)] <= 1.0e-5 || pne[H(i + 1, j - 1, k)] <= 1.0e-5) {
+              const double d1 = f3(g, U->fcn, i, j - 1, k, 1) * U->nx[NX(g, i, j - 1, k)] +
+                                f3(g, U->fcn, i, j - 1, k, 2) * U->ny[NX(g, i, j - 1, k)] +
+                                f3(g, U->fcn, i, j - 1, k, 3) * U->nz[NX(g, i, j - 1, k)] +
+                                U->phi[NX(g, i, j - 1, k)];
+              const double d2 = f3(g, U->fcn, i, j - 1, k, 1) * U->nx[NX(g, i, j, k)] +
+                                (f3(g, U->fcn, i, j - 1, k, 2) - VDY(g, j - 1)) * U->ny[NX(g, i, j, k)] +
+                                f3(g, U->fcn, i, j - 1, k, 3) * U->nz[NX(g, i, j, k)] +
+                                U->phi[NX(g, i, j, k)];
+              cut_tests(g, V, i < I, i, j - 1, k, i + 1, j - 1, k, vn, &vs, 0.5 * (d1 + d2), &dc, 0);
+            } else {
+              const double Sx = V->sxe[H(i, j - 1, k)];
+              eta = clamp_eta(fabs(f3(g, U->fcn, i, j - 1, k, 1) + 0.5 * UDX(g, i) -
+                                   cent(g, V, i, j - 1, k, 1)) / Sx);
+              vs = (1.0 - eta) * vn[H(i, j - 1, k)] + eta * vn[H(i + 1, j - 1, k)];
+            }
+            eta = U->etan[H(i, j - 1, k)];
+            const double us = (1.0 - eta) * un[H(i, j - 1, k)] + eta * un[H(i, j, k)];
+            flux = us * vs * U->aln[H(i, j - 1, k)] * U->ne[H(i, j - 1, k)] * UDX(g, i) * UDZ(g, k);
+          }
+          cfns[ns_ix(g, i, j, k, 1)] = flux;
+          /* v momentum, convective v (:2458-2471) */
+          vs = 0.5 * (vn[H(i, j, k)] + vn[H(i, j - 1, k)]);
+          if (j >= J) {
+            flux = vs * vs * V->ne[H(i, j - 1, k)] * VDX(g, i) * VDZ(g, k);
+          } else if (j == 1) {
+            flux = vs * vs * V->ne[H(i, j, k)] * VDX(g, i) * VDZ(g, k);
+          } else {
+            eta = V->etan[H(i, j - 1, k)];
+            vs = (1.0 - eta) * vn[H(i, j - 1, k)] + eta * vn[H(i, j, k)];
+            const double a = vs * V->aln[H(i, j - 1, k)];
+            flux = a * a * V->ne[H(i, j - 1, k)] * VDX(g, i) * VDZ(g, k);
+          }
+          cfns[ns_ix(g, i, j, k, 2)] = flux;
+          /* w momentum, convective v (:2472-2519) */
+          vs = 0.5 * (vn[H(i, j - 1, k)] + vn[H(i, j - 1, k + 1)]);
+          if (j > J) {
+            flux = vs * bcval(g, bcv, 15, i, k) * W->ne[H(i, j - 1, k)] * WDX(g, i) * WDZ(g, k);
+          } else if (j == 1) {
+            flux = vs * bcval(g, bcv, 14, i, k) * W->ne[H(i, j, k)] * WDX(g, i) * WDZ(g, k);
+          } else {
+            if (pne[H(i, j - 1, k)] <= 1.0e-5 || pne[H(i, j - 1, k + 1)] <= 1.0e-5) {
+              const double d1 = f3(g, W->fcn, i, j - 1, k, 1) * W->nx[NX(g, i, j - 1, k)] +
+                                f3(g, W->fcn, i, j - 1, k, 2) * W->ny[NX(g, i, j - 1, k)] +
+                                f3(g, W->fcn, i, j - 1, k, 3) * W->nz[NX(g, i, j - 1, k)] +
+                                W->phi[NX(g, i, j - 1, k)];
+              const double d2 = f3(g, W->fcn, i, j - 1, k, 1) * W->nx[NX(g, i, j, k)] +
+                                (f3(g, W->fcn, i, j - 1, k, 2) - WDY(g, j - 1)) * W->ny[NX(g, i, j, k)] +
+                                f3(g, W->fcn, i, j - 1, k, 3) * W->nz[NX(g, i, j, k)] +
+                                W->phi[NX(g, i, j, k)];
+              cut_tests(g, V, k < K, i, j - 1, k, i, j - 1, k + 1, vn, &vs, 0.5 * (d1 + d2), &dc, 0);
+            } else {
+              const double Sz = V->szt[H(i, j - 1, k)];
+              eta = clamp_eta(fabs(f3(g, W->fcn, i, j - 1, k, 3) + 0.5 * WDZ(g, k) -
+                                   cent(g, W, i, j - 1, k, 3)) / Sz);
+              vs = (1.0 - eta) * vn[H(i, j - 1, k)] + eta * vn[H(i, j - 1, k + 1)];
+            }
+            eta = W->etan[H(i, j - 1, k)];
+            const double ws = (1.0 - eta) * wn[H(i, j - 1, k)] + eta * wn[H(i, j, k)];
+            flux = vs * ws * W->aln[H(i, j - 1, k)] * W->ne[H(i, j - 1, k)] * WDX(g, i) * WDZ(g, k);
+          }
+          cfns[ns_ix(g, i, j, k, 3)] = flux;
+        }
+  }
+  /* ---- iw = 1 (:2521-2639) ---- */
+  {
+    delh_carry dc = {0, 0.0};
+    for (int i = 1; i <= I; ++i)
+      for (int j = 1; j <= J; ++j)
+        for (int k = 1; k <= K + 1; ++k) {
+          double wb, eta, flux;
+          /* u momentum, convective w: the average takes vn12(i+1,j,k-1)
+           * (:2523) */
+          wb = 0.5 * (wn[H(i, j, k - 1)] + vn[H(i + 1, j, k - 1)]);
+          if (k > K) {
+            flux = wb * bcval(g, bcv, 5, i, j) * U->te[H(i, j, k - 1)] * UDX(g, i) * UDY(g, j);
+          } else if (k == 1) {
+            flux = wb * bcval(g, bcv, 4, i, j) * U->te[H(i, j, k)] * UDX(g, i) * UDY(g, j);
+          } else {
+            if (pte[H(i, j, k - 1)] <= 1.0e-5 || pte[H(i + 1, j, k - 1)] <= 1.0e-5) {
+              /* (FCT(:,2) - dz) * ny, as :2537-2538 write it */
+              const double d1 = f3(g, U->fct, i, j, k - 1, 1) * U->nx[NX(g, i, j, k - 1)] +
+                                f3(g, U->fct, i, j, k - 1, 2) * U->ny[NX(g, i, j, k - 1)] +
+                                f3(g, U->fct, i, j, k - 1, 3) * U->nz[NX(g, i, j, k - 1)] +
+                                U->phi[NX(g, i, j, k - 1)];
+              const double d2 = f3(g, U->fct, i, j, k - 1, 1) * U->nx[NX(g, i, j, k)] +
+                                (f3(g, U->fct, i, j, k - 1, 2) - WDZ(g, k - 1)) * U->ny[NX(g, i, j, k)] +
+                                f3(g, U->fct, i, j, k - 1, 3) * U->nz[NX(g, i, j, k)] +
+                                U->phi[NX(g, i, j, k)];
+              cut_tests(g, W, i < I, i, j, k - 1, i + 1, j, k - 1, wn, &wb, 0.5 * (d1 + d2), &dc, 0);
+            } else {
+              const double Sx = W->sxe[H(i, j, k - 1)];
+              eta = clamp_eta(fabs(f3(g, U->fct, i, j, k - 1, 1) + 0.5 * UDX(g, i) -
+                                   cent(g, W, i, j, k - 1, 1)) / Sx);
+              wb = (1.0 - eta) * wn[H(i, j, k - 1)] + eta * wn[H(i + 1, j, k - 1)];
+            }
+            eta = U->etat[H(i, j, k - 1)];
+            const double ub = (1.0 - eta) * un[H(i, j, k - 1)] + eta * un[H(i, j, k)];
+            flux = wb * ub * U->alt[H(i, j, k - 1)] * U->te[H(i, j, k - 1)] * UDX(g, i) * UDY(g, j);
+          }
+          cftb[tb_ix(g, i, j, k, 1)] = flux;
+          /* v momentum, convective w: delh = delhec before the tests
+           * (:2596) */
+          wb = 0.5 * (wn[H(i, j + 1, k - 1)] + wn[H(i, j, k - 1)]);
+          if (k > K) {
+            flux = wb * bcval(g, bcv, 11, i, j) * V->te[H(i, j, k - 1)] * VDX(g, i) * VDY(g, j);
+          } else if (k == 1) {
+            flux = wb * bcval(g, bcv, 10, i, j) * V->te[H(i, j, k)] * VDX(g, i) * VDY(g, j);
+          } else {
+            if (pte[H(i, j, k - 1)] <= 1.0e-5 || pte[H(i, j + 1, k - 1)] <= 1.0e-5) {
+              /* (FCT(:,1) - dz) * nx, as :2592-2593 write it */
+              const double d1 = f3(g, V->fct, i, j, k - 1, 1) * V->nx[NX(g, i, j, k - 1)] +
+                                f3(g, V->fct, i, j, k - 1, 2) * V->ny[NX(g, i, j, k - 1)] +
+                                f3(g, V->fct, i, j, k - 1, 3) * V->nz[NX(g, i, j, k - 1)] +
+                                V->phi[NX(g, i, j, k - 1)];
+              const double d2 = (f3(g, V->fct, i, j, k - 1, 1) - WDZ(g, k - 1)) * V->nx[NX(g, i, j, k)] +
+                                f3(g, V->fct, i, j, k - 1, 2) * V->ny[NX(g, i, j, k)] +
+                                f3(g, V->fct, i, j, k - 1, 3) * V->nz[NX(g, i, j, k)] +
+                                V->phi[NX(g, i, j, k)];
+              cut_tests(g, W, j < J, i, j + 1, k - 1, i, j, k - 1, wn, &wb, 0.5 * (d1 + d2), &dc, 1);
+            } else {
+              const double Sy = W->syn[H(i, j, k - 1)];
+              eta = clamp_eta(fabs(f3(g, V->fct, i, j, k - 1, 2) + 0.5 * VDY(g, j) -
+                                   cent(g, W, i, j, k - 1, 2)) / Sy);
+              wb = (1.0 - eta) * wn[H(i, j, k - 1)] + eta * wn[H(i, j + 1, k - 1)];
+            }
+            eta = V->etat[H(i, j, k - 1)];
+            const double vb = (1.0 - eta) * vn[H(i, j, k - 1)] + eta * vn[H(i, j, k)];
+            flux = wb * vb * V->alt[H(i, j, k - 1)] * V->te[H(i, j, k - 1)] * VDX(g, i) * VDY(g, j);
+          }
+          cftb[tb_ix(g, i, j, k, 2)] = flux;
+          /* w momentum, convective w (:2625-2638) */
+          wb = 0.5 * (wn[H(i, j, k)] + wn[H(i, j, k - 1)]);
+          if (k >= K) {
+            flux = wb * wb * W->te[H(i, j, k - 1)] * WDX(g, i) * WDY(g, j);
+          } else if (k == 1) {
+            flux = wb * wb * W->te[H(i, j, k)] * WDX(g, i) * WDY(g, j);
+          } else {
+            eta = W->etat[H(i, j, k - 1)];
+            wb = (1.0 - eta) * wn[H(i, j, k - 1)] + eta * wn[H(i, j, k)];
+            const double a = wb * W->alt[H(i, j, k - 1)];
+            flux = a * a * W->te[H(i, j, k - 1)] * WDX(g, i) * WDY(g, j);
+          }
+          cftb[tb_ix(g, i, j, k, 3)] = flux;
+        }
+  }
+#undef H
+}
+
 /* Dp halo copies (:778-800): each statement copies a whole plane, in this
  * order, so the final value of a halo entry is the interior value at the
  * clamped index of every direction, with the extra plane Imax (U), Jmax (V)
@@ -505,7 +813,7 @@ int orc_predict(const orc_grid* g, const double* pee, const double* pne, const d
                 double* const pred[3], double* const dp[3]) {
   const int I = g->I, J = g->J, K = g->K;
   const int64_t N = (int64_t)I * J * K, Nh = (int64_t)(I + 2) * (J + 2) * (K + 2);
-  if (p->space_order != 1) return -1;
+  if (p->space_order != 1 && p->space_order != 2) return -1;
   /* un12 = TVar (TimeOrder 1, 3) or 1.5 TVar - 0.5 TVar_n (TimeOrder 2) (:149-159) */
   double* v12[3];
   for (int c = 0; c < 3; ++c) {
@@ -516,8 +824,13 @@ int orc_predict(const orc_grid* g, const double* pee, const double* pne, const d
   double* cfew = (double*)calloc((size_t)(I + 2) * J * K * 3, sizeof(double));
   double* cfns = (double*)calloc((size_t)I * (J + 2) * K * 3, sizeof(double));
   double* cftb = (double*)calloc((size_t)I * J * (K + 2) * 3, sizeof(double));
-  orc_conv_flux_first(g, pee, pne, pte, old[0], old[1], old[2], v12[0], v12[1], v12[2], bcv, cfew,
-                      cfns, cftb);
+  /* the first step always takes the first-order fluxes (:172-185) */
+  if (p->itt == 1 || p->space_order == 1)
+    orc_conv_flux_first(g, pee, pne, pte, old[0], old[1], old[2], v12[0], v12[1], v12[2], bcv,
+                        cfew, cfns, cftb);
+  else
+    orc_conv_flux_second(g, pee, pne, pte, old[0], old[1], old[2], v12[0], v12[1], v12[2], bcv,
+                         cfew, cfns, cftb);
   const double inv_fr2 = 1.0 / (p->fr * p->fr); /* 1.d0/Fr**2.d0 */
   for (int c = 0; c < 3; ++c) {
     const int iu = c == 0, iv = c == 1, iw = c == 2;

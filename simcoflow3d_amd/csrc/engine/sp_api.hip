@@ -1217,8 +1217,9 @@ sp_status sp_number_external_cell(sp_ctx* c, int iu, int iv, int iw, const int32
 
 static sp_status check_pred(sp_ctx* c, const sp_pred_inputs* in, const sp_pred_outputs* out) {
   if (!in || !out) return fail(c, SP_ERR_INVALID_ARG, "null predictor inputs/outputs");
-  if (in->space_order != 1)
-    return fail(c, SP_ERR_INVALID_ARG, "only SpaceOrder 1 (first-order upwind) is built");
+  if (in->space_order != 1 && in->space_order != 2)
+    return fail(c, SP_ERR_INVALID_ARG,
+                "SpaceOrder 1 (first-order upwind) and 2 (central) are built, not 3");
   if (in->time_order < 1 || in->time_order > 3)
     return fail(c, SP_ERR_INVALID_ARG, "time_order must be 1, 2 or 3");
   if (!(in->dt > 0.0) || !std::isfinite(in->dt))

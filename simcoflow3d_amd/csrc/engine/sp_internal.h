@@ -276,6 +276,8 @@ size_t clsvof_work_doubles(const Dims& D);
 sp_status predict_uvw(sp_ctx* c, const sp_pred_inputs& in, const sp_pred_outputs& out,
                       double* faces);
 size_t predict_face_doubles(const Dims& D);
+// doubles of the face fluxes alone (the start of that buffer)
+size_t predict_flux_doubles(const Dims& D);
 // ---- VTR output (sp_vtr.hip) ----
 sp_status write_vtr(sp_ctx* c, const char* dir, const char* prefix, int64_t itt,
                     const sp_vtr_fields& f, bool device);

@@ -1,6 +1,6 @@
 // The momentum predictor PredictingUVW (PredictUVW.f90:49-803) for
-// SpaceOrder 1: u*, v*, w* and the d-coefficients Dp that the projection
-// reads.
+// SpaceOrder 1 and 2: u*, v*, w* and the d-coefficients Dp that the
+// projection reads.
 //
 // DiffusiveFlux computes its coefficients and then zeroes them
 // (PredictUVW.f90:3672-3673), so every neighbour coefficient of the three
@@ -12,6 +12,8 @@
 //                 (:1338-1794): one thread per (i,j,k) writes the face
 //                 fluxes CFEW(i,j,k,1:3), CFNS(i,j,k,1:3), CFTB(i,j,k,1:3)
 //                 it owns (x, y and z faces at index i, j, k);
+//   k_pred_faces2 the SpaceOrder 2 fluxes, ModifiedConvectiveFlux
+//                 (:2253-2644), one thread per k-column (below);
 //   k_pred_cells  per U, V, W cell: the flux divergence, Adams-Bashforth
 //                 for TimeOrder 3 (:320-337), FluxDivOld, the wall friction
 //                 Sp (:404-410), aP and Dp (:1025-1037), the right-hand
@@ -485,6 +487,453 @@ __global__ void __launch_bounds__(256) k_pred_faces(PredArgs A) {
   if (i <= A.I && j <= A.J) flux_z(A, X, i, j, k);
 }
 
+// ---- SpaceOrder 2: ModifiedConvectiveFlux (:2253-2644) ----
+//
+// Central interpolation with the cut-cell corrections.  The reference loops
+// i outermost and k innermost (:2277-2279), and a cut face whose two
+// neighbour tests both fail divides by the local delh of the last cut face
+// of the same call that set one (the variable keeps its value between
+// iterations).  The GPU resolves that carry in passes per direction: one
+// thread walks each piece of kPieceK planes of a k-column (i, j) -- the
+// innermost loop -- and records the piece's last delh; one thread per column
+// chains its pieces; one workgroup scans the columns in loop order.  The
+// piece walk also records the carry into every iteration within its piece,
+// so the flux pass runs one thread per face set, as the first-order kernel
+// does, each starting from its iteration's carry.  Before any cut face of the call has set delh the reference
+// reads an undefined value; the face's own delhec is taken there (the
+// first-order routine's choice, :1415).  x**2.d0 is x*x, the exact square.
+
+struct Carry {
+  int have;
+  double v;
+};
+
+// The neighbour tests of a cut face (cells a then b), the delh carry and the
+// scaled convective velocity cv * delhec / (delh + epsi).  `avail` false
+// where the cells lie beyond the arrays (no neighbour, as the first-order
+// routine's guards read them); `init`: delh = delhec before the tests
+// (:2596).
+__device__ inline double cut2(const PredArgs& A, const Ix& X, const PredCell& C, bool avail,
+                              int ai, int aj, int ak, int bi, int bj, int bk, int q, double cv,
+                              double dhec, Carry& cr, bool init) {
+#pragma clang fp contract(off)
+  double delh = 0.0;
+  bool set = init;
+  if (init) delh = dhec;
+  if (avail && usable(X, C, ai, aj, ak) && C.vof[X.c(ai, aj, ak)] >= C.vof[X.c(bi, bj, bk)]) {
+    delh = plane_dist(X, C, ai, aj, ak);
+    cv = v12(A, q, X.h(ai, aj, ak));
+    set = true;
+  }
+  if (avail && usable(X, C, bi, bj, bk) && C.vof[X.c(bi, bj, bk)] > C.vof[X.c(ai, aj, ak)]) {
+    delh = plane_dist(X, C, bi, bj, bk);
+    cv = v12(A, q, X.h(bi, bj, bk));
+    set = true;
+  }
+  if (set) {
+    cr.have = 1;
+    cr.v = delh;
+  } else {
+    delh = cr.have ? cr.v : dhec;
+  }
+  return cv * dhec / (delh + kEps);
+}
+
+__device__ inline double clamp_eta(double eta) { return fabs(eta) >= 1.0 ? 0.5 : eta; }
+
+// x faces, iteration (i, j, k) of the iu = 1 call (:2280-2408)
+template <bool WRITE>
+__device__ void flux2_x(const PredArgs& A, const Ix& X, int i, int j, int k, Carry& cr) {
+#pragma clang fp contract(off)
+  const PredCell &U = A.old[0], &V = A.old[1], &W = A.old[2];
+  const int I = A.I, J = A.J, K = A.K;
+  double uw, eta, flux;
+  if (WRITE) {  // u momentum, convective u
+    const double dyu = A.dyp[j - 1], dzu = A.dzp[k - 1];
+    const double ui = v12(A, 0, X.h(i, j, k)), um = v12(A, 0, X.h(i - 1, j, k));
+    uw = 0.5 * (ui + um);
+    if (i == 1) {
+      flux = uw * uw * U.ee[X.h(i, j, k)] * dyu * dzu;
+    } else if (i >= I) {
+      flux = uw * uw * U.ee[X.h(i - 1, j, k)] * dyu * dzu;
+    } else {
+      eta = U.etae[X.h(i - 1, j, k)];
+      uw = (1.0 - eta) * um + eta * ui;
+      const double a = uw * U.ale[X.h(i - 1, j, k)];
+      flux = a * a * U.ee[X.h(i - 1, j, k)] * dyu * dzu;
+    }
+    A.cfew[X.ew(i, j, k, 0)] = flux;
+  }
+  {  // v momentum, convective u
+    const double dyv = A.dyv[j - 1], dzv = A.dzp[k - 1];
+    uw = 0.5 * (v12(A, 0, X.h(i - 1, j + 1, k)) + v12(A, 0, X.h(i - 1, j, k)));
+    if (i > I) {
+      flux = uw * bcval(A, 7, j, k) * V.ee[X.h(i - 1, j, k)] * dyv * dzv;
+    } else if (i == 1) {
+      flux = uw * bcval(A, 6, j, k) * V.ee[X.h(i, j, k)] * dyv * dzv;
+    } else {
+      if (A.pee[X.h(i - 1, j, k)] <= 1.0e-5 || A.pee[X.h(i - 1, j + 1, k)] <= 1.0e-5) {
+        const double dhec = cut_dhec(X, V, V.fce, i - 1, j, k, i - 1, j, k, i, j, k, 0,
+                                     A.dxu[i - 2]);
+        uw = cut2(A, X, U, j < J, i - 1, j + 1, k, i - 1, j, k, 0, uw, dhec, cr, false);
+      } else {
+        const double Sy = U.syn[X.h(i - 1, j, k)];
+        eta = clamp_eta(fabs(V.fce[X.h(i - 1, j, k) + X.Nh] + dyv / 2.0 -
+                             U.cent[X.c(i - 1, j, k) + X.N]) / Sy);
+        uw = (1.0 - eta) * v12(A, 0, X.h(i - 1, j, k)) + eta * v12(A, 0, X.h(i - 1, j + 1, k));
+      }
+      eta = V.etae[X.h(i - 1, j, k)];
+      const double vw = (1.0 - eta) * v12(A, 1, X.h(i - 1, j, k)) + eta * v12(A, 1, X.h(i, j, k));
+      flux = uw * vw * V.ale[X.h(i - 1, j, k)] * V.ee[X.h(i - 1, j, k)] * dyv * dzv;
+    }
+    if (WRITE) A.cfew[X.ew(i, j, k, 1)] = flux;
+  }
+  {  // w momentum, convective u
+    const double dyw = A.dyp[j - 1], dzw = A.dzw[k - 1];
+    uw = 0.5 * (v12(A, 0, X.h(i - 1, j, k + 1)) + v12(A, 0, X.h(i - 1, j, k)));
+    if (i > I) {
+      flux = uw * bcval(A, 13, j, k) * W.ee[X.h(i - 1, j, k)] * dyw * dzw;
+    } else if (i == 1) {
+      flux = uw * bcval(A, 12, j, k) * W.ee[X.h(i, j, k)] * dyw * dzw;
+    } else {
+      if (A.pee[X.h(i - 1, j, k)] <= 1.0e-5 || A.pee[X.h(i - 1, j, k + 1)] <= 1.0e-5) {
+        const double dhec = cut_dhec(X, W, W.fce, i - 1, j, k, i - 1, j, k, i, j, k, 0,
+                                     A.dxu[i - 2]);
+        uw = cut2(A, X, U, k < K, i - 1, j, k + 1, i - 1, j, k, 0, uw, dhec, cr, false);
+      } else {
+        const double Sz = U.szt[X.h(i - 1, j, k)];
+        eta = clamp_eta(fabs(W.fce[X.h(i - 1, j, k) + 2 * X.Nh] + dzw / 2.0 -
+                             U.cent[X.c(i - 1, j, k) + 2 * X.N]) / Sz);
+        uw = (1.0 - eta) * v12(A, 0, X.h(i - 1, j, k)) + eta * v12(A, 0, X.h(i - 1, j, k + 1));
+      }
+      eta = W.etae[X.h(i - 1, j, k)];
+      const double ww = (1.0 - eta) * v12(A, 2, X.h(i - 1, j, k)) + eta * v12(A, 2, X.h(i, j, k));
+      flux = uw * ww * W.ale[X.h(i - 1, j, k)] * W.ee[X.h(i - 1, j, k)] * dyw * dzw;
+    }
+    if (WRITE) A.cfew[X.ew(i, j, k, 2)] = flux;
+  }
+}
+
+// y faces, iteration (i, j, k) of the iv = 1 call (:2409-2520)
+template <bool WRITE>
+__device__ void flux2_y(const PredArgs& A, const Ix& X, int i, int j, int k, Carry& cr) {
+#pragma clang fp contract(off)
+  const PredCell &U = A.old[0], &V = A.old[1], &W = A.old[2];
+  const int I = A.I, J = A.J, K = A.K;
+  double vs, eta, flux;
+  {  // u momentum, convective v
+    const double dxu = A.dxu[i - 1], dzu = A.dzp[k - 1];
+    vs = 0.5 * (v12(A, 1, X.h(i, j - 1, k)) + v12(A, 1, X.h(i + 1, j - 1, k)));
+    if (j > J) {
+      flux = vs * bcval(A, 3, i, k) * U.ne[X.h(i, j - 1, k)] * dxu * dzu;
+    } else if (j == 1) {
+      flux = vs * bcval(A, 2, i, k) * U.ne[X.h(i, j, k)] * dxu * dzu;
+    } else {
+      if (A.pne[X.h(i, j - 1, k)] <= 1.0e-5 || A.pne[X.h(i + 1, j - 1, k)] <= 1.0e-5) {
+        const double dhec = cut_dhec(X, U, U.fcn, i, j - 1, k, i, j - 1, k, i, j, k, 1,
+                                     A.dyv[j - 2]);
+        vs = cut2(A, X, V, i < I, i, j - 1, k, i + 1, j - 1, k, 1, vs, dhec, cr, false);
+      } else {
+        const double Sx = V.sxe[X.h(i, j - 1, k)];
+        eta = clamp_eta(fabs(U.fcn[X.h(i, j - 1, k)] + 0.5 * dxu - V.cent[X.c(i, j - 1, k)]) / Sx);
+        vs = (1.0 - eta) * v12(A, 1, X.h(i, j - 1, k)) + eta * v12(A, 1, X.h(i + 1, j - 1, k));
+      }
+      eta = U.etan[X.h(i, j - 1, k)];
+      const double us = (1.0 - eta) * v12(A, 0, X.h(i, j - 1, k)) + eta * v12(A, 0, X.h(i, j, k));
+      flux = us * vs * U.aln[X.h(i, j - 1, k)] * U.ne[X.h(i, j - 1, k)] * dxu * dzu;
+    }
+    if (WRITE) A.cfns[X.ns(i, j, k, 0)] = flux;
+  }
+  if (WRITE) {  // v momentum, convective v
+    const double dxv = A.dxp[i - 1], dzv = A.dzp[k - 1];
+    const double vj = v12(A, 1, X.h(i, j, k)), vm = v12(A, 1, X.h(i, j - 1, k));
+    vs = 0.5 * (vj + vm);
+    if (j >= J) {
+      flux = vs * vs * V.ne[X.h(i, j - 1, k)] * dxv * dzv;
+    } else if (j == 1) {
+      flux = vs * vs * V.ne[X.h(i, j, k)] * dxv * dzv;
+    } else {
+      eta = V.etan[X.h(i, j - 1, k)];
+      vs = (1.0 - eta) * vm + eta * vj;
+      const double a = vs * V.aln[X.h(i, j - 1, k)];
+      flux = a * a * V.ne[X.h(i, j - 1, k)] * dxv * dzv;
+    }
+    A.cfns[X.ns(i, j, k, 1)] = flux;
+  }
+  {  // w momentum, convective v
+    const double dxw = A.dxp[i - 1], dzw = A.dzw[k - 1];
+    vs = 0.5 * (v12(A, 1, X.h(i, j - 1, k)) + v12(A, 1, X.h(i, j - 1, k + 1)));
+    if (j > J) {
+      flux = vs * bcval(A, 15, i, k) * W.ne[X.h(i, j - 1, k)] * dxw * dzw;
+    } else if (j == 1) {
+      flux = vs * bcval(A, 14, i, k) * W.ne[X.h(i, j, k)] * dxw * dzw;
+    } else {
+      if (A.pne[X.h(i, j - 1, k)] <= 1.0e-5 || A.pne[X.h(i, j - 1, k + 1)] <= 1.0e-5) {
+        const double dhec = cut_dhec(X, W, W.fcn, i, j - 1, k, i, j - 1, k, i, j, k, 1,
+                                     A.dyp[j - 2]);
+        vs = cut2(A, X, V, k < K, i, j - 1, k, i, j - 1, k + 1, 1, vs, dhec, cr, false);
+      } else {
+        const double Sz = V.szt[X.h(i, j - 1, k)];
+        eta = clamp_eta(fabs(W.fcn[X.h(i, j - 1, k) + 2 * X.Nh] + 0.5 * dzw -
+                             W.cent[X.c(i, j - 1, k) + 2 * X.N]) / Sz);
+        vs = (1.0 - eta) * v12(A, 1, X.h(i, j - 1, k)) + eta * v12(A, 1, X.h(i, j - 1, k + 1));
+      }
+      eta = W.etan[X.h(i, j - 1, k)];
+      const double ws = (1.0 - eta) * v12(A, 2, X.h(i, j - 1, k)) + eta * v12(A, 2, X.h(i, j, k));
+      flux = vs * ws * W.aln[X.h(i, j - 1, k)] * W.ne[X.h(i, j - 1, k)] * dxw * dzw;
+    }
+    if (WRITE) A.cfns[X.ns(i, j, k, 2)] = flux;
+  }
+}
+
+// z faces, iteration (i, j, k) of the iw = 1 call (:2521-2639)
+template <bool WRITE>
+__device__ void flux2_z(const PredArgs& A, const Ix& X, int i, int j, int k, Carry& cr) {
+#pragma clang fp contract(off)
+  const PredCell &U = A.old[0], &V = A.old[1], &W = A.old[2];
+  const int I = A.I, J = A.J, K = A.K;
+  double wb, eta, flux;
+  {  // u momentum, convective w: the average takes vn12(i+1,j,k-1) (:2523)
+    const double dxu = A.dxu[i - 1], dyu = A.dyp[j - 1];
+    wb = 0.5 * (v12(A, 2, X.h(i, j, k - 1)) + v12(A, 1, X.h(i + 1, j, k - 1)));
+    if (k > K) {
+      flux = wb * bcval(A, 5, i, j) * U.te[X.h(i, j, k - 1)] * dxu * dyu;
+    } else if (k == 1) {
+      flux = wb * bcval(A, 4, i, j) * U.te[X.h(i, j, k)] * dxu * dyu;
+    } else {
+      if (A.pte[X.h(i, j, k - 1)] <= 1.0e-5 || A.pte[X.h(i + 1, j, k - 1)] <= 1.0e-5) {
+        // (FCT(:,2) - dz) * ny, as :2537-2538 write it
+        const double dhec = cut_dhec(X, U, U.fct, i, j, k - 1, i, j, k - 1, i, j, k, 1,
+                                     A.dzw[k - 2]);
+        wb = cut2(A, X, W, i < I, i, j, k - 1, i + 1, j, k - 1, 2, wb, dhec, cr, false);
+      } else {
+        const double Sx = W.sxe[X.h(i, j, k - 1)];
+        eta = clamp_eta(fabs(U.fct[X.h(i, j, k - 1)] + 0.5 * dxu - W.cent[X.c(i, j, k - 1)]) / Sx);
+        wb = (1.0 - eta) * v12(A, 2, X.h(i, j, k - 1)) + eta * v12(A, 2, X.h(i + 1, j, k - 1));
+      }
+      eta = U.etat[X.h(i, j, k - 1)];
+      const double ub = (1.0 - eta) * v12(A, 0, X.h(i, j, k - 1)) + eta * v12(A, 0, X.h(i, j, k));
+      flux = wb * ub * U.alt[X.h(i, j, k - 1)] * U.te[X.h(i, j, k - 1)] * dxu * dyu;
+    }
+    if (WRITE) A.cftb[X.tb(i, j, k, 0)] = flux;
+  }
+  {  // v momentum, convective w: delh = delhec before the tests (:2596)
+    const double dxv = A.dxp[i - 1], dyv = A.dyv[j - 1];
+    wb = 0.5 * (v12(A, 2, X.h(i, j + 1, k - 1)) + v12(A, 2, X.h(i, j, k - 1)));
+    if (k > K) {
+      flux = wb * bcval(A, 11, i, j) * V.te[X.h(i, j, k - 1)] * dxv * dyv;
+    } else if (k == 1) {
+      flux = wb * bcval(A, 10, i, j) * V.te[X.h(i, j, k)] * dxv * dyv;
+    } else {
+      if (A.pte[X.h(i, j, k - 1)] <= 1.0e-5 || A.pte[X.h(i, j + 1, k - 1)] <= 1.0e-5) {
+        // (FCT(:,1) - dz) * nx, as :2592-2593 write it
+        const double dhec = cut_dhec(X, V, V.fct, i, j, k - 1, i, j, k - 1, i, j, k, 0,
+                                     A.dzw[k - 2]);
+        wb = cut2(A, X, W, j < J, i, j + 1, k - 1, i, j, k - 1, 2, wb, dhec, cr, true);
+      } else {
+        const double Sy = W.syn[X.h(i, j, k - 1)];
+        eta = clamp_eta(fabs(V.fct[X.h(i, j, k - 1) + X.Nh] + 0.5 * dyv -
+                             W.cent[X.c(i, j, k - 1) + X.N]) / Sy);
+        wb = (1.0 - eta) * v12(A, 2, X.h(i, j, k - 1)) + eta * v12(A, 2, X.h(i, j + 1, k - 1));
+      }
+      eta = V.etat[X.h(i, j, k - 1)];
+      const double vb = (1.0 - eta) * v12(A, 1, X.h(i, j, k - 1)) + eta * v12(A, 1, X.h(i, j, k));
+      flux = wb * vb * V.alt[X.h(i, j, k - 1)] * V.te[X.h(i, j, k - 1)] * dxv * dyv;
+    }
+    if (WRITE) A.cftb[X.tb(i, j, k, 1)] = flux;
+  }
+  if (WRITE) {  // w momentum, convective w
+    const double dxw = A.dxp[i - 1], dyw = A.dyp[j - 1];
+    const double wk = v12(A, 2, X.h(i, j, k)), wm = v12(A, 2, X.h(i, j, k - 1));
+    wb = 0.5 * (wk + wm);
+    if (k >= K) {
+      flux = wb * wb * W.te[X.h(i, j, k - 1)] * dxw * dyw;
+    } else if (k == 1) {
+      flux = wb * wb * W.te[X.h(i, j, k)] * dxw * dyw;
+    } else {
+      eta = W.etat[X.h(i, j, k - 1)];
+      wb = (1.0 - eta) * wm + eta * wk;
+      const double a = wb * W.alt[X.h(i, j, k - 1)];
+      flux = a * a * W.te[X.h(i, j, k - 1)] * dxw * dyw;
+    }
+    A.cftb[X.tb(i, j, k, 2)] = flux;
+  }
+}
+
+// k-planes of one column piece: a thread walks one piece
+constexpr int kPieceK = 16;
+
+__host__ __device__ inline int pieces_of(int nk) { return (nk + kPieceK - 1) / kPieceK; }
+
+// The carries of one direction: per iteration (i, j, k) the carry within
+// its piece (index (i-1) + NI*((j-1) + NJ*(k-1))), per piece (stored
+// piece-major, piece p of column c at p * ncol + c, columns numbered in loop
+// order (i-1)*NJ + j-1) and per column.
+struct CarryBufs {
+  double* iv;       // iteration: last delh set by an earlier iteration of its piece
+  uint8_t* ih;      //   and whether there was one
+  double *pv, *ph;  // piece: last delh set in the piece, then the carry into it
+  double *cv, *ch;  // column: last delh set in the column, then the carry into it
+};
+
+template <int D, bool WRITE>
+__device__ inline void flux2(const PredArgs& A, const Ix& X, int i, int j, int k, Carry& cr) {
+  if (D == 0) flux2_x<WRITE>(A, X, i, j, k, cr);
+  else if (D == 1) flux2_y<WRITE>(A, X, i, j, k, cr);
+  else flux2_z<WRITE>(A, X, i, j, k, cr);
+}
+
+// One thread per piece (i, j, p): planes k = p*kPieceK+1 .. of the k-column
+// (i, j) of direction D's loop.  Evaluates the cut-face tests of each
+// iteration in loop order and records the carry into every iteration and
+// the piece's last delh.
+template <int D>
+__global__ void __launch_bounds__(256) k_pred_carry_pieces(PredArgs A, CarryBufs B) {
+  const int NI = A.I + (D == 0), NJ = A.J + (D == 1), NK = A.K + (D == 2);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int p = blockIdx.z;
+  if (i > NI || j > NJ) return;
+  const int64_t ncol = (int64_t)NI * NJ, col = (int64_t)(i - 1) * NJ + (j - 1);
+  const Ix X = ix_of(A);
+  Carry cr{0, 0.0};
+  const int k0 = p * kPieceK + 1, k1 = min(NK, k0 + kPieceK - 1);
+  int64_t n = (i - 1) + (int64_t)NI * ((j - 1) + (int64_t)NJ * (k0 - 1));
+  for (int k = k0; k <= k1; ++k, n += ncol) {
+    B.ih[n] = (uint8_t)cr.have;
+    B.iv[n] = cr.v;
+    flux2<D, false>(A, X, i, j, k, cr);
+  }
+  const int64_t x = p * ncol + col;
+  B.ph[x] = cr.have ? 1.0 : 0.0;
+  B.pv[x] = cr.v;
+}
+
+// The fluxes of iteration (i, j, k) of direction D's loop from the carry
+// into it: within its piece, else into the piece, else into the column.
+template <int D>
+__device__ void face2(const PredArgs& A, const Ix& X, const CarryBufs& B, int i, int j, int k) {
+  const int NI = A.I + (D == 0), NJ = A.J + (D == 1);
+  const int64_t ncol = (int64_t)NI * NJ, col = (int64_t)(i - 1) * NJ + (j - 1);
+  const int64_t n = (i - 1) + (int64_t)NI * ((j - 1) + (int64_t)NJ * (k - 1));
+  Carry cr{0, 0.0};
+  if (B.ih[n]) {
+    cr.have = 1;
+    cr.v = B.iv[n];
+  } else {
+    const int64_t x = (int64_t)((k - 1) / kPieceK) * ncol + col;
+    if (B.ph[x] != 0.0) {
+      cr.have = 1;
+      cr.v = B.pv[x];
+    } else if (B.ch[col] != 0.0) {
+      cr.have = 1;
+      cr.v = B.cv[col];
+    }
+  }
+  flux2<D, true>(A, X, i, j, k, cr);
+}
+
+// One thread per (i, j, k) writes the x, y and z face fluxes it owns, as the
+// first-order kernel does.
+__global__ void __launch_bounds__(256) k_pred_faces2(PredArgs A, CarryBufs Bx, CarryBufs By,
+                                                     CarryBufs Bz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > A.I + 1 || j > A.J + 1 || k > A.K + 1) return;
+  const Ix X = ix_of(A);
+  if (j <= A.J && k <= A.K) face2<0>(A, X, Bx, i, j, k);
+  if (i <= A.I && k <= A.K) face2<1>(A, X, By, i, j, k);
+  if (i <= A.I && j <= A.J) face2<2>(A, X, Bz, i, j, k);
+}
+
+// One thread per column: the pieces' exclusive carries within the column
+// (in place) and the column's last delh.
+__global__ void __launch_bounds__(256) k_pred_piece_carry(CarryBufs B, int64_t ncol, int np) {
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  int h = 0;
+  double v = 0.0;
+  for (int p = 0; p < np; ++p) {
+    const int64_t x = p * ncol + col;
+    const bool xh = B.ph[x] != 0.0;
+    const double xv = B.pv[x];
+    B.ph[x] = h ? 1.0 : 0.0;
+    B.pv[x] = v;
+    if (xh) { h = 1; v = xv; }
+  }
+  B.ch[col] = h ? 1.0 : 0.0;
+  B.cv[col] = v;
+}
+
+// Exclusive scan of the column carries in loop order, in place: afterwards
+// column c holds the last delh set by a column before it (or none).  One
+// workgroup per direction walks the columns in tiles of 4096: coalesced
+// loads into LDS, four consecutive columns per thread, a scan over the
+// threads, the carry of the earlier tiles.
+constexpr int kScanTile = 4096;
+
+__global__ void __launch_bounds__(1024) k_pred_carry_scan(CarryBufs B0, CarryBufs B1, CarryBufs B2,
+                                                          int64_t n0, int64_t n1, int64_t n2) {
+  __shared__ double sv[kScanTile];
+  __shared__ int sh[kScanTile];
+  __shared__ double pv[1024];
+  __shared__ int ph[1024];
+  const int d = blockIdx.x, t = threadIdx.x;
+  const CarryBufs& B = d == 0 ? B0 : (d == 1 ? B1 : B2);
+  const int64_t n = d == 0 ? n0 : (d == 1 ? n1 : n2);
+  double* v = B.cv;
+  double* h = B.ch;
+  int carry_h = 0;
+  double carry_v = 0.0;
+  for (int64_t base = 0; base < n; base += kScanTile) {
+    for (int e = 0; e < 4; ++e) {
+      const int l = e * 1024 + t;
+      const int64_t x = base + l;
+      sh[l] = x < n ? (h[x] != 0.0) : 0;
+      sv[l] = x < n ? v[x] : 0.0;
+    }
+    __syncthreads();
+    int lh = 0;
+    double lv = 0.0;
+    for (int e = 0; e < 4; ++e)
+      if (sh[4 * t + e]) { lh = 1; lv = sv[4 * t + e]; }
+    ph[t] = lh;
+    pv[t] = lv;
+    __syncthreads();
+    // inclusive scan over the threads: no delh in a run takes the last before it
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int qh = t >= o ? ph[t - o] : 0;
+      const double qv = t >= o ? pv[t - o] : 0.0;
+      __syncthreads();
+      if (!ph[t] && qh) { ph[t] = 1; pv[t] = qv; }
+      __syncthreads();
+    }
+    int eh = t > 0 ? ph[t - 1] : 0;
+    double ev = t > 0 ? pv[t - 1] : 0.0;
+    if (!eh) { eh = carry_h; ev = carry_v; }
+    for (int e = 0; e < 4; ++e) {
+      const int l = 4 * t + e;
+      const int xh = sh[l];
+      const double xv = sv[l];
+      sh[l] = eh;
+      sv[l] = ev;
+      if (xh) { eh = 1; ev = xv; }
+    }
+    __syncthreads();
+    if (ph[1023]) { carry_h = 1; carry_v = pv[1023]; }
+    for (int e = 0; e < 4; ++e) {
+      const int l = e * 1024 + t;
+      const int64_t x = base + l;
+      if (x < n) {
+        h[x] = sh[l] ? 1.0 : 0.0;
+        v[x] = sv[l];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // One thread per cell (i, j, k): the U, V, W cells it owns.
 __global__ void __launch_bounds__(256) k_pred_cells(PredArgs A) {
 #pragma clang fp contract(off)
@@ -620,6 +1069,9 @@ sp_status predict_uvw(sp_ctx* c, const sp_pred_inputs& in, const sp_pred_outputs
   A.fdo = in.flux_div_old;
   const int64_t few = (int64_t)(D.I + 2) * D.J * D.K * 3, fns = (int64_t)D.I * (D.J + 2) * D.K * 3;
   A.cfew = faces; A.cfns = faces + few; A.cftb = faces + few + fns;
+  // SpaceOrder 2 from the second step on (the first step takes the
+  // first-order fluxes, :172-185)
+  const bool second = in.space_order == 2 && in.itt != 1;
   A.pred[0] = out.u; A.pred[1] = out.v; A.pred[2] = out.w;
   A.dp[0] = out.dp_u; A.dp[1] = out.dp_v; A.dp[2] = out.dp_w;
   A.dt = in.dt;
@@ -632,10 +1084,51 @@ sp_status predict_uvw(sp_ctx* c, const sp_pred_inputs& in, const sp_pred_outputs
   // per cell: old velocities (x3, x2 for TimeOrder 2), P apertures, the
   // old cells' apertures, interpolation and centre data read by the
   // non-cut branches -> 9 face fluxes
-  timer_begin(c, "pred_faces", NB * (8.0 * (3 + 3 + 9 + 4 + 6 + 6 + 6 + (A.extrapolate ? 3 : 0)) + 72.0), &slot);
-  hipLaunchKernelGGL(k_pred_faces, dim3((D.I + 1 + 63) / 64, (D.J + 1 + 3) / 4, D.K + 1), blk, 0,
-                     c->stream, A);
-  timer_end(c, slot);
+  if (!second) {
+    timer_begin(c, "pred_faces", NB * (8.0 * (3 + 3 + 9 + 4 + 6 + 6 + 6 + (A.extrapolate ? 3 : 0)) + 72.0), &slot);
+    hipLaunchKernelGGL(k_pred_faces, dim3((D.I + 1 + 63) / 64, (D.J + 1 + 3) / 4, D.K + 1), blk,
+                       0, c->stream, A);
+    timer_end(c, slot);
+  } else {
+    // the carries follow the face fluxes in the same buffer
+    const int np[3] = {pieces_of(D.K), pieces_of(D.K), pieces_of(D.K + 1)};
+    const int64_t nc[3] = {(int64_t)(D.I + 1) * D.J, (int64_t)D.I * (D.J + 1), (int64_t)D.I * D.J};
+    const int64_t nk[3] = {D.K, D.K, D.K + 1};
+    CarryBufs cb[3];
+    double* q = faces + predict_flux_doubles(D);
+    for (int d = 0; d < 3; ++d) {
+      cb[d].iv = q; q += nc[d] * nk[d];
+      cb[d].pv = q; q += nc[d] * np[d];
+      cb[d].ph = q; q += nc[d] * np[d];
+      cb[d].cv = q; q += nc[d];
+      cb[d].ch = q; q += nc[d];
+    }
+    uint8_t* qb = reinterpret_cast<uint8_t*>(q);
+    for (int d = 0; d < 3; ++d) {
+      cb[d].ih = qb;
+      qb += nc[d] * nk[d];
+    }
+    const dim3 g0((D.I + 1 + 63) / 64, (D.J + 3) / 4, np[0]),
+        g1((D.I + 63) / 64, (D.J + 1 + 3) / 4, np[1]), g2((D.I + 63) / 64, (D.J + 3) / 4, np[2]);
+    // latency-bound passes over the cut-face tests: no byte count is claimed
+    timer_begin(c, "pred_faces2_carry", 0.0, &slot);
+    hipLaunchKernelGGL(k_pred_carry_pieces<0>, g0, blk, 0, c->stream, A, cb[0]);
+    hipLaunchKernelGGL(k_pred_carry_pieces<1>, g1, blk, 0, c->stream, A, cb[1]);
+    hipLaunchKernelGGL(k_pred_carry_pieces<2>, g2, blk, 0, c->stream, A, cb[2]);
+    for (int d = 0; d < 3; ++d)
+      hipLaunchKernelGGL(k_pred_piece_carry, dim3((unsigned)((nc[d] + 255) / 256)), dim3(256), 0,
+                         c->stream, cb[d], nc[d], np[d]);
+    hipLaunchKernelGGL(k_pred_carry_scan, dim3(3), dim3(1024), 0, c->stream, cb[0], cb[1], cb[2],
+                       nc[0], nc[1], nc[2]);
+    timer_end(c, slot);
+    // the first-order kernel's bytes and the carry reads (9 B per face set)
+    timer_begin(c, "pred_faces2",
+                NB * (8.0 * (3 + 3 + 9 + 4 + 6 + 6 + 6 + (A.extrapolate ? 3 : 0)) + 72.0 + 27.0),
+                &slot);
+    hipLaunchKernelGGL(k_pred_faces2, dim3((D.I + 1 + 63) / 64, (D.J + 1 + 3) / 4, D.K + 1), blk,
+                       0, c->stream, A, cb[0], cb[1], cb[2]);
+    timer_end(c, slot);
+  }
   // per cell: 9 face fluxes, FluxDivOld, current vof/vofL/type/WlLh/delh,
   // old vof, velocities -> FluxDivOld, Dp, u*
   timer_begin(c, "pred_cells", NB * (72.0 + 48.0 + 3 * (8 + 8 + 4 + 8 + 8) + 24 + 24 + 48), &slot);
@@ -649,9 +1142,20 @@ sp_status predict_uvw(sp_ctx* c, const sp_pred_inputs& in, const sp_pred_outputs
   return e == hipSuccess ? SP_OK : hip_check(c, e, "predictor launch");
 }
 
-size_t predict_face_doubles(const Dims& D) {
+size_t predict_flux_doubles(const Dims& D) {
   return (size_t)(D.I + 2) * D.J * D.K * 3 + (size_t)D.I * (D.J + 2) * D.K * 3 +
          (size_t)D.I * D.J * (D.K + 2) * 3;
+}
+
+size_t predict_face_doubles(const Dims& D) {
+  // the face fluxes, then the SpaceOrder 2 carries (value and flag per
+  // k-column piece of each direction's loop)
+  const size_t p0 = (size_t)pieces_of(D.K), p2 = (size_t)pieces_of(D.K + 1);
+  const size_t c0 = (size_t)(D.I + 1) * D.J, c1 = (size_t)D.I * (D.J + 1), c2 = (size_t)D.I * D.J;
+  const size_t it = c0 * D.K + c1 * D.K + c2 * (D.K + 1);  // iterations of the three loops
+  // values, then the flags as bytes
+  return predict_flux_doubles(D) + it + 2 * (c0 * p0 + c1 * p0 + c2 * p2) + 2 * (c0 + c1 + c2) +
+         (it + 7) / 8;
 }
 
 }  // namespace sp

@@ -619,7 +619,8 @@ class Projector:
         return pos, ext.value
 
     def predict_uvw(self, p_areas, old, cur, fields, pred, dp, device=False):
-        """PredictingUVW (sp_predict_uvw) with SpaceOrder 1.  ``p_areas``:
+        """PredictingUVW (sp_predict_uvw) with SpaceOrder 1 or 2 (from
+        fields["params"]["space_order"]).  ``p_areas``:
         PCellO%EEArea, NEArea, TEArea; ``old``/``cur``: three dicts (U, V, W)
         keyed by configs.PRED_CELL_FIELDS (UCellO.. / UCell..); ``fields``: the
         dict of configs.predictor_fields (vel, vel_n, flux_div_old, bc_flag,
