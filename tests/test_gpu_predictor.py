@@ -110,6 +110,9 @@ CASES2 = [
     ((16, 16, 16), "cylinder", 15, False, 1, 2),
     ((12, 10, 9), "fluid", 16, True, 3, 2),
     ((64, 48, 40), "bodies", 17, True, 3, 3),
+    # a solid on the low side: cut faces read the carry before any set it
+    ((16, 12, 14), "axis_plane", 1, False, 1, 2),
+    ((16, 12, 14), "plane", 0, True, 3, 2),
 ]
 
 
@@ -138,6 +141,8 @@ def test_second_order_predictor_matches_oracle(gpu_available, case, device):
           flush=True)
     if shape != "fluid":
         assert carried > 0
+    if size == (16, 12, 14):
+        assert uncarried > 0
 
 
 def test_predictor_rejects_higher_space_order(gpu_available):
