@@ -178,7 +178,8 @@ static sp_status ensure_buffers(sp_ctx* c) {
   c->stale_cap = std::min<int64_t>(6 * N, std::max<int64_t>(1 << 20, N / 2));
   if ((s = dmalloc(c, &c->stale_list, 3 * c->stale_cap))) return s;
   if ((s = dmalloc(c, &c->col_last, 12 * (int64_t)D.I * D.J)) ||
-      (s = dmalloc(c, &c->col_prefix, 12 * (int64_t)D.I * D.J)))
+      (s = dmalloc(c, &c->col_prefix, 12 * (int64_t)D.I * D.J)) ||
+      (s = dmalloc(c, &c->gfm_cls, (int64_t)D.Nh)))
     return s;
   c->red_max_blocks = (uint32_t)std::max<int64_t>(
       4096, (int64_t)((D.I + 2 + 63) / 64) * ((D.J + 2 + 3) / 4) * (D.K + 2));

@@ -40,6 +40,7 @@ struct GfmArgs {
   uint32_t* stale_count;
   int64_t* col_last;       // [12][I*J] loop position of the last assignment
   int64_t* col_prefix;     // [12][I*J]
+  uint8_t* cls;            // GFM class of the interior cells (halo layout)
 };
 
 __device__ inline int64_t hstride(const GfmArgs& A, int dir) {
@@ -93,6 +94,7 @@ __global__ void __launch_bounds__(256) k_gfm(GfmArgs A) {
   for (int c = q.c0; c < q.c1; ++c, n += IJ, h += SXY) {
     const double fP = A.phiL[h];
     const int cP = gfm_class(A.vof[h], A.vofL[h], fP, A.epsi);
+    A.cls[h] = (uint8_t)cP;
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
       const int dir = f % 3;
@@ -136,9 +138,29 @@ __global__ void __launch_bounds__(256) k_gfm(GfmArgs A) {
   }
 }
 
+// Whether face f of 0-based cell (a,b,c) assigns BetaW (kWet) or BetaD
+// (kDry) in the reference's loop, from the classes k_gfm stored: both the
+// P cell and its neighbour classified (gfm_face's positive results).  A
+// neighbour in the halo is classified from its fields.
+__device__ inline int face_kind(const GfmArgs& A, int f, int a, int b, int c) {
+  const int dir = f % 3;
+  const int side = f < 3 ? -1 : 1;
+  const int64_t h = (int64_t)(a + 1) + (int64_t)A.SX * ((int64_t)(b + 1) + (int64_t)A.SY * (c + 1));
+  const int cP = A.cls[h];
+  if (cP == kNone) return kNone;
+  const int64_t q = h + side * hstride(A, dir);
+  const int coord = (dir == 0 ? a : (dir == 1 ? b : c)) + side;
+  const int lim = dir == 0 ? A.I : (dir == 1 ? A.J : A.K);
+  const int cQ = coord >= 0 && coord < lim ? (int)A.cls[q]
+                                           : gfm_class(A.vof[q], A.vofL[q], A.phiL[q], A.epsi);
+  return cQ == kNone ? kNone : cP;
+}
+
 // Per (column (i,j), face): loop position of the last cell of the column that
 // assigns BetaW (kind 1) and BetaD (kind 2), -1 if none.  Columns are the
-// innermost k loop of the reference (ProjectionP.f90:142-144).
+// innermost k loop of the reference (ProjectionP.f90:142-144).  One thread
+// per (face, column) walks the column's stored classes down from the top
+// until it has found both.
 __global__ void __launch_bounds__(256) k_gfm_col_last(GfmArgs A) {
   if (*A.stale_count == 0u) return;  // no face needs the previous beta
   const int64_t cols = (int64_t)A.I * A.J;
@@ -151,12 +173,17 @@ __global__ void __launch_bounds__(256) k_gfm_col_last(GfmArgs A) {
   const int a = (int)(u % A.I), b = (int)(u / A.I);
   const int64_t col = (int64_t)a * A.J + b;
   int64_t lastW = -1, lastD = -1;
-  for (int c = A.K - 1; c >= 0 && (lastW < 0 || lastD < 0); --c) {
-    double beta, pref;
-    const int r = gfm_face(A, f, a, b, c, beta, pref);
-    const int64_t pos = col * A.K + c;
-    if (r == kWet && lastW < 0) lastW = pos;
-    if (r == kDry && lastD < 0) lastD = pos;
+  // eight planes' classes in flight at a time
+  for (int c0 = A.K - 1; c0 >= 0 && (lastW < 0 || lastD < 0); c0 -= 8) {
+    int r[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) r[e] = c0 - e >= 0 ? face_kind(A, f, a, b, c0 - e) : kNone;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int64_t pos = col * A.K + (c0 - e);
+      if (r[e] == kWet && lastW < 0) lastW = pos;
+      if (r[e] == kDry && lastD < 0) lastD = pos;
+    }
   }
   A.col_last[(int64_t)(2 * f + 0) * cols + col] = lastW;
   A.col_last[(int64_t)(2 * f + 1) * cols + col] = lastD;
@@ -164,31 +191,51 @@ __global__ void __launch_bounds__(256) k_gfm_col_last(GfmArgs A) {
 
 // Exclusive prefix-max over columns (positions grow along the loop order, so
 // "last assignment before column c" is the maximum over earlier columns).
-// One workgroup of 1024 threads per (face, kind).
+// One workgroup of 1024 threads per (face, kind) walks the columns in tiles
+// of 4096: coalesced loads into LDS, four consecutive columns per thread, a
+// max-scan over the threads, the maximum of the earlier tiles.
 __global__ void __launch_bounds__(1024) k_gfm_col_prefix(GfmArgs A) {
   if (*A.stale_count == 0u) return;
+  constexpr int kTile = 4096;
+  __shared__ int64_t tile[kTile];
+  __shared__ int64_t sm[1024];
   const int64_t cols = (int64_t)A.I * A.J;
   const int64_t* src = A.col_last + (int64_t)blockIdx.x * cols;
   int64_t* dst = A.col_prefix + (int64_t)blockIdx.x * cols;
-  const int tid = threadIdx.x;
-  const int64_t chunk = (cols + 1023) / 1024;
-  const int64_t lo = tid * chunk, hi = lo + chunk < cols ? lo + chunk : cols;
-  int64_t m = -1;
-  for (int64_t q = lo; q < hi; ++q) m = src[q] > m ? src[q] : m;
-  __shared__ int64_t sm[1024];
-  sm[tid] = m;
-  __syncthreads();
-  // inclusive Hillis-Steele max-scan
-  for (int off = 1; off < 1024; off <<= 1) {
-    int64_t v = tid >= off ? sm[tid - off] : -1;
+  const int t = threadIdx.x;
+  int64_t carry = -1;
+  for (int64_t base = 0; base < cols; base += kTile) {
+    for (int e = 0; e < 4; ++e) {
+      const int l = e * 1024 + t;
+      tile[l] = base + l < cols ? src[base + l] : -1;
+    }
     __syncthreads();
-    sm[tid] = v > sm[tid] ? v : sm[tid];
+    int64_t m = -1;
+    for (int e = 0; e < 4; ++e) m = tile[4 * t + e] > m ? tile[4 * t + e] : m;
+    sm[t] = m;
     __syncthreads();
-  }
-  int64_t run = tid > 0 ? sm[tid - 1] : -1;
-  for (int64_t q = lo; q < hi; ++q) {
-    dst[q] = run;
-    run = src[q] > run ? src[q] : run;
+    // inclusive Hillis-Steele max-scan
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int64_t v = t >= off ? sm[t - off] : -1;
+      __syncthreads();
+      sm[t] = v > sm[t] ? v : sm[t];
+      __syncthreads();
+    }
+    int64_t run = t > 0 ? sm[t - 1] : -1;
+    run = carry > run ? carry : run;
+    for (int e = 0; e < 4; ++e) {
+      const int l = 4 * t + e;
+      const int64_t v = tile[l];
+      tile[l] = run;
+      run = v > run ? v : run;
+    }
+    __syncthreads();
+    carry = sm[1023] > carry ? sm[1023] : carry;
+    for (int e = 0; e < 4; ++e) {
+      const int l = e * 1024 + t;
+      if (base + l < cols) dst[base + l] = tile[l];
+    }
+    __syncthreads();
   }
 }
 
@@ -206,10 +253,8 @@ __global__ void __launch_bounds__(256) k_gfm_fixup(GfmArgs A) {
     const int64_t cols = (int64_t)A.I * A.J;
     const int64_t col = (int64_t)a * A.J + b;
     int64_t pos = -1;
-    for (int cc = c - 1; cc >= 0; --cc) {
-      double beta, pref;
-      if (gfm_face(A, f, a, b, cc, beta, pref) == kind) { pos = col * A.K + cc; break; }
-    }
+    for (int cc = c - 1; cc >= 0; --cc)
+      if (face_kind(A, f, a, b, cc) == kind) { pos = col * A.K + cc; break; }
     if (pos < 0) pos = A.col_prefix[(int64_t)(2 * f + kind - 1) * cols + col];
     double beta;
     if (pos >= 0) {
@@ -446,6 +491,7 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_out) {
     G.stale_count = &c->S->stale_count;
     G.col_last = c->col_last;
     G.col_prefix = c->col_prefix;
+    G.cls = c->gfm_cls;
     timer_begin(c, "gfm", (double)D.N * (3 * 8 + 3 * 8 + 6 * 8), &slot);
     hipLaunchKernelGGL(k_gfm, col_grid(D.I, D.J, D.K), blk, 0, c->stream, G);
     timer_end(c, slot);
@@ -459,7 +505,9 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_out) {
       hipLaunchKernelGGL(k_gfm_col_last, dim3((unsigned)((cols * 6 + 255) / 256)), dim3(256), 0,
                          c->stream, G);
       timer_end(c, slot);
+      timer_begin(c, "gfm_col_prefix", 0, &slot);
       hipLaunchKernelGGL(k_gfm_col_prefix, dim3(12), dim3(1024), 0, c->stream, G);
+      timer_end(c, slot);
       timer_begin(c, "gfm_fixup", 0, &slot);
       const int64_t fb = std::min<int64_t>((c->stale_cap + 255) / 256, 2048);
       hipLaunchKernelGGL(k_gfm_fixup, dim3((unsigned)fb), dim3(256), 0, c->stream, G);

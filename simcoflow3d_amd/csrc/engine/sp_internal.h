@@ -161,6 +161,7 @@ struct sp_ctx {
   int64_t stale_cap = 0;
   int64_t* col_last = nullptr;    // 12 * I*J loop positions
   int64_t* col_prefix = nullptr;  // 12 * I*J
+  uint8_t* gfm_cls = nullptr;     // GFM class of every interior cell (halo layout)
 
   // two-level reductions
   double* red_part = nullptr;
