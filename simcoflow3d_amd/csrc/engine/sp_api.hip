@@ -1268,7 +1268,8 @@ sp_status sp_predict_uvw(sp_ctx* c, const sp_pred_inputs* in, sp_pred_outputs* o
   const Dims& D = c->dm;
   const size_t N = (size_t)D.N, Nh = (size_t)D.Nh;
   void* faces;
-  if ((s = stage(c, 299, predict_face_doubles(D) * sizeof(double), &faces))) return s;
+  const bool second = in->space_order == 2 && in->itt != 1;
+  if ((s = stage(c, 299, predict_face_doubles(D, second) * sizeof(double), &faces))) return s;
   if (device) {
     if ((s = predict_uvw(c, *in, *out, static_cast<double*>(faces)))) return s;
   } else {

@@ -273,10 +273,11 @@ sp_status clsvof_scheme(sp_ctx* c, const sp_clsvof_inputs& in, const sp_clsvof_s
                         double* work);
 size_t clsvof_work_doubles(const Dims& D);
 // Momentum predictor (sp_predict.hip) on device arrays; faces holds
-// predict_face_doubles(D) doubles.
+// predict_face_doubles(D, second) doubles (second: SpaceOrder 2 after the
+// first step, which also needs the delh carries).
 sp_status predict_uvw(sp_ctx* c, const sp_pred_inputs& in, const sp_pred_outputs& out,
                       double* faces);
-size_t predict_face_doubles(const Dims& D);
+size_t predict_face_doubles(const Dims& D, bool second);
 // doubles of the face fluxes alone (the start of that buffer)
 size_t predict_flux_doubles(const Dims& D);
 // ---- VTR output (sp_vtr.hip) ----

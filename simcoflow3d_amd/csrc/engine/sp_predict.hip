@@ -1147,9 +1147,10 @@ size_t predict_flux_doubles(const Dims& D) {
          (size_t)D.I * D.J * (D.K + 2) * 3;
 }
 
-size_t predict_face_doubles(const Dims& D) {
-  // the face fluxes, then the SpaceOrder 2 carries (value and flag per
-  // k-column piece of each direction's loop)
+size_t predict_face_doubles(const Dims& D, bool second) {
+  if (!second) return predict_flux_doubles(D);
+  // the face fluxes, then the SpaceOrder 2 carries of each direction's
+  // loop: per iteration, per 16-plane piece and per k-column
   const size_t p0 = (size_t)pieces_of(D.K), p2 = (size_t)pieces_of(D.K + 1);
   const size_t c0 = (size_t)(D.I + 1) * D.J, c1 = (size_t)D.I * (D.J + 1), c2 = (size_t)D.I * D.J;
   const size_t it = c0 * D.K + c1 * D.K + c2 * (D.K + 1);  // iterations of the three loops
