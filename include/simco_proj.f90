@@ -172,6 +172,8 @@ module simco_proj
     type(c_ptr) :: ee_area = c_null_ptr, ne_area = c_null_ptr, te_area = c_null_ptr
     type(c_ptr) :: fce = c_null_ptr, fcn = c_null_ptr, fct = c_null_ptr
     type(c_ptr) :: cell_cent = c_null_ptr, wllh = c_null_ptr, delh = c_null_ptr
+    ! 1 when phi, nx, ny, nz, vof, vofL carry the halo (0:Imax+1,..)
+    integer(c_int32_t) :: halo_fields = 0
   end type sp_cell_geom
 
   ! NewCellFace members of one U/V/W cell besides sp_cell_geom: c_loc of
@@ -190,6 +192,8 @@ module simco_proj
     type(c_ptr) :: dx = c_null_ptr, dy = c_null_ptr, dz = c_null_ptr
     type(c_ptr) :: phi_l = c_null_ptr, nx_l = c_null_ptr, ny_l = c_null_ptr
     type(c_ptr) :: nz_l = c_null_ptr, vof_l = c_null_ptr
+    ! 1 when phiL, nxL, nyL, nzL, vofL carry the halo (0:Imax+1,..)
+    integer(c_int32_t) :: halo_fields = 0
   end type sp_liquid_field
 
   ! PredictingUVW inputs: c_loc of PCellO%EEArea.., of type(sp_cell_geom) /

@@ -306,7 +306,11 @@ sp_status sp_write_vtk(sp_ctx* ctx, const char* dir, int64_t itt, const sp_vtk_f
  * Cell arrays are (1:Imax,1:Jmax,1:Kmax); ee/ne/te_area are halo arrays;
  * fce/fcn/fct are TCell%FCE/FCN/FCT (0:Imax+1,0:Jmax+1,0:Kmax+1,3), the
  * component index slowest; cell_cent is TCell%Cell_Cent (1:Imax,..,3).
- * Allocated as in AllocateVar (Main.f90:229-399). */
+ * phi, nx, ny, nz, vof and vof_l are cell arrays as AllocateVar allocates
+ * them (Main.f90:229-399) when halo_fields = 0, and halo arrays
+ * (0:Imax+1,0:Jmax+1,0:Kmax+1) as the test drivers allocate them
+ * (Test/FlowSphere2Phases/FlowSphere2Phases.f90:316-322, 431-440) when
+ * halo_fields = 1; only their interior entries are read or written. */
 typedef struct sp_cell_geom {
   const double *dx, *dy, *dz;         /* TGrid%dx, dy, dz (cell arrays) */
   const double *phi, *nx, *ny, *nz;   /* TCell%phi, nx, ny, nz: solid level set, normal */
@@ -318,6 +322,7 @@ typedef struct sp_cell_geom {
   double* cell_cent;                  /* TCell%Cell_Cent: written */
   double* wllh;                       /* TCell%WlLh: written */
   double* delh;                       /* TCell%delh: updated where the reference sets it */
+  int32_t halo_fields;                /* 1: phi, nx, ny, nz, vof, vof_l carry the halo */
 } sp_cell_geom;
 
 /* GridPreprocess(PGrid,UGrid,VGrid,WGrid, PCell,UCell,VCell,WCell)
@@ -400,11 +405,15 @@ sp_status sp_number_external_cell_noalloc(sp_ctx* ctx, int iu, int iv, int iw,
                                           int32_t* ext_cell);
 
 /* The liquid fields of one grid for ComputeUVWLiquidField: TGrid%dx, dy,
- * dz and TCell%phiL, nxL, nyL, nzL, vofL (cell arrays (1:Imax,1:Jmax,
- * 1:Kmax)). */
+ * dz (cell arrays (1:Imax,1:Jmax,1:Kmax)) and TCell%phiL, nxL, nyL, nzL,
+ * vofL: cell arrays as AllocateVar allocates them (Main.f90:269-270,
+ * 383-387) when halo_fields = 0, halo arrays (0:Imax+1,..) as the test
+ * drivers and the projection's sp_inputs take them when halo_fields = 1
+ * (only the interior entries are read or written). */
 typedef struct sp_liquid_field {
   const double *dx, *dy, *dz;
   double *phi_l, *nx_l, *ny_l, *nz_l, *vof_l;
+  int32_t halo_fields;
 } sp_liquid_field;
 
 /* ComputeUVWLiquidField(PGrid,PCell,UCell,VCell,WCell,UGrid,VGrid,WGrid)

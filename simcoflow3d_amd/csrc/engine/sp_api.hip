@@ -796,6 +796,93 @@ sp_status sp_write_vtk(sp_ctx* c, const char* dir, int64_t itt, const sp_vtk_fie
   return s;
 }
 
+// The members of sp_cell_geom a host may allocate with the halo
+// (halo_fields = 1), in struct order: phi, nx, ny, nz, vof, vof_l.  The
+// kernels index cell arrays, so the entry points convert at the boundary.
+constexpr int kGeomHalo = 6;
+constexpr unsigned kGeomVof = 1u << 4, kGeomVofL = 1u << 5;
+
+static const double* geom_halo_get(const sp_cell_geom& g, int q) {
+  const double* const m[kGeomHalo] = {g.phi, g.nx, g.ny, g.nz, g.vof, g.vof_l};
+  return m[q];
+}
+static void geom_halo_set(sp_cell_geom& g, int q, double* p) {
+  switch (q) {
+    case 0: g.phi = p; break;
+    case 1: g.nx = p; break;
+    case 2: g.ny = p; break;
+    case 3: g.nz = p; break;
+    case 4: g.vof = p; break;
+    default: g.vof_l = p; break;
+  }
+}
+// Host arrays of the halo members are uploaded whole (Nh values).
+static size_t geom_count(const sp_cell_geom& g, size_t N, size_t Nh) {
+  return g.halo_fields ? Nh : N;
+}
+// Device pointers: the halo members of g replaced by cell-layout copies in
+// staging slots slot .. slot + 5.
+static sp_status geom_cells_in(sp_ctx* c, size_t slot, sp_cell_geom& g) {
+  if (!g.halo_fields) return SP_OK;
+  for (int q = 0; q < kGeomHalo; ++q) {
+    const double* h = geom_halo_get(g, q);
+    if (!h) continue;
+    void* b;
+    sp_status s = stage(c, slot + q, (size_t)c->dm.N * sizeof(double), &b);
+    if (s || (s = halo_to_cell(c, h, static_cast<double*>(b)))) return s;
+    geom_halo_set(g, q, static_cast<double*>(b));
+  }
+  return SP_OK;
+}
+// After the kernels: the members in `mask` (bit q) back into the halo
+// arrays of `orig` (interior entries only).
+static sp_status geom_cells_out(sp_ctx* c, const sp_cell_geom& orig, const sp_cell_geom& conv,
+                                unsigned mask) {
+  if (!orig.halo_fields) return SP_OK;
+  for (int q = 0; q < kGeomHalo; ++q) {
+    const double* h = geom_halo_get(orig, q);
+    if (!((mask >> q) & 1u) || !h) continue;
+    sp_status s = cell_to_halo(c, geom_halo_get(conv, q), const_cast<double*>(h));
+    if (s) return s;
+  }
+  return SP_OK;
+}
+
+// The same for sp_liquid_field: phi_l, nx_l, ny_l, nz_l, vof_l.
+constexpr int kLiqHalo = 5;
+static double* liq_halo_get(const sp_liquid_field& g, int q) {
+  double* const m[kLiqHalo] = {g.phi_l, g.nx_l, g.ny_l, g.nz_l, g.vof_l};
+  return m[q];
+}
+static void liq_halo_set(sp_liquid_field& g, int q, double* p) {
+  switch (q) {
+    case 0: g.phi_l = p; break;
+    case 1: g.nx_l = p; break;
+    case 2: g.ny_l = p; break;
+    case 3: g.nz_l = p; break;
+    default: g.vof_l = p; break;
+  }
+}
+static sp_status liq_cells_in(sp_ctx* c, size_t slot, sp_liquid_field& g) {
+  if (!g.halo_fields) return SP_OK;
+  for (int q = 0; q < kLiqHalo; ++q) {
+    void* b;
+    sp_status s = stage(c, slot + q, (size_t)c->dm.N * sizeof(double), &b);
+    if (s || (s = halo_to_cell(c, liq_halo_get(g, q), static_cast<double*>(b)))) return s;
+    liq_halo_set(g, q, static_cast<double*>(b));
+  }
+  return SP_OK;
+}
+static sp_status liq_cells_out(sp_ctx* c, const sp_liquid_field& orig,
+                               const sp_liquid_field& conv) {
+  if (!orig.halo_fields) return SP_OK;
+  for (int q = 0; q < kLiqHalo; ++q) {
+    sp_status s = cell_to_halo(c, liq_halo_get(conv, q), liq_halo_get(orig, q));
+    if (s) return s;
+  }
+  return SP_OK;
+}
+
 static sp_status check_geom(sp_ctx* c, const sp_cell_geom* const g[4]) {
   for (int m = 0; m < 4; ++m) {
     if (!g[m]) return fail(c, SP_ERR_INVALID_ARG, "null sp_cell_geom");
@@ -816,7 +903,17 @@ sp_status sp_grid_preprocess_dev(sp_ctx* c, const sp_cell_geom* p, const sp_cell
   const sp_cell_geom* g[4] = {p, u, v, w};
   sp_status s = check_geom(c, g);
   if (s) return s;
-  if ((s = grid_preprocess(c, g))) return s;
+  // halo-allocated level sets, normals and volume fractions as cell arrays
+  sp_cell_geom d[4];
+  const sp_cell_geom* dg[4];
+  for (int m = 0; m < 4; ++m) {
+    d[m] = *g[m];
+    if ((s = geom_cells_in(c, 600 + 6 * m, d[m]))) return s;
+    dg[m] = &d[m];
+  }
+  if ((s = grid_preprocess(c, dg))) return s;
+  for (int m = 0; m < 4; ++m)
+    if ((s = geom_cells_out(c, *g[m], d[m], kGeomVof | kGeomVofL))) return s;
   const hipError_t e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_check(c, e, "sp_grid_preprocess");
   timers_collect(c);
@@ -833,21 +930,23 @@ sp_status sp_grid_preprocess(sp_ctx* c, const sp_cell_geom* p, const sp_cell_geo
   const size_t N = (size_t)c->dm.N, Nh = (size_t)c->dm.Nh;
   // staging slots past those of the projection, adjoint and wall step
   size_t k = 100;
-  sp_cell_geom d[4];
+  sp_cell_geom d[4], up[4];
   for (int m = 0; m < 4; ++m) {
     const sp_cell_geom& h = *g[m];
     sp_cell_geom& o = d[m];
+    o.halo_fields = h.halo_fields;
+    const size_t nl = geom_count(h, N, Nh);  // phi, nx, ny, nz, vof, vof_l
     const double* t;
     const int32_t* ti;
     if ((s = upload(c, k++, h.dx, N, &o.dx)) || (s = upload(c, k++, h.dy, N, &o.dy)) ||
-        (s = upload(c, k++, h.dz, N, &o.dz)) || (s = upload(c, k++, h.phi, N, &o.phi)) ||
-        (s = upload(c, k++, h.nx, N, &o.nx)) || (s = upload(c, k++, h.ny, N, &o.ny)) ||
-        (s = upload(c, k++, h.nz, N, &o.nz)))
+        (s = upload(c, k++, h.dz, N, &o.dz)) || (s = upload(c, k++, h.phi, nl, &o.phi)) ||
+        (s = upload(c, k++, h.nx, nl, &o.nx)) || (s = upload(c, k++, h.ny, nl, &o.ny)) ||
+        (s = upload(c, k++, h.nz, nl, &o.nz)))
       return s;
     // read-write arrays: uploaded, written back below
-    if ((s = upload(c, k++, static_cast<const double*>(h.vof), N, &t))) return s;
+    if ((s = upload(c, k++, static_cast<const double*>(h.vof), nl, &t))) return s;
     o.vof = const_cast<double*>(t);
-    if ((s = upload(c, k++, static_cast<const double*>(h.vof_l), N, &t))) return s;
+    if ((s = upload(c, k++, static_cast<const double*>(h.vof_l), nl, &t))) return s;
     o.vof_l = const_cast<double*>(t);
     if ((s = upload(c, k++, static_cast<const int32_t*>(h.cell_type), N, &ti))) return s;
     o.cell_type = const_cast<int32_t*>(ti);
@@ -871,14 +970,18 @@ sp_status sp_grid_preprocess(sp_ctx* c, const sp_cell_geom* p, const sp_cell_geo
     o.cell_cent = static_cast<double*>(b);
     if ((s = stage(c, k++, N * sizeof(double), &b))) return s;
     o.wllh = static_cast<double*>(b);
+    up[m] = o;
+    if ((s = geom_cells_in(c, 600 + 6 * m, o))) return s;
   }
   const sp_cell_geom* dg[4] = {&d[0], &d[1], &d[2], &d[3]};
   if ((s = grid_preprocess(c, dg))) return s;
   for (int m = 0; m < 4; ++m) {
     const sp_cell_geom& h = *g[m];
     const sp_cell_geom& o = d[m];
+    if ((s = geom_cells_out(c, up[m], o, kGeomVof | kGeomVofL))) return s;
+    const size_t nl = geom_count(h, N, Nh);
     const struct { void* dst; const void* src; size_t bytes; } back[] = {
-        {h.vof, o.vof, N * 8},           {h.vof_l, o.vof_l, N * 8},
+        {h.vof, up[m].vof, nl * 8},      {h.vof_l, up[m].vof_l, nl * 8},
         {h.cell_type, o.cell_type, N * 4}, {h.ee_area, o.ee_area, Nh * 8},
         {h.ne_area, o.ne_area, Nh * 8},  {h.te_area, o.te_area, Nh * 8},
         {h.fce, o.fce, 3 * Nh * 8},      {h.fcn, o.fcn, 3 * Nh * 8},
@@ -925,7 +1028,14 @@ sp_status sp_new_cell_face(sp_ctx* c, const sp_cell_geom* p, const sp_cell_geom*
   sp_status s = stage(c, 150, 9 * N * sizeof(double), &scratch);
   if (s) return s;
   if (device) {
-    if ((s = new_cell_face(c, g, x, static_cast<double*>(scratch)))) return s;
+    sp_cell_geom d[4];
+    const sp_cell_geom* dg[4];
+    for (int m = 0; m < 4; ++m) {
+      d[m] = *g[m];
+      if ((s = geom_cells_in(c, 630 + 6 * m, d[m]))) return s;
+      dg[m] = &d[m];
+    }
+    if ((s = new_cell_face(c, dg, x, static_cast<double*>(scratch)))) return s;
   } else {
     size_t k = 151;
     sp_cell_geom d[4];
@@ -951,9 +1061,12 @@ sp_status sp_new_cell_face(sp_ctx* c, const sp_cell_geom* p, const sp_cell_geom*
       if ((s = up(h.dx, N, &o.dx)) || (s = up(h.dy, N, &o.dy)) || (s = up(h.dz, N, &o.dz)))
         return s;
       if (m == 0) continue;
-      if ((s = up(h.phi, N, &o.phi)) || (s = up(h.nx, N, &o.nx)) || (s = up(h.ny, N, &o.ny)) ||
-          (s = up(h.nz, N, &o.nz)))
+      const size_t nl = geom_count(h, N, Nh);
+      o.halo_fields = h.halo_fields;
+      if ((s = up(h.phi, nl, &o.phi)) || (s = up(h.nx, nl, &o.nx)) || (s = up(h.ny, nl, &o.ny)) ||
+          (s = up(h.nz, nl, &o.nz)))
         return s;
+      if ((s = geom_cells_in(c, 630 + 6 * m, o))) return s;
       const int32_t* ti;
       if ((s = upload(c, k++, static_cast<const int32_t*>(h.cell_type), N, &ti))) return s;
       o.cell_type = const_cast<int32_t*>(ti);
@@ -1104,25 +1217,38 @@ sp_status sp_compute_uvw_liquid_field(sp_ctx* c, const sp_liquid_field* p,
     for (const void* q : req)
       if (!q) return fail(c, SP_ERR_INVALID_ARG, "sp_liquid_field member is null");
   }
-  const size_t N = (size_t)c->dm.N;
+  const size_t N = (size_t)c->dm.N, Nh = (size_t)c->dm.Nh;
   sp_status s;
   if (device) {
-    if ((s = uvw_liquid_field(c, g))) return s;
+    // halo-allocated liquid fields as cell arrays, the U/V/W results back
+    sp_liquid_field d[4];
+    const sp_liquid_field* dg[4];
+    for (int m = 0; m < 4; ++m) {
+      d[m] = *g[m];
+      if ((s = liq_cells_in(c, 700 + 5 * m, d[m]))) return s;
+      dg[m] = &d[m];
+    }
+    if ((s = uvw_liquid_field(c, dg))) return s;
+    for (int m = 1; m < 4; ++m)
+      if ((s = liq_cells_out(c, *g[m], d[m]))) return s;
   } else {
     size_t k = 250;  // staging slots shared with the other geometry entry points' range
-    sp_liquid_field d[4];
+    sp_liquid_field d[4], up[4];
     for (int m = 0; m < 4; ++m) {
       const sp_liquid_field& h = *g[m];
       sp_liquid_field& o = d[m];
+      o.halo_fields = h.halo_fields;
       if ((s = upload(c, k++, h.dx, N, &o.dx)) || (s = upload(c, k++, h.dy, N, &o.dy)) ||
           (s = upload(c, k++, h.dz, N, &o.dz)))
         return s;
       double* const hs[5] = {h.phi_l, h.nx_l, h.ny_l, h.nz_l, h.vof_l};
       double** os[5] = {&o.phi_l, &o.nx_l, &o.ny_l, &o.nz_l, &o.vof_l};
       for (int f = 0; f < 5; ++f) {
-        if (m == 0) {
+        // P's fields are read; U/V/W's are written (their halo entries kept)
+        if (m == 0 || h.halo_fields) {
           const double* t;
-          if ((s = upload(c, k++, static_cast<const double*>(hs[f]), N, &t))) return s;
+          if ((s = upload(c, k++, static_cast<const double*>(hs[f]), h.halo_fields ? Nh : N, &t)))
+            return s;
           *os[f] = const_cast<double*>(t);
         } else {
           void* b;
@@ -1130,14 +1256,18 @@ sp_status sp_compute_uvw_liquid_field(sp_ctx* c, const sp_liquid_field* p,
           *os[f] = static_cast<double*>(b);
         }
       }
+      up[m] = o;
+      if ((s = liq_cells_in(c, 700 + 5 * m, o))) return s;
     }
     const sp_liquid_field* dg[4] = {&d[0], &d[1], &d[2], &d[3]};
     if ((s = uvw_liquid_field(c, dg))) return s;
     for (int m = 1; m < 4; ++m) {
+      if ((s = liq_cells_out(c, up[m], d[m]))) return s;
       double* const hs[5] = {g[m]->phi_l, g[m]->nx_l, g[m]->ny_l, g[m]->nz_l, g[m]->vof_l};
-      double* const os[5] = {d[m].phi_l, d[m].nx_l, d[m].ny_l, d[m].nz_l, d[m].vof_l};
+      double* const os[5] = {up[m].phi_l, up[m].nx_l, up[m].ny_l, up[m].nz_l, up[m].vof_l};
+      const size_t n = g[m]->halo_fields ? Nh : N;
       for (int f = 0; f < 5; ++f)
-        if ((s = pipe_download(c, hs[f], os[f], N * sizeof(double)))) return s;
+        if ((s = pipe_download(c, hs[f], os[f], n * sizeof(double)))) return s;
     }
   }
   const hipError_t e = hipStreamSynchronize(c->stream);
@@ -1271,10 +1401,20 @@ sp_status sp_predict_uvw(sp_ctx* c, const sp_pred_inputs* in, sp_pred_outputs* o
   const bool second = in->space_order == 2 && in->itt != 1;
   if ((s = stage(c, 299, predict_face_doubles(D, second) * sizeof(double), &faces))) return s;
   if (device) {
-    if ((s = predict_uvw(c, *in, *out, static_cast<double*>(faces)))) return s;
+    // halo-allocated cell fields of the old and current cells as cell arrays
+    sp_pred_inputs d = *in;
+    sp_cell_geom og[3] = {*in->u_old, *in->v_old, *in->w_old};
+    sp_cell_geom cg[3] = {*in->u_cur, *in->v_cur, *in->w_cur};
+    for (int q = 0; q < 3; ++q)
+      if ((s = geom_cells_in(c, 660 + 6 * q, og[q])) || (s = geom_cells_in(c, 680 + 6 * q, cg[q])))
+        return s;
+    d.u_old = &og[0]; d.v_old = &og[1]; d.w_old = &og[2];
+    d.u_cur = &cg[0]; d.v_cur = &cg[1]; d.w_cur = &cg[2];
+    if ((s = predict_uvw(c, d, *out, static_cast<double*>(faces)))) return s;
   } else {
-    // staging slots 300..399 belong to the predictor alone
-    size_t k = 300;
+    // staging slots from 1000 up belong to the predictor alone (its ~120
+    // arrays must not reach the projection's geometry slots at 400)
+    size_t k = 1000;
     sp_pred_inputs d = *in;
     auto upd = [&](const double* h, size_t n, const double** o) { return upload(c, k++, h, n, o); };
     auto upi = [&](const int32_t* h, size_t n, const int32_t** o) { return upload(c, k++, h, n, o); };
@@ -1290,12 +1430,14 @@ sp_status sp_predict_uvw(sp_ctx* c, const sp_pred_inputs* in, sp_pred_outputs* o
       const sp_cell_geom& h = *hg[q];
       sp_cell_geom& o = og[q];
       o = sp_cell_geom{};
+      o.halo_fields = h.halo_fields;
+      const size_t nl = geom_count(h, N, Nh);
       const double* t;
       const int32_t* ti;
-      if ((s = upd(h.phi, N, &o.phi)) || (s = upd(h.nx, N, &o.nx)) || (s = upd(h.ny, N, &o.ny)) ||
-          (s = upd(h.nz, N, &o.nz)))
+      if ((s = upd(h.phi, nl, &o.phi)) || (s = upd(h.nx, nl, &o.nx)) ||
+          (s = upd(h.ny, nl, &o.ny)) || (s = upd(h.nz, nl, &o.nz)))
         return s;
-      if ((s = upd(h.vof, N, &t))) return s;
+      if ((s = upd(h.vof, nl, &t))) return s;
       o.vof = const_cast<double*>(t);
       if ((s = upi(h.cell_type, N, &ti))) return s;
       o.cell_type = const_cast<int32_t*>(ti);
@@ -1320,14 +1462,18 @@ sp_status sp_predict_uvw(sp_ctx* c, const sp_pred_inputs* in, sp_pred_outputs* o
       const sp_cell_geom& hcq = *hc[q];
       sp_cell_geom& oc = cg[q];
       oc = sp_cell_geom{};
+      oc.halo_fields = hcq.halo_fields;
+      const size_t nc = geom_count(hcq, N, Nh);  // vof, vof_l
       double* const cv[4] = {hcq.vof, hcq.vof_l, hcq.wllh, hcq.delh};
       double** co[4] = {&oc.vof, &oc.vof_l, &oc.wllh, &oc.delh};
       for (int m = 0; m < 4; ++m) {
-        if ((s = upd(cv[m], N, &t))) return s;
+        if ((s = upd(cv[m], m < 2 ? nc : N, &t))) return s;
         *co[m] = const_cast<double*>(t);
       }
       if ((s = upi(hcq.cell_type, N, &ti))) return s;
       oc.cell_type = const_cast<int32_t*>(ti);
+      if ((s = geom_cells_in(c, 660 + 6 * q, o)) || (s = geom_cells_in(c, 680 + 6 * q, oc)))
+        return s;
     }
     d.u_old = &og[0]; d.v_old = &og[1]; d.w_old = &og[2];
     d.u_old_x = &ox[0]; d.v_old_x = &ox[1]; d.w_old_x = &ox[2];

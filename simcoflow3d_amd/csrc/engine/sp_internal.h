@@ -280,6 +280,11 @@ sp_status predict_uvw(sp_ctx* c, const sp_pred_inputs& in, const sp_pred_outputs
 size_t predict_face_doubles(const Dims& D, bool second);
 // doubles of the face fluxes alone (the start of that buffer)
 size_t predict_flux_doubles(const Dims& D);
+// ---- Layout conversion at the entry points (sp_layout.hip) ----
+// Interior of a halo array (Nh doubles) into a cell array (N) and back, on
+// the context's stream.
+sp_status halo_to_cell(sp_ctx* c, const double* halo, double* cell);
+sp_status cell_to_halo(sp_ctx* c, const double* cell, double* halo);
 // ---- VTR output (sp_vtr.hip) ----
 sp_status write_vtr(sp_ctx* c, const char* dir, const char* prefix, int64_t itt,
                     const sp_vtr_fields& f, bool device);

@@ -59,10 +59,18 @@ GEOM_UPDATED = ("vof", "vof_l", "cell_type", "fce", "fcn", "fct", "delh")
 GEOM_WRITTEN = ("ee_area", "ne_area", "te_area", "cell_cent", "wllh")
 
 
-def geom_shapes(I, J, K):
-    """Fortran shapes of the sp_cell_geom arrays for an I x J x K grid."""
+# members a host may allocate with the halo (sp_cell_geom.halo_fields)
+GEOM_HALO_FIELDS = ("phi", "nx", "ny", "nz", "vof", "vof_l")
+
+
+def geom_shapes(I, J, K, halo_fields=False):
+    """Fortran shapes of the sp_cell_geom arrays for an I x J x K grid
+    (halo_fields: phi, nx, ny, nz, vof, vof_l with the halo)."""
     cell, halo = (I, J, K), (I + 2, J + 2, K + 2)
     sh = {n: cell for n in GEOM_FIELDS}
+    if halo_fields:
+        for n in GEOM_HALO_FIELDS:
+            sh[n] = halo
     for n in ("ee_area", "ne_area", "te_area"):
         sh[n] = halo
     for n in ("fce", "fcn", "fct"):
@@ -200,7 +208,7 @@ class SpKernelTime(ctypes.Structure):
 
 
 class SpCellGeom(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p) for n in GEOM_FIELDS]
+    _fields_ = [(n, ctypes.c_void_p) for n in GEOM_FIELDS] + [("halo_fields", ctypes.c_int32)]
 
 
 class SpCellInterp(ctypes.Structure):
@@ -208,7 +216,7 @@ class SpCellInterp(ctypes.Structure):
 
 
 class SpLiquidField(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p) for n in LIQUID_FIELDS]
+    _fields_ = [(n, ctypes.c_void_p) for n in LIQUID_FIELDS] + [("halo_fields", ctypes.c_int32)]
 
 
 class SpPredInputs(ctypes.Structure):
@@ -467,15 +475,16 @@ class Projector:
         self._check(rc, "sp_wall_step")
         return out, st
 
-    def grid_preprocess(self, cells, device=False):
+    def grid_preprocess(self, cells, device=False, halo_fields=False):
         """GridPreprocess (sp_grid_preprocess) on the P, U, V, W grids.
         ``cells``: four dicts keyed like sp_cell_geom.  Host (numpy) inputs
         are copied; the result is four dicts with the updated and written
         arrays.  With device=True every member is a torch tensor on the
         context's device (Fortran shape reversed, C-contiguous), updated in
-        place; nothing is returned."""
+        place; nothing is returned.  ``halo_fields``: phi, nx, ny, nz, vof
+        and vof_l are halo arrays, as the reference's drivers allocate them."""
         I, J, K = self.shape
-        sh = geom_shapes(I, J, K)
+        sh = geom_shapes(I, J, K, halo_fields)
         structs = []
         if device:
             for c in cells:
@@ -484,7 +493,8 @@ class Projector:
                     if tuple(t.shape) != sh[n][::-1] or not t.is_contiguous():
                         raise ValueError(f"{n}: expected a contiguous tensor of shape "
                                          f"{sh[n][::-1]}, got {tuple(t.shape)}")
-                structs.append(SpCellGeom(*(_addr(c[n]) for n in GEOM_FIELDS)))
+                structs.append(SpCellGeom(*(_addr(c[n]) for n in GEOM_FIELDS),
+                                          1 if halo_fields else 0))
             rc = self.lib.sp_grid_preprocess_dev(self.h, *(ctypes.byref(s) for s in structs))
             self._check(rc, "sp_grid_preprocess_dev")
             return None
@@ -500,19 +510,21 @@ class Projector:
                 if o[n].shape != sh[n]:
                     raise ValueError(f"{n}: expected shape {sh[n]}, got {o[n].shape}")
             outs.append(o)
-            structs.append(SpCellGeom(*(_addr(o[n]) for n in GEOM_FIELDS)))
+            structs.append(SpCellGeom(*(_addr(o[n]) for n in GEOM_FIELDS), 1 if halo_fields else 0))
         rc = self.lib.sp_grid_preprocess(self.h, *(ctypes.byref(s) for s in structs))
         self._check(rc, "sp_grid_preprocess")
         return outs
 
-    def uvw_liquid_field(self, cells, device=False):
+    def uvw_liquid_field(self, cells, device=False, halo_fields=False):
         """ComputeUVWLiquidField (sp_compute_uvw_liquid_field).  cells: four
-        dicts (P, U, V, W) keyed like sp_liquid_field (cell arrays).  Host:
-        returns the U, V, W dicts with new phi_l, nx_l, ny_l, nz_l, vof_l;
-        device (torch tensors): writes them in place."""
+        dicts (P, U, V, W) keyed like sp_liquid_field (cell arrays; with
+        ``halo_fields`` phi_l, nx_l, ny_l, nz_l, vof_l are halo arrays).
+        Host: returns the U, V, W dicts with new phi_l, nx_l, ny_l, nz_l,
+        vof_l; device (torch tensors): writes them in place."""
         I, J, K = self.shape
+        hf = 1 if halo_fields else 0
         if device:
-            gs = [SpLiquidField(*(_addr(c.get(n)) for n in LIQUID_FIELDS)) for c in cells]
+            gs = [SpLiquidField(*(_addr(c.get(n)) for n in LIQUID_FIELDS), hf) for c in cells]
             rc = self.lib.sp_compute_uvw_liquid_field(self.h, *(ctypes.byref(a) for a in gs), 1)
             self._check(rc, "sp_compute_uvw_liquid_field")
             return None
@@ -523,29 +535,33 @@ class Projector:
                 a = c.get(n)
                 if a is None and (m == 0 or n in ("dx", "dy", "dz")):
                     raise ValueError(f"{n} is required")
-                o[n] = (np.zeros((I, J, K), order="F") if a is None else
+                shp = (I + 2, J + 2, K + 2) if halo_fields and n not in ("dx", "dy", "dz") \
+                    else (I, J, K)
+                o[n] = (np.zeros(shp, order="F") if a is None else
                         np.array(a, dtype=np.float64, order="F", copy=True))
-                if o[n].shape != (I, J, K):
-                    raise ValueError(f"{n}: expected shape {(I, J, K)}, got {o[n].shape}")
+                if o[n].shape != shp:
+                    raise ValueError(f"{n}: expected shape {shp}, got {o[n].shape}")
             outs.append(o)
-            gs.append(SpLiquidField(*(_addr(o[n]) for n in LIQUID_FIELDS)))
+            gs.append(SpLiquidField(*(_addr(o[n]) for n in LIQUID_FIELDS), hf))
         rc = self.lib.sp_compute_uvw_liquid_field(self.h, *(ctypes.byref(a) for a in gs), 0)
         self._check(rc, "sp_compute_uvw_liquid_field")
         return outs[1:]
 
-    def new_cell_face(self, cells, interp, device=False):
+    def new_cell_face(self, cells, interp, device=False, halo_fields=False):
         """NewCellFace (sp_new_cell_face).  cells: four dicts (P, U, V, W)
         keyed like sp_cell_geom; interp: three dicts (U, V, W) keyed like
         sp_cell_interp (moex required; msce and the interpolation arrays
         start values, zeros when missing).  Host: returns updated copies
-        (cells, interp); device (torch tensors): updates in place."""
+        (cells, interp); device (torch tensors): updates in place.
+        ``halo_fields`` as for grid_preprocess."""
         I, J, K = self.shape
-        sh = geom_shapes(I, J, K)
+        sh = geom_shapes(I, J, K, halo_fields)
         ish = {n: (I + 2, J + 2, K + 2) for n in INTERP_FIELDS}
         ish["moex"] = (I, J, K)
         ish["msce"] = (I, J, K, 3)
         if device:
-            gs = [SpCellGeom(*(_addr(c.get(n)) for n in GEOM_FIELDS)) for c in cells]
+            gs = [SpCellGeom(*(_addr(c.get(n)) for n in GEOM_FIELDS), 1 if halo_fields else 0)
+                  for c in cells]
             xs = [SpCellInterp(*(_addr(d.get(n)) for n in INTERP_FIELDS)) for d in interp]
             rc = self.lib.sp_new_cell_face(self.h, *(ctypes.byref(a) for a in gs),
                                            *(ctypes.byref(a) for a in xs), 1)
@@ -562,7 +578,7 @@ class Projector:
                 if o[n].shape != sh[n]:
                     raise ValueError(f"{n}: expected shape {sh[n]}, got {o[n].shape}")
             cs.append(o)
-            gs.append(SpCellGeom(*(_addr(o[n]) for n in GEOM_FIELDS)))
+            gs.append(SpCellGeom(*(_addr(o[n]) for n in GEOM_FIELDS), 1 if halo_fields else 0))
         xs, xstructs = [], []
         for d in interp:
             o = {}
@@ -618,7 +634,7 @@ class Projector:
         self._check(rc, "sp_number_external_cell")
         return pos, ext.value
 
-    def predict_uvw(self, p_areas, old, cur, fields, pred, dp, device=False):
+    def predict_uvw(self, p_areas, old, cur, fields, pred, dp, device=False, halo_fields=False):
         """PredictingUVW (sp_predict_uvw) with SpaceOrder 1 or 2 (from
         fields["params"]["space_order"]).  ``p_areas``:
         PCellO%EEArea, NEArea, TEArea; ``old``/``cur``: three dicts (U, V, W)
@@ -627,7 +643,8 @@ class Projector:
         bcv, params).  ``pred`` (Pred%u, v, w) and ``dp`` (PU/PV/PW%Dp) are
         written in place, and fields["flux_div_old"] is updated in place:
         numpy arrays in the host's layouts, or torch tensors with
-        device=True."""
+        device=True.  ``halo_fields``: the cells' phi, nx, ny, nz, vof and
+        vof_l are halo arrays."""
         keep = []
 
         def ptr(a, dt=None):
@@ -639,7 +656,7 @@ class Projector:
             return _addr(a)
 
         def geom(d):
-            g = SpCellGeom(*(None for _ in GEOM_FIELDS))
+            g = SpCellGeom(*(None for _ in GEOM_FIELDS), 1 if halo_fields else 0)
             for n in d:
                 if n in _PRED_INTERP:
                     continue

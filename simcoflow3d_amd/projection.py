@@ -196,9 +196,13 @@ def grid_preprocess(PGrid, UGrid, VGrid, WGrid, PCell, UCell, VCell, WCell):
     needs phi, nx, ny, nz, vof, vofL and delh (and FCE/FCN/FCT where their
     untouched entries matter); Cell_Type, vof, vofL, EEArea, NEArea,
     TEArea, FCE, FCN, FCT, Cell_Cent, WlLh and delh are written back (the
-    ones that are None are allocated)."""
-    I, J, K = np.shape(PCell.phi)
-    sh = _eng.geom_shapes(I, J, K)
+    ones that are None are allocated).  phi, nx, ny, nz, vof and vofL may
+    be (0:Imax+1, 0:Jmax+1, 0:Kmax+1) arrays, as the drivers allocate them
+    (FlowSphere2Phases.f90:316-322), or (Imax, Jmax, Kmax) ones, as
+    AllocateVar does (Main.f90:264-270): PCell%phi's shape says which."""
+    I, J, K = np.shape(PGrid.dx)
+    halo = np.shape(PCell.phi) == (I + 2, J + 2, K + 2)
+    sh = _eng.geom_shapes(I, J, K, halo)
     cells = []
     for G, C in ((PGrid, PCell), (UGrid, UCell), (VGrid, VCell), (WGrid, WCell)):
         d = {}
@@ -219,7 +223,7 @@ def grid_preprocess(PGrid, UGrid, VGrid, WGrid, PCell, UCell, VCell, WCell):
             d[n] = a
         cells.append(d)
     P = _projector((I, J, K), dict(pin_i=0, pin_j=0, pin_k=0))
-    outs = P.grid_preprocess(cells)
+    outs = P.grid_preprocess(cells, halo_fields=halo)
     for C, o in zip((PCell, UCell, VCell, WCell), outs):
         for n, m in _GEOM_CELL.items():
             if n in ("phi", "nx", "ny", "nz"):

@@ -1,0 +1,225 @@
+"""The geometry, predictor and liquid-field entry points on halo-allocated
+cell fields (sp_cell_geom.halo_fields / sp_liquid_field.halo_fields = 1).
+
+The reference's drivers allocate the level sets, normals and volume
+fractions of every Cell with the halo (FlowSphere2Phases.f90:316-322,
+431-440), its AllocateVar without it (Main.f90:264-270, 383-387).  Bar: on
+the same values, a call on halo arrays gives bit for bit what the call on
+cell arrays gives (interior entries of the halo arrays), and keeps every
+halo entry of the caller's arrays; host arrays and device tensors."""
+
+import numpy as np
+import pytest
+
+from oracle import orc
+from simcoflow3d_amd import configs
+from simcoflow3d_amd.engine import GEOM_FIELDS, GEOM_HALO_FIELDS, LIQUID_FIELDS, Projector
+from tests import pred_cases as PC
+
+pytestmark = pytest.mark.gpu
+
+SENTINEL = -777.25
+LIQ_HALO = ("phi_l", "nx_l", "ny_l", "nz_l", "vof_l")
+
+
+def halo(a, fill=SENTINEL):
+    h = np.full(tuple(s + 2 for s in a.shape), fill, order="F")
+    h[1:-1, 1:-1, 1:-1] = a
+    return h
+
+
+def interior(h):
+    return h[1:-1, 1:-1, 1:-1]
+
+
+def halo_cells(cells, names):
+    return [{n: (halo(a) if n in names else a) for n, a in c.items()} for c in cells]
+
+
+def check_halo(got, exp, names):
+    """got: halo arrays, exp: cell arrays; the ring must hold the sentinel."""
+    for m, (g, e) in enumerate(zip(got, exp)):
+        for n in names:
+            if n not in g:
+                continue
+            assert g[n].shape == tuple(s + 2 for s in e[n].shape), (m, n)
+            assert np.array_equal(interior(g[n]), e[n], equal_nan=True), (m, n)
+            ring = np.ones(g[n].shape, bool)
+            ring[1:-1, 1:-1, 1:-1] = False
+            assert np.all(g[n][ring] == SENTINEL), (m, n)
+
+
+def tdev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).to("cuda:0")
+
+
+def back(t):
+    return np.asfortranarray(t.cpu().numpy().T)
+
+
+@pytest.mark.parametrize("device", (False, True))
+@pytest.mark.parametrize("size,shape", [((13, 11, 9), "bodies"), ((9, 7, 12), "plane")])
+def test_grid_preprocess_on_halo_fields(gpu_available, size, shape, device):
+    print(f"[halo] GridPreprocess {shape} size={size} device={device}", flush=True)
+    cells = configs.geometry_case(size, shape, seed=sum(size), stretch=True, edge_values=True)
+    exp = orc.grid_preprocess(cells)
+    hc = halo_cells(cells, GEOM_HALO_FIELDS)
+    P = Projector(size)
+    if device:
+        dev = []
+        for c in hc:
+            d = {}
+            for n in GEOM_FIELDS:
+                a = c.get(n)
+                if a is None:  # written arrays: any start value
+                    a = np.zeros_like(exp[0][n])
+                d[n] = tdev(a)
+            dev.append(d)
+        P.grid_preprocess(dev, device=True, halo_fields=True)
+        got = [{n: back(d[n]) for n in GEOM_FIELDS} for d in dev]
+    else:
+        got = P.grid_preprocess(hc, halo_fields=True)
+    ref = P.grid_preprocess(cells)
+    P.close()
+    check_halo(got, exp, GEOM_HALO_FIELDS)
+    check_halo(got, ref, GEOM_HALO_FIELDS)
+    for m in range(4):
+        for n in GEOM_FIELDS:
+            if n not in GEOM_HALO_FIELDS:
+                assert np.array_equal(got[m][n], exp[m][n], equal_nan=True), (m, n)
+
+
+@pytest.mark.parametrize("device", (False, True))
+def test_new_cell_face_on_halo_fields(gpu_available, device):
+    size = (12, 10, 9)
+    print(f"[halo] NewCellFace size={size} device={device}", flush=True)
+    cells = configs.geometry_case(size, "bodies", seed=5, stretch=True)
+    out = orc.grid_preprocess(cells)
+    zeros = [np.zeros(size, np.int32, order="F") for _ in range(3)]
+    moex = orc.define_moex(*(o["cell_type"] for o in out), *zeros)
+    rng = np.random.default_rng(5)
+    interp = []
+    for m in range(3):
+        d = dict(moex=moex[m],
+                 msce=np.asfortranarray(rng.integers(-5, 5, size + (3,)).astype(np.int32)))
+        for n in orc.INTERP_FIELDS:
+            d[n] = np.asfortranarray(rng.uniform(-1, 1, tuple(s + 2 for s in size)))
+        interp.append(d)
+    ecs, exs = orc.new_cell_face(out, interp)
+    hout = halo_cells(out, GEOM_HALO_FIELDS)
+    P = Projector(size)
+    if device:
+        dc = [{n: tdev(c[n]) for n in GEOM_FIELDS} for c in hout]
+        dx = [{n: tdev(d[n]) for n in ("moex", "msce") + orc.INTERP_FIELDS} for d in interp]
+        P.new_cell_face(dc, dx, device=True, halo_fields=True)
+        cs = [{n: back(d[n]) for n in GEOM_FIELDS} for d in dc]
+        xs = [{n: back(d[n]) for n in ("msce",) + orc.INTERP_FIELDS} for d in dx]
+    else:
+        cs, xs = P.new_cell_face(hout, interp, halo_fields=True)
+    P.close()
+    for g in range(1, 4):
+        for n in ("cell_cent", "delh"):
+            assert np.array_equal(cs[g][n], ecs[g][n], equal_nan=True), (g, n)
+    for m in range(3):
+        for n in ("msce",) + orc.INTERP_FIELDS:
+            assert np.array_equal(xs[m][n], exs[m][n], equal_nan=True), (m, n)
+    # the halo inputs are read, not written
+    check_halo(cs, out, GEOM_HALO_FIELDS)
+
+
+@pytest.mark.parametrize("device", (False, True))
+@pytest.mark.parametrize("space_order", (1, 2))
+def test_predictor_on_halo_fields(gpu_available, space_order, device):
+    size = (24, 20, 18)
+    print(f"[halo] predictor size={size} space_order={space_order} device={device}", flush=True)
+    grid, geo_o, int_o = PC.geometry(size, "bodies", seed=2, stretch=True)
+    _, geo_c, int_c = PC.geometry(size, "bodies", seed=102, stretch=True)
+    f = configs.predictor_fields(size, seed=2, time_order=3, itt=2)
+    f["params"] = dict(f["params"], space_order=space_order)
+    hs = tuple(n + 2 for n in size)
+    pred0 = [configs.uniform(2, 90 + q, 3.0, 4.0, hs) for q in range(3)]
+    ref_pred, ref_dp, ref_fdo = PC.oracle_step(grid, geo_o, int_o, geo_c, int_c, f, pred=pred0)
+    old = halo_cells(configs.pred_cells(geo_o, int_o), GEOM_HALO_FIELDS)
+    cur = halo_cells(configs.pred_cells(geo_c, int_c), GEOM_HALO_FIELDS)
+    P = Projector(size)
+    P.set_grid_1d(grid.dxp, grid.dyp, grid.dzp, grid.dxu, grid.dyv, grid.dzw)
+    if device:
+        import torch
+
+        def dev(d):
+            return {n: tdev(a) for n, a in d.items()}
+        ff = dict(f, vel=[tdev(a) for a in f["vel"]], vel_n=[tdev(a) for a in f["vel_n"]],
+                  flux_div_old=tdev(f["flux_div_old"]), bcv=[tdev(a) for a in f["bcv"]])
+        pred = [tdev(a) for a in pred0]
+        dp = [torch.full(pred[0].shape, 77.0, dtype=torch.float64, device="cuda:0")
+              for _ in range(3)]
+        dold, dcur = [dev(d) for d in old], [dev(d) for d in cur]
+        P.predict_uvw([tdev(a) for a in PC.p_areas(geo_o)], dold, dcur, ff, pred, dp,
+                      device=True, halo_fields=True)
+        torch.cuda.synchronize()
+        pred, dp, fdo = [back(t) for t in pred], [back(t) for t in dp], back(ff["flux_div_old"])
+        old = [{n: back(t) for n, t in d.items()} for d in dold]
+        cur = [{n: back(t) for n, t in d.items()} for d in dcur]
+    else:
+        pred = [np.array(a, order="F", copy=True) for a in pred0]
+        dp = [np.full(hs, 77.0, order="F") for _ in range(3)]
+        ff = dict(f, flux_div_old=np.array(f["flux_div_old"], order="F", copy=True))
+        P.predict_uvw(PC.p_areas(geo_o), old, cur, ff, pred, dp, halo_fields=True)
+        fdo = ff["flux_div_old"]
+    P.close()
+    for q in range(3):
+        assert np.array_equal(pred[q], ref_pred[q], equal_nan=True), ("pred", q)
+        assert np.array_equal(dp[q], ref_dp[q], equal_nan=True), ("dp", q)
+    assert np.array_equal(fdo, ref_fdo, equal_nan=True)
+    # inputs untouched, halo rings included
+    check_halo(old, configs.pred_cells(geo_o, int_o), GEOM_HALO_FIELDS)
+    check_halo(cur, configs.pred_cells(geo_c, int_c), GEOM_HALO_FIELDS)
+
+
+@pytest.mark.parametrize("device", (False, True))
+@pytest.mark.parametrize("size", [(13, 11, 9), (70, 5, 6)])
+def test_uvw_liquid_field_on_halo_fields(gpu_available, size, device):
+    from tests.test_gpu_liquid import liquid_case
+    print(f"[halo] liquid field size={size} device={device}", flush=True)
+    cells = liquid_case(size, "bodies", sum(size), True)
+    P = Projector(size)
+    ref = P.uvw_liquid_field(cells)  # cell arrays
+    hc = [dict(c) for c in cells]
+    for m, c in enumerate(hc):
+        for n in LIQ_HALO:
+            # P's fields are read; U/V/W's are written (start values: noise)
+            src = c[n] if m == 0 else np.random.default_rng(m).uniform(size=size)
+            c[n] = halo(np.asfortranarray(src))
+    if device:
+        dev = [{n: tdev(c[n]) for n in LIQUID_FIELDS} for c in hc]
+        P.uvw_liquid_field(dev, device=True, halo_fields=True)
+        got = [{n: back(d[n]) for n in LIQUID_FIELDS} for d in dev[1:]]
+    else:
+        got = P.uvw_liquid_field(hc, halo_fields=True)
+    P.close()
+    check_halo(got, ref, LIQ_HALO)
+
+
+def test_reference_interface_mirror_on_halo_cells(gpu_available):
+    """grid_preprocess(PGrid, ..., WCell) with Cells allocated as the
+    drivers allocate them: vof and vofL written in place, inside the halo."""
+    from simcoflow3d_amd import projection as pj
+    size = (11, 9, 7)
+    print("[halo] reference interface mirror", flush=True)
+    cells = configs.geometry_case(size, "cylinder", seed=4, stretch=True, edge_values=True)
+    exp = orc.grid_preprocess(cells)
+    grids = [pj.Grid(c["dx"], c["dy"], c["dz"]) for c in cells]
+    objs = []
+    for c in cells:
+        objs.append(pj.Cell(c["cell_type"].copy(order="F"), vof=halo(c["vof"]),
+                            vofL=halo(c["vof_l"]), phi=halo(c["phi"]), nx=halo(c["nx"]),
+                            ny=halo(c["ny"]), nz=halo(c["nz"]), FCE=c["fce"].copy(order="F"),
+                            FCN=c["fcn"].copy(order="F"), FCT=c["fct"].copy(order="F"),
+                            delh=c["delh"].copy(order="F")))
+    pj.grid_preprocess(*grids, *objs)
+    for o, e in zip(objs, exp):
+        check_halo([dict(vof=o.vof, vof_l=o.vofL)], [e], ("vof", "vof_l"))
+        for n, m in dict(cell_type="Cell_Type", ee_area="EEArea", wllh="WlLh",
+                         cell_cent="Cell_Cent", delh="delh").items():
+            assert np.array_equal(getattr(o, m), e[n], equal_nan=True), m

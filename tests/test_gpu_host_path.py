@@ -109,3 +109,39 @@ def test_stale_face_overflow_has_its_own_status(gpu_available):
         P.project(f, c.bc_flag, c.dt)
     P.close()
     assert ei.value.status == SP_ERR_CAPACITY
+
+
+def test_static_geometry_survives_other_entry_points(gpu_available):
+    """The device copies of the geometry that a static_geometry projection
+    reuses (and the forward inputs a later adjoint reads) are not touched by
+    the host-array calls in between: the momentum predictor, the geometry
+    update and the liquid fields of the same grid size."""
+    from simcoflow3d_amd import configs
+    from tests import pred_cases as PC
+    c = H.case(0, size=(24, 24, 24))
+    print("[host-path] static geometry after other entry points", flush=True)
+    f1 = H.fields(c)
+    f2 = _new_velocity(c, seed=9)
+    P = H.projector(c, precision=1)
+    P.project(f1, c.bc_flag, c.dt, static_geometry=True)
+    # other host-array entry points on the same context
+    size = c.shape
+    grid, geo, interp = PC.geometry(size, shape="bodies", seed=2)
+    f = configs.predictor_fields(size, seed=4, itt=2)
+    cells = configs.pred_cells(geo, interp)
+    hs = tuple(n + 2 for n in size)
+    pred = [np.zeros(hs, order="F") for _ in range(3)]
+    dp = [np.zeros(hs, order="F") for _ in range(3)]
+    ff = dict(f, flux_div_old=np.array(f["flux_div_old"], order="F", copy=True))
+    for so in (1, 2):
+        ff["params"] = dict(f["params"], space_order=so)
+        P.predict_uvw(PC.p_areas(geo), cells, cells, ff, pred, dp)
+    P.grid_preprocess(configs.geometry_case(size, "sphere", seed=1))
+    a2, s2 = P.project(f2, c.bc_flag, c.dt, static_geometry=True)
+    P.close()
+    Q = H.projector(c, precision=1)
+    b2, t2 = Q.project(f2, c.bc_flag, c.dt)
+    Q.close()
+    assert s2.converged == 1 and t2.converged == 1
+    for n in ("p", "u", "v", "w"):
+        assert np.array_equal(a2[n], b2[n]), n
