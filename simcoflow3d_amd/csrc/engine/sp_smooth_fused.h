@@ -274,21 +274,26 @@ constexpr int fused_pipe() {
   return sizeof(T) == 4 ? (FusedShape<NU>::W % 3 == 0 ? 3 : 2) : 1;
 }
 
-// Steps t + U, t + U + 1, ..., t + W - 1 of the march (stopping at t1), each
-// followed by the load of a later plane into the buffer it used.
-template <typename T, int NU, int MODE, bool HAS_E, bool DOTS, int U>
+// Steps t + U, t + U + 1, ..., t + W - 1 of the march, each followed by the
+// load of a later plane into the buffer it used.  CHECK: stop at t1 (the
+// last, partial turn only).  Full turns carry no exit test, so the march loop
+// has one back-edge whose in-flight loads are the same on every pass: an exit
+// path per step would merge into the loop head with fewer loads in flight,
+// and the compiler's wait counts there would drain the load pipeline on
+// every turn.
+template <typename T, int NU, int MODE, bool HAS_E, bool DOTS, bool CHECK, int U>
 __device__ inline void march_steps(FusedState<T, NU, MODE>& F, PlaneLoad<T> (&LB)[fused_pipe<T, NU>()],
                                    const FusedArgs<T>& A, const FusedCol& C,
                                    T (*xs)[kFEY + 2][kFEX + 2], T (*rs)[kFEY][kFEX], int t, int t1,
                                    int k0, int k1) {
   if constexpr (U < FusedShape<NU>::W) {
-    if (t + U >= t1) return;
+    if (CHECK && t + U >= t1) return;
     constexpr int P = fused_pipe<T, NU>();
     static_assert(FusedShape<NU>::W % P == 0, "load buffers must tile the window");
     PlaneLoad<T>& L = LB[U % P];
     fused_step<T, NU, MODE, DOTS, U>(F, L, A, C, xs, rs, t + U, k0, k1);
     load_plane<T, MODE, HAS_E, DOTS, FusedShape<NU>::H>(L, A, C, t + U + P);
-    march_steps<T, NU, MODE, HAS_E, DOTS, U + 1>(F, LB, A, C, xs, rs, t, t1, k0, k1);
+    march_steps<T, NU, MODE, HAS_E, DOTS, CHECK, U + 1>(F, LB, A, C, xs, rs, t, t1, k0, k1);
   }
 }
 
@@ -346,8 +351,10 @@ __global__ void __launch_bounds__(kFEX * kFEY) k_smooth_fused(FusedArgs<T> A, co
   PlaneLoad<T> LB[P];
 #pragma unroll
   for (int u = 0; u < P; ++u) load_plane<T, MODE, HAS_E, DOTS, H>(LB[u], A, C, t0 + u);
-  for (int t = t0; t < t1; t += W)
-    march_steps<T, NU, MODE, HAS_E, DOTS, 0>(F, LB, A, C, xs, rs, t, t1, k0, k1);
+  int t = t0;
+  for (; t + W <= t1; t += W)
+    march_steps<T, NU, MODE, HAS_E, DOTS, false, 0>(F, LB, A, C, xs, rs, t, t1, k0, k1);
+  if (t < t1) march_steps<T, NU, MODE, HAS_E, DOTS, true, 0>(F, LB, A, C, xs, rs, t, t1, k0, k1);
   if constexpr (DOTS) {
     if (A.dots) block_partials<kFEX * kFEY, 3, 0u>(A.dsite, F.dacc);
   }
