@@ -20,11 +20,15 @@ namespace {
 // Cell classes of the GFM (ProjectionP.f90:170-174, 212-216).
 constexpr int kNone = 0, kWet = 1, kDry = 2;
 
+// The tests are evaluated without short-circuit (bitwise on the comparison
+// results): they have no side effects, so the class is the same, and the
+// kernels need no branch per test.
 __device__ inline int gfm_class(double vof, double vofL, double phiL, double epsi) {
   const double one_m = 1.0 - epsi;
-  if ((vofL >= 0.5 && vof > one_m) || (phiL < 0.0 && vof <= one_m && vof > epsi)) return kWet;
-  if ((vofL < 0.5 && vof > one_m) || (phiL >= 0.0 && vof <= one_m && vof > epsi)) return kDry;
-  return kNone;
+  const bool full = vof > one_m, cut = (vof <= one_m) & (vof > epsi);
+  const bool wet = ((vofL >= 0.5) & full) | ((phiL < 0.0) & cut);
+  const bool dry = ((vofL < 0.5) & full) | ((phiL >= 0.0) & cut);
+  return wet ? kWet : (dry ? kDry : kNone);
 }
 
 struct GfmArgs {
@@ -80,63 +84,80 @@ __device__ inline int gfm_face(const GfmArgs& A, int f, int a, int b, int c, dou
   return -kDry;
 }
 
-// GFM coefficients of the six faces of every cell.  Each thread walks a
-// column up k; the P cell's values and class are formed once per cell.  The
-// arithmetic is gfm_face's, expression for expression.
+// GFM coefficients of the six faces of every cell; KC cells of a column per
+// thread.  The arithmetic is gfm_face's, expression for expression.  Every
+// load of a cell (its own fields, the six neighbours', Dp on both faces of
+// each direction) is issued before any class test, at addresses that are
+// always valid (halo arrays), and the reference's branches select among the
+// results: neighbour loads behind the class test would add a memory latency
+// per cell.
+template <int KC>
 __global__ void __launch_bounds__(256) k_gfm(GfmArgs A) {
 #pragma clang fp contract(off)
-  const Col3 q = col_of(A.I, A.J, A.K);
+  const Col3 q = col_of<KC>(A.I, A.J, A.K);
   if (!q.in) return;
   const int a = q.a, b = q.b;
   const int64_t IJ = (int64_t)A.I * A.J, SXY = (int64_t)A.SX * A.SY;
   int64_t n = (int64_t)a + (int64_t)A.I * ((int64_t)b + (int64_t)A.J * q.c0);
   int64_t h = (int64_t)(a + 1) + (int64_t)A.SX * ((int64_t)(b + 1) + (int64_t)A.SY * (q.c0 + 1));
-  for (int c = q.c0; c < q.c1; ++c, n += IJ, h += SXY) {
-    const double fP = A.phiL[h];
-    const int cP = gfm_class(A.vof[h], A.vofL[h], fP, A.epsi);
+#pragma unroll
+  for (int cc = 0; cc < KC; ++cc, n += IJ, h += SXY) {
+    const int c = q.c0 + cc;
+    if (c >= q.c1) break;
+    const int64_t st[3] = {1, (int64_t)A.SX, SXY};
+    const double fP = A.phiL[h], vP = A.vof[h], lP = A.vofL[h];
+    double fQ[6], vQ[6], lQ[6], Dp[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      const int dir = f % 3;
+      const int64_t hq = f < 3 ? h - st[dir] : h + st[dir];
+      fQ[f] = A.phiL[hq];
+      vQ[f] = A.vof[hq];
+      lQ[f] = A.vofL[hq];
+      Dp[f] = A.dp[dir][f < 3 ? h - st[dir] : h];
+    }
+    const int cP = gfm_class(vP, lP, fP, A.epsi);
     A.cls[h] = (uint8_t)cP;
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
       const int dir = f % 3;
       const int side = f < 3 ? -1 : 1;
-      const int64_t st = dir == 0 ? 1 : (dir == 1 ? (int64_t)A.SX : SXY);
       const int coord = dir == 0 ? a : (dir == 1 ? b : c);
       // DGrid: TGrid%d(max(im,1)) with im = i-1 on the minus side (:151-153)
       const double dgrid = A.dgrid[dir][side < 0 ? (coord > 0 ? coord - 1 : 0) : coord];
-      const double Dp = A.dp[dir][side < 0 ? h - st : h];
-      const double pref = Dp / dgrid * A.betaP * A.betaM;  // :194 / :235
-      double coef = 0.0;
-      if (cP != kNone) {
-        const int64_t hq = h + side * st;
-        const double fQ = A.phiL[hq];
-        const double lam = fabs(fP) / (fabs(fP) + fabs(fQ) + 1.0e-30);  // :155
-        const int cQ = gfm_class(A.vof[hq], A.vofL[hq], fQ, A.epsi);
-        double beta = 0.0;
-        int r;
-        if (cP == kWet) {
-          if (cQ == kDry) { beta = lam * A.betaM + (1.0 - lam) * A.betaP; r = kWet; }  // :176
-          else if (cQ == kWet) { beta = A.betaM; r = kWet; }                          // :181
-          else r = -kWet;
-        } else {
-          if (cQ == kWet) { beta = lam * A.betaP + (1.0 - lam) * A.betaM; r = kDry; }  // :218
-          else if (cQ == kDry) { beta = A.betaP; r = kDry; }                         // :223
-          else r = -kDry;
-        }
-        if (r > 0) {
-          coef = pref / beta;
-        } else {
-          const uint32_t slot = atomicAdd(A.stale_count, 1u);
-          if (slot < A.stale_cap) {
-            A.stale[3 * (int64_t)slot + 0] = (int32_t)n;
-            A.stale[3 * (int64_t)slot + 1] = f;
-            A.stale[3 * (int64_t)slot + 2] = -r;
-          }
+      const double pref = Dp[f] / dgrid * A.betaP * A.betaM;  // :194 / :235
+      const int cQ = gfm_class(vQ[f], lQ[f], fQ[f], A.epsi);
+      // wet P: dry Q lam*BetaM + (1-lam)*BetaP (:176), wet Q BetaM (:181);
+      // dry P: wet Q lam*BetaP + (1-lam)*BetaM (:218), dry Q BetaP (:223).
+      // Operands are selected, the arithmetic is the reference's.
+      const bool pw = cP == kWet;
+      const double b1 = pw ? A.betaM : A.betaP, b2 = pw ? A.betaP : A.betaM;
+      const bool other = pw ? cQ == kDry : cQ == kWet;
+      const bool same = cQ == cP;
+      // lambda (:155) enters only on faces between a wet and a dry cell,
+      // a thin layer: its division is skipped by waves that have none
+      double beta = same ? b1 : 1.0;
+      if (other) {
+        const double lam = fabs(fP) / (fabs(fP) + fabs(fQ[f]) + 1.0e-30);  // :155
+        beta = lam * b1 + (1.0 - lam) * b2;
+      }
+      const bool assigns = (cP != kNone) & (other | same);
+      const double coef = assigns ? pref / beta : 0.0;
+      // P classified, neighbour not: the previous beta in loop order
+      if (cP != kNone && cQ == kNone) {
+        const uint32_t slot = atomicAdd(A.stale_count, 1u);
+        if (slot < A.stale_cap) {
+          A.stale[3 * (int64_t)slot + 0] = (int32_t)n;
+          A.stale[3 * (int64_t)slot + 1] = f;
+          A.stale[3 * (int64_t)slot + 2] = cP;
         }
       }
       A.pocoef[(int64_t)f * A.N + n] = coef;
     }
   }
 }
+// cells per thread of k_gfm
+constexpr int kGfmK = 1;
 
 // Whether face f of 0-based cell (a,b,c) assigns BetaW (kWet) or BetaD
 // (kDry) in the reference's loop, from the classes k_gfm stored: both the
@@ -369,7 +390,23 @@ __global__ void __launch_bounds__(256) k_operator(OpArgs A) {
       const int c = q.c0 + cc;
       if (c >= q.c1) break;
       const double dz = A.dzp[c];
+      // every load the row needs in every case, issued before any test at
+      // addresses that are always valid (a neighbour past the domain edge
+      // reads the cell itself); the reference's conditions then select
+      const bool hW = a > 0, hS = b > 0, hB = c > 0;
+      const bool hE = a < A.I - 1, hN = b < A.J - 1, hT = c < A.K - 1;
+      const int64_t nW = hW ? n - 1 : n, nS = hS ? n - A.I : n, nB = hB ? n - IJ : n;
+      const int64_t nE = hE ? n + 1 : n, nN = hN ? n + A.I : n, nT = hT ? n + IJ : n;
       const int ctP = A.ct[n];
+      const int ctW = A.ct[nW], ctS = A.ct[nS], ctB = A.ct[nB];
+      const int ctE = A.ct[nE], ctN = A.ct[nN], ctT = A.ct[nT];
+      const double E = A.pocoef[3 * N + n], Nn = A.pocoef[4 * N + n], T = A.pocoef[5 * N + n];
+      const double Ew = A.pocoef[3 * N + nW], Ns = A.pocoef[4 * N + nS], Tb = A.pocoef[5 * N + nB];
+      const double eeP = A.ee[h], eeW = A.ee[h - 1], neP = A.ne[h], neS = A.ne[h - A.SX];
+      const double teP = A.te[h], teB = A.te[h - SXY];
+      const double uP = A.u[h], uW = A.u[h - 1], vP = A.v[h], vS = A.v[h - A.SX];
+      const double wP = A.w[h], wB = A.w[h - SXY];
+      const double vofP = A.vof[h], vofOP = A.vofO[h];
       double cxv = 0, cyv = 0, czv = 0, cW = 0, cS = 0, cB = 0, ex = 0, rhs = 0, dval = 0;
       bool act = false;
       if (ctP != 2) {
@@ -378,27 +415,26 @@ __global__ void __launch_bounds__(256) k_operator(OpArgs A) {
         const double* pcW = A.pocoef;
         const double* pcS = A.pocoef + N;
         const double* pcB = A.pocoef + 2 * N;
-        const double E = A.pocoef[3 * N + n], Nn = A.pocoef[4 * N + n], T = A.pocoef[5 * N + n];
-        const bool wS = a > 0 ? A.ct[n - 1] == 2 : A.fl[0] == 0;
-        const bool sS = b > 0 ? A.ct[n - A.I] == 2 : A.fl[2] == 0;
-        const bool bS = c > 0 ? A.ct[n - IJ] == 2 : A.fl[4] == 0;
-        const bool eS = a < A.I - 1 ? A.ct[n + 1] == 2 : A.fl[1] == 0;
-        const bool nS = b < A.J - 1 ? A.ct[n + A.I] == 2 : A.fl[3] == 0;
-        const bool tS = c < A.K - 1 ? A.ct[n + IJ] == 2 : A.fl[5] == 0;
+        const bool wS = hW ? ctW == 2 : A.fl[0] == 0;
+        const bool sS = hS ? ctS == 2 : A.fl[2] == 0;
+        const bool bS = hB ? ctB == 2 : A.fl[4] == 0;
+        const bool eS = hE ? ctE == 2 : A.fl[1] == 0;
+        const bool nS = hN ? ctN == 2 : A.fl[3] == 0;
+        const bool tS = hT ? ctT == 2 : A.fl[5] == 0;
         // couplings to active neighbours: the lower cell's E/N/T coefficient
-        if (a < A.I - 1 && A.ct[n + 1] != 2) cxv = E * A.ee[h] * dy * dz;
-        if (b < A.J - 1 && A.ct[n + A.I] != 2) cyv = Nn * A.ne[h] * dx * dz;
-        if (c < A.K - 1 && A.ct[n + IJ] != 2) czv = T * A.te[h] * dx * dy;
-        if (a > 0 && A.ct[n - 1] != 2) cW = A.pocoef[3 * N + n - 1] * A.ee[h - 1] * dy * dz;
-        if (b > 0 && A.ct[n - A.I] != 2) cS = A.pocoef[4 * N + n - A.I] * A.ne[h - A.SX] * dx * dz;
-        if (c > 0 && A.ct[n - IJ] != 2) cB = A.pocoef[5 * N + n - IJ] * A.te[h - SXY] * dx * dy;
+        cxv = (hE && ctE != 2) ? E * eeP * dy * dz : 0.0;
+        cyv = (hN && ctN != 2) ? Nn * neP * dx * dz : 0.0;
+        czv = (hT && ctT != 2) ? T * teP * dx * dy : 0.0;
+        cW = (hW && ctW != 2) ? Ew * eeW * dy * dz : 0.0;
+        cS = (hS && ctS != 2) ? Ns * neS * dx * dz : 0.0;
+        cB = (hB && ctB != 2) ? Tb * teB * dx * dy : 0.0;
         // diagonal-only terms with the row's own coefficient (:379-430)
-        if (wS) ex = ex + pcW[n] * A.ee[h - 1] * dy * dz;
-        if (sS) ex = ex + pcS[n] * A.ne[h - A.SX] * dx * dz;
-        if (bS) ex = ex + pcB[n] * A.te[h - SXY] * dx * dy;
-        if (eS) ex = ex + E * A.ee[h] * dy * dz;
-        if (nS) ex = ex + Nn * A.ne[h] * dx * dz;
-        if (tS) ex = ex + T * A.te[h] * dx * dy;
+        if (wS) ex = ex + pcW[n] * eeW * dy * dz;
+        if (sS) ex = ex + pcS[n] * neS * dx * dz;
+        if (bS) ex = ex + pcB[n] * teB * dx * dy;
+        if (eS) ex = ex + E * eeP * dy * dz;
+        if (nS) ex = ex + Nn * neP * dx * dz;
+        if (tS) ex = ex + T * teP * dx * dy;
         double d = cW;
         d = d + cS;
         d = d + cB;
@@ -410,10 +446,10 @@ __global__ void __launch_bounds__(256) k_operator(OpArgs A) {
         dval = act ? d : 0.0;
         if (!is_finite(d)) v[4] = 1.0;
         // RHS (:521-546)
-        rhs = -dy * dz * (A.ee[h] * A.u[h] - A.ee[h - 1] * A.u[h - 1]) -
-              dx * dz * (A.ne[h] * A.v[h] - A.ne[h - A.SX] * A.v[h - A.SX]) -
-              dx * dy * (A.te[h] * A.w[h] - A.te[h - SXY] * A.w[h - SXY]);
-        rhs = rhs + (A.vof[h] - A.vofO[h]) * dx * dy * dz / A.dt;
+        rhs = -dy * dz * (eeP * uP - eeW * uW) -
+              dx * dz * (neP * vP - neS * vS) -
+              dx * dy * (teP * wP - teB * wB);
+        rhs = rhs + (vofP - vofOP) * dx * dy * dz / A.dt;
         if (A.fl[0] == 0 && a == 0) rhs = rhs + pcW[n] * A.bcv[0][b + (int64_t)A.J * c];
         if (A.fl[1] == 0 && a == A.I - 1) rhs = rhs + E * A.bcv[1][b + (int64_t)A.J * c];
         if (A.fl[2] == 0 && b == 0) rhs = rhs + pcS[n] * A.bcv[2][a + (int64_t)A.I * c];
@@ -493,7 +529,7 @@ sp_status assemble(sp_ctx* c, const sp_inputs& in, int64_t* stale_out) {
     G.col_prefix = c->col_prefix;
     G.cls = c->gfm_cls;
     timer_begin(c, "gfm", (double)D.N * (3 * 8 + 3 * 8 + 6 * 8), &slot);
-    hipLaunchKernelGGL(k_gfm, col_grid(D.I, D.J, D.K), blk, 0, c->stream, G);
+    hipLaunchKernelGGL(k_gfm<kGfmK>, col_grid_k<kGfmK>(D.I, D.J, D.K), blk, 0, c->stream, G);
     timer_end(c, slot);
     // faces next to unclassified cells take the previous beta in loop
     // order; the three kernels read the count on the device and return at

@@ -69,51 +69,68 @@ __device__ inline double pval(const CorrArgs& A, int64_t n, double shift) {
 
 // Correction of the faces on the + side of every cell, and the interior of
 // Proj%Pp (DeltaPressureGetValues, ProjectionP.f90:591-613).  Each thread
-// walks a column of cells up k.
+// handles KC cells of a column.  Every load of a cell is issued up front at
+// an address that is always valid (a neighbour past the domain edge reads
+// the cell itself), and the reference's conditions select among the loaded
+// values: loads behind data-dependent branches would cost a memory latency
+// per branch level.
+template <int KC>
 __global__ void __launch_bounds__(256) k_correct(CorrArgs A) {
 #pragma clang fp contract(off)
-  const Col3 q = col_of(A.I, A.J, A.K);
+  const Col3 q = col_of<KC>(A.I, A.J, A.K);
   if (!q.in) return;
   const int a = q.a, b = q.b;
   const double shift = A.S->shift;
   const int64_t IJ = (int64_t)A.I * A.J, N = A.N, SXY = (int64_t)A.SX * A.SY;
+  const bool hasE = a < A.I - 1, hasN = b < A.J - 1;
+  const int64_t oE = hasE ? 1 : 0, oN = hasN ? A.I : 0;
   int64_t n = a + (int64_t)A.I * (b + (int64_t)A.J * q.c0);
   int64_t h = (a + 1) + (int64_t)A.SX * ((b + 1) + (int64_t)A.SY * (q.c0 + 1));
-  for (int c = q.c0; c < q.c1; ++c, n += IJ, h += SXY) {
+#pragma unroll
+  for (int cc = 0; cc < KC; ++cc, n += IJ, h += SXY) {
+    const int c = q.c0 + cc;
+    if (c >= q.c1) break;
+    const bool hasT = c < A.K - 1;
+    const int64_t mE = n + oE, mN = n + oN, mT = hasT ? n + IJ : n;
+    const bool aP = A.active[n] != 0, aE = A.active[mE] != 0, aN = A.active[mN] != 0,
+               aT = A.active[mT] != 0;
+    const double xP = A.x[n], xE = A.x[mE], xN = A.x[mN], xT = A.x[mT];
+    double dP = 0.0, dE_ = 0.0, dN_ = 0.0, dT_ = 0.0;
+    if (A.xd) {  // uniform
+      dP = A.xd[n]; dE_ = A.xd[mE]; dN_ = A.xd[mN]; dT_ = A.xd[mT];
+    }
+    const int32_t mu = A.mu[n], ctu = A.ctu[n], mv = A.mv[n], ctv = A.ctv[n], mw = A.mw[n],
+                  ctw = A.ctw[n];
+    const double cE = A.pocoef[3 * N + n], cN = A.pocoef[4 * N + n], cT = A.pocoef[5 * N + n];
+    const double usv = A.us[h], vsv = A.vs[h], wsv = A.ws[h];
+    // pval: 0 in inactive cells, x - shift (+ the refinement) otherwise
+    auto pv = [&](bool act, double xv, double dv) {
+      return act ? (A.xd ? (xv - shift) + dv : xv - shift) : 0.0;
+    };
+    const double pP = pv(aP, xP, dP);
     // Pp(i+1)-Pp(i): between two active cells the gauge cancels, so the
     // difference is taken on the solver's iterate, whose stiff-phase values
     // sit near zero; faces touching inactive cells or walls use the output
     // field with its ghosts (ProjectionP.f90:591-643).
-    const bool aP = A.active[n] != 0;
-    const double pP = pval(A, n, shift);
-    auto dpres = [&](bool inside, int64_t m, int fl) {
-      if (inside && aP && A.active[m])
-        return A.xd ? (A.x[m] - A.x[n]) + (A.xd[m] - A.xd[n]) : A.x[m] - A.x[n];
-      const double pN = inside ? pval(A, m, shift) : (double)fl * pP;
-      return pN - pP;
+    auto dpres = [&](bool inside, bool am, double xm, double dm, int fl) {
+      const double both = A.xd ? (xm - xP) + (dm - dP) : xm - xP;
+      const double pN = inside ? pv(am, xm, dm) : (double)fl * pP;
+      return (inside && aP && am) ? both : pN - pP;
     };
-    const double dE = dpres(a < A.I - 1, n + 1, A.fl[1]);
-    const double dN = dpres(b < A.J - 1, n + A.I, A.fl[3]);
-    const double dT = dpres(c < A.K - 1, n + IJ, A.fl[5]);
-    double uu = 0.0, vv = 0.0, ww = 0.0;
-    if (A.mu[n] != 1 && A.ctu[n] != 2) {
-      const double g = dE * A.pocoef[3 * N + n];
-      uu = A.us[h] - g;
-    }
-    if (A.mv[n] != 1 && A.ctv[n] != 2) {
-      const double g = dN * A.pocoef[4 * N + n];
-      vv = A.vs[h] - g;
-    }
-    if (A.mw[n] != 1 && A.ctw[n] != 2) {
-      const double g = dT * A.pocoef[5 * N + n];
-      ww = A.ws[h] - g;
-    }
+    const double dE = dpres(hasE, aE, xE, dE_, A.fl[1]);
+    const double dN = dpres(hasN, aN, xN, dN_, A.fl[3]);
+    const double dT = dpres(hasT, aT, xT, dT_, A.fl[5]);
+    const double uu = (mu != 1 && ctu != 2) ? usv - dE * cE : 0.0;
+    const double vv = (mv != 1 && ctv != 2) ? vsv - dN * cN : 0.0;
+    const double ww = (mw != 1 && ctw != 2) ? wsv - dT * cT : 0.0;
     A.u[h] = uu;
     A.v[h] = vv;
     A.w[h] = ww;
     A.Pp[h] = pP;
   }
 }
+// cells per thread of k_correct
+constexpr int kCorrK = 1;
 
 // Ghost entries of Proj%Pp: faces flag*Pp(inner) (ProjectionP.f90:620-643),
 // edges and corners 0.  blockIdx.z selects the face: 0/1 bottom/top planes
@@ -217,7 +234,7 @@ sp_status correct(sp_ctx* c, const sp_inputs& in, sp_outputs& out, sp_stats* st)
   }
   // x, mask, masks x6, PoCoef x3, u*, v*, w* -> u, v, w, Pp
   timer_begin(c, "correct", NB * (8 + 1 + 6 * 4 + 3 * 8 + 3 * 8 + 3 * 8 + 8), &slot);
-  hipLaunchKernelGGL(k_correct, cg, fb, 0, c->stream, A);
+  hipLaunchKernelGGL(k_correct<kCorrK>, col_grid_k<kCorrK>(D.I, D.J, D.K), fb, 0, c->stream, A);
   timer_end(c, slot);
   {
     const int gx = std::max(D.I + 2, D.J), gy = std::max(D.J + 2, D.K);
