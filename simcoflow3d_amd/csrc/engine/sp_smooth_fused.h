@@ -53,13 +53,20 @@ struct FusedArgs {
   RedSite dsite;
 };
 
-template <int NU>
+// Coarse levels too small in (i, j) to fill the chip with extended 40 x 24
+// tiles march with 16 x 16 (an 8 x 8 interior at NU = 2): four waves per
+// workgroup, several workgroups per CU.
+constexpr int kFEXs = 16, kFEYs = 16;
+
+template <int NU, int EX = kFEX, int EY = kFEY>
 struct FusedShape {
   static constexpr int H = 2 * NU;
-  static constexpr int TX = kFEX - 2 * H;
-  static constexpr int TY = kFEY - 2 * H;
+  static constexpr int TX = EX - 2 * H;
+  static constexpr int TY = EY - 2 * H;
   static constexpr int W = H + 2;  // register window (planes)
   static constexpr int R = W;      // LDS ring (planes)
+  static_assert(TX > 0 && TY > 0 && TX % 2 == 0 && TY % 2 == 0,
+                "the restriction pairs cells inside the tile interior");
 };
 
 // Barrier that orders LDS only: global loads issued for later planes stay in
@@ -152,11 +159,11 @@ struct FusedState {
 // One pipeline step: plane t enters (from L), half-sweeps on the planes
 // behind it, output/residual of plane t - H, restriction after the barrier.
 // U = position of this step in the W-step unrolled march (t - t0 = U mod W).
-template <typename T, int NU, int MODE, bool DOTS, int U>
+template <typename T, int NU, int MODE, bool DOTS, int U, int EX, int EY>
 __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>& L,
                                   const FusedArgs<T>& A, const FusedCol& C,
-                                  T (*xs)[kFEY + 2][kFEX + 2], T (*rs)[kFEY][kFEX], int t, int k0, int k1) {
-  using Sh = FusedShape<NU>;
+                                  T (*xs)[EY + 2][EX + 2], T (*rs)[EY][EX], int t, int k0, int k1) {
+  using Sh = FusedShape<NU, EX, EY>;
   constexpr int H = Sh::H, W = Sh::W, R = Sh::R;
   static_assert(R == W && (W % 2) == 0, "ring slots are compile-time only for R == W, W even");
   auto S = [](int o) constexpr { return (o + W - U) % W; };        // register slot
@@ -266,9 +273,9 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
   }
 }
 
-// Planes of loads in flight: FP32 three, FP64 one (register budget).  The
-// window length W is a multiple of it, so the buffer of every step of the
-// unrolled march is a compile-time choice.
+// Planes of loads in flight: FP32 three, FP64 one (register budget; six
+// planes in FP32 measured slower).  The window length W is a multiple of it,
+// so the buffer of every step of the unrolled march is a compile-time choice.
 template <typename T, int NU>
 constexpr int fused_pipe() {
   return sizeof(T) == 4 ? (FusedShape<NU>::W % 3 == 0 ? 3 : 2) : 1;
@@ -281,29 +288,29 @@ constexpr int fused_pipe() {
 // path per step would merge into the loop head with fewer loads in flight,
 // and the compiler's wait counts there would drain the load pipeline on
 // every turn.
-template <typename T, int NU, int MODE, bool HAS_E, bool DOTS, bool CHECK, int U>
+template <typename T, int NU, int MODE, bool HAS_E, bool DOTS, bool CHECK, int U, int EX, int EY>
 __device__ inline void march_steps(FusedState<T, NU, MODE>& F, PlaneLoad<T> (&LB)[fused_pipe<T, NU>()],
                                    const FusedArgs<T>& A, const FusedCol& C,
-                                   T (*xs)[kFEY + 2][kFEX + 2], T (*rs)[kFEY][kFEX], int t, int t1,
+                                   T (*xs)[EY + 2][EX + 2], T (*rs)[EY][EX], int t, int t1,
                                    int k0, int k1) {
   if constexpr (U < FusedShape<NU>::W) {
     if (CHECK && t + U >= t1) return;
     constexpr int P = fused_pipe<T, NU>();
     static_assert(FusedShape<NU>::W % P == 0, "load buffers must tile the window");
     PlaneLoad<T>& L = LB[U % P];
-    fused_step<T, NU, MODE, DOTS, U>(F, L, A, C, xs, rs, t + U, k0, k1);
+    fused_step<T, NU, MODE, DOTS, U, EX, EY>(F, L, A, C, xs, rs, t + U, k0, k1);
     load_plane<T, MODE, HAS_E, DOTS, FusedShape<NU>::H>(L, A, C, t + U + P);
-    march_steps<T, NU, MODE, HAS_E, DOTS, CHECK, U + 1>(F, LB, A, C, xs, rs, t, t1, k0, k1);
+    march_steps<T, NU, MODE, HAS_E, DOTS, CHECK, U + 1, EX, EY>(F, LB, A, C, xs, rs, t, t1, k0, k1);
   }
 }
 
-template <typename T, int NU, int MODE, bool FINE, bool HAS_E>
-__global__ void __launch_bounds__(kFEX * kFEY) k_smooth_fused(FusedArgs<T> A, const Scalars* S) {
+template <typename T, int NU, int MODE, bool FINE, bool HAS_E, int EX = kFEX, int EY = kFEY>
+__global__ void __launch_bounds__(EX * EY) k_smooth_fused(FusedArgs<T> A, const Scalars* S) {
   if (S->done) return;
-  using Sh = FusedShape<NU>;
+  using Sh = FusedShape<NU, EX, EY>;
   constexpr int H = Sh::H, TX = Sh::TX, TY = Sh::TY, W = Sh::W, R = Sh::R;
-  __shared__ T xs[R][kFEY + 2][kFEX + 2];
-  __shared__ T rs[2][kFEY][kFEX];
+  __shared__ T xs[R][EY + 2][EX + 2];
+  __shared__ T rs[2][EY][EX];
   uint32_t bx, by, bz;
   xcd_block3(bx, by, bz);
   FusedCol C;
@@ -333,9 +340,8 @@ __global__ void __launch_bounds__(kFEX * kFEY) k_smooth_fused(FusedArgs<T> A, co
   }
   F.racc = T(0);
   F.dacc[0] = F.dacc[1] = F.dacc[2] = 0.0;
-  for (int q = C.ty * kFEX + C.tx; q < R * (kFEY + 2) * (kFEX + 2); q += kFEX * kFEY)
-    (&xs[0][0][0])[q] = T(0);
-  for (int q = C.ty * kFEX + C.tx; q < 2 * kFEY * kFEX; q += kFEX * kFEY) (&rs[0][0][0])[q] = T(0);
+  for (int q = C.ty * EX + C.tx; q < R * (EY + 2) * (EX + 2); q += EX * EY) (&xs[0][0][0])[q] = T(0);
+  for (int q = C.ty * EX + C.tx; q < 2 * EY * EX; q += EX * EY) (&rs[0][0][0])[q] = T(0);
   // cz of the plane below the first loaded one (bottom face of that plane):
   // logical plane 1 of the first step, physical slot 1
   {
@@ -353,10 +359,11 @@ __global__ void __launch_bounds__(kFEX * kFEY) k_smooth_fused(FusedArgs<T> A, co
   for (int u = 0; u < P; ++u) load_plane<T, MODE, HAS_E, DOTS, H>(LB[u], A, C, t0 + u);
   int t = t0;
   for (; t + W <= t1; t += W)
-    march_steps<T, NU, MODE, HAS_E, DOTS, false, 0>(F, LB, A, C, xs, rs, t, t1, k0, k1);
-  if (t < t1) march_steps<T, NU, MODE, HAS_E, DOTS, true, 0>(F, LB, A, C, xs, rs, t, t1, k0, k1);
+    march_steps<T, NU, MODE, HAS_E, DOTS, false, 0, EX, EY>(F, LB, A, C, xs, rs, t, t1, k0, k1);
+  if (t < t1)
+    march_steps<T, NU, MODE, HAS_E, DOTS, true, 0, EX, EY>(F, LB, A, C, xs, rs, t, t1, k0, k1);
   if constexpr (DOTS) {
-    if (A.dots) block_partials<kFEX * kFEY, 3, 0u>(A.dsite, F.dacc);
+    if (A.dots) block_partials<EX * EY, 3, 0u>(A.dsite, F.dacc);
   }
 }
 

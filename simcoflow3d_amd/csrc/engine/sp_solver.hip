@@ -884,9 +884,62 @@ void launch_cheb(sp_ctx* c, const Level<T>& L, int m, bool from_zero, bool fine)
     hipMemcpyAsync(L.x, in, sizeof(T) * (size_t)L.n, hipMemcpyDeviceToDevice, c->stream);
 }
 
+// One smoother leg with extended tiles of EX x EY threads; per_cu: how many
+// such workgroups a CU holds at once (registers and LDS).
+template <typename T, int NU, int EX, int EY>
+void launch_fused_tile(sp_ctx* c, const Level<T>& L, FusedArgs<T>& A, bool fine, int mode,
+                       int per_cu) {
+  using Sh = FusedShape<NU, EX, EY>;
+  const int gx = (L.nx + Sh::TX - 1) / Sh::TX, gy = (L.ny + Sh::TY - 1) / Sh::TY;
+  // k chunk: even (coarse cells never straddle chunks).  A workgroup marches
+  // kchunk + 2H planes one after the other, so pick the chunk that minimises
+  // rounds-of-workgroups x planes marched.
+  {
+    const int64_t tiles = (int64_t)gx * gy, slots = (int64_t)std::max(1, c->num_cu) * per_cu;
+    const int kmax = (L.nz + 1) & ~1;
+    int best = kmax;
+    int64_t best_cost = -1;
+    for (int kc = 2; kc <= kmax; kc += 2) {
+      const int64_t blocks = tiles * ((L.nz + kc - 1) / kc);
+      const int64_t cost = ((blocks + slots - 1) / slots) * (kc + 2 * Sh::H);
+      if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = kc; }
+    }
+    A.kchunk = best;
+  }
+  const dim3 grd(gx, gy, (L.nz + A.kchunk - 1) / A.kchunk), blk(EX, EY, 1);
+  int slot;
+  if (A.dots) c->dots_nb = grd.x * grd.y * grd.z;
+  // algorithmic words per cell: pre b + 3 faces -> x (+ coarse b);
+  // post x_pre + 3 faces + b -> x (+ coarse x); the dot products add q
+  // (FP64)
+  const double words = mode == 0 ? 5.0 + 0.125 : 6.0 + 0.125;
+  timer_begin(c, fine ? (mode == 0 ? "mg_pre_fine" : "mg_post_fine")
+                      : (mode == 0 ? "mg_pre_coarse" : "mg_post_coarse"),
+              level_bytes(L, words + (L.e ? 1.0 : 0.0)) + (A.dots ? 8.0 * (double)L.n : 0.0),
+              &slot);
+  const Scalars* S = c->S;
+#define SP_FUSED_LAUNCH(MODE_, FINE_)                                                         \
+  do {                                                                                        \
+    if (A.e)                                                                                  \
+      hipLaunchKernelGGL((k_smooth_fused<T, NU, MODE_, FINE_, true, EX, EY>), grd, blk, 0,    \
+                         c->stream, A, S);                                                    \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_smooth_fused<T, NU, MODE_, FINE_, false, EX, EY>), grd, blk, 0,   \
+                         c->stream, A, S);                                                    \
+  } while (0)
+  if (mode == 0) {
+    if (fine) SP_FUSED_LAUNCH(0, true);
+    else SP_FUSED_LAUNCH(0, false);
+  } else {
+    if (fine) SP_FUSED_LAUNCH(1, true);
+    else SP_FUSED_LAUNCH(1, false);
+  }
+#undef SP_FUSED_LAUNCH
+  timer_end(c, slot);
+}
+
 template <typename T, int NU>
 void launch_fused(sp_ctx* c, Level<T>* lv, int l, int mode) {
-  using Sh = FusedShape<NU>;
   const Level<T>& L = lv[l];
   const Level<T>& C = lv[l + 1];
   FusedArgs<T> A{};
@@ -902,51 +955,15 @@ void launch_fused(sp_ctx* c, Level<T>* lv, int l, int mode) {
     A.dq = c->q;
     A.dsite = red_site(c, kSiteDots);
   }
-  const int gx = (L.nx + Sh::TX - 1) / Sh::TX, gy = (L.ny + Sh::TY - 1) / Sh::TY;
-  // k chunk: even (coarse cells never straddle chunks).  One workgroup fills a
-  // CU, and a workgroup marches kchunk + 2H planes one after the other, so
-  // pick the chunk that minimises rounds-of-workgroups x planes marched.
-  {
-    const int64_t tiles = (int64_t)gx * gy, ncu = std::max(1, c->num_cu);
-    const int kmax = (L.nz + 1) & ~1;
-    int best = kmax;
-    int64_t best_cost = -1;
-    for (int kc = 2; kc <= kmax; kc += 2) {
-      const int64_t blocks = tiles * ((L.nz + kc - 1) / kc);
-      const int64_t cost = ((blocks + ncu - 1) / ncu) * (kc + 2 * Sh::H);
-      if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = kc; }
-    }
-    A.kchunk = best;
-  }
-  const dim3 grd(gx, gy, (L.nz + A.kchunk - 1) / A.kchunk), blk(kFEX, kFEY, 1);
   const bool fine = l == 0;
-  int slot;
-  if (A.dots) c->dots_nb = grd.x * grd.y * grd.z;
-  // algorithmic words per cell: pre b + 3 faces -> x (+ coarse b);
-  // post x_pre + 3 faces + b -> x (+ coarse x); the dot products add q
-  // (FP64)
-  const double words = mode == 0 ? 5.0 + 0.125 : 6.0 + 0.125;
-  timer_begin(c, fine ? (mode == 0 ? "mg_pre_fine" : "mg_post_fine")
-                      : (mode == 0 ? "mg_pre_coarse" : "mg_post_coarse"),
-              level_bytes(L, words + (L.e ? 1.0 : 0.0)) + (A.dots ? 8.0 * (double)L.n : 0.0),
-              &slot);
-  const Scalars* S = c->S;
-#define SP_FUSED_LAUNCH(MODE_, FINE_)                                                         \
-  do {                                                                                        \
-    if (A.e)                                                                                  \
-      hipLaunchKernelGGL((k_smooth_fused<T, NU, MODE_, FINE_, true>), grd, blk, 0, c->stream, A, S); \
-    else                                                                                      \
-      hipLaunchKernelGGL((k_smooth_fused<T, NU, MODE_, FINE_, false>), grd, blk, 0, c->stream, A, S); \
-  } while (0)
-  if (mode == 0) {
-    if (fine) SP_FUSED_LAUNCH(0, true);
-    else SP_FUSED_LAUNCH(0, false);
-  } else {
-    if (fine) SP_FUSED_LAUNCH(1, true);
-    else SP_FUSED_LAUNCH(1, false);
-  }
-#undef SP_FUSED_LAUNCH
-  timer_end(c, slot);
+  // a level of at most 32 x 32 columns has two 32 x 16 tiles: its 10-step
+  // marches (2 planes + the 8-plane pipeline) run on 32 CUs.  Small tiles put
+  // a workgroup on every CU (32^3: 10.5 -> 7.8 us per leg); at 64 x 64 they
+  // measured slower than the large ones (13.0 against 11.7 us)
+  if (!fine && (int64_t)L.nx * L.ny <= 32 * 32)
+    launch_fused_tile<T, NU, kFEXs, kFEYs>(c, L, A, fine, mode, 4);
+  else
+    launch_fused_tile<T, NU, kFEX, kFEY>(c, L, A, fine, mode, 1);
 }
 
 // The V-cycle uses the temporally blocked smoother legs
