@@ -166,30 +166,52 @@ __global__ void __launch_bounds__(256) k_pressure_ghosts(CorrArgs A) {
 // mres of the corrected field (Solver.f90:239-247); the west/south/bottom
 // wall faces (index 0) take the predictor's value, which the correction does
 // not touch.
+// max |div u| / V over the cells; KC cells of a column per thread, every
+// load of the KC cells issued before the arithmetic (the cell index clamped
+// into the column, so no load waits on a test).
+template <int KC>
 __global__ void __launch_bounds__(256) k_divergence(CorrArgs A) {
 #pragma clang fp contract(off)
-  const Col3 q = col_of(A.I, A.J, A.K);
+  const Col3 q = col_of<KC>(A.I, A.J, A.K);
   const int64_t SXY = (int64_t)A.SX * A.SY;
   double v[1] = {0.0};
   if (q.in) {
     const int a = q.a, b = q.b;
     const double dx = A.dxp[a], dy = A.dyp[b];
-    int64_t h = (a + 1) + (int64_t)A.SX * ((b + 1) + (int64_t)A.SY * (q.c0 + 1));
-    for (int c = q.c0; c < q.c1; ++c, h += SXY) {
-      const double dz = A.dzp[c];
-      const double uw = a > 0 ? A.u[h - 1] : A.us[h - 1];
-      const double vs = b > 0 ? A.v[h - A.SX] : A.vs[h - A.SX];
-      const double wb = c > 0 ? A.w[h - SXY] : A.ws[h - SXY];
-      const double r = dy * dz * (A.ee[h] * A.u[h] - A.ee[h - 1] * uw) +
-                       dx * dz * (A.ne[h] * A.v[h] - A.ne[h - A.SX] * vs) +
-                       dx * dy * (A.te[h] * A.w[h] - A.te[h - SXY] * wb);
-      double m = fabs(r) / (dx * dy * dz);
+    // the W / S faces of the first cells are the caller's u* / v* (walls)
+    const double* uw_src = a > 0 ? A.u : A.us;
+    const double* vs_src = b > 0 ? A.v : A.vs;
+    const int64_t h0 = (a + 1) + (int64_t)A.SX * ((b + 1) + (int64_t)A.SY * (q.c0 + 1));
+    double uP[KC], uW[KC], vP[KC], vS[KC], wP[KC], wB[KC], eP[KC], eW[KC], nP[KC], nS[KC],
+        tP[KC], tB[KC], dz[KC];
+#pragma unroll
+    for (int cc = 0; cc < KC; ++cc) {
+      const int c = min(q.c0 + cc, q.c1 - 1);
+      const int64_t h = h0 + SXY * (c - q.c0);
+      const double* wb_src = c > 0 ? A.w : A.ws;
+      uP[cc] = A.u[h]; uW[cc] = uw_src[h - 1];
+      vP[cc] = A.v[h]; vS[cc] = vs_src[h - A.SX];
+      wP[cc] = A.w[h]; wB[cc] = wb_src[h - SXY];
+      eP[cc] = A.ee[h]; eW[cc] = A.ee[h - 1];
+      nP[cc] = A.ne[h]; nS[cc] = A.ne[h - A.SX];
+      tP[cc] = A.te[h]; tB[cc] = A.te[h - SXY];
+      dz[cc] = A.dzp[c];
+    }
+#pragma unroll
+    for (int cc = 0; cc < KC; ++cc) {
+      if (q.c0 + cc >= q.c1) break;
+      const double r = dy * dz[cc] * (eP[cc] * uP[cc] - eW[cc] * uW[cc]) +
+                       dx * dz[cc] * (nP[cc] * vP[cc] - nS[cc] * vS[cc]) +
+                       dx * dy * (tP[cc] * wP[cc] - tB[cc] * wB[cc]);
+      double m = fabs(r) / (dx * dy * dz[cc]);
       if (!(m <= 1.0e300)) m = 1.0e300;  // NaN / Inf
       v[0] = m > v[0] ? m : v[0];
     }
   }
   block_partials<kTileThreads, 1, 1u>(A.site, v);
 }
+// cells per thread of k_divergence
+constexpr int kDivK = 4;
 
 __global__ void __launch_bounds__(kFinThreads) k_divergence_fin(CorrArgs A, uint32_t nb) {
   double t[1];
@@ -243,8 +265,9 @@ sp_status correct(sp_ctx* c, const sp_inputs& in, sp_outputs& out, sp_stats* st)
   }
   A.site = red_site(c, kSiteDiv);
   timer_begin(c, "divergence", NB * 6 * 8, &slot);
-  hipLaunchKernelGGL(k_divergence, cg, fb, 0, c->stream, A);
-  hipLaunchKernelGGL(k_divergence_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A, ncol);
+  hipLaunchKernelGGL(k_divergence<kDivK>, col_grid_k<kDivK>(D.I, D.J, D.K), fb, 0, c->stream, A);
+  hipLaunchKernelGGL(k_divergence_fin, dim3(1), dim3(kFinThreads), 0, c->stream, A,
+                     col_blocks_k<kDivK>(D.I, D.J, D.K));
   timer_end(c, slot);
   (void)st;
   return SP_OK;

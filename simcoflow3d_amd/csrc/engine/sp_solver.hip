@@ -566,22 +566,39 @@ __global__ void __launch_bounds__(256) k_cg_jacobi(CgArgs A) {
   }
 }
 
-// r = b - mean(b) - A x  (x = 0 when from_zero); resets p and q
+// r = b - mean(b) - A x  (x = 0 when from_zero); resets p and q.  KC cells
+// of a column per thread, the mask and b of all of them loaded first (cell
+// index clamped into the column).
+template <int KC>
 __global__ void __launch_bounds__(256) k_cg_residual(CgArgs A, int from_zero) {
-  const Col3 q = col_of(A.I, A.J, A.K);
+  const Col3 q = col_of<KC>(A.I, A.J, A.K);
   Scalars* S = A.S;
   const double bmean = S->bmean;
   double v[3] = {0.0, 0.0, 0.0};  // max |r|/V, non-finite, sum r
   if (q.in) {
     const int64_t IJ = (int64_t)A.I * A.J;
     const double dxy = A.dxp[q.a] * A.dyp[q.b];
-    int64_t n = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c0);
-    for (int c = q.c0; c < q.c1; ++c, n += IJ) {
+    const int64_t n0 = q.a + (int64_t)A.I * (q.b + (int64_t)A.J * q.c0);
+    bool act[KC];
+    double bv[KC], dz[KC];
+#pragma unroll
+    for (int cc = 0; cc < KC; ++cc) {
+      const int c = min(q.c0 + cc, q.c1 - 1);
+      const int64_t n = n0 + IJ * (c - q.c0);
+      act[cc] = A.active[n] != 0;
+      bv[cc] = A.b[n];
+      dz[cc] = A.dzp[c];
+    }
+#pragma unroll
+    for (int cc = 0; cc < KC; ++cc) {
+      const int c = q.c0 + cc;
+      if (c >= q.c1) break;
+      const int64_t n = n0 + IJ * cc;
       double rv = 0.0;
-      if (A.active[n]) {
-        rv = A.b[n] - bmean;
+      if (act[cc]) {
+        rv = bv[cc] - bmean;
         if (!from_zero) rv -= apply_row(A, A.x, q.a, q.b, c, n);
-        const double rr = fabs(rv) / (dxy * A.dzp[c]);
+        const double rr = fabs(rv) / (dxy * dz[cc]);
         v[0] = (rr > v[0] || rr != rr) ? rr : v[0];
         if (!is_finite(rv)) v[1] = 1.0;
         v[2] += rv;
@@ -595,6 +612,8 @@ __global__ void __launch_bounds__(256) k_cg_residual(CgArgs A, int from_zero) {
   }
   block_partials<kTileThreads, 3, 3u>(A.site, v);
 }
+// cells per thread of k_cg_residual
+constexpr int kResK = 4;
 
 __global__ void __launch_bounds__(kFinThreads) k_cg_residual_fin(CgArgs A, uint32_t nb) {
   double out[3];
@@ -1148,9 +1167,10 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st, bool che
     A.site = red_site(c, kSiteResidual);
     timer_begin(c, "cg_residual", NB * (from_zero ? 8 + 1 + 8 * 4 : 8 * 5 + eb + 1 + 8 * 4),
                 &slot);
-    hipLaunchKernelGGL(k_cg_residual, cgrd, blk, 0, c->stream, A, from_zero);
+    hipLaunchKernelGGL(k_cg_residual<kResK>, col_grid_k<kResK>(D.I, D.J, D.K), blk, 0, c->stream,
+                       A, from_zero);
     timer_end(c, slot);
-    finalize(k_cg_residual_fin, kSiteResidual, ncol);
+    finalize(k_cg_residual_fin, kSiteResidual, col_blocks_k<kResK>(D.I, D.J, D.K));
   };
   auto precondition = [&]() {
     if (mg) {
