@@ -152,6 +152,7 @@ template <typename T, int NU, int MODE>
 struct FusedState {
   static constexpr int H = 2 * NU, W = H + 2;
   T xw[W], bw[W], dw[W], rdw[W], czw[W], cxw[W], cxmw[W], cyw[W], cymw[W];
+  bool dpos[W];  // d > 0 of each plane, tested once when the plane enters
   T racc;
   double dacc[3];  // CG dot products (finest post-smoother)
 };
@@ -159,7 +160,7 @@ struct FusedState {
 // One pipeline step: plane t enters (from L), half-sweeps on the planes
 // behind it, output/residual of plane t - H, restriction after the barrier.
 // U = position of this step in the W-step unrolled march (t - t0 = U mod W).
-template <typename T, int NU, int MODE, bool DOTS, int U, int EX, int EY>
+template <typename T, int NU, int MODE, bool HAS_E, bool DOTS, int U, int EX, int EY>
 __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>& L,
                                   const FusedArgs<T>& A, const FusedCol& C,
                                   T (*xs)[EY + 2][EX + 2], T (*rs)[EY][EX], int t, int k0, int k1) {
@@ -189,11 +190,14 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
   d0 = d0 + lcx;
   d0 = d0 + lcy;
   d0 = d0 + lcz;
-  d0 = d0 + le;
+  // the diagonal-only term; without one the sum stays as it is (adding +0
+  // to a sum of non-negative couplings changes nothing)
+  if (HAS_E) d0 = d0 + le;
   T x0 = T(0);
   if (MODE == 1) x0 = d0 > T(0) ? L.xin + L.xc : T(0);
   F.bw[S(0)] = lb; F.cxw[S(0)] = lcx; F.cxmw[S(0)] = lcxm; F.cyw[S(0)] = lcy;
   F.cymw[S(0)] = lcym; F.czw[S(0)] = lcz; F.dw[S(0)] = d0; F.xw[S(0)] = x0;
+  F.dpos[S(0)] = d0 > T(0);
   if (kRcp) F.rdw[S(0)] = d0 > T(0) ? smoother_rcp(d0) : T(0);
   xs[Q(0)][ly][lx] = x0;
 
@@ -222,14 +226,14 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
     // keep v in a register on every lane: a select whose operand is only
     // needed on some lanes would otherwise become an exec-masked branch
     __asm__ volatile("" : "+v"(v));
-    const T xn = (act && F.dw[S(o)] > T(0)) ? v : F.xw[S(o)];
+    const T xn = (act && F.dpos[S(o)]) ? v : F.xw[S(o)];
     F.xw[S(o)] = xn;
     xs[Q(o)][ly][lx] = xn;
   }
 
   if (MODE == 0) {
     T r = T(0);
-    if (wr && F.dw[S(H)] > T(0)) {
+    if (wr && F.dpos[S(H)]) {
       const T xW = xs[Q(H)][ly][lx - 1], xS = xs[Q(H)][ly - 1][lx];
       const T xE = xs[Q(H)][ly][lx + 1], xN = xs[Q(H)][ly + 1][lx];
       T acc = T(0);
@@ -266,9 +270,8 @@ __device__ inline void fused_step(FusedState<T, NU, MODE>& F, const PlaneLoad<T>
     constexpr bool first = (U & 1) == 0;  // q even
     F.racc = first ? ps : F.racc + ps;
     if (!first || q == A.nz - 1) {
-      const uint32_t cidx = (uint32_t)(C.i >> 1) +
-                            (uint32_t)A.cnx * ((uint32_t)(C.j >> 1) + (uint32_t)A.cny * (uint32_t)(q >> 1));
-      A.bc[cidx] = F.racc;
+      // C.ccol: byte offset of the coarse column (i/2, j/2)
+      put_off(A.bc, C.ccol + C.cplane * (uint32_t)(q >> 1), F.racc);
     }
   }
 }
@@ -298,7 +301,7 @@ __device__ inline void march_steps(FusedState<T, NU, MODE>& F, PlaneLoad<T> (&LB
     constexpr int P = fused_pipe<T, NU>();
     static_assert(FusedShape<NU>::W % P == 0, "load buffers must tile the window");
     PlaneLoad<T>& L = LB[U % P];
-    fused_step<T, NU, MODE, DOTS, U, EX, EY>(F, L, A, C, xs, rs, t + U, k0, k1);
+    fused_step<T, NU, MODE, HAS_E, DOTS, U, EX, EY>(F, L, A, C, xs, rs, t + U, k0, k1);
     load_plane<T, MODE, HAS_E, DOTS, FusedShape<NU>::H>(L, A, C, t + U + P);
     march_steps<T, NU, MODE, HAS_E, DOTS, CHECK, U + 1, EX, EY>(F, LB, A, C, xs, rs, t, t1, k0, k1);
   }
@@ -337,6 +340,7 @@ __global__ void __launch_bounds__(EX * EY) k_smooth_fused(FusedArgs<T> A, const 
   for (int o = 0; o < W; ++o) {
     F.xw[o] = F.bw[o] = F.dw[o] = F.rdw[o] = F.czw[o] = F.cxw[o] = F.cxmw[o] = F.cyw[o] =
         F.cymw[o] = T(0);
+    F.dpos[o] = false;
   }
   F.racc = T(0);
   F.dacc[0] = F.dacc[1] = F.dacc[2] = 0.0;
