@@ -223,3 +223,64 @@ def test_reference_interface_mirror_on_halo_cells(gpu_available):
         for n, m in dict(cell_type="Cell_Type", ee_area="EEArea", wllh="WlLh",
                          cell_cent="Cell_Cent", delh="delh").items():
             assert np.array_equal(getattr(o, m), e[n], equal_nan=True), m
+
+
+def test_moving_solid_step_on_device(gpu_available):
+    """The first two steps of the moving-solid geometry update
+    (SolidBody.f90:810-824) on device tensors with no host round trip:
+    interpolateLvS writes phi, vof and the normals of the P, U, V, W grids
+    as halo arrays, and GridPreprocess runs on those same tensors
+    (halo_fields).  Bar: every GridPreprocess output bit-identical to the
+    oracle's GridPreprocess of the same fields in the cell layout."""
+    import torch
+    from tests.test_gpu_lvs import ellipsoid
+    from tests.test_oracle_lvs import body_mesh, rotation
+    size = (24, 20, 28)
+    I, J, K = size
+    print(f"[halo] interpolateLvS -> GridPreprocess on the device, size={size}", flush=True)
+    grids = configs.staggered_spacings(np.full(I, 1.0 / I), np.full(J, 1.0 / J),
+                                       np.full(K, 1.0 / K))
+    body = body_mesh(40, 1.0 / 40, ellipsoid)
+    body.update(cm=np.array([0.52, 0.47, 0.55]), rot=rotation((0.3, -0.2, 0.9)))
+    dbody = dict(body)
+    for n in ("x", "y", "z", "phi"):
+        dbody[n] = tdev(body[n])
+    hs = (I + 2, J + 2, K + 2)
+
+    def zeros(shape, dt=None):
+        return torch.zeros(shape[::-1], dtype=dt or torch.float64, device="cuda:0")
+    P = Projector(size)
+    dev = []
+    for g in grids:
+        lv = {n: tdev(g[n]) for n in ("x", "y", "z", "dx", "dy", "dz")}
+        for n in ("phi", "vof", "nx", "ny", "nz"):
+            lv[n] = zeros(hs)
+        P.interpolate_lvs(dbody, lv, device=True)
+        d = {n: lv[n] for n in ("dx", "dy", "dz", "phi", "nx", "ny", "nz", "vof")}
+        d["vof_l"] = (lv["vof"] * 0.6).contiguous()
+        d["cell_type"] = zeros(size, torch.int32)
+        for n in ("fce", "fcn", "fct"):
+            d[n] = zeros(hs + (3,))
+        for n in ("ee_area", "ne_area", "te_area"):
+            d[n] = zeros(hs)
+        d["cell_cent"] = zeros(size + (3,))
+        d["wllh"] = zeros(size)
+        d["delh"] = zeros(size)
+        dev.append(d)
+    torch.cuda.synchronize()
+    cells = []
+    for d in dev:
+        c = {n: back(d[n]) for n in GEOM_FIELDS}
+        for n in GEOM_HALO_FIELDS:
+            c[n] = np.asfortranarray(interior(c[n]))
+        cells.append(c)
+    exp = orc.grid_preprocess(cells)
+    P.grid_preprocess(dev, device=True, halo_fields=True)
+    got = [{n: back(d[n]) for n in GEOM_FIELDS} for d in dev]
+    P.close()
+    for m in range(4):
+        for n in GEOM_FIELDS:
+            g = interior(got[m][n]) if n in GEOM_HALO_FIELDS else got[m][n]
+            assert np.array_equal(g, exp[m][n], equal_nan=True), (m, n)
+    assert any((exp[m]["cell_type"] == 1).any() for m in range(4))
+    assert any((exp[m]["cell_type"] == 2).any() for m in range(4))
