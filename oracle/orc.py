@@ -186,7 +186,7 @@ def num_threads():
     return int(lib().orc_num_threads())
 
 
-def baseline_project(case, levels=0, pre=2, post=2, coarse_sweeps=4, coarse_scale=0.5,
+def baseline_project(case, levels=0, pre=2, post=2, coarse_sweeps=2, coarse_scale=0.5,
                      div_tol=1e-10, max_iter=200, omega=1.25, omega_coarse=1.5, pin=(1, 1, 1)):
     """The CPU baseline: the same path and multigrid-preconditioned CG as the
     engine, FP64, OpenMP over the host cores.  Returns (seconds, info); info

@@ -414,7 +414,10 @@ void sp_params_default(sp_params* p) {
   p->mg_levels = 0;
   p->mg_pre = 2;
   p->mg_post = 2;
-  p->mg_coarse_sweeps = 4;
+  // two forward and two backward red-black sweep pairs on the coarsest level
+  // (a few cells): one pair costs the 512x256x256 tank a CG iteration, more
+  // than two cost tail time without saving any
+  p->mg_coarse_sweeps = 2;
   p->mg_coarse_scale = 0.5;
   p->mg_omega = 1.25;
   p->mg_omega_coarse = 1.5;
