@@ -251,7 +251,7 @@ inline RedSite red_site(const sp_ctx* c, int id) {
   return r;
 }
 
-constexpr int kFinThreads = 1024;
+constexpr int kFinThreads = 256;
 
 // Flat launch: one cell per thread, 64x4 workgroups.
 inline dim3 flat_grid(int nx, int ny, int nz) {
