@@ -12,7 +12,7 @@
 namespace sp {
 
 constexpr int kMaxLevels = 24;
-constexpr int64_t kTailCells = 4096;  // levels this small run in one workgroup
+constexpr int64_t kTailCells = 512;  // levels this small run in one workgroup
 constexpr int kTailLdsBytes = 144 * 1024;  // LDS of that workgroup (7 arrays per level)
 constexpr int kRedBlockThreads = 256;
 constexpr int kMaxPartials = 1 << 16;  // reduction partials per slot (workgroups)
