@@ -15,7 +15,6 @@ constexpr int kMaxLevels = 24;
 constexpr int64_t kTailCells = 512;  // levels this small run in one workgroup
 constexpr int kTailLdsBytes = 144 * 1024;  // LDS of that workgroup (7 arrays per level)
 constexpr int kRedBlockThreads = 256;
-constexpr int kMaxPartials = 1 << 16;  // reduction partials per slot (workgroups)
 
 // Scalars of the outer CG, resident in device memory so no host round trip
 // is needed between the kernels of one iteration.
@@ -73,9 +72,6 @@ enum CounterSlot : int {
   kCtrAdj2 = 8,
   kCtrCheck = 9,
 };
-
-// Partial-sum slots (each kMaxPartials doubles)
-constexpr int kNumPartSlots = 8;
 
 // One multigrid level (views; the storage is owned by sp_ctx::allocs).
 template <typename T>
