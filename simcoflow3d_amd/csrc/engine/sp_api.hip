@@ -161,16 +161,19 @@ static sp_status ensure_buffers(sp_ctx* c) {
     L0.cx = c->cx; L0.cy = c->cy; L0.cz = c->cz; L0.e_store = c->ee; L0.e = c->ee;
     L0.b = c->r; L0.x = c->z64;
   }
-  // tail: the finest level of at most kTailCells cells from which every
-  // coarser level fits the tail kernel's LDS (seven arrays per level)
+  // tail: the levels from which every coarser level fits the tail kernel's
+  // LDS (seven arrays per level), each finer than the coarsest holding at
+  // most kTailCells cells.  c->levels when not even the coarsest level fits
+  // (a hierarchy cut short by mg_levels, or a large grid that cannot
+  // coarsen): the V-cycle then smooths the coarsest level in global memory.
   {
     const int64_t esz = mixed_mg ? 4 : 8;
-    c->tail_level = c->levels - 1;
+    c->tail_level = c->levels;
     int64_t bytes = 0;
     for (int l = c->levels - 1; l >= 0; --l) {
       const int64_t n = mixed_mg ? c->lf[l].n : c->ld[l].n;
       bytes += 7 * esz * n;
-      if (n > kTailCells || bytes > kTailLdsBytes) break;
+      if (bytes > kTailLdsBytes || (l < c->levels - 1 && n > kTailCells)) break;
       c->tail_level = l;
     }
   }

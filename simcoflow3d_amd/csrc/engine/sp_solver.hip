@@ -995,12 +995,15 @@ bool fused_vcycle(const sp_ctx* c) {
 template <typename T>
 void vcycle(sp_ctx* c, Level<T>* lv) {
   const int L = c->levels, Lt = c->tail_level;
+  // levels with a pre-smoothing leg and a restriction: those above the tail,
+  // or every level but the coarsest when no level fits the tail
+  const int Lp = Lt < L ? Lt : L - 1;
   const int pre = std::max(1, c->prm.mg_pre), post = c->prm.mg_post;
   const bool fused = fused_vcycle(c);
   const bool cheb = (c->prm.flags & SP_FLAG_CHEBYSHEV) != 0;
   dim3 blk(64, 4, 1);
   int slot;
-  for (int l = 0; l < Lt; ++l) {
+  for (int l = 0; l < Lp; ++l) {
     const bool fine = l == 0;
     if (fused) {
       if (pre == 2) launch_fused<T, 2>(c, lv, l, 0);
@@ -1025,7 +1028,18 @@ void vcycle(sp_ctx* c, Level<T>* lv) {
                          view_of(lv[l + 1]), (const Scalars*)c->S);
     timer_end(c, slot);
   }
-  {
+  if (Lt >= L) {
+    // the coarsest level does not fit the tail's LDS: the tail's coarsest
+    // sweeps (red-black pairs from x = 0, then as many black-red pairs)
+    // with the global-memory kernels
+    const int Lc = L - 1;
+    const bool fine = Lc == 0;
+    const int cs = std::max(1, c->prm.mg_coarse_sweeps);
+    launch_gs(c, lv[Lc], 0, 1, fine);
+    launch_gs(c, lv[Lc], 1, 0, fine);
+    for (int s = 1; s < cs; ++s) { launch_gs(c, lv[Lc], 0, 0, fine); launch_gs(c, lv[Lc], 1, 0, fine); }
+    for (int s = 0; s < cs; ++s) { launch_gs(c, lv[Lc], 1, 0, fine); launch_gs(c, lv[Lc], 0, 0, fine); }
+  } else {
     TailArgs<T> A;
     A.first = Lt;
     A.last = L;
@@ -1046,7 +1060,7 @@ void vcycle(sp_ctx* c, Level<T>* lv) {
     hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(1024), 0, c->stream, A, (const Scalars*)c->S);
     timer_end(c, slot);
   }
-  for (int l = Lt - 1; l >= 0; --l) {
+  for (int l = Lp - 1; l >= 0; --l) {
     const bool fine = l == 0;
     if (fused) {
       if (pre == 2) launch_fused<T, 2>(c, lv, l, 1);
@@ -1184,9 +1198,11 @@ sp_status solve(sp_ctx* c, const double* rhs, double tol, sp_stats* st, bool che
   hipError_t e;
   volatile int32_t* hflag = c->hflags;
   // the finest post-smoother forms the CG dot products when the V-cycle has
-  // a fused level above the tail (FP32 V-cycle: the FP64 smoother has no
-  // registers to spare for the extra inputs)
-  c->fuse_dots = mg && mixed && fused_vcycle(c) && c->tail_level > 0;
+  // a fused leg on the finest level (a level above the tail, or above the
+  // coarsest when no level fits the tail) and an FP32 V-cycle (the FP64
+  // smoother has no registers to spare for the extra inputs)
+  const int legs = c->tail_level < c->levels ? c->tail_level : c->levels - 1;
+  c->fuse_dots = mg && mixed && fused_vcycle(c) && legs > 0;
   c->dots_nb = 0;
   hipEvent_t evs[2];
   hipEventCreateWithFlags(&evs[0], hipEventDisableTiming);
