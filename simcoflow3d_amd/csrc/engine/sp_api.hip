@@ -69,9 +69,13 @@ static void timers_collect(sp_ctx* c) {
   c->ev_used.clear();
 }
 
+// Coarsen while every dimension is at least 4 and the level holds more than
+// 64 cells (256^3: 7 levels, down to 4^3; a level more, 2^3, costs tail
+// time without saving CG iterations, while the 512x256x256 tank keeps its
+// 4x2x2 level: stopping at 8x4x4 costs it an iteration).
 static int auto_levels(int I, int J, int K) {
   int L = 1, nx = I, ny = J, nz = K;
-  while (nx >= 4 && ny >= 4 && nz >= 4 && L < kMaxLevels) {
+  while (nx >= 4 && ny >= 4 && nz >= 4 && (int64_t)nx * ny * nz > 64 && L < kMaxLevels) {
     nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
     ++L;
   }
