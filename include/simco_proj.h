@@ -65,7 +65,8 @@ typedef struct sp_params {
   int32_t imax, jmax, kmax;
   int32_t precision;       /* sp_precision */
   int32_t precond;         /* sp_precond */
-  int32_t mg_levels;       /* 0: coarsen while every dimension >= 4 */
+  int32_t mg_levels;       /* 0: coarsen while every dimension >= 4 and a
+                              level holds more than 64 cells */
   int32_t mg_pre, mg_post; /* red-black Gauss-Seidel sweeps */
   int32_t mg_coarse_sweeps;
   double mg_coarse_scale;  /* coarse operator = scale * Galerkin face sums */
