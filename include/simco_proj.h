@@ -323,7 +323,9 @@ typedef struct sp_cell_geom {
   double* cell_cent;                  /* TCell%Cell_Cent: written */
   double* wllh;                       /* TCell%WlLh: written */
   double* delh;                       /* TCell%delh: updated where the reference sets it */
-  int32_t halo_fields;                /* 1: phi, nx, ny, nz, vof, vof_l carry the halo */
+  int32_t halo_fields;                /* 1: phi, nx, ny, nz, vof, vof_l carry the halo;
+                                         0: cell arrays; other values are rejected
+                                         (SP_ERR_INVALID_ARG): initialise the struct */
 } sp_cell_geom;
 
 /* GridPreprocess(PGrid,UGrid,VGrid,WGrid, PCell,UCell,VCell,WCell)
@@ -414,7 +416,7 @@ sp_status sp_number_external_cell_noalloc(sp_ctx* ctx, int iu, int iv, int iw,
 typedef struct sp_liquid_field {
   const double *dx, *dy, *dz;
   double *phi_l, *nx_l, *ny_l, *nz_l, *vof_l;
-  int32_t halo_fields;
+  int32_t halo_fields;  /* 0 or 1 as in sp_cell_geom; other values are rejected */
 } sp_liquid_field;
 
 /* ComputeUVWLiquidField(PGrid,PCell,UCell,VCell,WCell,UGrid,VGrid,WGrid)

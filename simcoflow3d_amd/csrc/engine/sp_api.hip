@@ -893,10 +893,19 @@ static sp_status liq_cells_out(sp_ctx* c, const sp_liquid_field& orig,
   return SP_OK;
 }
 
+// halo_fields selects the indexing of caller arrays, so a value from an
+// uninitialised struct must not reach the kernels
+static sp_status check_layout(sp_ctx* c, int32_t halo_fields, const char* what) {
+  if (halo_fields == 0 || halo_fields == 1) return SP_OK;
+  return fail(c, SP_ERR_INVALID_ARG, what);
+}
+
 static sp_status check_geom(sp_ctx* c, const sp_cell_geom* const g[4]) {
   for (int m = 0; m < 4; ++m) {
     if (!g[m]) return fail(c, SP_ERR_INVALID_ARG, "null sp_cell_geom");
     const sp_cell_geom& a = *g[m];
+    if (sp_status s = check_layout(c, a.halo_fields, "sp_cell_geom.halo_fields must be 0 or 1"))
+      return s;
     const void* req[] = {a.dx, a.dy, a.dz, a.phi, a.nx, a.ny, a.nz, a.vof, a.vof_l,
                          a.cell_type, a.ee_area, a.ne_area, a.te_area, a.fce, a.fcn, a.fct,
                          a.cell_cent, a.wllh, a.delh};
@@ -1016,6 +1025,8 @@ sp_status sp_new_cell_face(sp_ctx* c, const sp_cell_geom* p, const sp_cell_geom*
   for (int m = 0; m < 4; ++m) {
     if (!g[m]) return fail(c, SP_ERR_INVALID_ARG, "null sp_cell_geom");
     const sp_cell_geom& a = *g[m];
+    if (sp_status s = check_layout(c, a.halo_fields, "sp_cell_geom.halo_fields must be 0 or 1"))
+      return s;
     // P's spacing is read too: the face along a grid's own axis crosses a P cell
     const void* req[] = {a.dx, a.dy, a.dz, a.ee_area, a.ne_area, a.te_area, a.fce, a.fcn, a.fct};
     for (const void* q : req)
@@ -1223,6 +1234,8 @@ sp_status sp_compute_uvw_liquid_field(sp_ctx* c, const sp_liquid_field* p,
   for (int m = 0; m < 4; ++m) {
     if (!g[m]) return fail(c, SP_ERR_INVALID_ARG, "null sp_liquid_field");
     const sp_liquid_field& a = *g[m];
+    if (sp_status s = check_layout(c, a.halo_fields, "sp_liquid_field.halo_fields must be 0 or 1"))
+      return s;
     const void* req[] = {a.dx, a.dy, a.dz, a.phi_l, a.nx_l, a.ny_l, a.nz_l, a.vof_l};
     for (const void* q : req)
       if (!q) return fail(c, SP_ERR_INVALID_ARG, "sp_liquid_field member is null");
@@ -1396,6 +1409,9 @@ static sp_status check_pred(sp_ctx* c, const sp_pred_inputs* in, const sp_pred_o
     const void* rc[] = {h.vof, h.vof_l, h.cell_type, h.wllh, h.delh};
     for (const void* p : rc)
       if (!p) return fail(c, SP_ERR_INVALID_ARG, "a member of a current U/V/W cell is null");
+    sp_status s = check_layout(c, g.halo_fields, "sp_cell_geom.halo_fields must be 0 or 1");
+    if (s || (s = check_layout(c, h.halo_fields, "sp_cell_geom.halo_fields must be 0 or 1")))
+      return s;
   }
   return SP_OK;
 }

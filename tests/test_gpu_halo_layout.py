@@ -284,3 +284,29 @@ def test_moving_solid_step_on_device(gpu_available):
             assert np.array_equal(g, exp[m][n], equal_nan=True), (m, n)
     assert any((exp[m]["cell_type"] == 1).any() for m in range(4))
     assert any((exp[m]["cell_type"] == 2).any() for m in range(4))
+
+
+def test_layout_flag_other_than_0_or_1_is_rejected(gpu_available):
+    """halo_fields selects how caller arrays are indexed: a value left over
+    in an uninitialised struct is refused before any launch."""
+    import ctypes
+    from simcoflow3d_amd.engine import SpCellGeom, SpError, SpLiquidField, _addr, geom_shapes
+    size = (6, 5, 4)
+    print("[halo] layout flag validation", flush=True)
+    sh = geom_shapes(*size)
+    cells = configs.geometry_case(size, "sphere", seed=1)
+    for c in cells:  # every member present: only the flag is wrong
+        for n in GEOM_FIELDS:
+            if n not in c:
+                c[n] = np.zeros(sh[n], order="F")
+    P = Projector(size)
+    gs = [SpCellGeom(*(_addr(c[n]) for n in GEOM_FIELDS), 7) for c in cells]
+    with pytest.raises(SpError, match="halo_fields"):
+        P._check(P.lib.sp_grid_preprocess(P.h, *(ctypes.byref(g) for g in gs)), "grid")
+    lq = [dict(dx=c["dx"], dy=c["dy"], dz=c["dz"], phi_l=c["phi"], nx_l=c["nx"], ny_l=c["ny"],
+               nz_l=c["nz"], vof_l=c["vof"]) for c in cells]
+    ls = [SpLiquidField(*(_addr(d[n]) for n in LIQUID_FIELDS), -1) for d in lq]
+    with pytest.raises(SpError, match="halo_fields"):
+        P._check(P.lib.sp_compute_uvw_liquid_field(P.h, *(ctypes.byref(g) for g in ls), 0),
+                 "liquid")
+    P.close()
